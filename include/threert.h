@@ -1,0 +1,159 @@
+/*
+ * threert_amd: an MI355X-native (gfx950, HIP) core for ThreeRT's software path-trace pass.
+ *
+ * A plain C ABI: opaque handles, pointers and sizes, int status codes. No exceptions cross it; the library never
+ * keeps a caller's host pointer after a call returns. Every entry point names the reference interface it stands in
+ * for (paths relative to the ThreeRT repository root).
+ *
+ * Byte formats are the reference's kernel inputs:
+ *   node      32 B  bvh.rs:6-15 GpuNode   (bmin[3], left_or_first, bmax[3], count; count 0 = interior)
+ *   leaf      48 B  indexed.rs:19-26 Leaf (v0.xyz, section id bits), (e1.xyz, index bits), (e2.xyz, 0)
+ *   instance 160 B  scene.rs:229-234 GpuInstance (m rows, n rows, info, p rows); units.rs:329-334 unit_record bits
+ */
+#ifndef THREERT_H
+#define THREERT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes (the reference returns anyhow::Result; gpu.rs:2613, gpu.rs:4116). */
+#define TRT_OK 0
+#define TRT_E_INVALID 1 /* bad argument or scene data (the reference's upload/prepare errors)          */
+#define TRT_E_OOM 2     /* device or host allocation failed                                           */
+#define TRT_E_DEVICE 3  /* a HIP call failed                                                          */
+#define TRT_E_GUARD 4   /* a kernel stopped at its iteration guard (gpu.rs:4199-4203 loop-guard trip) */
+
+/* Trace modes (pathtrace.wgsl TRACE_CLOSEST / TRACE_SHADOW / TRACE_ANY). */
+#define TRT_TRACE_CLOSEST 0u
+#define TRT_TRACE_SHADOW 1u
+#define TRT_TRACE_ANY 2u
+
+/* trt_trace_rays flags: which node format the kernel walks. */
+#define TRT_WALK_WIDE 0u   /* the library's 64-byte 4-wide quantised tree (default)            */
+#define TRT_WALK_BINARY 1u /* the reference's own 32-byte binary node buffer, as uploaded      */
+
+#define TRT_NO_HIT 0xffffffffu /* trt_hit.tri / trt_hit.inst of a ray that hit nothing */
+
+typedef struct trt_device trt_device;
+typedef struct trt_scene trt_scene;
+
+/* The last error's message on this thread ("" after success). */
+const char *trt_last_error(void);
+
+/* --- device (gpu.rs:2113-2248 Gpu::new / new_with) ------------------------------------------------------------ */
+typedef struct {
+    char name[64];
+    uint32_t compute_units;
+    uint32_t wave_size;
+    uint64_t hbm_bytes;
+    uint32_t clock_khz;
+    uint32_t gcn_arch; /* e.g. 950 */
+} trt_device_info;
+
+int trt_device_create(int ordinal, trt_device **out);
+int trt_device_release(trt_device *dev);
+int trt_device_info_get(trt_device *dev, trt_device_info *out);
+/* Wait for all work the library queued on the device's stream (the reference's device.poll(Wait)). */
+int trt_device_sync(trt_device *dev);
+
+/* Device memory for callers that keep their inputs resident in HBM (the reference's wgpu::Buffer role). */
+int trt_mem_alloc(trt_device *dev, uint64_t bytes, void **dptr);
+int trt_mem_free(trt_device *dev, void *dptr);
+int trt_mem_upload(trt_device *dev, void *dst, const void *src, uint64_t bytes);
+int trt_mem_download(trt_device *dev, void *dst, const void *src, uint64_t bytes);
+
+/* HIP-event timer on the device's stream, the stream every kernel of the library runs on. */
+int trt_timer_start(trt_device *dev);
+int trt_timer_stop(trt_device *dev, float *ms);
+
+/* --- host tree builders (prepare side: lib.rs:836-913 unit_trees) ------------------------------------------- */
+typedef struct {
+    void *nodes;         /* 32-byte nodes */
+    uint64_t node_count;
+    uint32_t *order;     /* order[i]: the item now at position i */
+    uint64_t order_count;
+    uint32_t depth;
+    float lo[3], hi[3];  /* mesh trees: the model-space bounds */
+} trt_tree;
+
+/* bvh.rs:81-106 build_tris: tris n x 9 floats; max_depth as the reference's (48 for bvh::build). */
+int trt_build_tris(const float *tris, uint64_t n, uint32_t max_depth, trt_tree *out);
+/* units.rs:231-246 mesh_tree: binary SAH, median splits when deeper than 45. */
+int trt_build_mesh_tree(const float *tris, uint64_t n, trt_tree *out);
+/* units.rs:140-223 build_sah over boxes (n x 6: min xyz, max xyz). */
+int trt_build_sah(const float *boxes, uint64_t n, uint32_t max_depth, trt_tree *out);
+/* units.rs:84-118 build_median over boxes. */
+int trt_build_median(const float *boxes, uint64_t n, uint32_t max_leaf, trt_tree *out);
+/* units.rs:303-321 top_tree: leaves name records[]. */
+int trt_build_top_tree(const float *boxes, const uint32_t *records, uint64_t n, trt_tree *out);
+/* units.rs:288-297 UnitBox::new: world is a column-major 4x4; out = lo xyz, hi xyz. */
+void trt_unit_box(const float *world, const float *lo, const float *hi, float *out);
+/* units.rs:329-342 unit_record on a 160-byte record, in place. */
+void trt_unit_record(void *record, uint32_t root, int cast_shadow);
+void trt_tree_free(trt_tree *t);
+
+/* --- scene upload (gpu.rs:2613 Gpu::upload) ---------------------------------------------------------------- */
+typedef struct {
+    const void *nodes;       /* the node buffer: mesh trees, then the top tree (units.rs:532-595 Layout) */
+    uint64_t node_count;
+    const void *leaves;      /* 48-byte leaves, trees' order */
+    uint64_t leaf_count;
+    const void *instances;   /* 160-byte records */
+    uint64_t instance_count;
+    uint32_t top_first;      /* P.motion.z: the top tree's root */
+    uint32_t unit_count;     /* P.motion.w: units in the top tree */
+} trt_scene_desc;
+
+typedef struct {
+    uint64_t wide_nodes;     /* 64-byte nodes of the 4-wide tree */
+    uint64_t wide_bytes;
+    uint64_t binary_bytes;   /* the reference node buffer as uploaded */
+    uint64_t leaf_bytes;
+    uint64_t unit_bytes;
+    uint32_t max_stack;      /* the deepest a walk's stack can get over the 4-wide tree */
+    uint32_t units;
+    uint64_t device_bytes;   /* every device allocation the scene holds */
+} trt_scene_stats;
+
+/* Copies the host buffers to HBM and builds the 4-wide quantised tree from the node buffer. The caller's memory
+ * is borrowed for the call only. Refuses (TRT_E_INVALID) trees the reference's checks refuse (units.rs check_mesh /
+ * check_top) and records with cutout or glass bits (not supported). */
+int trt_scene_upload(trt_device *dev, const trt_scene_desc *desc, trt_scene **out);
+int trt_scene_release(trt_scene *scene);
+int trt_scene_stats_get(trt_scene *scene, trt_scene_stats *out);
+
+/* --- ray batches (trace_units.wgsl:44-135 via pathtrace.wgsl:764 `trace`; units.rs:1019 Kernel::trace) ------- */
+typedef struct {
+    float o[3];
+    float tmax;    /* > 3e38 (or NaN) is taken as 3e38, the reference's INF */
+    float d[3];
+    uint32_t mode; /* TRT_TRACE_* */
+} trt_ray;         /* 32 bytes */
+
+typedef struct {
+    float t;       /* tmax when nothing is hit */
+    uint32_t tri;  /* leaf position (first + tree position), TRT_NO_HIT when nothing is hit */
+    float u, v;
+    uint32_t inst; /* record, TRT_NO_HIT when nothing is hit */
+    uint32_t part; /* the leaf's section id (leaf_id(tri, 0)) */
+} trt_hit;         /* 24 bytes: pathtrace.wgsl Hit */
+
+/* Closest hits are the lowest (t, instance, triangle): the same answer whatever tree is walked. Any-hit modes
+ * report some hit exactly when there is one. Host buffers; blocks until the hits are written. */
+int trt_trace_rays(trt_scene *scene, const trt_ray *rays, uint64_t n, trt_hit *hits, uint32_t flags);
+/* The same with device buffers (trt_mem_alloc), queued on the device's stream; trt_device_sync waits. */
+int trt_trace_rays_device(trt_scene *scene, const trt_ray *d_rays, uint64_t n, trt_hit *d_hits, uint32_t flags);
+/* The same walk, counting what it reads (units.rs:922-927 Reads, units_cost.rs): reads[0] nodes fetched (64-byte
+ * wide or 32-byte binary), reads[1] triangles tested (48 bytes each), reads[2] unit records read (64-byte wide
+ * units or 160-byte records). Blocks. */
+int trt_trace_reads(trt_scene *scene, const trt_ray *d_rays, uint64_t n, trt_hit *d_hits, uint32_t flags,
+                    uint64_t *reads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,233 @@
+"""The oracle: ctypes bindings to liboracle.so, the plain-C CPU restatement of ThreeRT's software path-trace pass.
+
+Test infrastructure only. It is imported by tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg,
+always as the checker (or the timed CPU restatement), never by the threert_amd product.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+NODE = np.dtype([("bmin", "<f4", 3), ("left_or_first", "<u4"), ("bmax", "<f4", 3), ("count", "<u4")])
+INSTANCE = np.dtype([("m", "<f4", (3, 4)), ("n", "<f4", (3, 4)), ("info", "<u4", 4), ("p", "<f4", (3, 4))])
+RAY = np.dtype([("o", "<f4", 3), ("tmax", "<f4"), ("d", "<f4", 3), ("mode", "<u4")])
+HIT = np.dtype([("t", "<f4"), ("tri", "<u4"), ("u", "<f4"), ("v", "<f4"), ("inst", "<u4"), ("part", "<u4")])
+READS = np.dtype([("top_nodes", "<u8"), ("mesh_nodes", "<u8"), ("mesh_verts", "<u8"), ("records", "<u8"),
+                  ("steps", "<u8"), ("deepest", "<u8")])
+
+WALK_REF = 0
+WALK_EXACT = 1
+INF = np.float32(3.0e38)
+NO_HIT = 0xFFFFFFFF
+
+
+class _Tree(C.Structure):
+    _fields_ = [("nodes", C.c_void_p), ("node_count", C.c_uint64), ("order", C.POINTER(C.c_uint32)),
+                ("order_count", C.c_uint64), ("depth", C.c_uint32), ("lo", C.c_float * 3), ("hi", C.c_float * 3)]
+
+
+class _Scene(C.Structure):
+    _fields_ = [("nodes", C.c_void_p), ("node_count", C.c_uint64), ("leaves", C.c_void_p),
+                ("leaf_count", C.c_uint64), ("instances", C.c_void_p), ("instance_count", C.c_uint64),
+                ("top", C.c_uint32), ("units", C.c_uint32)]
+
+
+def build() -> str:
+    r = subprocess.run(["make", "-B", "-C", HERE], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"building the oracle failed:\n{r.stdout}\n{r.stderr}")
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P, U64, U32, I = C.c_void_p, C.c_uint64, C.c_uint32, C.c_int
+        for name, args in [("or_build_tris", [P, U64, U32, P]), ("or_build_median", [P, U64, U32, P]),
+                           ("or_build_sah", [P, U64, U32, P]), ("or_mesh_tree", [P, U64, P]),
+                           ("or_top_tree", [P, P, U64, P])]:
+            getattr(L, name).argtypes = args
+            getattr(L, name).restype = I
+        L.or_tree_free.argtypes = [P]
+        L.or_check_tree.argtypes = [P, U64, U32, U64, U32, C.c_char_p, U64]
+        L.or_check_tree.restype = I
+        L.or_check_top.argtypes = [P, U64, U32, C.c_char_p, U64]
+        L.or_check_top.restype = I
+        L.or_unit_box.argtypes = [P, P, P, P]
+        L.or_unit_record.argtypes = [P, U32, I]
+        L.or_top_cost.argtypes = [P, U64]
+        L.or_top_cost.restype = C.c_double
+        L.or_ray_aabb.argtypes = [P, P, P, P, C.c_float]
+        L.or_ray_aabb.restype = C.c_float
+        L.or_intersect_tri.argtypes = [P, P, P, C.c_float, P]
+        L.or_inv.argtypes = [P, P]
+        L.or_trace.argtypes = [P, P, P, C.c_float, U32, U32, U64, P, P]
+        L.or_trace.restype = I
+        L.or_brute.argtypes = [P, P, P, C.c_float, U32, U32, P]
+        L.or_brute.restype = I
+        L.or_trace_batch.argtypes = [P, P, U64, U32, I, P, P]
+        L.or_trace_batch.restype = C.c_int64
+        L.or_brute_batch.argtypes = [P, P, U64, U32, I, P]
+        L.or_brute_batch.restype = C.c_int64
+        _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _take_tree(t: _Tree):
+    nodes = np.frombuffer(C.string_at(t.nodes, t.node_count * 32), dtype=NODE).copy()
+    order = np.ctypeslib.as_array(t.order, shape=(t.order_count,)).copy() if t.order_count else np.zeros(0, np.uint32)
+    out = {"nodes": nodes, "order": order.astype(np.uint32), "depth": int(t.depth),
+           "lo": np.array(t.lo[:], np.float32), "hi": np.array(t.hi[:], np.float32)}
+    lib().or_tree_free(C.byref(t))
+    return out
+
+
+def build_tris(tris: np.ndarray, max_depth: int = 48):
+    tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
+    t = _Tree()
+    if lib().or_build_tris(_ptr(tris), len(tris), max_depth, C.byref(t)):
+        raise ValueError(f"{len(tris)} items do not fit a tree {max_depth} deep")
+    return _take_tree(t)
+
+
+def build_median(boxes: np.ndarray, max_leaf: int):
+    boxes = np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 6)
+    t = _Tree()
+    lib().or_build_median(_ptr(boxes), len(boxes), max_leaf, C.byref(t))
+    return _take_tree(t)
+
+
+def build_sah(boxes: np.ndarray, max_depth: int):
+    boxes = np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 6)
+    t = _Tree()
+    lib().or_build_sah(_ptr(boxes), len(boxes), max_depth, C.byref(t))
+    return _take_tree(t)
+
+
+def mesh_tree(tris: np.ndarray):
+    tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
+    t = _Tree()
+    if lib().or_mesh_tree(_ptr(tris), len(tris), C.byref(t)):
+        raise ValueError("no mesh tree (no triangles, or a vertex that is not finite)")
+    return _take_tree(t)
+
+
+def top_tree(boxes: np.ndarray, records: np.ndarray):
+    boxes = np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 6)
+    records = np.ascontiguousarray(records, dtype=np.uint32)
+    t = _Tree()
+    if lib().or_top_tree(_ptr(boxes), _ptr(records), len(boxes), C.byref(t)):
+        raise ValueError("no top tree")
+    return _take_tree(t)
+
+
+def check_tree(nodes: np.ndarray, max_depth: int, items: int, max_leaf: int = 16):
+    nodes = np.ascontiguousarray(nodes, dtype=NODE)
+    msg = C.create_string_buffer(256)
+    d = lib().or_check_tree(_ptr(nodes), len(nodes), max_depth, items, max_leaf, msg, 256)
+    return d if d >= 0 else msg.value.decode()
+
+
+def check_top(nodes: np.ndarray, records: int):
+    nodes = np.ascontiguousarray(nodes, dtype=NODE)
+    msg = C.create_string_buffer(256)
+    d = lib().or_check_top(_ptr(nodes), len(nodes), records, msg, 256)
+    return d if d >= 0 else msg.value.decode()
+
+
+def top_cost(nodes: np.ndarray) -> float:
+    nodes = np.ascontiguousarray(nodes, dtype=NODE)
+    return float(lib().or_top_cost(_ptr(nodes), len(nodes)))
+
+
+def unit_box(world_colmajor: np.ndarray, lo, hi) -> np.ndarray:
+    w = np.ascontiguousarray(world_colmajor, dtype=np.float32).reshape(16)
+    lo = np.ascontiguousarray(lo, dtype=np.float32)
+    hi = np.ascontiguousarray(hi, dtype=np.float32)
+    out = np.zeros(6, np.float32)
+    lib().or_unit_box(_ptr(w), _ptr(lo), _ptr(hi), _ptr(out))
+    return out
+
+
+def unit_record(rec: np.ndarray, root: int, cast_shadow: bool) -> None:
+    """In place on a one-element INSTANCE array."""
+    assert rec.dtype == INSTANCE and rec.flags.c_contiguous
+    lib().or_unit_record(_ptr(rec), root, 1 if cast_shadow else 0)
+
+
+def ray_aabb(o, inv, bmin, bmax, tmax) -> float:
+    a = [np.ascontiguousarray(x, dtype=np.float32) for x in (o, inv, bmin, bmax)]
+    return float(np.float32(lib().or_ray_aabb(*[_ptr(x) for x in a], C.c_float(tmax))))
+
+
+def intersect_tri(o, d, leaf: np.ndarray, tmax) -> np.ndarray:
+    o = np.ascontiguousarray(o, dtype=np.float32)
+    d = np.ascontiguousarray(d, dtype=np.float32)
+    leaf = np.ascontiguousarray(leaf, dtype=np.float32).reshape(12)
+    out = np.zeros(3, np.float32)
+    lib().or_intersect_tri(_ptr(o), _ptr(d), _ptr(leaf), C.c_float(tmax), _ptr(out))
+    return out
+
+
+class Scene:
+    """The kernel's buffers (nodes, leaves, records, P.motion.z/w) as the oracle walks them."""
+
+    def __init__(self, nodes: np.ndarray, leaves: np.ndarray, instances: np.ndarray, top: int, units: int):
+        self.nodes = np.ascontiguousarray(nodes, dtype=NODE)
+        self.leaves = np.ascontiguousarray(leaves, dtype=np.float32).reshape(-1, 3, 4)
+        self.instances = np.ascontiguousarray(instances, dtype=INSTANCE)
+        self._s = _Scene(_ptr(self.nodes), len(self.nodes), _ptr(self.leaves), len(self.leaves),
+                         _ptr(self.instances), len(self.instances), top, units)
+
+    def trace(self, o, d, tmax=INF, mode=0, variant=WALK_EXACT):
+        o = np.ascontiguousarray(o, dtype=np.float32)
+        d = np.ascontiguousarray(d, dtype=np.float32)
+        hit = np.zeros(1, HIT)
+        reads = np.zeros(1, READS)
+        rc = lib().or_trace(C.byref(self._s), _ptr(o), _ptr(d), C.c_float(tmax), mode, variant, 0, _ptr(hit),
+                            _ptr(reads))
+        if rc:
+            raise RuntimeError("the walk read outside its buffers or did not stop")
+        return hit[0], reads[0]
+
+    def brute(self, o, d, tmax=INF, mode=0, variant=WALK_EXACT):
+        o = np.ascontiguousarray(o, dtype=np.float32)
+        d = np.ascontiguousarray(d, dtype=np.float32)
+        hit = np.zeros(1, HIT)
+        if lib().or_brute(C.byref(self._s), _ptr(o), _ptr(d), C.c_float(tmax), mode, variant, _ptr(hit)):
+            raise RuntimeError("brute force read outside its buffers")
+        return hit[0]
+
+    def trace_batch(self, rays: np.ndarray, variant=WALK_EXACT, threads=0):
+        rays = np.ascontiguousarray(rays, dtype=RAY)
+        hits = np.zeros(len(rays), HIT)
+        reads = np.zeros(1, READS)
+        failed = lib().or_trace_batch(C.byref(self._s), _ptr(rays), len(rays), variant, threads, _ptr(hits),
+                                      _ptr(reads))
+        if failed:
+            raise RuntimeError(f"{failed} walks read outside their buffers or did not stop")
+        return hits, reads[0]
+
+    def brute_batch(self, rays: np.ndarray, variant=WALK_EXACT, threads=0):
+        rays = np.ascontiguousarray(rays, dtype=RAY)
+        hits = np.zeros(len(rays), HIT)
+        failed = lib().or_brute_batch(C.byref(self._s), _ptr(rays), len(rays), variant, threads, _ptr(hits))
+        if failed:
+            raise RuntimeError(f"{failed} brute-force rays read outside their buffers")
+        return hits

@@ -1,0 +1,40 @@
+// Launch interface of the ray-batch traversal kernels (trace.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "trt_internal.h"
+
+namespace trt {
+
+constexpr int kTraceBlock = 256;        // threads per block: four wave64s
+constexpr uint32_t kLdsStack = 16;      // stack entries per thread held in LDS (16 KiB a block)
+constexpr uint32_t kRefStack = 64;      // units.rs STACK: the binary walk drops pushes past it, as the reference
+constexpr uint32_t kGuardSteps = 1u << 24;  // a walk's iterations before it stops and reports (gpu.rs loop guard)
+
+struct TraceArgs {
+    const uint4 *wnodes;      // WideNode, 4 x uint4 each
+    const float4 *wunits;     // WideUnit, 4 x float4 each
+    const float4 *leaves;     // Leaf, 3 x float4 each
+    const float4 *bnodes;     // Node, 2 x float4 each
+    const float4 *instances;  // Instance, 10 x float4 each
+    const float4 *rays;       // trt_ray, 2 x float4 each
+    uint32_t *hits;           // trt_hit, 6 words each
+    uint32_t *counter;        // rays handed out so far
+    uint32_t *guard_flag;     // set when a walk stops at kGuardSteps
+    uint32_t *spill;          // stack entries past kLdsStack, [entry][thread]
+    unsigned long long *reads;  // counting launches: nodes, triangles, units
+    uint64_t n;
+    uint32_t spill_stride;    // threads in the grid
+    uint32_t top_ref;         // the wide top tree's root reference
+    uint32_t top_first;       // the binary top tree's root (P.motion.z)
+    uint32_t units;           // P.motion.w
+    float top_box[6];
+};
+
+int trace_occupancy(bool wide, int *blocks_per_cu);
+int launch_trace(const TraceArgs &a, bool wide, bool count, int grid, hipStream_t s);
+
+}  // namespace trt

@@ -1,0 +1,251 @@
+// The library's own node format: the reference's binary trees (bvh.rs GpuNode, units.rs Layout) collapsed into
+// 4-wide nodes of 64 bytes with 8-bit child bounds against an FP32 origin and a power-of-two step per axis.
+//
+// Collapse: a binary interior node becomes a wide node whose children are found by opening, while fewer than four,
+// the child of largest surface area that is itself interior. Binary leaves stay leaves (a run of <= 16 triangles,
+// or one unit in the top tree), so triangle positions, and with them hit ids, are the reference's.
+//
+// Quantisation is conservative: origin = (node low - one step) rounded down; each child's low byte is floored and
+// lowered by one step, its high byte ceiled and raised by one, checked in exact (double) arithmetic. The extra step
+// covers the walk's rounding when it evaluates a plane as fma(q, step / d, (origin - o) / d).
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "trt_internal.h"
+
+namespace trt {
+namespace {
+
+inline float node_area(const Node &n) {
+    float ex = std::max(n.bmax[0] - n.bmin[0], 0.0f), ey = std::max(n.bmax[1] - n.bmin[1], 0.0f),
+          ez = std::max(n.bmax[2] - n.bmin[2], 0.0f);
+    return 2.0f * (ex * ey + ey * ez + ez * ex);
+}
+
+bool quantise(WideNode &w, const Node *const *kids, int nc) {
+    for (int k = 0; k < 3; k++) {
+        double lo = INFINITY, hi = -INFINITY;
+        for (int c = 0; c < nc; c++) lo = std::min(lo, double(kids[c]->bmin[k])), hi = std::max(hi, double(kids[c]->bmax[k]));
+        const double ext = hi - lo;
+        int e = -100;
+        if (ext > 0) e = std::max(-100, int(std::ceil(std::log2(ext / 252.0))));
+        for (;; e++) {
+            if (e > 126) return false;
+            const double s = std::ldexp(1.0, e);
+            const double want = lo - s;
+            float of = static_cast<float>(want);
+            if (double(of) > want) of = std::nextafter(of, -INFINITY);
+            if (!std::isfinite(of)) return false;
+            bool ok = true;
+            uint32_t qlo = 0, qhi = 0;
+            for (int c = 0; c < nc && ok; c++) {
+                const double clo = kids[c]->bmin[k], chi = kids[c]->bmax[k];
+                double ql = std::floor((clo - double(of)) / s) - 1.0;
+                double qh = std::ceil((chi - double(of)) / s) + 1.0;
+                if (ql < 0) ql = 0;
+                while (ql > 0 && double(of) + ql * s > clo) ql -= 1.0;
+                while (double(of) + qh * s < chi) qh += 1.0;
+                if (qh > 255.0 || double(of) + ql * s > clo) {
+                    ok = false;
+                    break;
+                }
+                qlo |= uint32_t(ql) << (8 * c);
+                qhi |= uint32_t(qh) << (8 * c);
+            }
+            if (!ok) continue;
+            for (int c = nc; c < 4; c++) qlo |= 0xffu << (8 * c);  // an empty slot: low 255, high 0
+            w.origin[k] = of;
+            w.exp[k] = static_cast<uint8_t>(e + 127);
+            w.qlo[k] = qlo;
+            w.qhi[k] = qhi;
+            break;
+        }
+    }
+    w.nchild = static_cast<uint8_t>(nc);
+    return true;
+}
+
+struct Collapser {
+    const Node *nodes;
+    uint64_t node_count;
+    WideScene &out;
+    // The stack bound of a unit's mesh tree, by record (top tree leaves).
+    const std::vector<uint32_t> *unit_bound = nullptr;
+    std::string err;
+
+    // Converts the binary subtree at absolute index b (children relative to `off`). Returns its reference and
+    // writes the most entries a walk under it can push.
+    uint32_t convert(uint32_t b, uint32_t off, bool top, uint32_t &bound) {
+        const Node &n = nodes[b];
+        if (n.count > 0) {
+            if (top) {
+                bound = (*unit_bound)[n.left_or_first];
+                return ref_unit(n.left_or_first);
+            }
+            bound = 0;
+            if (n.left_or_first > kRefTriFirstMask) err = "a mesh has more triangles than a reference holds (2^26)";
+            return ref_tris(n.left_or_first, n.count);
+        }
+        uint32_t kid[4] = {off + n.left_or_first, off + n.left_or_first + 1, 0, 0};
+        int nc = 2;
+        while (nc < 4) {
+            int best = -1;
+            float best_area = -1.0f;
+            for (int c = 0; c < nc; c++) {
+                const Node &k = nodes[kid[c]];
+                if (k.count == 0 && node_area(k) > best_area) best_area = node_area(k), best = c;
+            }
+            if (best < 0) break;
+            const uint32_t l = off + nodes[kid[best]].left_or_first;
+            for (int c = nc; c > best + 1; c--) kid[c] = kid[c - 1];
+            kid[best] = l;
+            kid[best + 1] = l + 1;
+            nc++;
+        }
+        const uint32_t wi = static_cast<uint32_t>(out.nodes.size());
+        out.nodes.emplace_back();
+        WideNode w{};
+        const Node *kp[4];
+        uint32_t deepest = 0;
+        for (int c = 0; c < nc; c++) {
+            uint32_t cb = 0;
+            w.child[c] = convert(kid[c], off, top, cb);
+            deepest = std::max(deepest, cb);
+            kp[c] = &nodes[kid[c]];
+        }
+        for (int c = nc; c < 4; c++) w.child[c] = kNone;
+        if (!quantise(w, kp, nc)) err = "a node's box cannot be quantised";
+        out.nodes[wi] = w;
+        bound = deepest + static_cast<uint32_t>(nc - 1);
+        return ref_node(wi);
+    }
+};
+
+// Walks a binary tree from `root` (children relative to `off`) the way the kernel would, checking every index, the
+// depth and the leaves; returns false with a reason. (units.rs check_mesh / check_top on the uploaded buffer: the
+// node buffer may hold room after a tree, so nodes past the tree are not required to be reached.)
+bool check_walkable(const Node *nodes, uint64_t count, uint32_t root, uint32_t off, uint32_t max_depth, bool top,
+                    uint64_t leaves_or_records, uint32_t first, std::vector<uint32_t> *records, std::string *why) {
+    std::vector<std::pair<uint32_t, uint32_t>> st{{root, 0}};
+    uint64_t visits = 0;
+    while (!st.empty()) {
+        auto [i, depth] = st.back();
+        st.pop_back();
+        if (i >= count) {
+            *why = "a node index is outside the node buffer";
+            return false;
+        }
+        if (++visits > count) {
+            *why = "a tree reaches a node twice";
+            return false;
+        }
+        const Node &n = nodes[i];
+        if (n.count > 0) {
+            if (top) {
+                if (n.count != 1 || n.left_or_first >= leaves_or_records) {
+                    *why = "a top-tree leaf does not name one record";
+                    return false;
+                }
+                records->push_back(n.left_or_first);
+            } else if (n.count > kMaxLeaf || uint64_t(first) + n.left_or_first + n.count > leaves_or_records) {
+                *why = "a mesh-tree leaf names triangles outside the leaves";
+                return false;
+            }
+            continue;
+        }
+        for (int k = 0; k < 3; k++)
+            if (!(std::isfinite(n.bmin[k]) && std::isfinite(n.bmax[k]) && n.bmin[k] <= n.bmax[k])) {
+                *why = "an interior node's box is not finite";
+                return false;
+            }
+        const uint64_t l = uint64_t(off) + n.left_or_first;
+        if (l <= i || l + 1 >= count) {
+            *why = "an interior node's children are not after it inside the buffer";
+            return false;
+        }
+        if (depth + 1 > max_depth) {
+            *why = "a tree is deeper than the walk's stack allows";
+            return false;
+        }
+        st.push_back({uint32_t(l), depth + 1});
+        st.push_back({uint32_t(l + 1), depth + 1});
+    }
+    return true;
+}
+
+}  // namespace
+
+bool build_wide(const Node *nodes, uint64_t node_count, const Instance *inst, uint64_t inst_count,
+                uint64_t leaf_count, uint32_t top_first, uint32_t unit_count, WideScene &out, std::string *why) {
+    out = WideScene();
+    out.units.assign(inst_count, WideUnit{});
+    if (unit_count == 0) return true;
+    if (top_first >= node_count) {
+        *why = "the top tree's root is outside the node buffer";
+        return false;
+    }
+    std::vector<uint32_t> records;
+    if (!check_walkable(nodes, node_count, top_first, top_first, kTopDepth, true, inst_count, 0, &records, why))
+        return false;
+    {
+        std::vector<uint8_t> seen(inst_count, 0);
+        for (uint32_t r : records) {
+            if (seen[r]) {
+                *why = "a record is in the top tree twice";
+                return false;
+            }
+            seen[r] = 1;
+        }
+    }
+    Collapser col{nodes, node_count, out};
+    std::unordered_map<uint32_t, std::pair<uint32_t, uint32_t>> mesh;  // binary root -> (wide ref, bound)
+    std::vector<uint32_t> bound(inst_count, 0);
+    for (uint32_t r : records) {
+        const Instance &u = inst[r];
+        const uint32_t bits = to_bits(u.n[1][3]);
+        if (bits & (kInstCutout | kInstGlass)) {
+            *why = "records with cutout or glass sections are not supported";
+            return false;
+        }
+        const uint32_t root = to_bits(u.n[0][3]);
+        auto it = mesh.find(root);
+        if (it == mesh.end()) {
+            if (!check_walkable(nodes, node_count, root, root, kMeshDepth, false, leaf_count, u.info[0], nullptr, why))
+                return false;
+            uint32_t b = 0;
+            const uint32_t ref = col.convert(root, root, false, b);
+            if (!col.err.empty()) {
+                *why = col.err;
+                return false;
+            }
+            it = mesh.emplace(root, std::make_pair(ref, b)).first;
+        } else if (!check_walkable(nodes, node_count, root, root, kMeshDepth, false, leaf_count, u.info[0], nullptr,
+                                   why)) {
+            return false;
+        }
+        WideUnit w{};
+        w.t[0] = u.m[0][3], w.t[1] = u.m[1][3], w.t[2] = u.m[2][3];
+        w.flags = bits;
+        for (int k = 0; k < 3; k++) w.n0[k] = u.n[0][k], w.n1[k] = u.n[1][k], w.n2[k] = u.n[2][k];
+        w.root = it->second.first;
+        w.first = u.info[0];
+        out.units[r] = w;
+        bound[r] = it->second.second;
+    }
+    col.unit_bound = &bound;
+    uint32_t b = 0;
+    out.top_ref = col.convert(top_first, top_first, true, b);
+    if (!col.err.empty()) {
+        *why = col.err;
+        return false;
+    }
+    out.max_stack = b;
+    const Node &t = nodes[top_first];
+    for (int k = 0; k < 3; k++) out.top_box[k] = t.bmin[k], out.top_box[3 + k] = t.bmax[k];
+    return true;
+}
+
+}  // namespace trt
