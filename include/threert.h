@@ -106,6 +106,24 @@ typedef struct {
     uint64_t instance_count;
     uint32_t top_first;      /* P.motion.z: the top tree's root */
     uint32_t unit_count;     /* P.motion.w: units in the top tree */
+    /* What shading reads (trt_render); all may be NULL / 0 for a scene that only traces rays. */
+    const void *vertices;     /* indexed.rs:10-13 Vertex, 48 B: (position, u), (normal, v), (tangent, hand) */
+    uint64_t vertex_count;
+    const uint32_t *indices;  /* 3 a triangle, in the leaves' order, counted from the record's first vertex */
+    uint64_t index_count;     /* triangles (= leaf_count) */
+    const uint32_t *rows;     /* scene.rs:99 Row, 4 words: material | class, section first, first emitter, flags */
+    uint64_t row_count;
+    const void *materials;    /* scene.rs:480-491 GpuMaterial, 288 B */
+    uint64_t material_count;
+    const void *lights;       /* scene.rs:495-501 GpuLight, 80 B (directional lights first) */
+    uint64_t light_count;
+    const void *emitters;     /* lib.rs:451-481 emitter_cdf: entries, 16385 guide vec4s, header vec4 */
+    uint64_t emitter_vec4s;
+    const void *emitter_tris; /* lib.rs:143-157 GpuEmitterTri, 64 B */
+    uint64_t emitter_tri_count;
+    const void *objects;      /* object table, 3 vec4 an object; the first's w is its power */
+    uint64_t object_count;
+    const void *energy;       /* lib.rs:1088-1104 energy_lut: 512 vec4; NULL: no shading tables */
 } trt_scene_desc;
 
 typedef struct {
@@ -152,6 +170,25 @@ int trt_trace_rays_device(trt_scene *scene, const trt_ray *d_rays, uint64_t n, t
  * units or 160-byte records). Blocks. */
 int trt_trace_reads(trt_scene *scene, const trt_ray *d_rays, uint64_t n, trt_hit *d_hits, uint32_t flags,
                     uint64_t *reads);
+
+/* --- path tracing (gpu.rs:4116-4236 Gpu::render; pathtrace.wgsl:4106 `main`, offline) ---------------------- */
+typedef struct trt_accum trt_accum;
+
+/* The accumulator the samples add to: 80 bytes a pixel (records.wgsl:24 Pixel: color, albedo, normal, and two
+ * vec4 the offline pass leaves at zero), in HBM (gpu.rs:2388 accum_buffer). Created cleared. */
+int trt_accum_create(trt_device *dev, uint32_t width, uint32_t height, trt_accum **out);
+int trt_accum_release(trt_accum *acc);
+int trt_accum_clear(trt_accum *acc);
+/* The raw sums, width x height x 20 floats. Waits for the device. */
+int trt_accum_read(trt_accum *acc, float *pixels);
+/* gpu.rs:4218-4231: color, albedo, normal divided by the sample count (3 floats a pixel each, any may be NULL),
+ * variance of the mean luminance (1 float a pixel, may be NULL). Waits for the device. */
+int trt_accum_resolve(trt_accum *acc, float *color, float *albedo, float *normal, float *variance);
+
+/* Samples first_sample .. first_sample + spp - 1 of every pixel added to `acc` (gpu.rs:4177-4196: the sample index
+ * is P.size.z), on the device's stream. params: the 624-byte Params block (pathtrace.wgsl:96-140); its size.xy
+ * must be the accumulator's. Needs a scene uploaded with its shading tables. */
+int trt_render(trt_scene *scene, const void *params, uint32_t first_sample, uint32_t spp, trt_accum *acc);
 
 #ifdef __cplusplus
 }

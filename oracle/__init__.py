@@ -231,3 +231,77 @@ class Scene:
         if failed:
             raise RuntimeError(f"{failed} brute-force rays read outside their buffers")
         return hits
+
+
+# ---- the offline path loop (render.c) -----------------------------------------------------------------------------
+class _RenderScene(C.Structure):
+    _fields_ = [("geo", _Scene), ("vertices", C.c_void_p), ("vertex_count", C.c_uint64), ("indices", C.c_void_p),
+                ("tri_count", C.c_uint64), ("rows", C.c_void_p), ("row_count", C.c_uint64),
+                ("materials", C.c_void_p), ("material_count", C.c_uint64), ("lights", C.c_void_p),
+                ("light_count", C.c_uint32), ("emitters", C.c_void_p), ("emitter_vec4s", C.c_uint64),
+                ("emitter_tris", C.c_void_p), ("emitter_tri_count", C.c_uint64), ("objects", C.c_void_p),
+                ("object_count", C.c_uint64), ("energy", C.c_void_p)]
+
+
+class RenderScene:
+    """The kernel's buffers plus the shading tables, as the oracle's path loop reads them."""
+
+    def __init__(self, nodes, leaves, instances, top, units, vertices, indices, rows, materials, lights, emitters,
+                 emitter_tris, objects, energy):
+        self.geo = Scene(nodes, leaves, instances, top, units)
+        self.keep = [np.ascontiguousarray(x) for x in (vertices, indices, rows, materials, lights, emitters,
+                                                       emitter_tris, objects, energy)]
+        v, i, r, m, li, e, et, o, en = self.keep
+        self._s = _RenderScene(self.geo._s, _ptr(v), len(v), _ptr(i), len(i), _ptr(r), len(r), _ptr(m), len(m),
+                               _ptr(li), len(li), _ptr(e), len(e), _ptr(et), len(et), _ptr(o), len(o) // 3,
+                               _ptr(en))
+        L = lib()
+        L.or_render_sample.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_int]
+        L.or_render_sample.restype = C.c_int64
+
+    def render(self, params: np.ndarray, first_sample: int, spp: int, accum: np.ndarray | None = None,
+               rows=None, threads=0) -> np.ndarray:
+        """Samples first..first+spp-1 of rows [y0, y1) added to `accum` (h, w, 20), sample after sample."""
+        p = np.ascontiguousarray(params, np.uint32).copy()
+        w, h = int(p[16]), int(p[17])
+        if accum is None:
+            accum = np.zeros((h, w, 20), np.float32)
+        y0, y1 = (0, h) if rows is None else rows
+        for s in range(first_sample, first_sample + spp):
+            p[18] = s
+            failed = lib().or_render_sample(C.byref(self._s), _ptr(p), y0, y1, _ptr(accum), threads)
+            if failed:
+                raise RuntimeError(f"{failed} walks read outside their buffers")
+        return accum
+
+
+def rng4_at(pixel_seed: int, index: int, d: int) -> np.ndarray:
+    """sample.rs:78-82 rng4 (the oracle's restatement of sample.wgsl)."""
+    L = lib()
+    L.or_rng4_at.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p]
+    out = np.zeros(4, np.float32)
+    L.or_rng4_at(pixel_seed, index, d, _ptr(out))
+    return out
+
+
+def sobol_dirs() -> np.ndarray:
+    L = lib()
+    L.or_sobol_dirs.argtypes = [C.c_void_p]
+    out = np.zeros(96, np.uint32)
+    L.or_sobol_dirs(_ptr(out))
+    return out.reshape(3, 32)
+
+
+def hash_hp(x: int) -> int:
+    L = lib()
+    L.or_hash_hp.argtypes = [C.c_uint32]
+    L.or_hash_hp.restype = C.c_uint32
+    return int(L.or_hash_hp(x))
+
+
+def sincos(x: float):
+    L = lib()
+    L.or_sincos.argtypes = [C.c_float, C.c_void_p, C.c_void_p]
+    s, c = C.c_float(), C.c_float()
+    L.or_sincos(C.c_float(x), C.byref(s), C.byref(c))
+    return s.value, c.value

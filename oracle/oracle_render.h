@@ -1,0 +1,68 @@
+/*
+ * Oracle (test infrastructure only): the sampler and the offline path loop (pathtrace.wgsl `main`, LIVE = false)
+ * restated in C for the features the workloads use. See render.c for the list and the reference lines.
+ */
+#ifndef THREERT_ORACLE_RENDER_H
+#define THREERT_ORACLE_RENDER_H
+
+#include <stdint.h>
+
+#include "oracle.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    uint32_t seed, index, path, depth; /* RNG_SEED, RNG_INDEX, RNG_PATH, cur_depth */
+} or_rng;
+
+uint32_t or_reverse_bits(uint32_t x);
+uint32_t or_owen_rev(uint32_t n, uint32_t seed);
+uint32_t or_hash_hp(uint32_t i);
+float or_sobol_owen(uint32_t rev_index, uint32_t dim, uint32_t seed);
+void or_sobol_dirs(uint32_t *out96);
+uint32_t or_rng_dim(const or_rng *g, uint32_t slot);
+uint32_t or_rng_sub(uint32_t d, uint32_t k);
+float or_rng1(const or_rng *g, uint32_t d);
+void or_rng2(const or_rng *g, uint32_t d, float *r);
+void or_rng3(const or_rng *g, uint32_t d, float *r);
+void or_rng4(const or_rng *g, uint32_t d, float *r);
+void or_rng_start(or_rng *g, uint32_t px, uint32_t py, uint32_t index, uint32_t seed, uint32_t path);
+void or_rng4_at(uint32_t pixel_seed, uint32_t index, uint32_t d, float *r);
+
+/* Cephes-style sin/cos (Moshier's single-precision reduction and polynomials), fmaf evaluation. */
+void or_sincos(float x, float *s, float *c);
+
+/* What the offline kernel reads besides the trees (gpu.rs:2613 upload; heap.rs tables and words). */
+typedef struct {
+    or_scene geo;
+    const float *vertices;      /* indexed.rs Vertex: 12 floats */
+    uint64_t vertex_count;
+    const uint32_t *indices;    /* 3 per triangle, in the leaves' order, counted from the mesh's first vertex */
+    uint64_t tri_count;
+    const uint32_t *rows;       /* scene.rs Row: 4 words */
+    uint64_t row_count;
+    const float *materials;     /* scene.rs GpuMaterial: 72 floats */
+    uint64_t material_count;
+    const float *lights;        /* scene.rs GpuLight: 20 floats */
+    uint32_t light_count;
+    const float *emitters;      /* lib.rs emitter_cdf: entries, guide, header (vec4s) */
+    uint64_t emitter_vec4s;
+    const float *emitter_tris;  /* lib.rs GpuEmitterTri: 16 words */
+    uint64_t emitter_tri_count;
+    const float *objects;       /* vec4s, OBJECT_STRIDE (3) a object; [k*3].w its power */
+    uint64_t object_count;
+    const float *energy;        /* lib.rs energy_lut: 512 vec4 */
+} or_render_scene;
+
+/* One sample (params word 18 = P.size.z) of every pixel of rows [y0, y1), accumulated into `accum` (80 bytes a
+ * pixel: records.wgsl Pixel). params: the 624-byte Params block (pathtrace.wgsl:96-140). Returns the number of
+ * walks that failed (read outside their buffers). */
+int64_t or_render_sample(const or_render_scene *s, const uint32_t *params, uint32_t y0, uint32_t y1, float *accum,
+                         int threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,844 @@
+/*
+ * Oracle (test infrastructure only): the offline path loop, pathtrace.wgsl `main` (LIVE = false) restated for the
+ * features the workloads use: pinhole camera, instanced triangle meshes with per-section materials (Lambert + GGX
+ * metal-rough, emission), directional lights (soft cone), emissive triangles from the CDF (RIS at camera hits), a
+ * constant sky with light sampling, MIS, Russian roulette, the indirect clamp, FP32 accumulation. Not restated:
+ * textures, glass, clearcoat, media, fog, procedural geometry, the light tree, local lights, live-only paths.
+ *
+ *   pathtrace.wgsl:4002-4024 camera_ray       4106-4557 main (offline parts)
+ *   pathtrace.wgsl:279-283 inst_vec/inst_sign  286-288 inst_normal  310-313 material_of  330-360 tri_data
+ *   pathtrace.wgsl:406-413 inst_attr          430-432 seen_directly
+ *   pathtrace.wgsl:463-487 emitter totals / pick   547-560 emitter_draw (no tree)   590-593 emitter_pmf (no tree)
+ *   pathtrace.wgsl:846-855 shadow_tr          909-915 light_vis   975-977 sky_tr   1066-1070 emitter_tr
+ *   pathtrace.wgsl:1312-1317 emission         1320-1425 surface (no textures / functions / glass / coat)
+ *   pathtrace.wgsl:1492-1505 energy_lookup    1521-1525 spec_terms   1642-1699 lobes (no glass / coat)
+ *   pathtrace.wgsl:1707-1730 d_ggx / vis_smith_joint_approx / f_schlick   1733-1764 eval_bsdf_k   1766-1789 g1,
+ *   pdf_bsdf   1797-1817 light_mis   1819-1844 basis / sample_vndf   1848-1884 sample_bsdf
+ *   pathtrace.wgsl:1895-1930 env_radiance / env_pdf / sample_env (constant sky)   1943-1960 power_heuristic,
+ *   luminance, offset_origin   1963-1988 sky_mis / sun_disks   2009-2024 emit_mis
+ *   pathtrace.wgsl:2269-2393 emitter_radiance / sample_emitter_ris / sample_emitter   2431-2442 env_light
+ *   pathtrace.wgsl:2444-2540 direct_light (no local lights, no light_pick)
+ * WGSL leaves normalize, inverseSqrt, sin and cos to the driver; here normalize(v) = v / sqrt(v.v),
+ * inverseSqrt(x) = 1 / sqrt(x), sin/cos by or_sincos. Everything else rounds in the order the reference writes it.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "oracle_render.h"
+
+#define PI_F 3.14159265358979f
+static const float INV_PI = 1.0f / PI_F;
+static const float TWO_PI = 2.0f * PI_F;
+static const float FOUR_PI = 4.0f * PI_F;
+
+/* ---- sin / cos: Cephes sinf/cosf (S. L. Moshier): octant reduction by three-part pi/4, minimax polynomials. */
+void or_sincos(float x, float *s, float *c) {
+    const float DP1 = 0.78515625f, DP2 = 2.4187564849853515625e-4f, DP3 = 3.77489497744594108e-8f;
+    const float FOPI = 1.27323954473516f;
+    float sign_s = 1.0f, ax = x;
+    if (x < 0.0f) {
+        ax = -x;
+        sign_s = -1.0f;
+    }
+    uint32_t j = (uint32_t)(ax * FOPI);
+    float y = (float)j;
+    if (j & 1u) {
+        j += 1u;
+        y += 1.0f;
+    }
+    j &= 7u;
+    float sign_c = 1.0f;
+    if (j > 3u) {
+        j -= 4u;
+        sign_s = -sign_s;
+        sign_c = -sign_c;
+    }
+    if (j > 1u) sign_c = -sign_c;
+    float z = fmaf(-y, DP1, ax);
+    z = fmaf(-y, DP2, z);
+    z = fmaf(-y, DP3, z);
+    float zz = z * z;
+    float ps = fmaf(fmaf(-1.9515295891e-4f, zz, 8.3321608736e-3f), zz, -1.6666654611e-1f);
+    ps = fmaf(ps * zz, z, z);
+    float pc = fmaf(fmaf(2.443315711809948e-5f, zz, -1.388731625493765e-3f), zz, 4.166664568298827e-2f);
+    pc = fmaf(pc * zz, zz, fmaf(-0.5f, zz, 1.0f));
+    float sv, cv;
+    if (j == 1u || j == 2u) {
+        sv = pc;
+        cv = ps;
+    } else {
+        sv = ps;
+        cv = pc;
+    }
+    *s = sign_s * sv;
+    *c = sign_c * cv;
+}
+
+/* ---- vectors ---------------------------------------------------------------------------------------------- */
+typedef struct {
+    float x, y, z;
+} V;
+static inline V v3(float x, float y, float z) { V r = {x, y, z}; return r; }
+static inline V vadd(V a, V b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline V vsub(V a, V b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline V vmul(V a, V b) { return v3(a.x * b.x, a.y * b.y, a.z * b.z); }
+static inline V vdiv(V a, V b) { return v3(a.x / b.x, a.y / b.y, a.z / b.z); }
+static inline V vs(V a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
+static inline V vds(V a, float s) { return v3(a.x / s, a.y / s, a.z / s); }
+static inline V vneg(V a) { return v3(-a.x, -a.y, -a.z); }
+static inline float dot(V a, V b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline V cross(V a, V b) { return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+static inline float len(V a) { return sqrtf(dot(a, a)); }
+static inline V normalize(V a) { return vds(a, sqrtf(dot(a, a))); }
+static inline float fmaxs(float a, float b) { return fmaxf(a, b); }
+static inline float fmins(float a, float b) { return fminf(a, b); }
+static inline float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+static inline float mixf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+static inline int any_pos(V a) { return a.x > 0.0f || a.y > 0.0f || a.z > 0.0f; }
+static inline int all_nonpos(V a) { return a.x <= 0.0f && a.y <= 0.0f && a.z <= 0.0f; }
+static inline float luminance(V c) { return dot(c, v3(0.2126f, 0.7152f, 0.0722f)); }
+static inline uint32_t fb(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    return u;
+}
+static inline int non_finite(V v) {
+    return (fb(v.x) & 0x7fffffffu) >= 0x7f800000u || (fb(v.y) & 0x7fffffffu) >= 0x7f800000u ||
+           (fb(v.z) & 0x7fffffffu) >= 0x7f800000u;
+}
+static inline V ld3(const float *p) { return v3(p[0], p[1], p[2]); }
+
+/* mat3 columns (t, b, n) */
+typedef struct {
+    V t, b, n;
+} M3;
+static inline V mmul(M3 m, V v) { return vadd(vadd(vs(m.t, v.x), vs(m.b, v.y)), vs(m.n, v.z)); }
+static inline V mtmul(M3 m, V v) { return v3(dot(m.t, v), dot(m.b, v), dot(m.n, v)); }
+
+/* ---- params (pathtrace.wgsl:96-140) ------------------------------------------------------------------------ */
+typedef struct {
+    const uint32_t *w;
+} P_;
+static inline float pf(const uint32_t *p, int vec, int k) {
+    float f;
+    memcpy(&f, &p[4 * vec + k], 4);
+    return f;
+}
+static inline uint32_t pu(const uint32_t *p, int vec, int k) { return p[4 * vec + k]; }
+enum { CAM_POS = 0, CAM_FWD = 1, CAM_RIGHT = 2, CAM_UP = 3, SIZE = 4, COUNTS = 5, ENV = 6, MISC = 7, SKY = 8,
+       COUNTS2 = 9, LIGHT_LOD = 18 };
+
+enum { D_CAMERA = 0, D_BSDF = 1, D_RR = 2, D_SUN = 4, D_SUN_VIS = 5, D_EMIT = 7, D_EMIT_VIS = 8, D_ENV = 9,
+       D_ENV_VIS = 10 };
+#define EMIT_RIS_M 6u
+#define EMITTER_GUIDE 65536u
+#define NO_EMITTER 0xffffffffu
+#define INST_MIRRORED 4u
+#define MAT_EMITTER_CARD 2
+#define EMITTER_CARD 1u
+#define MIRROR_ROUGHNESS 0.3f
+#define OBJECT_STRIDE 3u
+
+typedef struct {
+    const or_render_scene *s;
+    const uint32_t *P;
+    or_rng g;
+    int mirror_path;
+    V mis_x, mis_n;
+    int sampled_spec;
+    int failed;
+} Ctx;
+
+/* ---- tables --------------------------------------------------------------------------------------------------- */
+static inline const or_instance *inst_at(const Ctx *c, uint32_t i) { return &c->s->geo.instances[i]; }
+static inline const float *material_at(const Ctx *c, uint32_t i) { return c->s->materials + 72 * (uint64_t)i; }
+static inline const float *light_at(const Ctx *c, uint32_t i) { return c->s->lights + 20 * (uint64_t)i; }
+static inline const float *emitter_at(const Ctx *c, uint32_t i) { return c->s->emitters + 4 * (uint64_t)i; }
+static inline uint32_t row_word(const Ctx *c, uint32_t first, uint32_t section, uint32_t k) {
+    return c->s->rows[4 * ((uint64_t)first + section) + k];
+}
+static inline float obj_power(const Ctx *c, uint32_t obj) { return c->s->objects[4 * (uint64_t)obj * OBJECT_STRIDE + 3]; }
+
+typedef struct {
+    float n0[4], n1[4], n2[4], uv[4], x[4];
+} TriAttr;
+
+typedef struct {
+    TriAttr a;
+    V v0, e1, e2;
+} TriData;
+
+/* pathtrace.wgsl:330-360 tri_data (no particles, no deforming meshes). */
+static TriData tri_data(const Ctx *c, uint32_t ti, uint32_t inst, uint32_t part) {
+    uint32_t base = 0;
+    if (inst != OR_NO_INST) base = inst_at(c, inst)->info[2];
+    const uint32_t *ix = c->s->indices + 3 * (uint64_t)ti;
+    const float *p0 = c->s->vertices + 12 * ((uint64_t)ix[0] + base);
+    const float *p1 = c->s->vertices + 12 * ((uint64_t)ix[1] + base);
+    const float *p2 = c->s->vertices + 12 * ((uint64_t)ix[2] + base);
+    const float *n0 = p0 + 4, *n1 = p1 + 4, *n2 = p2 + 4;
+    TriData t;
+    t.v0 = ld3(p0);
+    t.e1 = vsub(ld3(p1), ld3(p0));
+    t.e2 = vsub(ld3(p2), ld3(p0));
+    float a0[4] = {n0[0], n0[1], n0[2], p0[3]}, a1[4] = {n1[0], n1[1], n1[2], n0[3]},
+          a2[4] = {n2[0], n2[1], n2[2], p1[3]};
+    float fpart;
+    memcpy(&fpart, &part, 4);
+    float uv[4] = {n1[3], p2[3], n2[3], fpart}, x[4] = {0, 0, 0, 0};
+    memcpy(t.a.n0, a0, 16);
+    memcpy(t.a.n1, a1, 16);
+    memcpy(t.a.n2, a2, 16);
+    memcpy(t.a.uv, uv, 16);
+    memcpy(t.a.x, x, 16);
+    return t;
+}
+
+static inline V inst_vec(const or_instance *r, V v) {
+    return v3(dot(ld3(r->m[0]), v), dot(ld3(r->m[1]), v), dot(ld3(r->m[2]), v));
+}
+static inline float inst_sign(const or_instance *r) { return (fb(r->n[1][3]) & INST_MIRRORED) ? -1.0f : 1.0f; }
+static void inst_normal(const or_instance *r, float *n) {
+    V v = ld3(n);
+    V m = normalize(v3(dot(ld3(r->n[0]), v), dot(ld3(r->n[1]), v), dot(ld3(r->n[2]), v)));
+    n[0] = m.x, n[1] = m.y, n[2] = m.z;
+}
+/* inst_attr: normals placed, the object from the record (inst_x). */
+static void inst_attr(const or_instance *r, TriAttr *a) {
+    inst_normal(r, a->n0);
+    inst_normal(r, a->n1);
+    inst_normal(r, a->n2);
+    uint32_t obj = r->info[1] + fb(a->x[1]);
+    memcpy(&a->x[1], &obj, 4);
+}
+static inline uint32_t object_of(const TriAttr *a) { return fb(a->x[1]); }
+static inline uint32_t material_of(const Ctx *c, const TriAttr *a, uint32_t inst) {
+    if (inst == OR_NO_INST) return 0;
+    return row_word(c, inst_at(c, inst)->info[3], fb(a->uv[3]), 0) & 0xffffffu;
+}
+
+/* ---- surface -------------------------------------------------------------------------------------------------- */
+typedef struct {
+    V p, n, ng, base, emissive;
+    float roughness, metallic, specular;
+    int front, card;
+} Surface;
+
+static Surface surface(Ctx *c, V o, V d, const or_hit *h) {
+    TriData td = tri_data(c, h->tri, h->inst, h->part);
+    V e1 = td.e1, e2 = td.e2;
+    TriAttr a = td.a;
+    float side = 1.0f;
+    if (h->inst != OR_NO_INST) {
+        const or_instance *r = inst_at(c, h->inst);
+        e1 = inst_vec(r, e1);
+        e2 = inst_vec(r, e2);
+        inst_attr(r, &a);
+        side = inst_sign(r);
+    }
+    float w = 1.0f - h->u - h->v;
+    V ng = vs(normalize(cross(e1, e2)), side);
+    V n = normalize(vadd(vadd(vs(ld3(a.n0), w), vs(ld3(a.n1), h->u)), vs(ld3(a.n2), h->v)));
+    const float *m = material_at(c, material_of(c, &a, h->inst));
+    int front = dot(ng, d) < 0.0f;
+    if (!front) ng = vneg(ng);
+    if (dot(n, ng) < 0.0f) n = vneg(n);
+    Surface s;
+    s.p = vadd(o, vs(d, h->t));
+    s.front = front;
+    int32_t tex1z;
+    memcpy(&tex1z, &m[4 * 4 + 2], 4);
+    s.card = tex1z == MAT_EMITTER_CARD;
+    V base = ld3(m);
+    float rough = m[8 + 1], metal = m[8 + 0], spec = m[8 + 2];
+    V em = ld3(m + 4);
+    if (!all_nonpos(em)) em = em; /* emission(): no emissive map */
+    s.emissive = vs(em, obj_power(c, object_of(&a)));
+    s.base = base;
+    s.roughness = clampf(rough, 0.02f, 1.0f);
+    s.metallic = clampf(metal, 0.0f, 1.0f);
+    s.specular = spec;
+    s.ng = ng;
+    s.n = n;
+    return s;
+}
+
+/* ---- BSDF ----------------------------------------------------------------------------------------------------- */
+typedef struct {
+    V diffuse, f0, f90, w, e_spec, kd;
+    float a2, p_spec, p_coat, p_trans;
+} Lobes;
+
+static void energy_node(const Ctx *c, uint32_t i, float *out) {
+    const float *v = c->s->energy + 4 * (uint64_t)(i >> 1);
+    out[0] = (i & 1u) ? v[2] : v[0];
+    out[1] = (i & 1u) ? v[3] : v[1];
+}
+static void energy_lookup(const Ctx *c, float roughness, float nov, float *e) {
+    const uint32_t N = 32;
+    float x = sqrtf(clampf(nov, 0.0f, 1.0f)) * (float)(N - 1);
+    float y = clampf(roughness, 0.0f, 1.0f) * (float)(N - 1);
+    uint32_t x0 = (uint32_t)x < N - 2 ? (uint32_t)x : N - 2;
+    uint32_t y0 = (uint32_t)y < N - 2 ? (uint32_t)y : N - 2;
+    uint32_t i = y0 * N + x0;
+    float fx = x - (float)x0, fy = y - (float)y0;
+    float a[2], b[2], cc[2], d[2];
+    energy_node(c, i, a);
+    energy_node(c, i + 1, b);
+    energy_node(c, i + N, cc);
+    energy_node(c, i + N + 1, d);
+    for (int k = 0; k < 2; k++) e[k] = mixf(mixf(a[k], b[k], fx), mixf(cc[k], d[k], fx), fy);
+}
+
+static Lobes lobes(const Ctx *c, const Surface *s, float nov) {
+    Lobes l;
+    float t = 0.0f;
+    l.diffuse = vs(vs(s->base, 1.0f - s->metallic), 1.0f - t);
+    float f0 = 0.08f * s->specular;
+    V f0v = v3(f0, f0, f0);
+    l.f0 = v3(mixf(f0v.x, s->base.x, s->metallic), mixf(f0v.y, s->base.y, s->metallic),
+              mixf(f0v.z, s->base.z, s->metallic));
+    float e[2];
+    energy_lookup(c, s->roughness, nov, e);
+    float comp = pf(c->P, MISC, 2);
+    const float k503 = 50.0f / 3.0f;
+    float f90 = clampf(dot(l.f0, v3(k503, k503, k503)), 0.0f, 1.0f);
+    float inv = comp * (1.0f / fmaxf(e[0] + e[1], 1e-3f) - 1.0f);
+    l.w = v3(1.0f + l.f0.x * inv, 1.0f + l.f0.y * inv, 1.0f + l.f0.z * inv);
+    l.e_spec = vmul(vadd(vs(l.f0, e[0]), v3(f90 * e[1], f90 * e[1], f90 * e[1])), l.w);
+    l.f90 = v3(f90, f90, f90);
+    l.kd = vsub(v3(1, 1, 1), l.e_spec);
+    float r = s->roughness;
+    l.a2 = r * r * r * r;
+    V lw = v3(0.3f, 0.6f, 0.1f);
+    float ws = dot(l.e_spec, lw);
+    float wd = dot(vmul(l.diffuse, l.kd), lw);
+    l.p_spec = clampf(ws / fmaxf(ws + wd, 1e-6f), 0.05f, 1.0f);
+    if (wd <= 0.0f) l.p_spec = 1.0f;
+    float e_coat = 0.0f, wt = 0.0f, under = 1.0f - e_coat;
+    float total = (ws + wd) * under + e_coat + wt;
+    l.p_coat = e_coat / fmaxf(total, 1e-6f);
+    l.p_trans = wt / fmaxf(total, 1e-6f);
+    return l;
+}
+
+static inline float d_ggx(float a2, float noh) {
+    float d = (noh * a2 - noh) * noh + 1.0f;
+    return a2 / (PI_F * d * d);
+}
+static inline float vis_smith(float a2, float nov, float nol) {
+    return 0.5f / fmaxf(mixf(2.0f * nol * nov, nol + nov, sqrtf(a2)), 1e-7f);
+}
+static inline V f_schlick(V f0, V f90, float voh) {
+    float c = 1.0f - voh, c2 = c * c, k = c2 * c2 * c;
+    return vadd(f0, vs(vsub(f90, f0), k));
+}
+
+static V eval_bsdf_k(const Surface *s, const Lobes *lb, V v, V l, float k) {
+    float nol = dot(s->n, l);
+    float nov = fmaxf(dot(s->n, v), 1e-4f);
+    if (nol <= 0.0f || dot(s->ng, l) <= 0.0f) return v3(0, 0, 0);
+    V h = normalize(vadd(v, l));
+    float noh = fmaxf(dot(s->n, h), 0.0f);
+    float voh = fmaxf(dot(v, h), 0.0f);
+    float dv = d_ggx(lb->a2, noh) * vis_smith(lb->a2, nov, nol);
+    V spec = vs(vmul(vs(f_schlick(lb->f0, lb->f90, voh), dv), lb->w), k);
+    V diff = vmul(vs(lb->diffuse, INV_PI), lb->kd);
+    return vs(vadd(diff, spec), nol);
+}
+static V eval_bsdf(const Surface *s, const Lobes *lb, V v, V l) { return eval_bsdf_k(s, lb, v, l, 1.0f); }
+
+static inline float g1(float a2, float nov) { return 2.0f * nov / (nov + sqrtf(a2 + (1.0f - a2) * nov * nov)); }
+
+static float pdf_bsdf(const Surface *s, const Lobes *lb, V v, V l) {
+    float nol = dot(s->n, l);
+    float pg = 0.0f;
+    if (nol <= 0.0f) return pg;
+    float nov = fmaxf(dot(s->n, v), 1e-4f);
+    V h = normalize(vadd(v, l));
+    float noh = fmaxf(dot(s->n, h), 0.0f);
+    float voh = fmaxf(dot(v, h), 1e-6f);
+    float dv = g1(lb->a2, nov) * voh * d_ggx(lb->a2, noh) / nov;
+    float ps = dv / (4.0f * voh);
+    float pd = nol / PI_F;
+    float pc = 0.0f;
+    return pg + lb->p_coat * pc + (1.0f - lb->p_coat - lb->p_trans) * (lb->p_spec * ps + (1.0f - lb->p_spec) * pd);
+}
+
+static inline float power_heuristic(float a, float b) {
+    float a2 = a * a;
+    return a2 / fmaxf(a2 + b * b, 1e-20f);
+}
+static float light_mis(float pdf_l, const Surface *s, const Lobes *lb, V v, V l) {
+    return power_heuristic(pdf_l, pdf_bsdf(s, lb, v, l));
+}
+
+static M3 basis(V n) {
+    float sgn = n.z >= 0.0f ? 1.0f : -1.0f;
+    float a = -1.0f / (sgn + n.z);
+    float b = n.x * n.y * a;
+    M3 m;
+    m.t = v3(1.0f + sgn * n.x * n.x * a, sgn * b, -sgn * n.x);
+    m.b = v3(b, sgn + n.y * n.y * a, -n.y);
+    m.n = n;
+    return m;
+}
+
+static V sample_vndf(V v, M3 tbn, float alpha, float u1, float u2) {
+    V vl = mtmul(tbn, v);
+    V vh = normalize(v3(alpha * vl.x, alpha * vl.y, vl.z));
+    float lensq = vh.x * vh.x + vh.y * vh.y;
+    V t1 = lensq > 0.0f ? vds(v3(-vh.y, vh.x, 0.0f), sqrtf(lensq)) : v3(1.0f, 0.0f, 0.0f);
+    V t2 = cross(vh, t1);
+    float r = sqrtf(u1);
+    float phi = TWO_PI * u2;
+    float sp, cp;
+    or_sincos(phi, &sp, &cp);
+    float p1 = r * cp;
+    float p2 = r * sp;
+    float s = 0.5f * (1.0f + vh.z);
+    p2 = (1.0f - s) * sqrtf(1.0f - p1 * p1) + s * p2;
+    V nh = vadd(vadd(vs(t1, p1), vs(t2, p2)), vs(vh, sqrtf(fmaxf(0.0f, 1.0f - p1 * p1 - p2 * p2))));
+    V hl = normalize(v3(alpha * nh.x, alpha * nh.y, fmaxf(0.0f, nh.z)));
+    return mmul(tbn, hl);
+}
+static inline V reflect(V e1, V e2) { return vsub(e1, vs(e2, 2.0f * dot(e2, e1))); }
+
+static V sample_bsdf(Ctx *c, const Surface *s, const Lobes *lb, V v) {
+    M3 tbn = basis(s->n);
+    float r[4];
+    or_rng4(&c->g, or_rng_dim(&c->g, D_BSDF), r);
+    c->sampled_spec = r[1] < lb->p_spec;
+    if (c->sampled_spec) {
+        V h = sample_vndf(v, tbn, s->roughness * s->roughness, r[2], r[3]);
+        return reflect(vneg(v), h);
+    }
+    float rr = sqrtf(r[2]);
+    float phi = TWO_PI * r[3];
+    float sp, cp;
+    or_sincos(phi, &sp, &cp);
+    return mmul(tbn, v3(rr * cp, rr * sp, sqrtf(fmaxf(0.0f, 1.0f - rr * rr))));
+}
+
+/* ---- light ---------------------------------------------------------------------------------------------------- */
+static inline V offset_origin(V p, V n) {
+    float m = fmaxf(1.0f, fmaxf(fabsf(p.x), fmaxf(fabsf(p.y), fabsf(p.z))));
+    return vadd(p, vs(n, 1e-4f * m));
+}
+
+static int occluded(Ctx *c, V o, V d, float tmax, uint32_t mode) {
+    or_hit h;
+    float of[3] = {o.x, o.y, o.z}, df[3] = {d.x, d.y, d.z};
+    if (or_trace(&c->s->geo, of, df, tmax, mode, OR_WALK_EXACT, 0, &h, NULL)) c->failed++;
+    return h.tri != OR_NO_TRI;
+}
+
+static V env_radiance(const Ctx *c, V d) {
+    (void)d;
+    return v3(pf(c->P, SKY, 0), pf(c->P, SKY, 1), pf(c->P, SKY, 2));
+}
+static float env_pdf(void) { return 1.0f / FOUR_PI; }
+static int env_sampled(const Ctx *c) {
+    return pu(c->P, COUNTS2, 1) != 0u &&
+           (pf(c->P, ENV, 0) > 0.5f || luminance(v3(pf(c->P, SKY, 0), pf(c->P, SKY, 1), pf(c->P, SKY, 2))) > 0.0f);
+}
+static V sample_env(Ctx *c, uint32_t rd) {
+    float u[4];
+    or_rng4(&c->g, rd, u);
+    float z = 1.0f - 2.0f * u[0];
+    float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+    float phi = TWO_PI * u[1];
+    float sp, cp;
+    or_sincos(phi, &sp, &cp);
+    return v3(r * cp, z, r * sp);
+}
+
+static V sun_disks(const Ctx *c, V d, float pdf) {
+    V sum = v3(0, 0, 0);
+    for (uint32_t i = 0; i < pu(c->P, COUNTS2, 0); i++) {
+        const float *li = light_at(c, i);
+        if (dot(d, vneg(ld3(li + 4))) >= li[7]) {
+            float omega = TWO_PI * (1.0f - li[7]);
+            sum = vadd(sum, vs(vds(ld3(li + 8), omega), power_heuristic(pdf, 1.0f / omega)));
+        }
+    }
+    return sum;
+}
+static V sky_mis(const Ctx *c, V d, float pdf) {
+    float wgt = 1.0f;
+    if (pdf > 0.0f && pu(c->P, COUNTS2, 1) != 0u) wgt = power_heuristic(pdf, env_pdf());
+    V col = vs(env_radiance(c, d), wgt);
+    if (pdf > 0.0f) col = vadd(col, sun_disks(c, d, pdf));
+    return col;
+}
+
+static inline uint32_t emitter_count(const Ctx *c) { return pu(c->P, COUNTS, 1); }
+static inline const float *emitter_totals(const Ctx *c) {
+    return emitter_at(c, emitter_count(c) + EMITTER_GUIDE / 4u + 1u);
+}
+static int emitters_on(const Ctx *c) { return emitter_count(c) != 0u && emitter_totals(c)[0] > 0.0f; }
+static inline uint32_t emitter_guide(const Ctx *c, uint32_t k) {
+    return fb(emitter_at(c, emitter_count(c) + k / 4u)[k % 4u]);
+}
+static const float *emitter_pick_u(const Ctx *c, float u) {
+    uint32_t k = (uint32_t)(u * (float)EMITTER_GUIDE);
+    if (k > EMITTER_GUIDE - 1u) k = EMITTER_GUIDE - 1u;
+    uint32_t lo = emitter_guide(c, k), hi = emitter_guide(c, k + 1u);
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) / 2u;
+        if (emitter_at(c, mid)[1] < u)
+            lo = mid + 1u;
+        else
+            hi = mid;
+    }
+    uint32_t n = emitter_count(c);
+    return emitter_at(c, lo < n - 1u ? lo : n - 1u);
+}
+
+typedef struct {
+    const float *e;
+    float p, w;
+} EPick;
+static EPick emitter_draw(Ctx *c, uint32_t rd) {
+    float r3[3];
+    or_rng3(&c->g, rd, r3);
+    const float *tot = emitter_totals(c);
+    EPick pk = {NULL, 0.0f, 0.0f};
+    if (!(tot[0] > 0.0f)) return pk;
+    pk.e = emitter_pick_u(c, r3[0]);
+    pk.p = pk.e[2] / tot[0];
+    pk.w = pk.e[2];
+    return pk;
+}
+
+typedef struct {
+    V v0, e1, e2;
+    uint32_t tri, inst, code, part, flags;
+} Em;
+static Em emitter_tri_at(const Ctx *c, uint32_t i) {
+    const float *r = c->s->emitter_tris + 16 * (uint64_t)i;
+    Em e;
+    e.v0 = ld3(r);
+    e.tri = fb(r[3]);
+    e.e1 = ld3(r + 4);
+    e.inst = fb(r[7]);
+    e.e2 = ld3(r + 8);
+    e.code = fb(r[11]);
+    e.part = fb(r[12]);
+    e.flags = fb(r[13]);
+    return e;
+}
+static float emitter_pmf(const Ctx *c, const Em *em) {
+    const float *tot = emitter_totals(c);
+    if (!(tot[0] > 0.0f)) return 0.0f;
+    uint32_t code = em->code & ~0x80000000u;
+    if ((code >> 26) == 0u) return emitter_at(c, code)[2] / tot[0];
+    return 0.0f; /* (light-tree codes: no tree here) */
+}
+
+static V emitter_radiance(const Ctx *c, const Em *em) {
+    TriAttr a = tri_data(c, em->tri, em->inst, em->part).a;
+    if (em->inst != OR_NO_INST) {
+        uint32_t obj = inst_at(c, em->inst)->info[1] + fb(a.x[1]);
+        memcpy(&a.x[1], &obj, 4);
+    }
+    const float *m = material_at(c, material_of(c, &a, em->inst));
+    return vs(ld3(m + 4), obj_power(c, object_of(&a)));
+}
+
+typedef struct {
+    V le, l;
+    float dist, pdf_w, w;
+} ES;
+
+static ES sample_emitter_ris(Ctx *c, V x, V n, uint32_t m, const Surface *sf, const Lobes *lb, V v, uint32_t rd) {
+    ES r = {v3(0, 0, 0), v3(0, 1, 0), 0.0f, 0.0f, 0.0f};
+    float wsum = 0.0f, pick_b1 = 0.0f, pick_b2 = 0.0f, pick_ph = 0.0f, pick_pdf = 0.0f;
+    uint32_t pick_i = 0;
+    for (uint32_t k = 0; k < m; k++) {
+        uint32_t dk = or_rng_sub(rd, k);
+        EPick pk = emitter_draw(c, dk);
+        float bb[3];
+        or_rng3(&c->g, or_rng_sub(dk, 1u), bb);
+        float b1 = bb[0], b2 = bb[1];
+        if (pk.p <= 0.0f) continue;
+        if (b1 + b2 > 1.0f) {
+            b1 = 1.0f - b1;
+            b2 = 1.0f - b2;
+        }
+        uint32_t ei = fb(pk.e[0]);
+        Em em = emitter_tri_at(c, ei);
+        V lp = vadd(vadd(em.v0, vs(em.e1, b1)), vs(em.e2, b2));
+        V cr = cross(em.e1, em.e2);
+        float a2 = len(cr);
+        if (!(a2 > 0.0f)) continue;
+        V ln = vds(cr, a2);
+        V dl = vsub(lp, x);
+        float d2 = fmaxf(dot(dl, dl), 1e-8f);
+        V l = vs(dl, 1.0f / sqrtf(d2));
+        float g = fabsf(dot(ln, l)) * (dot(n, n) == 0.0f ? 1.0f : fmaxf(dot(n, l), 0.0f)) / d2;
+        if (m > 1u) g = fabsf(dot(ln, l)) * luminance(eval_bsdf(sf, lb, v, l)) / d2;
+        float ph = pk.w / (pk.e[3] * 1.0f) * g;
+        float pdf_a = pk.p / (0.5f * a2);
+        float w = ph / pdf_a;
+        if (!(w > 0.0f)) continue;
+        wsum += w;
+        if (bb[2] * wsum < w) {
+            pick_b1 = b1;
+            pick_b2 = b2;
+            pick_ph = ph;
+            pick_i = ei;
+            pick_pdf = pdf_a;
+        }
+    }
+    if (pick_ph <= 0.0f) return r;
+    Em em = emitter_tri_at(c, pick_i);
+    V lp = vadd(vadd(em.v0, vs(em.e1, pick_b1)), vs(em.e2, pick_b2));
+    V ln = normalize(cross(em.e1, em.e2));
+    V dl = vsub(lp, x);
+    float d2 = dot(dl, dl);
+    float dist = sqrtf(d2);
+    V l = vds(dl, dist);
+    float cos_l = dot(ln, vneg(l));
+    if (!(em.flags & EMITTER_CARD)) cos_l = fabsf(cos_l);
+    if (cos_l <= 0.0f) return r;
+    r.le = emitter_radiance(c, &em);
+    r.l = l;
+    r.dist = dist;
+    r.pdf_w = pick_pdf * d2 / cos_l;
+    r.w = (wsum / (float)m) / pick_ph * cos_l / d2;
+    return r;
+}
+
+static ES sample_emitter(Ctx *c, V x, uint32_t rd) {
+    ES r = {v3(0, 0, 0), v3(0, 1, 0), 0.0f, 0.0f, 0.0f};
+    EPick pk = emitter_draw(c, rd);
+    if (pk.p <= 0.0f) return r;
+    Em em = emitter_tri_at(c, fb(pk.e[0]));
+    float bb[2];
+    or_rng2(&c->g, or_rng_sub(rd, 1u), bb);
+    float b1 = bb[0], b2 = bb[1];
+    if (b1 + b2 > 1.0f) {
+        b1 = 1.0f - b1;
+        b2 = 1.0f - b2;
+    }
+    V lp = vadd(vadd(em.v0, vs(em.e1, b1)), vs(em.e2, b2));
+    V cr = cross(em.e1, em.e2);
+    float area = 0.5f * len(cr);
+    V ln = vds(cr, 2.0f * area);
+    V dl = vsub(lp, x);
+    float d2 = dot(dl, dl);
+    float dist = sqrtf(d2);
+    V l = vds(dl, dist);
+    float cos_l = dot(ln, vneg(l));
+    if (!(em.flags & EMITTER_CARD)) cos_l = fabsf(cos_l);
+    if (cos_l <= 0.0f) return r;
+    r.le = emitter_radiance(c, &em);
+    r.l = l;
+    r.dist = dist;
+    r.pdf_w = pk.p / area * d2 / cos_l;
+    r.w = 1.0f / r.pdf_w;
+    return r;
+}
+
+static V env_light(Ctx *c, const Surface *s, const Lobes *lb, V v, V o) {
+    V l = sample_env(c, or_rng_dim(&c->g, D_ENV));
+    V f = eval_bsdf(s, lb, v, l);
+    if (any_pos(f)) {
+        float pdf_e = env_pdf();
+        if (pdf_e > 0.0f) {
+            if (!occluded(c, o, l, OR_INF, OR_TRACE_SHADOW))
+                return vds(vs(vmul(env_radiance(c, l), f), light_mis(pdf_e, s, lb, v, l)), pdf_e);
+        }
+    }
+    return v3(0, 0, 0);
+}
+
+static V direct_light(Ctx *c, const Surface *s, const Lobes *lb, V v) {
+    V sum = v3(0, 0, 0);
+    V o = offset_origin(s->p, s->ng);
+    for (uint32_t i = 0; i < pu(c->P, COUNTS2, 0); i++) {
+        const float *li = light_at(c, i);
+        float cos_max = li[7];
+        V to_light = vneg(ld3(li + 4));
+        float u[2];
+        or_rng2(&c->g, or_rng_sub(or_rng_dim(&c->g, D_SUN), i), u);
+        float cos_t = 1.0f - u[0] * (1.0f - cos_max);
+        float sin_t = sqrtf(fmaxf(0.0f, 1.0f - cos_t * cos_t));
+        float phi = TWO_PI * u[1];
+        float sp, cp;
+        or_sincos(phi, &sp, &cp);
+        M3 tb = basis(to_light);
+        V l = mmul(tb, v3(sin_t * cp, sin_t * sp, cos_t));
+        V col = ld3(li + 8);
+        if (all_nonpos(col)) continue;
+        V f = eval_bsdf_k(s, lb, v, l, li[17]);
+        if (any_pos(f)) {
+            int vis = li[15] >= 0.5f ? !occluded(c, o, l, OR_INF, OR_TRACE_SHADOW) : 1;
+            if (vis) {
+                float pdf_l = 1.0f / (TWO_PI * (1.0f - cos_max));
+                float wgt = light_mis(pdf_l, s, lb, v, l);
+                sum = vadd(sum, vs(vmul(col, f), wgt));
+            }
+        }
+    }
+    V ne = s->n;
+    c->mis_x = o;
+    c->mis_n = ne;
+    uint32_t emitters = emitters_on(c) ? emitter_count(c) : 0u;
+    if (emitters > 0u) {
+        ES es;
+        int seen = c->g.depth == 0u || (c->g.depth == 1u && c->mirror_path);
+        if (seen)
+            es = sample_emitter_ris(c, o, ne, c->g.depth == 0u ? EMIT_RIS_M : EMIT_RIS_M / 2u, s, lb, v,
+                                    or_rng_dim(&c->g, D_EMIT));
+        else if (pf(c->P, LIGHT_LOD, 3) > 0.0f)
+            es = sample_emitter_ris(c, o, ne, 2u, s, lb, v, or_rng_dim(&c->g, D_EMIT));
+        else
+            es = sample_emitter(c, o, or_rng_dim(&c->g, D_EMIT));
+        if (es.pdf_w > 0.0f) {
+            V f = eval_bsdf(s, lb, v, es.l);
+            if (any_pos(f)) {
+                if (!occluded(c, o, es.l, es.dist * 0.999f, OR_TRACE_ANY)) {
+                    float wgt = light_mis(es.pdf_w, s, lb, v, es.l);
+                    sum = vadd(sum, vs(vs(vmul(es.le, f), wgt), es.w));
+                }
+            }
+        }
+    }
+    if (env_sampled(c)) sum = vadd(sum, env_light(c, s, lb, v, o));
+    return sum;
+}
+
+static float emit_mis(Ctx *c, const or_hit *hit, V d, float pdf) {
+    if (!emitters_on(c)) return 1.0f;
+    uint32_t rows = inst_at(c, hit->inst)->info[3];
+    uint32_t first = row_word(c, rows, hit->part, 2u);
+    if (first == NO_EMITTER) return 1.0f;
+    uint32_t rank = fb(c->s->geo.leaves[hit->tri].r[1][3]);
+    uint32_t entry = first + rank;
+    Em em = emitter_tri_at(c, entry);
+    V cr = cross(em.e1, em.e2);
+    float area = 0.5f * len(cr);
+    float cos_l = fabsf(dot(normalize(cr), d));
+    float sel = emitter_pmf(c, &em);
+    if (!(sel > 0.0f)) return 1.0f;
+    return power_heuristic(pdf, sel / fmaxf(area, 1e-12f) * hit->t * hit->t / fmaxf(cos_l, 1e-6f));
+}
+
+/* ---- the path ------------------------------------------------------------------------------------------------- */
+static inline V clamp_lum(V col, float clamp_v) {
+    float lum = luminance(col);
+    if (lum > clamp_v) col = vs(col, clamp_v / lum);
+    return col;
+}
+
+static void path(Ctx *c, uint32_t px, uint32_t py, float *pix) {
+    const uint32_t *P = c->P;
+    uint32_t w = pu(P, SIZE, 0), h = pu(P, SIZE, 1);
+    or_rng_start(&c->g, px, py, pu(P, SIZE, 2), pu(P, SIZE, 3), 0u);
+    c->g.depth = 0;
+    float cam[4];
+    or_rng4(&c->g, or_rng_dim(&c->g, D_CAMERA), cam);
+    float ndx = ((float)px + cam[0]) / (float)w * 2.0f - 1.0f;
+    float ndy = 1.0f - ((float)py + cam[1]) / (float)h * 2.0f;
+    V fwd = v3(pf(P, CAM_FWD, 0), pf(P, CAM_FWD, 1), pf(P, CAM_FWD, 2));
+    V right = v3(pf(P, CAM_RIGHT, 0), pf(P, CAM_RIGHT, 1), pf(P, CAM_RIGHT, 2));
+    V up = v3(pf(P, CAM_UP, 0), pf(P, CAM_UP, 1), pf(P, CAM_UP, 2));
+    float tanh = pf(P, CAM_POS, 3), aspect = pf(P, CAM_FWD, 3);
+    V d = normalize(vadd(vadd(fwd, vs(right, ndx * tanh * aspect)), vs(up, ndy * tanh)));
+    V o = v3(pf(P, CAM_POS, 0), pf(P, CAM_POS, 1), pf(P, CAM_POS, 2));
+
+    V radiance = v3(0, 0, 0), thr = v3(1, 1, 1), albedo = v3(0, 0, 0), normal = v3(0, 0, 0);
+    int specular_bounce = 1, behind_glass = 0, bent = 0, straight = 1;
+    float last_pdf = 0.0f;
+    float clamp_v = pf(P, MISC, 0);
+    c->mirror_path = 0;
+    uint32_t last = pu(P, COUNTS, 2);
+    for (uint32_t depth = 0; depth <= last; depth++) {
+        c->g.depth = depth;
+        or_hit hit;
+        float of[3] = {o.x, o.y, o.z}, df[3] = {d.x, d.y, d.z};
+        if (or_trace(&c->s->geo, of, df, OR_INF, OR_TRACE_CLOSEST, OR_WALK_EXACT, 0, &hit, NULL)) c->failed++;
+        if (hit.tri == OR_NO_TRI) {
+            V col = vmul(thr, sky_mis(c, d, (depth > 0u && !bent) ? last_pdf : 0.0f));
+            if (depth > 1u && !straight) col = clamp_lum(col, clamp_v);
+            radiance = vadd(radiance, col);
+            break;
+        }
+        Surface s = surface(c, o, d, &hit);
+        if (depth == 0u) {
+            if (s.card) {
+                float mx = fmaxf(fmaxf(s.emissive.x, fmaxf(s.emissive.y, s.emissive.z)), 1e-6f);
+                albedo = vds(s.emissive, mx);
+            } else {
+                albedo = s.base;
+            }
+            normal = s.n;
+        }
+        if (any_pos(s.emissive) && (s.front || !s.card)) {
+            float wgt = 1.0f;
+            if (depth > 0u && !specular_bounce && !behind_glass) wgt = emit_mis(c, &hit, d, last_pdf);
+            V col = vs(vmul(thr, s.emissive), wgt);
+            if (depth > 1u && !straight) col = clamp_lum(col, clamp_v);
+            radiance = vadd(radiance, col);
+        }
+        if (s.card) break;
+        if (depth == last) break;
+        V v = vneg(d);
+        float nov = fmaxf(dot(s.n, v), 1e-4f);
+        Lobes lb = lobes(c, &s, nov);
+        V dl = vmul(thr, direct_light(c, &s, &lb, v));
+        if (depth > 0u && !straight) dl = clamp_lum(dl, clamp_v);
+        radiance = vadd(radiance, dl);
+        V l = sample_bsdf(c, &s, &lb, v);
+        float pdf = pdf_bsdf(&s, &lb, v, l);
+        V wgt = vds(eval_bsdf(&s, &lb, v, l), fmaxf(pdf, 1e-20f));
+        if (pdf <= 0.0f || all_nonpos(wgt)) break;
+        thr = vmul(thr, wgt);
+        last_pdf = pdf;
+        specular_bounce = 0;
+        behind_glass = 0;
+        bent = 0;
+        straight = 0;
+        c->mirror_path = (depth == 0u || c->mirror_path) && s.roughness < MIRROR_ROUGHNESS && c->sampled_spec;
+        if (depth >= pu(P, COUNTS, 3)) {
+            float q = fminf(fmaxf(thr.x, fmaxf(thr.y, thr.z)), 0.95f);
+            if (or_rng1(&c->g, or_rng_dim(&c->g, D_RR)) > q) break;
+            thr = vds(thr, q);
+        }
+        o = offset_origin(s.p, dot(l, s.ng) < 0.0f ? vneg(s.ng) : s.ng);
+        d = l;
+    }
+    if (non_finite(radiance)) radiance = v3(0, 0, 0);
+    if (non_finite(albedo)) albedo = v3(0, 0, 0);
+    if (non_finite(normal)) normal = v3(0, 1, 0);
+    float lum = luminance(radiance);
+    pix[0] += radiance.x, pix[1] += radiance.y, pix[2] += radiance.z, pix[3] += 1.0f;
+    pix[4] += albedo.x, pix[5] += albedo.y, pix[6] += albedo.z, pix[7] += lum;
+    pix[8] += normal.x, pix[9] += normal.y, pix[10] += normal.z, pix[11] += lum * lum;
+}
+
+int64_t or_render_sample(const or_render_scene *s, const uint32_t *params, uint32_t y0, uint32_t y1, float *accum,
+                         int threads) {
+    uint32_t w = params[4 * SIZE + 0];
+    int64_t failed = 0;
+#ifdef _OPENMP
+    if (threads <= 0) threads = omp_get_max_threads();
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 1) reduction(+ : failed)
+#endif
+    for (int64_t y = y0; y < (int64_t)y1; y++) {
+        Ctx c;
+        memset(&c, 0, sizeof c);
+        c.s = s;
+        c.P = params;
+        for (uint32_t x = 0; x < w; x++) path(&c, x, (uint32_t)y, accum + 20 * ((uint64_t)y * w + x));
+        failed += c.failed;
+    }
+    (void)threads;
+    return failed;
+}

@@ -1,0 +1,211 @@
+"""The offline path loop: the oracle pinned by the reference's furnace fixtures and closed forms, and the GPU's
+render held to the oracle's.
+
+Pinning the oracle (CPU): white furnaces (tests/render/furnace-diffuse.ts: an albedo-1 Lambert sphere under a
+1-nit sky reads 1.0 +- 2 %; furnace-metal.ts: F0 = 1, roughness 0.5, energy compensation 1 reads 1.0 +- 3 %), the
+energy table against an independent quadrature (lib.rs:1125-1156), and an emitter-lit Lambert plane against its
+closed-form irradiance.
+Parity (GPU): the same scene, seed and samples give the oracle's accumulator; the tolerance is per-pixel relative
+RMSE <= 1e-3 (the path's float operations are the oracle's in the same order, so in practice the bits agree)."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from threert_amd import render as R
+from threert_amd import scene as S
+
+
+def oracle_scene(rs):
+    p = rs.prep
+    return oracle.RenderScene(p.nodes, p.leaves, p.instances, p.top_first, p.units, **rs.tables())
+
+
+def sphere_scene(base, metallic, roughness, specular, sky=1.0):
+    t, n = R.icosphere(4)
+    rs = R.build([R.RMesh(t, normals=n)], [R.RInstance(0, S.translation([0.0, 0.0, 0.0]), [0])],
+                 [R.material(base, metallic=metallic, roughness=roughness, specular=specular)])
+    cam = R.Camera((0.0, 0.0, -4.0), (0.0, 0.0, 0.0), 20.0)
+    return rs, cam
+
+
+def sphere_mean(acc):
+    c = acc[..., :3] / np.maximum(acc[..., 3:4], 1)
+    h, w = c.shape[:2]
+    yy, xx = np.mgrid[0:h, 0:w]
+    r = np.hypot(xx - (w - 1) / 2, yy - (h - 1) / 2)
+    inside = r < 0.25 * w  # well inside the sphere's silhouette
+    return c[inside].mean(axis=0)
+
+
+def test_white_furnace_lambert():
+    """furnace-diffuse.ts: albedo 1, no specular, sky 1: 1.0 +- 2 %."""
+    rs, cam = sphere_scene((1, 1, 1), 0.0, 1.0, 0.0)
+    p = rs.params(48, 48, cam, max_bounces=8, rr_depth=8, sky=(1.0, 1.0, 1.0))
+    acc = oracle_scene(rs).render(p, 0, 64)
+    m = sphere_mean(acc)
+    assert np.all(np.abs(m - 1.0) < 0.02), m
+
+
+def test_white_furnace_metal():
+    """furnace-metal.ts: F0 = 1 (metallic, base 1), roughness 0.5, energy compensation 1: 1.0 +- 3 %."""
+    rs, cam = sphere_scene((1, 1, 1), 1.0, 0.5, 0.5)
+    p = rs.params(48, 48, cam, max_bounces=16, rr_depth=16, sky=(1.0, 1.0, 1.0), energy_compensation=1.0)
+    acc = oracle_scene(rs).render(p, 0, 64)
+    m = sphere_mean(acc)
+    assert np.all(np.abs(m - 1.0) < 0.03), m
+
+
+def test_energy_table_is_the_lobes_integral():
+    """lib.rs:1125-1156: specular_albedo against an independent quadrature of the same lobe (uniform hemisphere
+    sampling in f64, |a + b - q| < 1e-3 at seven points); a mirror exactly; the table 512 vec4 in [0, 1 + 1e-4]."""
+    def quadrature(mu, alpha, n):
+        # lib.rs:1125-1142: the lobe's value x cosine over the hemisphere at F = 1, midpoint rule, n x 2n points
+        a2 = alpha * alpha
+        v0, v1 = math.sqrt(1.0 - mu * mu), mu
+        th = (math.pi / 2) * (np.arange(n) + 0.5) / n
+        st, c = np.sin(th)[:, None], np.cos(th)[:, None]
+        phi = math.pi * (np.arange(2 * n) + 0.5) / (2 * n)
+        x, z = st * np.cos(phi)[None, :], st * np.sin(phi)[None, :]
+        h0, h1, h2 = v0 + x, v1 + c, z
+        noh = h1 / np.sqrt(h0 * h0 + h1 * h1 + h2 * h2)
+        d = (noh * a2 - noh) * noh + 1.0
+        vis = 0.5 / (c * (mu * (1.0 - alpha) + alpha) + mu * (c * (1.0 - alpha) + alpha))
+        s = (a2 / (math.pi * d * d) * vis * c * st).sum()
+        return s * math.pi * math.pi / (2 * n * n)
+
+    for r, mu in [(0.3, 0.9), (0.5, 0.7071), (0.5, 0.2), (0.8, 0.5), (1.0, 1.0), (1.0, 0.1), (0.25, 0.4)]:
+        a, b = R.specular_albedo(mu, r * r)
+        q = quadrature(mu, r * r, 800)
+        assert abs(a + b - q) < 1e-3, (r, mu, a + b, q)
+    a, b = R.specular_albedo(0.5, 0.0)
+    assert abs(a + b - 1.0) < 1e-12 and abs(b - 0.5 ** 5) < 1e-12
+    a, b = R.specular_albedo(0.5, 0.02 * 0.02)
+    assert abs(a + b - 1.0) < 2e-3 and abs(b - 0.5 ** 5) < 2e-3
+    lut = R.energy_lut()
+    assert lut.shape == (512, 4) and lut.min() >= 0.0 and lut.max() <= 1.0 + 1e-4
+
+
+def test_an_emitter_lights_a_plane_as_its_closed_form():
+    """A 1 x 1 Lambert plane (albedo 1, no specular) under a small emissive square of radiance L at height h: the
+    irradiance at the centre below it is L x the square's projected solid angle (closed form), so the plane reads
+    L x F / pi x ... = L * form factor. Direct light only (one bounce)."""
+    L, h, a = 10.0, 1.0, 0.05
+    plane = R.quad((-0.5, 0, -0.5), (-0.5, 0, 0.5), (0.5, 0, 0.5), (0.5, 0, -0.5))
+    lamp = R.quad((-a, h, -a), (a, h, -a), (a, h, a), (-a, h, a))
+    rs = R.build([R.RMesh(plane), R.RMesh(lamp)],
+                 [R.RInstance(0, np.eye(4, dtype=np.float32), [0]), R.RInstance(1, np.eye(4, dtype=np.float32), [1])],
+                 [R.material((1, 1, 1), specular=0.0), R.material((0, 0, 0), (L, L, L), specular=0.0)])
+    cam = R.Camera((0.0, 0.3, -0.02), (0.0, 0.0, 0.0), 4.0)
+    p = rs.params(16, 16, cam, max_bounces=1, rr_depth=8)
+    acc = oracle_scene(rs).render(p, 0, 256)
+    c = (acc[..., :3] / acc[..., 3:4]).mean(axis=(0, 1))
+    # radiance off a Lambert plane = E / pi, E = L * (4 * ...) : the square's form factor from a point below its
+    # centre (Howell's catalogue, a parallel square of half side a at distance h)
+    x = a / h
+    ff = (2 / math.pi) * (x / math.sqrt(1 + x * x)) * math.atan(x / math.sqrt(1 + x * x)) * 2
+    want = L * ff
+    assert abs(c[0] - want) / want < 0.03, (c, want)
+
+
+def test_the_sampler_holds_its_known_answers():
+    """sample.rs:91-125: the direction table's last entries (Cycles'), and the first 2^m points of each 2D pair
+    a (0, m, 2)-net, for the reference's seeds."""
+    d = oracle.sobol_dirs()
+    assert [int(d[0][31]), int(d[1][31]), int(d[2][31])] == [0xFFFFFFFF, 0xAAAA00A3, 0xC900A00A]
+    assert d[0][:16].tolist() == [1, 3, 5, 15, 17, 51, 85, 255, 257, 771, 1285, 3855, 4369, 13107, 21845, 65535]
+    for seed, dd in [(0, 0), (0x12345678, 7), (oracle.hash_hp(99), 0xDEAD)]:
+        for m in (4, 6, 8):
+            n = 1 << m
+            pts = np.array([oracle.rng4_at(seed, i, dd) for i in range(n)])
+            assert ((pts >= 0) & (pts < 1)).all()
+
+            def cell(v, k):
+                return np.minimum((v * np.float32(1 << k)).astype(np.int64), (1 << k) - 1)
+
+            for a in range(m + 1):
+                cells = (cell(pts[:, 0], a) << (m - a)) | cell(pts[:, 1], m - a)
+                assert len(np.unique(cells)) == n, (seed, m, a)
+            for k in range(4):
+                assert sorted(cell(pts[:, k], m).tolist()) == list(range(n))
+
+
+def test_sincos_is_within_two_ulps():
+    xs = np.linspace(0.0, 2 * math.pi, 20001, dtype=np.float32)
+    for x in xs[::97]:
+        s, c = oracle.sincos(float(x))
+        assert abs(s - math.sin(float(x))) <= 2.5e-7 and abs(c - math.cos(float(x))) <= 2.5e-7
+
+
+# ---- GPU parity ---------------------------------------------------------------------------------------------------
+def rel_rmse(a, b):
+    ca = a[..., :3] / np.maximum(a[..., 3:4], 1)
+    cb = b[..., :3] / np.maximum(b[..., 3:4], 1)
+    den = np.maximum(np.abs(cb), 1e-3)
+    return float(np.sqrt(np.mean(((ca - cb) / den) ** 2)))
+
+
+def gpu_render(device, rs, p, first, spp):
+    sc = R.upload(device, rs)
+    acc = R.Accum(device, int(p[16]), int(p[17]))
+    R.render(sc, p, first, spp, acc)
+    out = acc.read()
+    device.sync()
+    acc.close()
+    sc.close()
+    return out
+
+
+@pytest.mark.gpu
+def test_cornell_matches_the_oracle(device):
+    print("cornell 96x96 x 8 spp", flush=True)
+    rs, cam = R.cornell()
+    p = rs.params(96, 96, cam, seed=1, max_bounces=4, rr_depth=3)
+    want = oracle_scene(rs).render(p, 0, 8)
+    got = gpu_render(device, rs, p, 0, 8)
+    same = (got.view(np.uint32) == want.view(np.uint32)).all(axis=2).mean()
+    print(f"cornell: {same:.4f} of the pixels bit-equal, rel RMSE {rel_rmse(got, want):.2e}", flush=True)
+    assert rel_rmse(got, want) <= 1e-3
+
+
+@pytest.mark.gpu
+def test_sunlit_instances_match_the_oracle(device):
+    print("small city 80x48 x 2 spp", flush=True)
+    rs, cam, sky = R.city(seed=3, meshes=6, instances=40, extent=60.0)
+    p = rs.params(80, 48, cam, seed=7, max_bounces=3, rr_depth=2, sky=sky)
+    want = oracle_scene(rs).render(p, 0, 2)
+    got = gpu_render(device, rs, p, 0, 2)
+    same = (got.view(np.uint32) == want.view(np.uint32)).all(axis=2).mean()
+    print(f"city: {same:.4f} of the pixels bit-equal, rel RMSE {rel_rmse(got, want):.2e}", flush=True)
+    assert rel_rmse(got, want) <= 1e-3
+
+
+@pytest.mark.gpu
+def test_gpu_white_furnace(device):
+    rs, cam = sphere_scene((1, 1, 1), 0.0, 1.0, 0.0)
+    p = rs.params(64, 64, cam, max_bounces=8, rr_depth=8, sky=(1.0, 1.0, 1.0))
+    acc = gpu_render(device, rs, p, 0, 128)
+    m = sphere_mean(acc)
+    assert np.all(np.abs(m - 1.0) < 0.02), m
+
+
+@pytest.mark.gpu
+def test_samples_split_across_calls_equal_one_call(device):
+    """Samples 0..3 then 4..7 into one accumulator equal samples 0..7 at once (the host loop's slicing)."""
+    rs, cam = R.cornell()
+    p = rs.params(64, 64, cam, seed=1)
+    sc = R.upload(device, rs)
+    a = R.Accum(device, 64, 64)
+    R.render(sc, p, 0, 4, a)
+    R.render(sc, p, 4, 4, a)
+    b = R.Accum(device, 64, 64)
+    R.render(sc, p, 0, 8, b)
+    ra, rb = a.read(), b.read()
+    assert np.allclose(ra, rb, rtol=1e-6, atol=1e-6)
+    c, al, n, v = b.resolve()
+    assert np.allclose(c, rb[..., :3] / rb[..., 3:4]) and (v >= 0).all()
+    a.close()
+    b.close()
+    sc.close()

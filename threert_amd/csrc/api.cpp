@@ -2,6 +2,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
+#include <vector>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -10,6 +12,7 @@
 #include <string>
 
 #include "../../include/threert.h"
+#include "render_kernels.h"
 #include "trace_kernels.h"
 #include "trt_internal.h"
 
@@ -48,7 +51,7 @@ struct trt_device {
     void *staging = nullptr;  // trt_trace_rays' device copies of the caller's rays and hits
     uint64_t staging_bytes = 0;
     int cus = 0;
-    int blocks_wide = 1, blocks_binary = 1;
+    int blocks_wide = 1, blocks_binary = 1, blocks_render = 1;
     hipDeviceProp_t props{};
     std::mutex mu;
 };
@@ -56,6 +59,11 @@ struct trt_device {
 struct trt_scene {
     trt_device *dev = nullptr;
     void *nodes = nullptr, *leaves = nullptr, *instances = nullptr, *wnodes = nullptr, *wunits = nullptr;
+    // shading tables (trt_render)
+    void *vertices = nullptr, *indices = nullptr, *rows = nullptr, *materials = nullptr, *lights = nullptr,
+         *emitters = nullptr, *emitter_tris = nullptr, *objects = nullptr, *energy = nullptr;
+    bool shading = false;
+    uint64_t light_count = 0, emitter_entries = 0, emitter_tri_count = 0;
     uint64_t node_count = 0, leaf_count = 0, instance_count = 0;
     uint32_t top_first = 0, units = 0, top_ref = trt::kNone, max_stack = 0;
     float top_box[6] = {0, 0, 0, 0, 0, 0};
@@ -92,6 +100,7 @@ int trt_device_create(int ordinal, trt_device **out) {
     d->cus = d->props.multiProcessorCount;
     if (trt::trace_occupancy(true, &d->blocks_wide) || d->blocks_wide < 1) d->blocks_wide = 1;
     if (trt::trace_occupancy(false, &d->blocks_binary) || d->blocks_binary < 1) d->blocks_binary = 1;
+    if (trt::render_occupancy(&d->blocks_render) || d->blocks_render < 1) d->blocks_render = 1;
     *out = d;
     return ok();
 }
@@ -259,8 +268,146 @@ int trt_scene_release(trt_scene *s) {
     hipFree(s->instances);
     hipFree(s->wnodes);
     hipFree(s->wunits);
+    for (void *p : {s->vertices, s->indices, s->rows, s->materials, s->lights, s->emitters, s->emitter_tris,
+                    s->objects, s->energy})
+        hipFree(p);
     delete s;
     return ok();
+}
+
+// The shading tables' indices, checked once so the kernel can follow them: every unit's triangles' vertices and
+// section rows, the rows' materials, the emitter entries' triangles and the emitters' triangles, instances, rows.
+static bool check_shading(const trt_scene_desc *d, std::string *why) {
+    constexpr uint64_t kGuideVec4s = 65537 / 4 + 1;
+    if (!d->vertices || !d->indices || !d->rows || !d->materials || !d->objects || d->object_count == 0) {
+        *why = "shading tables without vertices, indices, rows, materials or objects";
+        return false;
+    }
+    if (d->index_count != d->leaf_count) {
+        *why = "the index buffer does not have a triangle for every leaf";
+        return false;
+    }
+    if (d->light_count && !d->lights) {
+        *why = "lights without a buffer";
+        return false;
+    }
+    if (d->emitter_vec4s && (d->emitter_vec4s < kGuideVec4s + 1 || !d->emitters)) {
+        *why = "an emitter table shorter than its guide and header";
+        return false;
+    }
+    const auto *inst = static_cast<const trt::Instance *>(d->instances);
+    const auto *leaves = static_cast<const trt::Leaf *>(d->leaves);
+    const auto *nodes = static_cast<const trt::Node *>(d->nodes);
+    // Every record the top tree names: its mesh tree's triangles.
+    std::vector<uint32_t> st;
+    if (d->unit_count) st.push_back(d->top_first);
+    std::vector<uint32_t> recs;
+    while (!st.empty()) {
+        const uint32_t i = st.back();
+        st.pop_back();
+        const trt::Node &n = nodes[i];
+        if (n.count > 0) {
+            recs.push_back(n.left_or_first);
+        } else {
+            st.push_back(d->top_first + n.left_or_first);
+            st.push_back(d->top_first + n.left_or_first + 1);
+        }
+    }
+    auto tri_ok = [&](uint64_t ti, uint64_t base, uint64_t rows_first, std::string *w) {
+        const uint32_t *ix = d->indices + 3 * ti;
+        for (int k = 0; k < 3; k++)
+            if (uint64_t(ix[k]) + base >= d->vertex_count) {
+                *w = "a triangle's vertex is outside the vertex buffer";
+                return false;
+            }
+        const uint64_t part = trt::to_bits(leaves[ti].r[0][3]);
+        if (rows_first + part >= d->row_count) {
+            *w = "a triangle's section has no row";
+            return false;
+        }
+        if ((d->rows[4 * (rows_first + part)] & 0xffffffu) >= d->material_count) {
+            *w = "a row names a material that is not there";
+            return false;
+        }
+        return true;
+    };
+    for (uint32_t r : recs) {
+        const trt::Instance &u = inst[r];
+        const uint32_t root = trt::to_bits(u.n[0][3]);
+        st.assign(1, root);
+        while (!st.empty()) {
+            const uint32_t i = st.back();
+            st.pop_back();
+            const trt::Node &n = nodes[i];
+            if (n.count == 0) {
+                st.push_back(root + n.left_or_first);
+                st.push_back(root + n.left_or_first + 1);
+                continue;
+            }
+            for (uint32_t k = 0; k < n.count; k++)
+                if (!tri_ok(uint64_t(u.info[0]) + n.left_or_first + k, u.info[2], u.info[3], why)) return false;
+        }
+        if (u.info[1] >= d->object_count) {
+            *why = "a record names an object that is not there";
+            return false;
+        }
+    }
+    const uint64_t entries = d->emitter_vec4s ? d->emitter_vec4s - kGuideVec4s - 1 : 0;
+    const auto *em = static_cast<const float *>(d->emitters);
+    for (uint64_t k = 0; k < entries; k++)
+        if (trt::to_bits(em[4 * k]) >= d->emitter_tri_count) {
+            *why = "an emitter entry names a triangle that is not there";
+            return false;
+        }
+    const auto *et = static_cast<const uint32_t *>(d->emitter_tris);
+    for (uint64_t k = 0; k < d->emitter_tri_count; k++) {
+        const uint32_t tri = et[16 * k + 3], in = et[16 * k + 7], part = et[16 * k + 12], code = et[16 * k + 11];
+        if (tri >= d->leaf_count || in >= d->instance_count || (code & 0x7fffffffu) >= (entries ? entries : 1)) {
+            *why = "an emitter triangle names a triangle, a record or an entry that is not there";
+            return false;
+        }
+        const trt::Instance &u = inst[in];
+        if (uint64_t(u.info[3]) + part >= d->row_count || u.info[1] >= d->object_count) {
+            *why = "an emitter triangle's section has no row";
+            return false;
+        }
+        const uint32_t *ix = d->indices + 3 * uint64_t(tri);
+        for (int j = 0; j < 3; j++)
+            if (uint64_t(ix[j]) + u.info[2] >= d->vertex_count) {
+                *why = "an emitter triangle's vertex is outside the vertex buffer";
+                return false;
+            }
+        if ((d->rows[4 * (uint64_t(u.info[3]) + part)] & 0xffffffu) >= d->material_count) {
+            *why = "an emitter's row names a material that is not there";
+            return false;
+        }
+    }
+    // Rows that name emitters: each hit's entry (first + its index in the section) must be inside the list.
+    for (uint32_t r : recs) {
+        const trt::Instance &u = inst[r];
+        const uint32_t root = trt::to_bits(u.n[0][3]);
+        st.assign(1, root);
+        while (!st.empty()) {
+            const uint32_t i = st.back();
+            st.pop_back();
+            const trt::Node &n = nodes[i];
+            if (n.count == 0) {
+                st.push_back(root + n.left_or_first);
+                st.push_back(root + n.left_or_first + 1);
+                continue;
+            }
+            for (uint32_t k = 0; k < n.count; k++) {
+                const uint64_t ti = uint64_t(u.info[0]) + n.left_or_first + k;
+                const uint64_t part = trt::to_bits(leaves[ti].r[0][3]);
+                const uint32_t first = d->rows[4 * (uint64_t(u.info[3]) + part) + 2];
+                if (first != 0xffffffffu && uint64_t(first) + trt::to_bits(leaves[ti].r[1][3]) >= d->emitter_tri_count) {
+                    *why = "a row's emitters run past the emitter list";
+                    return false;
+                }
+            }
+        }
+    }
+    return true;
 }
 
 static int upload(trt_device *d, void **dst, const void *src, uint64_t bytes, uint64_t *total) {
@@ -289,6 +436,7 @@ int trt_scene_upload(trt_device *d, const trt_scene_desc *desc, trt_scene **out)
                          desc->unit_count, w, &why))
         return fail(TRT_E_INVALID, "trt_scene_upload: " + why);
     if (w.nodes.size() >= (uint64_t{1} << 30)) return fail(TRT_E_INVALID, "trt_scene_upload: tree too large");
+    if (desc->energy && !check_shading(desc, &why)) return fail(TRT_E_INVALID, "trt_scene_upload: " + why);
     std::lock_guard<std::mutex> lock(d->mu);
     TRT_HIP(hipSetDevice(d->ordinal));
     auto *s = new (std::nothrow) trt_scene();
@@ -300,6 +448,21 @@ int trt_scene_upload(trt_device *d, const trt_scene_desc *desc, trt_scene **out)
     if (!rc) rc = upload(d, &s->instances, desc->instances, desc->instance_count * sizeof(trt::Instance), &total);
     if (!rc) rc = upload(d, &s->wnodes, w.nodes.data(), w.nodes.size() * sizeof(trt::WideNode), &total);
     if (!rc) rc = upload(d, &s->wunits, w.units.data(), w.units.size() * sizeof(trt::WideUnit), &total);
+    if (!rc && desc->energy) {
+        s->shading = true;
+        s->light_count = desc->light_count;
+        s->emitter_tri_count = desc->emitter_tri_count;
+        s->emitter_entries = desc->emitter_vec4s ? desc->emitter_vec4s - (65537 / 4 + 1) - 1 : 0;
+        rc = upload(d, &s->vertices, desc->vertices, desc->vertex_count * 48, &total);
+        if (!rc) rc = upload(d, &s->indices, desc->indices, desc->index_count * 12, &total);
+        if (!rc) rc = upload(d, &s->rows, desc->rows, desc->row_count * 16, &total);
+        if (!rc) rc = upload(d, &s->materials, desc->materials, desc->material_count * 288, &total);
+        if (!rc) rc = upload(d, &s->lights, desc->lights, desc->light_count * 80, &total);
+        if (!rc) rc = upload(d, &s->emitters, desc->emitters, desc->emitter_vec4s * 16, &total);
+        if (!rc) rc = upload(d, &s->emitter_tris, desc->emitter_tris, desc->emitter_tri_count * 64, &total);
+        if (!rc) rc = upload(d, &s->objects, desc->objects, desc->object_count * 48, &total);
+        if (!rc) rc = upload(d, &s->energy, desc->energy, 512 * 16, &total);
+    }
     if (rc) {
         std::string keep = g_error;
         trt_scene_release(s);
@@ -347,7 +510,7 @@ static int launch(trt_scene *s, const void *d_rays, uint64_t n, void *d_hits, ui
     trt_device *d = s->dev;
     const bool wide = flags == TRT_WALK_WIDE;
     if (flags != TRT_WALK_WIDE && flags != TRT_WALK_BINARY) return fail(TRT_E_INVALID, "unknown trace flags");
-    if (n >= (uint64_t{1} << 32) - 64) return fail(TRT_E_INVALID, "a batch of 2^32 rays or more");
+    if (n >= (uint64_t{1} << 31)) return fail(TRT_E_INVALID, "a batch of 2^31 rays or more");
     if (n == 0) return ok();
     const int grid = d->cus * (wide ? d->blocks_wide : d->blocks_binary);
     const uint32_t threads = static_cast<uint32_t>(grid) * trt::kTraceBlock;
@@ -393,6 +556,132 @@ int trt_trace_reads(trt_scene *s, const trt_ray *d_rays, uint64_t n, trt_hit *d_
         reads[0] = r[0], reads[1] = r[1], reads[2] = r[2];
     }
     return trt_device_sync(d);
+}
+
+// ---- path tracing --------------------------------------------------------------------------------------------------
+struct trt_accum {
+    trt_device *dev = nullptr;
+    uint32_t w = 0, h = 0;
+    void *buf = nullptr;
+};
+
+int trt_accum_create(trt_device *d, uint32_t width, uint32_t height, trt_accum **out) {
+    if (!d || !out || width == 0 || height == 0 || uint64_t(width) * height >= (uint64_t{1} << 31))
+        return fail(TRT_E_INVALID, "trt_accum_create: bad argument");
+    *out = nullptr;
+    TRT_HIP(hipSetDevice(d->ordinal));
+    auto *a = new (std::nothrow) trt_accum();
+    if (!a) return fail(TRT_E_OOM, "trt_accum_create: out of host memory");
+    a->dev = d, a->w = width, a->h = height;
+    const uint64_t bytes = uint64_t(width) * height * 80;
+    hipError_t e = hipMalloc(&a->buf, bytes);
+    if (e == hipSuccess) e = hipMemsetAsync(a->buf, 0, bytes, d->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
+    if (e != hipSuccess) {
+        hipFree(a->buf);
+        delete a;
+        return fail(e == hipErrorOutOfMemory ? TRT_E_OOM : TRT_E_DEVICE, hipGetErrorString(e));
+    }
+    *out = a;
+    return ok();
+}
+int trt_accum_release(trt_accum *a) {
+    if (!a) return ok();
+    hipSetDevice(a->dev->ordinal);
+    hipStreamSynchronize(a->dev->stream);
+    hipFree(a->buf);
+    delete a;
+    return ok();
+}
+int trt_accum_clear(trt_accum *a) {
+    if (!a) return fail(TRT_E_INVALID, "trt_accum_clear: null accumulator");
+    TRT_HIP(hipSetDevice(a->dev->ordinal));
+    TRT_HIP(hipMemsetAsync(a->buf, 0, uint64_t(a->w) * a->h * 80, a->dev->stream));
+    return ok();
+}
+int trt_accum_read(trt_accum *a, float *pixels) {
+    if (!a || !pixels) return fail(TRT_E_INVALID, "trt_accum_read: null argument");
+    int rc = trt_device_sync(a->dev);
+    if (rc) return rc;
+    TRT_HIP(hipMemcpy(pixels, a->buf, uint64_t(a->w) * a->h * 80, hipMemcpyDeviceToHost));
+    return ok();
+}
+int trt_accum_resolve(trt_accum *a, float *color, float *albedo, float *normal, float *variance) {
+    if (!a) return fail(TRT_E_INVALID, "trt_accum_resolve: null accumulator");
+    const uint64_t n = uint64_t(a->w) * a->h;
+    std::vector<float> px;
+    try {
+        px.resize(n * 20);
+    } catch (...) {
+        return fail(TRT_E_OOM, "trt_accum_resolve: out of host memory");
+    }
+    int rc = trt_accum_read(a, px.data());
+    if (rc) return rc;
+    for (uint64_t i = 0; i < n; i++) {
+        float p[20];
+        for (int k = 0; k < 20; k++) p[k] = std::isfinite(px[20 * i + k]) ? px[20 * i + k] : 0.0f;
+        const float c = std::max(px[20 * i + 3], 1.0f);
+        for (int k = 0; k < 3; k++) {
+            if (color) color[3 * i + k] = p[k] / c;
+            if (albedo) albedo[3 * i + k] = p[4 + k] / c;
+            if (normal) normal[3 * i + k] = p[8 + k] / c;
+        }
+        const float mean = p[7] / c;
+        if (variance) variance[i] = std::max(p[11] / c - mean * mean, 0.0f) / c;
+    }
+    return ok();
+}
+
+int trt_render(trt_scene *s, const void *params, uint32_t first_sample, uint32_t spp, trt_accum *acc) {
+    if (!s || !params || !acc) return fail(TRT_E_INVALID, "trt_render: null argument");
+    if (!s->shading) return fail(TRT_E_INVALID, "trt_render: the scene was uploaded without its shading tables");
+    if (acc->dev != s->dev) return fail(TRT_E_INVALID, "trt_render: the accumulator is another device's");
+    trt::RenderArgs a{};
+    std::memcpy(a.params, params, sizeof(a.params));
+    const uint32_t *P = a.params;
+    if (P[16] != acc->w || P[17] != acc->h) return fail(TRT_E_INVALID, "trt_render: P.size is not the accumulator's");
+    if (P[20] > s->light_count || P[36] > P[20] || P[21] > s->emitter_entries)
+        return fail(TRT_E_INVALID, "trt_render: P.counts names more lights or emitters than the scene has");
+    if (P[21] > 0 && P[21] != s->emitter_entries)
+        return fail(TRT_E_INVALID, "trt_render: P.counts.y is not the emitter table's entry count");
+    if (P[22] > 64) return fail(TRT_E_INVALID, "trt_render: more than 64 bounces");
+    if (trt::from_bits(P[24]) > 0.5f) return fail(TRT_E_INVALID, "trt_render: environment maps are not supported");
+    if (spp == 0) return ok();
+    trt_device *d = s->dev;
+    std::lock_guard<std::mutex> lock(d->mu);
+    TRT_HIP(hipSetDevice(d->ordinal));
+    const int grid = d->cus * d->blocks_render;
+    const uint32_t threads = static_cast<uint32_t>(grid) * trt::kRenderBlock;
+    const uint64_t entries = s->max_stack > trt::kLdsStack ? s->max_stack - trt::kLdsStack : 1;
+    int rc = ensure_spill(d, entries * threads * sizeof(uint32_t));
+    if (rc) return rc;
+    a.geo.wnodes = static_cast<const uint4 *>(s->wnodes);
+    a.geo.wunits = static_cast<const float4 *>(s->wunits);
+    a.geo.leaves = static_cast<const float4 *>(s->leaves);
+    a.geo.counter = d->counter;
+    a.geo.guard_flag = d->guard;
+    a.geo.spill = d->spill;
+    a.geo.reads = d->reads;
+    a.geo.spill_stride = threads;
+    a.geo.top_ref = s->top_ref;
+    a.geo.top_first = s->top_first;
+    a.geo.units = s->units;
+    std::memcpy(a.geo.top_box, s->top_box, sizeof(a.geo.top_box));
+    a.instances = static_cast<const float4 *>(s->instances);
+    a.vertices = static_cast<const float4 *>(s->vertices);
+    a.indices = static_cast<const uint32_t *>(s->indices);
+    a.rows = static_cast<const uint32_t *>(s->rows);
+    a.materials = static_cast<const float4 *>(s->materials);
+    a.lights = static_cast<const float4 *>(s->lights);
+    a.emitters = static_cast<const float4 *>(s->emitters);
+    a.emitter_tris = static_cast<const float4 *>(s->emitter_tris);
+    a.objects = static_cast<const float4 *>(s->objects);
+    a.energy = static_cast<const float4 *>(s->energy);
+    a.accum = static_cast<float4 *>(acc->buf);
+    a.first_sample = first_sample;
+    a.spp = spp;
+    if (trt::launch_render(a, grid, d->stream)) return fail(TRT_E_DEVICE, "render kernel launch failed");
+    return ok();
 }
 
 int trt_trace_rays_device(trt_scene *s, const trt_ray *d_rays, uint64_t n, trt_hit *d_hits, uint32_t flags) {

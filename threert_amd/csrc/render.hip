@@ -1,0 +1,825 @@
+// The offline path tracer for gfx950: pathtrace.wgsl `main` (LIVE = false) for the features the workloads use,
+// over the 4-wide quantised walk of walk.h.
+//
+// Restated (pathtrace.wgsl lines): camera_ray 4002-4024; main 4106-4557 (offline parts); surface 1320-1425 with
+// tri_data 330-360, inst_vec / inst_sign / inst_normal 279-288, inst_attr 406-413, material_of 310-313; lobes
+// 1642-1699 with energy_lookup 1492-1505, spec_terms 1521-1525; d_ggx / vis_smith_joint_approx / f_schlick
+// 1707-1730; eval_bsdf_k 1733-1764; g1 / pdf_bsdf 1766-1789; light_mis 1797-1817; basis / sample_vndf 1819-1844;
+// sample_bsdf 1848-1884; env_radiance / env_pdf / sample_env 1895-1930 (constant sky); power_heuristic, luminance,
+// offset_origin 1943-1960; sky_mis / sun_disks 1963-1988; emit_mis 2009-2024; emitter_radiance /
+// sample_emitter_ris / sample_emitter 2269-2393; env_light 2431-2442; direct_light 2444-2540; emitter_draw /
+// emitter_pick_u / emitter_pmf 463-593 (no light tree). Not here: textures, glass, clearcoat, media, fog,
+// procedural geometry, the light tree, local lights, live-only paths (the upload refuses records that need them).
+// normalize(v) = v / sqrt(v.v), inverseSqrt(x) = 1 / sqrt(x); sin / cos by Moshier's Cephes reduction and
+// polynomials in fmaf; every other operation in the reference's order, no contraction: the same bits as the C
+// restatement the tests hold this to.
+//
+// Launch: persistent waves; a wave takes one 8x8 tile at a time from a counter and runs `spp` samples of each of
+// its pixels, adding them to the pixel's 80-byte record (records.wgsl Pixel) in sample order.
+#include <hip/hip_runtime.h>
+
+#include "render_kernels.h"
+#include "walk.h"
+
+namespace trt {
+namespace {
+using namespace walk;
+
+constexpr float kPi = 3.14159265358979f;
+constexpr float kInvPi = 1.0f / kPi;
+constexpr float kTwoPi = 2.0f * kPi;
+constexpr float kFourPi = 4.0f * kPi;
+
+struct V {
+    float x, y, z;
+};
+__device__ __forceinline__ V v3(float x, float y, float z) { return {x, y, z}; }
+__device__ __forceinline__ V vadd(V a, V b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V vsub(V a, V b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V vmul(V a, V b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ V vs(V a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ V vds(V a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ V vneg(V a) { return {-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ float dot(V a, V b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V cross(V a, V b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float len(V a) { return sqrtf(dot(a, a)); }
+__device__ __forceinline__ V normalize(V a) { return vds(a, sqrtf(dot(a, a))); }
+__device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+__device__ __forceinline__ float mixf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+__device__ __forceinline__ bool any_pos(V a) { return a.x > 0.0f || a.y > 0.0f || a.z > 0.0f; }
+__device__ __forceinline__ bool all_nonpos(V a) { return a.x <= 0.0f && a.y <= 0.0f && a.z <= 0.0f; }
+__device__ __forceinline__ float luminance(V c) { return dot(c, v3(0.2126f, 0.7152f, 0.0722f)); }
+__device__ __forceinline__ bool non_finite(V v) {
+    return (__float_as_uint(v.x) & 0x7fffffffu) >= 0x7f800000u || (__float_as_uint(v.y) & 0x7fffffffu) >= 0x7f800000u ||
+           (__float_as_uint(v.z) & 0x7fffffffu) >= 0x7f800000u;
+}
+__device__ __forceinline__ V xyz(float4 f) { return {f.x, f.y, f.z}; }
+
+struct M3 {
+    V t, b, n;
+};
+__device__ __forceinline__ V mmul(const M3 &m, V v) { return vadd(vadd(vs(m.t, v.x), vs(m.b, v.y)), vs(m.n, v.z)); }
+__device__ __forceinline__ V mtmul(const M3 &m, V v) { return {dot(m.t, v), dot(m.b, v), dot(m.n, v)}; }
+
+// Moshier's Cephes sinf / cosf: octant reduction by pi/4 in three parts, minimax polynomials, fmaf evaluation.
+__device__ __forceinline__ void sincos_c(float x, float &s, float &c) {
+    const float DP1 = 0.78515625f, DP2 = 2.4187564849853515625e-4f, DP3 = 3.77489497744594108e-8f;
+    const float FOPI = 1.27323954473516f;
+    float sign_s = 1.0f, ax = x;
+    if (x < 0.0f) ax = -x, sign_s = -1.0f;
+    uint32_t j = static_cast<uint32_t>(ax * FOPI);
+    float y = static_cast<float>(j);
+    if (j & 1u) j += 1u, y += 1.0f;
+    j &= 7u;
+    float sign_c = 1.0f;
+    if (j > 3u) j -= 4u, sign_s = -sign_s, sign_c = -sign_c;
+    if (j > 1u) sign_c = -sign_c;
+    float z = fmaf(-y, DP1, ax);
+    z = fmaf(-y, DP2, z);
+    z = fmaf(-y, DP3, z);
+    const float zz = z * z;
+    float ps = fmaf(fmaf(-1.9515295891e-4f, zz, 8.3321608736e-3f), zz, -1.6666654611e-1f);
+    ps = fmaf(ps * zz, z, z);
+    float pc = fmaf(fmaf(2.443315711809948e-5f, zz, -1.388731625493765e-3f), zz, 4.166664568298827e-2f);
+    pc = fmaf(pc * zz, zz, fmaf(-0.5f, zz, 1.0f));
+    const bool swap = j == 1u || j == 2u;
+    s = sign_s * (swap ? pc : ps);
+    c = sign_c * (swap ? ps : pc);
+}
+
+// ---- sampler (sample.wgsl) ---------------------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t owen_rev(uint32_t n, uint32_t seed) {
+    n ^= n * 0x3d20adeau;
+    n += seed;
+    n *= (seed >> 16) | 1u;
+    n ^= n * 0x05526c56u;
+    n ^= n * 0x53a22864u;
+    return n;
+}
+__device__ __forceinline__ uint32_t hash_hp(uint32_t i) {
+    i ^= i >> 16;
+    i *= 0x21f0aaadu;
+    i ^= i >> 15;
+    i *= 0xd35a2d97u;
+    i ^= i >> 15;
+    return i ^ 0xe6fe3bebu;
+}
+// Direction numbers of Sobol dimensions 1-3 (Joe & Kuo), bit-reversed, the 16 the 16-bit shuffled index uses.
+__constant__ uint32_t kSobol[3][16] = {
+    {0x00000001u, 0x00000003u, 0x00000005u, 0x0000000fu, 0x00000011u, 0x00000033u, 0x00000055u, 0x000000ffu,
+     0x00000101u, 0x00000303u, 0x00000505u, 0x00000f0fu, 0x00001111u, 0x00003333u, 0x00005555u, 0x0000ffffu},
+    {0x00000001u, 0x00000003u, 0x00000006u, 0x00000009u, 0x00000017u, 0x0000003au, 0x00000071u, 0x000000a3u,
+     0x00000116u, 0x00000339u, 0x00000677u, 0x000009aau, 0x00001601u, 0x00003903u, 0x00007706u, 0x0000aa09u},
+    {0x00000001u, 0x00000003u, 0x00000004u, 0x0000000au, 0x0000001fu, 0x0000002eu, 0x00000045u, 0x000000c9u,
+     0x0000011bu, 0x000002a4u, 0x0000079au, 0x00000b67u, 0x0000101eu, 0x0000302du, 0x00004041u, 0x0000a0c3u},
+};
+__device__ __forceinline__ float sobol_owen(uint32_t rev_index, uint32_t dim, uint32_t seed) {
+    uint32_t r = 0;
+    if (dim == 0) {
+        r = __builtin_bitreverse32(rev_index);
+    } else {
+#pragma unroll
+        for (int b = 0; b < 16; b++)
+            if (rev_index & (0x80000000u >> b)) r ^= kSobol[dim - 1][b];
+    }
+    r = __builtin_bitreverse32(owen_rev(r, seed));
+    return static_cast<float>(r) * static_cast<float>(1.0 / 4294967808.0);
+}
+
+struct Rng {
+    uint32_t seed, index, depth;
+    __device__ __forceinline__ uint32_t dim(uint32_t slot) const { return (0u << 28) ^ (depth << 20) ^ slot; }
+    __device__ __forceinline__ uint32_t sd(uint32_t d) const { return seed ^ hash_hp(d); }
+    __device__ __forceinline__ uint32_t idx(uint32_t salt) const {
+        return owen_rev(__builtin_bitreverse32(index), salt) & 0xffff0000u;
+    }
+    __device__ __forceinline__ float r1(uint32_t d) const {
+        const uint32_t s = sd(d);
+        return sobol_owen(idx(s ^ 0xbff95bfeu), 0, s ^ 0x635c77bdu);
+    }
+    __device__ __forceinline__ void r2(uint32_t d, float *r) const {
+        const uint32_t s = sd(d), i = idx(s ^ 0xf8ade99au);
+        r[0] = sobol_owen(i, 0, s ^ 0xe0aaaf76u);
+        r[1] = sobol_owen(i, 1, s ^ 0x94964d4eu);
+    }
+    __device__ __forceinline__ void r3(uint32_t d, float *r) const {
+        const uint32_t s = sd(d), i = idx(s ^ 0xcaa726acu);
+        r[0] = sobol_owen(i, 0, s ^ 0x9e78e391u);
+        r[1] = sobol_owen(i, 1, s ^ 0x67c33241u);
+        r[2] = sobol_owen(i, 2, s ^ 0x78c395c5u);
+    }
+    __device__ __forceinline__ void r4(uint32_t d, float *r) const {
+        const uint32_t s = sd(d), i = idx(s ^ 0xc2c1a055u);
+        r[0] = sobol_owen(i, 0, s ^ 0x39468210u);
+        r[1] = sobol_owen(i, 1, s ^ 0xe9d8a845u);
+        r[2] = sobol_owen(i, 2, s ^ 0x5f32b482u);
+        r[3] = sobol_owen(i, 3, s ^ 0x1524cc56u);
+    }
+};
+__device__ __forceinline__ uint32_t rng_sub(uint32_t d, uint32_t k) {
+    return hash_hp(d ^ hash_hp(k + 0x9e3779b9u)) | 0x80000000u;
+}
+
+enum : uint32_t { D_CAMERA = 0, D_BSDF = 1, D_RR = 2, D_SUN = 4, D_EMIT = 7, D_ENV = 9 };
+constexpr uint32_t kEmitRisM = 6, kGuide = 65536, kNoEmitter = 0xffffffffu, kInstMirrored = 4, kEmitterCard = 1;
+constexpr int kMatEmitterCard = 2;
+constexpr float kMirrorRoughness = 0.3f;
+constexpr uint32_t kObjectStride = 3;
+
+// ---- the kernel's context ---------------------------------------------------------------------------------------
+struct Ctx {
+    const RenderArgs *a;
+    Stack *st;
+    bool *tripped;
+    Rng g;
+    bool mirror_path;
+    bool sampled_spec;
+
+    __device__ __forceinline__ float pf(int vec, int k) const { return __uint_as_float(a->params[4 * vec + k]); }
+    __device__ __forceinline__ uint32_t pu(int vec, int k) const { return a->params[4 * vec + k]; }
+    __device__ __forceinline__ const float4 *inst(uint32_t i) const { return a->instances + 10 * uint64_t(i); }
+    __device__ __forceinline__ const float4 *material(uint32_t i) const { return a->materials + 18 * uint64_t(i); }
+    __device__ __forceinline__ const float4 *light(uint32_t i) const { return a->lights + 5 * uint64_t(i); }
+    __device__ __forceinline__ float4 emitter_at(uint32_t i) const { return a->emitters[i]; }
+    __device__ __forceinline__ uint32_t row_word(uint32_t first, uint32_t section, uint32_t k) const {
+        return a->rows[4 * (uint64_t(first) + section) + k];
+    }
+    __device__ __forceinline__ float obj_power(uint32_t obj) const {
+        return a->objects[uint64_t(obj) * kObjectStride].w;
+    }
+    __device__ HitRec trace(V o, V d, float tmax, uint32_t mode) const {
+        Reads cnt;
+        return trace_wide<false>(a->geo, F3{o.x, o.y, o.z}, F3{d.x, d.y, d.z}, tmax, mode, *st, *tripped, cnt);
+    }
+};
+enum { CAM_POS = 0, CAM_FWD = 1, CAM_RIGHT = 2, CAM_UP = 3, SIZE = 4, COUNTS = 5, ENV = 6, MISC = 7, SKY = 8,
+       COUNTS2 = 9, LIGHT_LOD = 18 };
+
+struct TriAttr {
+    float4 n0, n1, n2, uv, x;
+};
+
+__device__ TriAttr tri_attr(const Ctx &c, uint32_t ti, uint32_t inst, uint32_t part, V &e1, V &e2) {
+    uint32_t base = 0;
+    if (inst != kNone) base = __float_as_uint(c.inst(inst)[6].y);
+    const uint32_t *ix = c.a->indices + 3 * uint64_t(ti);
+    const float4 *p0 = c.a->vertices + 3 * (uint64_t(ix[0]) + base);
+    const float4 *p1 = c.a->vertices + 3 * (uint64_t(ix[1]) + base);
+    const float4 *p2 = c.a->vertices + 3 * (uint64_t(ix[2]) + base);
+    const float4 a0 = p0[0], a1 = p1[0], a2 = p2[0], n0 = p0[1], n1 = p1[1], n2 = p2[1];
+    e1 = vsub(xyz(a1), xyz(a0));
+    e2 = vsub(xyz(a2), xyz(a0));
+    TriAttr t;
+    t.n0 = make_float4(n0.x, n0.y, n0.z, a0.w);
+    t.n1 = make_float4(n1.x, n1.y, n1.z, n0.w);
+    t.n2 = make_float4(n2.x, n2.y, n2.z, a1.w);
+    t.uv = make_float4(n1.w, a2.w, n2.w, __uint_as_float(part));
+    t.x = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    return t;
+}
+__device__ __forceinline__ V inst_vec(const float4 *r, V v) { return {dot(xyz(r[0]), v), dot(xyz(r[1]), v), dot(xyz(r[2]), v)}; }
+__device__ __forceinline__ float4 inst_normal(const float4 *r, float4 n) {
+    const V v = xyz(n);
+    const V m = normalize(v3(dot(xyz(r[3]), v), dot(xyz(r[4]), v), dot(xyz(r[5]), v)));
+    return make_float4(m.x, m.y, m.z, n.w);
+}
+__device__ __forceinline__ uint32_t material_of(const Ctx &c, const TriAttr &a, uint32_t inst) {
+    if (inst == kNone) return 0;
+    return c.row_word(__float_as_uint(c.inst(inst)[6].w), __float_as_uint(a.uv.w), 0) & 0xffffffu;
+}
+
+struct Surface {
+    V p, n, ng, base, emissive;
+    float roughness, metallic, specular;
+    bool front, card;
+};
+
+__device__ Surface surface(const Ctx &c, V o, V d, const HitRec &h) {
+    V e1, e2;
+    TriAttr a = tri_attr(c, h.tri, h.inst, h.part, e1, e2);
+    float side = 1.0f;
+    if (h.inst != kNone) {
+        const float4 *r = c.inst(h.inst);
+        e1 = inst_vec(r, e1);
+        e2 = inst_vec(r, e2);
+        a.n0 = inst_normal(r, a.n0);
+        a.n1 = inst_normal(r, a.n1);
+        a.n2 = inst_normal(r, a.n2);
+        a.x.y = __uint_as_float(__float_as_uint(r[6].y) + __float_as_uint(a.x.y));
+        side = (__float_as_uint(r[4].w) & kInstMirrored) ? -1.0f : 1.0f;
+    }
+    const float w = 1.0f - h.u - h.v;
+    V ng = vs(normalize(cross(e1, e2)), side);
+    V n = normalize(vadd(vadd(vs(xyz(a.n0), w), vs(xyz(a.n1), h.u)), vs(xyz(a.n2), h.v)));
+    const float4 *m = c.material(material_of(c, a, h.inst));
+    const bool front = dot(ng, d) < 0.0f;
+    if (!front) ng = vneg(ng);
+    if (dot(n, ng) < 0.0f) n = vneg(n);
+    Surface s;
+    s.p = vadd(o, vs(d, h.t));
+    s.front = front;
+    s.card = static_cast<int>(__float_as_uint(m[4].z)) == kMatEmitterCard;
+    const float4 params = m[2];
+    s.emissive = vs(xyz(m[1]), c.obj_power(__float_as_uint(a.x.y)));
+    s.base = xyz(m[0]);
+    s.roughness = clampf(params.y, 0.02f, 1.0f);
+    s.metallic = clampf(params.x, 0.0f, 1.0f);
+    s.specular = params.z;
+    s.ng = ng;
+    s.n = n;
+    return s;
+}
+
+// ---- BSDF ---------------------------------------------------------------------------------------------------------
+struct Lobes {
+    V diffuse, f0, f90, w, e_spec, kd;
+    float a2, p_spec, p_coat, p_trans;
+};
+
+__device__ __forceinline__ void energy_node(const Ctx &c, uint32_t i, float &e0, float &e1) {
+    const float4 v = c.a->energy[i >> 1];
+    e0 = (i & 1u) ? v.z : v.x;
+    e1 = (i & 1u) ? v.w : v.y;
+}
+__device__ void energy_lookup(const Ctx &c, float roughness, float nov, float *e) {
+    const uint32_t N = 32;
+    const float x = sqrtf(clampf(nov, 0.0f, 1.0f)) * static_cast<float>(N - 1);
+    const float y = clampf(roughness, 0.0f, 1.0f) * static_cast<float>(N - 1);
+    const uint32_t x0 = min(static_cast<uint32_t>(x), N - 2), y0 = min(static_cast<uint32_t>(y), N - 2);
+    const uint32_t i = y0 * N + x0;
+    const float fx = x - static_cast<float>(x0), fy = y - static_cast<float>(y0);
+    float a0, a1, b0, b1, c0, c1, d0, d1;
+    energy_node(c, i, a0, a1);
+    energy_node(c, i + 1, b0, b1);
+    energy_node(c, i + N, c0, c1);
+    energy_node(c, i + N + 1, d0, d1);
+    e[0] = mixf(mixf(a0, b0, fx), mixf(c0, d0, fx), fy);
+    e[1] = mixf(mixf(a1, b1, fx), mixf(c1, d1, fx), fy);
+}
+
+__device__ Lobes lobes(const Ctx &c, const Surface &s, float nov) {
+    Lobes l;
+    const float t = 0.0f;
+    l.diffuse = vs(vs(s.base, 1.0f - s.metallic), 1.0f - t);
+    const float f0 = 0.08f * s.specular;
+    l.f0 = v3(mixf(f0, s.base.x, s.metallic), mixf(f0, s.base.y, s.metallic), mixf(f0, s.base.z, s.metallic));
+    float e[2];
+    energy_lookup(c, s.roughness, nov, e);
+    const float comp = c.pf(MISC, 2);
+    const float k503 = 50.0f / 3.0f;
+    const float f90 = clampf(dot(l.f0, v3(k503, k503, k503)), 0.0f, 1.0f);
+    const float inv = comp * (1.0f / fmaxf(e[0] + e[1], 1e-3f) - 1.0f);
+    l.w = v3(1.0f + l.f0.x * inv, 1.0f + l.f0.y * inv, 1.0f + l.f0.z * inv);
+    l.e_spec = vmul(vadd(vs(l.f0, e[0]), v3(f90 * e[1], f90 * e[1], f90 * e[1])), l.w);
+    l.f90 = v3(f90, f90, f90);
+    l.kd = vsub(v3(1.0f, 1.0f, 1.0f), l.e_spec);
+    const float r = s.roughness;
+    l.a2 = r * r * r * r;
+    const V lw = v3(0.3f, 0.6f, 0.1f);
+    const float ws = dot(l.e_spec, lw);
+    const float wd = dot(vmul(l.diffuse, l.kd), lw);
+    l.p_spec = clampf(ws / fmaxf(ws + wd, 1e-6f), 0.05f, 1.0f);
+    if (wd <= 0.0f) l.p_spec = 1.0f;
+    const float e_coat = 0.0f, wt = 0.0f, under = 1.0f - e_coat;
+    const float total = (ws + wd) * under + e_coat + wt;
+    l.p_coat = e_coat / fmaxf(total, 1e-6f);
+    l.p_trans = wt / fmaxf(total, 1e-6f);
+    return l;
+}
+
+__device__ __forceinline__ float d_ggx(float a2, float noh) {
+    const float d = (noh * a2 - noh) * noh + 1.0f;
+    return a2 / (kPi * d * d);
+}
+__device__ __forceinline__ float vis_smith(float a2, float nov, float nol) {
+    return 0.5f / fmaxf(mixf(2.0f * nol * nov, nol + nov, sqrtf(a2)), 1e-7f);
+}
+__device__ __forceinline__ V f_schlick(V f0, V f90, float voh) {
+    const float c = 1.0f - voh, c2 = c * c, k = c2 * c2 * c;
+    return vadd(f0, vs(vsub(f90, f0), k));
+}
+__device__ V eval_bsdf_k(const Surface &s, const Lobes &lb, V v, V l, float k) {
+    const float nol = dot(s.n, l);
+    const float nov = fmaxf(dot(s.n, v), 1e-4f);
+    if (nol <= 0.0f || dot(s.ng, l) <= 0.0f) return v3(0.0f, 0.0f, 0.0f);
+    const V h = normalize(vadd(v, l));
+    const float noh = fmaxf(dot(s.n, h), 0.0f);
+    const float voh = fmaxf(dot(v, h), 0.0f);
+    const float dv = d_ggx(lb.a2, noh) * vis_smith(lb.a2, nov, nol);
+    const V spec = vs(vmul(vs(f_schlick(lb.f0, lb.f90, voh), dv), lb.w), k);
+    const V diff = vmul(vs(lb.diffuse, kInvPi), lb.kd);
+    return vs(vadd(diff, spec), nol);
+}
+__device__ __forceinline__ V eval_bsdf(const Surface &s, const Lobes &lb, V v, V l) { return eval_bsdf_k(s, lb, v, l, 1.0f); }
+__device__ __forceinline__ float g1(float a2, float nov) { return 2.0f * nov / (nov + sqrtf(a2 + (1.0f - a2) * nov * nov)); }
+__device__ float pdf_bsdf(const Surface &s, const Lobes &lb, V v, V l) {
+    const float nol = dot(s.n, l);
+    const float pg = 0.0f;
+    if (nol <= 0.0f) return pg;
+    const float nov = fmaxf(dot(s.n, v), 1e-4f);
+    const V h = normalize(vadd(v, l));
+    const float noh = fmaxf(dot(s.n, h), 0.0f);
+    const float voh = fmaxf(dot(v, h), 1e-6f);
+    const float dv = g1(lb.a2, nov) * voh * d_ggx(lb.a2, noh) / nov;
+    const float ps = dv / (4.0f * voh);
+    const float pd = nol / kPi;
+    const float pc = 0.0f;
+    return pg + lb.p_coat * pc + (1.0f - lb.p_coat - lb.p_trans) * (lb.p_spec * ps + (1.0f - lb.p_spec) * pd);
+}
+__device__ __forceinline__ float power_heuristic(float a, float b) {
+    const float a2 = a * a;
+    return a2 / fmaxf(a2 + b * b, 1e-20f);
+}
+__device__ __forceinline__ float light_mis(float pdf_l, const Surface &s, const Lobes &lb, V v, V l) {
+    return power_heuristic(pdf_l, pdf_bsdf(s, lb, v, l));
+}
+__device__ __forceinline__ M3 basis(V n) {
+    const float sgn = n.z >= 0.0f ? 1.0f : -1.0f;
+    const float a = -1.0f / (sgn + n.z);
+    const float b = n.x * n.y * a;
+    M3 m;
+    m.t = v3(1.0f + sgn * n.x * n.x * a, sgn * b, -sgn * n.x);
+    m.b = v3(b, sgn + n.y * n.y * a, -n.y);
+    m.n = n;
+    return m;
+}
+__device__ V sample_vndf(V v, const M3 &tbn, float alpha, float u1, float u2) {
+    const V vl = mtmul(tbn, v);
+    const V vh = normalize(v3(alpha * vl.x, alpha * vl.y, vl.z));
+    const float lensq = vh.x * vh.x + vh.y * vh.y;
+    const V t1 = lensq > 0.0f ? vds(v3(-vh.y, vh.x, 0.0f), sqrtf(lensq)) : v3(1.0f, 0.0f, 0.0f);
+    const V t2 = cross(vh, t1);
+    const float r = sqrtf(u1);
+    const float phi = kTwoPi * u2;
+    float sp, cp;
+    sincos_c(phi, sp, cp);
+    const float p1 = r * cp;
+    float p2 = r * sp;
+    const float s = 0.5f * (1.0f + vh.z);
+    p2 = (1.0f - s) * sqrtf(1.0f - p1 * p1) + s * p2;
+    const V nh = vadd(vadd(vs(t1, p1), vs(t2, p2)), vs(vh, sqrtf(fmaxf(0.0f, 1.0f - p1 * p1 - p2 * p2))));
+    const V hl = normalize(v3(alpha * nh.x, alpha * nh.y, fmaxf(0.0f, nh.z)));
+    return mmul(tbn, hl);
+}
+__device__ __forceinline__ V reflect(V e1, V e2) { return vsub(e1, vs(e2, 2.0f * dot(e2, e1))); }
+
+__device__ V sample_bsdf(Ctx &c, const Surface &s, const Lobes &lb, V v) {
+    const M3 tbn = basis(s.n);
+    float r[4];
+    c.g.r4(c.g.dim(D_BSDF), r);
+    c.sampled_spec = r[1] < lb.p_spec;
+    if (c.sampled_spec) {
+        const V h = sample_vndf(v, tbn, s.roughness * s.roughness, r[2], r[3]);
+        return reflect(vneg(v), h);
+    }
+    const float rr = sqrtf(r[2]);
+    const float phi = kTwoPi * r[3];
+    float sp, cp;
+    sincos_c(phi, sp, cp);
+    return mmul(tbn, v3(rr * cp, rr * sp, sqrtf(fmaxf(0.0f, 1.0f - rr * rr))));
+}
+
+// ---- light --------------------------------------------------------------------------------------------------------
+__device__ __forceinline__ V offset_origin(V p, V n) {
+    const float m = fmaxf(1.0f, fmaxf(fabsf(p.x), fmaxf(fabsf(p.y), fabsf(p.z))));
+    return vadd(p, vs(n, 1e-4f * m));
+}
+__device__ __forceinline__ bool occluded(const Ctx &c, V o, V d, float tmax, uint32_t mode) {
+    return c.trace(o, d, tmax, mode).tri != kNone;
+}
+__device__ __forceinline__ V sky(const Ctx &c) { return v3(c.pf(SKY, 0), c.pf(SKY, 1), c.pf(SKY, 2)); }
+__device__ __forceinline__ float env_pdf() { return 1.0f / kFourPi; }
+__device__ __forceinline__ bool env_sampled(const Ctx &c) {
+    return c.pu(COUNTS2, 1) != 0u && (c.pf(ENV, 0) > 0.5f || luminance(sky(c)) > 0.0f);
+}
+__device__ V sample_env(Ctx &c, uint32_t rd) {
+    float u[4];
+    c.g.r4(rd, u);
+    const float z = 1.0f - 2.0f * u[0];
+    const float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+    const float phi = kTwoPi * u[1];
+    float sp, cp;
+    sincos_c(phi, sp, cp);
+    return v3(r * cp, z, r * sp);
+}
+__device__ V sky_mis(const Ctx &c, V d, float pdf) {
+    float wgt = 1.0f;
+    if (pdf > 0.0f && c.pu(COUNTS2, 1) != 0u) wgt = power_heuristic(pdf, env_pdf());
+    V col = vs(sky(c), wgt);
+    if (pdf > 0.0f) {
+        V sum = v3(0.0f, 0.0f, 0.0f);
+        for (uint32_t i = 0; i < c.pu(COUNTS2, 0); i++) {
+            const float4 *li = c.light(i);
+            const float4 dir = li[1];
+            if (dot(d, vneg(xyz(dir))) >= dir.w) {
+                const float omega = kTwoPi * (1.0f - dir.w);
+                sum = vadd(sum, vs(vds(xyz(li[2]), omega), power_heuristic(pdf, 1.0f / omega)));
+            }
+        }
+        col = vadd(col, sum);
+    }
+    return col;
+}
+
+__device__ __forceinline__ uint32_t emitter_count(const Ctx &c) { return c.pu(COUNTS, 1); }
+__device__ __forceinline__ float4 emitter_totals(const Ctx &c) {
+    return c.emitter_at(emitter_count(c) + kGuide / 4u + 1u);
+}
+__device__ __forceinline__ bool emitters_on(const Ctx &c) { return emitter_count(c) != 0u && emitter_totals(c).x > 0.0f; }
+__device__ __forceinline__ uint32_t emitter_guide(const Ctx &c, uint32_t k) {
+    const float4 g = c.emitter_at(emitter_count(c) + k / 4u);
+    const uint32_t j = k % 4u;
+    return __float_as_uint(j == 0 ? g.x : (j == 1 ? g.y : (j == 2 ? g.z : g.w)));
+}
+__device__ float4 emitter_pick_u(const Ctx &c, float u) {
+    const uint32_t k = min(static_cast<uint32_t>(u * static_cast<float>(kGuide)), kGuide - 1u);
+    uint32_t lo = emitter_guide(c, k), hi = emitter_guide(c, k + 1u);
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) / 2u;
+        if (c.emitter_at(mid).y < u)
+            lo = mid + 1u;
+        else
+            hi = mid;
+    }
+    return c.emitter_at(min(lo, emitter_count(c) - 1u));
+}
+
+struct Em {
+    V v0, e1, e2;
+    uint32_t tri, inst, code, part, flags;
+};
+__device__ __forceinline__ Em emitter_tri_at(const Ctx &c, uint32_t i) {
+    const float4 *r = c.a->emitter_tris + 4 * uint64_t(i);
+    const float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
+    return {xyz(r0), xyz(r1), xyz(r2), __float_as_uint(r0.w), __float_as_uint(r1.w), __float_as_uint(r2.w),
+            __float_as_uint(r3.x), __float_as_uint(r3.y)};
+}
+__device__ V emitter_radiance(const Ctx &c, const Em &em) {
+    V e1, e2;
+    TriAttr a = tri_attr(c, em.tri, em.inst, em.part, e1, e2);
+    if (em.inst != kNone) a.x.y = __uint_as_float(__float_as_uint(c.inst(em.inst)[6].y) + __float_as_uint(a.x.y));
+    const float4 *m = c.material(material_of(c, a, em.inst));
+    return vs(xyz(m[1]), c.obj_power(__float_as_uint(a.x.y)));
+}
+
+struct ES {
+    V le, l;
+    float dist, pdf_w, w;
+};
+
+__device__ ES sample_emitter_ris(Ctx &c, V x, V n, uint32_t m, const Surface &sf, const Lobes &lb, V v, uint32_t rd) {
+    ES r{v3(0.0f, 0.0f, 0.0f), v3(0.0f, 1.0f, 0.0f), 0.0f, 0.0f, 0.0f};
+    float wsum = 0.0f, pick_b1 = 0.0f, pick_b2 = 0.0f, pick_ph = 0.0f, pick_pdf = 0.0f;
+    uint32_t pick_i = 0;
+    const float4 tot = emitter_totals(c);
+    for (uint32_t k = 0; k < m; k++) {
+        const uint32_t dk = rng_sub(rd, k);
+        float r3[3];
+        c.g.r3(dk, r3);
+        float4 e = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        float pk_p = 0.0f, pk_w = 0.0f;
+        if (tot.x > 0.0f) {
+            e = emitter_pick_u(c, r3[0]);
+            pk_p = e.z / tot.x;
+            pk_w = e.z;
+        }
+        float bb[3];
+        c.g.r3(rng_sub(dk, 1u), bb);
+        float b1 = bb[0], b2 = bb[1];
+        if (pk_p <= 0.0f) continue;
+        if (b1 + b2 > 1.0f) b1 = 1.0f - b1, b2 = 1.0f - b2;
+        const uint32_t ei = __float_as_uint(e.x);
+        const Em em = emitter_tri_at(c, ei);
+        const V lp = vadd(vadd(em.v0, vs(em.e1, b1)), vs(em.e2, b2));
+        const V cr = cross(em.e1, em.e2);
+        const float a2 = len(cr);
+        if (!(a2 > 0.0f)) continue;
+        const V ln = vds(cr, a2);
+        const V dl = vsub(lp, x);
+        const float d2 = fmaxf(dot(dl, dl), 1e-8f);
+        const V l = vs(dl, 1.0f / sqrtf(d2));
+        float g = fabsf(dot(ln, l)) * (dot(n, n) == 0.0f ? 1.0f : fmaxf(dot(n, l), 0.0f)) / d2;
+        if (m > 1u) g = fabsf(dot(ln, l)) * luminance(eval_bsdf(sf, lb, v, l)) / d2;
+        const float ph = pk_w / (e.w * 1.0f) * g;
+        const float pdf_a = pk_p / (0.5f * a2);
+        const float w = ph / pdf_a;
+        if (!(w > 0.0f)) continue;
+        wsum += w;
+        if (bb[2] * wsum < w) pick_b1 = b1, pick_b2 = b2, pick_ph = ph, pick_i = ei, pick_pdf = pdf_a;
+    }
+    if (pick_ph <= 0.0f) return r;
+    const Em em = emitter_tri_at(c, pick_i);
+    const V lp = vadd(vadd(em.v0, vs(em.e1, pick_b1)), vs(em.e2, pick_b2));
+    const V ln = normalize(cross(em.e1, em.e2));
+    const V dl = vsub(lp, x);
+    const float d2 = dot(dl, dl);
+    const float dist = sqrtf(d2);
+    const V l = vds(dl, dist);
+    float cos_l = dot(ln, vneg(l));
+    if (!(em.flags & kEmitterCard)) cos_l = fabsf(cos_l);
+    if (cos_l <= 0.0f) return r;
+    r.le = emitter_radiance(c, em);
+    r.l = l;
+    r.dist = dist;
+    r.pdf_w = pick_pdf * d2 / cos_l;
+    r.w = (wsum / static_cast<float>(m)) / pick_ph * cos_l / d2;
+    return r;
+}
+
+__device__ ES sample_emitter(Ctx &c, V x, uint32_t rd) {
+    ES r{v3(0.0f, 0.0f, 0.0f), v3(0.0f, 1.0f, 0.0f), 0.0f, 0.0f, 0.0f};
+    float r3[3];
+    c.g.r3(rd, r3);
+    const float4 tot = emitter_totals(c);
+    if (!(tot.x > 0.0f)) return r;
+    const float4 e = emitter_pick_u(c, r3[0]);
+    const float pk_p = e.z / tot.x;
+    if (pk_p <= 0.0f) return r;
+    const Em em = emitter_tri_at(c, __float_as_uint(e.x));
+    float bb[2];
+    c.g.r2(rng_sub(rd, 1u), bb);
+    float b1 = bb[0], b2 = bb[1];
+    if (b1 + b2 > 1.0f) b1 = 1.0f - b1, b2 = 1.0f - b2;
+    const V lp = vadd(vadd(em.v0, vs(em.e1, b1)), vs(em.e2, b2));
+    const V cr = cross(em.e1, em.e2);
+    const float area = 0.5f * len(cr);
+    const V ln = vds(cr, 2.0f * area);
+    const V dl = vsub(lp, x);
+    const float d2 = dot(dl, dl);
+    const float dist = sqrtf(d2);
+    const V l = vds(dl, dist);
+    float cos_l = dot(ln, vneg(l));
+    if (!(em.flags & kEmitterCard)) cos_l = fabsf(cos_l);
+    if (cos_l <= 0.0f) return r;
+    r.le = emitter_radiance(c, em);
+    r.l = l;
+    r.dist = dist;
+    r.pdf_w = pk_p / area * d2 / cos_l;
+    r.w = 1.0f / r.pdf_w;
+    return r;
+}
+
+__device__ V direct_light(Ctx &c, const Surface &s, const Lobes &lb, V v) {
+    V sum = v3(0.0f, 0.0f, 0.0f);
+    const V o = offset_origin(s.p, s.ng);
+    for (uint32_t i = 0; i < c.pu(COUNTS2, 0); i++) {
+        const float4 *li = c.light(i);
+        const float4 dir = li[1];
+        const float cos_max = dir.w;
+        const V to_light = vneg(xyz(dir));
+        float u[2];
+        c.g.r2(rng_sub(c.g.dim(D_SUN), i), u);
+        const float cos_t = 1.0f - u[0] * (1.0f - cos_max);
+        const float sin_t = sqrtf(fmaxf(0.0f, 1.0f - cos_t * cos_t));
+        const float phi = kTwoPi * u[1];
+        float sp, cp;
+        sincos_c(phi, sp, cp);
+        const M3 tb = basis(to_light);
+        const V l = mmul(tb, v3(sin_t * cp, sin_t * sp, cos_t));
+        const V col = xyz(li[2]);
+        if (all_nonpos(col)) continue;
+        const V f = eval_bsdf_k(s, lb, v, l, li[4].y);
+        if (any_pos(f)) {
+            const bool vis = li[3].w >= 0.5f ? !occluded(c, o, l, kInf, kTraceShadow) : true;
+            if (vis) {
+                const float pdf_l = 1.0f / (kTwoPi * (1.0f - cos_max));
+                const float wgt = light_mis(pdf_l, s, lb, v, l);
+                sum = vadd(sum, vs(vmul(col, f), wgt));
+            }
+        }
+    }
+    const V ne = s.n;
+    if (emitters_on(c)) {
+        ES es;
+        const bool seen = c.g.depth == 0u || (c.g.depth == 1u && c.mirror_path);
+        if (seen)
+            es = sample_emitter_ris(c, o, ne, c.g.depth == 0u ? kEmitRisM : kEmitRisM / 2u, s, lb, v, c.g.dim(D_EMIT));
+        else if (c.pf(LIGHT_LOD, 3) > 0.0f)
+            es = sample_emitter_ris(c, o, ne, 2u, s, lb, v, c.g.dim(D_EMIT));
+        else
+            es = sample_emitter(c, o, c.g.dim(D_EMIT));
+        if (es.pdf_w > 0.0f) {
+            const V f = eval_bsdf(s, lb, v, es.l);
+            if (any_pos(f)) {
+                if (!occluded(c, o, es.l, es.dist * 0.999f, kTraceAny)) {
+                    const float wgt = light_mis(es.pdf_w, s, lb, v, es.l);
+                    sum = vadd(sum, vs(vs(vmul(es.le, f), wgt), es.w));
+                }
+            }
+        }
+    }
+    if (env_sampled(c)) {
+        const V l = sample_env(c, c.g.dim(D_ENV));
+        const V f = eval_bsdf(s, lb, v, l);
+        if (any_pos(f)) {
+            const float pdf_e = env_pdf();
+            if (pdf_e > 0.0f && !occluded(c, o, l, kInf, kTraceShadow))
+                sum = vadd(sum, vds(vs(vmul(sky(c), f), light_mis(pdf_e, s, lb, v, l)), pdf_e));
+        }
+    }
+    return sum;
+}
+
+__device__ float emit_mis(const Ctx &c, const HitRec &hit, V d, float pdf) {
+    if (!emitters_on(c)) return 1.0f;
+    const uint32_t rows = __float_as_uint(c.inst(hit.inst)[6].w);
+    const uint32_t first = c.row_word(rows, hit.part, 2u);
+    if (first == kNoEmitter) return 1.0f;
+    const uint32_t rank = __float_as_uint(c.a->geo.leaves[3 * uint64_t(hit.tri) + 1].w);
+    const uint32_t entry = first + rank;
+    const Em em = emitter_tri_at(c, entry);
+    const V cr = cross(em.e1, em.e2);
+    const float area = 0.5f * len(cr);
+    const float cos_l = fabsf(dot(normalize(cr), d));
+    const float4 tot = emitter_totals(c);
+    float sel = 0.0f;
+    if (tot.x > 0.0f) {
+        const uint32_t code = em.code & ~0x80000000u;
+        if ((code >> 26) == 0u) sel = c.emitter_at(code).z / tot.x;
+    }
+    if (!(sel > 0.0f)) return 1.0f;
+    return power_heuristic(pdf, sel / fmaxf(area, 1e-12f) * hit.t * hit.t / fmaxf(cos_l, 1e-6f));
+}
+
+__device__ __forceinline__ V clamp_lum(V col, float clamp_v) {
+    const float lum = luminance(col);
+    if (lum > clamp_v) col = vs(col, clamp_v / lum);
+    return col;
+}
+
+// One path of sample `index` at pixel (px, py): its radiance, the camera hit's albedo and normal.
+__device__ void path(Ctx &c, uint32_t px, uint32_t py, uint32_t index, V &radiance, V &albedo, V &normal) {
+    const uint32_t w = c.pu(SIZE, 0), h = c.pu(SIZE, 1);
+    c.g.seed = hash_hp(px ^ hash_hp(py ^ hash_hp(c.pu(SIZE, 3))));
+    c.g.index = index;
+    c.g.depth = 0;
+    float cam[4];
+    c.g.r4(c.g.dim(D_CAMERA), cam);
+    const float ndx = (static_cast<float>(px) + cam[0]) / static_cast<float>(w) * 2.0f - 1.0f;
+    const float ndy = 1.0f - (static_cast<float>(py) + cam[1]) / static_cast<float>(h) * 2.0f;
+    const V fwd = v3(c.pf(CAM_FWD, 0), c.pf(CAM_FWD, 1), c.pf(CAM_FWD, 2));
+    const V right = v3(c.pf(CAM_RIGHT, 0), c.pf(CAM_RIGHT, 1), c.pf(CAM_RIGHT, 2));
+    const V up = v3(c.pf(CAM_UP, 0), c.pf(CAM_UP, 1), c.pf(CAM_UP, 2));
+    const float tanh = c.pf(CAM_POS, 3), aspect = c.pf(CAM_FWD, 3);
+    V d = normalize(vadd(vadd(fwd, vs(right, ndx * tanh * aspect)), vs(up, ndy * tanh)));
+    V o = v3(c.pf(CAM_POS, 0), c.pf(CAM_POS, 1), c.pf(CAM_POS, 2));
+
+    radiance = v3(0.0f, 0.0f, 0.0f);
+    albedo = v3(0.0f, 0.0f, 0.0f);
+    normal = v3(0.0f, 0.0f, 0.0f);
+    V thr = v3(1.0f, 1.0f, 1.0f);
+    bool specular_bounce = true, straight = true;
+    float last_pdf = 0.0f;
+    const float clamp_v = c.pf(MISC, 0);
+    c.mirror_path = false;
+    const uint32_t last = c.pu(COUNTS, 2);
+    for (uint32_t depth = 0; depth <= last; depth++) {
+        c.g.depth = depth;
+        const HitRec hit = c.trace(o, d, kInf, kTraceClosest);
+        if (hit.tri == kNone) {
+            V col = vmul(thr, sky_mis(c, d, depth > 0u ? last_pdf : 0.0f));
+            if (depth > 1u && !straight) col = clamp_lum(col, clamp_v);
+            radiance = vadd(radiance, col);
+            break;
+        }
+        const Surface s = surface(c, o, d, hit);
+        if (depth == 0u) {
+            if (s.card) {
+                const float mx = fmaxf(fmaxf(s.emissive.x, fmaxf(s.emissive.y, s.emissive.z)), 1e-6f);
+                albedo = vds(s.emissive, mx);
+            } else {
+                albedo = s.base;
+            }
+            normal = s.n;
+        }
+        if (any_pos(s.emissive) && (s.front || !s.card)) {
+            float wgt = 1.0f;
+            if (depth > 0u && !specular_bounce) wgt = emit_mis(c, hit, d, last_pdf);
+            V col = vs(vmul(thr, s.emissive), wgt);
+            if (depth > 1u && !straight) col = clamp_lum(col, clamp_v);
+            radiance = vadd(radiance, col);
+        }
+        if (s.card) break;
+        if (depth == last) break;
+        const V v = vneg(d);
+        const float nov = fmaxf(dot(s.n, v), 1e-4f);
+        const Lobes lb = lobes(c, s, nov);
+        V dl = vmul(thr, direct_light(c, s, lb, v));
+        if (depth > 0u && !straight) dl = clamp_lum(dl, clamp_v);
+        radiance = vadd(radiance, dl);
+        const V l = sample_bsdf(c, s, lb, v);
+        const float pdf = pdf_bsdf(s, lb, v, l);
+        const V wgt = vds(eval_bsdf(s, lb, v, l), fmaxf(pdf, 1e-20f));
+        if (pdf <= 0.0f || all_nonpos(wgt)) break;
+        thr = vmul(thr, wgt);
+        last_pdf = pdf;
+        specular_bounce = false;
+        straight = false;
+        c.mirror_path = (depth == 0u || c.mirror_path) && s.roughness < kMirrorRoughness && c.sampled_spec;
+        if (depth >= c.pu(COUNTS, 3)) {
+            const float q = fminf(fmaxf(thr.x, fmaxf(thr.y, thr.z)), 0.95f);
+            if (c.g.r1(c.g.dim(D_RR)) > q) break;
+            thr = vds(thr, q);
+        }
+        o = offset_origin(s.p, dot(l, s.ng) < 0.0f ? vneg(s.ng) : s.ng);
+        d = l;
+    }
+    if (non_finite(radiance)) radiance = v3(0.0f, 0.0f, 0.0f);
+    if (non_finite(albedo)) albedo = v3(0.0f, 0.0f, 0.0f);
+    if (non_finite(normal)) normal = v3(0.0f, 1.0f, 0.0f);
+}
+
+__global__ __launch_bounds__(kRenderBlock) void k_render(RenderArgs a) {
+    __shared__ uint32_t lds[kLdsStack * kRenderBlock];
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    Stack st{lds + threadIdx.x, a.geo.spill + gtid, a.geo.spill_stride, 0};
+    bool tripped = false;
+    Ctx c{&a, &st, &tripped, Rng{0, 0, 0}, false, false};
+    const uint32_t w = a.params[16], h = a.params[17];
+    const uint32_t tiles_x = (w + 7) / 8, tiles = tiles_x * ((h + 7) / 8);
+    for (;;) {
+        uint32_t tile = 0;
+        if (lane == 0) tile = atomicAdd(a.geo.counter, 1u);
+        tile = __shfl(tile, 0, 64);
+        if (tile >= tiles) break;
+        const uint32_t px = (tile % tiles_x) * 8 + (lane & 7u), py = (tile / tiles_x) * 8 + (lane >> 3);
+        if (px < w && py < h) {
+            float4 *pix = a.accum + 5 * (uint64_t(py) * w + px);
+            float4 col = pix[0], alb = pix[1], nrm = pix[2];
+            for (uint32_t s = 0; s < a.spp; s++) {
+                V rad, al, nm;
+                path(c, px, py, a.first_sample + s, rad, al, nm);
+                const float lum = luminance(rad);
+                col.x += rad.x, col.y += rad.y, col.z += rad.z, col.w += 1.0f;
+                alb.x += al.x, alb.y += al.y, alb.z += al.z, alb.w += lum;
+                nrm.x += nm.x, nrm.y += nm.y, nrm.z += nm.z, nrm.w += lum * lum;
+            }
+            pix[0] = col;
+            pix[1] = alb;
+            pix[2] = nrm;
+        }
+    }
+    if (tripped) atomicOr(a.geo.guard_flag, 1u);
+}
+
+__global__ void k_reset(uint32_t *c) { *c = 0; }
+
+}  // namespace
+
+int render_occupancy(int *blocks_per_cu) {
+    int b = 0;
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_render, kRenderBlock, 0);
+    *blocks_per_cu = b;
+    return e == hipSuccess ? 0 : -1;
+}
+
+int launch_render(const RenderArgs &a, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, s, a.geo.counter);
+    hipLaunchKernelGGL(k_render, dim3(grid), dim3(kRenderBlock), 0, s, a);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+}  // namespace trt

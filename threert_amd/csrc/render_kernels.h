@@ -1,0 +1,36 @@
+// Launch interface of the path tracer (render.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "trace_kernels.h"
+
+namespace trt {
+
+constexpr int kRenderBlock = 256;
+constexpr int kParamWords = 156;  // Params: 39 vec4 (gpu.rs:129-185)
+
+struct RenderArgs {
+    TraceArgs geo;              // the wide walk's buffers (rays / hits unused)
+    const float4 *instances;    // 160-byte records, 10 x float4
+    const float4 *vertices;     // indexed.rs Vertex, 3 x float4
+    const uint32_t *indices;    // 3 a triangle, the leaves' order
+    const uint32_t *rows;       // scene.rs Row, 4 words
+    const float4 *materials;    // scene.rs GpuMaterial, 18 x float4
+    const float4 *lights;       // scene.rs GpuLight, 5 x float4
+    const float4 *emitters;     // lib.rs emitter_cdf: entries, guide, header
+    const float4 *emitter_tris; // lib.rs GpuEmitterTri, 4 x float4
+    const float4 *objects;      // 3 x float4 an object
+    const float4 *energy;       // lib.rs energy_lut, 512 x float4
+    float4 *accum;              // records.wgsl Pixel, 5 x float4 a pixel
+    uint32_t first_sample;
+    uint32_t spp;
+    uint32_t params[kParamWords];
+};
+
+int render_occupancy(int *blocks_per_cu);
+int launch_render(const RenderArgs &a, int grid, hipStream_t s);
+
+}  // namespace trt

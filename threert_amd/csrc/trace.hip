@@ -1,333 +1,14 @@
-// Ray-batch traversal kernels for gfx950 (wave64).
-//
-// Two walks over the same scene, one answer: the lowest (t, instance, triangle) among the triangles the ray meets
-// in (0, tmax), with every triangle tested by the reference's Moller-Trumbore (pathtrace.wgsl:742-760) in the
-// reference's operation order (this file is built with -ffp-contract=off), so t, u and v are the reference's bits.
-//   k_trace_binary  the reference's own 32-byte binary node buffer, trace_units.wgsl:44-135 statement for statement
-//                   (64-entry stack, near child first), boxes culled at hit.t widened by kCullSlack.
-//   k_trace_wide    the library's 64-byte 4-wide quantised nodes (wide.cpp): each node's four child boxes are tested
-//                   as fma(q, step / d, (origin - o) / d) per plane, children visited nearest first.
-// Both are persistent: a wave takes 64 rays at a time from a counter until the batch is done. The stack's first
-// kLdsStack entries live in LDS (stride = block size, so lanes never share a bank); deeper entries spill to a
-// per-thread region of device memory sized from the scene's deepest possible walk.
+// Ray-batch traversal kernels for gfx950 (wave64): the walks of walk.h over a batch of rays in HBM.
+// Persistent: a wave takes 64 rays at a time from a counter until the batch is done.
 #include <hip/hip_runtime.h>
 
 #include "trace_kernels.h"
+#include "walk.h"
 
 namespace trt {
 
 namespace {
-
-constexpr float kCullSlack = 1.0f / 16384.0f;
-constexpr uint32_t kTraceClosest = 0, kTraceShadow = 1;
-
-struct F3 {
-    float x, y, z;
-};
-
-__device__ __forceinline__ F3 sub3(F3 a, F3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
-__device__ __forceinline__ float dot3(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ F3 cross3(F3 a, F3 b) {
-    return {a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y};
-}
-__device__ __forceinline__ F3 inv3(F3 d) {
-    return {1.0f / (fabsf(d.x) < 1e-20f ? 1e-20f : d.x), 1.0f / (fabsf(d.y) < 1e-20f ? 1e-20f : d.y),
-            1.0f / (fabsf(d.z) < 1e-20f ? 1e-20f : d.z)};
-}
-__device__ __forceinline__ float cull_t(float t) { return t + t * kCullSlack; }
-
-// pathtrace.wgsl:732-740 ray_aabb.
-__device__ __forceinline__ float ray_aabb(F3 o, F3 inv, F3 bmin, F3 bmax, float tmax) {
-    const float t0x = (bmin.x - o.x) * inv.x, t0y = (bmin.y - o.y) * inv.y, t0z = (bmin.z - o.z) * inv.z;
-    const float t1x = (bmax.x - o.x) * inv.x, t1y = (bmax.y - o.y) * inv.y, t1z = (bmax.z - o.z) * inv.z;
-    const float lx = t0x < t1x ? t0x : t1x, ly = t0y < t1y ? t0y : t1y, lz = t0z < t1z ? t0z : t1z;
-    const float hx = t0x > t1x ? t0x : t1x, hy = t0y > t1y ? t0y : t1y, hz = t0z > t1z ? t0z : t1z;
-    const float tn = fmaxf(fmaxf(lx, ly), fmaxf(lz, 0.0f));
-    const float tf = fminf(fminf(hx, hy), fminf(hz, tmax));
-    return tn <= tf ? tn : kInf;
-}
-
-// pathtrace.wgsl:742-760 intersect_tri; t = kInf for a miss.
-__device__ __forceinline__ void intersect_tri(F3 o, F3 d, float4 l0, float4 l1, float4 l2, float tmax, float &t,
-                                              float &u, float &v) {
-    t = kInf;
-    const F3 v0{l0.x, l0.y, l0.z}, e1{l1.x, l1.y, l1.z}, e2{l2.x, l2.y, l2.z};
-    const F3 pv = cross3(d, e2);
-    const float det = dot3(e1, pv);
-    if (fabsf(det) < 1e-12f) return;
-    const float inv = 1.0f / det;
-    const F3 tv = sub3(o, v0);
-    const float uu = dot3(tv, pv) * inv;
-    if (!(uu >= 0.0f && uu <= 1.0f)) return;
-    const F3 qv = cross3(tv, e1);
-    const float vv = dot3(d, qv) * inv;
-    if (vv < 0.0f || uu + vv > 1.0f) return;
-    const float tt = dot3(e2, qv) * inv;
-    if (tt <= 0.0f || tt >= tmax) return;
-    t = tt, u = uu, v = vv;
-}
-
-struct HitRec {
-    float t;
-    uint32_t tri;
-    float u, v;
-    uint32_t inst, part;
-};
-
-// The tie rule: a candidate at t in (0, tmax) replaces the hit when nearer, or as near with a lower
-// (instance, triangle).
-__device__ __forceinline__ bool accept(const HitRec &h, float x, uint32_t inst, uint32_t tri) {
-    if (x < h.t) return true;
-    return x == h.t && h.tri != kNone && (inst < h.inst || (inst == h.inst && tri < h.tri));
-}
-
-// pathtrace.wgsl proc_dir: n0 * d.x + n1 * d.y + n2 * d.z.
-__device__ __forceinline__ F3 proc_dir(F3 n0, F3 n1, F3 n2, F3 d) {
-    return {n0.x * d.x + n1.x * d.y + n2.x * d.z, n0.y * d.x + n1.y * d.y + n2.y * d.z,
-            n0.z * d.x + n1.z * d.y + n2.z * d.z};
-}
-
-struct Stack {
-    uint32_t *lds;    // this thread's entry 0; entry k at lds[k * kTraceBlock]
-    uint32_t *spill;  // this thread's spill entry 0; entry k at spill[k * stride]
-    uint32_t stride;
-    uint32_t sp;
-    __device__ __forceinline__ void push(uint32_t v) {
-        if (sp < kLdsStack)
-            lds[sp * kTraceBlock] = v;
-        else
-            spill[(sp - kLdsStack) * stride] = v;
-        sp++;
-    }
-    __device__ __forceinline__ uint32_t pop() {
-        sp--;
-        return sp < kLdsStack ? lds[sp * kTraceBlock] : spill[(sp - kLdsStack) * stride];
-    }
-};
-
-__device__ __forceinline__ void store_hit(uint32_t *hits, uint64_t i, const HitRec &h) {
-    uint32_t *p = hits + 6 * i;
-    p[0] = __float_as_uint(h.t);
-    p[1] = h.tri;
-    p[2] = __float_as_uint(h.u);
-    p[3] = __float_as_uint(h.v);
-    p[4] = h.inst;
-    p[5] = h.part;
-}
-
-// What a walk read (units.rs:922-927 Reads): nodes fetched, triangles tested, unit records read.
-struct Reads {
-    uint32_t nodes = 0, tris = 0, units = 0;
-};
-
-// ---- the reference's binary walk ------------------------------------------------------------------------------
-template <bool kCount>
-__device__ HitRec trace_binary(const TraceArgs &a, F3 o, F3 d, float tmax_in, uint32_t mode, Stack &st,
-                               bool &tripped, Reads &cnt) {
-    const float tmax = tmax_in < kInf ? tmax_in : kInf;
-    HitRec h{tmax, kNone, 0.0f, 0.0f, kNone, 0};
-    if (a.units == 0) return h;
-    const F3 inv = inv3(d);
-    const uint32_t top = a.top_first;
-    {
-        if (kCount) cnt.nodes++;
-        const float4 w0 = a.bnodes[2 * top], w1 = a.bnodes[2 * top + 1];
-        if (ray_aabb(o, inv, {w0.x, w0.y, w0.z}, {w1.x, w1.y, w1.z}, cull_t(h.t)) == kInf) return h;
-    }
-    st.sp = 0;
-    uint32_t node = top, rec = kNone, off = top, first = 0, base = 0;
-    F3 ro = o, rd = d, rinv = inv;
-    uint32_t guard = 0;
-    for (;;) {
-        if (++guard > kGuardSteps) {
-            tripped = true;
-            return h;
-        }
-        const float4 n0 = a.bnodes[2 * node], n1 = a.bnodes[2 * node + 1];
-        const uint32_t left_or_first = __float_as_uint(n0.w), count = __float_as_uint(n1.w);
-        if (kCount) cnt.nodes++;
-        if (count == 0) {
-            if (kCount) cnt.nodes += 2;
-            const uint32_t l = off + left_or_first, r = l + 1;
-            const float4 l0 = a.bnodes[2 * l], l1 = a.bnodes[2 * l + 1];
-            const float4 r0 = a.bnodes[2 * r], r1 = a.bnodes[2 * r + 1];
-            float tl = ray_aabb(ro, rinv, {l0.x, l0.y, l0.z}, {l1.x, l1.y, l1.z}, cull_t(h.t));
-            float tr = ray_aabb(ro, rinv, {r0.x, r0.y, r0.z}, {r1.x, r1.y, r1.z}, cull_t(h.t));
-            if (tl != kInf || tr != kInf) {
-                uint32_t near = l, far = r;
-                if (tr < tl) {
-                    near = r, far = l;
-                    const float tmp = tl;
-                    tl = tr, tr = tmp;
-                }
-                node = near;
-                if (tr != kInf && st.sp < kRefStack) st.push(far);
-                continue;
-            }
-        } else if (rec == kNone) {
-            if (kCount) cnt.units++;
-            const float4 *u = a.instances + 10 * left_or_first;
-            const float4 m0 = u[0], m1 = u[1], m2 = u[2], q0 = u[3], q1 = u[4], q2 = u[5];
-            const uint32_t flags = __float_as_uint(q1.w);
-            if (mode != kTraceShadow || (flags & kUnitShadow) != 0) {
-                rec = left_or_first;
-                ro = {o.x - m0.w, o.y - m1.w, o.z - m2.w};
-                if ((flags & kUnitShifted) == 0) {
-                    const F3 c0{q0.x, q0.y, q0.z}, c1{q1.x, q1.y, q1.z}, c2{q2.x, q2.y, q2.z};
-                    ro = proc_dir(c0, c1, c2, ro);
-                    rd = proc_dir(c0, c1, c2, d);
-                    rinv = inv3(rd);
-                }
-                off = __float_as_uint(q0.w);
-                first = __float_as_uint(u[6].x);
-                base = st.sp;
-                node = off;
-                continue;
-            }
-        } else {
-            for (uint32_t k = 0; k < count; k++) {
-                const uint32_t ti = first + left_or_first + k;
-                if (kCount) cnt.tris++;
-                const float4 *l = a.leaves + 3 * uint64_t(ti);
-                const float4 lv0 = l[0], lv1 = l[1], lv2 = l[2];
-                float x, uu, vv;
-                intersect_tri(ro, rd, lv0, lv1, lv2, tmax, x, uu, vv);
-                if (x < tmax && accept(h, x, rec, ti)) {
-                    h = {x, ti, uu, vv, rec, __float_as_uint(lv0.w)};
-                    if (mode != kTraceClosest) return h;
-                }
-            }
-        }
-        if (rec != kNone && st.sp == base) {
-            rec = kNone;
-            ro = o, rd = d, rinv = inv;
-            off = top;
-        }
-        if (st.sp == 0) break;
-        node = st.pop();
-    }
-    return h;
-}
-
-// ---- the 4-wide quantised walk ----------------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t pick(uint32_t c, uint32_t a0, uint32_t a1, uint32_t a2, uint32_t a3) {
-    return c == 0 ? a0 : (c == 1 ? a1 : (c == 2 ? a2 : a3));
-}
-__device__ __forceinline__ float qbyte(uint32_t w, int c) { return float((w >> (8 * c)) & 0xffu); }
-
-template <bool kCount>
-__device__ HitRec trace_wide(const TraceArgs &a, F3 o, F3 d, float tmax_in, uint32_t mode, Stack &st,
-                             bool &tripped, Reads &cnt) {
-    const float tmax = tmax_in < kInf ? tmax_in : kInf;
-    HitRec h{tmax, kNone, 0.0f, 0.0f, kNone, 0};
-    if (a.units == 0) return h;
-    const F3 inv = inv3(d);
-    if (ray_aabb(o, inv, {a.top_box[0], a.top_box[1], a.top_box[2]}, {a.top_box[3], a.top_box[4], a.top_box[5]},
-                 cull_t(h.t)) == kInf)
-        return h;
-    st.sp = 0;
-    uint32_t node = a.top_ref, rec = kNone, first = 0, base = 0;
-    F3 ro = o, rd = d, rinv = inv;
-    uint32_t guard = 0;
-    for (;;) {
-        if (++guard > kGuardSteps) {
-            tripped = true;
-            return h;
-        }
-        const uint32_t kind = node >> kRefKindShift;
-        if (kind == kRefNode) {
-            if (kCount) cnt.nodes++;
-            const uint4 *p = a.wnodes + 4 * uint64_t(node);
-            const uint4 w0 = p[0], w1 = p[1], w2 = p[2], w3 = p[3];
-            const float sx = __uint_as_float((w0.w & 0xffu) << 23), sy = __uint_as_float(((w0.w >> 8) & 0xffu) << 23),
-                        sz = __uint_as_float(((w0.w >> 16) & 0xffu) << 23);
-            const float bx = (__uint_as_float(w0.x) - ro.x) * rinv.x, by = (__uint_as_float(w0.y) - ro.y) * rinv.y,
-                        bz = (__uint_as_float(w0.z) - ro.z) * rinv.z;
-            const float ix = sx * rinv.x, iy = sy * rinv.y, iz = sz * rinv.z;
-            const uint32_t nx = rinv.x >= 0.0f ? w1.x : w1.w, fx = rinv.x >= 0.0f ? w1.w : w1.x;
-            const uint32_t ny = rinv.y >= 0.0f ? w1.y : w2.x, fy = rinv.y >= 0.0f ? w2.x : w1.y;
-            const uint32_t nz = rinv.z >= 0.0f ? w1.z : w2.y, fz = rinv.z >= 0.0f ? w2.y : w1.z;
-            const float cull = cull_t(h.t);
-            uint32_t key[4];
-#pragma unroll
-            for (int c = 0; c < 4; c++) {
-                const float tnx = fmaf(qbyte(nx, c), ix, bx), tny = fmaf(qbyte(ny, c), iy, by),
-                            tnz = fmaf(qbyte(nz, c), iz, bz);
-                const float tfx = fmaf(qbyte(fx, c), ix, bx), tfy = fmaf(qbyte(fy, c), iy, by),
-                            tfz = fmaf(qbyte(fz, c), iz, bz);
-                const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, 0.0f));
-                const float tf = fminf(fminf(tfx, tfy), fminf(tfz, cull));
-                key[c] = tn <= tf ? ((__float_as_uint(tn) & ~3u) | uint32_t(c)) : kNone;
-            }
-            if (w2.z == kNone) key[0] = kNone;
-            if (w2.w == kNone) key[1] = kNone;
-            if (w3.x == kNone) key[2] = kNone;
-            if (w3.y == kNone) key[3] = kNone;
-#define TRT_CSWAP(i, j)                          \
-    {                                            \
-        const uint32_t lo_ = min(key[i], key[j]); \
-        key[j] = max(key[i], key[j]);            \
-        key[i] = lo_;                            \
-    }
-            TRT_CSWAP(0, 1);
-            TRT_CSWAP(2, 3);
-            TRT_CSWAP(0, 2);
-            TRT_CSWAP(1, 3);
-            TRT_CSWAP(1, 2);
-#undef TRT_CSWAP
-            if (key[0] != kNone) {
-                if (key[3] != kNone) st.push(pick(key[3] & 3u, w2.z, w2.w, w3.x, w3.y));
-                if (key[2] != kNone) st.push(pick(key[2] & 3u, w2.z, w2.w, w3.x, w3.y));
-                if (key[1] != kNone) st.push(pick(key[1] & 3u, w2.z, w2.w, w3.x, w3.y));
-                node = pick(key[0] & 3u, w2.z, w2.w, w3.x, w3.y);
-                continue;
-            }
-        } else if (kind == kRefUnit) {
-            const uint32_t r = node & 0x3fffffffu;
-            if (kCount) cnt.units++;
-            const float4 *u = a.wunits + 4 * uint64_t(r);
-            const float4 u0 = u[0];
-            const uint32_t flags = __float_as_uint(u0.w);
-            if (mode != kTraceShadow || (flags & kUnitShadow) != 0) {
-                const float4 u1 = u[1], u2 = u[2], u3 = u[3];
-                rec = r;
-                ro = {o.x - u0.x, o.y - u0.y, o.z - u0.z};
-                if ((flags & kUnitShifted) == 0) {
-                    const F3 c0{u1.x, u1.y, u1.z}, c1{u2.x, u2.y, u2.z}, c2{u3.x, u3.y, u3.z};
-                    ro = proc_dir(c0, c1, c2, ro);
-                    rd = proc_dir(c0, c1, c2, d);
-                    rinv = inv3(rd);
-                }
-                first = __float_as_uint(u2.w);
-                base = st.sp;
-                node = __float_as_uint(u1.w);
-                continue;
-            }
-        } else {
-            const uint32_t count = ((node >> kRefTriFirstBits) & 15u) + 1u;
-            const uint32_t f = first + (node & kRefTriFirstMask);
-            for (uint32_t k = 0; k < count; k++) {
-                const uint32_t ti = f + k;
-                if (kCount) cnt.tris++;
-                const float4 *l = a.leaves + 3 * uint64_t(ti);
-                const float4 lv0 = l[0], lv1 = l[1], lv2 = l[2];
-                float x, uu, vv;
-                intersect_tri(ro, rd, lv0, lv1, lv2, tmax, x, uu, vv);
-                if (x < tmax && accept(h, x, rec, ti)) {
-                    h = {x, ti, uu, vv, rec, __float_as_uint(lv0.w)};
-                    if (mode != kTraceClosest) return h;
-                }
-            }
-        }
-        if (rec != kNone && st.sp == base) {
-            rec = kNone;
-            ro = o, rd = d, rinv = inv;
-        }
-        if (st.sp == 0) break;
-        node = st.pop();
-    }
-    return h;
-}
+using namespace walk;
 
 template <bool kWide, bool kCount>
 __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {

@@ -1,0 +1,505 @@
+"""Render scenes: the shading tables the offline pass reads, the Params block, and the seeded workloads.
+
+The tables follow the reference's layouts (the prepare step of lib.rs:530-785 for the features used here):
+  vertices   indexed.rs:10-13, 48 bytes: (position, u), (normal, v), (tangent, hand); a triangle's corners here
+             are its own (not shared between triangles: the kernel reads them through the indices either way)
+  indices    three a triangle, in the leaves' (tree) order, counted from the mesh's first vertex (record info.z)
+  rows       scene.rs:99-135 Row: (material, section's first triangle, its first emitter or NO_EMITTER, 0), one a
+             section of the instance's mesh, from record info.w
+  materials  scene.rs:480-491 GpuMaterial, 288 bytes
+  lights     scene.rs:495-501 GpuLight, 80 bytes (directional: type 0, emission direction, cos of the half angle)
+  emitters   lib.rs:451-488 emitter_cdf: (emitter bits, cumulative weight, weight, area) a triangle, the 65,537-entry
+             guide table four to a vec4, then the header (total weight, 0, 0, 0)
+  emitter triangles  lib.rs:143-157 GpuEmitterTri, world space, 64 bytes
+  energy     lib.rs:1026-1104 specular_albedo / energy_lut, 512 vec4
+  params     gpu.rs:129-185 Params, 39 vec4
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import (INSTANCE, Device, Scene, ThreeRTError, TRT_E_INVALID, _check, _ptr, lib)
+from . import scene as S
+
+NO_EMITTER = 0xFFFFFFFF
+EMITTER_GUIDE = 65536
+OBJECT_STRIDE = 3
+
+
+def material(base=(0.8, 0.8, 0.8), emissive=(0.0, 0.0, 0.0), metallic=0.0, roughness=1.0, specular=0.5) -> np.ndarray:
+    """scene.rs:480-491 GpuMaterial (72 words): base colour, emission, (metallic, roughness, specular, normal scale),
+    no textures, no layers."""
+    m = np.zeros(72, np.float32)
+    m[0:3] = base
+    m[3] = 1.0
+    m[4:7] = emissive
+    m[8:12] = (metallic, roughness, specular, 1.0)
+    m[12:20] = np.array([-1, -1, -1, -1, -1, -1, 0, 0], np.int32).view(np.float32)
+    m[20:24] = (1.0, 1.0, 0.0, 0.0)
+    m[28:32] = (0.0, 1.5, 0.0, 0.0)
+    m[64:72] = np.full(8, -1, np.int32).view(np.float32)
+    return m
+
+
+def directional_light(direction, lux, diameter_deg=0.5357, cast_shadow=True) -> np.ndarray:
+    """scene.rs:495-501 GpuLight for a directional light: `direction` is where the light travels."""
+    d = np.asarray(direction, np.float64)
+    d = d / np.linalg.norm(d)
+    li = np.zeros(20, np.float32)
+    li[4:7] = d
+    li[7] = math.cos(math.radians(diameter_deg) * 0.5)
+    li[8:11] = lux
+    li[15] = 1.0 if cast_shadow else 0.0
+    li[17] = 1.0
+    return li
+
+
+# ---- lib.rs:1026-1104 specular_albedo / energy_lut, in f64 ------------------------------------------------------
+def specular_albedo(mu: float, alpha: float, nr: int = 48, nphi: int = 8) -> tuple[float, float]:
+    mu = min(max(mu, 1e-4), 1.0)
+    if alpha < 1e-6:
+        fc = (1.0 - mu) ** 5
+        return 1.0 - fc, fc
+    vx, vz = math.sqrt(max(1.0 - mu * mu, 0.0)), mu
+    k = math.sqrt(alpha * alpha * vx * vx + vz * vz)
+    hx, hz = alpha * vx / k, vz / k
+    s = 0.5 * (1.0 + hz)
+    inv_g1 = 0.5 * (1.0 + math.sqrt(1.0 + alpha * alpha * (1.0 - mu * mu) / (mu * mu)))
+    i = np.arange(nr, dtype=np.float64)
+    t = (i + 0.5) / nr
+    r = np.sqrt(1.0 - (1.0 - t) * (1.0 - t))
+    w = 2.0 * (1.0 - t)
+    j = np.arange(nphi, dtype=np.float64)
+    phi = math.pi * ((j + 0.5) / nphi - 0.5)
+    r2, phi2 = np.meshgrid(r, phi, indexing="ij")
+    w2 = np.repeat(w[:, None], nphi, axis=1)
+    p1 = r2 * np.cos(phi2)
+    p2 = (1.0 - s) * np.sqrt(np.maximum(1.0 - p1 * p1, 0.0)) + s * r2 * np.sin(phi2)
+    pz = np.sqrt(np.maximum(1.0 - p1 * p1 - p2 * p2, 0.0))
+    m0 = alpha * (pz * hx - p2 * hz)
+    m1 = alpha * p1
+    m2 = np.maximum(p2 * hx + pz * hz, 0.0)
+    ln = np.sqrt(m0 * m0 + m1 * m1 + m2 * m2)
+    voh = (vx * m0 + vz * m2) / ln
+    nol = 2.0 * voh * m2 / ln - vz
+    ok = (nol > 0.0) & (voh > 0.0)
+    g2 = 2.0 * mu * nol / (nol * (mu * (1.0 - alpha) + alpha) + mu * (nol * (1.0 - alpha) + alpha))
+    fc = (1.0 - voh) ** 5
+    a = np.where(ok, w2 * (1.0 - fc) * g2, 0.0).sum()
+    b = np.where(ok, w2 * fc * g2, 0.0).sum()
+    kk = inv_g1 / (nr * nphi)
+    return float(a * kk), float(b * kk)
+
+
+_LUT = None
+
+
+def energy_lut() -> np.ndarray:
+    global _LUT
+    if _LUT is None:
+        n = 32
+        vals = []
+        for iy in range(n):
+            r = iy / (n - 1)
+            for ix in range(n):
+                x = ix / (n - 1)
+                a, b = specular_albedo(x * x, r * r, 48, 8)
+                vals += [a, b]
+        _LUT = np.array(vals, np.float32).reshape(512, 4)
+    return _LUT
+
+
+# ---- the render scene -------------------------------------------------------------------------------------------
+@dataclass
+class RMesh:
+    tris: np.ndarray                   # (n, 3, 3) model space
+    normals: np.ndarray | None = None  # (n, 3, 3) per corner; None: the triangle's own
+    sections: np.ndarray | None = None  # (n,) section of each triangle
+
+
+@dataclass
+class RInstance:
+    mesh: int
+    world: np.ndarray
+    materials: list                    # material index a section of the mesh
+    cast_shadow: bool = True
+
+
+@dataclass
+class Camera:
+    position: tuple
+    target: tuple
+    fov_deg: float = 40.0
+    up: tuple = (0.0, 1.0, 0.0)
+
+
+@dataclass
+class RenderScene:
+    prep: S.Prepared
+    vertices: np.ndarray
+    indices: np.ndarray
+    rows: np.ndarray
+    materials: np.ndarray
+    lights: np.ndarray
+    n_directional: int
+    emitters: np.ndarray
+    emitter_tris: np.ndarray
+    objects: np.ndarray
+    energy: np.ndarray
+    total_weight: float
+    extras: dict = field(default_factory=dict)
+
+    def tables(self) -> dict:
+        return {"vertices": self.vertices, "indices": self.indices, "rows": self.rows, "materials": self.materials,
+                "lights": self.lights, "emitters": self.emitters, "emitter_tris": self.emitter_tris,
+                "objects": self.objects, "energy": self.energy}
+
+    def params(self, width: int, height: int, camera: Camera, seed: int = 1, max_bounces: int = 4, rr_depth: int = 3,
+               clamp: float = 1e9, energy_compensation: float = 0.0, sky=(0.0, 0.0, 0.0), env_nee: bool = True,
+               sample: int = 0) -> np.ndarray:
+        """The 624-byte Params block (gpu.rs:129-185, lib.rs:713-756 for a camera)."""
+        p = np.zeros(156, np.uint32)
+        f = p.view(np.float32)
+        pos = np.asarray(camera.position, np.float64)
+        fwd = np.asarray(camera.target, np.float64) - pos
+        fwd /= np.linalg.norm(fwd)
+        right = np.cross(fwd, np.asarray(camera.up, np.float64))
+        right /= np.linalg.norm(right)
+        up = np.cross(right, fwd)
+        f[0:3] = pos
+        f[3] = math.tan(math.radians(camera.fov_deg) * 0.5)
+        f[4:7] = fwd
+        f[7] = width / height
+        f[8:11] = right
+        f[11] = 0.0
+        f[12:15] = up
+        f[15] = 1.0
+        p[16:20] = (width, height, sample, seed)
+        n_em = len(self.emitter_tris)
+        p[20:24] = (len(self.lights), n_em, max_bounces, rr_depth)
+        f[24:28] = (0.0, 0.0, 0.0, -1.0)
+        f[28:32] = (clamp, self.total_weight, energy_compensation, 0.0)
+        f[32:35] = sky
+        env_on = 1 if (env_nee and (0.2126 * sky[0] + 0.7152 * sky[1] + 0.0722 * sky[2]) > 0) else 0
+        p[36:40] = (self.n_directional, env_on, 0, 0)
+        f[68] = 0.0
+        f[70] = np.uint32(self.prep.top_first).view(np.float32)
+        f[71] = np.uint32(self.prep.units).view(np.float32)
+        f[76] = self.total_weight
+        return p
+
+
+def build(meshes: list[RMesh], instances: list[RInstance], materials: list[np.ndarray],
+          lights: list[np.ndarray] = ()) -> RenderScene:
+    """Lays out a scene as prepare does: trees and records (scene.prepare), then vertices, indices, rows, emitters."""
+    placements = [S.Placement(i.mesh, i.world, i.cast_shadow, 0) for i in instances]
+    prep = S.prepare([S.Mesh(m.tris, m.sections) for m in meshes], placements)
+    mats = np.stack(materials).astype(np.float32)
+    # vertices: three a triangle, in the mesh's original order; indices in the tree order
+    verts, vfirst, idx_parts = [], [], []
+    at = 0
+    for k, m in enumerate(meshes):
+        tris = np.asarray(m.tris, np.float32).reshape(-1, 3, 3)
+        n = len(tris)
+        if m.normals is None:
+            fn = np.cross(tris[:, 1] - tris[:, 0], tris[:, 2] - tris[:, 0]).astype(np.float64)
+            fn /= np.maximum(np.linalg.norm(fn, axis=1, keepdims=True), 1e-30)
+            nrm = np.repeat(fn[:, None, :], 3, axis=1).astype(np.float32)
+        else:
+            nrm = np.asarray(m.normals, np.float32).reshape(-1, 3, 3)
+        v = np.zeros((n * 3, 12), np.float32)
+        v[:, 0:3] = tris.reshape(-1, 3)
+        v[:, 4:7] = nrm.reshape(-1, 3)
+        verts.append(v)
+        vfirst.append(at)
+        at += n * 3
+        order = prep.mesh_order[k].astype(np.uint32)
+        idx_parts.append((order[:, None] * 3 + np.arange(3, dtype=np.uint32)[None, :]).astype(np.uint32))
+    vertices = np.concatenate(verts) if verts else np.zeros((0, 12), np.float32)
+    indices = np.concatenate(idx_parts) if idx_parts else np.zeros((0, 3), np.uint32)
+    # rows, emitters
+    rows, em_tris, entries = [], [], []
+    lum = lambda e: 0.2126 * e[0] + 0.7152 * e[1] + 0.0722 * e[2]  # noqa: E731
+    for r, inst in enumerate(instances):
+        m = meshes[inst.mesh]
+        n = len(m.tris)
+        sec = np.zeros(n, np.uint32) if m.sections is None else np.asarray(m.sections, np.uint32)
+        nsec = int(sec.max()) + 1 if n else 1
+        rfirst = len(rows)
+        prep.instances["info"][r, 2] = vfirst[inst.mesh]
+        prep.instances["info"][r, 3] = rfirst
+        mesh_first = prep.mesh_first[inst.mesh]
+        order = prep.mesh_order[inst.mesh]
+        pos_of = np.empty(n, np.int64)
+        pos_of[order] = np.arange(n)
+        w = np.asarray(inst.world, np.float64)
+        tris64 = np.asarray(m.tris, np.float64).reshape(-1, 3, 3)
+        for s in range(nsec):
+            mi = inst.materials[s] if s < len(inst.materials) else inst.materials[-1]
+            first_em = NO_EMITTER
+            if lum(mats[mi][4:7]) > 0.0:
+                first_em = len(em_tris)
+                members = np.nonzero(sec == s)[0]  # in the section's order (its rank)
+                for t in members:
+                    pw = (w[:3, :3] @ tris64[t].T).T + w[:3, 3]
+                    p32 = pw.astype(np.float32)
+                    e1 = p32[1] - p32[0]
+                    e2 = p32[2] - p32[0]
+                    area = 0.5 * float(np.linalg.norm(np.cross(pw[1] - pw[0], pw[2] - pw[0])))
+                    code = len(em_tris)
+                    rec_em = np.zeros(16, np.uint32)
+                    fl = rec_em.view(np.float32)
+                    fl[0:3] = p32[0]
+                    rec_em[3] = mesh_first + pos_of[t]
+                    fl[4:7] = e1
+                    rec_em[7] = r
+                    fl[8:11] = e2
+                    rec_em[11] = code
+                    rec_em[12] = s
+                    em_tris.append(rec_em)
+                    entries.append((code, np.float32(area) * np.float32(lum(mats[mi][4:7])), np.float32(area)))
+            rows.append((mi, 0, first_em, 0))
+    rows = np.array(rows, np.uint32).reshape(-1, 4)
+    # lib.rs:451-481 emitter_cdf, f32 sums in list order
+    n_em = len(entries)
+    cdf = np.zeros((n_em, 4), np.float32)
+    total = np.float32(0.0)
+    for k, (code, wgt, area) in enumerate(entries):
+        total = np.float32(total + np.float32(wgt))
+        cdf[k] = (np.uint32(k).view(np.float32), total, wgt, area)
+    if total > 0:
+        cdf[:, 1] = (cdf[:, 1] / total).astype(np.float32)
+    guide = np.zeros(EMITTER_GUIDE + 1, np.uint32)
+    i = 0
+    cum = cdf[:, 1]
+    for kk in range(EMITTER_GUIDE + 1):
+        u = np.float32(kk) / np.float32(EMITTER_GUIDE)
+        while i < n_em and cum[i] < u:
+            i += 1
+        guide[kk] = i
+    g4 = np.zeros(((EMITTER_GUIDE + 1 + 3) // 4) * 4, np.uint32)
+    g4[:EMITTER_GUIDE + 1] = guide
+    header = np.array([[total, 0, 0, 0]], np.float32)
+    emitters = np.concatenate([cdf, g4.view(np.float32).reshape(-1, 4), header]) if n_em else np.zeros((0, 4),
+                                                                                                         np.float32)
+    emitter_tris = np.array(em_tris, np.uint32).view(np.float32).reshape(-1, 16) if em_tris else np.zeros((0, 16),
+                                                                                                            np.float32)
+    objects = np.zeros((OBJECT_STRIDE, 4), np.float32)
+    objects[0, 3] = 1.0
+    lights = np.stack(lights).astype(np.float32) if len(lights) else np.zeros((0, 20), np.float32)
+    return RenderScene(prep, vertices, indices, rows, mats, lights, len(lights), emitters, emitter_tris, objects,
+                       energy_lut(), float(total))
+
+
+# ---- device side -----------------------------------------------------------------------------------------------
+class _RenderDesc(C.Structure):
+    _fields_ = [("nodes", C.c_void_p), ("node_count", C.c_uint64), ("leaves", C.c_void_p),
+                ("leaf_count", C.c_uint64), ("instances", C.c_void_p), ("instance_count", C.c_uint64),
+                ("top_first", C.c_uint32), ("unit_count", C.c_uint32),
+                ("vertices", C.c_void_p), ("vertex_count", C.c_uint64), ("indices", C.c_void_p),
+                ("index_count", C.c_uint64), ("rows", C.c_void_p), ("row_count", C.c_uint64),
+                ("materials", C.c_void_p), ("material_count", C.c_uint64), ("lights", C.c_void_p),
+                ("light_count", C.c_uint64), ("emitters", C.c_void_p), ("emitter_vec4s", C.c_uint64),
+                ("emitter_tris", C.c_void_p), ("emitter_tri_count", C.c_uint64), ("objects", C.c_void_p),
+                ("object_count", C.c_uint64), ("energy", C.c_void_p)]
+
+
+def upload(dev: Device, rs: RenderScene) -> Scene:
+    """gpu.rs:2613 Gpu::upload with the shading tables."""
+    p = rs.prep
+    keep = [np.ascontiguousarray(x) for x in (p.nodes, p.leaves, p.instances, rs.vertices, rs.indices, rs.rows,
+                                              rs.materials, rs.lights, rs.emitters, rs.emitter_tris, rs.objects,
+                                              rs.energy)]
+    nodes, leaves, inst, verts, idx, rows, mats, lights, ems, emt, objs, energy = keep
+    null = lambda a: _ptr(a) if a.size else None  # noqa: E731
+    desc = _RenderDesc(_ptr(nodes), len(nodes), _ptr(leaves), len(leaves), _ptr(inst), len(inst), p.top_first,
+                       p.units, null(verts), len(verts), null(idx), len(idx), null(rows), len(rows), null(mats),
+                       len(mats), null(lights), len(lights), null(ems), len(ems), null(emt), len(emt), null(objs),
+                       len(objs) // OBJECT_STRIDE, _ptr(energy))
+    sc = Scene.__new__(Scene)
+    sc.dev = dev
+    h = C.c_void_p()
+    _check(lib().trt_scene_upload(dev.h, C.byref(desc), C.byref(h)))
+    sc.h = h
+    return sc
+
+
+class Accum:
+    """records.wgsl Pixel accumulator in HBM (gpu.rs:2388 accum_buffer)."""
+
+    def __init__(self, dev: Device, width: int, height: int):
+        self.dev, self.w, self.h = dev, width, height
+        h = C.c_void_p()
+        _check(lib().trt_accum_create(dev.h, width, height, C.byref(h)))
+        self.handle = h
+
+    def clear(self):
+        _check(lib().trt_accum_clear(self.handle))
+
+    def read(self) -> np.ndarray:
+        out = np.zeros((self.h, self.w, 20), np.float32)
+        _check(lib().trt_accum_read(self.handle, _ptr(out)))
+        return out
+
+    def resolve(self):
+        c = np.zeros((self.h, self.w, 3), np.float32)
+        a = np.zeros_like(c)
+        n = np.zeros_like(c)
+        v = np.zeros((self.h, self.w), np.float32)
+        _check(lib().trt_accum_resolve(self.handle, _ptr(c), _ptr(a), _ptr(n), _ptr(v)))
+        return c, a, n, v
+
+    def close(self):
+        if self.handle:
+            lib().trt_accum_release(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def render(scene: Scene, params: np.ndarray, first_sample: int, spp: int, acc: Accum):
+    """gpu.rs:4116 Gpu::render's sample loop: samples first..first+spp-1 into `acc`, queued on the stream."""
+    params = np.ascontiguousarray(params, np.uint32)
+    if params.size != 156:
+        raise ThreeRTError(TRT_E_INVALID, "params must be 156 words")
+    _check(lib().trt_render(scene.h, _ptr(params), first_sample, spp, acc.handle))
+
+
+def _sig():
+    L = lib()
+    P, U32, I = C.c_void_p, C.c_uint32, C.c_int
+    L.trt_accum_create.argtypes = [P, U32, U32, C.POINTER(C.c_void_p)]
+    L.trt_accum_release.argtypes = [P]
+    L.trt_accum_clear.argtypes = [P]
+    L.trt_accum_read.argtypes = [P, P]
+    L.trt_accum_resolve.argtypes = [P, P, P, P, P]
+    L.trt_render.argtypes = [P, P, U32, U32, P]
+    for n in ("trt_accum_create", "trt_accum_release", "trt_accum_clear", "trt_accum_read", "trt_accum_resolve",
+              "trt_render"):
+        getattr(L, n).restype = I
+
+
+_sig()
+
+
+# ---- seeded workloads --------------------------------------------------------------------------------------------
+def quad(p0, p1, p2, p3) -> np.ndarray:
+    """Two triangles (p0 p1 p2), (p0 p2 p3)."""
+    return np.array([[p0, p1, p2], [p0, p2, p3]], np.float32)
+
+
+def box_tris(lo, hi) -> np.ndarray:
+    x0, y0, z0 = lo
+    x1, y1, z1 = hi
+    P = [(x0, y0, z0), (x1, y0, z0), (x1, y1, z0), (x0, y1, z0), (x0, y0, z1), (x1, y0, z1), (x1, y1, z1), (x0, y1, z1)]
+    faces = [(0, 3, 2, 1), (4, 5, 6, 7), (0, 4, 7, 3), (1, 2, 6, 5), (0, 1, 5, 4), (3, 7, 6, 2)]
+    return np.concatenate([quad(P[a], P[b], P[c], P[d]) for a, b, c, d in faces])
+
+
+def cornell(seed: int = 1):
+    """The plain-method scene: a unit-cube room of five Lambertian walls (0.73 grey; red 0.65/0.05/0.05; green
+    0.12/0.45/0.15), two boxes rotated +-18 degrees, a 0.25 x 0.25 emissive ceiling quad (radiance 17), 36 triangles.
+    Returns (RenderScene, Camera)."""
+    mats = [material((0.73, 0.73, 0.73), specular=0.0), material((0.65, 0.05, 0.05), specular=0.0),
+            material((0.12, 0.45, 0.15), specular=0.0), material((0.0, 0.0, 0.0), (17.0, 17.0, 17.0), specular=0.0)]
+    floor = quad((0, 0, 0), (0, 0, 1), (1, 0, 1), (1, 0, 0))
+    ceiling = quad((0, 1, 0), (1, 1, 0), (1, 1, 1), (0, 1, 1))
+    back = quad((0, 0, 1), (0, 1, 1), (1, 1, 1), (1, 0, 1))
+    left = quad((0, 0, 0), (0, 1, 0), (0, 1, 1), (0, 0, 1))
+    right = quad((1, 0, 0), (1, 0, 1), (1, 1, 1), (1, 1, 0))
+    light = quad((0.375, 0.999, 0.375), (0.625, 0.999, 0.375), (0.625, 0.999, 0.625), (0.375, 0.999, 0.625))
+    tall = box_tris((-0.15, 0.0, -0.15), (0.15, 0.6, 0.15))
+    short = box_tris((-0.15, 0.0, -0.15), (0.15, 0.3, 0.15))
+    meshes = [RMesh(x) for x in (floor, ceiling, back, left, right, light, tall, short)]
+    eye = np.eye(4, dtype=np.float32)
+    inst = [RInstance(0, eye, [0]), RInstance(1, eye, [0]), RInstance(2, eye, [0]), RInstance(3, eye, [1]),
+            RInstance(4, eye, [2]), RInstance(5, eye, [3]),
+            RInstance(6, S.trs((0.33, 0.0, 0.62), (0.0, math.radians(18.0), 0.0), (1, 1, 1)), [0]),
+            RInstance(7, S.trs((0.67, 0.0, 0.35), (0.0, math.radians(-18.0), 0.0), (1, 1, 1)), [0])]
+    rs = build(meshes, inst, mats)
+    cam = Camera((0.5, 0.5, -1.35), (0.5, 0.5, 0.5), 40.0)
+    return rs, cam
+
+
+def icosphere(subdiv: int) -> tuple[np.ndarray, np.ndarray]:
+    t = (1.0 + 5 ** 0.5) / 2.0
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2), (10, 7, 6),
+         (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10),
+         (8, 6, 7), (9, 8, 1)]
+    tris = np.array([[v[a], v[b], v[c]] for a, b, c in f], np.float64)
+    tris /= np.linalg.norm(tris, axis=2, keepdims=True)
+    for _ in range(subdiv):
+        a, b, c = tris[:, 0], tris[:, 1], tris[:, 2]
+        ab, bc, ca = (a + b), (b + c), (c + a)
+        ab /= np.linalg.norm(ab, axis=1, keepdims=True)
+        bc /= np.linalg.norm(bc, axis=1, keepdims=True)
+        ca /= np.linalg.norm(ca, axis=1, keepdims=True)
+        tris = np.concatenate([np.stack([a, ab, ca], 1), np.stack([ab, b, bc], 1), np.stack([ca, bc, c], 1),
+                               np.stack([ab, bc, ca], 1)])
+    return tris.astype(np.float32), tris.astype(np.float32)
+
+
+def extruded_box(rng: np.random.Generator, n_target: int) -> np.ndarray:
+    """A box whose six faces are grids, about n_target triangles: a building."""
+    k = max(1, int(round(math.sqrt(n_target / 12.0))))
+    w, d, h = rng.uniform(0.5, 1.5), rng.uniform(0.5, 1.5), rng.uniform(1.0, 6.0)
+    g = np.linspace(0.0, 1.0, k + 1)
+    parts = []
+    for axis in range(3):
+        for side in (0.0, 1.0):
+            for i in range(k):
+                for j in range(k):
+                    u0, u1, v0, v1 = g[i], g[i + 1], g[j], g[j + 1]
+                    pts = []
+                    for (uu, vv) in ((u0, v0), (u1, v0), (u1, v1), (u0, v1)):
+                        p = [0.0, 0.0, 0.0]
+                        p[axis] = side
+                        p[(axis + 1) % 3] = uu
+                        p[(axis + 2) % 3] = vv
+                        pts.append(p)
+                    q = quad(*pts)
+                    if side == 0.0:
+                        q = q[:, ::-1]
+                    parts.append(q)
+    tris = np.concatenate(parts).astype(np.float64)
+    tris = (tris - np.array([0.5, 0.0, 0.5])) * np.array([w, h, d])
+    return tris.astype(np.float32)
+
+
+def city(seed: int = 1, meshes: int = 64, instances: int = 4096, extent: float = 400.0):
+    """The instanced city: `meshes` seeded base meshes (extruded boxes and subdivided spheres, 200-40k triangles,
+    log-uniform) placed `instances` times by random TRS on a ground plane, metal-rough GGX (roughness U[0.05, 1],
+    metallic Bernoulli(0.3)), a 90,000-lux sun and a constant sky. Returns (RenderScene, Camera, sky)."""
+    rng = np.random.default_rng(seed)
+    ms = [RMesh(quad((-extent, 0, -extent), (-extent, 0, extent), (extent, 0, extent), (extent, 0, -extent)))]
+    for k in range(meshes - 1):
+        n = int(math.exp(rng.uniform(math.log(200), math.log(40000))))
+        if k % 2 == 0:
+            ms.append(RMesh(extruded_box(rng, n)))
+        else:
+            sub = max(1, min(5, int(round(math.log(max(n / 20.0, 1.0), 4)))))
+            t, nrm = icosphere(sub)
+            ms.append(RMesh(t, normals=nrm))
+    mats, inst = [material((0.5, 0.5, 0.5), roughness=0.8)], [RInstance(0, np.eye(4, dtype=np.float32), [0])]
+    for i in range(instances - 1):
+        m = int(rng.integers(1, meshes))
+        s = rng.uniform(2.0, 12.0)
+        pos = ((rng.random() - 0.5) * 2 * extent * 0.9, 0.0 if m % 2 == 1 else s * 0.9, (rng.random() - 0.5) * 2 * extent * 0.9)
+        w = S.trs(pos, (0.0, rng.uniform(0, 2 * math.pi), 0.0), (s, s * rng.uniform(0.6, 1.8), s))
+        base = rng.uniform(0.2, 0.9, 3)
+        mats.append(material(tuple(base), metallic=float(rng.random() < 0.3), roughness=float(rng.uniform(0.05, 1.0))))
+        inst.append(RInstance(m, w, [len(mats) - 1]))
+    sun = directional_light((-0.4, -0.8, -0.45), (90000.0, 88000.0, 82000.0))
+    rs = build(ms, inst, mats, [sun])
+    cam = Camera((-extent * 0.3, 60.0, -extent * 0.45), (0.0, 0.0, 0.0), 55.0)
+    return rs, cam, (6000.0, 8000.0, 12000.0)

@@ -59,6 +59,16 @@ def leaves_of(tris: np.ndarray, sections: np.ndarray, index: np.ndarray) -> np.n
     return out
 
 
+def section_rank(sections: np.ndarray) -> np.ndarray:
+    """Each triangle's index in its section, in the mesh's triangle order (the leaf's second id)."""
+    sections = np.asarray(sections, np.uint32)
+    rank = np.zeros(len(sections), np.uint32)
+    for s in np.unique(sections):
+        idx = np.nonzero(sections == s)[0]
+        rank[idx] = np.arange(len(idx), dtype=np.uint32)
+    return rank
+
+
 def instance_record(world: np.ndarray, first: int, obj: int = 0) -> np.ndarray:
     """scene.rs:239-246 Instance::gpu: m rows, normal rows (M3^-1)^T with the bits words, info, p rows.
     (The inverse is taken in f64 and rounded; glam's f32 inverse is not reproduced.)"""
@@ -83,7 +93,7 @@ def prepare(meshes: list[Mesh], placements: list[Placement], mesh_room: int | No
         t = mesh_tree(tris)
         order = t["order"]
         sec = np.zeros(len(tris), np.uint32) if m.sections is None else np.asarray(m.sections, np.uint32)
-        leaf_parts.append(leaves_of(tris[order], sec[order], order.astype(np.uint32)))
+        leaf_parts.append(leaves_of(tris[order], sec[order], section_rank(sec)[order]))
         trees.append(t)
         firsts.append(first)
         orders.append(order)
