@@ -28,7 +28,9 @@ def test_every_declared_symbol_is_exported():
 def test_the_structs_match_the_header_sizes():
     assert T.NODE.itemsize == 32 and T.INSTANCE.itemsize == 160
     assert T.RAY.itemsize == 32 and T.HIT.itemsize == 24
-    assert ctypes.sizeof(T._SceneDesc) == 8 * 6 + 8
+    from threert_amd import render as R
+
+    assert ctypes.sizeof(R._RenderDesc) == 8 * 6 + 8 + 8 * 17
     assert ctypes.sizeof(T._Tree) == 8 * 4 + 8 + 24
 
 

@@ -40,6 +40,7 @@ EXPORTS = [
     "trt_build_tris", "trt_build_mesh_tree", "trt_build_sah", "trt_build_median", "trt_build_top_tree",
     "trt_unit_box", "trt_unit_record", "trt_tree_free", "trt_scene_upload", "trt_scene_release",
     "trt_scene_stats_get", "trt_trace_rays", "trt_trace_rays_device", "trt_trace_reads",
+    "trt_accum_create", "trt_accum_release", "trt_accum_clear", "trt_accum_read", "trt_accum_resolve", "trt_render",
 ]
 
 
@@ -55,9 +56,16 @@ class _Tree(C.Structure):
 
 
 class _SceneDesc(C.Structure):
+    """trt_scene_desc: the trees, then the shading tables (NULL for a scene that only traces rays)."""
     _fields_ = [("nodes", C.c_void_p), ("node_count", C.c_uint64), ("leaves", C.c_void_p),
                 ("leaf_count", C.c_uint64), ("instances", C.c_void_p), ("instance_count", C.c_uint64),
-                ("top_first", C.c_uint32), ("unit_count", C.c_uint32)]
+                ("top_first", C.c_uint32), ("unit_count", C.c_uint32),
+                ("vertices", C.c_void_p), ("vertex_count", C.c_uint64), ("indices", C.c_void_p),
+                ("index_count", C.c_uint64), ("rows", C.c_void_p), ("row_count", C.c_uint64),
+                ("materials", C.c_void_p), ("material_count", C.c_uint64), ("lights", C.c_void_p),
+                ("light_count", C.c_uint64), ("emitters", C.c_void_p), ("emitter_vec4s", C.c_uint64),
+                ("emitter_tris", C.c_void_p), ("emitter_tri_count", C.c_uint64), ("objects", C.c_void_p),
+                ("object_count", C.c_uint64), ("energy", C.c_void_p)]
 
 
 class _SceneStats(C.Structure):

@@ -296,16 +296,7 @@ def build(meshes: list[RMesh], instances: list[RInstance], materials: list[np.nd
 
 
 # ---- device side -----------------------------------------------------------------------------------------------
-class _RenderDesc(C.Structure):
-    _fields_ = [("nodes", C.c_void_p), ("node_count", C.c_uint64), ("leaves", C.c_void_p),
-                ("leaf_count", C.c_uint64), ("instances", C.c_void_p), ("instance_count", C.c_uint64),
-                ("top_first", C.c_uint32), ("unit_count", C.c_uint32),
-                ("vertices", C.c_void_p), ("vertex_count", C.c_uint64), ("indices", C.c_void_p),
-                ("index_count", C.c_uint64), ("rows", C.c_void_p), ("row_count", C.c_uint64),
-                ("materials", C.c_void_p), ("material_count", C.c_uint64), ("lights", C.c_void_p),
-                ("light_count", C.c_uint64), ("emitters", C.c_void_p), ("emitter_vec4s", C.c_uint64),
-                ("emitter_tris", C.c_void_p), ("emitter_tri_count", C.c_uint64), ("objects", C.c_void_p),
-                ("object_count", C.c_uint64), ("energy", C.c_void_p)]
+from . import _SceneDesc as _RenderDesc  # noqa: E402  (trt_scene_desc with its shading tables)
 
 
 def upload(dev: Device, rs: RenderScene) -> Scene:
