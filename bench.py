@@ -53,6 +53,39 @@ def cpu_baseline(prep, rays: np.ndarray, sample: int) -> dict:
                       f"reference's binary node buffer (-O3, OpenMP, {threads} threads), {dt:.1f} s"}
 
 
+def bench_render(dev, frames: int) -> dict:
+    """Samples/s at 720p: the instanced city (64 seeded meshes x 4096 instances), 1280 x 720, 1 spp a frame,
+    3 bounces, sun + constant sky; `frames` frames (one sample each) timed with HIP events."""
+    from threert_amd import render as R
+
+    log("bench: building the instanced city")
+    t0 = time.perf_counter()
+    rs, cam, sky = R.city(seed=1)
+    log(f"bench: city ready in {time.perf_counter() - t0:.1f} s ({len(rs.prep.leaves)} unique triangles)")
+    w, h = 1280, 720
+    p = rs.params(w, h, cam, seed=1, max_bounces=3, rr_depth=3, sky=sky)
+    sc = R.upload(dev, rs)
+    acc = R.Accum(dev, w, h)
+    R.render(sc, p, 0, 1, acc)  # warm
+    dev.sync()
+    dev.timer_start()
+    for f in range(frames):
+        R.render(sc, p, 1 + f, 1, acc)
+    ms = dev.timer_stop()
+    dev.sync()
+    ms_frame = ms / frames
+    sc.close()
+    acc.close()
+    sps = w * h / (ms_frame * 1e-3)
+    log(f"bench: city 720p {ms_frame:.2f} ms/frame, {sps / 1e6:.1f} Msamples/s")
+    return {"metric": "samples/s at 720p", "value": round(sps, 1), "unit": "samples/s",
+            "ms_per_frame": round(ms_frame, 3), "frames": frames,
+            "workload": "instanced city: 64 seeded meshes (extruded boxes, subdivided spheres, 200-40k triangles) x "
+                        "4096 instances, metal-rough GGX, 90,000-lux sun + constant sky, 1280x720, 1 spp a frame, "
+                        "3 bounces, megakernel over the 4-wide quantised walk",
+            "unique_triangles": int(len(rs.prep.leaves))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -60,6 +93,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-sample", type=int, default=2_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-render", action="store_true")
+    ap.add_argument("--render-frames", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus})", file=sys.stderr)
@@ -132,6 +167,8 @@ def main():
     sc.close()
     d_rays.free()
     d_hits.free()
+    if not args.no_render:
+        out["render_720p"] = bench_render(dev, args.render_frames)
     if not args.no_cpu:
         log("bench: CPU baseline")
         out["cpu_baseline"] = cpu_baseline(prep, rays, min(args.cpu_sample, N_RAYS))
