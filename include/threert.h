@@ -190,6 +190,12 @@ int trt_accum_resolve(trt_accum *acc, float *color, float *albedo, float *normal
  * must be the accumulator's. Needs a scene uploaded with its shading tables. */
 int trt_render(trt_scene *scene, const void *params, uint32_t first_sample, uint32_t spp, trt_accum *acc);
 
+/* --- denoising (denoise.rs:10-74 denoise: edge-avoiding a-trous guided by albedo and normal) ---------------- */
+/* Device buffers (trt_mem_alloc): color, albedo, normal 3 floats a pixel, variance 1 (trt_accum_resolve's
+ * outputs), out 3 floats a pixel. Queued on the device's stream. */
+int trt_denoise_atrous(trt_device *dev, uint32_t width, uint32_t height, const float *d_color, const float *d_albedo,
+                       const float *d_normal, const float *d_variance, uint32_t iterations, float *d_out);
+
 #ifdef __cplusplus
 }
 #endif

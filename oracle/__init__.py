@@ -299,6 +299,25 @@ def hash_hp(x: int) -> int:
     return int(L.or_hash_hp(x))
 
 
+def denoise_atrous(color, albedo, normal, variance, iterations: int = 4) -> np.ndarray:
+    """denoise.rs:10-74 restated (render.c's sibling denoise.c)."""
+    L = lib()
+    L.or_denoise_atrous.argtypes = [C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_uint32, C.c_void_p]
+    h, w = color.shape[:2]
+    a = [np.ascontiguousarray(x, np.float32) for x in (color, albedo, normal, variance)]
+    out = np.zeros((h, w, 3), np.float32)
+    L.or_denoise_atrous(w, h, *[_ptr(x) for x in a], iterations, _ptr(out))
+    return out
+
+
+def expf(x: float) -> float:
+    L = lib()
+    L.or_expf.argtypes = [C.c_float]
+    L.or_expf.restype = C.c_float
+    return float(L.or_expf(C.c_float(x)))
+
+
 def sincos(x: float):
     L = lib()
     L.or_sincos.argtypes = [C.c_float, C.c_void_p, C.c_void_p]

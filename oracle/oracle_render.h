@@ -62,6 +62,11 @@ typedef struct {
 int64_t or_render_sample(const or_render_scene *s, const uint32_t *params, uint32_t y0, uint32_t y1, float *accum,
                          int threads);
 
+/* denoise.rs:10-74 denoise (a-trous), and the expf it uses here. Images: 3 floats a pixel (variance: 1). */
+float or_expf(float x);
+void or_denoise_atrous(uint32_t w, uint32_t h, const float *color, const float *albedo, const float *normal,
+                       const float *variance, uint32_t iterations, float *out);
+
 #ifdef __cplusplus
 }
 #endif

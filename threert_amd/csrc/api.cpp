@@ -12,6 +12,7 @@
 #include <string>
 
 #include "../../include/threert.h"
+#include "denoise_kernels.h"
 #include "render_kernels.h"
 #include "trace_kernels.h"
 #include "trt_internal.h"
@@ -48,6 +49,8 @@ struct trt_device {
     uint32_t *spill = nullptr;
     uint64_t spill_bytes = 0;
     unsigned long long *reads = nullptr;
+    void *denoise_scratch = nullptr;  // trt_denoise_atrous' intermediate images
+    uint64_t denoise_bytes = 0;
     void *staging = nullptr;  // trt_trace_rays' device copies of the caller's rays and hits
     uint64_t staging_bytes = 0;
     int cus = 0;
@@ -112,6 +115,7 @@ int trt_device_release(trt_device *d) {
     hipFree(d->counter);
     hipFree(d->guard);
     hipFree(d->reads);
+    hipFree(d->denoise_scratch);
     hipFree(d->spill);
     hipFree(d->staging);
     if (d->ev0) hipEventDestroy(d->ev0);
@@ -681,6 +685,29 @@ int trt_render(trt_scene *s, const void *params, uint32_t first_sample, uint32_t
     a.first_sample = first_sample;
     a.spp = spp;
     if (trt::launch_render(a, grid, d->stream)) return fail(TRT_E_DEVICE, "render kernel launch failed");
+    return ok();
+}
+
+int trt_denoise_atrous(trt_device *d, uint32_t w, uint32_t h, const float *color, const float *albedo,
+                       const float *normal, const float *variance, uint32_t iterations, float *out) {
+    if (!d || !color || !albedo || !normal || !variance || !out || w == 0 || h == 0)
+        return fail(TRT_E_INVALID, "trt_denoise_atrous: bad argument");
+    if (uint64_t(w) * h >= (uint64_t{1} << 31) || iterations > 16)
+        return fail(TRT_E_INVALID, "trt_denoise_atrous: image too large or too many iterations");
+    std::lock_guard<std::mutex> lock(d->mu);
+    TRT_HIP(hipSetDevice(d->ordinal));
+    const uint64_t need = uint64_t(w) * h * 9 * sizeof(float);
+    if (need > d->denoise_bytes) {
+        TRT_HIP(hipStreamSynchronize(d->stream));
+        TRT_HIP(hipFree(d->denoise_scratch));
+        d->denoise_scratch = nullptr;
+        d->denoise_bytes = 0;
+        TRT_HIP(hipMalloc(&d->denoise_scratch, need));
+        d->denoise_bytes = need;
+    }
+    if (trt::launch_atrous(w, h, color, albedo, normal, variance, iterations, out,
+                           static_cast<float *>(d->denoise_scratch), d->stream))
+        return fail(TRT_E_DEVICE, "denoise kernel launch failed");
     return ok();
 }
 
