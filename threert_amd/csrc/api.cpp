@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <atomic>
 #include <mutex>
 #include <new>
 #include <string>
@@ -57,6 +58,7 @@ struct trt_device {
     int blocks_wide = 1, blocks_binary = 1, blocks_render = 1;
     hipDeviceProp_t props{};
     std::mutex mu;
+    std::atomic<int> refs{1};
 };
 
 struct trt_scene {
@@ -108,8 +110,9 @@ int trt_device_create(int ordinal, trt_device **out) {
     return ok();
 }
 
-int trt_device_release(trt_device *d) {
-    if (!d) return ok();
+// A device lives while its handle or any scene or accumulator made on it does: the last release frees it.
+static void device_unref(trt_device *d) {
+    if (--d->refs > 0) return;
     hipSetDevice(d->ordinal);
     if (d->stream) hipStreamSynchronize(d->stream);
     hipFree(d->counter);
@@ -122,6 +125,11 @@ int trt_device_release(trt_device *d) {
     if (d->ev1) hipEventDestroy(d->ev1);
     if (d->stream) hipStreamDestroy(d->stream);
     delete d;
+}
+
+int trt_device_release(trt_device *d) {
+    if (!d) return ok();
+    device_unref(d);
     return ok();
 }
 
@@ -275,7 +283,9 @@ int trt_scene_release(trt_scene *s) {
     for (void *p : {s->vertices, s->indices, s->rows, s->materials, s->lights, s->emitters, s->emitter_tris,
                     s->objects, s->energy})
         hipFree(p);
+    trt_device *d = s->dev;
     delete s;
+    device_unref(d);
     return ok();
 }
 
@@ -446,6 +456,7 @@ int trt_scene_upload(trt_device *d, const trt_scene_desc *desc, trt_scene **out)
     auto *s = new (std::nothrow) trt_scene();
     if (!s) return fail(TRT_E_OOM, "trt_scene_upload: out of host memory");
     s->dev = d;
+    d->refs++;
     uint64_t total = 0;
     int rc = upload(d, &s->nodes, desc->nodes, desc->node_count * sizeof(trt::Node), &total);
     if (!rc) rc = upload(d, &s->leaves, desc->leaves, desc->leaf_count * sizeof(trt::Leaf), &total);
@@ -586,6 +597,7 @@ int trt_accum_create(trt_device *d, uint32_t width, uint32_t height, trt_accum *
         delete a;
         return fail(e == hipErrorOutOfMemory ? TRT_E_OOM : TRT_E_DEVICE, hipGetErrorString(e));
     }
+    d->refs++;
     *out = a;
     return ok();
 }
@@ -594,7 +606,9 @@ int trt_accum_release(trt_accum *a) {
     hipSetDevice(a->dev->ordinal);
     hipStreamSynchronize(a->dev->stream);
     hipFree(a->buf);
+    trt_device *d = a->dev;
     delete a;
+    device_unref(d);
     return ok();
 }
 int trt_accum_clear(trt_accum *a) {
