@@ -203,7 +203,7 @@ struct TriAttr {
 
 __device__ TriAttr tri_attr(const Ctx &c, uint32_t ti, uint32_t inst, uint32_t part, V &e1, V &e2) {
     uint32_t base = 0;
-    if (inst != kNone) base = __float_as_uint(c.inst(inst)[6].y);
+    if (inst != kNone) base = __float_as_uint(c.inst(inst)[6].z);
     const uint32_t *ix = c.a->indices + 3 * uint64_t(ti);
     const float4 *p0 = c.a->vertices + 3 * (uint64_t(ix[0]) + base);
     const float4 *p1 = c.a->vertices + 3 * (uint64_t(ix[1]) + base);
