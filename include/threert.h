@@ -190,6 +190,19 @@ int trt_accum_resolve(trt_accum *acc, float *color, float *albedo, float *normal
  * must be the accumulator's. Needs a scene uploaded with its shading tables. */
 int trt_render(trt_scene *scene, const void *params, uint32_t first_sample, uint32_t spp, trt_accum *acc);
 
+/* Several render jobs in one launch (continuous batching for a render server; the reference's `serve` loop,
+ * main.rs:345-398, renders one job at a time and has no counterpart): each job is trt_render's arguments; their
+ * tiles share the GPU. A job's accumulator must not appear twice in one batch. Each job's result is the one
+ * trt_render gives it alone. Queued on the device's stream. */
+typedef struct {
+    trt_scene *scene;
+    const void *params; /* 624-byte Params */
+    uint32_t first_sample;
+    uint32_t spp;
+    trt_accum *acc;
+} trt_render_job;
+int trt_render_batch(trt_device *dev, const trt_render_job *jobs, uint32_t n);
+
 /* --- denoising (denoise.rs:10-74 denoise: edge-avoiding a-trous guided by albedo and normal) ---------------- */
 /* Device buffers (trt_mem_alloc): color, albedo, normal 3 floats a pixel, variance 1 (trt_accum_resolve's
  * outputs), out 3 floats a pixel. Queued on the device's stream. */

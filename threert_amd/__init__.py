@@ -41,7 +41,7 @@ EXPORTS = [
     "trt_unit_box", "trt_unit_record", "trt_tree_free", "trt_scene_upload", "trt_scene_release",
     "trt_scene_stats_get", "trt_trace_rays", "trt_trace_rays_device", "trt_trace_reads",
     "trt_accum_create", "trt_accum_release", "trt_accum_clear", "trt_accum_read", "trt_accum_resolve", "trt_render",
-    "trt_denoise_atrous",
+    "trt_denoise_atrous", "trt_render_batch",
 ]
 
 

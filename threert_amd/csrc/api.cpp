@@ -41,8 +41,18 @@ int ok() {
 
 }  // namespace
 
+constexpr int kJobRing = 8;
+struct JobRing {
+    void *host[kJobRing] = {};
+    void *dev[kJobRing] = {};
+    uint64_t cap[kJobRing] = {};
+    hipEvent_t ev[kJobRing] = {};
+    int next = 0;
+};
+
 struct trt_device {
     int ordinal = 0;
+    JobRing jobs;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     uint32_t *counter = nullptr;
@@ -121,6 +131,11 @@ static void device_unref(trt_device *d) {
     hipFree(d->denoise_scratch);
     hipFree(d->spill);
     hipFree(d->staging);
+    for (int k = 0; k < kJobRing; k++) {
+        hipHostFree(d->jobs.host[k]);
+        hipFree(d->jobs.dev[k]);
+        if (d->jobs.ev[k]) hipEventDestroy(d->jobs.ev[k]);
+    }
     if (d->ev0) hipEventDestroy(d->ev0);
     if (d->ev1) hipEventDestroy(d->ev1);
     if (d->stream) hipStreamDestroy(d->stream);
@@ -650,37 +665,26 @@ int trt_accum_resolve(trt_accum *a, float *color, float *albedo, float *normal, 
     return ok();
 }
 
-int trt_render(trt_scene *s, const void *params, uint32_t first_sample, uint32_t spp, trt_accum *acc) {
-    if (!s || !params || !acc) return fail(TRT_E_INVALID, "trt_render: null argument");
-    if (!s->shading) return fail(TRT_E_INVALID, "trt_render: the scene was uploaded without its shading tables");
-    if (acc->dev != s->dev) return fail(TRT_E_INVALID, "trt_render: the accumulator is another device's");
-    trt::RenderArgs a{};
-    std::memcpy(a.params, params, sizeof(a.params));
+// One job's arguments, checked against its scene and accumulator.
+static int job_args(const trt_render_job &j, trt::RenderArgs &a, std::string *why) {
+    trt_scene *s = j.scene;
+    trt_accum *acc = j.acc;
+    if (!s || !j.params || !acc) return *why = "null argument", TRT_E_INVALID;
+    if (!s->shading) return *why = "the scene was uploaded without its shading tables", TRT_E_INVALID;
+    if (acc->dev != s->dev) return *why = "the accumulator is another device's", TRT_E_INVALID;
+    a = trt::RenderArgs{};
+    std::memcpy(a.params, j.params, sizeof(a.params));
     const uint32_t *P = a.params;
-    if (P[16] != acc->w || P[17] != acc->h) return fail(TRT_E_INVALID, "trt_render: P.size is not the accumulator's");
+    if (P[16] != acc->w || P[17] != acc->h) return *why = "P.size is not the accumulator's", TRT_E_INVALID;
     if (P[20] > s->light_count || P[36] > P[20] || P[21] > s->emitter_entries)
-        return fail(TRT_E_INVALID, "trt_render: P.counts names more lights or emitters than the scene has");
+        return *why = "P.counts names more lights or emitters than the scene has", TRT_E_INVALID;
     if (P[21] > 0 && P[21] != s->emitter_entries)
-        return fail(TRT_E_INVALID, "trt_render: P.counts.y is not the emitter table's entry count");
-    if (P[22] > 64) return fail(TRT_E_INVALID, "trt_render: more than 64 bounces");
-    if (trt::from_bits(P[24]) > 0.5f) return fail(TRT_E_INVALID, "trt_render: environment maps are not supported");
-    if (spp == 0) return ok();
-    trt_device *d = s->dev;
-    std::lock_guard<std::mutex> lock(d->mu);
-    TRT_HIP(hipSetDevice(d->ordinal));
-    const int grid = d->cus * d->blocks_render;
-    const uint32_t threads = static_cast<uint32_t>(grid) * trt::kRenderBlock;
-    const uint64_t entries = s->max_stack > trt::kLdsStack ? s->max_stack - trt::kLdsStack : 1;
-    int rc = ensure_spill(d, entries * threads * sizeof(uint32_t));
-    if (rc) return rc;
+        return *why = "P.counts.y is not the emitter table's entry count", TRT_E_INVALID;
+    if (P[22] > 64) return *why = "more than 64 bounces", TRT_E_INVALID;
+    if (trt::from_bits(P[24]) > 0.5f) return *why = "environment maps are not supported", TRT_E_INVALID;
     a.geo.wnodes = static_cast<const uint4 *>(s->wnodes);
     a.geo.wunits = static_cast<const float4 *>(s->wunits);
     a.geo.leaves = static_cast<const float4 *>(s->leaves);
-    a.geo.counter = d->counter;
-    a.geo.guard_flag = d->guard;
-    a.geo.spill = d->spill;
-    a.geo.reads = d->reads;
-    a.geo.spill_stride = threads;
     a.geo.top_ref = s->top_ref;
     a.geo.top_first = s->top_first;
     a.geo.units = s->units;
@@ -696,10 +700,84 @@ int trt_render(trt_scene *s, const void *params, uint32_t first_sample, uint32_t
     a.objects = static_cast<const float4 *>(s->objects);
     a.energy = static_cast<const float4 *>(s->energy);
     a.accum = static_cast<float4 *>(acc->buf);
-    a.first_sample = first_sample;
-    a.spp = spp;
-    if (trt::launch_render(a, grid, d->stream)) return fail(TRT_E_DEVICE, "render kernel launch failed");
+    a.first_sample = j.first_sample;
+    a.spp = j.spp;
+    return TRT_OK;
+}
+
+int trt_render_batch(trt_device *d, const trt_render_job *jobs, uint32_t n) {
+    if (!d || (n && !jobs)) return fail(TRT_E_INVALID, "trt_render_batch: null argument");
+    if (n > 4096) return fail(TRT_E_INVALID, "trt_render_batch: more than 4096 jobs in one launch");
+    std::vector<trt::RenderJob> table;
+    table.reserve(n);
+    uint64_t tiles = 0;
+    uint32_t deepest = 0;
+    for (uint32_t k = 0; k < n; k++) {
+        if (jobs[k].spp == 0) continue;
+        if (!jobs[k].scene || jobs[k].scene->dev != d)
+            return fail(TRT_E_INVALID, "trt_render_batch: job " + std::to_string(k) + "'s scene is not this device's");
+        trt::RenderJob j{};
+        std::string why;
+        int rc = job_args(jobs[k], j.args, &why);
+        if (rc) return fail(rc, "trt_render_batch: job " + std::to_string(k) + ": " + why);
+        for (uint32_t q = 0; q < k; q++)
+            if (jobs[q].spp && jobs[q].acc == jobs[k].acc)
+                return fail(TRT_E_INVALID, "trt_render_batch: an accumulator appears twice in one batch");
+        j.tile_first = static_cast<uint32_t>(tiles);
+        tiles += uint64_t((jobs[k].acc->w + 7) / 8) * ((jobs[k].acc->h + 7) / 8);
+        deepest = std::max(deepest, jobs[k].scene->max_stack);
+        table.push_back(j);
+    }
+    if (tiles >= (uint64_t{1} << 31)) return fail(TRT_E_INVALID, "trt_render_batch: too many tiles in one launch");
+    if (table.empty()) return ok();
+    std::lock_guard<std::mutex> lock(d->mu);
+    TRT_HIP(hipSetDevice(d->ordinal));
+    const int grid = d->cus * d->blocks_render;
+    const uint32_t threads = static_cast<uint32_t>(grid) * trt::kRenderBlock;
+    const uint64_t entries = deepest > trt::kLdsStack ? deepest - trt::kLdsStack : 1;
+    int rc = ensure_spill(d, entries * threads * sizeof(uint32_t));
+    if (rc) return rc;
+    for (auto &j : table) {
+        j.args.geo.counter = d->counter;
+        j.args.geo.guard_flag = d->guard;
+        j.args.geo.spill = d->spill;
+        j.args.geo.reads = d->reads;
+        j.args.geo.spill_stride = threads;
+    }
+    // The job table goes through a ring of pinned host slots and device slots: a slot is written again only after
+    // the launch that read it is done (its event), so launches queue without waiting for each other.
+    auto &ring = d->jobs;
+    const int k = ring.next;
+    ring.next = (ring.next + 1) % kJobRing;
+    if (ring.ev[k]) TRT_HIP(hipEventSynchronize(ring.ev[k]));
+    const uint64_t bytes = table.size() * sizeof(trt::RenderJob);
+    if (bytes > ring.cap[k]) {
+        TRT_HIP(hipHostFree(ring.host[k]));
+        TRT_HIP(hipFree(ring.dev[k]));
+        ring.host[k] = nullptr, ring.dev[k] = nullptr, ring.cap[k] = 0;
+        const uint64_t cap = std::max<uint64_t>(bytes, 64 * sizeof(trt::RenderJob));
+        TRT_HIP(hipHostMalloc(&ring.host[k], cap));
+        TRT_HIP(hipMalloc(&ring.dev[k], cap));
+        ring.cap[k] = cap;
+    }
+    if (!ring.ev[k]) TRT_HIP(hipEventCreateWithFlags(&ring.ev[k], hipEventDisableTiming));
+    std::memcpy(ring.host[k], table.data(), bytes);
+    TRT_HIP(hipMemcpyAsync(ring.dev[k], ring.host[k], bytes, hipMemcpyHostToDevice, d->stream));
+    if (trt::launch_render(static_cast<const trt::RenderJob *>(ring.dev[k]), static_cast<uint32_t>(table.size()),
+                           static_cast<uint32_t>(tiles), d->counter, d->spill, threads, d->guard, grid, d->stream))
+        return fail(TRT_E_DEVICE, "render kernel launch failed");
+    TRT_HIP(hipEventRecord(ring.ev[k], d->stream));
     return ok();
+}
+
+int trt_render(trt_scene *s, const void *params, uint32_t first_sample, uint32_t spp, trt_accum *acc) {
+    if (!s) return fail(TRT_E_INVALID, "trt_render: null scene");
+    const trt_render_job j{s, params, first_sample, spp, acc};
+    trt::RenderArgs a;
+    std::string why;
+    int rc = job_args(j, a, &why);
+    if (rc) return fail(rc, "trt_render: " + why);
+    return trt_render_batch(s->dev, &j, 1);
 }
 
 int trt_denoise_atrous(trt_device *d, uint32_t w, uint32_t h, const float *color, const float *albedo,

@@ -771,27 +771,42 @@ __device__ void path(Ctx &c, uint32_t px, uint32_t py, uint32_t index, V &radian
     if (non_finite(normal)) normal = v3(0.0f, 1.0f, 0.0f);
 }
 
-__global__ __launch_bounds__(kRenderBlock) void k_render(RenderArgs a) {
+// Every job of the launch is a run of 8x8 tiles (RenderJob::tile_first on); a wave takes the next tile of the whole
+// launch, finds its job (a binary search over the prefix, the same for the whole wave) and renders it with that
+// job's scene, parameters and accumulator. Jobs of any size share the launch: what one job leaves idle, another fills.
+__global__ __launch_bounds__(kRenderBlock) void k_render(const RenderJob *jobs, uint32_t njobs, uint32_t total,
+                                                         uint32_t *counter, uint32_t *spill, uint32_t spill_stride,
+                                                         uint32_t *guard) {
     __shared__ uint32_t lds[kLdsStack * kRenderBlock];
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    Stack st{lds + threadIdx.x, a.geo.spill + gtid, a.geo.spill_stride, 0};
+    Stack st{lds + threadIdx.x, spill + gtid, spill_stride, 0};
     bool tripped = false;
-    Ctx c{&a, &st, &tripped, Rng{0, 0, 0}, false, false};
-    const uint32_t w = a.params[16], h = a.params[17];
-    const uint32_t tiles_x = (w + 7) / 8, tiles = tiles_x * ((h + 7) / 8);
     for (;;) {
         uint32_t tile = 0;
-        if (lane == 0) tile = atomicAdd(a.geo.counter, 1u);
-        tile = __shfl(tile, 0, 64);
-        if (tile >= tiles) break;
-        const uint32_t px = (tile % tiles_x) * 8 + (lane & 7u), py = (tile / tiles_x) * 8 + (lane >> 3);
+        if (lane == 0) tile = atomicAdd(counter, 1u);
+        tile = __builtin_amdgcn_readfirstlane(__shfl(tile, 0, 64));
+        if (tile >= total) break;
+        uint32_t lo = 0, hi = njobs;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (jobs[mid].tile_first <= tile)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        const RenderJob *job = jobs + lo;
+        const RenderArgs *a = &job->args;
+        const uint32_t t = tile - job->tile_first;
+        const uint32_t w = a->params[16], h = a->params[17], tiles_x = (w + 7) / 8;
+        Ctx c{a, &st, &tripped, Rng{0, 0, 0}, false, false};
+        const uint32_t px = (t % tiles_x) * 8 + (lane & 7u), py = (t / tiles_x) * 8 + (lane >> 3);
         if (px < w && py < h) {
-            float4 *pix = a.accum + 5 * (uint64_t(py) * w + px);
+            float4 *pix = a->accum + 5 * (uint64_t(py) * w + px);
             float4 col = pix[0], alb = pix[1], nrm = pix[2];
-            for (uint32_t s = 0; s < a.spp; s++) {
+            for (uint32_t s = 0; s < a->spp; s++) {
                 V rad, al, nm;
-                path(c, px, py, a.first_sample + s, rad, al, nm);
+                path(c, px, py, a->first_sample + s, rad, al, nm);
                 const float lum = luminance(rad);
                 col.x += rad.x, col.y += rad.y, col.z += rad.z, col.w += 1.0f;
                 alb.x += al.x, alb.y += al.y, alb.z += al.z, alb.w += lum;
@@ -802,7 +817,7 @@ __global__ __launch_bounds__(kRenderBlock) void k_render(RenderArgs a) {
             pix[2] = nrm;
         }
     }
-    if (tripped) atomicOr(a.geo.guard_flag, 1u);
+    if (tripped) atomicOr(guard, 1u);
 }
 
 __global__ void k_reset(uint32_t *c) { *c = 0; }
@@ -816,9 +831,11 @@ int render_occupancy(int *blocks_per_cu) {
     return e == hipSuccess ? 0 : -1;
 }
 
-int launch_render(const RenderArgs &a, int grid, hipStream_t s) {
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, s, a.geo.counter);
-    hipLaunchKernelGGL(k_render, dim3(grid), dim3(kRenderBlock), 0, s, a);
+int launch_render(const RenderJob *d_jobs, uint32_t njobs, uint32_t total_tiles, uint32_t *counter,
+                  uint32_t *spill, uint32_t spill_stride, uint32_t *guard, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, s, counter);
+    hipLaunchKernelGGL(k_render, dim3(grid), dim3(kRenderBlock), 0, s, d_jobs, njobs, total_tiles, counter, spill,
+                       spill_stride, guard);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -30,7 +30,16 @@ struct RenderArgs {
     uint32_t params[kParamWords];
 };
 
+// One job of a launch: its arguments and where its 8x8 tiles start among the launch's.
+struct RenderJob {
+    RenderArgs args;
+    uint32_t tile_first;
+    uint32_t pad[3];
+};
+
 int render_occupancy(int *blocks_per_cu);
-int launch_render(const RenderArgs &a, int grid, hipStream_t s);
+// d_jobs: njobs jobs in device memory, tile_first ascending from 0; total_tiles: all their tiles.
+int launch_render(const RenderJob *d_jobs, uint32_t njobs, uint32_t total_tiles, uint32_t *counter,
+                  uint32_t *spill, uint32_t spill_stride, uint32_t *guard, int grid, hipStream_t s);
 
 }  // namespace trt

@@ -223,6 +223,7 @@ def build(meshes: list[RMesh], instances: list[RInstance], materials: list[np.nd
     indices = np.concatenate(idx_parts) if idx_parts else np.zeros((0, 3), np.uint32)
     # rows, emitters
     rows, em_tris, entries = [], [], []
+    n_em_so_far = 0
     lum = lambda e: 0.2126 * e[0] + 0.7152 * e[1] + 0.0722 * e[2]  # noqa: E731
     for r, inst in enumerate(instances):
         m = meshes[inst.mesh]
@@ -242,52 +243,50 @@ def build(meshes: list[RMesh], instances: list[RInstance], materials: list[np.nd
             mi = inst.materials[s] if s < len(inst.materials) else inst.materials[-1]
             first_em = NO_EMITTER
             if lum(mats[mi][4:7]) > 0.0:
-                first_em = len(em_tris)
+                first_em = n_em_so_far
                 members = np.nonzero(sec == s)[0]  # in the section's order (its rank)
-                for t in members:
-                    pw = (w[:3, :3] @ tris64[t].T).T + w[:3, 3]
-                    p32 = pw.astype(np.float32)
-                    e1 = p32[1] - p32[0]
-                    e2 = p32[2] - p32[0]
-                    area = 0.5 * float(np.linalg.norm(np.cross(pw[1] - pw[0], pw[2] - pw[0])))
-                    code = len(em_tris)
-                    rec_em = np.zeros(16, np.uint32)
-                    fl = rec_em.view(np.float32)
-                    fl[0:3] = p32[0]
-                    rec_em[3] = mesh_first + pos_of[t]
-                    fl[4:7] = e1
-                    rec_em[7] = r
-                    fl[8:11] = e2
-                    rec_em[11] = code
-                    rec_em[12] = s
-                    em_tris.append(rec_em)
-                    entries.append((code, np.float32(area) * np.float32(lum(mats[mi][4:7])), np.float32(area)))
+                pw = np.einsum("ij,tcj->tci", w[:3, :3], tris64[members]) + w[:3, 3]
+                p32 = pw.astype(np.float32)
+                area = 0.5 * np.linalg.norm(np.cross(pw[:, 1] - pw[:, 0], pw[:, 2] - pw[:, 0]), axis=1)
+                rec_em = np.zeros((len(members), 16), np.uint32)
+                fl = rec_em.view(np.float32)
+                fl[:, 0:3] = p32[:, 0]
+                rec_em[:, 3] = mesh_first + pos_of[members]
+                fl[:, 4:7] = p32[:, 1] - p32[:, 0]
+                rec_em[:, 7] = r
+                fl[:, 8:11] = p32[:, 2] - p32[:, 0]
+                rec_em[:, 11] = np.arange(first_em, first_em + len(members), dtype=np.uint32)
+                rec_em[:, 12] = s
+                em_tris.append(rec_em)
+                a32 = area.astype(np.float32)
+                entries.append((a32 * np.float32(lum(mats[mi][4:7])), a32))
+                n_em_so_far += len(members)
             rows.append((mi, 0, first_em, 0))
     rows = np.array(rows, np.uint32).reshape(-1, 4)
-    # lib.rs:451-481 emitter_cdf, f32 sums in list order
-    n_em = len(entries)
+    # lib.rs:451-481 emitter_cdf: f32 running sum in list order (np.cumsum accumulates in order, in f32)
+    n_em = n_em_so_far
     cdf = np.zeros((n_em, 4), np.float32)
-    total = np.float32(0.0)
-    for k, (code, wgt, area) in enumerate(entries):
-        total = np.float32(total + np.float32(wgt))
-        cdf[k] = (np.uint32(k).view(np.float32), total, wgt, area)
+    if n_em:
+        wgt = np.concatenate([e[0] for e in entries]).astype(np.float32)
+        cdf[:, 0] = np.arange(n_em, dtype=np.uint32).view(np.float32)
+        cdf[:, 1] = np.cumsum(wgt, dtype=np.float32)
+        cdf[:, 2] = wgt
+        cdf[:, 3] = np.concatenate([e[1] for e in entries])
+        total = np.float32(cdf[-1, 1])
+    else:
+        total = np.float32(0.0)
     if total > 0:
         cdf[:, 1] = (cdf[:, 1] / total).astype(np.float32)
-    guide = np.zeros(EMITTER_GUIDE + 1, np.uint32)
-    i = 0
-    cum = cdf[:, 1]
-    for kk in range(EMITTER_GUIDE + 1):
-        u = np.float32(kk) / np.float32(EMITTER_GUIDE)
-        while i < n_em and cum[i] < u:
-            i += 1
-        guide[kk] = i
+    # the guide: for u = k / 65536, the first entry whose cumulative weight is not below u
+    us = np.arange(EMITTER_GUIDE + 1, dtype=np.float32) / np.float32(EMITTER_GUIDE)
+    guide = np.searchsorted(cdf[:, 1], us, side="left").astype(np.uint32)
     g4 = np.zeros(((EMITTER_GUIDE + 1 + 3) // 4) * 4, np.uint32)
     g4[:EMITTER_GUIDE + 1] = guide
     header = np.array([[total, 0, 0, 0]], np.float32)
     emitters = np.concatenate([cdf, g4.view(np.float32).reshape(-1, 4), header]) if n_em else np.zeros((0, 4),
                                                                                                          np.float32)
-    emitter_tris = np.array(em_tris, np.uint32).view(np.float32).reshape(-1, 16) if em_tris else np.zeros((0, 16),
-                                                                                                            np.float32)
+    emitter_tris = (np.concatenate(em_tris).view(np.float32).reshape(-1, 16) if em_tris
+                    else np.zeros((0, 16), np.float32))
     objects = np.zeros((OBJECT_STRIDE, 4), np.float32)
     objects[0, 3] = 1.0
     lights = np.stack(lights).astype(np.float32) if len(lights) else np.zeros((0, 20), np.float32)
