@@ -1,0 +1,292 @@
+"""A render server with continuous batching (the serving counterpart of the reference's `serve` loop,
+main.rs:345-398, which renders one JSON job at a time, rebuilding and re-uploading its scene each time).
+
+Jobs arrive at any time. Each is admitted when the HBM budget has room for its accumulator (and its scene, if that
+scene is not resident yet); admitted jobs share every launch (`trt_render_batch`): each launch carries a slice of
+samples of every active job, so a job that arrives while others render joins at the next launch instead of waiting
+for them to finish. Scenes stay resident in HBM keyed by their description and are evicted least-recently-used
+when the budget needs their room; accumulators live from admission to the job's last sample. A job's image is the
+one a lone render of it gives, bit for bit: each pixel adds its samples in sample order whatever else shares the
+launch.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import heapq
+import threading
+import time
+from collections import OrderedDict, deque
+from concurrent.futures import ThreadPoolExecutor
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import Device, _check, _ptr, lib
+from . import render as R
+
+
+class _Job(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("params", C.c_void_p), ("first_sample", C.c_uint32), ("spp", C.c_uint32),
+                ("acc", C.c_void_p)]
+
+
+def _sig():
+    L = lib()
+    L.trt_render_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32]
+    L.trt_render_batch.restype = C.c_int
+
+
+_sig()
+
+
+def render_batch(dev: Device, jobs: list[tuple]):
+    """jobs: (Scene, params (156 u32), first_sample, spp, Accum) each; one launch, queued on the stream."""
+    arr = (_Job * len(jobs))()
+    keep = []
+    for k, (sc, p, first, spp, acc) in enumerate(jobs):
+        p = np.ascontiguousarray(p, np.uint32)
+        keep.append(p)
+        arr[k] = _Job(sc.h.value if isinstance(sc.h, C.c_void_p) else sc.h, p.ctypes.data, first, spp,
+                      acc.handle.value if isinstance(acc.handle, C.c_void_p) else acc.handle)
+    _check(lib().trt_render_batch(dev.h, arr, len(jobs)))
+
+
+@dataclass
+class Job:
+    id: int
+    scene: tuple          # the scene's description (a key for the store; `make_scene(*scene)` builds it)
+    width: int
+    height: int
+    spp: int
+    bounces: int
+    arrival: float = 0.0  # seconds from the start of the run
+    seed: int = 1
+
+
+@dataclass
+class Result:
+    id: int
+    arrival: float
+    admitted: float
+    finished: float
+    samples: int
+    image: np.ndarray | None = None
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+
+@dataclass
+class _Active:
+    job: Job
+    entry: "_Resident"
+    acc: R.Accum
+    params: np.ndarray
+    done: int = 0
+    admitted: float = 0.0
+
+
+@dataclass
+class _Resident:
+    scene: object
+    rs: R.RenderScene
+    cam: R.Camera
+    sky: tuple
+    bytes: int
+    users: int = 0
+
+
+class SceneStore:
+    """Scenes resident in HBM by key, least recently used first out; host builds on a worker pool."""
+
+    def __init__(self, dev: Device, make_scene, workers: int = 4):
+        self.dev = dev
+        self.make_scene = make_scene
+        self.resident: "OrderedDict[tuple, _Resident]" = OrderedDict()
+        self.host: dict = {}
+        self.pool = ThreadPoolExecutor(max_workers=workers)
+        self.lock = threading.Lock()
+
+    def prefetch(self, key):
+        with self.lock:
+            if key not in self.host:
+                self.host[key] = self.pool.submit(self.make_scene, *key)
+
+    def ready(self, key) -> bool:
+        return key in self.resident or (key in self.host and self.host[key].done())
+
+    def bytes_of(self, key) -> int:
+        if key in self.resident:
+            return 0
+        rs = self.host[key].result()[0]
+        return scene_bytes(rs)
+
+    def acquire(self, key) -> _Resident:
+        if key in self.resident:
+            e = self.resident.pop(key)
+            self.resident[key] = e
+        else:
+            rs, cam, sky = self.host[key].result()
+            sc = R.upload(self.dev, rs)
+            e = _Resident(sc, rs, cam, sky, scene_bytes(rs))
+            self.resident[key] = e
+        e.users += 1
+        return e
+
+    def release(self, e: _Resident):
+        e.users -= 1
+
+    def evict_for(self, need: int, used: int, budget: int) -> int:
+        """Frees idle scenes, least recently used first, until `need` more bytes fit; returns bytes freed."""
+        freed = 0
+        for key in list(self.resident):
+            if used - freed + need <= budget:
+                break
+            e = self.resident[key]
+            if e.users == 0:
+                e.scene.close()
+                freed += e.bytes
+                del self.resident[key]
+        return freed
+
+    def resident_bytes(self) -> int:
+        return sum(e.bytes for e in self.resident.values())
+
+
+def scene_bytes(rs: R.RenderScene) -> int:
+    p = rs.prep
+    b = p.nodes.nbytes + p.leaves.nbytes + p.instances.nbytes
+    b += sum(a.nbytes for a in rs.tables().values())
+    return int(b + p.nodes.nbytes * 2)  # the wide tree is at most twice the binary buffer
+
+
+class Server:
+    """Continuous batching over one device. `run(jobs)` replays jobs at their arrival times (real time) and returns
+    one Result per job."""
+
+    def __init__(self, dev: Device, make_scene, budget_bytes: int = 200 << 30, launch_samples: float = 4e7,
+                 keep_images: bool = False, max_jobs: int = 256):
+        self.dev = dev
+        self.store = SceneStore(dev, make_scene)
+        self.budget = budget_bytes
+        self.launch_samples = launch_samples  # path samples one launch aims at (all jobs together)
+        self.keep_images = keep_images
+        self.max_jobs = max_jobs
+        self.launches = 0
+        self.samples = 0
+
+    def _slices(self, active: list[_Active]) -> list[int]:
+        """Samples each active job takes this launch: an equal share of the launch's samples, at least one."""
+        share = self.launch_samples / max(len(active), 1)
+        out = []
+        for a in active:
+            px = a.job.width * a.job.height
+            k = int(max(1, share // px))
+            out.append(min(k, a.job.spp - a.done))
+        return out
+
+    def run(self, jobs: list[Job], prefetch: bool = True) -> list[Result]:
+        jobs = sorted(jobs, key=lambda j: j.arrival)
+        if prefetch:
+            for j in jobs:
+                self.store.prefetch(j.scene)
+        pending = deque(jobs)
+        waiting: deque[Job] = deque()
+        active: list[_Active] = []
+        results: list[Result] = []
+        used = 0
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0  # noqa: E731
+        while pending or waiting or active:
+            t = now()
+            while pending and pending[0].arrival <= t:
+                j = pending.popleft()
+                self.store.prefetch(j.scene)
+                waiting.append(j)
+            # admission: first come, first served, while the budget has room
+            while waiting and len(active) < self.max_jobs:
+                j = waiting[0]
+                if not self.store.ready(j.scene):
+                    break
+                need = j.width * j.height * 80 + self.store.bytes_of(j.scene)
+                if used + need > self.budget:
+                    used -= self.store.evict_for(need, used, self.budget)
+                    if used + need > self.budget:
+                        break
+                waiting.popleft()
+                e = self.store.acquire(j.scene)
+                used += need
+                acc = R.Accum(self.dev, j.width, j.height)
+                p = e.rs.params(j.width, j.height, e.cam, seed=j.seed, max_bounces=j.bounces, rr_depth=3, sky=e.sky)
+                active.append(_Active(j, e, acc, p, 0, now()))
+            if not active:
+                if pending:
+                    time.sleep(max(0.0, min(pending[0].arrival - now(), 0.002)))
+                else:
+                    time.sleep(0.001)
+                continue
+            ks = self._slices(active)
+            render_batch(self.dev, [(a.entry.scene, a.params, a.done, k, a.acc) for a, k in zip(active, ks)])
+            self.dev.sync()
+            self.launches += 1
+            still = []
+            for a, k in zip(active, ks):
+                a.done += k
+                self.samples += k * a.job.width * a.job.height
+                if a.done >= a.job.spp:
+                    img = a.acc.resolve()[0] if self.keep_images else None
+                    results.append(Result(a.job.id, a.job.arrival, a.admitted, now(), a.job.spp * a.job.width * a.job.height,
+                                          img))
+                    a.acc.close()
+                    used -= a.job.width * a.job.height * 80
+                    self.store.release(a.entry)
+                else:
+                    still.append(a)
+            active = still
+        return sorted(results, key=lambda r: r.id)
+
+
+# ---- the serving workload ----------------------------------------------------------------------------------------
+def workload(n_jobs: int = 512, rate: float = 40.0, seed: int = 1, n_scenes: int = 24, min_tris: int = 10_000,
+             max_tris: int = 4_000_000, scale: float = 1.0) -> list[Job]:
+    """Poisson arrivals at `rate` jobs/s; scenes seeded procedural, log-uniform in [min_tris, max_tris] triangles
+    (`n_scenes` distinct ones, each job picks one); resolution from 256x256 to 1920x1080 and spp from 1 to 256,
+    log-uniform; bounces 2-6. `scale` shrinks resolution and spp for quick runs."""
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(min_tris), np.log(max_tris), n_scenes)).astype(int)
+    gaps = rng.exponential(1.0 / rate, n_jobs)
+    t = np.cumsum(gaps) - gaps[0]
+    jobs = []
+    for k in range(n_jobs):
+        s = int(rng.integers(0, n_scenes))
+        a = np.exp(rng.uniform(0, 1))  # 256x256 .. 1920x1080 along a log-uniform path
+        f = rng.uniform(0, 1)
+        w = int(round(np.exp(np.log(256) + f * (np.log(1920) - np.log(256))) * scale))
+        h = int(round(np.exp(np.log(256) + f * (np.log(1080) - np.log(256))) * scale))
+        spp = max(1, int(round(np.exp(rng.uniform(0, np.log(256))) * scale)))
+        jobs.append(Job(k, (int(sizes[s]), 1000 + s), max(8, w), max(8, h), spp, int(rng.integers(2, 7)),
+                        float(t[k]), seed=k + 1))
+        del a
+    return jobs
+
+
+def soup_scene(n_tris: int, seed: int):
+    """A seeded procedural scene of `n_tris` clustered triangles (the precision scene's generator), 5 % emissive,
+    Lambert/GGX materials in four sections, under a sun and a constant sky; a camera outside the clusters' box."""
+    from . import scene as S
+
+    tris, emissive = S.clustered_soup(n_tris, seed, extent=100.0, median_area=0.02)
+    rng = np.random.default_rng(seed)
+    sections = np.where(emissive > 0, 3, rng.integers(0, 3, n_tris)).astype(np.uint32)
+    mats = [R.material((0.7, 0.7, 0.7), roughness=0.9), R.material((0.8, 0.5, 0.3), metallic=1.0, roughness=0.3),
+            R.material((0.3, 0.6, 0.8), roughness=0.5, specular=0.5),
+            R.material((0, 0, 0), (float(np.mean(emissive[emissive > 0])) if (emissive > 0).any() else 10.0,) * 3)]
+    rs = R.build([R.RMesh(tris, sections=sections)], [R.RInstance(0, np.eye(4, dtype=np.float32), [0, 1, 2, 3])],
+                 mats, [R.directional_light((-0.3, -1.0, -0.2), (3.0, 3.0, 3.0))])
+    cam = R.Camera((0.0, 40.0, -160.0), (0.0, 0.0, 0.0), 50.0)
+    return rs, cam, (0.3, 0.4, 0.6)
+
+
+def percentile(xs, q):
+    return float(np.percentile(np.asarray(xs, np.float64), q)) if len(xs) else float("nan")
