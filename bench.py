@@ -86,6 +86,97 @@ def bench_render(dev, frames: int) -> dict:
             "unique_triangles": int(len(rs.prep.leaves))}
 
 
+def bench_cornell(dev) -> dict:
+    """Samples/s on the seeded Cornell box (36 triangles): 512 x 512, 16 spp, 4 bounces, RR from depth 3, clamp
+    off, seed 1, one launch for the 16 samples, timed with HIP events."""
+    from threert_amd import render as R
+
+    rs, cam = R.cornell()
+    w = h = 512
+    spp = 16
+    p = rs.params(w, h, cam, seed=1, max_bounces=4, rr_depth=3)
+    sc = R.upload(dev, rs)
+    acc = R.Accum(dev, w, h)
+    R.render(sc, p, 0, spp, acc)  # warm
+    dev.sync()
+    acc.clear()
+    reps = 5
+    dev.timer_start()
+    for _ in range(reps):
+        R.render(sc, p, 0, spp, acc)
+    ms = dev.timer_stop() / reps
+    sc.close()
+    acc.close()
+    sps = w * h * spp / (ms * 1e-3)
+    log(f"bench: cornell 512x512x16 {ms:.2f} ms, {sps / 1e6:.1f} Msamples/s")
+    return {"metric": "samples/s", "value": round(sps, 1), "unit": "samples/s", "ms_per_image": round(ms, 3),
+            "workload": "seeded Cornell box, 36 triangles, 512x512, 16 spp, 4 bounces, RR from depth 3, seed 1"}
+
+
+def bench_precision_render(dev) -> dict:
+    """Samples/s on the 3M-triangle clustered scene (5 % emissive, sun + sky): 1920 x 1080, 64 spp in 8 launches
+    of 8 samples, 4 bounces, RR from depth 3."""
+    from threert_amd import render as R
+    from threert_amd import serve
+
+    log("bench: building the 3M-triangle render scene")
+    rs, cam, sky = serve.soup_scene(N_TRIS, 1)
+    w, h, spp, per = 1920, 1080, 64, 8
+    p = rs.params(w, h, cam, seed=1, max_bounces=4, rr_depth=3, sky=sky)
+    sc = R.upload(dev, rs)
+    acc = R.Accum(dev, w, h)
+    R.render(sc, p, 0, 1, acc)  # warm
+    dev.sync()
+    acc.clear()
+    dev.timer_start()
+    for s in range(0, spp, per):
+        R.render(sc, p, s, per, acc)
+    ms = dev.timer_stop()
+    sc.close()
+    acc.close()
+    sps = w * h * spp / (ms * 1e-3)
+    log(f"bench: precision 1080p x 64 spp {ms:.1f} ms, {sps / 1e6:.1f} Msamples/s")
+    return {"metric": "samples/s", "value": round(sps, 1), "unit": "samples/s", "ms_per_image": round(ms, 2),
+            "workload": "3M clustered triangles (5 % emissive, 150k emitter triangles), sun + constant sky, "
+                        "1920x1080, 64 spp (8 launches of 8), 4 bounces, RR from depth 3"}
+
+
+def bench_copy(dev) -> dict:
+    """The library's plain copy kernel over 1 GiB: the HBM bandwidth a streaming kernel reaches, and the
+    yardstick for the HBM byte counters."""
+    n = 1 << 30
+    a, b = dev.alloc(n), dev.alloc(n)
+    b.copy_from(a, n)  # warm
+    dev.timer_start()
+    for _ in range(4):
+        b.copy_from(a, n)
+    ms = dev.timer_stop() / 4
+    a.free()
+    b.free()
+    gbs = 2 * n / (ms * 1e-3) / 1e9
+    log(f"bench: copy 1 GiB {ms:.3f} ms, {gbs:.0f} GB/s read+write")
+    return {"bytes": n, "ms": round(ms, 4), "gbs_read_plus_write": round(gbs, 1)}
+
+
+def bench_serve(dev, n_jobs: int, rate: float) -> dict:
+    """The serving workload: `n_jobs` jobs, Poisson arrivals at `rate` per second, seeded clustered scenes of 10k
+    to 4M triangles (log-uniform, 24 of them), 256x256 to 1920x1080, 1 to 256 spp, 2 to 6 bounces; continuous
+    batching under a 200 GB HBM budget; scenes built on host workers when their first job arrives."""
+    from threert_amd import serve
+
+    jobs = serve.workload(n_jobs=n_jobs, rate=rate, seed=1)
+    srv = serve.Server(dev, serve.soup_scene, budget_bytes=200 << 30)
+    t0 = time.perf_counter()
+    res = srv.run(jobs, prefetch=False,
+                  progress=lambda d, a, w: log(f"bench: serving {d} done, {a} rendering, {w} waiting"))
+    wall = time.perf_counter() - t0
+    srv.close()
+    s = serve.summary(res, wall)
+    s["launches"] = srv.launches
+    log(f"bench: serving {s}")
+    return s
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,10 +186,23 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-render", action="store_true")
     ap.add_argument("--render-frames", type=int, default=10)
+    ap.add_argument("--serve", action="store_true", help="measure the serving workload instead (p50/p99 latency)")
+    ap.add_argument("--serve-jobs", type=int, default=512)
+    ap.add_argument("--serve-rate", type=float, default=40.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus})", file=sys.stderr)
         sys.exit(2)
+    if args.serve:
+        dev = T.Device(0)
+        s = bench_serve(dev, args.serve_jobs, args.serve_rate)
+        print(json.dumps({"metric": "p99 job latency (serving)", "value": s["p99_s"], "unit": "s", "n_gpus": 1,
+                          "higher_is_better": False, "dtype": "f32", "data": "synthetic",
+                          "config": {"workload": f"{args.serve_jobs} render jobs, Poisson arrivals at "
+                                                 f"{args.serve_rate}/s, seeded scenes of 10k-4M triangles, "
+                                                 "256x256-1920x1080, 1-256 spp, 2-6 bounces, 200 GB budget"},
+                          "serving": s}), flush=True)
+        return
 
     log("bench: building the 3M-triangle scene")
     t0 = time.perf_counter()
@@ -167,8 +271,11 @@ def main():
     sc.close()
     d_rays.free()
     d_hits.free()
+    out["config"]["copy_1gib"] = bench_copy(dev)
     if not args.no_render:
         out["render_720p"] = bench_render(dev, args.render_frames)
+        out["render_cornell_512"] = bench_cornell(dev)
+        out["render_1080p_precision"] = bench_precision_render(dev)
     if not args.no_cpu:
         log("bench: CPU baseline")
         out["cpu_baseline"] = cpu_baseline(prep, rays, min(args.cpu_sample, N_RAYS))

@@ -65,6 +65,9 @@ int trt_mem_alloc(trt_device *dev, uint64_t bytes, void **dptr);
 int trt_mem_free(trt_device *dev, void *dptr);
 int trt_mem_upload(trt_device *dev, void *dst, const void *src, uint64_t bytes);
 int trt_mem_download(trt_device *dev, void *dst, const void *src, uint64_t bytes);
+/* Device-to-device copy queued on the device's stream (the reference's encoder.copy_buffer_to_buffer role); dst,
+ * src and bytes multiples of 16. Its kernel is a plain streaming copy, the yardstick for HBM byte counters. */
+int trt_mem_copy(trt_device *dev, void *dst, const void *src, uint64_t bytes);
 
 /* HIP-event timer on the device's stream, the stream every kernel of the library runs on. */
 int trt_timer_start(trt_device *dev);

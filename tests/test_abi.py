@@ -1,8 +1,10 @@
 """The C ABI: libthreert.so loads without a GPU and exports every entry point include/threert.h declares; the
-oracle library loads too. No compute call is made here."""
+oracle library loads too. No compute call is made here, except in the GPU-marked copy test."""
 import ctypes
 import os
 import re
+
+import pytest
 
 import oracle
 import threert_amd as T
@@ -47,3 +49,20 @@ def test_errors_come_back_as_codes_and_messages():
     with pytest.raises(T.ThreeRTError) as e:
         T.mesh_tree(np.zeros((0, 3, 3), np.float32))
     assert e.value.code == T.TRT_E_INVALID and "without triangles" in str(e.value)
+
+
+@pytest.mark.gpu
+def test_device_copy_moves_the_bytes(device):
+    import numpy as np
+
+    a = np.arange(1 << 20, dtype=np.uint32) * np.uint32(2654435761)
+    src, dst = device.alloc(a.nbytes), device.alloc(a.nbytes)
+    src.upload(a)
+    dst.copy_from(src, a.nbytes - 64)
+    device.sync()
+    got = dst.download(np.uint32, len(a) - 16)
+    assert (got == a[:-16]).all()
+    with pytest.raises(T.ThreeRTError):
+        dst.copy_from(src, 24)  # not a multiple of 16
+    src.free()
+    dst.free()

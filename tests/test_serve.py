@@ -1,0 +1,104 @@
+"""Continuous batching: a job's image is the same alone and in a mix (bit for bit), and the serving workload's
+shape (512 Poisson jobs, log-uniform scenes, resolutions and samples).
+
+The reference's `serve` loop (main.rs:345-398) renders one job at a time, so it has no mixed result to compare
+with; the bar is the lone render of the same job through trt_render, which the render tests hold to the oracle."""
+import numpy as np
+import pytest
+
+from threert_amd import serve
+
+
+def test_the_workload_is_seeded_and_in_range():
+    a = serve.workload(n_jobs=512, rate=40.0, seed=1)
+    b = serve.workload(n_jobs=512, rate=40.0, seed=1)
+    assert [(j.scene, j.width, j.height, j.spp, j.bounces, j.arrival) for j in a] == \
+           [(j.scene, j.width, j.height, j.spp, j.bounces, j.arrival) for j in b]
+    t = np.array([j.arrival for j in a])
+    assert len(a) == 512 and t[0] == 0.0 and (np.diff(t) >= 0).all()
+    # Poisson at 40/s: 511 gaps of mean 1/40 s, so the span is 12.8 s within a few standard deviations
+    assert abs(t[-1] - 511 / 40) < 4 * np.sqrt(511) / 40
+    assert all(256 <= j.width <= 1920 and 256 <= j.height <= 1080 for j in a)
+    assert all(1 <= j.spp <= 256 and 2 <= j.bounces <= 6 for j in a)
+    sizes = {j.scene[0] for j in a}
+    assert min(sizes) >= 10_000 and max(sizes) <= 4_000_000 and len({j.scene for j in a}) <= 24
+
+
+def test_equal_shares_and_the_last_slice():
+    class A:
+        def __init__(self, w, h, spp, done):
+            self.job = serve.Job(0, (1, 1), w, h, spp, 2)
+            self.done = done
+
+    s = serve.Server.__new__(serve.Server)
+    s.launch_samples = 1000.0
+    # two jobs share 1000 samples: 500 each; a 10x10 job takes 5 samples, a 100x100 job at least one
+    assert s._slices([A(10, 10, 64, 0), A(100, 100, 64, 0)]) == [5, 1]
+    # never more than what is left
+    assert s._slices([A(10, 10, 64, 62)]) == [2]
+
+
+def test_percentiles_and_summary():
+    r = [serve.Result(k, 0.0, 0.0, float(k + 1), 10) for k in range(100)]
+    s = serve.summary(r, 10.0)
+    assert s["p50_s"] == pytest.approx(50.5) and s["p99_s"] == pytest.approx(99.01)
+    assert s["samples"] == 1000 and s["samples_per_s"] == 100.0
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+@pytest.mark.gpu
+def test_a_job_renders_the_same_alone_and_in_a_batch(device):
+    from threert_amd import render as R
+
+    print("batch of 3 jobs over 2 scenes vs each alone", flush=True)
+    rs1, cam1 = R.cornell()
+    rs2, cam2, sky2 = serve.soup_scene(4000, 5)
+    s1, s2 = R.upload(device, rs1), R.upload(device, rs2)
+    jobs = [(s1, rs1.params(40, 24, cam1, seed=3, max_bounces=4, rr_depth=3), 0, 3),
+            (s2, rs2.params(33, 17, cam2, seed=9, max_bounces=2, rr_depth=3, sky=sky2), 0, 2),
+            (s1, rs1.params(16, 16, cam1, seed=5, max_bounces=3, rr_depth=2), 4, 5)]
+    alone = []
+    for sc, p, first, spp in jobs:
+        acc = R.Accum(device, int(p[16]), int(p[17]))
+        R.render(sc, p, first, spp, acc)
+        device.sync()
+        alone.append(acc.read())
+        acc.close()
+    accs = [R.Accum(device, int(p[16]), int(p[17])) for _, p, _, _ in jobs]
+    serve.render_batch(device, [(sc, p, first, spp, a) for (sc, p, first, spp), a in zip(jobs, accs)])
+    device.sync()
+    for k, (a, want) in enumerate(zip(accs, alone)):
+        got = a.read()
+        assert (_bits(got) == _bits(want)).all(), f"job {k} differs in the batch"
+        a.close()
+    s1.close()
+    s2.close()
+
+
+@pytest.mark.gpu
+def test_the_server_gives_each_job_its_lone_image(device):
+    from threert_amd import render as R
+
+    jobs = serve.workload(n_jobs=8, rate=400.0, seed=4, n_scenes=3, min_tris=2000, max_tris=20000, scale=0.06)
+    print("server: 8 jobs", [(j.width, j.height, j.spp, j.bounces) for j in jobs], flush=True)
+    srv = serve.Server(device, serve.soup_scene, launch_samples=3000, keep_images=True, workers=2)
+    res = srv.run(jobs)
+    srv.close()
+    assert [r.id for r in res] == [j.id for j in jobs] and srv.launches > 1
+    for j, r in zip(jobs, res):
+        rs, cam, sky = serve.soup_scene(*j.scene)
+        e = serve._Resident(None, rs, cam, sky, 0)
+        sc = R.upload(device, rs)
+        acc = R.Accum(device, j.width, j.height)
+        R.render(sc, serve.params_of(e, j), 0, j.spp, acc)
+        device.sync()
+        want = acc.resolve()[0]
+        assert (_bits(r.image) == _bits(want)).all(), f"job {j.id} differs from its lone render"
+        assert r.finished >= r.admitted >= r.arrival
+        acc.close()
+        sc.close()
+    s = serve.summary(res, max(r.finished for r in res))
+    assert s["jobs"] == 8 and s["samples"] == sum(j.width * j.height * j.spp for j in jobs)

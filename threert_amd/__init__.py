@@ -41,7 +41,7 @@ EXPORTS = [
     "trt_unit_box", "trt_unit_record", "trt_tree_free", "trt_scene_upload", "trt_scene_release",
     "trt_scene_stats_get", "trt_trace_rays", "trt_trace_rays_device", "trt_trace_reads",
     "trt_accum_create", "trt_accum_release", "trt_accum_clear", "trt_accum_read", "trt_accum_resolve", "trt_render",
-    "trt_denoise_atrous", "trt_render_batch",
+    "trt_denoise_atrous", "trt_render_batch", "trt_mem_copy",
 ]
 
 
@@ -96,7 +96,7 @@ def lib():
             "trt_device_create": [I, C.POINTER(C.c_void_p)], "trt_device_release": [P],
             "trt_device_info_get": [P, P], "trt_device_sync": [P],
             "trt_mem_alloc": [P, U64, C.POINTER(C.c_void_p)], "trt_mem_free": [P, P],
-            "trt_mem_upload": [P, P, P, U64], "trt_mem_download": [P, P, P, U64],
+            "trt_mem_upload": [P, P, P, U64], "trt_mem_download": [P, P, P, U64], "trt_mem_copy": [P, P, P, U64],
             "trt_timer_start": [P], "trt_timer_stop": [P, C.POINTER(C.c_float)],
             "trt_build_tris": [P, U64, U32, P], "trt_build_mesh_tree": [P, U64, P],
             "trt_build_sah": [P, U64, U32, P], "trt_build_median": [P, U64, U32, P],
@@ -256,6 +256,11 @@ class DeviceBuffer:
         assert out.nbytes <= self.nbytes
         _check(lib().trt_mem_download(self.dev.h, _ptr(out), self.p, out.nbytes))
         return out
+
+    def copy_from(self, src: "DeviceBuffer", nbytes: int):
+        """Queues a device-to-device copy of `nbytes` (a multiple of 16) from `src`."""
+        assert nbytes <= self.nbytes and nbytes <= src.nbytes
+        _check(lib().trt_mem_copy(self.dev.h, self.p, src.p, nbytes))
 
     def free(self):
         if self.p:

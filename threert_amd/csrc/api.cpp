@@ -200,6 +200,16 @@ int trt_mem_download(trt_device *d, void *dst, const void *src, uint64_t bytes) 
     return ok();
 }
 
+int trt_mem_copy(trt_device *d, void *dst, const void *src, uint64_t bytes) {
+    if (!d || (bytes && (!dst || !src))) return fail(TRT_E_INVALID, "trt_mem_copy: null argument");
+    if ((bytes | reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src)) & 15)
+        return fail(TRT_E_INVALID, "trt_mem_copy: pointers and size must be multiples of 16 bytes");
+    TRT_HIP(hipSetDevice(d->ordinal));
+    if (trt::launch_copy(dst, src, bytes / 16, d->cus * 8, d->stream) != 0)
+        return fail(TRT_E_DEVICE, "trt_mem_copy: launch failed");
+    return ok();
+}
+
 int trt_timer_start(trt_device *d) {
     if (!d) return fail(TRT_E_INVALID, "trt_timer_start: null device");
     TRT_HIP(hipSetDevice(d->ordinal));

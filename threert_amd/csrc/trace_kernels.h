@@ -36,5 +36,7 @@ struct TraceArgs {
 
 int trace_occupancy(bool wide, int *blocks_per_cu);
 int launch_trace(const TraceArgs &a, bool wide, bool count, int grid, hipStream_t s);
+// n16 16-byte words from src to dst, both 16-byte aligned; at most `grid` blocks of 256
+int launch_copy(void *dst, const void *src, uint64_t n16, int grid, hipStream_t s);
 
 }  // namespace trt

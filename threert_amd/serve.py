@@ -153,6 +153,21 @@ class SceneStore:
     def resident_bytes(self) -> int:
         return sum(e.bytes for e in self.resident.values())
 
+    def close(self):
+        for e in self.resident.values():
+            e.scene.close()
+        self.resident.clear()
+        self.host.clear()
+        self.pool.shutdown(wait=True)
+
+
+ACCUM_BYTES = 80  # records.wgsl Pixel: five vec4 per pixel
+
+
+def params_of(e: _Resident, j: Job) -> np.ndarray:
+    """The job's Params: its size, seed and bounces over the scene's camera and sky (RR from depth 3)."""
+    return e.rs.params(j.width, j.height, e.cam, seed=j.seed, max_bounces=j.bounces, rr_depth=3, sky=e.sky)
+
 
 def scene_bytes(rs: R.RenderScene) -> int:
     p = rs.prep
@@ -166,9 +181,9 @@ class Server:
     one Result per job."""
 
     def __init__(self, dev: Device, make_scene, budget_bytes: int = 200 << 30, launch_samples: float = 4e7,
-                 keep_images: bool = False, max_jobs: int = 256):
+                 keep_images: bool = False, max_jobs: int = 256, workers: int = 8):
         self.dev = dev
-        self.store = SceneStore(dev, make_scene)
+        self.store = SceneStore(dev, make_scene, workers)
         self.budget = budget_bytes
         self.launch_samples = launch_samples  # path samples one launch aims at (all jobs together)
         self.keep_images = keep_images
@@ -186,7 +201,9 @@ class Server:
             out.append(min(k, a.job.spp - a.done))
         return out
 
-    def run(self, jobs: list[Job], prefetch: bool = True) -> list[Result]:
+    def run(self, jobs: list[Job], prefetch: bool = True, progress=None, every: float = 5.0) -> list[Result]:
+        """`prefetch` starts every scene's host build at once (else at its first job's arrival); `progress(done,
+        active, waiting)` is called at most every `every` seconds."""
         jobs = sorted(jobs, key=lambda j: j.arrival)
         if prefetch:
             for j in jobs:
@@ -198,28 +215,33 @@ class Server:
         used = 0
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0  # noqa: E731
+        told = 0.0
         while pending or waiting or active:
             t = now()
+            if progress and t - told >= every:
+                told = t
+                progress(len(results), len(active), len(waiting))
             while pending and pending[0].arrival <= t:
                 j = pending.popleft()
                 self.store.prefetch(j.scene)
                 waiting.append(j)
-            # admission: first come, first served, while the budget has room
-            while waiting and len(active) < self.max_jobs:
-                j = waiting[0]
-                if not self.store.ready(j.scene):
+            # admission in arrival order among jobs whose scene is built; the first that does not fit the budget
+            # stops admission, so a large job is not overtaken for ever by small ones
+            for j in list(waiting):
+                if len(active) >= self.max_jobs:
                     break
-                need = j.width * j.height * 80 + self.store.bytes_of(j.scene)
+                if not self.store.ready(j.scene):
+                    continue
+                need = j.width * j.height * ACCUM_BYTES + self.store.bytes_of(j.scene)
                 if used + need > self.budget:
                     used -= self.store.evict_for(need, used, self.budget)
                     if used + need > self.budget:
                         break
-                waiting.popleft()
+                waiting.remove(j)
                 e = self.store.acquire(j.scene)
                 used += need
                 acc = R.Accum(self.dev, j.width, j.height)
-                p = e.rs.params(j.width, j.height, e.cam, seed=j.seed, max_bounces=j.bounces, rr_depth=3, sky=e.sky)
-                active.append(_Active(j, e, acc, p, 0, now()))
+                active.append(_Active(j, e, acc, params_of(e, j), 0, now()))
             if not active:
                 if pending:
                     time.sleep(max(0.0, min(pending[0].arrival - now(), 0.002)))
@@ -239,12 +261,15 @@ class Server:
                     results.append(Result(a.job.id, a.job.arrival, a.admitted, now(), a.job.spp * a.job.width * a.job.height,
                                           img))
                     a.acc.close()
-                    used -= a.job.width * a.job.height * 80
+                    used -= a.job.width * a.job.height * ACCUM_BYTES
                     self.store.release(a.entry)
                 else:
                     still.append(a)
             active = still
         return sorted(results, key=lambda r: r.id)
+
+    def close(self):
+        self.store.close()
 
 
 # ---- the serving workload ----------------------------------------------------------------------------------------
@@ -260,15 +285,23 @@ def workload(n_jobs: int = 512, rate: float = 40.0, seed: int = 1, n_scenes: int
     jobs = []
     for k in range(n_jobs):
         s = int(rng.integers(0, n_scenes))
-        a = np.exp(rng.uniform(0, 1))  # 256x256 .. 1920x1080 along a log-uniform path
-        f = rng.uniform(0, 1)
+        f = rng.uniform(0, 1)  # 256x256 .. 1920x1080, log-uniform along the path between them
         w = int(round(np.exp(np.log(256) + f * (np.log(1920) - np.log(256))) * scale))
         h = int(round(np.exp(np.log(256) + f * (np.log(1080) - np.log(256))) * scale))
         spp = max(1, int(round(np.exp(rng.uniform(0, np.log(256))) * scale)))
         jobs.append(Job(k, (int(sizes[s]), 1000 + s), max(8, w), max(8, h), spp, int(rng.integers(2, 7)),
                         float(t[k]), seed=k + 1))
-        del a
     return jobs
+
+
+def summary(results: list[Result], wall: float) -> dict:
+    """Latency percentiles (arrival to result), queueing (arrival to admission) and aggregate samples/s."""
+    lat = [r.latency for r in results]
+    queue = [r.admitted - r.arrival for r in results]
+    samples = sum(r.samples for r in results)
+    return {"jobs": len(results), "p50_s": round(percentile(lat, 50), 3), "p99_s": round(percentile(lat, 99), 3),
+            "queue_p50_s": round(percentile(queue, 50), 3), "queue_p99_s": round(percentile(queue, 99), 3),
+            "samples": int(samples), "wall_s": round(wall, 3), "samples_per_s": round(samples / wall, 1)}
 
 
 def soup_scene(n_tris: int, seed: int):
