@@ -25,7 +25,9 @@ extern "C" {
 #define TRT_E_INVALID 1 /* bad argument or scene data (the reference's upload/prepare errors)          */
 #define TRT_E_OOM 2     /* device or host allocation failed                                           */
 #define TRT_E_DEVICE 3  /* a HIP call failed                                                          */
-#define TRT_E_GUARD 4   /* a kernel stopped at its iteration guard (gpu.rs:4199-4203 loop-guard trip) */
+#define TRT_E_GUARD 4   /* a kernel stopped a walk and reported instead of reading on: its iteration guard
+                         * (gpu.rs:4199-4203 loop-guard trip), a tree reference outside the scene's buffers or a
+                         * stack past its room, or a shading-table index outside its table (trt_last_error says) */
 
 /* Trace modes (pathtrace.wgsl TRACE_CLOSEST / TRACE_SHADOW / TRACE_ANY). */
 #define TRT_TRACE_CLOSEST 0u

@@ -58,7 +58,7 @@ typedef struct {
 
 /* One sample (params word 18 = P.size.z) of every pixel of rows [y0, y1), accumulated into `accum` (80 bytes a
  * pixel: records.wgsl Pixel). params: the 624-byte Params block (pathtrace.wgsl:96-140). Returns the number of
- * walks that failed (read outside their buffers). */
+ * walks that failed (read outside their buffers) plus the table reads whose index was outside its table. */
 int64_t or_render_sample(const or_render_scene *s, const uint32_t *params, uint32_t y0, uint32_t y1, float *accum,
                          int threads);
 

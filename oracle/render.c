@@ -151,17 +151,33 @@ typedef struct {
     V mis_x, mis_n;
     int sampled_spec;
     int failed;
+    int64_t *oob; /* table reads outside their table (each reads entry 0 instead) */
 } Ctx;
 
-/* ---- tables --------------------------------------------------------------------------------------------------- */
-static inline const or_instance *inst_at(const Ctx *c, uint32_t i) { return &c->s->geo.instances[i]; }
-static inline const float *material_at(const Ctx *c, uint32_t i) { return c->s->materials + 72 * (uint64_t)i; }
-static inline const float *light_at(const Ctx *c, uint32_t i) { return c->s->lights + 20 * (uint64_t)i; }
-static inline const float *emitter_at(const Ctx *c, uint32_t i) { return c->s->emitters + 4 * (uint64_t)i; }
-static inline uint32_t row_word(const Ctx *c, uint32_t first, uint32_t section, uint32_t k) {
-    return c->s->rows[4 * ((uint64_t)first + section) + k];
+/* Every table index is checked: one outside its table counts in *oob and reads entry 0. */
+static inline uint64_t ck(const Ctx *c, uint64_t i, uint64_t n) {
+    if (i < n) return i;
+    (*c->oob)++;
+    return 0;
 }
-static inline float obj_power(const Ctx *c, uint32_t obj) { return c->s->objects[4 * (uint64_t)obj * OBJECT_STRIDE + 3]; }
+
+/* ---- tables --------------------------------------------------------------------------------------------------- */
+static inline const or_instance *inst_at(const Ctx *c, uint32_t i) {
+    return &c->s->geo.instances[ck(c, i, c->s->geo.instance_count)];
+}
+static inline const float *material_at(const Ctx *c, uint32_t i) {
+    return c->s->materials + 72 * ck(c, i, c->s->material_count);
+}
+static inline const float *light_at(const Ctx *c, uint32_t i) { return c->s->lights + 20 * ck(c, i, c->s->light_count); }
+static inline const float *emitter_at(const Ctx *c, uint32_t i) {
+    return c->s->emitters + 4 * ck(c, i, c->s->emitter_vec4s);
+}
+static inline uint32_t row_word(const Ctx *c, uint32_t first, uint32_t section, uint32_t k) {
+    return c->s->rows[4 * ck(c, (uint64_t)first + section, c->s->row_count) + k];
+}
+static inline float obj_power(const Ctx *c, uint32_t obj) {
+    return c->s->objects[4 * ck(c, obj, c->s->object_count) * OBJECT_STRIDE + 3];
+}
 
 typedef struct {
     float n0[4], n1[4], n2[4], uv[4], x[4];
@@ -176,10 +192,10 @@ typedef struct {
 static TriData tri_data(const Ctx *c, uint32_t ti, uint32_t inst, uint32_t part) {
     uint32_t base = 0;
     if (inst != OR_NO_INST) base = inst_at(c, inst)->info[2];
-    const uint32_t *ix = c->s->indices + 3 * (uint64_t)ti;
-    const float *p0 = c->s->vertices + 12 * ((uint64_t)ix[0] + base);
-    const float *p1 = c->s->vertices + 12 * ((uint64_t)ix[1] + base);
-    const float *p2 = c->s->vertices + 12 * ((uint64_t)ix[2] + base);
+    const uint32_t *ix = c->s->indices + 3 * ck(c, ti, c->s->tri_count);
+    const float *p0 = c->s->vertices + 12 * ck(c, (uint64_t)ix[0] + base, c->s->vertex_count);
+    const float *p1 = c->s->vertices + 12 * ck(c, (uint64_t)ix[1] + base, c->s->vertex_count);
+    const float *p2 = c->s->vertices + 12 * ck(c, (uint64_t)ix[2] + base, c->s->vertex_count);
     const float *n0 = p0 + 4, *n1 = p1 + 4, *n2 = p2 + 4;
     TriData t;
     t.v0 = ld3(p0);
@@ -274,7 +290,7 @@ typedef struct {
 } Lobes;
 
 static void energy_node(const Ctx *c, uint32_t i, float *out) {
-    const float *v = c->s->energy + 4 * (uint64_t)(i >> 1);
+    const float *v = c->s->energy + 4 * ck(c, i >> 1, 512);
     out[0] = (i & 1u) ? v[2] : v[0];
     out[1] = (i & 1u) ? v[3] : v[1];
 }
@@ -520,7 +536,7 @@ typedef struct {
     uint32_t tri, inst, code, part, flags;
 } Em;
 static Em emitter_tri_at(const Ctx *c, uint32_t i) {
-    const float *r = c->s->emitter_tris + 16 * (uint64_t)i;
+    const float *r = c->s->emitter_tris + 16 * ck(c, i, c->s->emitter_tri_count);
     Em e;
     e.v0 = ld3(r);
     e.tri = fb(r[3]);
@@ -719,7 +735,7 @@ static float emit_mis(Ctx *c, const or_hit *hit, V d, float pdf) {
     uint32_t rows = inst_at(c, hit->inst)->info[3];
     uint32_t first = row_word(c, rows, hit->part, 2u);
     if (first == NO_EMITTER) return 1.0f;
-    uint32_t rank = fb(c->s->geo.leaves[hit->tri].r[1][3]);
+    uint32_t rank = fb(c->s->geo.leaves[ck(c, hit->tri, c->s->geo.leaf_count)].r[1][3]);
     uint32_t entry = first + rank;
     Em em = emitter_tri_at(c, entry);
     V cr = cross(em.e1, em.e2);
@@ -826,19 +842,20 @@ static void path(Ctx *c, uint32_t px, uint32_t py, float *pix) {
 int64_t or_render_sample(const or_render_scene *s, const uint32_t *params, uint32_t y0, uint32_t y1, float *accum,
                          int threads) {
     uint32_t w = params[4 * SIZE + 0];
-    int64_t failed = 0;
+    int64_t failed = 0, oob = 0;
 #ifdef _OPENMP
     if (threads <= 0) threads = omp_get_max_threads();
-#pragma omp parallel for num_threads(threads) schedule(dynamic, 1) reduction(+ : failed)
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 1) reduction(+ : failed, oob)
 #endif
     for (int64_t y = y0; y < (int64_t)y1; y++) {
         Ctx c;
         memset(&c, 0, sizeof c);
         c.s = s;
         c.P = params;
+        c.oob = &oob;
         for (uint32_t x = 0; x < w; x++) path(&c, x, (uint32_t)y, accum + 20 * ((uint64_t)y * w + x));
         failed += c.failed;
     }
     (void)threads;
-    return failed;
+    return failed + oob;
 }

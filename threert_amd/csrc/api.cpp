@@ -79,7 +79,8 @@ struct trt_scene {
          *emitters = nullptr, *emitter_tris = nullptr, *objects = nullptr, *energy = nullptr;
     bool shading = false;
     uint64_t light_count = 0, emitter_entries = 0, emitter_tri_count = 0;
-    uint64_t node_count = 0, leaf_count = 0, instance_count = 0;
+    uint64_t vertex_count = 0, tri_count = 0, row_count = 0, material_count = 0, object_count = 0, emitter_vec4s = 0;
+    uint64_t node_count = 0, leaf_count = 0, instance_count = 0, wnode_count = 0, wunit_count = 0;
     uint32_t top_first = 0, units = 0, top_ref = trt::kNone, max_stack = 0;
     float top_box[6] = {0, 0, 0, 0, 0, 0};
     trt_scene_stats stats{};
@@ -168,6 +169,9 @@ int trt_device_sync(trt_device *d) {
     TRT_HIP(hipMemcpy(&g, d->guard, 4, hipMemcpyDeviceToHost));
     if (g) {
         TRT_HIP(hipMemset(d->guard, 0, 4));
+        if (g & trt::kFlagWalk)
+            return fail(TRT_E_GUARD, "a walk met a reference outside its scene's buffers or a stack past its room");
+        if (g & trt::kFlagTable) return fail(TRT_E_GUARD, "a shading-table index was outside its table");
         return fail(TRT_E_GUARD, "a walk stopped at its iteration guard");
     }
     return ok();
@@ -465,8 +469,14 @@ int trt_scene_upload(trt_device *d, const trt_scene_desc *desc, trt_scene **out)
         return fail(TRT_E_INVALID, "trt_scene_upload: a buffer with a count has no pointer");
     if (desc->unit_count > 0 && (desc->node_count == 0 || desc->instance_count == 0))
         return fail(TRT_E_INVALID, "trt_scene_upload: units without nodes or records");
-    if (desc->instance_count >= (uint64_t{1} << 30) || desc->leaf_count >= (uint64_t{1} << 32))
-        return fail(TRT_E_INVALID, "trt_scene_upload: too many records or leaves");
+    if (desc->instance_count >= (uint64_t{1} << 30) || desc->leaf_count >= (uint64_t{1} << 32) ||
+        desc->node_count >= (uint64_t{1} << 32))
+        return fail(TRT_E_INVALID, "trt_scene_upload: too many records, leaves or nodes");
+    if (desc->energy && (desc->vertex_count >= (uint64_t{1} << 32) || desc->index_count >= (uint64_t{1} << 32) ||
+                         desc->row_count >= (uint64_t{1} << 32) || desc->material_count >= (uint64_t{1} << 32) ||
+                         desc->light_count >= (uint64_t{1} << 32) || desc->emitter_vec4s >= (uint64_t{1} << 32) ||
+                         desc->emitter_tri_count >= (uint64_t{1} << 32) || desc->object_count >= (uint64_t{1} << 32)))
+        return fail(TRT_E_INVALID, "trt_scene_upload: a shading table of 2^32 records or more");
     const auto *nodes = static_cast<const trt::Node *>(desc->nodes);
     const auto *inst = static_cast<const trt::Instance *>(desc->instances);
     trt::WideScene w;
@@ -493,6 +503,9 @@ int trt_scene_upload(trt_device *d, const trt_scene_desc *desc, trt_scene **out)
         s->light_count = desc->light_count;
         s->emitter_tri_count = desc->emitter_tri_count;
         s->emitter_entries = desc->emitter_vec4s ? desc->emitter_vec4s - (65537 / 4 + 1) - 1 : 0;
+        s->vertex_count = desc->vertex_count, s->tri_count = desc->index_count, s->row_count = desc->row_count;
+        s->material_count = desc->material_count, s->object_count = desc->object_count;
+        s->emitter_vec4s = desc->emitter_vec4s;
         rc = upload(d, &s->vertices, desc->vertices, desc->vertex_count * 48, &total);
         if (!rc) rc = upload(d, &s->indices, desc->indices, desc->index_count * 12, &total);
         if (!rc) rc = upload(d, &s->rows, desc->rows, desc->row_count * 16, &total);
@@ -511,6 +524,8 @@ int trt_scene_upload(trt_device *d, const trt_scene_desc *desc, trt_scene **out)
     s->node_count = desc->node_count;
     s->leaf_count = desc->leaf_count;
     s->instance_count = desc->instance_count;
+    s->wnode_count = w.nodes.size();
+    s->wunit_count = w.units.size();
     s->top_first = desc->top_first;
     s->units = desc->unit_count;
     s->top_ref = w.top_ref;
@@ -572,6 +587,12 @@ static int launch(trt_scene *s, const void *d_rays, uint64_t n, void *d_hits, ui
     a.reads = d->reads;
     a.n = n;
     a.spill_stride = threads;
+    a.spill_entries = static_cast<uint32_t>(entries);
+    a.wnode_count = static_cast<uint32_t>(s->wnode_count);
+    a.wunit_count = static_cast<uint32_t>(s->wunit_count);
+    a.bnode_count = static_cast<uint32_t>(s->node_count);
+    a.instance_count = static_cast<uint32_t>(s->instance_count);
+    a.leaf_count = static_cast<uint32_t>(s->leaf_count);
     a.top_ref = s->top_ref;
     a.top_first = s->top_first;
     a.units = s->units;
@@ -698,6 +719,11 @@ static int job_args(const trt_render_job &j, trt::RenderArgs &a, std::string *wh
     a.geo.top_ref = s->top_ref;
     a.geo.top_first = s->top_first;
     a.geo.units = s->units;
+    a.geo.wnode_count = static_cast<uint32_t>(s->wnode_count);
+    a.geo.wunit_count = static_cast<uint32_t>(s->wunit_count);
+    a.geo.bnode_count = static_cast<uint32_t>(s->node_count);
+    a.geo.instance_count = static_cast<uint32_t>(s->instance_count);
+    a.geo.leaf_count = static_cast<uint32_t>(s->leaf_count);
     std::memcpy(a.geo.top_box, s->top_box, sizeof(a.geo.top_box));
     a.instances = static_cast<const float4 *>(s->instances);
     a.vertices = static_cast<const float4 *>(s->vertices);
@@ -710,6 +736,14 @@ static int job_args(const trt_render_job &j, trt::RenderArgs &a, std::string *wh
     a.objects = static_cast<const float4 *>(s->objects);
     a.energy = static_cast<const float4 *>(s->energy);
     a.accum = static_cast<float4 *>(acc->buf);
+    a.vertex_count = static_cast<uint32_t>(s->vertex_count);
+    a.tri_count = static_cast<uint32_t>(s->tri_count);
+    a.row_count = static_cast<uint32_t>(s->row_count);
+    a.material_count = static_cast<uint32_t>(s->material_count);
+    a.light_count = static_cast<uint32_t>(s->light_count);
+    a.emitter_vec4s = static_cast<uint32_t>(s->emitter_vec4s);
+    a.emitter_tri_count = static_cast<uint32_t>(s->emitter_tri_count);
+    a.object_count = static_cast<uint32_t>(s->object_count);
     a.first_sample = j.first_sample;
     a.spp = j.spp;
     return TRT_OK;
@@ -753,6 +787,7 @@ int trt_render_batch(trt_device *d, const trt_render_job *jobs, uint32_t n) {
         j.args.geo.spill = d->spill;
         j.args.geo.reads = d->reads;
         j.args.geo.spill_stride = threads;
+        j.args.geo.spill_entries = static_cast<uint32_t>(entries);
     }
     // The job table goes through a ring of pinned host slots and device slots: a slot is written again only after
     // the launch that read it is done (its event), so launches queue without waiting for each other.
@@ -774,7 +809,8 @@ int trt_render_batch(trt_device *d, const trt_render_job *jobs, uint32_t n) {
     std::memcpy(ring.host[k], table.data(), bytes);
     TRT_HIP(hipMemcpyAsync(ring.dev[k], ring.host[k], bytes, hipMemcpyHostToDevice, d->stream));
     if (trt::launch_render(static_cast<const trt::RenderJob *>(ring.dev[k]), static_cast<uint32_t>(table.size()),
-                           static_cast<uint32_t>(tiles), d->counter, d->spill, threads, d->guard, grid, d->stream))
+                           static_cast<uint32_t>(tiles), d->counter, d->spill, threads,
+                           static_cast<uint32_t>(entries), d->guard, grid, d->stream))
         return fail(TRT_E_DEVICE, "render kernel launch failed");
     TRT_HIP(hipEventRecord(ring.ev[k], d->stream));
     return ok();

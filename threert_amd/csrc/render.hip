@@ -172,26 +172,36 @@ constexpr uint32_t kObjectStride = 3;
 struct Ctx {
     const RenderArgs *a;
     Stack *st;
-    bool *tripped;
+    uint32_t *flags;  // kFlag* bits: why a walk or a table read stopped and reported
     Rng g;
     bool mirror_path;
     bool sampled_spec;
 
+    // Every table index is checked against its table: one outside reads entry 0 and reports kFlagTable.
+    __device__ __forceinline__ uint64_t ck(uint64_t i, uint64_t n) const {
+        if (i < n) return i;
+        *flags |= kFlagTable;
+        return 0;
+    }
     __device__ __forceinline__ float pf(int vec, int k) const { return __uint_as_float(a->params[4 * vec + k]); }
     __device__ __forceinline__ uint32_t pu(int vec, int k) const { return a->params[4 * vec + k]; }
-    __device__ __forceinline__ const float4 *inst(uint32_t i) const { return a->instances + 10 * uint64_t(i); }
-    __device__ __forceinline__ const float4 *material(uint32_t i) const { return a->materials + 18 * uint64_t(i); }
-    __device__ __forceinline__ const float4 *light(uint32_t i) const { return a->lights + 5 * uint64_t(i); }
-    __device__ __forceinline__ float4 emitter_at(uint32_t i) const { return a->emitters[i]; }
+    __device__ __forceinline__ const float4 *inst(uint32_t i) const {
+        return a->instances + 10 * ck(i, a->geo.instance_count);
+    }
+    __device__ __forceinline__ const float4 *material(uint32_t i) const {
+        return a->materials + 18 * ck(i, a->material_count);
+    }
+    __device__ __forceinline__ const float4 *light(uint32_t i) const { return a->lights + 5 * ck(i, a->light_count); }
+    __device__ __forceinline__ float4 emitter_at(uint32_t i) const { return a->emitters[ck(i, a->emitter_vec4s)]; }
     __device__ __forceinline__ uint32_t row_word(uint32_t first, uint32_t section, uint32_t k) const {
-        return a->rows[4 * (uint64_t(first) + section) + k];
+        return a->rows[4 * ck(uint64_t(first) + section, a->row_count) + k];
     }
     __device__ __forceinline__ float obj_power(uint32_t obj) const {
-        return a->objects[uint64_t(obj) * kObjectStride].w;
+        return a->objects[ck(obj, a->object_count) * kObjectStride].w;
     }
     __device__ HitRec trace(V o, V d, float tmax, uint32_t mode) const {
         Reads cnt;
-        return trace_wide<false>(a->geo, F3{o.x, o.y, o.z}, F3{d.x, d.y, d.z}, tmax, mode, *st, *tripped, cnt);
+        return trace_wide<false>(a->geo, F3{o.x, o.y, o.z}, F3{d.x, d.y, d.z}, tmax, mode, *st, *flags, cnt);
     }
 };
 enum { CAM_POS = 0, CAM_FWD = 1, CAM_RIGHT = 2, CAM_UP = 3, SIZE = 4, COUNTS = 5, ENV = 6, MISC = 7, SKY = 8,
@@ -204,10 +214,10 @@ struct TriAttr {
 __device__ TriAttr tri_attr(const Ctx &c, uint32_t ti, uint32_t inst, uint32_t part, V &e1, V &e2) {
     uint32_t base = 0;
     if (inst != kNone) base = __float_as_uint(c.inst(inst)[6].z);
-    const uint32_t *ix = c.a->indices + 3 * uint64_t(ti);
-    const float4 *p0 = c.a->vertices + 3 * (uint64_t(ix[0]) + base);
-    const float4 *p1 = c.a->vertices + 3 * (uint64_t(ix[1]) + base);
-    const float4 *p2 = c.a->vertices + 3 * (uint64_t(ix[2]) + base);
+    const uint32_t *ix = c.a->indices + 3 * c.ck(ti, c.a->tri_count);
+    const float4 *p0 = c.a->vertices + 3 * c.ck(uint64_t(ix[0]) + base, c.a->vertex_count);
+    const float4 *p1 = c.a->vertices + 3 * c.ck(uint64_t(ix[1]) + base, c.a->vertex_count);
+    const float4 *p2 = c.a->vertices + 3 * c.ck(uint64_t(ix[2]) + base, c.a->vertex_count);
     const float4 a0 = p0[0], a1 = p1[0], a2 = p2[0], n0 = p0[1], n1 = p1[1], n2 = p2[1];
     e1 = vsub(xyz(a1), xyz(a0));
     e2 = vsub(xyz(a2), xyz(a0));
@@ -279,7 +289,7 @@ struct Lobes {
 };
 
 __device__ __forceinline__ void energy_node(const Ctx &c, uint32_t i, float &e0, float &e1) {
-    const float4 v = c.a->energy[i >> 1];
+    const float4 v = c.a->energy[c.ck(i >> 1, 512)];
     e0 = (i & 1u) ? v.z : v.x;
     e1 = (i & 1u) ? v.w : v.y;
 }
@@ -491,7 +501,7 @@ struct Em {
     uint32_t tri, inst, code, part, flags;
 };
 __device__ __forceinline__ Em emitter_tri_at(const Ctx &c, uint32_t i) {
-    const float4 *r = c.a->emitter_tris + 4 * uint64_t(i);
+    const float4 *r = c.a->emitter_tris + 4 * c.ck(i, c.a->emitter_tri_count);
     const float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
     return {xyz(r0), xyz(r1), xyz(r2), __float_as_uint(r0.w), __float_as_uint(r1.w), __float_as_uint(r2.w),
             __float_as_uint(r3.x), __float_as_uint(r3.y)};
@@ -667,7 +677,7 @@ __device__ float emit_mis(const Ctx &c, const HitRec &hit, V d, float pdf) {
     const uint32_t rows = __float_as_uint(c.inst(hit.inst)[6].w);
     const uint32_t first = c.row_word(rows, hit.part, 2u);
     if (first == kNoEmitter) return 1.0f;
-    const uint32_t rank = __float_as_uint(c.a->geo.leaves[3 * uint64_t(hit.tri) + 1].w);
+    const uint32_t rank = __float_as_uint(c.a->geo.leaves[3 * c.ck(hit.tri, c.a->geo.leaf_count) + 1].w);
     const uint32_t entry = first + rank;
     const Em em = emitter_tri_at(c, entry);
     const V cr = cross(em.e1, em.e2);
@@ -776,12 +786,12 @@ __device__ void path(Ctx &c, uint32_t px, uint32_t py, uint32_t index, V &radian
 // job's scene, parameters and accumulator. Jobs of any size share the launch: what one job leaves idle, another fills.
 __global__ __launch_bounds__(kRenderBlock) void k_render(const RenderJob *jobs, uint32_t njobs, uint32_t total,
                                                          uint32_t *counter, uint32_t *spill, uint32_t spill_stride,
-                                                         uint32_t *guard) {
+                                                         uint32_t spill_entries, uint32_t *guard) {
     __shared__ uint32_t lds[kLdsStack * kRenderBlock];
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    Stack st{lds + threadIdx.x, spill + gtid, spill_stride, 0};
-    bool tripped = false;
+    Stack st{lds + threadIdx.x, spill + gtid, spill_stride, 0, kLdsStack + spill_entries};
+    uint32_t flags = 0;
     for (;;) {
         uint32_t tile = 0;
         if (lane == 0) tile = atomicAdd(counter, 1u);
@@ -799,7 +809,7 @@ __global__ __launch_bounds__(kRenderBlock) void k_render(const RenderJob *jobs, 
         const RenderArgs *a = &job->args;
         const uint32_t t = tile - job->tile_first;
         const uint32_t w = a->params[16], h = a->params[17], tiles_x = (w + 7) / 8;
-        Ctx c{a, &st, &tripped, Rng{0, 0, 0}, false, false};
+        Ctx c{a, &st, &flags, Rng{0, 0, 0}, false, false};
         const uint32_t px = (t % tiles_x) * 8 + (lane & 7u), py = (t / tiles_x) * 8 + (lane >> 3);
         if (px < w && py < h) {
             float4 *pix = a->accum + 5 * (uint64_t(py) * w + px);
@@ -817,7 +827,7 @@ __global__ __launch_bounds__(kRenderBlock) void k_render(const RenderJob *jobs, 
             pix[2] = nrm;
         }
     }
-    if (tripped) atomicOr(guard, 1u);
+    if (flags) atomicOr(guard, flags);
 }
 
 __global__ void k_reset(uint32_t *c) { *c = 0; }
@@ -832,10 +842,11 @@ int render_occupancy(int *blocks_per_cu) {
 }
 
 int launch_render(const RenderJob *d_jobs, uint32_t njobs, uint32_t total_tiles, uint32_t *counter,
-                  uint32_t *spill, uint32_t spill_stride, uint32_t *guard, int grid, hipStream_t s) {
+                  uint32_t *spill, uint32_t spill_stride, uint32_t spill_entries, uint32_t *guard, int grid,
+                  hipStream_t s) {
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, s, counter);
     hipLaunchKernelGGL(k_render, dim3(grid), dim3(kRenderBlock), 0, s, d_jobs, njobs, total_tiles, counter, spill,
-                       spill_stride, guard);
+                       spill_stride, spill_entries, guard);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -25,6 +25,9 @@ struct RenderArgs {
     const float4 *objects;      // 3 x float4 an object
     const float4 *energy;       // lib.rs energy_lut, 512 x float4
     float4 *accum;              // records.wgsl Pixel, 5 x float4 a pixel
+    // the tables' sizes (in their records), against which every index is checked
+    uint32_t vertex_count, tri_count, row_count, material_count, light_count, emitter_vec4s, emitter_tri_count,
+        object_count;
     uint32_t first_sample;
     uint32_t spp;
     uint32_t params[kParamWords];
@@ -40,6 +43,7 @@ struct RenderJob {
 int render_occupancy(int *blocks_per_cu);
 // d_jobs: njobs jobs in device memory, tile_first ascending from 0; total_tiles: all their tiles.
 int launch_render(const RenderJob *d_jobs, uint32_t njobs, uint32_t total_tiles, uint32_t *counter,
-                  uint32_t *spill, uint32_t spill_stride, uint32_t *guard, int grid, hipStream_t s);
+                  uint32_t *spill, uint32_t spill_stride, uint32_t spill_entries, uint32_t *guard, int grid,
+                  hipStream_t s);
 
 }  // namespace trt

@@ -15,8 +15,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
     __shared__ uint32_t lds[kLdsStack * kTraceBlock];
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    Stack st{lds + threadIdx.x, a.spill + gtid, a.spill_stride, 0};
-    bool tripped = false;
+    Stack st{lds + threadIdx.x, a.spill + gtid, a.spill_stride, 0, kLdsStack + a.spill_entries};
+    uint32_t flags = 0;
     Reads cnt;
     for (;;) {
         uint32_t first_ray = 0;
@@ -28,12 +28,12 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
             const float4 r0 = a.rays[2 * i], r1 = a.rays[2 * i + 1];
             const F3 o{r0.x, r0.y, r0.z}, d{r1.x, r1.y, r1.z};
             const uint32_t mode = __float_as_uint(r1.w);
-            const HitRec h = kWide ? trace_wide<kCount>(a, o, d, r0.w, mode, st, tripped, cnt)
-                                   : trace_binary<kCount>(a, o, d, r0.w, mode, st, tripped, cnt);
+            const HitRec h = kWide ? trace_wide<kCount>(a, o, d, r0.w, mode, st, flags, cnt)
+                                   : trace_binary<kCount>(a, o, d, r0.w, mode, st, flags, cnt);
             store_hit(a.hits, i, h);
         }
     }
-    if (tripped) atomicOr(a.guard_flag, 1u);
+    if (flags) atomicOr(a.guard_flag, flags);
     if (kCount) {
         atomicAdd(a.reads + 0, (unsigned long long)cnt.nodes);
         atomicAdd(a.reads + 1, (unsigned long long)cnt.tris);

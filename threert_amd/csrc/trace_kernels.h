@@ -13,6 +13,10 @@ constexpr int kTraceBlock = 256;        // threads per block: four wave64s
 constexpr uint32_t kLdsStack = 16;      // stack entries per thread held in LDS (16 KiB a block)
 constexpr uint32_t kRefStack = 64;      // units.rs STACK: the binary walk drops pushes past it, as the reference
 constexpr uint32_t kGuardSteps = 1u << 24;  // a walk's iterations before it stops and reports (gpu.rs loop guard)
+// Bits of the device's guard word, each a reason a kernel stopped a walk or a path and reported instead of reading:
+constexpr uint32_t kFlagGuard = 1;   // a walk reached kGuardSteps
+constexpr uint32_t kFlagWalk = 2;    // a walk met a reference outside its scene's buffers, or a stack past its room
+constexpr uint32_t kFlagTable = 4;   // a shading-table index outside its table
 
 struct TraceArgs {
     const uint4 *wnodes;      // WideNode, 4 x uint4 each
@@ -23,11 +27,13 @@ struct TraceArgs {
     const float4 *rays;       // trt_ray, 2 x float4 each
     uint32_t *hits;           // trt_hit, 6 words each
     uint32_t *counter;        // rays handed out so far
-    uint32_t *guard_flag;     // set when a walk stops at kGuardSteps
+    uint32_t *guard_flag;     // kFlag* bits of the walks that stopped and reported
     uint32_t *spill;          // stack entries past kLdsStack, [entry][thread]
     unsigned long long *reads;  // counting launches: nodes, triangles, units
     uint64_t n;
     uint32_t spill_stride;    // threads in the grid
+    uint32_t spill_entries;   // stack entries a thread has past kLdsStack
+    uint32_t wnode_count, wunit_count, bnode_count, instance_count, leaf_count;  // the buffers' sizes, checked
     uint32_t top_ref;         // the wide top tree's root reference
     uint32_t top_first;       // the binary top tree's root (P.motion.z)
     uint32_t units;           // P.motion.w

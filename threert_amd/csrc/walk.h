@@ -92,6 +92,7 @@ struct Stack {
     uint32_t *spill;  // this thread's spill entry 0; entry k at spill[k * stride]
     uint32_t stride;
     uint32_t sp;
+    uint32_t cap;     // entries in all: kLdsStack + the spill entries; the walks check it before they push
     __device__ __forceinline__ void push(uint32_t v) {
         if (sp < kLdsStack)
             lds[sp * kTraceBlock] = v;
@@ -123,12 +124,16 @@ struct Reads {
 // ---- the reference's binary walk ------------------------------------------------------------------------------
 template <bool kCount>
 __device__ HitRec trace_binary(const TraceArgs &a, F3 o, F3 d, float tmax_in, uint32_t mode, Stack &st,
-                               bool &tripped, Reads &cnt) {
+                               uint32_t &flags, Reads &cnt) {
     const float tmax = tmax_in < kInf ? tmax_in : kInf;
     HitRec h{tmax, kNone, 0.0f, 0.0f, kNone, 0};
     if (a.units == 0) return h;
     const F3 inv = inv3(d);
     const uint32_t top = a.top_first;
+    if (top >= a.bnode_count) {
+        flags |= kFlagWalk;
+        return h;
+    }
     {
         if (kCount) cnt.nodes++;
         const float4 w0 = a.bnodes[2 * top], w1 = a.bnodes[2 * top + 1];
@@ -140,7 +145,11 @@ __device__ HitRec trace_binary(const TraceArgs &a, F3 o, F3 d, float tmax_in, ui
     uint32_t guard = 0;
     for (;;) {
         if (++guard > kGuardSteps) {
-            tripped = true;
+            flags |= kFlagGuard;
+            return h;
+        }
+        if (node >= a.bnode_count) {
+            flags |= kFlagWalk;
             return h;
         }
         const float4 n0 = a.bnodes[2 * node], n1 = a.bnodes[2 * node + 1];
@@ -149,6 +158,10 @@ __device__ HitRec trace_binary(const TraceArgs &a, F3 o, F3 d, float tmax_in, ui
         if (count == 0) {
             if (kCount) cnt.nodes += 2;
             const uint32_t l = off + left_or_first, r = l + 1;
+            if (uint64_t(off) + left_or_first + 1 >= a.bnode_count) {
+                flags |= kFlagWalk;
+                return h;
+            }
             const float4 l0 = a.bnodes[2 * l], l1 = a.bnodes[2 * l + 1];
             const float4 r0 = a.bnodes[2 * r], r1 = a.bnodes[2 * r + 1];
             float tl = ray_aabb(ro, rinv, {l0.x, l0.y, l0.z}, {l1.x, l1.y, l1.z}, cull_t(h.t));
@@ -161,12 +174,22 @@ __device__ HitRec trace_binary(const TraceArgs &a, F3 o, F3 d, float tmax_in, ui
                     tl = tr, tr = tmp;
                 }
                 node = near;
-                if (tr != kInf && st.sp < kRefStack) st.push(far);
+                if (tr != kInf && st.sp < kRefStack) {
+                    if (st.sp >= st.cap) {
+                        flags |= kFlagWalk;
+                        return h;
+                    }
+                    st.push(far);
+                }
                 continue;
             }
         } else if (rec == kNone) {
             if (kCount) cnt.units++;
-            const float4 *u = a.instances + 10 * left_or_first;
+            if (left_or_first >= a.instance_count) {
+                flags |= kFlagWalk;
+                return h;
+            }
+            const float4 *u = a.instances + 10 * uint64_t(left_or_first);
             const float4 m0 = u[0], m1 = u[1], m2 = u[2], q0 = u[3], q1 = u[4], q2 = u[5];
             const uint32_t flags = __float_as_uint(q1.w);
             if (mode != kTraceShadow || (flags & kUnitShadow) != 0) {
@@ -185,6 +208,10 @@ __device__ HitRec trace_binary(const TraceArgs &a, F3 o, F3 d, float tmax_in, ui
                 continue;
             }
         } else {
+            if (uint64_t(first) + left_or_first + count > a.leaf_count) {
+                flags |= kFlagWalk;
+                return h;
+            }
             for (uint32_t k = 0; k < count; k++) {
                 const uint32_t ti = first + left_or_first + k;
                 if (kCount) cnt.tris++;
@@ -217,7 +244,7 @@ __device__ __forceinline__ float qbyte(uint32_t w, int c) { return float((w >> (
 
 template <bool kCount>
 __device__ HitRec trace_wide(const TraceArgs &a, F3 o, F3 d, float tmax_in, uint32_t mode, Stack &st,
-                             bool &tripped, Reads &cnt) {
+                             uint32_t &flags, Reads &cnt) {
     const float tmax = tmax_in < kInf ? tmax_in : kInf;
     HitRec h{tmax, kNone, 0.0f, 0.0f, kNone, 0};
     if (a.units == 0) return h;
@@ -231,12 +258,16 @@ __device__ HitRec trace_wide(const TraceArgs &a, F3 o, F3 d, float tmax_in, uint
     uint32_t guard = 0;
     for (;;) {
         if (++guard > kGuardSteps) {
-            tripped = true;
+            flags |= kFlagGuard;
             return h;
         }
         const uint32_t kind = node >> kRefKindShift;
         if (kind == kRefNode) {
             if (kCount) cnt.nodes++;
+            if (node >= a.wnode_count) {
+                flags |= kFlagWalk;
+                return h;
+            }
             const uint4 *p = a.wnodes + 4 * uint64_t(node);
             const uint4 w0 = p[0], w1 = p[1], w2 = p[2], w3 = p[3];
             const float sx = __uint_as_float((w0.w & 0xffu) << 23), sy = __uint_as_float(((w0.w >> 8) & 0xffu) << 23),
@@ -276,6 +307,12 @@ __device__ HitRec trace_wide(const TraceArgs &a, F3 o, F3 d, float tmax_in, uint
             TRT_CSWAP(1, 2);
 #undef TRT_CSWAP
             if (key[0] != kNone) {
+                const uint32_t pushes =
+                    uint32_t(key[1] != kNone) + uint32_t(key[2] != kNone) + uint32_t(key[3] != kNone);
+                if (st.sp + pushes > st.cap) {
+                    flags |= kFlagWalk;
+                    return h;
+                }
                 if (key[3] != kNone) st.push(pick(key[3] & 3u, w2.z, w2.w, w3.x, w3.y));
                 if (key[2] != kNone) st.push(pick(key[2] & 3u, w2.z, w2.w, w3.x, w3.y));
                 if (key[1] != kNone) st.push(pick(key[1] & 3u, w2.z, w2.w, w3.x, w3.y));
@@ -285,6 +322,10 @@ __device__ HitRec trace_wide(const TraceArgs &a, F3 o, F3 d, float tmax_in, uint
         } else if (kind == kRefUnit) {
             const uint32_t r = node & 0x3fffffffu;
             if (kCount) cnt.units++;
+            if (r >= a.wunit_count) {
+                flags |= kFlagWalk;
+                return h;
+            }
             const float4 *u = a.wunits + 4 * uint64_t(r);
             const float4 u0 = u[0];
             const uint32_t flags = __float_as_uint(u0.w);
@@ -306,6 +347,10 @@ __device__ HitRec trace_wide(const TraceArgs &a, F3 o, F3 d, float tmax_in, uint
         } else {
             const uint32_t count = ((node >> kRefTriFirstBits) & 15u) + 1u;
             const uint32_t f = first + (node & kRefTriFirstMask);
+            if (kind != kRefTris || uint64_t(first) + (node & kRefTriFirstMask) + count > a.leaf_count) {
+                flags |= kFlagWalk;
+                return h;
+            }
             for (uint32_t k = 0; k < count; k++) {
                 const uint32_t ti = f + k;
                 if (kCount) cnt.tris++;
