@@ -33,6 +33,22 @@ N_TRIS = 3_000_000
 N_RAYS = 16 * 1024 * 1024
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_trace_wide_precision3M_hbm_traffic.md")
+
+
+def profiled_traffic():
+    """HBM bytes per launch of the walk on this workload, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+    (calibrated against the copy kernel); None when the summary is missing."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -237,7 +253,7 @@ def main():
             "wall_s_timed": round(wall, 3),
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic()},
         "cpu_baseline": None,
     }
     sc.close()

@@ -214,6 +214,31 @@ int trt_render_batch(trt_device *dev, const trt_render_job *jobs, uint32_t n);
 int trt_denoise_atrous(trt_device *dev, uint32_t width, uint32_t height, const float *d_color, const float *d_albedo,
                        const float *d_normal, const float *d_variance, uint32_t iterations, float *d_out);
 
+/* --- learned denoiser (replaces denoise.rs:81-103 denoise::oidn; OIDN's network is not in the reference) ------ */
+/* A five-level U-Net of 3x3 convolutions (32 to 128 channels) on BF16 MFMA with FP32 accumulation, trained and
+ * run on the device. Inputs per pixel (3 floats each): color, albedo, normal, as trt_accum_resolve gives them; the
+ * output is the denoised color. Images: n of h x w, h and w multiples of 16, n * h * w at most max_pixels. The
+ * parameters are FP32 (trt_unet_param_count of them: each layer's weights [cout][9][cin] then its biases); every
+ * call but create / release / param_count / params_get is queued on the device's stream. */
+typedef struct trt_unet trt_unet;
+int trt_unet_create(trt_device *dev, uint64_t max_pixels, trt_unet **out);
+int trt_unet_release(trt_unet *net);
+int trt_unet_param_count(trt_unet *net, uint64_t *count);
+/* Sets the parameters (host memory) and restarts the optimizer. */
+int trt_unet_params_set(trt_unet *net, const float *params, uint64_t count);
+int trt_unet_params_get(trt_unet *net, float *params, uint64_t count);
+/* Device buffers in and out (the reference: denoise::oidn(width, height, color, albedo, normal)). */
+int trt_unet_denoise(trt_unet *net, uint32_t n, uint32_t h, uint32_t w, const float *d_color, const float *d_albedo,
+                     const float *d_normal, float *d_out);
+/* The loss (mean |y - log(1 + target)| over the three color channels, y the network's log-space output) and its
+ * gradient with respect to every parameter, into host memory; waits for the device. d_target: 3 floats a pixel. */
+int trt_unet_gradients(trt_unet *net, uint32_t n, uint32_t h, uint32_t w, const float *d_color, const float *d_albedo,
+                       const float *d_normal, const float *d_target, float *loss, float *grads);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) on that loss; loss (host, may be NULL: then nothing waits) gets the
+ * step's loss before the update. */
+int trt_unet_train_step(trt_unet *net, uint32_t n, uint32_t h, uint32_t w, const float *d_color,
+                        const float *d_albedo, const float *d_normal, const float *d_target, float lr, float *loss);
+
 #ifdef __cplusplus
 }
 #endif

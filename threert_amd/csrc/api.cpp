@@ -148,6 +148,27 @@ int trt_device_release(trt_device *d) {
     device_unref(d);
     return ok();
 }
+}  // extern "C"
+
+// Accessors for the library's other translation units (unet.cpp).
+hipStream_t trt_device_stream(trt_device *d) { return d->stream; }
+int trt_device_ordinal(trt_device *d) { return d->ordinal; }
+void trt_device_lock(trt_device *d, bool lock) {
+    if (lock)
+        d->mu.lock();
+    else
+        d->mu.unlock();
+}
+void trt_device_ref(trt_device *d, int delta) {
+    if (delta > 0)
+        d->refs += delta;
+    else
+        device_unref(d);
+}
+int trt_fail(int code, const std::string &msg) { return fail(code, msg); }
+int trt_ok() { return ok(); }
+
+extern "C" {
 
 int trt_device_info_get(trt_device *d, trt_device_info *out) {
     if (!d || !out) return fail(TRT_E_INVALID, "trt_device_info_get: null argument");
@@ -165,13 +186,29 @@ int trt_device_sync(trt_device *d) {
     if (!d) return fail(TRT_E_INVALID, "trt_device_sync: null device");
     TRT_HIP(hipSetDevice(d->ordinal));
     TRT_HIP(hipStreamSynchronize(d->stream));
-    uint32_t g = 0;
-    TRT_HIP(hipMemcpy(&g, d->guard, 4, hipMemcpyDeviceToHost));
-    if (g) {
-        TRT_HIP(hipMemset(d->guard, 0, 4));
-        if (g & trt::kFlagWalk)
-            return fail(TRT_E_GUARD, "a walk met a reference outside its scene's buffers or a stack past its room");
-        if (g & trt::kFlagTable) return fail(TRT_E_GUARD, "a shading-table index was outside its table");
+    uint32_t g[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    TRT_HIP(hipMemcpy(g, d->guard, sizeof(g), hipMemcpyDeviceToHost));
+    if (g[0]) {
+        TRT_HIP(hipMemset(d->guard, 0, sizeof(g)));
+        // the first out-of-range read (walk.h report_range ids)
+        static const char *what[] = {"?", "binary top node", "binary node", "binary children", "binary stack",
+                                     "unit record", "binary leaf run", "wide node", "wide stack", "wide unit",
+                                     "wide leaf run"};
+        static const char *table[] = {"instance", "material", "light", "emitter entry", "row", "object",
+                                      "triangle indices", "vertex", "energy table", "emitter triangle (RIS candidate)",
+                                      "leaf", "emitter triangle (RIS pick)", "emitter triangle (single draw)",
+                                      "emitter triangle (a hit's own)"};
+        std::string where;
+        if (g[1]) {
+            const char *name = g[1] <= 10 ? what[g[1]] : (g[1] >= 16 && g[1] <= 29 ? table[g[1] - 16] : "?");
+            where = std::string(" (first: ") + name + " index " +
+                    std::to_string((uint64_t(g[3]) << 32) | g[2]) + " of " + std::to_string(g[4]) + "; context " +
+                    std::to_string(g[5]) + " " + std::to_string(g[6]) + " " + std::to_string(g[7]) + ")";
+        }
+        if (g[0] & trt::kFlagWalk)
+            return fail(TRT_E_GUARD,
+                        "a walk met a reference outside its scene's buffers or a stack past its room" + where);
+        if (g[0] & trt::kFlagTable) return fail(TRT_E_GUARD, "a shading-table index was outside its table" + where);
         return fail(TRT_E_GUARD, "a walk stopped at its iteration guard");
     }
     return ok();

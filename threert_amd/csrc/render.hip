@@ -178,26 +178,28 @@ struct Ctx {
     bool sampled_spec;
 
     // Every table index is checked against its table: one outside reads entry 0 and reports kFlagTable.
-    __device__ __forceinline__ uint64_t ck(uint64_t i, uint64_t n) const {
+    __device__ __forceinline__ uint64_t ck(uint64_t i, uint64_t n, uint32_t table,
+                                           const uint32_t *context = nullptr) const {
         if (i < n) return i;
         *flags |= kFlagTable;
+        report_range(a->geo.guard_flag, table, i, n, context);
         return 0;
     }
     __device__ __forceinline__ float pf(int vec, int k) const { return __uint_as_float(a->params[4 * vec + k]); }
     __device__ __forceinline__ uint32_t pu(int vec, int k) const { return a->params[4 * vec + k]; }
     __device__ __forceinline__ const float4 *inst(uint32_t i) const {
-        return a->instances + 10 * ck(i, a->geo.instance_count);
+        return a->instances + 10 * ck(i, a->geo.instance_count, 16);
     }
     __device__ __forceinline__ const float4 *material(uint32_t i) const {
-        return a->materials + 18 * ck(i, a->material_count);
+        return a->materials + 18 * ck(i, a->material_count, 17);
     }
-    __device__ __forceinline__ const float4 *light(uint32_t i) const { return a->lights + 5 * ck(i, a->light_count); }
-    __device__ __forceinline__ float4 emitter_at(uint32_t i) const { return a->emitters[ck(i, a->emitter_vec4s)]; }
+    __device__ __forceinline__ const float4 *light(uint32_t i) const { return a->lights + 5 * ck(i, a->light_count, 18); }
+    __device__ __forceinline__ float4 emitter_at(uint32_t i) const { return a->emitters[ck(i, a->emitter_vec4s, 19)]; }
     __device__ __forceinline__ uint32_t row_word(uint32_t first, uint32_t section, uint32_t k) const {
-        return a->rows[4 * ck(uint64_t(first) + section, a->row_count) + k];
+        return a->rows[4 * ck(uint64_t(first) + section, a->row_count, 20) + k];
     }
     __device__ __forceinline__ float obj_power(uint32_t obj) const {
-        return a->objects[ck(obj, a->object_count) * kObjectStride].w;
+        return a->objects[ck(obj, a->object_count, 21) * kObjectStride].w;
     }
     __device__ HitRec trace(V o, V d, float tmax, uint32_t mode) const {
         Reads cnt;
@@ -214,10 +216,10 @@ struct TriAttr {
 __device__ TriAttr tri_attr(const Ctx &c, uint32_t ti, uint32_t inst, uint32_t part, V &e1, V &e2) {
     uint32_t base = 0;
     if (inst != kNone) base = __float_as_uint(c.inst(inst)[6].z);
-    const uint32_t *ix = c.a->indices + 3 * c.ck(ti, c.a->tri_count);
-    const float4 *p0 = c.a->vertices + 3 * c.ck(uint64_t(ix[0]) + base, c.a->vertex_count);
-    const float4 *p1 = c.a->vertices + 3 * c.ck(uint64_t(ix[1]) + base, c.a->vertex_count);
-    const float4 *p2 = c.a->vertices + 3 * c.ck(uint64_t(ix[2]) + base, c.a->vertex_count);
+    const uint32_t *ix = c.a->indices + 3 * c.ck(ti, c.a->tri_count, 22);
+    const float4 *p0 = c.a->vertices + 3 * c.ck(uint64_t(ix[0]) + base, c.a->vertex_count, 23);
+    const float4 *p1 = c.a->vertices + 3 * c.ck(uint64_t(ix[1]) + base, c.a->vertex_count, 23);
+    const float4 *p2 = c.a->vertices + 3 * c.ck(uint64_t(ix[2]) + base, c.a->vertex_count, 23);
     const float4 a0 = p0[0], a1 = p1[0], a2 = p2[0], n0 = p0[1], n1 = p1[1], n2 = p2[1];
     e1 = vsub(xyz(a1), xyz(a0));
     e2 = vsub(xyz(a2), xyz(a0));
@@ -289,7 +291,7 @@ struct Lobes {
 };
 
 __device__ __forceinline__ void energy_node(const Ctx &c, uint32_t i, float &e0, float &e1) {
-    const float4 v = c.a->energy[c.ck(i >> 1, 512)];
+    const float4 v = c.a->energy[c.ck(i >> 1, 512, 24)];
     e0 = (i & 1u) ? v.z : v.x;
     e1 = (i & 1u) ? v.w : v.y;
 }
@@ -500,8 +502,10 @@ struct Em {
     V v0, e1, e2;
     uint32_t tri, inst, code, part, flags;
 };
-__device__ __forceinline__ Em emitter_tri_at(const Ctx &c, uint32_t i) {
-    const float4 *r = c.a->emitter_tris + 4 * c.ck(i, c.a->emitter_tri_count);
+// site: which caller reads the table (25 RIS candidate, 27 RIS pick, 28 single draw, 29 a hit's own emitter)
+__device__ __forceinline__ Em emitter_tri_at(const Ctx &c, uint32_t i, uint32_t site,
+                                             const uint32_t *context = nullptr) {
+    const float4 *r = c.a->emitter_tris + 4 * c.ck(i, c.a->emitter_tri_count, site, context);
     const float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
     return {xyz(r0), xyz(r1), xyz(r2), __float_as_uint(r0.w), __float_as_uint(r1.w), __float_as_uint(r2.w),
             __float_as_uint(r3.x), __float_as_uint(r3.y)};
@@ -541,7 +545,7 @@ __device__ ES sample_emitter_ris(Ctx &c, V x, V n, uint32_t m, const Surface &sf
         if (pk_p <= 0.0f) continue;
         if (b1 + b2 > 1.0f) b1 = 1.0f - b1, b2 = 1.0f - b2;
         const uint32_t ei = __float_as_uint(e.x);
-        const Em em = emitter_tri_at(c, ei);
+        const Em em = emitter_tri_at(c, ei, 25);
         const V lp = vadd(vadd(em.v0, vs(em.e1, b1)), vs(em.e2, b2));
         const V cr = cross(em.e1, em.e2);
         const float a2 = len(cr);
@@ -560,7 +564,7 @@ __device__ ES sample_emitter_ris(Ctx &c, V x, V n, uint32_t m, const Surface &sf
         if (bb[2] * wsum < w) pick_b1 = b1, pick_b2 = b2, pick_ph = ph, pick_i = ei, pick_pdf = pdf_a;
     }
     if (pick_ph <= 0.0f) return r;
-    const Em em = emitter_tri_at(c, pick_i);
+    const Em em = emitter_tri_at(c, pick_i, 27);
     const V lp = vadd(vadd(em.v0, vs(em.e1, pick_b1)), vs(em.e2, pick_b2));
     const V ln = normalize(cross(em.e1, em.e2));
     const V dl = vsub(lp, x);
@@ -587,7 +591,7 @@ __device__ ES sample_emitter(Ctx &c, V x, uint32_t rd) {
     const float4 e = emitter_pick_u(c, r3[0]);
     const float pk_p = e.z / tot.x;
     if (pk_p <= 0.0f) return r;
-    const Em em = emitter_tri_at(c, __float_as_uint(e.x));
+    const Em em = emitter_tri_at(c, __float_as_uint(e.x), 28);
     float bb[2];
     c.g.r2(rng_sub(rd, 1u), bb);
     float b1 = bb[0], b2 = bb[1];
@@ -677,9 +681,10 @@ __device__ float emit_mis(const Ctx &c, const HitRec &hit, V d, float pdf) {
     const uint32_t rows = __float_as_uint(c.inst(hit.inst)[6].w);
     const uint32_t first = c.row_word(rows, hit.part, 2u);
     if (first == kNoEmitter) return 1.0f;
-    const uint32_t rank = __float_as_uint(c.a->geo.leaves[3 * c.ck(hit.tri, c.a->geo.leaf_count) + 1].w);
+    const uint32_t rank = __float_as_uint(c.a->geo.leaves[3 * c.ck(hit.tri, c.a->geo.leaf_count, 26) + 1].w);
     const uint32_t entry = first + rank;
-    const Em em = emitter_tri_at(c, entry);
+    const uint32_t context[3] = {hit.tri, hit.part, first};
+    const Em em = emitter_tri_at(c, entry, 29, context);
     const V cr = cross(em.e1, em.e2);
     const float area = 0.5f * len(cr);
     const float cos_l = fabsf(dot(normalize(cr), d));

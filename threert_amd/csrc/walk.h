@@ -67,6 +67,20 @@ __device__ __forceinline__ void intersect_tri(F3 o, F3 d, float4 l0, float4 l1, 
     t = tt, u = uu, v = vv;
 }
 
+// The first out-of-range read of a launch, for the host's error message: guard word 1 = which check (a kRange id),
+// words 2-3 = the index (low, high), word 4 = the size it was checked against. Word 0 holds the kFlag bits.
+// Words 5-7 may carry three values of the reporting thread's context (ctx non-null). Returns whether this thread's
+// report is the one kept.
+__device__ __forceinline__ bool report_range(uint32_t *guard, uint32_t id, uint64_t i, uint64_t n,
+                                             const uint32_t *ctx = nullptr) {
+    if (atomicCAS(guard + 1, 0u, id) != 0u) return false;
+    guard[2] = static_cast<uint32_t>(i);
+    guard[3] = static_cast<uint32_t>(i >> 32);
+    guard[4] = static_cast<uint32_t>(n);
+    if (ctx) guard[5] = ctx[0], guard[6] = ctx[1], guard[7] = ctx[2];
+    return true;
+}
+
 struct HitRec {
     float t;
     uint32_t tri;
@@ -132,6 +146,7 @@ __device__ HitRec trace_binary(const TraceArgs &a, F3 o, F3 d, float tmax_in, ui
     const uint32_t top = a.top_first;
     if (top >= a.bnode_count) {
         flags |= kFlagWalk;
+        report_range(a.guard_flag, 1, top, a.bnode_count);
         return h;
     }
     {
@@ -150,6 +165,7 @@ __device__ HitRec trace_binary(const TraceArgs &a, F3 o, F3 d, float tmax_in, ui
         }
         if (node >= a.bnode_count) {
             flags |= kFlagWalk;
+            report_range(a.guard_flag, 2, node, a.bnode_count);
             return h;
         }
         const float4 n0 = a.bnodes[2 * node], n1 = a.bnodes[2 * node + 1];
@@ -160,6 +176,7 @@ __device__ HitRec trace_binary(const TraceArgs &a, F3 o, F3 d, float tmax_in, ui
             const uint32_t l = off + left_or_first, r = l + 1;
             if (uint64_t(off) + left_or_first + 1 >= a.bnode_count) {
                 flags |= kFlagWalk;
+                report_range(a.guard_flag, 3, uint64_t(off) + left_or_first + 1, a.bnode_count);
                 return h;
             }
             const float4 l0 = a.bnodes[2 * l], l1 = a.bnodes[2 * l + 1];
@@ -177,6 +194,7 @@ __device__ HitRec trace_binary(const TraceArgs &a, F3 o, F3 d, float tmax_in, ui
                 if (tr != kInf && st.sp < kRefStack) {
                     if (st.sp >= st.cap) {
                         flags |= kFlagWalk;
+                        report_range(a.guard_flag, 4, st.sp, st.cap);
                         return h;
                     }
                     st.push(far);
@@ -187,6 +205,7 @@ __device__ HitRec trace_binary(const TraceArgs &a, F3 o, F3 d, float tmax_in, ui
             if (kCount) cnt.units++;
             if (left_or_first >= a.instance_count) {
                 flags |= kFlagWalk;
+                report_range(a.guard_flag, 5, left_or_first, a.instance_count);
                 return h;
             }
             const float4 *u = a.instances + 10 * uint64_t(left_or_first);
@@ -210,6 +229,7 @@ __device__ HitRec trace_binary(const TraceArgs &a, F3 o, F3 d, float tmax_in, ui
         } else {
             if (uint64_t(first) + left_or_first + count > a.leaf_count) {
                 flags |= kFlagWalk;
+                report_range(a.guard_flag, 6, uint64_t(first) + left_or_first + count, a.leaf_count);
                 return h;
             }
             for (uint32_t k = 0; k < count; k++) {
@@ -266,6 +286,7 @@ __device__ HitRec trace_wide(const TraceArgs &a, F3 o, F3 d, float tmax_in, uint
             if (kCount) cnt.nodes++;
             if (node >= a.wnode_count) {
                 flags |= kFlagWalk;
+                report_range(a.guard_flag, 7, node, a.wnode_count);
                 return h;
             }
             const uint4 *p = a.wnodes + 4 * uint64_t(node);
@@ -311,6 +332,7 @@ __device__ HitRec trace_wide(const TraceArgs &a, F3 o, F3 d, float tmax_in, uint
                     uint32_t(key[1] != kNone) + uint32_t(key[2] != kNone) + uint32_t(key[3] != kNone);
                 if (st.sp + pushes > st.cap) {
                     flags |= kFlagWalk;
+                    report_range(a.guard_flag, 8, st.sp + pushes, st.cap);
                     return h;
                 }
                 if (key[3] != kNone) st.push(pick(key[3] & 3u, w2.z, w2.w, w3.x, w3.y));
@@ -324,6 +346,7 @@ __device__ HitRec trace_wide(const TraceArgs &a, F3 o, F3 d, float tmax_in, uint
             if (kCount) cnt.units++;
             if (r >= a.wunit_count) {
                 flags |= kFlagWalk;
+                report_range(a.guard_flag, 9, r, a.wunit_count);
                 return h;
             }
             const float4 *u = a.wunits + 4 * uint64_t(r);
@@ -349,6 +372,7 @@ __device__ HitRec trace_wide(const TraceArgs &a, F3 o, F3 d, float tmax_in, uint
             const uint32_t f = first + (node & kRefTriFirstMask);
             if (kind != kRefTris || uint64_t(first) + (node & kRefTriFirstMask) + count > a.leaf_count) {
                 flags |= kFlagWalk;
+                report_range(a.guard_flag, 10, (uint64_t(kind) << 32) | (uint64_t(first) + (node & kRefTriFirstMask) + count), a.leaf_count);
                 return h;
             }
             for (uint32_t k = 0; k < count; k++) {
