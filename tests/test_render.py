@@ -209,3 +209,27 @@ def test_samples_split_across_calls_equal_one_call(device):
     a.close()
     b.close()
     sc.close()
+
+
+@pytest.mark.gpu
+def test_exact_ties_keep_their_own_section(device):
+    """Four instances of one soup at one place: instance 0 places the mesh whose triangles are in a plain section,
+    instances 1-3 a copy whose triangles are in an emissive section. Every ray that meets the soup meets an exact
+    tie, won by instance 0, so no emission may show; the winner's section must come with it whichever instance the
+    walk met first. (Written as a branch around the hit update and inlined into the path tracer, the compiler once
+    kept the replaced hit's section.)"""
+    print("exact ties: 2000 triangles x 4 stacked instances, 64x64 x 4 spp", flush=True)
+    tris, _ = S.clustered_soup(2000, 12, clusters=4, extent=10.0, median_area=0.05)
+    plain = R.RMesh(tris, sections=np.zeros(len(tris), np.uint32))
+    glowing = R.RMesh(tris, sections=np.ones(len(tris), np.uint32))
+    eye = np.eye(4, dtype=np.float32)
+    rs = R.build([plain, glowing], [R.RInstance(0, eye, [0, 1])] + [R.RInstance(1, eye, [0, 1]) for _ in range(3)],
+                 [R.material((0.7, 0.7, 0.7)), R.material((0.0, 0.0, 0.0), (40.0, 40.0, 40.0))],
+                 [R.directional_light((-0.3, -1.0, -0.2), (3.0, 3.0, 3.0))])
+    cam = R.Camera((0.0, 4.0, -16.0), (0.0, 0.0, 0.0), 50.0)
+    p = rs.params(64, 64, cam, seed=3, max_bounces=3, rr_depth=2, sky=(0.2, 0.3, 0.4))
+    want = oracle_scene(rs).render(p, 0, 4)
+    got = gpu_render(device, rs, p, 0, 4)
+    same = (got.view(np.uint32) == want.view(np.uint32)).all(axis=2).mean()
+    print(f"ties: {same:.4f} of the pixels bit-equal, rel RMSE {rel_rmse(got, want):.2e}", flush=True)
+    assert same == 1.0 and rel_rmse(got, want) <= 1e-3

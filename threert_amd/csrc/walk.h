@@ -95,6 +95,19 @@ __device__ __forceinline__ bool accept(const HitRec &h, float x, uint32_t inst, 
     return x == h.t && h.tri != kNone && (inst < h.inst || (inst == h.inst && tri < h.tri));
 }
 
+// The hit update as one select per field, with no branch around it. Written as a branch (`if (take) h = {...}`)
+// and inlined into the path tracer, hipcc for gfx950 (ROCm 7.2, -O3) kept the section id of the replaced hit when
+// the new one won an exact-t tie: the hit then named one triangle and another's section.
+__device__ __forceinline__ void take_hit(HitRec &h, bool take, float x, uint32_t tri, float u, float v,
+                                         uint32_t inst, uint32_t part) {
+    h.t = take ? x : h.t;
+    h.tri = take ? tri : h.tri;
+    h.u = take ? u : h.u;
+    h.v = take ? v : h.v;
+    h.inst = take ? inst : h.inst;
+    h.part = take ? part : h.part;
+}
+
 // pathtrace.wgsl proc_dir: n0 * d.x + n1 * d.y + n2 * d.z.
 __device__ __forceinline__ F3 proc_dir(F3 n0, F3 n1, F3 n2, F3 d) {
     return {n0.x * d.x + n1.x * d.y + n2.x * d.z, n0.y * d.x + n1.y * d.y + n2.y * d.z,
@@ -239,10 +252,9 @@ __device__ HitRec trace_binary(const TraceArgs &a, F3 o, F3 d, float tmax_in, ui
                 const float4 lv0 = l[0], lv1 = l[1], lv2 = l[2];
                 float x, uu, vv;
                 intersect_tri(ro, rd, lv0, lv1, lv2, tmax, x, uu, vv);
-                if (x < tmax && accept(h, x, rec, ti)) {
-                    h = {x, ti, uu, vv, rec, __float_as_uint(lv0.w)};
-                    if (mode != kTraceClosest) return h;
-                }
+                const bool take = x < tmax && accept(h, x, rec, ti);
+                take_hit(h, take, x, ti, uu, vv, rec, __float_as_uint(lv0.w));
+                if (take && mode != kTraceClosest) return h;
             }
         }
         if (rec != kNone && st.sp == base) {
@@ -382,10 +394,9 @@ __device__ HitRec trace_wide(const TraceArgs &a, F3 o, F3 d, float tmax_in, uint
                 const float4 lv0 = l[0], lv1 = l[1], lv2 = l[2];
                 float x, uu, vv;
                 intersect_tri(ro, rd, lv0, lv1, lv2, tmax, x, uu, vv);
-                if (x < tmax && accept(h, x, rec, ti)) {
-                    h = {x, ti, uu, vv, rec, __float_as_uint(lv0.w)};
-                    if (mode != kTraceClosest) return h;
-                }
+                const bool take = x < tmax && accept(h, x, rec, ti);
+                take_hit(h, take, x, ti, uu, vv, rec, __float_as_uint(lv0.w));
+                if (take && mode != kTraceClosest) return h;
             }
         }
         if (rec != kNone && st.sp == base) {
