@@ -215,13 +215,15 @@ def test_samples_split_across_calls_equal_one_call(device):
 def test_exact_ties_keep_their_own_section(device):
     """Four instances of one soup at one place: instance 0 places the mesh whose triangles are in a plain section,
     instances 1-3 a copy whose triangles are in an emissive section. Every ray that meets the soup meets an exact
-    tie, won by instance 0, so no emission may show; the winner's section must come with it whichever instance the
-    walk met first. (Written as a branch around the hit update and inlined into the path tracer, the compiler once
+    tie, won by instance 0, and the copies' boxes (stretched by an off-screen triangle toward the camera) are entered
+    first, so the winner replaces an earlier hit on an exact tie; the winner's section must come with it. (Written as a branch around the hit update and inlined into the path tracer, the compiler once
     kept the replaced hit's section.)"""
     print("exact ties: 2000 triangles x 4 stacked instances, 64x64 x 4 spp", flush=True)
     tris, _ = S.clustered_soup(2000, 12, clusters=4, extent=10.0, median_area=0.05)
     plain = R.RMesh(tris, sections=np.zeros(len(tris), np.uint32))
-    glowing = R.RMesh(tris, sections=np.ones(len(tris), np.uint32))
+    # the copies carry one more, off-screen triangle toward the camera: their boxes are entered first
+    far = np.array([[[8.0, 8.0, -12.0], [8.5, 8.0, -12.0], [8.0, 8.5, -12.0]]], np.float32)
+    glowing = R.RMesh(np.concatenate([tris, far]), sections=np.ones(len(tris) + 1, np.uint32))
     eye = np.eye(4, dtype=np.float32)
     rs = R.build([plain, glowing], [R.RInstance(0, eye, [0, 1])] + [R.RInstance(1, eye, [0, 1]) for _ in range(3)],
                  [R.material((0.7, 0.7, 0.7)), R.material((0.0, 0.0, 0.0), (40.0, 40.0, 40.0))],
@@ -233,3 +235,30 @@ def test_exact_ties_keep_their_own_section(device):
     same = (got.view(np.uint32) == want.view(np.uint32)).all(axis=2).mean()
     print(f"ties: {same:.4f} of the pixels bit-equal, rel RMSE {rel_rmse(got, want):.2e}", flush=True)
     assert same == 1.0 and rel_rmse(got, want) <= 1e-3
+
+
+@pytest.mark.gpu
+def test_the_serving_job_that_met_a_tie_matches_the_oracle(device):
+    """The serving workload's job 163 (a 2.94M-triangle clustered scene, 1788 x 1026, 4 bounces, seed 164) is where
+    a miscompiled hit update first showed: at samples 20, 32 and 48 a bounce met an exact tie and kept the replaced
+    hit's section, and the path read its 'own' emitter past the end of the emitter list. Whole rows through those
+    pixels must equal the oracle's, sample by sample."""
+    from threert_amd import serve
+
+    print("serving job 163: scene (2940646, 1003)", flush=True)
+    rs, cam, sky = serve.soup_scene(2940646, 1003)
+    job = [j for j in serve.workload() if j.id == 163][0]
+    p = serve.params_of(serve._Resident(None, rs, cam, sky, 0), job)
+    sc = R.upload(device, rs)
+    orc = oracle_scene(rs)
+    for s, rows in ((20, (194,)), (32, (223, 311)), (48, (131,))):
+        acc = R.Accum(device, job.width, job.height)
+        R.render(sc, p, s, 1, acc)
+        got = acc.read()
+        acc.close()
+        for y in rows:
+            want = orc.render(p, s, 1, None, rows=(y, y + 1), threads=8)[y]
+            bad = int((got[y].view(np.uint32) != want.view(np.uint32)).any(axis=1).sum())
+            print(f"sample {s} row {y}: {bad} pixels differ", flush=True)
+            assert bad == 0
+    sc.close()

@@ -10,6 +10,7 @@ accumulator, and the launches' sums are added on the host in float64 (an FP64-ac
 from __future__ import annotations
 
 import argparse
+import json
 import math
 import time
 
@@ -148,6 +149,61 @@ def train(dev: Device, net: UNet, inputs: np.ndarray, targets: np.ndarray, steps
     return losses
 
 
+def spp_curve(dev: Device, kind: str, seed: int, w: int, h: int, spps=(1, 4, 16, 64), bounces: int = 4):
+    """Color of plain renders at each spp in `spps` (sample indices 200000 on), for the fallback comparison."""
+    rs, cam, sky = scene_of(kind, seed)
+    p = rs.params(w, h, cam, seed=seed, max_bounces=bounces, rr_depth=3, sky=sky)
+    sc = R.upload(dev, rs)
+    out = {}
+    for n in spps:
+        acc = R.Accum(dev, w, h)
+        R.render(sc, p, 200_000, n, acc)
+        out[n] = acc.resolve()[0]
+        acc.close()
+    sc.close()
+    return out
+
+
+def evaluate(dev: Device, net: UNet, scenes, w: int = 1280, h: int = 720, target_spp: int = 1024, log=print,
+             timing_reps: int = 20) -> dict:
+    """Per held-out scene: relative RMSE to the FP64-accumulated target of the 1-sample input, the a-trous baseline
+    (denoise.rs's stage), the U-Net, and plain renders of 4 / 16 / 64 samples; and the U-Net's ms a frame at w x h."""
+    from .denoise import atrous
+
+    rows = []
+    for kind, seed in scenes:
+        inp, tgt = render_set(dev, kind, seed, w, h, 1, target_spp)
+        color, albedo, normal = inp[0, ..., 0:3], inp[0, ..., 3:6], inp[0, ..., 6:9]
+        den = net.denoise(color, albedo, normal)
+        var = np.zeros((h, w), np.float32)  # one sample: the a-trous luminance weight sees no variance estimate
+        at = atrous(dev, color, albedo, normal, var)
+        curve = spp_curve(dev, kind, seed, w, h)
+        row = {"scene": f"{kind}-{seed}", "noisy_1spp": rel_rmse(color, tgt), "atrous": rel_rmse(at, tgt),
+               "unet": rel_rmse(den, tgt)}
+        for n, img in curve.items():
+            row[f"plain_{n}spp"] = rel_rmse(img, tgt)
+        rows.append(row)
+        log(f"eval: {row}")
+    # inference time at w x h: inputs resident in HBM, HIP events on the library's stream
+    c = np.ascontiguousarray(color)
+    bufs = {k: dev.alloc(c.nbytes) for k in ("c", "a", "n", "o")}
+    bufs["c"].upload(color)
+    bufs["a"].upload(albedo)
+    bufs["n"].upload(normal)
+    net.denoise_device(1, h, w, bufs["c"], bufs["a"], bufs["n"], bufs["o"])
+    dev.sync()
+    dev.timer_start()
+    for _ in range(timing_reps):
+        net.denoise_device(1, h, w, bufs["c"], bufs["a"], bufs["n"], bufs["o"])
+    ms = dev.timer_stop() / timing_reps
+    for b in bufs.values():
+        b.free()
+    keys = [k for k in rows[0] if k != "scene"]
+    mean = {k: float(np.mean([r[k] for r in rows])) for k in keys}
+    return {"scenes": rows, "mean": mean, "ms_per_frame": round(ms, 3), "width": w, "height": h,
+            "target_spp": target_spp}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--scenes", type=int, default=200)
@@ -158,6 +214,10 @@ def main():
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--out", default="unet_weights.npy")
+    ap.add_argument("--eval-scenes", type=int, default=3)
+    ap.add_argument("--eval-width", type=int, default=1280)
+    ap.add_argument("--eval-height", type=int, default=720)
+    ap.add_argument("--report", default="")
     args = ap.parse_args()
     dev = Device(0)
     ins, tgs = [], []
@@ -168,9 +228,20 @@ def main():
         tgs.append(t)
         if k % 10 == 0:
             print(f"data: {k + 1}/{args.scenes} scenes ({time.perf_counter() - t0:.1f} s)", flush=True)
-    net = UNet(dev, 2 * args.width * args.height, init_params(0))
-    train(dev, net, np.stack(ins), np.stack(tgs), args.steps, args.lr, log=lambda s: print(s, flush=True))
+    net = UNet(dev, max(2 * args.width * args.height, args.eval_width * args.eval_height), init_params(0))
+    t1 = time.perf_counter()
+    losses = train(dev, net, np.stack(ins), np.stack(tgs), args.steps, args.lr, log=lambda s: print(s, flush=True))
+    train_s = time.perf_counter() - t1
     np.save(args.out, net.params())
+    held_out = scene_list(args.eval_scenes, first_seed=10_000)
+    rep = evaluate(dev, net, held_out, args.eval_width, args.eval_height, args.target_spp,
+                   log=lambda s: print(s, flush=True))
+    rep.update({"train_scenes": args.scenes, "train_size": [args.width, args.height], "noisy_per_scene": args.noisy,
+                "steps": args.steps, "train_seconds": round(train_s, 1), "losses": losses})
+    print("report:", json.dumps(rep), flush=True)
+    if args.report:
+        with open(args.report, "w") as f:
+            json.dump(rep, f, indent=1)
     net.close()
 
 
