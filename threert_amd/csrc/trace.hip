@@ -1,5 +1,9 @@
 // Ray-batch traversal kernels for gfx950 (wave64): the walks of walk.h over a batch of rays in HBM.
-// Persistent: a wave takes 64 rays at a time from a counter until the batch is done.
+// Binary walk: persistent waves take 64 rays at a time from a counter and walk them to the end.
+// Wide walk: a lane whose ray is finished takes a new one while the rest of its wave walks on. Incoherent rays differ
+// several-fold in length (on the 3M-triangle workload a wave's longest ray takes about four times its mean), so a
+// wave that waited for its longest ray would leave three lanes in four idle; instead, when at least kRefill lanes
+// are idle (or all are), the wave takes that many ray indices with one atomic and hands them out by lane rank.
 #include <hip/hip_runtime.h>
 
 #include "trace_kernels.h"
@@ -41,13 +45,65 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
     }
 }
 
+constexpr uint32_t kRefill = 16;
+
+template <bool kCount>
+__global__ __launch_bounds__(kTraceBlock) void k_trace_dyn(TraceArgs a) {
+    __shared__ uint32_t lds[kLdsStack * kTraceBlock];
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    Stack st{lds + threadIdx.x, a.spill + gtid, a.spill_stride, 0, kLdsStack + a.spill_entries};
+    uint32_t flags = 0;
+    Reads cnt;
+    WideWalk w;
+    uint64_t ray = 0;
+    bool active = false, exhausted = false;
+    const uint64_t below = (uint64_t{1} << lane) - 1;
+    for (;;) {
+        const uint64_t idle = __ballot(!active && !exhausted);
+        const uint64_t busy = __ballot(active);
+        if (idle != 0 && (__popcll(idle) >= kRefill || busy == 0)) {
+            uint32_t first = 0;
+            if (lane == 0) first = atomicAdd(a.counter, uint32_t(__popcll(idle)));
+            first = __builtin_amdgcn_readfirstlane(__shfl(first, 0, 64));
+            if (!active && !exhausted) {
+                const uint64_t i = uint64_t(first) + uint64_t(__popcll(idle & below));
+                if (i < a.n) {
+                    ray = i;
+                    const float4 r0 = a.rays[2 * i], r1 = a.rays[2 * i + 1];
+                    if (wide_begin(a, F3{r0.x, r0.y, r0.z}, F3{r1.x, r1.y, r1.z}, r0.w, __float_as_uint(r1.w), w, st))
+                        active = true;
+                    else
+                        store_hit(a.hits, i, w.h);
+                } else {
+                    exhausted = true;
+                }
+            }
+        }
+        if (__ballot(active) == 0) {
+            if (__ballot(!exhausted) == 0) break;
+            continue;
+        }
+        if (active && wide_step<kCount>(a, w, st, flags, cnt)) {
+            store_hit(a.hits, ray, w.h);
+            active = false;
+        }
+    }
+    if (flags) atomicOr(a.guard_flag, flags);
+    if (kCount) {
+        atomicAdd(a.reads + 0, (unsigned long long)cnt.nodes);
+        atomicAdd(a.reads + 1, (unsigned long long)cnt.tris);
+        atomicAdd(a.reads + 2, (unsigned long long)cnt.units);
+    }
+}
+
 __global__ void k_reset_counter(uint32_t *c) { *c = 0; }
 
 }  // namespace
 
 int trace_occupancy(bool wide, int *blocks_per_cu) {
     int b = 0;
-    hipError_t e = wide ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_trace<true, false>, kTraceBlock, 0)
+    hipError_t e = wide ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_trace_dyn<false>, kTraceBlock, 0)
                         : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_trace<false, false>, kTraceBlock, 0);
     *blocks_per_cu = b;
     return e == hipSuccess ? 0 : -1;
@@ -56,9 +112,9 @@ int trace_occupancy(bool wide, int *blocks_per_cu) {
 int launch_trace(const TraceArgs &a, bool wide, bool count, int grid, hipStream_t s) {
     hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, s, a.counter);
     if (wide && count)
-        hipLaunchKernelGGL((k_trace<true, true>), dim3(grid), dim3(kTraceBlock), 0, s, a);
+        hipLaunchKernelGGL((k_trace_dyn<true>), dim3(grid), dim3(kTraceBlock), 0, s, a);
     else if (wide)
-        hipLaunchKernelGGL((k_trace<true, false>), dim3(grid), dim3(kTraceBlock), 0, s, a);
+        hipLaunchKernelGGL((k_trace_dyn<false>), dim3(grid), dim3(kTraceBlock), 0, s, a);
     else if (count)
         hipLaunchKernelGGL((k_trace<false, true>), dim3(grid), dim3(kTraceBlock), 0, s, a);
     else

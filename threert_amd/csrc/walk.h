@@ -274,139 +274,165 @@ __device__ __forceinline__ uint32_t pick(uint32_t c, uint32_t a0, uint32_t a1, u
 }
 __device__ __forceinline__ float qbyte(uint32_t w, int c) { return float((w >> (8 * c)) & 0xffu); }
 
-template <bool kCount>
-__device__ HitRec trace_wide(const TraceArgs &a, F3 o, F3 d, float tmax_in, uint32_t mode, Stack &st,
-                             uint32_t &flags, Reads &cnt) {
-    const float tmax = tmax_in < kInf ? tmax_in : kInf;
-    HitRec h{tmax, kNone, 0.0f, 0.0f, kNone, 0};
-    if (a.units == 0) return h;
-    const F3 inv = inv3(d);
-    if (ray_aabb(o, inv, {a.top_box[0], a.top_box[1], a.top_box[2]}, {a.top_box[3], a.top_box[4], a.top_box[5]},
-                 cull_t(h.t)) == kInf)
-        return h;
+// A wide walk in flight: the ray in world space and in the current unit's space, the node to visit, the unit being
+// walked (its record, first triangle, and the stack depth at its entry) and the best hit so far.
+struct WideWalk {
+    F3 o, d, inv, ro, rd, rinv;
+    float tmax;
+    uint32_t mode, node, rec, first, base, guard;
+    HitRec h;
+};
+
+// Starts a walk; false when it is already finished (no units, or the ray misses the top box): w.h is the answer.
+__device__ __forceinline__ bool wide_begin(const TraceArgs &a, F3 o, F3 d, float tmax_in, uint32_t mode, WideWalk &w,
+                                           Stack &st) {
+    w.tmax = tmax_in < kInf ? tmax_in : kInf;
+    w.mode = mode;
+    w.h = HitRec{w.tmax, kNone, 0.0f, 0.0f, kNone, 0};
+    if (a.units == 0) return false;
+    w.o = o, w.d = d, w.inv = inv3(d);
+    if (ray_aabb(o, w.inv, {a.top_box[0], a.top_box[1], a.top_box[2]}, {a.top_box[3], a.top_box[4], a.top_box[5]},
+                 cull_t(w.h.t)) == kInf)
+        return false;
     st.sp = 0;
-    uint32_t node = a.top_ref, rec = kNone, first = 0, base = 0;
-    F3 ro = o, rd = d, rinv = inv;
-    uint32_t guard = 0;
-    for (;;) {
-        if (++guard > kGuardSteps) {
-            flags |= kFlagGuard;
-            return h;
+    w.node = a.top_ref, w.rec = kNone, w.first = 0, w.base = 0, w.guard = 0;
+    w.ro = o, w.rd = d, w.rinv = w.inv;
+    return true;
+}
+
+// One step of the walk: a node's four children tested and the nearer ones queued, a unit entered, or a run of
+// triangles tested; then the next node from the stack. True when the walk is finished (w.h is the answer).
+template <bool kCount>
+__device__ __forceinline__ bool wide_step(const TraceArgs &a, WideWalk &w, Stack &st, uint32_t &flags, Reads &cnt) {
+    if (++w.guard > kGuardSteps) {
+        flags |= kFlagGuard;
+        return true;
+    }
+    const uint32_t node = w.node;
+    const uint32_t kind = node >> kRefKindShift;
+    if (kind == kRefNode) {
+        if (kCount) cnt.nodes++;
+        if (node >= a.wnode_count) {
+            flags |= kFlagWalk;
+            report_range(a.guard_flag, 7, node, a.wnode_count);
+            return true;
         }
-        const uint32_t kind = node >> kRefKindShift;
-        if (kind == kRefNode) {
-            if (kCount) cnt.nodes++;
-            if (node >= a.wnode_count) {
-                flags |= kFlagWalk;
-                report_range(a.guard_flag, 7, node, a.wnode_count);
-                return h;
-            }
-            const uint4 *p = a.wnodes + 4 * uint64_t(node);
-            const uint4 w0 = p[0], w1 = p[1], w2 = p[2], w3 = p[3];
-            const float sx = __uint_as_float((w0.w & 0xffu) << 23), sy = __uint_as_float(((w0.w >> 8) & 0xffu) << 23),
-                        sz = __uint_as_float(((w0.w >> 16) & 0xffu) << 23);
-            const float bx = (__uint_as_float(w0.x) - ro.x) * rinv.x, by = (__uint_as_float(w0.y) - ro.y) * rinv.y,
-                        bz = (__uint_as_float(w0.z) - ro.z) * rinv.z;
-            const float ix = sx * rinv.x, iy = sy * rinv.y, iz = sz * rinv.z;
-            const uint32_t nx = rinv.x >= 0.0f ? w1.x : w1.w, fx = rinv.x >= 0.0f ? w1.w : w1.x;
-            const uint32_t ny = rinv.y >= 0.0f ? w1.y : w2.x, fy = rinv.y >= 0.0f ? w2.x : w1.y;
-            const uint32_t nz = rinv.z >= 0.0f ? w1.z : w2.y, fz = rinv.z >= 0.0f ? w2.y : w1.z;
-            const float cull = cull_t(h.t);
-            uint32_t key[4];
+        const uint4 *p = a.wnodes + 4 * uint64_t(node);
+        const uint4 w0 = p[0], w1 = p[1], w2 = p[2], w3 = p[3];
+        const float sx = __uint_as_float((w0.w & 0xffu) << 23), sy = __uint_as_float(((w0.w >> 8) & 0xffu) << 23),
+                    sz = __uint_as_float(((w0.w >> 16) & 0xffu) << 23);
+        const F3 ro = w.ro, rinv = w.rinv;
+        const float bx = (__uint_as_float(w0.x) - ro.x) * rinv.x, by = (__uint_as_float(w0.y) - ro.y) * rinv.y,
+                    bz = (__uint_as_float(w0.z) - ro.z) * rinv.z;
+        const float ix = sx * rinv.x, iy = sy * rinv.y, iz = sz * rinv.z;
+        const uint32_t nx = rinv.x >= 0.0f ? w1.x : w1.w, fx = rinv.x >= 0.0f ? w1.w : w1.x;
+        const uint32_t ny = rinv.y >= 0.0f ? w1.y : w2.x, fy = rinv.y >= 0.0f ? w2.x : w1.y;
+        const uint32_t nz = rinv.z >= 0.0f ? w1.z : w2.y, fz = rinv.z >= 0.0f ? w2.y : w1.z;
+        const float cull = cull_t(w.h.t);
+        uint32_t key[4];
 #pragma unroll
-            for (int c = 0; c < 4; c++) {
-                const float tnx = fmaf(qbyte(nx, c), ix, bx), tny = fmaf(qbyte(ny, c), iy, by),
-                            tnz = fmaf(qbyte(nz, c), iz, bz);
-                const float tfx = fmaf(qbyte(fx, c), ix, bx), tfy = fmaf(qbyte(fy, c), iy, by),
-                            tfz = fmaf(qbyte(fz, c), iz, bz);
-                const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, 0.0f));
-                const float tf = fminf(fminf(tfx, tfy), fminf(tfz, cull));
-                key[c] = tn <= tf ? ((__float_as_uint(tn) & ~3u) | uint32_t(c)) : kNone;
-            }
-            if (w2.z == kNone) key[0] = kNone;
-            if (w2.w == kNone) key[1] = kNone;
-            if (w3.x == kNone) key[2] = kNone;
-            if (w3.y == kNone) key[3] = kNone;
+        for (int c = 0; c < 4; c++) {
+            const float tnx = fmaf(qbyte(nx, c), ix, bx), tny = fmaf(qbyte(ny, c), iy, by),
+                        tnz = fmaf(qbyte(nz, c), iz, bz);
+            const float tfx = fmaf(qbyte(fx, c), ix, bx), tfy = fmaf(qbyte(fy, c), iy, by),
+                        tfz = fmaf(qbyte(fz, c), iz, bz);
+            const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, 0.0f));
+            const float tf = fminf(fminf(tfx, tfy), fminf(tfz, cull));
+            key[c] = tn <= tf ? ((__float_as_uint(tn) & ~3u) | uint32_t(c)) : kNone;
+        }
+        if (w2.z == kNone) key[0] = kNone;
+        if (w2.w == kNone) key[1] = kNone;
+        if (w3.x == kNone) key[2] = kNone;
+        if (w3.y == kNone) key[3] = kNone;
 #define TRT_CSWAP(i, j)                          \
     {                                            \
         const uint32_t lo_ = min(key[i], key[j]); \
         key[j] = max(key[i], key[j]);            \
         key[i] = lo_;                            \
     }
-            TRT_CSWAP(0, 1);
-            TRT_CSWAP(2, 3);
-            TRT_CSWAP(0, 2);
-            TRT_CSWAP(1, 3);
-            TRT_CSWAP(1, 2);
+        TRT_CSWAP(0, 1);
+        TRT_CSWAP(2, 3);
+        TRT_CSWAP(0, 2);
+        TRT_CSWAP(1, 3);
+        TRT_CSWAP(1, 2);
 #undef TRT_CSWAP
-            if (key[0] != kNone) {
-                const uint32_t pushes =
-                    uint32_t(key[1] != kNone) + uint32_t(key[2] != kNone) + uint32_t(key[3] != kNone);
-                if (st.sp + pushes > st.cap) {
-                    flags |= kFlagWalk;
-                    report_range(a.guard_flag, 8, st.sp + pushes, st.cap);
-                    return h;
-                }
-                if (key[3] != kNone) st.push(pick(key[3] & 3u, w2.z, w2.w, w3.x, w3.y));
-                if (key[2] != kNone) st.push(pick(key[2] & 3u, w2.z, w2.w, w3.x, w3.y));
-                if (key[1] != kNone) st.push(pick(key[1] & 3u, w2.z, w2.w, w3.x, w3.y));
-                node = pick(key[0] & 3u, w2.z, w2.w, w3.x, w3.y);
-                continue;
-            }
-        } else if (kind == kRefUnit) {
-            const uint32_t r = node & 0x3fffffffu;
-            if (kCount) cnt.units++;
-            if (r >= a.wunit_count) {
+        if (key[0] != kNone) {
+            const uint32_t pushes =
+                uint32_t(key[1] != kNone) + uint32_t(key[2] != kNone) + uint32_t(key[3] != kNone);
+            if (st.sp + pushes > st.cap) {
                 flags |= kFlagWalk;
-                report_range(a.guard_flag, 9, r, a.wunit_count);
-                return h;
+                report_range(a.guard_flag, 8, st.sp + pushes, st.cap);
+                return true;
             }
-            const float4 *u = a.wunits + 4 * uint64_t(r);
-            const float4 u0 = u[0];
-            const uint32_t flags = __float_as_uint(u0.w);
-            if (mode != kTraceShadow || (flags & kUnitShadow) != 0) {
-                const float4 u1 = u[1], u2 = u[2], u3 = u[3];
-                rec = r;
-                ro = {o.x - u0.x, o.y - u0.y, o.z - u0.z};
-                if ((flags & kUnitShifted) == 0) {
-                    const F3 c0{u1.x, u1.y, u1.z}, c1{u2.x, u2.y, u2.z}, c2{u3.x, u3.y, u3.z};
-                    ro = proc_dir(c0, c1, c2, ro);
-                    rd = proc_dir(c0, c1, c2, d);
-                    rinv = inv3(rd);
-                }
-                first = __float_as_uint(u2.w);
-                base = st.sp;
-                node = __float_as_uint(u1.w);
-                continue;
-            }
-        } else {
-            const uint32_t count = ((node >> kRefTriFirstBits) & 15u) + 1u;
-            const uint32_t f = first + (node & kRefTriFirstMask);
-            if (kind != kRefTris || uint64_t(first) + (node & kRefTriFirstMask) + count > a.leaf_count) {
-                flags |= kFlagWalk;
-                report_range(a.guard_flag, 10, (uint64_t(kind) << 32) | (uint64_t(first) + (node & kRefTriFirstMask) + count), a.leaf_count);
-                return h;
-            }
-            for (uint32_t k = 0; k < count; k++) {
-                const uint32_t ti = f + k;
-                if (kCount) cnt.tris++;
-                const float4 *l = a.leaves + 3 * uint64_t(ti);
-                const float4 lv0 = l[0], lv1 = l[1], lv2 = l[2];
-                float x, uu, vv;
-                intersect_tri(ro, rd, lv0, lv1, lv2, tmax, x, uu, vv);
-                const bool take = x < tmax && accept(h, x, rec, ti);
-                take_hit(h, take, x, ti, uu, vv, rec, __float_as_uint(lv0.w));
-                if (take && mode != kTraceClosest) return h;
-            }
+            if (key[3] != kNone) st.push(pick(key[3] & 3u, w2.z, w2.w, w3.x, w3.y));
+            if (key[2] != kNone) st.push(pick(key[2] & 3u, w2.z, w2.w, w3.x, w3.y));
+            if (key[1] != kNone) st.push(pick(key[1] & 3u, w2.z, w2.w, w3.x, w3.y));
+            w.node = pick(key[0] & 3u, w2.z, w2.w, w3.x, w3.y);
+            return false;
         }
-        if (rec != kNone && st.sp == base) {
-            rec = kNone;
-            ro = o, rd = d, rinv = inv;
+    } else if (kind == kRefUnit) {
+        const uint32_t r = node & 0x3fffffffu;
+        if (kCount) cnt.units++;
+        if (r >= a.wunit_count) {
+            flags |= kFlagWalk;
+            report_range(a.guard_flag, 9, r, a.wunit_count);
+            return true;
         }
-        if (st.sp == 0) break;
-        node = st.pop();
+        const float4 *u = a.wunits + 4 * uint64_t(r);
+        const float4 u0 = u[0];
+        const uint32_t uflags = __float_as_uint(u0.w);
+        if (w.mode != kTraceShadow || (uflags & kUnitShadow) != 0) {
+            const float4 u1 = u[1], u2 = u[2], u3 = u[3];
+            w.rec = r;
+            w.ro = {w.o.x - u0.x, w.o.y - u0.y, w.o.z - u0.z};
+            if ((uflags & kUnitShifted) == 0) {
+                const F3 c0{u1.x, u1.y, u1.z}, c1{u2.x, u2.y, u2.z}, c2{u3.x, u3.y, u3.z};
+                w.ro = proc_dir(c0, c1, c2, w.ro);
+                w.rd = proc_dir(c0, c1, c2, w.d);
+                w.rinv = inv3(w.rd);
+            }
+            w.first = __float_as_uint(u2.w);
+            w.base = st.sp;
+            w.node = __float_as_uint(u1.w);
+            return false;
+        }
+    } else {
+        const uint32_t count = ((node >> kRefTriFirstBits) & 15u) + 1u;
+        const uint32_t f = w.first + (node & kRefTriFirstMask);
+        if (kind != kRefTris || uint64_t(w.first) + (node & kRefTriFirstMask) + count > a.leaf_count) {
+            flags |= kFlagWalk;
+            report_range(a.guard_flag, 10, (uint64_t(kind) << 32) | (uint64_t(w.first) + (node & kRefTriFirstMask) + count), a.leaf_count);
+            return true;
+        }
+        for (uint32_t k = 0; k < count; k++) {
+            const uint32_t ti = f + k;
+            if (kCount) cnt.tris++;
+            const float4 *l = a.leaves + 3 * uint64_t(ti);
+            const float4 lv0 = l[0], lv1 = l[1], lv2 = l[2];
+            float x, uu, vv;
+            intersect_tri(w.ro, w.rd, lv0, lv1, lv2, w.tmax, x, uu, vv);
+            const bool take = x < w.tmax && accept(w.h, x, w.rec, ti);
+            take_hit(w.h, take, x, ti, uu, vv, w.rec, __float_as_uint(lv0.w));
+            if (take && w.mode != kTraceClosest) return true;
+        }
     }
-    return h;
+    if (w.rec != kNone && st.sp == w.base) {
+        w.rec = kNone;
+        w.ro = w.o, w.rd = w.d, w.rinv = w.inv;
+    }
+    if (st.sp == 0) return true;
+    w.node = st.pop();
+    return false;
+}
+
+template <bool kCount>
+__device__ HitRec trace_wide(const TraceArgs &a, F3 o, F3 d, float tmax_in, uint32_t mode, Stack &st,
+                             uint32_t &flags, Reads &cnt) {
+    WideWalk w;
+    if (!wide_begin(a, o, d, tmax_in, mode, w, st)) return w.h;
+    while (!wide_step<kCount>(a, w, st, flags, cnt)) {
+    }
+    return w.h;
 }
 
 }  // namespace walk

@@ -52,6 +52,13 @@ def init_params(seed: int = 0) -> np.ndarray:
     return p
 
 
+def shipped_params() -> np.ndarray:
+    """The weights trained by denoise_train.py on 200 seeded scenes (threert_amd/data/unet_weights.npy)."""
+    import os
+
+    return np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "unet_weights.npy"))
+
+
 def _sig():
     L = lib()
     P, U32, U64, I = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int
