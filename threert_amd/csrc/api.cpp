@@ -73,7 +73,8 @@ struct trt_device {
 
 struct trt_scene {
     trt_device *dev = nullptr;
-    void *nodes = nullptr, *leaves = nullptr, *instances = nullptr, *wnodes = nullptr, *wunits = nullptr;
+    void *nodes = nullptr, *leaves = nullptr, *instances = nullptr, *wnodes = nullptr, *wunits = nullptr,
+         *wleaves = nullptr;
     // shading tables (trt_render)
     void *vertices = nullptr, *indices = nullptr, *rows = nullptr, *materials = nullptr, *lights = nullptr,
          *emitters = nullptr, *emitter_tris = nullptr, *objects = nullptr, *energy = nullptr;
@@ -346,6 +347,7 @@ int trt_scene_release(trt_scene *s) {
     hipFree(s->instances);
     hipFree(s->wnodes);
     hipFree(s->wunits);
+    hipFree(s->wleaves);
     for (void *p : {s->vertices, s->indices, s->rows, s->materials, s->lights, s->emitters, s->emitter_tris,
                     s->objects, s->energy})
         hipFree(p);
@@ -518,8 +520,8 @@ int trt_scene_upload(trt_device *d, const trt_scene_desc *desc, trt_scene **out)
     const auto *inst = static_cast<const trt::Instance *>(desc->instances);
     trt::WideScene w;
     std::string why;
-    if (!trt::build_wide(nodes, desc->node_count, inst, desc->instance_count, desc->leaf_count, desc->top_first,
-                         desc->unit_count, w, &why))
+    if (!trt::build_wide(nodes, desc->node_count, static_cast<const trt::Leaf *>(desc->leaves), inst,
+                         desc->instance_count, desc->leaf_count, desc->top_first, desc->unit_count, w, &why))
         return fail(TRT_E_INVALID, "trt_scene_upload: " + why);
     if (w.nodes.size() >= (uint64_t{1} << 30)) return fail(TRT_E_INVALID, "trt_scene_upload: tree too large");
     if (desc->energy && !check_shading(desc, &why)) return fail(TRT_E_INVALID, "trt_scene_upload: " + why);
@@ -535,6 +537,7 @@ int trt_scene_upload(trt_device *d, const trt_scene_desc *desc, trt_scene **out)
     if (!rc) rc = upload(d, &s->instances, desc->instances, desc->instance_count * sizeof(trt::Instance), &total);
     if (!rc) rc = upload(d, &s->wnodes, w.nodes.data(), w.nodes.size() * sizeof(trt::WideNode), &total);
     if (!rc) rc = upload(d, &s->wunits, w.units.data(), w.units.size() * sizeof(trt::WideUnit), &total);
+    if (!rc) rc = upload(d, &s->wleaves, w.wleaves.data(), w.wleaves.size() * sizeof(trt::Leaf), &total);
     if (!rc && desc->energy) {
         s->shading = true;
         s->light_count = desc->light_count;
@@ -614,6 +617,7 @@ static int launch(trt_scene *s, const void *d_rays, uint64_t n, void *d_hits, ui
     a.wnodes = static_cast<const uint4 *>(s->wnodes);
     a.wunits = static_cast<const float4 *>(s->wunits);
     a.leaves = static_cast<const float4 *>(s->leaves);
+    a.wleaves = static_cast<const float4 *>(s->wleaves);
     a.bnodes = static_cast<const float4 *>(s->nodes);
     a.instances = static_cast<const float4 *>(s->instances);
     a.rays = static_cast<const float4 *>(d_rays);
@@ -753,6 +757,7 @@ static int job_args(const trt_render_job &j, trt::RenderArgs &a, std::string *wh
     a.geo.wnodes = static_cast<const uint4 *>(s->wnodes);
     a.geo.wunits = static_cast<const float4 *>(s->wunits);
     a.geo.leaves = static_cast<const float4 *>(s->leaves);
+    a.geo.wleaves = static_cast<const float4 *>(s->wleaves);
     a.geo.top_ref = s->top_ref;
     a.geo.top_first = s->top_first;
     a.geo.units = s->units;

@@ -21,7 +21,8 @@ constexpr uint32_t kFlagTable = 4;   // a shading-table index outside its table
 struct TraceArgs {
     const uint4 *wnodes;      // WideNode, 4 x uint4 each
     const float4 *wunits;     // WideUnit, 4 x float4 each
-    const float4 *leaves;     // Leaf, 3 x float4 each
+    const float4 *leaves;     // Leaf, 3 x float4 each, the reference's order (the binary walk, shading)
+    const float4 *wleaves;    // the same leaves in the wide trees' order, e2.w = the reference index
     const float4 *bnodes;     // Node, 2 x float4 each
     const float4 *instances;  // Instance, 10 x float4 each
     const float4 *rays;       // trt_ray, 2 x float4 each

@@ -108,6 +108,8 @@ static_assert(sizeof(WideUnit) == 64, "a wide unit is 64 bytes");
 struct WideScene {
     std::vector<WideNode> nodes;
     std::vector<WideUnit> units;
+    // The leaves in the wide trees' own order, each mesh's within its range; e2.w holds the reference leaf index.
+    std::vector<Leaf> wleaves;
     uint32_t top_ref = kNone;
     uint32_t max_stack = 0;
     float top_box[6] = {0, 0, 0, 0, 0, 0};
@@ -125,7 +127,7 @@ bool top_tree(const float *boxes, const uint32_t *records, uint64_t n, Tree &out
 void unit_record(Instance &r, uint32_t root, bool cast_shadow);
 
 // The wide tree of an uploaded scene (wide.cpp).
-bool build_wide(const Node *nodes, uint64_t node_count, const Instance *inst, uint64_t inst_count,
+bool build_wide(const Node *nodes, uint64_t node_count, const Leaf *leaves, const Instance *inst, uint64_t inst_count,
                 uint64_t leaf_count, uint32_t top_first, uint32_t unit_count, WideScene &out, std::string *why);
 
 }  // namespace trt

@@ -405,10 +405,10 @@ __device__ __forceinline__ bool wide_step(const TraceArgs &a, WideWalk &w, Stack
             return true;
         }
         for (uint32_t k = 0; k < count; k++) {
-            const uint32_t ti = f + k;
             if (kCount) cnt.tris++;
-            const float4 *l = a.leaves + 3 * uint64_t(ti);
+            const float4 *l = a.wleaves + 3 * uint64_t(f + k);
             const float4 lv0 = l[0], lv1 = l[1], lv2 = l[2];
+            const uint32_t ti = __float_as_uint(lv2.w);  // the reference leaf index: what hits report and ties order
             float x, uu, vv;
             intersect_tri(w.ro, w.rd, lv0, lv1, lv2, w.tmax, x, uu, vv);
             const bool take = x < w.tmax && accept(w.h, x, w.rec, ti);

@@ -1,9 +1,15 @@
-// The library's own node format: the reference's binary trees (bvh.rs GpuNode, units.rs Layout) collapsed into
-// 4-wide nodes of 64 bytes with 8-bit child bounds against an FP32 origin and a power-of-two step per axis.
+// The library's own node format: 4-wide nodes of 64 bytes with 8-bit child bounds against an FP32 origin and a
+// power-of-two step per axis.
+//
+// Mesh trees: per mesh, the library builds its own binary SAH tree over the mesh's triangles (16 bins on each axis,
+// leaves of at most kOwnLeaf triangles) and lays the mesh's leaves out in that tree's order in a copy of the leaf
+// buffer (`wleaves`), each carrying its reference leaf index in e2.w: the walk reports and breaks ties by that index,
+// so hits are the reference's whatever the layout. (The reference's trees allow leaves of 16, which costs the
+// incoherent workload about 23 triangle tests a ray.) The top tree is the reference's.
 //
 // Collapse: a binary interior node becomes a wide node whose children are found by opening, while fewer than four,
-// the child of largest surface area that is itself interior. Binary leaves stay leaves (a run of <= 16 triangles,
-// or one unit in the top tree), so triangle positions, and with them hit ids, are the reference's.
+// the child of largest surface area that is itself interior. Binary leaves stay leaves (a run of triangles, or one
+// unit in the top tree).
 //
 // Quantisation is conservative: origin = (node low - one step) rounded down; each child's low byte is floored and
 // lowered by one step, its high byte ceiled and raised by one, checked in exact (double) arithmetic. The extra step
@@ -176,12 +182,191 @@ bool check_walkable(const Node *nodes, uint64_t count, uint32_t root, uint32_t o
     return true;
 }
 
+constexpr uint32_t kOwnLeaf = 4;   // triangles a leaf of the library's mesh trees holds at most
+constexpr int kOwnBins = 16;
+constexpr uint32_t kOwnDepth = 64;  // past this depth a node is split at its median
+
+// The number of triangles a reference mesh tree covers (its leaves name positions 0 .. n-1 past the mesh's first).
+uint32_t mesh_size(const Node *nodes, uint32_t root) {
+    uint32_t n = 0;
+    std::vector<uint32_t> st{root};
+    while (!st.empty()) {
+        const Node &x = nodes[st.back()];
+        st.pop_back();
+        if (x.count > 0) {
+            n = std::max(n, x.left_or_first + x.count);
+        } else {
+            st.push_back(root + x.left_or_first);
+            st.push_back(root + x.left_or_first + 1);
+        }
+    }
+    return n;
+}
+
+inline float down(double v) {
+    float f = static_cast<float>(v);
+    if (double(f) > v) f = std::nextafter(f, -INFINITY);
+    return f;
+}
+inline float up(double v) {
+    float f = static_cast<float>(v);
+    if (double(f) < v) f = std::nextafter(f, INFINITY);
+    return f;
+}
+
+// The library's binary SAH tree over the triangles leaves[first .. first + n): nodes[0] is the root, children at
+// left_or_first and left_or_first + 1 (absolute), leaves name positions in `order` (relative to first), whose
+// entries are positions in the reference order. A triangle whose corners are not finite can never be hit and is
+// left out of the tree (it keeps a place in `order`).
+struct OwnTree {
+    std::vector<Node> nodes;
+    std::vector<uint32_t> order;
+
+    struct Box {
+        float lo[3], hi[3];
+    };
+    static float area(const Box &b) {
+        const float x = b.hi[0] - b.lo[0], y = b.hi[1] - b.lo[1], z = b.hi[2] - b.lo[2];
+        return 2.0f * (x * y + y * z + z * x);
+    }
+    static void grow(Box &b, const Box &o) {
+        for (int k = 0; k < 3; k++) b.lo[k] = std::min(b.lo[k], o.lo[k]), b.hi[k] = std::max(b.hi[k], o.hi[k]);
+    }
+    static Box empty() { return Box{{INFINITY, INFINITY, INFINITY}, {-INFINITY, -INFINITY, -INFINITY}}; }
+
+    void build(const Leaf *leaves, uint32_t first, uint32_t n) {
+        std::vector<Box> box(n);
+        std::vector<float> cen(3 * size_t(n));
+        std::vector<uint32_t> live, dead;
+        live.reserve(n);
+        for (uint32_t i = 0; i < n; i++) {
+            const Leaf &l = leaves[uint64_t(first) + i];
+            bool finite = true;
+            for (int k = 0; k < 3; k++) {
+                const double v0 = l.r[0][k], v1 = v0 + double(l.r[1][k]), v2 = v0 + double(l.r[2][k]);
+                if (!std::isfinite(v0) || !std::isfinite(v1) || !std::isfinite(v2)) finite = false;
+                box[i].lo[k] = down(std::min(v0, std::min(v1, v2)));
+                box[i].hi[k] = up(std::max(v0, std::max(v1, v2)));
+                cen[3 * i + k] = 0.5f * box[i].lo[k] + 0.5f * box[i].hi[k];
+            }
+            (finite ? live : dead).push_back(i);
+        }
+        nodes.clear();
+        nodes.reserve(2 * size_t(live.size()) / 2 + 2);
+        order = live;
+        nodes.emplace_back();
+        if (live.empty()) {
+            // nothing can be hit: one leaf over the first triangle, with a point box
+            order = dead;
+            Node &r = nodes[0];
+            for (int k = 0; k < 3; k++) r.bmin[k] = r.bmax[k] = 0.0f;
+            r.left_or_first = 0;
+            r.count = 1;
+            return;
+        }
+        struct Item {
+            uint32_t node, b, e, depth;
+        };
+        std::vector<Item> st{{0, 0, uint32_t(live.size()), 0}};
+        while (!st.empty()) {
+            const Item it = st.back();
+            st.pop_back();
+            Box nb = empty(), cb = empty();
+            for (uint32_t j = it.b; j < it.e; j++) {
+                const uint32_t p = order[j];
+                grow(nb, box[p]);
+                const Box c{{cen[3 * p], cen[3 * p + 1], cen[3 * p + 2]}, {cen[3 * p], cen[3 * p + 1], cen[3 * p + 2]}};
+                grow(cb, c);
+            }
+            Node &nd = nodes[it.node];
+            for (int k = 0; k < 3; k++) nd.bmin[k] = nb.lo[k], nd.bmax[k] = nb.hi[k];
+            const uint32_t cnt = it.e - it.b;
+            const float leaf_cost = float(cnt);
+            int best_axis = -1, best_bin = -1;
+            float best_cost = INFINITY;
+            if (cnt > 1 && it.depth < kOwnDepth) {
+                for (int k = 0; k < 3; k++) {
+                    const float ext = cb.hi[k] - cb.lo[k];
+                    if (!(ext > 0.0f)) continue;
+                    Box bins[kOwnBins];
+                    uint32_t counts[kOwnBins] = {};
+                    for (auto &x : bins) x = empty();
+                    const float scale = float(kOwnBins) / ext;
+                    for (uint32_t j = it.b; j < it.e; j++) {
+                        const uint32_t p = order[j];
+                        const int bi = std::min(kOwnBins - 1, int((cen[3 * p + k] - cb.lo[k]) * scale));
+                        counts[bi]++;
+                        grow(bins[bi], box[p]);
+                    }
+                    float right_area[kOwnBins];
+                    uint32_t right_count[kOwnBins];
+                    Box acc = empty();
+                    uint32_t ac = 0;
+                    for (int bi = kOwnBins - 1; bi > 0; bi--) {
+                        grow(acc, bins[bi]);
+                        ac += counts[bi];
+                        right_area[bi] = ac ? area(acc) : 0.0f;
+                        right_count[bi] = ac;
+                    }
+                    acc = empty();
+                    ac = 0;
+                    for (int bi = 0; bi < kOwnBins - 1; bi++) {
+                        grow(acc, bins[bi]);
+                        ac += counts[bi];
+                        if (ac == 0 || right_count[bi + 1] == 0) continue;
+                        const float cost = area(acc) * float(ac) + right_area[bi + 1] * float(right_count[bi + 1]);
+                        if (cost < best_cost) best_cost = cost, best_axis = k, best_bin = bi;
+                    }
+                }
+            }
+            const float na = area(nb);
+            const float split_cost = best_axis >= 0 ? 1.0f + (na > 0.0f ? best_cost / na : float(cnt)) : INFINITY;
+            if (cnt <= kOwnLeaf && (cnt == 1 || !(split_cost < leaf_cost))) {
+                nd.left_or_first = it.b;
+                nd.count = cnt;
+                continue;
+            }
+            uint32_t mid;
+            if (best_axis >= 0) {
+                const int k = best_axis;
+                const float scale = float(kOwnBins) / (cb.hi[k] - cb.lo[k]);
+                const float lo = cb.lo[k];
+                auto pm = std::partition(order.begin() + it.b, order.begin() + it.e, [&](uint32_t p) {
+                    return std::min(kOwnBins - 1, int((cen[3 * p + k] - lo) * scale)) <= best_bin;
+                });
+                mid = uint32_t(pm - order.begin());
+            } else {
+                // all centroids equal, or too deep: split the run at its median along the widest centroid axis
+                int k = 0;
+                for (int a = 1; a < 3; a++)
+                    if (cb.hi[a] - cb.lo[a] > cb.hi[k] - cb.lo[k]) k = a;
+                mid = it.b + cnt / 2;
+                std::nth_element(order.begin() + it.b, order.begin() + mid, order.begin() + it.e,
+                                 [&](uint32_t x, uint32_t y) {
+                                     return cen[3 * x + k] < cen[3 * y + k] || (cen[3 * x + k] == cen[3 * y + k] && x < y);
+                                 });
+            }
+            if (mid == it.b || mid == it.e) mid = it.b + cnt / 2;
+            const uint32_t l = static_cast<uint32_t>(nodes.size());
+            nodes.emplace_back();
+            nodes.emplace_back();
+            nodes[it.node].left_or_first = l;
+            nodes[it.node].count = 0;
+            st.push_back({l, it.b, mid, it.depth + 1});
+            st.push_back({l + 1, mid, it.e, it.depth + 1});
+        }
+        for (uint32_t p : dead) order.push_back(p);
+    }
+};
+
 }  // namespace
 
-bool build_wide(const Node *nodes, uint64_t node_count, const Instance *inst, uint64_t inst_count,
+bool build_wide(const Node *nodes, uint64_t node_count, const Leaf *leaves, const Instance *inst, uint64_t inst_count,
                 uint64_t leaf_count, uint32_t top_first, uint32_t unit_count, WideScene &out, std::string *why) {
     out = WideScene();
     out.units.assign(inst_count, WideUnit{});
+    out.wleaves.assign(leaves, leaves + leaf_count);
+    for (uint64_t i = 0; i < leaf_count; i++) out.wleaves[i].r[2][3] = from_bits(static_cast<uint32_t>(i));
     if (unit_count == 0) return true;
     if (top_first >= node_count) {
         *why = "the top tree's root is outside the node buffer";
@@ -215,10 +400,19 @@ bool build_wide(const Node *nodes, uint64_t node_count, const Instance *inst, ui
         if (it == mesh.end()) {
             if (!check_walkable(nodes, node_count, root, root, kMeshDepth, false, leaf_count, u.info[0], nullptr, why))
                 return false;
+            const uint32_t first = u.info[0], n = mesh_size(nodes, root);
+            OwnTree own;
+            own.build(leaves, first, n);
+            for (uint32_t j = 0; j < n; j++) {
+                Leaf l = leaves[uint64_t(first) + own.order[j]];
+                l.r[2][3] = from_bits(first + own.order[j]);
+                out.wleaves[uint64_t(first) + j] = l;
+            }
+            Collapser mc{own.nodes.data(), own.nodes.size(), out};
             uint32_t b = 0;
-            const uint32_t ref = col.convert(root, root, false, b);
-            if (!col.err.empty()) {
-                *why = col.err;
+            const uint32_t ref = mc.convert(0, 0, false, b);
+            if (!mc.err.empty()) {
+                *why = mc.err;
                 return false;
             }
             it = mesh.emplace(root, std::make_pair(ref, b)).first;
