@@ -274,8 +274,19 @@ __device__ __forceinline__ uint32_t pick(uint32_t c, uint32_t a0, uint32_t a1, u
 }
 __device__ __forceinline__ float qbyte(uint32_t w, int c) { return float((w >> (8 * c)) & 0xffu); }
 
-// A wide walk in flight: the ray in world space and in the current unit's space, the node to visit, the unit being
-// walked (its record, first triangle, and the stack depth at its entry) and the best hit so far.
+// A wide walk in flight: the ray in world space and in the current unit's space, the next reference to visit
+// (kWalkDone once the walk is over), the unit being walked (its record, first triangle, and the stack depth at its
+// entry) and the best hit so far.
+//
+// A wave runs its walks in two alternating phases. In the node phase, lanes whose next reference is a node or a
+// unit take steps while lanes that have reached a run of triangles wait; once a set share of the walking lanes wait
+// (or no lane has a node left), the leaf phase tests every waiting lane's triangles at once. Mixing the two
+// in every step would run the triangle loop, for a lane or two, on nearly every step of the wave. Each lane still
+// visits its references in the order of a lone walk, so the hits and the reads are a lone walk's.
+constexpr uint32_t kWalkDone = 0xffffffffu;  // kind 3: never a reference
+constexpr uint32_t kLeafShareBatch = 4;  // ray batches (trace.hip): the leaf phase once a quarter of the lanes wait
+constexpr uint32_t kLeafShareCall = 2;   // trace_wide (the path tracer's rays): once half of them wait
+
 struct WideWalk {
     F3 o, d, inv, ro, rd, rinv;
     float tmax;
@@ -283,13 +294,22 @@ struct WideWalk {
     HitRec h;
 };
 
+__device__ __forceinline__ uint32_t ref_kind(uint32_t r) { return r >> kRefKindShift; }
+// Next reference is a node or a unit: the lane takes node-phase steps.
+__device__ __forceinline__ bool wide_at_inner(const WideWalk &w) {
+    return ref_kind(w.node) == kRefNode || ref_kind(w.node) == kRefUnit;
+}
+__device__ __forceinline__ bool wide_at_leaf(const WideWalk &w) { return ref_kind(w.node) == kRefTris; }
+// The walk is over (w.node holds kWalkDone, the only value of kind 3 it can hold).
+__device__ __forceinline__ bool wide_over(const WideWalk &w) { return ref_kind(w.node) == 3u; }
+
 // Starts a walk; false when it is already finished (no units, or the ray misses the top box): w.h is the answer.
 __device__ __forceinline__ bool wide_begin(const TraceArgs &a, F3 o, F3 d, float tmax_in, uint32_t mode, WideWalk &w,
                                            Stack &st) {
     w.tmax = tmax_in < kInf ? tmax_in : kInf;
     w.mode = mode;
     w.h = HitRec{w.tmax, kNone, 0.0f, 0.0f, kNone, 0};
-    if (a.units == 0) return false;
+    if (a.units == 0 || ref_kind(a.top_ref) == 3u) return false;
     w.o = o, w.d = d, w.inv = inv3(d);
     if (ray_aabb(o, w.inv, {a.top_box[0], a.top_box[1], a.top_box[2]}, {a.top_box[3], a.top_box[4], a.top_box[5]},
                  cull_t(w.h.t)) == kInf)
@@ -300,22 +320,33 @@ __device__ __forceinline__ bool wide_begin(const TraceArgs &a, F3 o, F3 d, float
     return true;
 }
 
-// One step of the walk: a node's four children tested and the nearer ones queued, a unit entered, or a run of
-// triangles tested; then the next node from the stack. True when the walk is finished (w.h is the answer).
+// Moves past a finished node or leaf: leaves the unit when the stack is back at its entry depth, then pops, or ends
+// the walk when the stack is empty.
+__device__ __forceinline__ void wide_next(WideWalk &w, Stack &st) {
+    if (w.rec != kNone && st.sp == w.base) {
+        w.rec = kNone;
+        w.ro = w.o, w.rd = w.d, w.rinv = w.inv;
+    }
+    w.node = st.sp == 0 ? kWalkDone : st.pop();
+}
+
+// One node-phase step (wide_at_inner holds): a node's four children tested and the nearer ones queued, or a unit
+// entered.
 template <bool kCount>
-__device__ __forceinline__ bool wide_step(const TraceArgs &a, WideWalk &w, Stack &st, uint32_t &flags, Reads &cnt) {
+__device__ __forceinline__ void wide_inner(const TraceArgs &a, WideWalk &w, Stack &st, uint32_t &flags, Reads &cnt) {
     if (++w.guard > kGuardSteps) {
         flags |= kFlagGuard;
-        return true;
+        w.node = kWalkDone;
+        return;
     }
     const uint32_t node = w.node;
-    const uint32_t kind = node >> kRefKindShift;
-    if (kind == kRefNode) {
+    if (ref_kind(node) == kRefNode) {
         if (kCount) cnt.nodes++;
         if (node >= a.wnode_count) {
             flags |= kFlagWalk;
             report_range(a.guard_flag, 7, node, a.wnode_count);
-            return true;
+            w.node = kWalkDone;
+            return;
         }
         const uint4 *p = a.wnodes + 4 * uint64_t(node);
         const uint4 w0 = p[0], w1 = p[1], w2 = p[2], w3 = p[3];
@@ -329,6 +360,7 @@ __device__ __forceinline__ bool wide_step(const TraceArgs &a, WideWalk &w, Stack
         const uint32_t ny = rinv.y >= 0.0f ? w1.y : w2.x, fy = rinv.y >= 0.0f ? w2.x : w1.y;
         const uint32_t nz = rinv.z >= 0.0f ? w1.z : w2.y, fz = rinv.z >= 0.0f ? w2.y : w1.z;
         const float cull = cull_t(w.h.t);
+        const uint32_t child[4] = {w2.z, w2.w, w3.x, w3.y};
         uint32_t key[4];
 #pragma unroll
         for (int c = 0; c < 4; c++) {
@@ -338,12 +370,9 @@ __device__ __forceinline__ bool wide_step(const TraceArgs &a, WideWalk &w, Stack
                         tfz = fmaf(qbyte(fz, c), iz, bz);
             const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, 0.0f));
             const float tf = fminf(fminf(tfx, tfy), fminf(tfz, cull));
-            key[c] = tn <= tf ? ((__float_as_uint(tn) & ~3u) | uint32_t(c)) : kNone;
+            // an empty slot holds kNone; no reference is of kind 3, so a walk never queues one
+            key[c] = tn <= tf && ref_kind(child[c]) != 3u ? ((__float_as_uint(tn) & ~3u) | uint32_t(c)) : kNone;
         }
-        if (w2.z == kNone) key[0] = kNone;
-        if (w2.w == kNone) key[1] = kNone;
-        if (w3.x == kNone) key[2] = kNone;
-        if (w3.y == kNone) key[3] = kNone;
 #define TRT_CSWAP(i, j)                          \
     {                                            \
         const uint32_t lo_ = min(key[i], key[j]); \
@@ -357,32 +386,53 @@ __device__ __forceinline__ bool wide_step(const TraceArgs &a, WideWalk &w, Stack
         TRT_CSWAP(1, 2);
 #undef TRT_CSWAP
         if (key[0] != kNone) {
-            const uint32_t pushes =
-                uint32_t(key[1] != kNone) + uint32_t(key[2] != kNone) + uint32_t(key[3] != kNone);
-            if (st.sp + pushes > st.cap) {
-                flags |= kFlagWalk;
-                report_range(a.guard_flag, 8, st.sp + pushes, st.cap);
-                return true;
+            // the children to queue are key[1..np]; they go on the stack farthest first, so the second nearest is
+            // on top
+            const uint32_t np = uint32_t(key[1] != kNone) + uint32_t(key[2] != kNone) + uint32_t(key[3] != kNone);
+            if (st.sp + 3 <= kLdsStack) {
+                // all three slots in LDS: store the three candidates unconditionally, count only np of them
+                const uint32_t k0 = np == 3 ? key[3] : (np == 2 ? key[2] : key[1]);
+                const uint32_t k1 = np == 3 ? key[2] : key[1];
+                uint32_t *top = st.lds + st.sp * kTraceBlock;
+                top[0] = pick(k0 & 3u, child[0], child[1], child[2], child[3]);
+                top[kTraceBlock] = pick(k1 & 3u, child[0], child[1], child[2], child[3]);
+                top[2 * kTraceBlock] = pick(key[1] & 3u, child[0], child[1], child[2], child[3]);
+                st.sp += np;
+            } else {
+                if (st.sp + np > st.cap) {
+                    flags |= kFlagWalk;
+                    report_range(a.guard_flag, 8, st.sp + np, st.cap);
+                    w.node = kWalkDone;
+                    return;
+                }
+                if (key[3] != kNone) st.push(pick(key[3] & 3u, child[0], child[1], child[2], child[3]));
+                if (key[2] != kNone) st.push(pick(key[2] & 3u, child[0], child[1], child[2], child[3]));
+                if (key[1] != kNone) st.push(pick(key[1] & 3u, child[0], child[1], child[2], child[3]));
             }
-            if (key[3] != kNone) st.push(pick(key[3] & 3u, w2.z, w2.w, w3.x, w3.y));
-            if (key[2] != kNone) st.push(pick(key[2] & 3u, w2.z, w2.w, w3.x, w3.y));
-            if (key[1] != kNone) st.push(pick(key[1] & 3u, w2.z, w2.w, w3.x, w3.y));
-            w.node = pick(key[0] & 3u, w2.z, w2.w, w3.x, w3.y);
-            return false;
+            w.node = pick(key[0] & 3u, child[0], child[1], child[2], child[3]);
+            return;
         }
-    } else if (kind == kRefUnit) {
+    } else {
         const uint32_t r = node & 0x3fffffffu;
         if (kCount) cnt.units++;
         if (r >= a.wunit_count) {
             flags |= kFlagWalk;
             report_range(a.guard_flag, 9, r, a.wunit_count);
-            return true;
+            w.node = kWalkDone;
+            return;
         }
         const float4 *u = a.wunits + 4 * uint64_t(r);
         const float4 u0 = u[0];
         const uint32_t uflags = __float_as_uint(u0.w);
         if (w.mode != kTraceShadow || (uflags & kUnitShadow) != 0) {
             const float4 u1 = u[1], u2 = u[2], u3 = u[3];
+            const uint32_t root = __float_as_uint(u1.w);
+            if (ref_kind(root) == 3u) {
+                flags |= kFlagWalk;
+                report_range(a.guard_flag, 10, root, a.wnode_count);
+                w.node = kWalkDone;
+                return;
+            }
             w.rec = r;
             w.ro = {w.o.x - u0.x, w.o.y - u0.y, w.o.z - u0.z};
             if ((uflags & kUnitShifted) == 0) {
@@ -393,36 +443,47 @@ __device__ __forceinline__ bool wide_step(const TraceArgs &a, WideWalk &w, Stack
             }
             w.first = __float_as_uint(u2.w);
             w.base = st.sp;
-            w.node = __float_as_uint(u1.w);
-            return false;
-        }
-    } else {
-        const uint32_t count = ((node >> kRefTriFirstBits) & 15u) + 1u;
-        const uint32_t f = w.first + (node & kRefTriFirstMask);
-        if (kind != kRefTris || uint64_t(w.first) + (node & kRefTriFirstMask) + count > a.leaf_count) {
-            flags |= kFlagWalk;
-            report_range(a.guard_flag, 10, (uint64_t(kind) << 32) | (uint64_t(w.first) + (node & kRefTriFirstMask) + count), a.leaf_count);
-            return true;
-        }
-        for (uint32_t k = 0; k < count; k++) {
-            if (kCount) cnt.tris++;
-            const float4 *l = a.wleaves + 3 * uint64_t(f + k);
-            const float4 lv0 = l[0], lv1 = l[1], lv2 = l[2];
-            const uint32_t ti = __float_as_uint(lv2.w);  // the reference leaf index: what hits report and ties order
-            float x, uu, vv;
-            intersect_tri(w.ro, w.rd, lv0, lv1, lv2, w.tmax, x, uu, vv);
-            const bool take = x < w.tmax && accept(w.h, x, w.rec, ti);
-            take_hit(w.h, take, x, ti, uu, vv, w.rec, __float_as_uint(lv0.w));
-            if (take && w.mode != kTraceClosest) return true;
+            w.node = root;
+            return;
         }
     }
-    if (w.rec != kNone && st.sp == w.base) {
-        w.rec = kNone;
-        w.ro = w.o, w.rd = w.d, w.rinv = w.inv;
+    wide_next(w, st);
+}
+
+// One leaf step (wide_at_leaf holds): the run's triangles tested, then on to the next reference. Shadow and any-hit
+// walks end at the first hit they take.
+template <bool kCount>
+__device__ __forceinline__ void wide_leaf(const TraceArgs &a, WideWalk &w, Stack &st, uint32_t &flags, Reads &cnt) {
+    if (++w.guard > kGuardSteps) {
+        flags |= kFlagGuard;
+        w.node = kWalkDone;
+        return;
     }
-    if (st.sp == 0) return true;
-    w.node = st.pop();
-    return false;
+    const uint32_t node = w.node;
+    const uint32_t count = ((node >> kRefTriFirstBits) & 15u) + 1u;
+    const uint64_t f = uint64_t(w.first) + (node & kRefTriFirstMask);
+    if (f + count > a.leaf_count) {
+        flags |= kFlagWalk;
+        report_range(a.guard_flag, 10, f + count, a.leaf_count);
+        w.node = kWalkDone;
+        return;
+    }
+    bool taken = false;
+    for (uint32_t k = 0; k < count; k++) {
+        if (kCount) cnt.tris++;
+        const float4 *l = a.wleaves + 3 * (f + k);
+        const float4 lv0 = l[0], lv1 = l[1], lv2 = l[2];
+        const uint32_t ti = __float_as_uint(lv2.w);  // the reference leaf index: what hits report and ties order
+        float x, uu, vv;
+        intersect_tri(w.ro, w.rd, lv0, lv1, lv2, w.tmax, x, uu, vv);
+        const bool take = x < w.tmax && accept(w.h, x, w.rec, ti);
+        take_hit(w.h, take, x, ti, uu, vv, w.rec, __float_as_uint(lv0.w));
+        taken = taken || take;
+    }
+    if (taken && w.mode != kTraceClosest)
+        w.node = kWalkDone;
+    else
+        wide_next(w, st);
 }
 
 template <bool kCount>
@@ -430,7 +491,16 @@ __device__ HitRec trace_wide(const TraceArgs &a, F3 o, F3 d, float tmax_in, uint
                              uint32_t &flags, Reads &cnt) {
     WideWalk w;
     if (!wide_begin(a, o, d, tmax_in, mode, w, st)) return w.h;
-    while (!wide_step<kCount>(a, w, st, flags, cnt)) {
+    // The lanes in this call walk their rays together (see WideWalk); a lane leaves when its walk is over.
+    for (;;) {
+        const uint32_t walking = __popcll(__ballot(true));
+        for (;;) {
+            const bool inner = wide_at_inner(w);
+            if (__ballot(inner) == 0 || kLeafShareCall * __popcll(__ballot(wide_at_leaf(w))) >= walking) break;
+            if (inner) wide_inner<kCount>(a, w, st, flags, cnt);
+        }
+        if (wide_at_leaf(w)) wide_leaf<kCount>(a, w, st, flags, cnt);
+        if (wide_over(w)) break;
     }
     return w.h;
 }
