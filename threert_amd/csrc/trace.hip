@@ -5,7 +5,7 @@
 // wave that waited for its longest ray would leave three lanes in four idle. Each lane holds its next ray already
 // loaded and starts it as soon as its walk ends; once kRefill lanes have used theirs (or no lane walks), the wave
 // takes that many ray indices with one atomic, hands them out by lane rank and loads the rays while it walks on.
-// Between refills the wave alternates node phases and leaf phases of its walks (walk.h WideWalk). On the
+// Between refills the wave alternates the two phases of its walks (walk.h WideWalk). On the
 // 3M-triangle workload, refilling every 16 lanes cost 18 % against every 24 (more stops of the whole wave), and
 // loading each ray only when its lane needed it cost 3 %.
 #include <hip/hip_runtime.h>
@@ -23,7 +23,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
     __shared__ uint32_t lds[kLdsStack * kTraceBlock];
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    Stack st{lds + threadIdx.x, a.spill + gtid, a.spill_stride, 0, kLdsStack + a.spill_entries};
+    Stack st{(TRT_LDS uint32_t *)(lds + threadIdx.x), a.spill + gtid, a.spill_stride, 0, kLdsStack + a.spill_entries};
     uint32_t flags = 0;
     Reads cnt;
     for (;;) {
@@ -56,7 +56,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_dyn(TraceArgs a) {
     __shared__ uint32_t lds[kLdsStack * kTraceBlock];
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    Stack st{lds + threadIdx.x, a.spill + gtid, a.spill_stride, 0, kLdsStack + a.spill_entries};
+    Stack st{(TRT_LDS uint32_t *)(lds + threadIdx.x), a.spill + gtid, a.spill_stride, 0, kLdsStack + a.spill_entries};
     uint32_t flags = 0;
     Reads cnt;
     WideWalk w;
@@ -99,25 +99,25 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_dyn(TraceArgs a) {
             if (__ballot(nidx != kNoRay || !exhausted) == 0) break;
             continue;
         }
-        // Node phase: until a kLeafShareBatch-th of the walking lanes wait at a leaf, no lane has a node left, or
-        // kRefill lanes have finished.
+        // Node phase: until a kLeafShareBatch-th of the walking lanes wait, no lane has a node left, or kRefill
+        // lanes have finished.
         const uint32_t walking = __popcll(busy);
         for (;;) {
-            const bool inner = active && wide_at_inner(w);
-            if (__ballot(inner) == 0 || kLeafShareBatch * __popcll(__ballot(active && wide_at_leaf(w))) >= walking ||
+            const bool at_node = active && wide_at_node(w);
+            if (__ballot(at_node) == 0 || kLeafShareBatch * __popcll(__ballot(active && wide_waits(w))) >= walking ||
                 __popcll(__ballot(!active && nidx != kNoRay)) >= kRefill)
                 break;
-            if (inner) {
-                wide_inner<kCount>(a, w, st, flags, cnt);
+            if (at_node) {
+                wide_node<kCount>(a, w, st, flags, cnt);
                 if (wide_over(w)) {
                     store_hit(a.hits, ray, w.h);
                     active = false;
                 }
             }
         }
-        // Leaf phase: every waiting lane tests its run of triangles.
-        if (active && wide_at_leaf(w)) {
-            wide_leaf<kCount>(a, w, st, flags, cnt);
+        // The other phase: every waiting lane tests its run of triangles, or enters or leaves a unit.
+        if (active && wide_waits(w)) {
+            wide_other<kCount>(a, w, st, flags, cnt);
             if (wide_over(w)) {
                 store_hit(a.hits, ray, w.h);
                 active = false;

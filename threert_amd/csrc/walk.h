@@ -114,8 +114,14 @@ __device__ __forceinline__ F3 proc_dir(F3 n0, F3 n1, F3 n2, F3 d) {
             n0.z * d.x + n1.z * d.y + n2.z * d.z};
 }
 
+// The LDS part of a stack is reached through an LDS-qualified pointer: with a generic one the compiler merges the
+// LDS and the spill read of a pop into one flat load, which waits for every memory access the wave has in flight.
+#ifndef TRT_LDS
+#define TRT_LDS __attribute__((address_space(3)))
+#endif
+
 struct Stack {
-    uint32_t *lds;    // this thread's entry 0; entry k at lds[k * kTraceBlock]
+    TRT_LDS uint32_t *lds;  // this thread's entry 0; entry k at lds[k * kTraceBlock]
     uint32_t *spill;  // this thread's spill entry 0; entry k at spill[k * stride]
     uint32_t stride;
     uint32_t sp;
@@ -269,22 +275,22 @@ __device__ HitRec trace_binary(const TraceArgs &a, F3 o, F3 d, float tmax_in, ui
 }
 
 // ---- the 4-wide quantised walk ----------------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t pick(uint32_t c, uint32_t a0, uint32_t a1, uint32_t a2, uint32_t a3) {
-    return c == 0 ? a0 : (c == 1 ? a1 : (c == 2 ? a2 : a3));
-}
 __device__ __forceinline__ float qbyte(uint32_t w, int c) { return float((w >> (8 * c)) & 0xffu); }
 
-// A wide walk in flight: the ray in world space and in the current unit's space, the next reference to visit
-// (kWalkDone once the walk is over), the unit being walked (its record, first triangle, and the stack depth at its
-// entry) and the best hit so far.
+// A wide walk in flight: the ray in world space and in the current unit's space, the next reference to visit or a
+// state of kind 3 (kWalkLeave, kWalkDone), the unit being walked (its record, first triangle, and the stack depth at
+// its entry) and the best hit so far.
 //
-// A wave runs its walks in two alternating phases. In the node phase, lanes whose next reference is a node or a
-// unit take steps while lanes that have reached a run of triangles wait; once a set share of the walking lanes wait
-// (or no lane has a node left), the leaf phase tests every waiting lane's triangles at once. Mixing the two
-// in every step would run the triangle loop, for a lane or two, on nearly every step of the wave. Each lane still
-// visits its references in the order of a lone walk, so the hits and the reads are a lone walk's.
-constexpr uint32_t kWalkDone = 0xffffffffu;  // kind 3: never a reference
-constexpr uint32_t kLeafShareBatch = 4;  // ray batches (trace.hip): the leaf phase once a quarter of the lanes wait
+// A wave runs its walks in two alternating phases. In the node phase, lanes whose next reference is a node test it
+// and move on, while lanes that have reached anything else wait: a run of triangles, a unit to enter, or the end of
+// a unit's tree (kWalkLeave). Once a set share of the walking lanes wait (or no lane has a node left), the other
+// phase serves every waiting lane at once. Mixing them in every step would run the triangle loop, for a lane or two,
+// on nearly every step of the wave; and with the ray's unit space fixed during the node phase, the node loop keeps
+// no copies of it. Each lane still visits its references in the order of a lone walk, so the hits and the reads are
+// a lone walk's.
+constexpr uint32_t kWalkDone = 0xffffffffu;   // the walk is over
+constexpr uint32_t kWalkLeave = 0xfffffffeu;  // the stack is back at the unit's entry depth: leave the unit, then pop
+constexpr uint32_t kLeafShareBatch = 3;  // ray batches (trace.hip): the other phase once a third of the lanes wait
 constexpr uint32_t kLeafShareCall = 2;   // trace_wide (the path tracer's rays): once half of them wait
 
 struct WideWalk {
@@ -295,13 +301,14 @@ struct WideWalk {
 };
 
 __device__ __forceinline__ uint32_t ref_kind(uint32_t r) { return r >> kRefKindShift; }
-// Next reference is a node or a unit: the lane takes node-phase steps.
-__device__ __forceinline__ bool wide_at_inner(const WideWalk &w) {
-    return ref_kind(w.node) == kRefNode || ref_kind(w.node) == kRefUnit;
+// The lane takes node-phase steps.
+__device__ __forceinline__ bool wide_at_node(const WideWalk &w) { return ref_kind(w.node) == kRefNode; }
+// The lane waits for the other phase: triangles, a unit to enter, or a unit to leave.
+__device__ __forceinline__ bool wide_waits(const WideWalk &w) {
+    return ref_kind(w.node) != kRefNode && w.node != kWalkDone;
 }
-__device__ __forceinline__ bool wide_at_leaf(const WideWalk &w) { return ref_kind(w.node) == kRefTris; }
-// The walk is over (w.node holds kWalkDone, the only value of kind 3 it can hold).
-__device__ __forceinline__ bool wide_over(const WideWalk &w) { return ref_kind(w.node) == 3u; }
+// The walk is over.
+__device__ __forceinline__ bool wide_over(const WideWalk &w) { return w.node == kWalkDone; }
 
 // Starts a walk; false when it is already finished (no units, or the ray misses the top box): w.h is the answer.
 __device__ __forceinline__ bool wide_begin(const TraceArgs &a, F3 o, F3 d, float tmax_in, uint32_t mode, WideWalk &w,
@@ -320,34 +327,31 @@ __device__ __forceinline__ bool wide_begin(const TraceArgs &a, F3 o, F3 d, float
     return true;
 }
 
-// Moves past a finished node or leaf: leaves the unit when the stack is back at its entry depth, then pops, or ends
-// the walk when the stack is empty.
-__device__ __forceinline__ void wide_next(WideWalk &w, Stack &st) {
-    if (w.rec != kNone && st.sp == w.base) {
-        w.rec = kNone;
-        w.ro = w.o, w.rd = w.d, w.rinv = w.inv;
-    }
-    w.node = st.sp == 0 ? kWalkDone : st.pop();
+// Moves past a finished node or leaf: kWalkLeave when the stack is back at the unit's entry depth, else the next
+// reference from the stack, or the end of the walk.
+__device__ __forceinline__ void wide_pop(WideWalk &w, Stack &st) {
+    if (w.rec != kNone && st.sp == w.base)
+        w.node = kWalkLeave;
+    else
+        w.node = st.sp == 0 ? kWalkDone : st.pop();
 }
 
-// One node-phase step (wide_at_inner holds): a node's four children tested and the nearer ones queued, or a unit
-// entered.
+// One node-phase step (wide_at_node holds): the node's four children tested and the nearer ones queued.
 template <bool kCount>
-__device__ __forceinline__ void wide_inner(const TraceArgs &a, WideWalk &w, Stack &st, uint32_t &flags, Reads &cnt) {
+__device__ __forceinline__ void wide_node(const TraceArgs &a, WideWalk &w, Stack &st, uint32_t &flags, Reads &cnt) {
+    const uint32_t node = w.node;
     if (++w.guard > kGuardSteps) {
         flags |= kFlagGuard;
         w.node = kWalkDone;
         return;
     }
-    const uint32_t node = w.node;
-    if (ref_kind(node) == kRefNode) {
-        if (kCount) cnt.nodes++;
-        if (node >= a.wnode_count) {
-            flags |= kFlagWalk;
-            report_range(a.guard_flag, 7, node, a.wnode_count);
-            w.node = kWalkDone;
-            return;
-        }
+    if (kCount) cnt.nodes++;
+    if (node >= a.wnode_count) {
+        flags |= kFlagWalk;
+        report_range(a.guard_flag, 7, node, a.wnode_count);
+        w.node = kWalkDone;
+        return;
+    }
         const uint4 *p = a.wnodes + 4 * uint64_t(node);
         const uint4 w0 = p[0], w1 = p[1], w2 = p[2], w3 = p[3];
         const float sx = __uint_as_float((w0.w & 0xffu) << 23), sy = __uint_as_float(((w0.w >> 8) & 0xffu) << 23),
@@ -360,8 +364,7 @@ __device__ __forceinline__ void wide_inner(const TraceArgs &a, WideWalk &w, Stac
         const uint32_t ny = rinv.y >= 0.0f ? w1.y : w2.x, fy = rinv.y >= 0.0f ? w2.x : w1.y;
         const uint32_t nz = rinv.z >= 0.0f ? w1.z : w2.y, fz = rinv.z >= 0.0f ? w2.y : w1.z;
         const float cull = cull_t(w.h.t);
-        const uint32_t child[4] = {w2.z, w2.w, w3.x, w3.y};
-        uint32_t key[4];
+        uint32_t ref[4] = {w2.z, w2.w, w3.x, w3.y}, key[4];
 #pragma unroll
         for (int c = 0; c < 4; c++) {
             const float tnx = fmaf(qbyte(nx, c), ix, bx), tny = fmaf(qbyte(ny, c), iy, by),
@@ -371,13 +374,15 @@ __device__ __forceinline__ void wide_inner(const TraceArgs &a, WideWalk &w, Stac
             const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, 0.0f));
             const float tf = fminf(fminf(tfx, tfy), fminf(tfz, cull));
             // an empty slot holds kNone; no reference is of kind 3, so a walk never queues one
-            key[c] = tn <= tf && ref_kind(child[c]) != 3u ? ((__float_as_uint(tn) & ~3u) | uint32_t(c)) : kNone;
+            key[c] = tn <= tf && ref_kind(ref[c]) != 3u ? ((__float_as_uint(tn) & ~3u) | uint32_t(c)) : kNone;
         }
-#define TRT_CSWAP(i, j)                          \
-    {                                            \
-        const uint32_t lo_ = min(key[i], key[j]); \
-        key[j] = max(key[i], key[j]);            \
-        key[i] = lo_;                            \
+        // nearest first; each reference moves with its key (selects only: a lookup by index compiles to branches)
+#define TRT_CSWAP(i, j)                                                          \
+    {                                                                            \
+        const bool s_ = key[j] < key[i];                                         \
+        const uint32_t ki_ = s_ ? key[j] : key[i], kj_ = s_ ? key[i] : key[j];   \
+        const uint32_t ri_ = s_ ? ref[j] : ref[i], rj_ = s_ ? ref[i] : ref[j];   \
+        key[i] = ki_, key[j] = kj_, ref[i] = ri_, ref[j] = rj_;                  \
     }
         TRT_CSWAP(0, 1);
         TRT_CSWAP(2, 3);
@@ -386,17 +391,19 @@ __device__ __forceinline__ void wide_inner(const TraceArgs &a, WideWalk &w, Stac
         TRT_CSWAP(1, 2);
 #undef TRT_CSWAP
         if (key[0] != kNone) {
-            // the children to queue are key[1..np]; they go on the stack farthest first, so the second nearest is
+            // the children to queue are ref[1..np]; they go on the stack farthest first, so the second nearest is
             // on top
             const uint32_t np = uint32_t(key[1] != kNone) + uint32_t(key[2] != kNone) + uint32_t(key[3] != kNone);
             if (st.sp + 3 <= kLdsStack) {
-                // all three slots in LDS: store the three candidates unconditionally, count only np of them
-                const uint32_t k0 = np == 3 ? key[3] : (np == 2 ? key[2] : key[1]);
-                const uint32_t k1 = np == 3 ? key[2] : key[1];
-                uint32_t *top = st.lds + st.sp * kTraceBlock;
-                top[0] = pick(k0 & 3u, child[0], child[1], child[2], child[3]);
-                top[kTraceBlock] = pick(k1 & 3u, child[0], child[1], child[2], child[3]);
-                top[2 * kTraceBlock] = pick(key[1] & 3u, child[0], child[1], child[2], child[3]);
+                // all three slots in LDS: store three entries unconditionally, count only np of them
+                uint32_t s0 = ref[1];
+                s0 = np == 2 ? ref[2] : s0;
+                s0 = np == 3 ? ref[3] : s0;
+                const uint32_t s1 = np == 3 ? ref[2] : ref[1];
+                TRT_LDS uint32_t *top = st.lds + st.sp * kTraceBlock;
+                top[0] = s0;
+                top[kTraceBlock] = s1;
+                top[2 * kTraceBlock] = ref[1];
                 st.sp += np;
             } else {
                 if (st.sp + np > st.cap) {
@@ -405,14 +412,33 @@ __device__ __forceinline__ void wide_inner(const TraceArgs &a, WideWalk &w, Stac
                     w.node = kWalkDone;
                     return;
                 }
-                if (key[3] != kNone) st.push(pick(key[3] & 3u, child[0], child[1], child[2], child[3]));
-                if (key[2] != kNone) st.push(pick(key[2] & 3u, child[0], child[1], child[2], child[3]));
-                if (key[1] != kNone) st.push(pick(key[1] & 3u, child[0], child[1], child[2], child[3]));
+                if (key[3] != kNone) st.push(ref[3]);
+                if (key[2] != kNone) st.push(ref[2]);
+                if (key[1] != kNone) st.push(ref[1]);
             }
-            w.node = pick(key[0] & 3u, child[0], child[1], child[2], child[3]);
+            w.node = ref[0];
             return;
         }
-    } else {
+    wide_pop(w, st);
+}
+
+// One step of the other phase (wide_waits holds): a run of triangles tested, a unit entered or left. Shadow and
+// any-hit walks end at the first hit they take.
+template <bool kCount>
+__device__ __forceinline__ void wide_other(const TraceArgs &a, WideWalk &w, Stack &st, uint32_t &flags, Reads &cnt) {
+    const uint32_t node = w.node;
+    if (++w.guard > kGuardSteps) {
+        flags |= kFlagGuard;
+        w.node = kWalkDone;
+        return;
+    }
+    if (node == kWalkLeave) {
+        w.rec = kNone;
+        w.ro = w.o, w.rd = w.d, w.rinv = w.inv;
+        w.node = st.sp == 0 ? kWalkDone : st.pop();
+        return;
+    }
+    if (ref_kind(node) == kRefUnit) {
         const uint32_t r = node & 0x3fffffffu;
         if (kCount) cnt.units++;
         if (r >= a.wunit_count) {
@@ -446,23 +472,12 @@ __device__ __forceinline__ void wide_inner(const TraceArgs &a, WideWalk &w, Stac
             w.node = root;
             return;
         }
-    }
-    wide_next(w, st);
-}
-
-// One leaf step (wide_at_leaf holds): the run's triangles tested, then on to the next reference. Shadow and any-hit
-// walks end at the first hit they take.
-template <bool kCount>
-__device__ __forceinline__ void wide_leaf(const TraceArgs &a, WideWalk &w, Stack &st, uint32_t &flags, Reads &cnt) {
-    if (++w.guard > kGuardSteps) {
-        flags |= kFlagGuard;
-        w.node = kWalkDone;
+        wide_pop(w, st);
         return;
     }
-    const uint32_t node = w.node;
     const uint32_t count = ((node >> kRefTriFirstBits) & 15u) + 1u;
     const uint64_t f = uint64_t(w.first) + (node & kRefTriFirstMask);
-    if (f + count > a.leaf_count) {
+    if (ref_kind(node) != kRefTris || f + count > a.leaf_count) {
         flags |= kFlagWalk;
         report_range(a.guard_flag, 10, f + count, a.leaf_count);
         w.node = kWalkDone;
@@ -483,7 +498,7 @@ __device__ __forceinline__ void wide_leaf(const TraceArgs &a, WideWalk &w, Stack
     if (taken && w.mode != kTraceClosest)
         w.node = kWalkDone;
     else
-        wide_next(w, st);
+        wide_pop(w, st);
 }
 
 template <bool kCount>
@@ -495,11 +510,11 @@ __device__ HitRec trace_wide(const TraceArgs &a, F3 o, F3 d, float tmax_in, uint
     for (;;) {
         const uint32_t walking = __popcll(__ballot(true));
         for (;;) {
-            const bool inner = wide_at_inner(w);
-            if (__ballot(inner) == 0 || kLeafShareCall * __popcll(__ballot(wide_at_leaf(w))) >= walking) break;
-            if (inner) wide_inner<kCount>(a, w, st, flags, cnt);
+            const bool at_node = wide_at_node(w);
+            if (__ballot(at_node) == 0 || kLeafShareCall * __popcll(__ballot(wide_waits(w))) >= walking) break;
+            if (at_node) wide_node<kCount>(a, w, st, flags, cnt);
         }
-        if (wide_at_leaf(w)) wide_leaf<kCount>(a, w, st, flags, cnt);
+        if (wide_waits(w)) wide_other<kCount>(a, w, st, flags, cnt);
         if (wide_over(w)) break;
     }
     return w.h;
