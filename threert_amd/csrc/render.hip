@@ -795,7 +795,7 @@ __global__ __launch_bounds__(kRenderBlock) void k_render(const RenderJob *jobs, 
     __shared__ uint32_t lds[kLdsStack * kRenderBlock];
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    Stack st{(TRT_LDS uint32_t *)(lds + threadIdx.x), spill + gtid, spill_stride, 0, kLdsStack + spill_entries};
+    Stack st{(TRT_LDS uint32_t *)(lds + threadIdx.x), spill, gtid, spill_stride, 0, kLdsStack + spill_entries};
     uint32_t flags = 0;
     for (;;) {
         uint32_t tile = 0;

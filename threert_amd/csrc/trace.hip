@@ -2,12 +2,12 @@
 // Binary walk: persistent waves take 64 rays at a time from a counter and walk them to the end.
 // Wide walk: a lane whose ray is finished takes a new one while the rest of its wave walks on. Incoherent rays differ
 // several-fold in length (on the 3M-triangle workload a wave's longest ray takes about four times its mean), so a
-// wave that waited for its longest ray would leave three lanes in four idle. Each lane holds its next ray already
-// loaded and starts it as soon as its walk ends; once kRefill lanes have used theirs (or no lane walks), the wave
-// takes that many ray indices with one atomic, hands them out by lane rank and loads the rays while it walks on.
-// Between refills the wave alternates the two phases of its walks (walk.h WideWalk). On the
-// 3M-triangle workload, refilling every 16 lanes cost 18 % against every 24 (more stops of the whole wave), and
-// loading each ray only when its lane needed it cost 3 %.
+// wave that waited for its longest ray would leave three lanes in four idle; instead, once kRefill lanes are idle
+// (or all are), the wave takes that many ray indices with one atomic and hands them out by lane rank. Between
+// refills the wave alternates the two phases of its walks (walk.h WideWalk). A refill stops the whole wave on the
+// new rays' loads, so refills are batched: on the 3M-triangle workload every 32 lanes ran 1.37x faster than every
+// 16 and 1.18x faster than every 48. Loading each lane's next ray ahead of time hid those loads but took eight more
+// registers, one wave per SIMD fewer (four instead of five), and was 6 % slower.
 #include <hip/hip_runtime.h>
 
 #include "trace_kernels.h"
@@ -23,7 +23,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
     __shared__ uint32_t lds[kLdsStack * kTraceBlock];
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    Stack st{(TRT_LDS uint32_t *)(lds + threadIdx.x), a.spill + gtid, a.spill_stride, 0, kLdsStack + a.spill_entries};
+    Stack st{(TRT_LDS uint32_t *)(lds + threadIdx.x), a.spill, gtid, a.spill_stride, 0, kLdsStack + a.spill_entries};
     uint32_t flags = 0;
     Reads cnt;
     for (;;) {
@@ -49,54 +49,45 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
     }
 }
 
-constexpr uint32_t kRefill = 24;
+constexpr uint32_t kRefill = 32;
 
 template <bool kCount>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_dyn(TraceArgs a) {
     __shared__ uint32_t lds[kLdsStack * kTraceBlock];
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    Stack st{(TRT_LDS uint32_t *)(lds + threadIdx.x), a.spill + gtid, a.spill_stride, 0, kLdsStack + a.spill_entries};
+    Stack st{(TRT_LDS uint32_t *)(lds + threadIdx.x), a.spill, gtid, a.spill_stride, 0, kLdsStack + a.spill_entries};
     uint32_t flags = 0;
     Reads cnt;
     WideWalk w;
-    uint64_t ray = 0;
-    // Each lane holds its next ray, loaded ahead (index nidx, data n0 and n1), so a lane whose walk ends starts
-    // the next one at once, and the wave never waits on a ray load.
-    constexpr uint32_t kNoRay = 0xffffffffu;
-    uint32_t nidx = kNoRay;
-    float4 n0{}, n1{};
+    uint32_t ray = 0;  // the host keeps batches under 2^31 rays
     bool active = false, exhausted = false;
     const uint64_t below = (uint64_t{1} << lane) - 1;
     for (;;) {
-        if (!active && nidx != kNoRay) {
-            ray = nidx;
-            nidx = kNoRay;
-            if (wide_begin(a, F3{n0.x, n0.y, n0.z}, F3{n1.x, n1.y, n1.z}, n0.w, __float_as_uint(n1.w), w, st))
-                active = true;
-            else
-                store_hit(a.hits, ray, w.h);
-        }
-        // lanes that used their loaded ray take new indices, kRefill at a time (or all, when no lane walks)
-        const uint64_t need = __ballot(nidx == kNoRay && !exhausted);
-        const uint64_t busy = __ballot(active);
-        if (need != 0 && (__popcll(need) >= kRefill || busy == 0)) {
+        // lanes without a walk take new rays, kRefill at a time (or all, when no lane walks)
+        const uint64_t idle = __ballot(!active && !exhausted);
+        uint64_t busy = __ballot(active);
+        if (idle != 0 && (__popcll(idle) >= kRefill || busy == 0)) {
             uint32_t first = 0;
-            if (lane == 0) first = atomicAdd(a.counter, uint32_t(__popcll(need)));
+            if (lane == 0) first = atomicAdd(a.counter, uint32_t(__popcll(idle)));
             first = __builtin_amdgcn_readfirstlane(__shfl(first, 0, 64));
-            if (nidx == kNoRay && !exhausted) {
-                const uint64_t i = uint64_t(first) + uint64_t(__popcll(need & below));
+            if (!active && !exhausted) {
+                const uint64_t i = uint64_t(first) + uint64_t(__popcll(idle & below));
                 if (i < a.n) {
-                    nidx = uint32_t(i);
-                    n0 = a.rays[2 * i];
-                    n1 = a.rays[2 * i + 1];
+                    ray = uint32_t(i);
+                    const float4 r0 = a.rays[2 * i], r1 = a.rays[2 * i + 1];
+                    if (wide_begin(a, F3{r0.x, r0.y, r0.z}, F3{r1.x, r1.y, r1.z}, r0.w, __float_as_uint(r1.w), w, st))
+                        active = true;
+                    else
+                        store_hit(a.hits, i, w.h);
                 } else {
                     exhausted = true;
                 }
             }
+            busy = __ballot(active);
         }
         if (busy == 0) {
-            if (__ballot(nidx != kNoRay || !exhausted) == 0) break;
+            if (__ballot(!exhausted) == 0) break;
             continue;
         }
         // Node phase: until a kLeafShareBatch-th of the walking lanes wait, no lane has a node left, or kRefill
@@ -105,7 +96,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_dyn(TraceArgs a) {
         for (;;) {
             const bool at_node = active && wide_at_node(w);
             if (__ballot(at_node) == 0 || kLeafShareBatch * __popcll(__ballot(active && wide_waits(w))) >= walking ||
-                __popcll(__ballot(!active && nidx != kNoRay)) >= kRefill)
+                __popcll(__ballot(!active && !exhausted)) >= kRefill)
                 break;
             if (at_node) {
                 wide_node<kCount>(a, w, st, flags, cnt);

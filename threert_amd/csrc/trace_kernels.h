@@ -10,7 +10,7 @@
 namespace trt {
 
 constexpr int kTraceBlock = 256;        // threads per block: four wave64s
-constexpr uint32_t kLdsStack = 16;      // stack entries per thread held in LDS (16 KiB a block)
+constexpr uint32_t kLdsStack = 24;      // stack entries per thread held in LDS (24 KiB a block)
 constexpr uint32_t kRefStack = 64;      // units.rs STACK: the binary walk drops pushes past it, as the reference
 constexpr uint32_t kGuardSteps = 1u << 24;  // a walk's iterations before it stops and reports (gpu.rs loop guard)
 // Bits of the device's guard word, each a reason a kernel stopped a walk or a path and reported instead of reading:

@@ -122,7 +122,8 @@ __device__ __forceinline__ F3 proc_dir(F3 n0, F3 n1, F3 n2, F3 d) {
 
 struct Stack {
     TRT_LDS uint32_t *lds;  // this thread's entry 0; entry k at lds[k * kTraceBlock]
-    uint32_t *spill;  // this thread's spill entry 0; entry k at spill[k * stride]
+    uint32_t *spill;        // the launch's spill region: this thread's entry k at spill[k * stride + slot]
+    uint32_t slot;
     uint32_t stride;
     uint32_t sp;
     uint32_t cap;     // entries in all: kLdsStack + the spill entries; the walks check it before they push
@@ -130,12 +131,12 @@ struct Stack {
         if (sp < kLdsStack)
             lds[sp * kTraceBlock] = v;
         else
-            spill[(sp - kLdsStack) * stride] = v;
+            spill[uint64_t(sp - kLdsStack) * stride + slot] = v;
         sp++;
     }
     __device__ __forceinline__ uint32_t pop() {
         sp--;
-        return sp < kLdsStack ? lds[sp * kTraceBlock] : spill[(sp - kLdsStack) * stride];
+        return sp < kLdsStack ? lds[sp * kTraceBlock] : spill[uint64_t(sp - kLdsStack) * stride + slot];
     }
 };
 
@@ -294,7 +295,7 @@ constexpr uint32_t kLeafShareBatch = 3;  // ray batches (trace.hip): the other p
 constexpr uint32_t kLeafShareCall = 2;   // trace_wide (the path tracer's rays): once half of them wait
 
 struct WideWalk {
-    F3 o, d, inv, ro, rd, rinv;
+    F3 o, d, ro, rd, rinv;
     float tmax;
     uint32_t mode, node, rec, first, base, guard;
     HitRec h;
@@ -317,13 +318,14 @@ __device__ __forceinline__ bool wide_begin(const TraceArgs &a, F3 o, F3 d, float
     w.mode = mode;
     w.h = HitRec{w.tmax, kNone, 0.0f, 0.0f, kNone, 0};
     if (a.units == 0 || ref_kind(a.top_ref) == 3u) return false;
-    w.o = o, w.d = d, w.inv = inv3(d);
-    if (ray_aabb(o, w.inv, {a.top_box[0], a.top_box[1], a.top_box[2]}, {a.top_box[3], a.top_box[4], a.top_box[5]},
+    const F3 inv = inv3(d);
+    w.o = o, w.d = d;
+    if (ray_aabb(o, inv, {a.top_box[0], a.top_box[1], a.top_box[2]}, {a.top_box[3], a.top_box[4], a.top_box[5]},
                  cull_t(w.h.t)) == kInf)
         return false;
     st.sp = 0;
     w.node = a.top_ref, w.rec = kNone, w.first = 0, w.base = 0, w.guard = 0;
-    w.ro = o, w.rd = d, w.rinv = w.inv;
+    w.ro = o, w.rd = d, w.rinv = inv;
     return true;
 }
 
@@ -434,7 +436,7 @@ __device__ __forceinline__ void wide_other(const TraceArgs &a, WideWalk &w, Stac
     }
     if (node == kWalkLeave) {
         w.rec = kNone;
-        w.ro = w.o, w.rd = w.d, w.rinv = w.inv;
+        w.ro = w.o, w.rd = w.d, w.rinv = inv3(w.d);  // the same bits as at the start: kept no register for it
         w.node = st.sp == 0 ? kWalkDone : st.pop();
         return;
     }
