@@ -3,11 +3,12 @@
 // Wide walk: a lane whose ray is finished takes a new one while the rest of its wave walks on. Incoherent rays differ
 // several-fold in length (on the 3M-triangle workload a wave's longest ray takes about four times its mean), so a
 // wave that waited for its longest ray would leave three lanes in four idle; instead, once kRefill lanes are idle
-// (or all are), the wave takes that many ray indices with one atomic and hands them out by lane rank. Between
-// refills the wave alternates the two phases of its walks (walk.h WideWalk). A refill stops the whole wave on the
-// new rays' loads, so refills are batched: on the 3M-triangle workload every 32 lanes ran 1.37x faster than every
-// 16 and 1.18x faster than every 48. Loading each lane's next ray ahead of time hid those loads but took eight more
-// registers, one wave per SIMD fewer (four instead of five), and was 6 % slower.
+// (or all are), the wave hands out that many ray indices by lane rank from a chunk it reserved ahead with one
+// atomic. Between refills the wave alternates the two phases of its walks (walk.h WideWalk). A refill stops the
+// whole wave on the new rays' loads, so refills are batched (on the 3M-triangle workload, every 16 or 24 lanes ran
+// 4 % faster than every 32 and 1 % faster than every 8); with an atomic per refill as well, the best batch was 32
+// and 3.5 % slower. Loading each lane's next ray ahead of time hid the loads too, but took eight more registers, one
+// wave per SIMD fewer (four instead of five), and was 6 % slower.
 #include <hip/hip_runtime.h>
 
 #include "trace_kernels.h"
@@ -49,7 +50,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
     }
 }
 
-constexpr uint32_t kRefill = 32;
+constexpr uint32_t kRefill = 24;
+constexpr uint32_t kChunk = 64;  // at least a wave: one refill switches chunks at most once
 
 template <bool kCount>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_dyn(TraceArgs a) {
@@ -63,16 +65,31 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_dyn(TraceArgs a) {
     uint32_t ray = 0;  // the host keeps batches under 2^31 rays
     bool active = false, exhausted = false;
     const uint64_t below = (uint64_t{1} << lane) - 1;
+    // Ray indices come to a wave in chunks of kChunk: [res, res_end) is being handed out, and the chunk after it was
+    // requested when this one was taken (its start arrives in lane 0's next_v), so a refill waits on no atomic.
+    uint32_t next_v = 0;
+    if (lane == 0) next_v = atomicAdd(a.counter, 2 * kChunk);
+    uint32_t res = __builtin_amdgcn_readfirstlane(__shfl(next_v, 0, 64));
+    uint32_t res_end = res + kChunk;
+    next_v = res + kChunk;
     for (;;) {
         // lanes without a walk take new rays, kRefill at a time (or all, when no lane walks)
         const uint64_t idle = __ballot(!active && !exhausted);
         uint64_t busy = __ballot(active);
         if (idle != 0 && (__popcll(idle) >= kRefill || busy == 0)) {
-            uint32_t first = 0;
-            if (lane == 0) first = atomicAdd(a.counter, uint32_t(__popcll(idle)));
-            first = __builtin_amdgcn_readfirstlane(__shfl(first, 0, 64));
+            const uint32_t need = uint32_t(__popcll(idle)), avail = res_end - res;
+            const uint32_t rank = uint32_t(__popcll(idle & below));
+            uint32_t i = res + rank;
+            if (need <= avail) {
+                res += need;
+            } else {
+                const uint32_t nx = __builtin_amdgcn_readfirstlane(__shfl(next_v, 0, 64));
+                i = rank < avail ? i : nx + (rank - avail);
+                res = nx + (need - avail);
+                res_end = nx + kChunk;
+                if (lane == 0) next_v = atomicAdd(a.counter, kChunk);
+            }
             if (!active && !exhausted) {
-                const uint64_t i = uint64_t(first) + uint64_t(__popcll(idle & below));
                 if (i < a.n) {
                     ray = uint32_t(i);
                     const float4 r0 = a.rays[2 * i], r1 = a.rays[2 * i + 1];
