@@ -69,6 +69,37 @@ def cpu_baseline(prep, rays: np.ndarray, sample: int) -> dict:
                       f"reference's binary node buffer (-O3, OpenMP, {threads} threads), {dt:.1f} s"}
 
 
+def fp64_check(dev, sc, d_rays, d_hits, hits) -> dict:
+    """Every hit of the batch intersected again in double precision on the GPU (trt_trace_check_fp64): the
+    distribution of |t - t_FP64| / t_FP64 over the hits, and the disputed rays (FP32 hits the FP64 test misses)."""
+    n = len(hits)
+    d_t = dev.alloc(8 * n)
+    sc.check_fp64(d_rays, n, d_hits, d_t)
+    dev.sync()
+    t64 = d_t.download(np.float64, n)
+    d_t.free()
+    hit = hits["tri"] != T.NO_HIT
+    fine = hit & (t64 > 0)
+    rel = np.abs(hits["t"][fine].astype(np.float64) - t64[fine]) / t64[fine]
+    q = np.quantile(rel, [0.5, 0.99, 0.999]) if len(rel) else [0.0, 0.0, 0.0]
+    return {"hits": int(hit.sum()), "disputed": int((t64[hit] == -1).sum()), "invalid": int((t64 == -2).sum()),
+            "t_rel_err": {"p50": float(q[0]), "p99": float(q[1]), "p999": float(q[2]),
+                          "max": float(rel.max()) if len(rel) else 0.0}}
+
+
+def hit_agreement(prep, rays: np.ndarray, hits: np.ndarray, sample: int) -> dict:
+    """The GPU's hits against the oracle's exact walk (lowest (t, instance, triangle)) over the first `sample` rays:
+    the fraction whose whole hit record is the same bits."""
+    import oracle
+
+    threads = min(int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1, 16)
+    sc = oracle.Scene(*prep.desc())
+    want, _ = sc.trace_batch(rays[:sample].view(oracle.RAY), oracle.WALK_EXACT, threads=threads)
+    got = hits[:sample].view(np.uint32).reshape(sample, 6)
+    same = (got == want.view(np.uint32).reshape(sample, 6)).all(axis=1)
+    return {"rays": sample, "same": int(same.sum()), "fraction": float(same.mean())}
+
+
 def bench_render(dev, frames: int) -> dict:
     """Samples/s at 720p: the instanced city (64 seeded meshes x 4096 instances), 1280 x 720, 1 spp a frame,
     3 bounces, sun + constant sky; `frames` frames (one sample each) timed with HIP events."""
@@ -84,18 +115,24 @@ def bench_render(dev, frames: int) -> dict:
     acc = R.Accum(dev, w, h)
     R.render(sc, p, 0, 1, acc)  # warm
     dev.sync()
+    dev.render_ray_counts()  # zero the counters
     dev.timer_start()
     for f in range(frames):
         R.render(sc, p, 1 + f, 1, acc)
     ms = dev.timer_stop()
     dev.sync()
+    closest, shadow = dev.render_ray_counts()
     ms_frame = ms / frames
     sc.close()
     acc.close()
     sps = w * h / (ms_frame * 1e-3)
-    log(f"bench: city 720p {ms_frame:.2f} ms/frame, {sps / 1e6:.1f} Msamples/s")
+    log(f"bench: city 720p {ms_frame:.2f} ms/frame, {sps / 1e6:.1f} Msamples/s, "
+        f"{closest / (ms * 1e-3) / 1e6:.1f} M closest + {shadow / (ms * 1e-3) / 1e6:.1f} M shadow rays/s")
     return {"metric": "samples/s at 720p", "value": round(sps, 1), "unit": "samples/s",
             "ms_per_frame": round(ms_frame, 3), "frames": frames,
+            "rays": {"closest": closest, "shadow": shadow,
+                     "closest_mrays_s": round(closest / (ms * 1e-3) / 1e6, 1),
+                     "shadow_mrays_s": round(shadow / (ms * 1e-3) / 1e6, 1)},
             "workload": "instanced city: 64 seeded meshes (extruded boxes, subdivided spheres, 200-40k triangles) x "
                         "4096 instances, metal-rough GGX, 90,000-lux sun + constant sky, 1280x720, 1 spp a frame, "
                         "3 bounces, megakernel over the 4-wide quantised walk",
@@ -200,6 +237,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-sample", type=int, default=2_000_000)
+    ap.add_argument("--agree-sample", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-render", action="store_true")
     ap.add_argument("--render-frames", type=int, default=10)
@@ -255,6 +293,8 @@ def main():
     hits = d_hits.download(T.HIT, N_RAYS)
     hit_rate = float((hits["tri"] != T.NO_HIT).mean())
     log(f"bench: {ms_step:.3f} ms/step, {mrays:.1f} Mrays/s, wall {wall:.2f} s, hit rate {hit_rate:.3f}")
+    fp64 = fp64_check(dev, sc, d_rays, d_hits, hits)
+    log(f"bench: FP64 check {fp64}")
 
     achieved = reads["bytes"] / (ms_step * 1e-3) / 1e9
     out = {
@@ -280,6 +320,7 @@ def main():
             "reads_per_ray": {"nodes": round(reads["nodes"] / N_RAYS, 2), "tris": round(reads["tris"] / N_RAYS, 2),
                               "bytes": round(reads["bytes"] / N_RAYS, 1)},
             "wall_s_timed": round(wall, 3),
+            "fp64_check": fp64,
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic()},
@@ -296,6 +337,8 @@ def main():
     if not args.no_cpu:
         log("bench: CPU baseline")
         out["cpu_baseline"] = cpu_baseline(prep, rays, min(args.cpu_sample, N_RAYS))
+        out["config"]["hit_id_agreement"] = hit_agreement(prep, rays, hits, min(args.agree_sample, N_RAYS))
+        log(f"bench: hit-ID agreement {out['config']['hit_id_agreement']}")
     print(json.dumps(out), flush=True)
 
 

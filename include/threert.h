@@ -175,6 +175,12 @@ int trt_trace_rays_device(trt_scene *scene, const trt_ray *d_rays, uint64_t n, t
  * units or 160-byte records). Blocks. */
 int trt_trace_reads(trt_scene *scene, const trt_ray *d_rays, uint64_t n, trt_hit *d_hits, uint32_t flags,
                     uint64_t *reads);
+/* The FP64 check of a batch's hits (no counterpart in the reference, whose tests compare hits and never a distance
+ * against double precision; it is the precision workload's oracle on the GPU): each hit's ray is placed in its unit
+ * and the hit triangle intersected again, every operation in double. d_t64[i] = the FP64 distance, 0 where ray i hit
+ * nothing, -1 where the FP64 test misses the triangle the walk reported (a disputed ray), -2 where the hit names a
+ * triangle or unit outside the scene. Device buffers, queued on the device's stream. */
+int trt_trace_check_fp64(trt_scene *scene, const trt_ray *d_rays, const trt_hit *d_hits, uint64_t n, double *d_t64);
 
 /* --- path tracing (gpu.rs:4116-4236 Gpu::render; pathtrace.wgsl:4106 `main`, offline) ---------------------- */
 typedef struct trt_accum trt_accum;
@@ -194,6 +200,10 @@ int trt_accum_resolve(trt_accum *acc, float *color, float *albedo, float *normal
  * is P.size.z), on the device's stream. params: the 624-byte Params block (pathtrace.wgsl:96-140); its size.xy
  * must be the accumulator's. Needs a scene uploaded with its shading tables. */
 int trt_render(trt_scene *scene, const void *params, uint32_t first_sample, uint32_t spp, trt_accum *acc);
+/* The rays the device's render launches traced since the last call, which resets them: closest-hit rays (camera and
+ * bounce) and shadow rays (sun and emitter visibility). Waits for the device. A measurement with no counterpart in
+ * the reference, whose passes report timings only (gpu.rs:4198-4208). */
+int trt_render_ray_counts(trt_device *dev, uint64_t *closest, uint64_t *shadow);
 
 /* Several render jobs in one launch (continuous batching for a render server; the reference's `serve` loop,
  * main.rs:345-398, renders one job at a time and has no counterpart): each job is trt_render's arguments; their

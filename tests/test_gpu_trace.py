@@ -183,3 +183,73 @@ def test_full_size_walks_agree(device):
     want, _ = oracle.Scene(*prep.desc()).trace_batch(rays[pick].view(oracle.RAY), oracle.WALK_EXACT)
     assert same(wide[pick], want).all()
     sc.close()
+
+
+def fp64_distances(prep: S.Prepared, rays: np.ndarray, hits: np.ndarray) -> np.ndarray:
+    """What trt_trace_check_fp64 states, in numpy float64: the ray placed in the hit's record (o - translation, then
+    the normal rows unless the record is a pure shift) and the hit triangle's v0, e1, e2 tested by Moller-Trumbore,
+    each sum in the kernel's order. 0 for no hit, -1 where the double-precision test misses."""
+    out = np.zeros(len(rays))
+    hit = hits["tri"] != T.NO_HIT
+    idx = np.nonzero(hit)[0]
+    rec = prep.instances[hits["inst"][idx]]
+    o = rays["o"][idx].astype(np.float64) - rec["m"][:, :, 3].astype(np.float64)
+    d = rays["d"][idx].astype(np.float64)
+    rows = rec["n"][:, :, :3].astype(np.float64)
+    shifted = (rec["n"][:, 1, 3].view(np.uint32) & 2) != 0
+
+    def place(v):
+        return (rows[:, 0] * v[:, 0:1] + rows[:, 1] * v[:, 1:2]) + rows[:, 2] * v[:, 2:3]
+
+    o = np.where(shifted[:, None], o, place(o))
+    d = np.where(shifted[:, None], d, place(d))
+    lv = prep.leaves[hits["tri"][idx]].astype(np.float64)
+    v0, e1, e2 = lv[:, 0, :3], lv[:, 1, :3], lv[:, 2, :3]
+
+    def dot(a, b):
+        return (a[:, 0] * b[:, 0] + a[:, 1] * b[:, 1]) + a[:, 2] * b[:, 2]
+
+    def cross(a, b):
+        return np.stack([a[:, 1] * b[:, 2] - b[:, 1] * a[:, 2], a[:, 2] * b[:, 0] - b[:, 2] * a[:, 0],
+                         a[:, 0] * b[:, 1] - b[:, 0] * a[:, 1]], axis=1)
+
+    pv = cross(d, e2)
+    det = dot(e1, pv)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        inv = 1.0 / det
+        tv = o - v0
+        u = dot(tv, pv) * inv
+        qv = cross(tv, e1)
+        v = dot(d, qv) * inv
+        t = dot(e2, qv) * inv
+        ok = (det != 0) & (u >= 0) & (u <= 1) & (v >= 0) & (u + v <= 1) & (t > 0)
+    out[idx] = np.where(ok, t, -1.0)
+    return out
+
+
+def test_fp64_check_is_the_double_precision_test(device):
+    """trt_trace_check_fp64 against its statement in numpy float64, bit for bit, on rotated and scaled instances;
+    and the walk's FP32 distances within FP32 rounding of the FP64 ones."""
+    prep = S.instanced_scene(3, meshes=5, instances=40)
+    rays = rays_at(prep, 6000, 3, T.TRACE_CLOSEST)
+    sc = T.Scene(device, *prep.desc())
+    d_r, d_h, d_t = device.alloc(rays.nbytes), device.alloc(len(rays) * T.HIT.itemsize), device.alloc(8 * len(rays))
+    d_r.upload(rays)
+    sc.trace_device(d_r, len(rays), d_h, T.WALK_WIDE)
+    sc.check_fp64(d_r, len(rays), d_h, d_t)
+    device.sync()
+    hits = d_h.download(T.HIT, len(rays))
+    t64 = d_t.download(np.float64, len(rays))
+    for b in (d_r, d_h, d_t):
+        b.free()
+    sc.close()
+    want = fp64_distances(prep, rays, hits)
+    hit = hits["tri"] != T.NO_HIT
+    say(f"{hit.sum()} hits, {(t64 == -1).sum()} disputed")
+    assert hit.sum() > 1000
+    assert (t64[~hit] == 0).all()
+    assert (t64.view(np.uint64) == want.view(np.uint64)).all(), np.nonzero(t64 != want)[0][:5]
+    fine = hit & (t64 > 0)
+    rel = np.abs(hits["t"][fine].astype(np.float64) - t64[fine]) / t64[fine]
+    assert rel.max() < 1e-4, rel.max()
+    assert (t64[hit] == -1).mean() < 0.01

@@ -183,6 +183,28 @@ def test_sunlit_instances_match_the_oracle(device):
 
 
 @pytest.mark.gpu
+def test_render_ray_counts(device):
+    """trt_render_ray_counts: a camera that looks away from the scene traces exactly one closest ray a sample and no
+    shadow ray; the Cornell box traces at least that, the same counts for the same render, and reading resets."""
+    rs, cam = R.cornell()
+    w, h, spp = 48, 32, 4
+    away = R.Camera((0.5, 0.5, -1.35), (0.5, 0.5, -3.0), 40.0)
+    device.render_ray_counts()
+    gpu_render(device, rs, rs.params(w, h, away, seed=1, max_bounces=4, rr_depth=3), 0, spp)
+    assert device.render_ray_counts() == (w * h * spp, 0)
+    assert device.render_ray_counts() == (0, 0)
+    p = rs.params(w, h, cam, seed=1, max_bounces=4, rr_depth=3)
+    gpu_render(device, rs, p, 0, spp)
+    first = device.render_ray_counts()
+    gpu_render(device, rs, p, 0, spp)
+    assert device.render_ray_counts() == first
+    closest, shadow = first
+    print(f"cornell {w}x{h}x{spp}: {closest} closest, {shadow} shadow rays", flush=True)
+    assert closest > w * h * spp and shadow > 0
+    assert closest <= w * h * spp * 5  # a camera ray and at most 4 bounces a sample
+
+
+@pytest.mark.gpu
 def test_gpu_white_furnace(device):
     rs, cam = sphere_scene((1, 1, 1), 0.0, 1.0, 0.0)
     p = rs.params(64, 64, cam, max_bounces=8, rr_depth=8, sky=(1.0, 1.0, 1.0))

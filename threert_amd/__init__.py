@@ -41,7 +41,7 @@ EXPORTS = [
     "trt_unit_box", "trt_unit_record", "trt_tree_free", "trt_scene_upload", "trt_scene_release",
     "trt_scene_stats_get", "trt_trace_rays", "trt_trace_rays_device", "trt_trace_reads",
     "trt_accum_create", "trt_accum_release", "trt_accum_clear", "trt_accum_read", "trt_accum_resolve", "trt_render",
-    "trt_denoise_atrous", "trt_render_batch", "trt_mem_copy",
+    "trt_denoise_atrous", "trt_render_batch", "trt_mem_copy", "trt_trace_check_fp64", "trt_render_ray_counts",
     "trt_unet_create", "trt_unet_release", "trt_unet_param_count", "trt_unet_params_set", "trt_unet_params_get",
     "trt_unet_denoise", "trt_unet_gradients", "trt_unet_train_step",
 ]
@@ -106,6 +106,8 @@ def lib():
             "trt_scene_upload": [P, P, C.POINTER(C.c_void_p)], "trt_scene_release": [P],
             "trt_scene_stats_get": [P, P], "trt_trace_rays": [P, P, U64, P, U32],
             "trt_trace_rays_device": [P, P, U64, P, U32], "trt_trace_reads": [P, P, U64, P, U32, P],
+            "trt_trace_check_fp64": [P, P, P, U64, P],
+            "trt_render_ray_counts": [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -216,6 +218,12 @@ class Device:
     def sync(self):
         _check(lib().trt_device_sync(self.h))
 
+    def render_ray_counts(self) -> tuple:
+        """(closest, shadow) rays the render launches traced since the last call; waits for the device."""
+        c, sh = C.c_uint64(), C.c_uint64()
+        _check(lib().trt_render_ray_counts(self.h, C.byref(c), C.byref(sh)))
+        return int(c.value), int(sh.value)
+
     def alloc(self, nbytes: int) -> "DeviceBuffer":
         return DeviceBuffer(self, nbytes)
 
@@ -317,6 +325,10 @@ class Scene:
         nodes, tris, units = (int(x) for x in out)
         return {"nodes": nodes, "tris": tris, "units": units,
                 "bytes": nodes * node_b + tris * 48 + units * unit_b}
+
+    def check_fp64(self, rays: DeviceBuffer, n: int, hits: DeviceBuffer, t64: DeviceBuffer):
+        """Queues the FP64 re-intersection of a batch's hits (trt_trace_check_fp64): t64 receives n doubles."""
+        _check(lib().trt_trace_check_fp64(self.h, rays.p, hits.p, n, t64.p))
 
     def close(self):
         if self.h:

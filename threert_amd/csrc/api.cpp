@@ -71,6 +71,9 @@ struct trt_device {
     std::atomic<int> refs{1};
 };
 
+// d->reads: [0..2] a counting trace launch's reads, [kRenderRays], [kRenderRays + 1] rays the render launches traced
+constexpr int kRenderRays = 4;
+
 struct trt_scene {
     trt_device *dev = nullptr;
     void *nodes = nullptr, *leaves = nullptr, *instances = nullptr, *wnodes = nullptr, *wunits = nullptr,
@@ -110,6 +113,7 @@ int trt_device_create(int ordinal, trt_device **out) {
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&d->guard), 256);
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&d->reads), 256);
     if (e == hipSuccess) e = hipMemset(d->guard, 0, 256);
+    if (e == hipSuccess) e = hipMemset(d->reads, 0, 256);
     if (e != hipSuccess) {
         trt_device_release(d);
         return fail(TRT_E_DEVICE, std::string("trt_device_create: ") + hipGetErrorString(e));
@@ -660,6 +664,20 @@ int trt_trace_reads(trt_scene *s, const trt_ray *d_rays, uint64_t n, trt_hit *d_
     return trt_device_sync(d);
 }
 
+int trt_trace_check_fp64(trt_scene *s, const trt_ray *d_rays, const trt_hit *d_hits, uint64_t n, double *d_t64) {
+    if (!s || (n && (!d_rays || !d_hits || !d_t64))) return fail(TRT_E_INVALID, "trt_trace_check_fp64: null argument");
+    if (n == 0) return ok();
+    trt_device *d = s->dev;
+    std::lock_guard<std::mutex> lock(d->mu);
+    TRT_HIP(hipSetDevice(d->ordinal));
+    const uint64_t units = s->wunits ? s->wunit_count : 0;
+    if (trt::launch_check64(reinterpret_cast<const float4 *>(d_rays), reinterpret_cast<const uint32_t *>(d_hits),
+                            static_cast<const float4 *>(s->leaves), static_cast<const float4 *>(s->wunits), n,
+                            s->leaf_count, units, d_t64, d->cus * 8, d->stream))
+        return fail(TRT_E_DEVICE, "FP64 check kernel launch failed");
+    return ok();
+}
+
 // ---- path tracing --------------------------------------------------------------------------------------------------
 struct trt_accum {
     trt_device *dev = nullptr;
@@ -852,10 +870,24 @@ int trt_render_batch(trt_device *d, const trt_render_job *jobs, uint32_t n) {
     TRT_HIP(hipMemcpyAsync(ring.dev[k], ring.host[k], bytes, hipMemcpyHostToDevice, d->stream));
     if (trt::launch_render(static_cast<const trt::RenderJob *>(ring.dev[k]), static_cast<uint32_t>(table.size()),
                            static_cast<uint32_t>(tiles), d->counter, d->spill, threads,
-                           static_cast<uint32_t>(entries), d->guard, grid, d->stream))
+                           static_cast<uint32_t>(entries), d->guard, d->reads + kRenderRays, grid, d->stream))
         return fail(TRT_E_DEVICE, "render kernel launch failed");
     TRT_HIP(hipEventRecord(ring.ev[k], d->stream));
     return ok();
+}
+
+int trt_render_ray_counts(trt_device *d, uint64_t *closest, uint64_t *shadow) {
+    if (!d || !closest || !shadow) return fail(TRT_E_INVALID, "trt_render_ray_counts: null argument");
+    {
+        std::lock_guard<std::mutex> lock(d->mu);
+        TRT_HIP(hipSetDevice(d->ordinal));
+        unsigned long long r[2] = {0, 0};
+        TRT_HIP(hipMemcpyAsync(r, d->reads + kRenderRays, sizeof(r), hipMemcpyDeviceToHost, d->stream));
+        TRT_HIP(hipMemsetAsync(d->reads + kRenderRays, 0, sizeof(r), d->stream));
+        TRT_HIP(hipStreamSynchronize(d->stream));
+        *closest = r[0], *shadow = r[1];
+    }
+    return trt_device_sync(d);
 }
 
 int trt_render(trt_scene *s, const void *params, uint32_t first_sample, uint32_t spp, trt_accum *acc) {

@@ -173,6 +173,7 @@ struct Ctx {
     const RenderArgs *a;
     Stack *st;
     uint32_t *flags;  // kFlag* bits: why a walk or a table read stopped and reported
+    uint32_t *traced; // rays this thread traced: [0] closest, [1] shadow and any-hit
     Rng g;
     bool mirror_path;
     bool sampled_spec;
@@ -203,6 +204,8 @@ struct Ctx {
     }
     __device__ HitRec trace(V o, V d, float tmax, uint32_t mode) const {
         Reads cnt;
+        traced[0] += mode == kTraceClosest ? 1u : 0u;
+        traced[1] += mode == kTraceClosest ? 0u : 1u;
         return trace_wide<false>(a->geo, F3{o.x, o.y, o.z}, F3{d.x, d.y, d.z}, tmax, mode, *st, *flags, cnt);
     }
 };
@@ -791,12 +794,14 @@ __device__ void path(Ctx &c, uint32_t px, uint32_t py, uint32_t index, V &radian
 // job's scene, parameters and accumulator. Jobs of any size share the launch: what one job leaves idle, another fills.
 __global__ __launch_bounds__(kRenderBlock) void k_render(const RenderJob *jobs, uint32_t njobs, uint32_t total,
                                                          uint32_t *counter, uint32_t *spill, uint32_t spill_stride,
-                                                         uint32_t spill_entries, uint32_t *guard) {
+                                                         uint32_t spill_entries, uint32_t *guard,
+                                                         unsigned long long *rays) {
     __shared__ uint32_t lds[kLdsStack * kRenderBlock];
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     Stack st{(TRT_LDS uint32_t *)(lds + threadIdx.x), spill, gtid, spill_stride, 0, kLdsStack + spill_entries};
     uint32_t flags = 0;
+    uint32_t traced[2] = {0, 0};
     for (;;) {
         uint32_t tile = 0;
         if (lane == 0) tile = atomicAdd(counter, 1u);
@@ -814,7 +819,7 @@ __global__ __launch_bounds__(kRenderBlock) void k_render(const RenderJob *jobs, 
         const RenderArgs *a = &job->args;
         const uint32_t t = tile - job->tile_first;
         const uint32_t w = a->params[16], h = a->params[17], tiles_x = (w + 7) / 8;
-        Ctx c{a, &st, &flags, Rng{0, 0, 0}, false, false};
+        Ctx c{a, &st, &flags, traced, Rng{0, 0, 0}, false, false};
         const uint32_t px = (t % tiles_x) * 8 + (lane & 7u), py = (t / tiles_x) * 8 + (lane >> 3);
         if (px < w && py < h) {
             float4 *pix = a->accum + 5 * (uint64_t(py) * w + px);
@@ -833,6 +838,15 @@ __global__ __launch_bounds__(kRenderBlock) void k_render(const RenderJob *jobs, 
         }
     }
     if (flags) atomicOr(guard, flags);
+    // the wave's ray counts, summed across lanes, one atomic each
+    for (int k = 32; k >= 1; k >>= 1) {
+        traced[0] += __shfl_xor(traced[0], k, 64);
+        traced[1] += __shfl_xor(traced[1], k, 64);
+    }
+    if (lane == 0) {
+        atomicAdd(rays, (unsigned long long)traced[0]);
+        atomicAdd(rays + 1, (unsigned long long)traced[1]);
+    }
 }
 
 __global__ void k_reset(uint32_t *c) { *c = 0; }
@@ -847,11 +861,11 @@ int render_occupancy(int *blocks_per_cu) {
 }
 
 int launch_render(const RenderJob *d_jobs, uint32_t njobs, uint32_t total_tiles, uint32_t *counter,
-                  uint32_t *spill, uint32_t spill_stride, uint32_t spill_entries, uint32_t *guard, int grid,
-                  hipStream_t s) {
+                  uint32_t *spill, uint32_t spill_stride, uint32_t spill_entries, uint32_t *guard,
+                  unsigned long long *rays, int grid, hipStream_t s) {
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, s, counter);
     hipLaunchKernelGGL(k_render, dim3(grid), dim3(kRenderBlock), 0, s, d_jobs, njobs, total_tiles, counter, spill,
-                       spill_stride, spill_entries, guard);
+                       spill_stride, spill_entries, guard, rays);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -42,8 +42,9 @@ struct RenderJob {
 
 int render_occupancy(int *blocks_per_cu);
 // d_jobs: njobs jobs in device memory, tile_first ascending from 0; total_tiles: all their tiles.
+// rays: two counters the launch adds its traced rays to (closest; shadow and any-hit)
 int launch_render(const RenderJob *d_jobs, uint32_t njobs, uint32_t total_tiles, uint32_t *counter,
-                  uint32_t *spill, uint32_t spill_stride, uint32_t spill_entries, uint32_t *guard, int grid,
-                  hipStream_t s);
+                  uint32_t *spill, uint32_t spill_stride, uint32_t spill_entries, uint32_t *guard,
+                  unsigned long long *rays, int grid, hipStream_t s);
 
 }  // namespace trt

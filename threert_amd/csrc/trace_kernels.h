@@ -45,5 +45,8 @@ int trace_occupancy(bool wide, int *blocks_per_cu);
 int launch_trace(const TraceArgs &a, bool wide, bool count, int grid, hipStream_t s);
 // n16 16-byte words from src to dst, both 16-byte aligned; at most `grid` blocks of 256
 int launch_copy(void *dst, const void *src, uint64_t n16, int grid, hipStream_t s);
+// FP64 re-intersection of n hits (check64.hip); at most `grid` blocks of 256
+int launch_check64(const float4 *rays, const uint32_t *hits, const float4 *leaves, const float4 *units, uint64_t n,
+                   uint64_t leaf_count, uint64_t unit_count, double *t64, int grid, hipStream_t s);
 
 }  // namespace trt
