@@ -236,8 +236,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-sample", type=int, default=2_000_000)
-    ap.add_argument("--agree-sample", type=int, default=1_000_000)
+    ap.add_argument("--cpu-sample", type=int, default=N_RAYS)
+    ap.add_argument("--agree-sample", type=int, default=N_RAYS)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-render", action="store_true")
     ap.add_argument("--render-frames", type=int, default=10)
