@@ -244,10 +244,29 @@ def main():
     ap.add_argument("--serve", action="store_true", help="measure the serving workload instead (p50/p99 latency)")
     ap.add_argument("--serve-jobs", type=int, default=512)
     ap.add_argument("--serve-rate", type=float, default=40.0)
+    ap.add_argument("--serve-equal", action="store_true",
+                    help="the serving workload twice at equal image error: plain tracing, then the learned denoiser "
+                         "with plain tracing as fallback (p99 latency of each)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus})", file=sys.stderr)
         sys.exit(2)
+    if args.serve_equal:
+        from threert_amd import equal_error
+
+        dev = T.Device(0)
+        r = equal_error.run(dev, n_jobs=args.serve_jobs, rate=args.serve_rate, log=log)
+        d = r["runs"]["denoised"]
+        print(json.dumps({"metric": "p99 job latency at equal image error", "value": d["p99_s"], "unit": "s",
+                          "n_gpus": 1, "higher_is_better": False, "dtype": "f32", "data": "synthetic",
+                          "vs_plain_p99": r["p99_ratio"],
+                          "config": {"workload": f"{args.serve_jobs} render jobs, Poisson arrivals at "
+                                                 f"{args.serve_rate}/s, seeded scenes of 10k-4M triangles, "
+                                                 "256x256-1920x1080, 1-256 spp, 2-6 bounces, 200 GB budget; "
+                                                 "the learned denoiser over 1 sample where its calibrated error is "
+                                                 "at most the job's plain error, plain tracing otherwise"},
+                          "equal_error": r}), flush=True)
+        return
     if args.serve:
         dev = T.Device(0)
         s = bench_serve(dev, args.serve_jobs, args.serve_rate)

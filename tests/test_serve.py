@@ -102,3 +102,25 @@ def test_the_server_gives_each_job_its_lone_image(device):
         sc.close()
     s = serve.summary(res, max(r.finished for r in res))
     assert s["jobs"] == 8 and s["samples"] == sum(j.width * j.height * j.spp for j in jobs)
+
+
+def test_equal_error_plan_never_raises_a_jobs_error():
+    """The denoising server's plan: the fewest calibrated samples whose denoised error is at most the job's plain
+    error, else the job as asked; no job's calibrated error goes up."""
+    from threert_amd import equal_error as E
+
+    plain = {k: 1.0 / np.sqrt(k) for k in E.CHECKPOINTS}
+    den = {k: 0.3 / k ** 0.4 for k in E.CHECKPOINTS}
+    cal = {"plain": plain, "denoised": den}
+    jobs = serve.workload(n_jobs=64, rate=40.0, seed=3)
+    c = {j.scene: cal for j in jobs}
+    planned = E.plan(jobs, c)
+    for j, d in zip(jobs, planned):
+        target = E.plain_error(cal, j.spp)
+        assert E.job_error(cal, d) <= target + 1e-12
+        if d.denoise:
+            assert d.spp < j.spp and (d.spp == 1 or den[d.spp // 2] > target)
+        else:
+            assert d == j and all(den[k] > target for k in E.CHECKPOINTS if k < j.spp)
+    assert any(d.denoise for d in planned) and E.plain_error(cal, 4) == pytest.approx(0.5)
+    assert E.plain_error(cal, 1024) == pytest.approx(plain[256] / 2)

@@ -1,11 +1,12 @@
 """Training data and training for the learned denoiser, all on the GPU through the library.
 
 Data: seeded scenes from the Cornell-box and clustered-soup generators (the plain and precision workloads) and
-small instanced cities; per scene, several independent 1-sample renders (color, albedo, normal from the
-accumulator) and one many-sample target. The target is rendered in launches of `chunk` samples, each into a fresh
-accumulator, and the launches' sums are added on the host in float64 (an FP64-accumulated reference).
+small instanced cities; per scene, inputs (color, albedo, normal from the accumulator) read from one render after
+1, 2, 4, ... 128 samples, so the network meets every noise level a server would hand it, and one many-sample
+target. The target is rendered in launches of `chunk` samples, each into a fresh accumulator, and the launches'
+sums are added on the host in float64 (an FP64-accumulated reference).
 
-    python -m threert_amd.denoise_train --scenes 200 --steps 4000 --out weights.npy
+    python -m threert_amd.denoise_train --scenes 200 --steps 12000 --out weights.npy
 """
 from __future__ import annotations
 
@@ -70,13 +71,23 @@ def scene_list(count: int, first_seed: int = 1):
 
 
 def render_set(dev: Device, kind: str, seed: int, w: int, h: int, noisy: int, target_spp: int, chunk: int = 64,
-               bounces: int = 4):
-    """(inputs (noisy, h, w, 9): color, albedo, normal of independent 1-sample renders; target (h, w, 3))."""
+               bounces: int = 4, levels: tuple = ()):
+    """(inputs (k, h, w, 9): color, albedo, normal; target (h, w, 3)). The inputs are `noisy` independent 1-sample
+    renders, or, given `levels` (ascending sample counts), one render read after each level's samples."""
     rs, cam, sky = scene_of(kind, seed)
     p = rs.params(w, h, cam, seed=seed, max_bounces=bounces, rr_depth=3, sky=sky)
     sc = R.upload(dev, rs)
-    inputs = np.zeros((noisy, h, w, 9), np.float32)
-    for k in range(noisy):
+    inputs = np.zeros((len(levels) or noisy, h, w, 9), np.float32)
+    if levels:
+        acc = R.Accum(dev, w, h)
+        done = 0
+        for k, n in enumerate(levels):
+            R.render(sc, p, 100_000 + done, n - done, acc)  # sample indices far from the target's
+            done = n
+            c, a, nm, _ = acc.resolve()
+            inputs[k, ..., 0:3], inputs[k, ..., 3:6], inputs[k, ..., 6:9] = c, a, nm
+        acc.close()
+    for k in range(0 if levels else noisy):
         acc = R.Accum(dev, w, h)
         R.render(sc, p, 100_000 + k, 1, acc)  # sample indices far from the target's
         c, a, n, _ = acc.resolve()
@@ -165,14 +176,15 @@ def spp_curve(dev: Device, kind: str, seed: int, w: int, h: int, spps=(1, 4, 16,
 
 
 def evaluate(dev: Device, net: UNet, scenes, w: int = 1280, h: int = 720, target_spp: int = 1024, log=print,
-             timing_reps: int = 20) -> dict:
+             timing_reps: int = 20, levels: tuple = (1, 4, 16, 64)) -> dict:
     """Per held-out scene: relative RMSE to the FP64-accumulated target of the 1-sample input, the a-trous baseline
-    (denoise.rs's stage), the U-Net, and plain renders of 4 / 16 / 64 samples; and the U-Net's ms a frame at w x h."""
+    (denoise.rs's stage), the U-Net over one render read at each of `levels` samples (`unet` is its 1-sample
+    figure), and plain renders of 4 / 16 / 64 samples; and the U-Net's ms a frame at w x h."""
     from .denoise import atrous
 
     rows = []
     for kind, seed in scenes:
-        inp, tgt = render_set(dev, kind, seed, w, h, 1, target_spp)
+        inp, tgt = render_set(dev, kind, seed, w, h, 1, target_spp, levels=levels)
         color, albedo, normal = inp[0, ..., 0:3], inp[0, ..., 3:6], inp[0, ..., 6:9]
         den = net.denoise(color, albedo, normal)
         var = np.zeros((h, w), np.float32)  # one sample: the a-trous luminance weight sees no variance estimate
@@ -180,6 +192,8 @@ def evaluate(dev: Device, net: UNet, scenes, w: int = 1280, h: int = 720, target
         curve = spp_curve(dev, kind, seed, w, h)
         row = {"scene": f"{kind}-{seed}", "noisy_1spp": rel_rmse(color, tgt), "atrous": rel_rmse(at, tgt),
                "unet": rel_rmse(den, tgt)}
+        for k, n in enumerate(levels[1:], 1):
+            row[f"unet_{n}spp"] = rel_rmse(net.denoise(inp[k, ..., 0:3], inp[k, ..., 3:6], inp[k, ..., 6:9]), tgt)
         for n, img in curve.items():
             row[f"plain_{n}spp"] = rel_rmse(img, tgt)
         rows.append(row)
@@ -209,7 +223,9 @@ def main():
     ap.add_argument("--scenes", type=int, default=200)
     ap.add_argument("--width", type=int, default=384)
     ap.add_argument("--height", type=int, default=224)
-    ap.add_argument("--noisy", type=int, default=4)
+    ap.add_argument("--noisy", type=int, default=4, help="independent 1-sample inputs a scene (without --levels)")
+    ap.add_argument("--levels", default="1,2,4,8,16,32,64,128",
+                    help="sample counts of the inputs, read from one render a scene; empty for --noisy")
     ap.add_argument("--target-spp", type=int, default=1024)
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--lr", type=float, default=1e-3)
@@ -222,8 +238,9 @@ def main():
     dev = Device(0)
     ins, tgs = [], []
     t0 = time.perf_counter()
+    levels = tuple(int(x) for x in args.levels.split(",") if x)
     for k, (kind, seed) in enumerate(scene_list(args.scenes)):
-        i, t = render_set(dev, kind, seed, args.width, args.height, args.noisy, args.target_spp)
+        i, t = render_set(dev, kind, seed, args.width, args.height, args.noisy, args.target_spp, levels=levels)
         ins.append(i)
         tgs.append(t)
         if k % 10 == 0:
@@ -236,7 +253,8 @@ def main():
     held_out = scene_list(args.eval_scenes, first_seed=10_000)
     rep = evaluate(dev, net, held_out, args.eval_width, args.eval_height, args.target_spp,
                    log=lambda s: print(s, flush=True))
-    rep.update({"train_scenes": args.scenes, "train_size": [args.width, args.height], "noisy_per_scene": args.noisy,
+    rep.update({"train_scenes": args.scenes, "train_size": [args.width, args.height],
+                "inputs_per_scene": list(levels) if levels else f"{args.noisy} x 1 sample",
                 "steps": args.steps, "train_seconds": round(train_s, 1), "losses": losses})
     print("report:", json.dumps(rep), flush=True)
     if args.report:

@@ -61,6 +61,7 @@ class Job:
     bounces: int
     arrival: float = 0.0  # seconds from the start of the run
     seed: int = 1
+    denoise: bool = False  # the finished image goes through the server's learned denoiser
 
 
 @dataclass
@@ -71,6 +72,7 @@ class Result:
     finished: float
     samples: int
     image: np.ndarray | None = None
+    denoised: bool = False
 
     @property
     def latency(self) -> float:
@@ -164,6 +166,15 @@ class SceneStore:
 ACCUM_BYTES = 80  # records.wgsl Pixel: five vec4 per pixel
 
 
+def denoise_image(net, color: np.ndarray, albedo: np.ndarray, normal: np.ndarray) -> np.ndarray:
+    """The learned denoiser over an image of any size: edge-padded to multiples of 16 (the network's four poolings),
+    denoised on the device, cropped back."""
+    h, w = color.shape[:2]
+    ph, pw = (-h) % 16, (-w) % 16
+    pad = lambda x: np.pad(x, ((0, ph), (0, pw), (0, 0)), mode="edge")  # noqa: E731
+    return net.denoise(pad(color), pad(albedo), pad(normal))[:h, :w]
+
+
 def params_of(e: _Resident, j: Job) -> np.ndarray:
     """The job's Params: its size, seed and bounces over the scene's camera and sky (RR from depth 3)."""
     return e.rs.params(j.width, j.height, e.cam, seed=j.seed, max_bounces=j.bounces, rr_depth=3, sky=e.sky)
@@ -173,7 +184,8 @@ def scene_bytes(rs: R.RenderScene) -> int:
     p = rs.prep
     b = p.nodes.nbytes + p.leaves.nbytes + p.instances.nbytes
     b += sum(a.nbytes for a in rs.tables().values())
-    return int(b + p.nodes.nbytes * 2)  # the wide tree is at most twice the binary buffer
+    # the wide tree is at most twice the binary buffer; the leaves are copied once more in the wide trees' order
+    return int(b + p.nodes.nbytes * 2 + p.leaves.nbytes)
 
 
 class Server:
@@ -181,8 +193,9 @@ class Server:
     one Result per job."""
 
     def __init__(self, dev: Device, make_scene, budget_bytes: int = 200 << 30, launch_samples: float = 4e7,
-                 keep_images: bool = False, max_jobs: int = 256, workers: int = 8):
+                 keep_images: bool = False, max_jobs: int = 256, workers: int = 8, unet=None):
         self.dev = dev
+        self.unet = unet  # a threert_amd.unet.UNet for jobs with `denoise` set
         self.store = SceneStore(dev, make_scene, workers)
         self.budget = budget_bytes
         self.launch_samples = launch_samples  # path samples one launch aims at (all jobs together)
@@ -257,9 +270,13 @@ class Server:
                 a.done += k
                 self.samples += k * a.job.width * a.job.height
                 if a.done >= a.job.spp:
-                    img = a.acc.resolve()[0] if self.keep_images else None
+                    img = None
+                    if a.job.denoise:
+                        img = denoise_image(self.unet, *a.acc.resolve()[:3])
+                    elif self.keep_images:
+                        img = a.acc.resolve()[0]
                     results.append(Result(a.job.id, a.job.arrival, a.admitted, now(), a.job.spp * a.job.width * a.job.height,
-                                          img))
+                                          img if self.keep_images else None, a.job.denoise))
                     a.acc.close()
                     used -= a.job.width * a.job.height * ACCUM_BYTES
                     self.store.release(a.entry)
