@@ -100,7 +100,9 @@ class _Resident:
 
 
 class SceneStore:
-    """Scenes resident in HBM by key, least recently used first out; host builds on a worker pool."""
+    """Scenes resident in HBM by key, least recently used first out. A worker pool builds each scene on the host and
+    uploads it (the library builds its wide trees during the upload, seconds for millions of triangles), so the
+    serving loop never waits for a build; the budget counts a scene from its first job's admission."""
 
     def __init__(self, dev: Device, make_scene, workers: int = 4):
         self.dev = dev
@@ -110,10 +112,14 @@ class SceneStore:
         self.pool = ThreadPoolExecutor(max_workers=workers)
         self.lock = threading.Lock()
 
+    def _build(self, key):
+        rs, cam, sky = self.make_scene(*key)
+        return [rs, cam, sky, R.upload(self.dev, rs)]
+
     def prefetch(self, key):
         with self.lock:
             if key not in self.host:
-                self.host[key] = self.pool.submit(self.make_scene, *key)
+                self.host[key] = self.pool.submit(self._build, key)
 
     def ready(self, key) -> bool:
         return key in self.resident or (key in self.host and self.host[key].done())
@@ -129,8 +135,11 @@ class SceneStore:
             e = self.resident.pop(key)
             self.resident[key] = e
         else:
-            rs, cam, sky = self.host[key].result()
-            sc = R.upload(self.dev, rs)
+            built = self.host[key].result()
+            rs, cam, sky, sc = built
+            if sc is None:  # evicted since: upload again
+                sc = R.upload(self.dev, rs)
+            built[3] = None  # the upload now belongs to the resident entry
             e = _Resident(sc, rs, cam, sky, scene_bytes(rs))
             self.resident[key] = e
         e.users += 1
@@ -156,11 +165,14 @@ class SceneStore:
         return sum(e.bytes for e in self.resident.values())
 
     def close(self):
+        self.pool.shutdown(wait=True)
         for e in self.resident.values():
             e.scene.close()
         self.resident.clear()
+        for f in self.host.values():
+            if f.done() and f.exception() is None and f.result()[3] is not None:
+                f.result()[3].close()  # built and uploaded, never used
         self.host.clear()
-        self.pool.shutdown(wait=True)
 
 
 ACCUM_BYTES = 80  # records.wgsl Pixel: five vec4 per pixel
