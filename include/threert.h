@@ -195,6 +195,11 @@ int trt_accum_read(trt_accum *acc, float *pixels);
 /* gpu.rs:4218-4231: color, albedo, normal divided by the sample count (3 floats a pixel each, any may be NULL),
  * variance of the mean luminance (1 float a pixel, may be NULL). Waits for the device. */
 int trt_accum_resolve(trt_accum *acc, float *color, float *albedo, float *normal, float *variance);
+/* The same means into device images of pw x ph >= the accumulator's width x height (3 floats a pixel each, rows and
+ * columns past the accumulator's repeating its last), on the device's stream: the input of trt_unet_denoise without
+ * a trip through host memory (the denoiser wants sides that are multiples of 16). No counterpart in the reference,
+ * whose denoisers read the frame back first (denoise.rs:10-103). */
+int trt_accum_resolve_device(trt_accum *acc, uint32_t pw, uint32_t ph, float *d_color, float *d_albedo, float *d_normal);
 
 /* Samples first_sample .. first_sample + spp - 1 of every pixel added to `acc` (gpu.rs:4177-4196: the sample index
  * is P.size.z), on the device's stream. params: the 624-byte Params block (pathtrace.wgsl:96-140); its size.xy

@@ -26,16 +26,22 @@ def test_the_workload_is_seeded_and_in_range():
 
 def test_equal_shares_and_the_last_slice():
     class A:
-        def __init__(self, w, h, spp, done):
-            self.job = serve.Job(0, (1, 1), w, h, spp, 2)
+        def __init__(self, w, h, spp, done, arrival=0.0):
+            self.job = serve.Job(0, (1, 1), w, h, spp, 2, arrival)
             self.done = done
 
     s = serve.Server.__new__(serve.Server)
     s.launch_samples = 1000.0
+    s.policy = "equal"
     # two jobs share 1000 samples: 500 each; a 10x10 job takes 5 samples, a 100x100 job at least one
-    assert s._slices([A(10, 10, 64, 0), A(100, 100, 64, 0)]) == [5, 1]
+    assert s._slices([A(10, 10, 64, 0), A(100, 100, 64, 0)], 1.0) == [5, 1]
     # never more than what is left
-    assert s._slices([A(10, 10, 64, 62)]) == [2]
+    assert s._slices([A(10, 10, 64, 62)], 1.0) == [2]
+    # by age: a job waiting 3 s takes three times the share of one waiting 1 s
+    s.policy = "age"
+    assert s._slices([A(10, 10, 64, 0, 9.0), A(10, 10, 64, 0, 7.0)], 10.0) == [2, 7]
+    with pytest.raises(ValueError):
+        serve.Server(None, None, policy="shortest")
 
 
 def test_percentiles_and_summary():
@@ -124,3 +130,39 @@ def test_equal_error_plan_never_raises_a_jobs_error():
             assert d == j and all(den[k] > target for k in E.CHECKPOINTS if k < j.spp)
     assert any(d.denoise for d in planned) and E.plain_error(cal, 4) == pytest.approx(0.5)
     assert E.plain_error(cal, 1024) == pytest.approx(plain[256] / 2)
+
+
+@pytest.mark.gpu
+def test_the_device_resolve_and_denoise_match_the_host_path(device):
+    """trt_accum_resolve_device gives trt_accum_resolve's means, edge-padded; and the server's denoising step (all on
+    the device) gives the same image as resolving to the host and denoising there."""
+    from threert_amd import render as R
+    from threert_amd.unet import UNet, shipped_params
+
+    rs, cam = R.cornell()
+    w, h, pw, ph = 72, 40, 80, 48
+    sc = R.upload(device, rs)
+    acc = R.Accum(device, w, h)
+    R.render(sc, rs.params(w, h, cam, seed=1), 0, 2, acc)
+    c, a, n, _ = acc.resolve()
+    bufs = [device.alloc(12 * pw * ph) for _ in range(3)]
+    acc.resolve_device(pw, ph, *bufs)
+    device.sync()
+    for b, want in zip(bufs, (c, a, n)):
+        got = b.download(np.float32, 3 * pw * ph).reshape(ph, pw, 3)
+        assert (got == np.pad(want, ((0, ph - h), (0, pw - w), (0, 0)), mode="edge")).all()
+        b.free()
+    net = UNet(device, pw * ph, shipped_params())
+    host = serve.denoise_image(net, c, a, n)
+    srv = serve.Server(device, lambda *k: None, unet=net, keep_images=True)
+
+    class Active:
+        job = serve.Job(0, (1, 1), w, h, 2, 2, denoise=True)
+
+    Active.acc = acc
+    on_device = srv._denoise(Active)
+    assert on_device.shape == host.shape and (on_device == host).all()
+    srv.close()
+    net.close()
+    acc.close()
+    sc.close()

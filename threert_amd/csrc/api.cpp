@@ -755,6 +755,19 @@ int trt_accum_resolve(trt_accum *a, float *color, float *albedo, float *normal, 
     return ok();
 }
 
+int trt_accum_resolve_device(trt_accum *a, uint32_t pw, uint32_t ph, float *d_color, float *d_albedo,
+                             float *d_normal) {
+    if (!a || !d_color || !d_albedo || !d_normal) return fail(TRT_E_INVALID, "trt_accum_resolve_device: null argument");
+    if (pw < a->w || ph < a->h) return fail(TRT_E_INVALID, "trt_accum_resolve_device: images smaller than the accumulator");
+    trt_device *d = a->dev;
+    std::lock_guard<std::mutex> lock(d->mu);
+    TRT_HIP(hipSetDevice(d->ordinal));
+    if (trt::launch_resolve(static_cast<const float4 *>(a->buf), a->w, a->h, pw, ph, d_color, d_albedo, d_normal,
+                            d->stream))
+        return fail(TRT_E_DEVICE, "resolve kernel launch failed");
+    return ok();
+}
+
 // One job's arguments, checked against its scene and accumulator.
 static int job_args(const trt_render_job &j, trt::RenderArgs &a, std::string *why) {
     trt_scene *s = j.scene;

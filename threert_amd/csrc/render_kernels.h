@@ -41,6 +41,9 @@ struct RenderJob {
 };
 
 int render_occupancy(int *blocks_per_cu);
+// accum.hip: the accumulator's color / albedo / normal means into pw x ph images (edge rows and columns repeated)
+int launch_resolve(const float4 *acc, uint32_t w, uint32_t h, uint32_t pw, uint32_t ph, float *color, float *albedo,
+                   float *normal, hipStream_t s);
 // d_jobs: njobs jobs in device memory, tile_first ascending from 0; total_tiles: all their tiles.
 // rays: two counters the launch adds its traced rays to (closest; shadow and any-hit)
 int launch_render(const RenderJob *d_jobs, uint32_t njobs, uint32_t total_tiles, uint32_t *counter,

@@ -89,7 +89,8 @@ def job_error(c: dict, j: serve.Job) -> float:
     return c["denoised"][j.spp] if j.denoise else plain_error(c, j.spp)
 
 
-def run(dev: Device, n_jobs: int = 512, rate: float = 40.0, seed: int = 1, log=print) -> dict:
+def run(dev: Device, n_jobs: int = 512, rate: float = 40.0, seed: int = 1, log=print,
+        policies: tuple = ("equal", "age")) -> dict:
     jobs = serve.workload(n_jobs=n_jobs, rate=rate, seed=seed)
     keys = sorted({j.scene for j in jobs})
     net = UNet(dev, 1920 * 1088, params=shipped_params())
@@ -106,8 +107,9 @@ def run(dev: Device, n_jobs: int = 512, rate: float = 40.0, seed: int = 1, log=p
                 f"{c['denoised'][256]:.3f} ({time.perf_counter() - t0:.0f} s)")
     dn_jobs = plan(jobs, cal)
     runs = {}
-    for name, js, unet in (("plain", jobs, None), ("denoised", dn_jobs, net)):
-        srv = serve.Server(dev, serve.soup_scene, budget_bytes=200 << 30, unet=unet)
+    for name, js, unet, policy in [(f"{m}_{pol}", js, un, pol) for pol in policies
+                                   for m, js, un in (("plain", jobs, None), ("denoised", dn_jobs, net))]:
+        srv = serve.Server(dev, serve.soup_scene, budget_bytes=200 << 30, unet=unet, policy=policy)
         w0 = time.perf_counter()
         res = srv.run(js, prefetch=False,
                       progress=lambda d, a, w, name=name: log(f"equal error: {name} {d} done, {a} rendering, {w} waiting"))
@@ -121,7 +123,9 @@ def run(dev: Device, n_jobs: int = 512, rate: float = 40.0, seed: int = 1, log=p
         log(f"equal error: {name}: {s}")
     net.close()
     per_job_ok = all(job_error(cal[d.scene], d) <= plain_error(cal[j.scene], j.spp) for d, j in zip(dn_jobs, jobs))
-    return {"runs": runs, "p99_ratio": round(runs["denoised"]["p99_s"] / runs["plain"]["p99_s"], 4),
+    return {"runs": runs,
+            "p99_ratio": {pol: round(runs[f"denoised_{pol}"]["p99_s"] / runs[f"plain_{pol}"]["p99_s"], 4)
+                          for pol in policies},
             "each_job_error_at_most_plain": per_job_ok,
             "calibration": {f"{k[0]}:{k[1]}": {m: {str(n): round(e, 4) for n, e in v[m].items()}
                                                for m in ("plain", "denoised")} for k, v in cal.items()},

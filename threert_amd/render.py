@@ -343,6 +343,10 @@ class Accum:
         _check(lib().trt_accum_resolve(self.handle, _ptr(c), _ptr(a), _ptr(n), _ptr(v)))
         return c, a, n, v
 
+    def resolve_device(self, pw: int, ph: int, color, albedo, normal):
+        """trt_accum_resolve_device: the means into device buffers of pw x ph x 3 floats (queued on the stream)."""
+        _check(lib().trt_accum_resolve_device(self.handle, pw, ph, color.p, albedo.p, normal.p))
+
     def close(self):
         if self.handle:
             lib().trt_accum_release(self.handle)
@@ -371,9 +375,10 @@ def _sig():
     L.trt_accum_clear.argtypes = [P]
     L.trt_accum_read.argtypes = [P, P]
     L.trt_accum_resolve.argtypes = [P, P, P, P, P]
+    L.trt_accum_resolve_device.argtypes = [P, U32, U32, P, P, P]
     L.trt_render.argtypes = [P, P, U32, U32, P]
     for n in ("trt_accum_create", "trt_accum_release", "trt_accum_clear", "trt_accum_read", "trt_accum_resolve",
-              "trt_render"):
+              "trt_accum_resolve_device", "trt_render"):
         getattr(L, n).restype = I
 
 

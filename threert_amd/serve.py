@@ -205,9 +205,13 @@ class Server:
     one Result per job."""
 
     def __init__(self, dev: Device, make_scene, budget_bytes: int = 200 << 30, launch_samples: float = 4e7,
-                 keep_images: bool = False, max_jobs: int = 256, workers: int = 8, unet=None):
+                 keep_images: bool = False, max_jobs: int = 256, workers: int = 8, unet=None, policy: str = "equal"):
         self.dev = dev
         self.unet = unet  # a threert_amd.unet.UNet for jobs with `denoise` set
+        if policy not in ("equal", "age"):
+            raise ValueError(f"unknown policy {policy!r}")
+        self.policy = policy
+        self._dn = None  # device buffers of the denoising step
         self.store = SceneStore(dev, make_scene, workers)
         self.budget = budget_bytes
         self.launch_samples = launch_samples  # path samples one launch aims at (all jobs together)
@@ -216,11 +220,38 @@ class Server:
         self.launches = 0
         self.samples = 0
 
-    def _slices(self, active: list[_Active]) -> list[int]:
-        """Samples each active job takes this launch: an equal share of the launch's samples, at least one."""
-        share = self.launch_samples / max(len(active), 1)
+    def _denoise(self, a: _Active):
+        """The finished job's image through the learned denoiser, all on the device: the accumulator resolved into
+        edge-padded buffers (sides multiples of 16), the network, one wait; the image comes back to the host only
+        when the server keeps images."""
+        w, h = a.job.width, a.job.height
+        pw, ph = w + (-w) % 16, h + (-h) % 16
+        nbytes = 12 * pw * ph
+        if self._dn is None or self._dn[0].nbytes < nbytes:
+            if self._dn is not None:
+                for b in self._dn:
+                    b.free()
+            self._dn = [self.dev.alloc(max(nbytes, 12 * 1920 * 1088)) for _ in range(4)]
+        c, al, nm, out = self._dn
+        a.acc.resolve_device(pw, ph, c, al, nm)
+        self.unet.denoise_device(1, ph, pw, c, al, nm, out)
+        self.dev.sync()
+        if not self.keep_images:
+            return None
+        return out.download(np.float32, 3 * pw * ph).reshape(ph, pw, 3)[:h, :w]
+
+    def _slices(self, active: list[_Active], now: float) -> list[int]:
+        """Passes each active job takes this launch: its share of the launch's samples in whole passes over its
+        pixels, at least one. `equal`: the same share for every job; `age`: shares in proportion to the time each job
+        has waited since it arrived, so old jobs (large ones, mostly) catch up."""
+        if self.policy == "age":
+            ages = [max(now - a.job.arrival, 1e-3) for a in active]
+            tot = sum(ages)
+            shares = [self.launch_samples * g / tot for g in ages]
+        else:
+            shares = [self.launch_samples / max(len(active), 1)] * len(active)
         out = []
-        for a in active:
+        for a, share in zip(active, shares):
             px = a.job.width * a.job.height
             k = int(max(1, share // px))
             out.append(min(k, a.job.spp - a.done))
@@ -273,7 +304,7 @@ class Server:
                 else:
                     time.sleep(0.001)
                 continue
-            ks = self._slices(active)
+            ks = self._slices(active, now())
             render_batch(self.dev, [(a.entry.scene, a.params, a.done, k, a.acc) for a, k in zip(active, ks)])
             self.dev.sync()
             self.launches += 1
@@ -284,7 +315,7 @@ class Server:
                 if a.done >= a.job.spp:
                     img = None
                     if a.job.denoise:
-                        img = denoise_image(self.unet, *a.acc.resolve()[:3])
+                        img = self._denoise(a)
                     elif self.keep_images:
                         img = a.acc.resolve()[0]
                     results.append(Result(a.job.id, a.job.arrival, a.admitted, now(), a.job.spp * a.job.width * a.job.height,
@@ -299,6 +330,9 @@ class Server:
 
     def close(self):
         self.store.close()
+        for b in self._dn or []:
+            b.free()
+        self._dn = None
 
 
 # ---- the serving workload ----------------------------------------------------------------------------------------
