@@ -264,8 +264,9 @@ def main():
                           "config": {"workload": f"{args.serve_jobs} render jobs, Poisson arrivals at "
                                                  f"{args.serve_rate}/s, seeded scenes of 10k-4M triangles, "
                                                  "256x256-1920x1080, 1-256 spp, 2-6 bounces, 200 GB budget; "
-                                                 "the learned denoiser over 1 sample where its calibrated error is "
-                                                 "at most the job's plain error, plain tracing otherwise"},
+                                                 "each job at the fewest samples whose calibrated denoised error is "
+                                                 "at most its plain error, denoised on the device, plain tracing "
+                                                 "otherwise"},
                           "equal_error": r}), flush=True)
         return
     if args.serve:
