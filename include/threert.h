@@ -200,6 +200,11 @@ int trt_accum_resolve(trt_accum *acc, float *color, float *albedo, float *normal
  * a trip through host memory (the denoiser wants sides that are multiples of 16). No counterpart in the reference,
  * whose denoisers read the frame back first (denoise.rs:10-103). */
 int trt_accum_resolve_device(trt_accum *acc, uint32_t pw, uint32_t ph, float *d_color, float *d_albedo, float *d_normal);
+/* Converged reference images in FP64 on the device: d_sum (4 doubles a pixel, width x height, zeroed by the caller)
+ * += this accumulator's color sums (a non-finite sum adds zero) and its sample count; the caller renders a
+ * reference in launches of a few samples, each into a cleared accumulator, and adds each once. No counterpart in the
+ * reference, which keeps FP32 sums only (records.wgsl Pixel). */
+int trt_accum_add_fp64(trt_accum *acc, double *d_sum);
 
 /* Samples first_sample .. first_sample + spp - 1 of every pixel added to `acc` (gpu.rs:4177-4196: the sample index
  * is P.size.z), on the device's stream. params: the 624-byte Params block (pathtrace.wgsl:96-140); its size.xy

@@ -205,6 +205,30 @@ def test_render_ray_counts(device):
 
 
 @pytest.mark.gpu
+def test_the_fp64_reference_is_the_float64_sum_of_the_launches(device):
+    """render.reference_fp64 (trt_accum_add_fp64 on the device) equals adding each launch's FP32 sums in float64 on
+    the host, bit for bit, non-finite sums dropped the same way."""
+    rs, cam = R.cornell()
+    w, h = 40, 32
+    p = rs.params(w, h, cam, seed=1, max_bounces=4, rr_depth=3)
+    sc = R.upload(device, rs)
+    got = R.reference_fp64(device, sc, p, w, h, 96, chunk=16)
+    total = np.zeros((h, w, 3), np.float64)
+    count = np.zeros((h, w, 1), np.float64)
+    for s in range(0, 96, 16):
+        acc = R.Accum(device, w, h)
+        R.render(sc, p, s, 16, acc)
+        px = acc.read()
+        acc.close()
+        fin = np.isfinite(px[..., 0:3]).all(axis=2, keepdims=True)
+        total += np.where(fin, px[..., 0:3], 0.0)
+        count += px[..., 3:4]
+    sc.close()
+    want = total / np.maximum(count, 1.0)
+    assert got.dtype == np.float64 and (got == want).all()
+
+
+@pytest.mark.gpu
 def test_gpu_white_furnace(device):
     rs, cam = sphere_scene((1, 1, 1), 0.0, 1.0, 0.0)
     p = rs.params(64, 64, cam, max_bounces=8, rr_depth=8, sky=(1.0, 1.0, 1.0))

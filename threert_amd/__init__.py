@@ -42,7 +42,7 @@ EXPORTS = [
     "trt_scene_stats_get", "trt_trace_rays", "trt_trace_rays_device", "trt_trace_reads",
     "trt_accum_create", "trt_accum_release", "trt_accum_clear", "trt_accum_read", "trt_accum_resolve", "trt_render",
     "trt_denoise_atrous", "trt_render_batch", "trt_mem_copy", "trt_trace_check_fp64", "trt_render_ray_counts",
-    "trt_accum_resolve_device",
+    "trt_accum_resolve_device", "trt_accum_add_fp64",
     "trt_unet_create", "trt_unet_release", "trt_unet_param_count", "trt_unet_params_set", "trt_unet_params_get",
     "trt_unet_denoise", "trt_unet_gradients", "trt_unet_train_step",
 ]

@@ -755,6 +755,16 @@ int trt_accum_resolve(trt_accum *a, float *color, float *albedo, float *normal, 
     return ok();
 }
 
+int trt_accum_add_fp64(trt_accum *a, double *d_sum) {
+    if (!a || !d_sum) return fail(TRT_E_INVALID, "trt_accum_add_fp64: null argument");
+    trt_device *d = a->dev;
+    std::lock_guard<std::mutex> lock(d->mu);
+    TRT_HIP(hipSetDevice(d->ordinal));
+    if (trt::launch_add64(static_cast<const float4 *>(a->buf), uint64_t(a->w) * a->h, d_sum, d->stream))
+        return fail(TRT_E_DEVICE, "FP64 accumulation kernel launch failed");
+    return ok();
+}
+
 int trt_accum_resolve_device(trt_accum *a, uint32_t pw, uint32_t ph, float *d_color, float *d_albedo,
                              float *d_normal) {
     if (!a || !d_color || !d_albedo || !d_normal) return fail(TRT_E_INVALID, "trt_accum_resolve_device: null argument");

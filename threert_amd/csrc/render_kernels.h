@@ -41,6 +41,8 @@ struct RenderJob {
 };
 
 int render_occupancy(int *blocks_per_cu);
+// accum.hip: an accumulator's color sums (non-finite ones as zero) and count added to an FP64 image, 4 doubles a pixel
+int launch_add64(const float4 *acc, uint64_t n, double *sum, hipStream_t s);
 // accum.hip: the accumulator's color / albedo / normal means into pw x ph images (edge rows and columns repeated)
 int launch_resolve(const float4 *acc, uint32_t w, uint32_t h, uint32_t pw, uint32_t ph, float *color, float *albedo,
                    float *normal, hipStream_t s);

@@ -3,8 +3,8 @@
 Data: seeded scenes from the Cornell-box and clustered-soup generators (the plain and precision workloads) and
 small instanced cities; per scene, inputs (color, albedo, normal from the accumulator) read from one render after
 1, 2, 4, ... 128 samples, so the network meets every noise level a server would hand it, and one many-sample
-target. The target is rendered in launches of `chunk` samples, each into a fresh accumulator, and the launches'
-sums are added on the host in float64 (an FP64-accumulated reference).
+target. The target is rendered in launches of `chunk` samples, and the launches' sums are added on the GPU in FP64
+(render.reference_fp64, an FP64-accumulated reference).
 
     python -m threert_amd.denoise_train --scenes 200 --steps 12000 --out weights.npy
 """
@@ -93,18 +93,8 @@ def render_set(dev: Device, kind: str, seed: int, w: int, h: int, noisy: int, ta
         c, a, n, _ = acc.resolve()
         inputs[k, ..., 0:3], inputs[k, ..., 3:6], inputs[k, ..., 6:9] = c, a, n
         acc.close()
-    total = np.zeros((h, w, 3), np.float64)
-    count = np.zeros((h, w, 1), np.float64)
-    for s in range(0, target_spp, chunk):
-        acc = R.Accum(dev, w, h)
-        R.render(sc, p, s, min(chunk, target_spp - s), acc)
-        px = acc.read()
-        acc.close()
-        fin = np.isfinite(px[..., 0:3]).all(axis=2, keepdims=True)
-        total += np.where(fin, px[..., 0:3], 0.0)
-        count += px[..., 3:4]
+    target = R.reference_fp64(dev, sc, p, w, h, target_spp, chunk).astype(np.float32)
     sc.close()
-    target = (total / np.maximum(count, 1.0)).astype(np.float32)
     return inputs, target
 
 

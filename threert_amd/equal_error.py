@@ -28,21 +28,6 @@ CAL_W, CAL_H = 320, 176
 CHECKPOINTS = (1, 2, 4, 8, 16, 32, 64, 128, 256)
 
 
-def reference(dev: Device, sc, p, w: int, h: int, spp: int, chunk: int = 64) -> np.ndarray:
-    """Samples 0 .. spp - 1 summed in float64 across launches of `chunk` samples: the converged image."""
-    total = np.zeros((h, w, 3), np.float64)
-    count = np.zeros((h, w, 1), np.float64)
-    for s in range(0, spp, chunk):
-        acc = R.Accum(dev, w, h)
-        R.render(sc, p, s, min(chunk, spp - s), acc)
-        px = acc.read()
-        acc.close()
-        fin = np.isfinite(px[..., 0:3]).all(axis=2, keepdims=True)
-        total += np.where(fin, px[..., 0:3], 0.0)
-        count += px[..., 3:4]
-    return total / np.maximum(count, 1.0)
-
-
 def calibrate(dev: Device, net: UNet, built, ref_spp: int = 1024, bounces: int = 4, w: int = CAL_W,
               h: int = CAL_H) -> dict:
     """built = soup_scene's (rs, cam, sky). The plain and the denoised error after each checkpoint's samples (sample
@@ -50,7 +35,7 @@ def calibrate(dev: Device, net: UNet, built, ref_spp: int = 1024, bounces: int =
     rs, cam, sky = built
     p = rs.params(w, h, cam, seed=1, max_bounces=bounces, rr_depth=3, sky=sky)
     sc = R.upload(dev, rs)
-    ref = reference(dev, sc, p, w, h, ref_spp)
+    ref = R.reference_fp64(dev, sc, p, w, h, ref_spp)
     acc = R.Accum(dev, w, h)
     plain, den, done = {}, {}, 0
     for k in CHECKPOINTS:

@@ -343,6 +343,11 @@ class Accum:
         _check(lib().trt_accum_resolve(self.handle, _ptr(c), _ptr(a), _ptr(n), _ptr(v)))
         return c, a, n, v
 
+    def add_fp64(self, total):
+        """trt_accum_add_fp64: this accumulator's color sums and count added to the FP64 image `total` (a device
+        buffer of width x height x 4 doubles), queued on the stream."""
+        _check(lib().trt_accum_add_fp64(self.handle, total.p))
+
     def resolve_device(self, pw: int, ph: int, color, albedo, normal):
         """trt_accum_resolve_device: the means into device buffers of pw x ph x 3 floats (queued on the stream)."""
         _check(lib().trt_accum_resolve_device(self.handle, pw, ph, color.p, albedo.p, normal.p))
@@ -357,6 +362,24 @@ class Accum:
             self.close()
         except Exception:
             pass
+
+
+def reference_fp64(dev: Device, scene: Scene, params: np.ndarray, width: int, height: int, spp: int,
+                   chunk: int = 64) -> np.ndarray:
+    """A converged reference image: samples 0 .. spp - 1 rendered in launches of `chunk` samples, each launch's FP32
+    sums added once into an FP64 image on the device (trt_accum_add_fp64); the mean color (height, width, 3) in
+    float64."""
+    total = dev.alloc(32 * width * height)
+    total.upload(np.zeros(4 * width * height, np.float64))
+    acc = Accum(dev, width, height)
+    for s in range(0, spp, chunk):
+        acc.clear()
+        render(scene, params, s, min(chunk, spp - s), acc)
+        acc.add_fp64(total)
+    out = total.download(np.float64, 4 * width * height).reshape(height, width, 4)
+    acc.close()
+    total.free()
+    return out[..., 0:3] / np.maximum(out[..., 3:4], 1.0)
 
 
 def render(scene: Scene, params: np.ndarray, first_sample: int, spp: int, acc: Accum):
@@ -376,9 +399,10 @@ def _sig():
     L.trt_accum_read.argtypes = [P, P]
     L.trt_accum_resolve.argtypes = [P, P, P, P, P]
     L.trt_accum_resolve_device.argtypes = [P, U32, U32, P, P, P]
+    L.trt_accum_add_fp64.argtypes = [P, P]
     L.trt_render.argtypes = [P, P, U32, U32, P]
     for n in ("trt_accum_create", "trt_accum_release", "trt_accum_clear", "trt_accum_read", "trt_accum_resolve",
-              "trt_accum_resolve_device", "trt_render"):
+              "trt_accum_resolve_device", "trt_accum_add_fp64", "trt_render"):
         getattr(L, n).restype = I
 
 
