@@ -87,6 +87,25 @@ def fp64_check(dev, sc, d_rays, d_hits, hits) -> dict:
                           "max": float(rel.max()) if len(rel) else 0.0}}
 
 
+def binary_walk(dev, sc, d_rays, d_hits, wide_hits, launches: int = 3) -> dict:
+    """The same batch through the reference's FP32 binary walk (its own 32-byte nodes, 3 node reads an interior step,
+    leaves of up to 16 triangles): rays/s, what it reads, and whether its hits are the wide walk's, bit for bit."""
+    reads = sc.reads(d_rays, N_RAYS, d_hits, T.WALK_BINARY)
+    sc.trace_device(d_rays, N_RAYS, d_hits, T.WALK_BINARY)
+    dev.sync()
+    dev.timer_start()
+    for _ in range(launches):
+        sc.trace_device(d_rays, N_RAYS, d_hits, T.WALK_BINARY)
+    ms = dev.timer_stop() / launches
+    got = d_hits.download(T.HIT, N_RAYS)
+    same = bool((got.view(np.uint32) == wide_hits.view(np.uint32)).all())
+    return {"mrays_s": round(N_RAYS / (ms * 1e-3) / 1e6, 1), "ms_per_launch": round(ms, 3),
+            "reads_per_ray": {"nodes": round(reads["nodes"] / N_RAYS, 2), "tris": round(reads["tris"] / N_RAYS, 2),
+                              "records": round(reads["units"] / N_RAYS, 2),
+                              "bytes": round(reads["bytes"] / N_RAYS, 1)},
+            "hits_equal_to_wide": same}
+
+
 def hit_agreement(prep, rays: np.ndarray, hits: np.ndarray, sample: int) -> dict:
     """The GPU's hits against the oracle's exact walk (lowest (t, instance, triangle)) over the first `sample` rays:
     the fraction whose whole hit record is the same bits."""
@@ -316,6 +335,8 @@ def main():
     log(f"bench: {ms_step:.3f} ms/step, {mrays:.1f} Mrays/s, wall {wall:.2f} s, hit rate {hit_rate:.3f}")
     fp64 = fp64_check(dev, sc, d_rays, d_hits, hits)
     log(f"bench: FP64 check {fp64}")
+    binary = binary_walk(dev, sc, d_rays, d_hits, hits)
+    log(f"bench: binary walk {binary}")
 
     achieved = reads["bytes"] / (ms_step * 1e-3) / 1e9
     out = {
@@ -342,6 +363,7 @@ def main():
                               "bytes": round(reads["bytes"] / N_RAYS, 1)},
             "wall_s_timed": round(wall, 3),
             "fp64_check": fp64,
+            "binary_walk": binary,
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic()},
