@@ -1,0 +1,115 @@
+"""The `serve` wire format (main.rs:345-398): job lines in, a ready line and one reply a job out; refused jobs get
+`ok: false` with the reason (and "unsupported" for what the library does not render, as tools/test-render.ts reads
+it); on the GPU, each job's HDR file holds the image a lone render of it gives, bit for bit."""
+import io
+import json
+
+import numpy as np
+import pytest
+
+from threert_amd import wire as W
+
+
+def test_a_job_line_and_its_defaults():
+    jid, job, outs = W.parse_job('{"id": "a7", "scene": {"kind": "cornell"}, "hdr": "x.raw"}', 3)
+    assert jid == "a7" and job.id == 3 and job.scene == ("cornell",)
+    assert (job.width, job.height, job.spp, job.bounces, job.seed) == (1280, 720, 64, 4, 1)
+    assert outs == {"hdr": "x.raw"} and job.keep and not job.denoise and job.camera is None
+    _, job, outs = W.parse_job(json.dumps({"id": 1, "scene": {"kind": "soup", "triangles": 5000, "seed": 9},
+                                           "spp": 8, "width": 64, "height": 32.0, "bounces": 2, "seed": 77,
+                                           "camera": {"position": [0, 1, -5], "target": [0, 0, 0], "fov": 30}}), 0)
+    assert job.scene == ("soup", 5000, 9) and (job.width, job.height, job.spp, job.bounces, job.seed) == (64, 32, 8, 2, 77)
+    assert job.camera.position == (0.0, 1.0, -5.0) and job.camera.fov_deg == 30.0 and outs == {} and not job.keep
+    _, job, _ = W.parse_job('{"scene": {"kind": "city", "seed": 4, "meshes": 8, "instances": 100}}', 0)
+    assert job.scene == ("city", 4, 8, 100)
+
+
+@pytest.mark.parametrize("line, says", [
+    ("not json", "not JSON"),
+    ("[1, 2]", "JSON object"),
+    ('{"id": 1}', "scene"),
+    ('{"id": 1, "game": "g.js", "spp": 4}', "unsupported"),
+    ('{"id": 1, "scene": {"kind": "cornell"}, "live": 30}', "unsupported"),
+    ('{"id": 1, "scene": {"kind": "cornell"}, "overrides": {"a": 1}}', "unsupported"),
+    ('{"id": 1, "scene": {"kind": "teapot"}}', "unknown scene"),
+    ('{"id": 1, "scene": {"kind": "cornell"}, "spp": 0}', "spp must be in"),
+    ('{"id": 1, "scene": {"kind": "cornell"}, "spp": 2.5}', "whole number"),
+    ('{"id": 1, "scene": {"kind": "cornell"}, "width": true}', "whole number"),
+    ('{"id": 1, "scene": {"kind": "cornell"}, "height": 20000}', "height must be in"),
+    ('{"id": 1, "scene": {"kind": "soup", "triangles": 0}}', "triangles must be in"),
+    ('{"id": 1, "scene": {"kind": "cornell"}, "camera": {"position": [0, 0], "target": [0, 0, 1]}}', "position"),
+    ('{"id": 1, "scene": {"kind": "cornell"}, "camera": {"position": [1, 1, 1], "target": [1, 1, 1]}}', "position =="),
+    ('{"id": 1, "scene": {"kind": "cornell"}, "camera": {"position": [0, 0, 0], "target": [0, 0, 1], "fov": 180}}', "fov"),
+    ('{"id": 1, "scene": {"kind": "cornell"}, "denoise": "yes"}', "denoise"),
+    ('{"id": 1, "scene": {"kind": "cornell"}, "hdr": 5}', "hdr must be"),
+])
+def test_refused_jobs_say_why(line, says):
+    with pytest.raises(W.JobError) as e:
+        W.parse_job(line, 0)
+    assert says in str(e.value)
+
+
+def test_the_reader_replies_to_refused_lines_and_queues_the_rest():
+    lines = "\n".join([
+        '{"id": 1, "scene": {"kind": "cornell"}, "spp": 2, "width": 8, "height": 8}',
+        "",
+        '{"id": 2, "game": "bundle.js"}',
+        '{"id": 3, "scene": {"kind": "cornell"}, "denoise": true}',
+        '{"id": 4, "scene": {"kind": "soup", "triangles": 100}, "spp": 1}',
+        "{oops",
+    ]) + "\n"
+    out = io.StringIO()
+    ls = W.LineServer(None, io.StringIO(lines), out, workers=1)
+    ls._read()
+    replies = [json.loads(x) for x in out.getvalue().splitlines()]
+    assert [(r["id"], r["ok"]) for r in replies] == [(2, False), (3, False), (None, False)]
+    assert replies[0]["error"].startswith("unsupported") and replies[1]["error"].startswith("unsupported")
+    jobs = ls._arrivals(0.0)
+    assert [j.id for j in jobs] == [0, 1] and [ls.jobs[j.id][0] for j in jobs] == [1, 4]
+    assert all(j.arrival >= 0.0 for j in jobs) and ls._closed()
+    ls.close()
+
+
+@pytest.mark.gpu
+def test_jobs_over_the_wire_are_their_lone_renders(device, tmp_path):
+    from threert_amd import render as R
+    from threert_amd import serve as V
+
+    print("wire: 3 jobs (Cornell with hdr + png, a refused line, a soup with a camera)", flush=True)
+    jobs = [
+        {"id": "c", "scene": {"kind": "cornell"}, "spp": 5, "width": 48, "height": 40, "bounces": 3, "seed": 7,
+         "hdr": str(tmp_path / "c.raw"), "out": str(tmp_path / "c.png")},
+        {"id": "bad", "scene": {"kind": "cornell"}, "spp": -1},
+        {"id": 9, "scene": {"kind": "soup", "triangles": 3000, "seed": 2}, "spp": 3, "width": 33, "height": 17,
+         "bounces": 2, "camera": {"position": [10.0, 60.0, -120.0], "target": [0.0, 0.0, 0.0], "fov": 35},
+         "hdr": str(tmp_path / "s.raw")},
+    ]
+    out = io.StringIO()
+    ls = W.LineServer(device, io.StringIO("\n".join(json.dumps(j) for j in jobs) + "\n"), out, workers=2,
+                      launch_samples=2000)
+    ls.run(0.0)
+    ls.close()
+    replies = [json.loads(x) for x in out.getvalue().splitlines()]
+    print(replies, flush=True)
+    assert replies[0]["ready"] is True and len(replies) == 4
+    by = {r["id"]: r for r in replies[1:]}
+    assert by["bad"]["ok"] is False and "spp" in by["bad"]["error"]
+    for j in (jobs[0], jobs[2]):
+        r = by[j["id"]]
+        assert r["ok"], r
+        st = r["stats"]
+        assert (st["width"], st["height"], st["spp"]) == (j["width"], j["height"], j["spp"]) and st["samples_per_sec"] > 0
+        _, job, _ = W.parse_job(json.dumps(j), 0)
+        rs, cam, sky = W.make_scene(*job.scene)
+        sc = R.upload(device, rs)
+        acc = R.Accum(device, job.width, job.height)
+        R.render(sc, V.params_of(V._Resident(None, rs, cam, sky, 0), job), 0, job.spp, acc)
+        device.sync()
+        want = acc.resolve()[0]
+        got = np.fromfile(j["hdr"], "<f4").reshape(job.height, job.width, 3)
+        assert (got.view(np.uint32) == want.view(np.uint32)).all(), f"job {j['id']} differs from its lone render"
+        acc.close()
+        sc.close()
+    from threert_amd import output as O
+
+    assert (O.read_rgb8(jobs[0]["out"]) == O.srgb8(np.fromfile(jobs[0]["hdr"], "<f4").reshape(40, 48, 3))).all()

@@ -743,14 +743,14 @@ int trt_accum_resolve(trt_accum *a, float *color, float *albedo, float *normal, 
     for (uint64_t i = 0; i < n; i++) {
         float p[20];
         for (int k = 0; k < 20; k++) p[k] = std::isfinite(px[20 * i + k]) ? px[20 * i + k] : 0.0f;
-        const float c = std::max(px[20 * i + 3], 1.0f);
+        const float c = std::fmax(px[20 * i + 3], 1.0f);  // f32::max: a NaN count gives 1
         for (int k = 0; k < 3; k++) {
             if (color) color[3 * i + k] = p[k] / c;
             if (albedo) albedo[3 * i + k] = p[4 + k] / c;
             if (normal) normal[3 * i + k] = p[8 + k] / c;
         }
         const float mean = p[7] / c;
-        if (variance) variance[i] = std::max(p[11] / c - mean * mean, 0.0f) / c;
+        if (variance) variance[i] = std::fmax(p[11] / c - mean * mean, 0.0f) / c;
     }
     return ok();
 }

@@ -62,6 +62,8 @@ class Job:
     arrival: float = 0.0  # seconds from the start of the run
     seed: int = 1
     denoise: bool = False  # the finished image goes through the server's learned denoiser
+    keep: bool = False     # the finished image comes back to the host (also when the server keeps no images)
+    camera: object = None  # a render.Camera in place of the scene's own
 
 
 @dataclass
@@ -73,6 +75,7 @@ class Result:
     samples: int
     image: np.ndarray | None = None
     denoised: bool = False
+    error: str | None = None  # the job failed (its scene could not be built or uploaded); it rendered nothing
 
     @property
     def latency(self) -> float:
@@ -188,8 +191,9 @@ def denoise_image(net, color: np.ndarray, albedo: np.ndarray, normal: np.ndarray
 
 
 def params_of(e: _Resident, j: Job) -> np.ndarray:
-    """The job's Params: its size, seed and bounces over the scene's camera and sky (RR from depth 3)."""
-    return e.rs.params(j.width, j.height, e.cam, seed=j.seed, max_bounces=j.bounces, rr_depth=3, sky=e.sky)
+    """The job's Params: its size, seed and bounces over the scene's camera (or the job's) and sky (RR from depth 3)."""
+    return e.rs.params(j.width, j.height, j.camera or e.cam, seed=j.seed, max_bounces=j.bounces, rr_depth=3,
+                       sky=e.sky)
 
 
 def scene_bytes(rs: R.RenderScene) -> int:
@@ -219,6 +223,7 @@ class Server:
         self.max_jobs = max_jobs
         self.launches = 0
         self.samples = 0
+        self.t0 = time.perf_counter()  # the clock arrivals are measured on: reset when a run starts
 
     def _denoise(self, a: _Active):
         """The finished job's image through the learned denoiser, all on the device: the accumulator resolved into
@@ -236,7 +241,7 @@ class Server:
         a.acc.resolve_device(pw, ph, c, al, nm)
         self.unet.denoise_device(1, ph, pw, c, al, nm, out)
         self.dev.sync()
-        if not self.keep_images:
+        if not (self.keep_images or a.job.keep):
             return None
         return out.download(np.float32, 3 * pw * ph).reshape(ph, pw, 3)[:h, :w]
 
@@ -265,22 +270,45 @@ class Server:
             for j in jobs:
                 self.store.prefetch(j.scene)
         pending = deque(jobs)
+        results: list[Result] = []
+
+        def arrivals(t):
+            out = []
+            while pending and pending[0].arrival <= t:
+                out.append(pending.popleft())
+            return out
+
+        def idle(t):
+            time.sleep(max(0.0, min(pending[0].arrival - t, 0.002)) if pending else 0.001)
+
+        def tick(active, waiting):
+            if progress:
+                progress(len(results), active, waiting)
+
+        self.serve(arrivals, lambda: not pending, results.append, idle, tick, every)
+        return sorted(results, key=lambda r: r.id)
+
+    def serve(self, arrivals, closed, emit, idle, tick=None, every: float = 5.0) -> None:
+        """The serving loop over a source of jobs: `arrivals(t)` returns the jobs that have arrived by `t` seconds
+        (their `arrival` set), `closed()` says no more will come, `idle(t)` waits a little when nothing is active;
+        `emit(result)` gets each job's Result as it finishes (or fails); `tick(active, waiting)` at most every
+        `every` seconds. Returns when the source is closed and every job has finished."""
         waiting: deque[Job] = deque()
         active: list[_Active] = []
-        results: list[Result] = []
         used = 0
-        t0 = time.perf_counter()
-        now = lambda: time.perf_counter() - t0  # noqa: E731
+        self.t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - self.t0  # noqa: E731
         told = 0.0
-        while pending or waiting or active:
+        while True:
             t = now()
-            if progress and t - told >= every:
-                told = t
-                progress(len(results), len(active), len(waiting))
-            while pending and pending[0].arrival <= t:
-                j = pending.popleft()
+            for j in arrivals(t):
                 self.store.prefetch(j.scene)
                 waiting.append(j)
+            if not (waiting or active) and closed():
+                break
+            if tick and t - told >= every:
+                told = t
+                tick(len(active), len(waiting))
             # admission in arrival order among jobs whose scene is built; the first that does not fit the budget
             # stops admission, so a large job is not overtaken for ever by small ones
             for j in list(waiting):
@@ -288,7 +316,12 @@ class Server:
                     break
                 if not self.store.ready(j.scene):
                     continue
-                need = j.width * j.height * ACCUM_BYTES + self.store.bytes_of(j.scene)
+                try:
+                    need = j.width * j.height * ACCUM_BYTES + self.store.bytes_of(j.scene)
+                except Exception as e:  # the scene's build or upload failed: the job fails, the server goes on
+                    waiting.remove(j)
+                    emit(Result(j.id, j.arrival, now(), now(), 0, None, False, f"{type(e).__name__}: {e}"))
+                    continue
                 if used + need > self.budget:
                     used -= self.store.evict_for(need, used, self.budget)
                     if used + need > self.budget:
@@ -299,10 +332,7 @@ class Server:
                 acc = R.Accum(self.dev, j.width, j.height)
                 active.append(_Active(j, e, acc, params_of(e, j), 0, now()))
             if not active:
-                if pending:
-                    time.sleep(max(0.0, min(pending[0].arrival - now(), 0.002)))
-                else:
-                    time.sleep(0.001)
+                idle(now())
                 continue
             ks = self._slices(active, now())
             render_batch(self.dev, [(a.entry.scene, a.params, a.done, k, a.acc) for a, k in zip(active, ks)])
@@ -313,20 +343,20 @@ class Server:
                 a.done += k
                 self.samples += k * a.job.width * a.job.height
                 if a.done >= a.job.spp:
+                    keep = self.keep_images or a.job.keep
                     img = None
                     if a.job.denoise:
                         img = self._denoise(a)
-                    elif self.keep_images:
+                    elif keep:
                         img = a.acc.resolve()[0]
-                    results.append(Result(a.job.id, a.job.arrival, a.admitted, now(), a.job.spp * a.job.width * a.job.height,
-                                          img if self.keep_images else None, a.job.denoise))
+                    emit(Result(a.job.id, a.job.arrival, a.admitted, now(), a.job.spp * a.job.width * a.job.height,
+                                img if keep else None, a.job.denoise))
                     a.acc.close()
                     used -= a.job.width * a.job.height * ACCUM_BYTES
                     self.store.release(a.entry)
                 else:
                     still.append(a)
             active = still
-        return sorted(results, key=lambda r: r.id)
 
     def close(self):
         self.store.close()
