@@ -140,6 +140,10 @@ typedef struct {
     uint32_t max_stack;      /* the deepest a walk's stack can get over the 4-wide tree */
     uint32_t units;
     uint64_t device_bytes;   /* every device allocation the scene holds */
+    float build_ms;          /* the upload's wide-tree build (own mesh trees, collapse, quantisation), wall clock */
+    float device_build_ms;   /* of which the mesh trees' build on the device, first kernel to last */
+    uint32_t device_meshes;  /* mesh trees built on the device */
+    uint32_t host_meshes;    /* mesh trees built on the host (TRT_BUILD_HOST, or cases the device leaves to it) */
 } trt_scene_stats;
 
 /* Copies the host buffers to HBM and builds the 4-wide quantised tree from the node buffer. The caller's memory
@@ -148,6 +152,22 @@ typedef struct {
 int trt_scene_upload(trt_device *dev, const trt_scene_desc *desc, trt_scene **out);
 int trt_scene_release(trt_scene *scene);
 int trt_scene_stats_get(trt_scene *scene, trt_scene_stats *out);
+
+/* Where later uploads on this device build the library's own mesh trees: TRT_BUILD_DEVICE (the default; binned
+ * SAH one tree level at a time over every mesh together, on a stream of the upload's own) or TRT_BUILD_HOST (one
+ * host thread a mesh). Both give the same bytes; a mesh the device cannot build as the host would (a median split
+ * of distinct centroids past depth 64) is built on the host. No reference counterpart: the reference builds its
+ * trees on the host in `prepare` (lib.rs:836-913, bvh.rs:130-239); the library's own trees replace only the walk's
+ * copy of them. */
+#define TRT_BUILD_DEVICE 0u
+#define TRT_BUILD_HOST 1u
+int trt_device_set_build(trt_device *dev, uint32_t where);
+
+/* Copies one of the scene's wide-tree buffers to host memory: which = 0 the 64-byte wide nodes
+ * (stats.wide_nodes of them), 1 the leaves in the wide trees' order (48 bytes each, as many as the scene's leaves),
+ * 2 the 64-byte units (one a record). `bytes` must be the buffer's size. No reference counterpart (for checking
+ * that builds agree). */
+int trt_scene_wide_read(trt_scene *scene, uint32_t which, void *dst, uint64_t bytes);
 
 /* --- ray batches (trace_units.wgsl:44-135 via pathtrace.wgsl:764 `trace`; units.rs:1019 Kernel::trace) ------- */
 typedef struct {

@@ -31,6 +31,7 @@ assert NODE.itemsize == 32 and INSTANCE.itemsize == 160 and RAY.itemsize == 32 a
 TRT_OK, TRT_E_INVALID, TRT_E_OOM, TRT_E_DEVICE, TRT_E_GUARD = 0, 1, 2, 3, 4
 TRACE_CLOSEST, TRACE_SHADOW, TRACE_ANY = 0, 1, 2
 WALK_WIDE, WALK_BINARY = 0, 1
+BUILD_DEVICE, BUILD_HOST = 0, 1  # trt_device_set_build
 NO_HIT = 0xFFFFFFFF
 INF = np.float32(3.0e38)
 
@@ -42,7 +43,7 @@ EXPORTS = [
     "trt_scene_stats_get", "trt_trace_rays", "trt_trace_rays_device", "trt_trace_reads",
     "trt_accum_create", "trt_accum_release", "trt_accum_clear", "trt_accum_read", "trt_accum_resolve", "trt_render",
     "trt_denoise_atrous", "trt_render_batch", "trt_mem_copy", "trt_trace_check_fp64", "trt_render_ray_counts",
-    "trt_accum_resolve_device", "trt_accum_add_fp64",
+    "trt_accum_resolve_device", "trt_accum_add_fp64", "trt_device_set_build", "trt_scene_wide_read",
     "trt_unet_create", "trt_unet_release", "trt_unet_param_count", "trt_unet_params_set", "trt_unet_params_get",
     "trt_unet_denoise", "trt_unet_gradients", "trt_unet_train_step",
 ]
@@ -75,7 +76,8 @@ class _SceneDesc(C.Structure):
 class _SceneStats(C.Structure):
     _fields_ = [("wide_nodes", C.c_uint64), ("wide_bytes", C.c_uint64), ("binary_bytes", C.c_uint64),
                 ("leaf_bytes", C.c_uint64), ("unit_bytes", C.c_uint64), ("max_stack", C.c_uint32),
-                ("units", C.c_uint32), ("device_bytes", C.c_uint64)]
+                ("units", C.c_uint32), ("device_bytes", C.c_uint64), ("build_ms", C.c_float),
+                ("device_build_ms", C.c_float), ("device_meshes", C.c_uint32), ("host_meshes", C.c_uint32)]
 
 
 class _DeviceInfo(C.Structure):
@@ -109,6 +111,7 @@ def lib():
             "trt_trace_rays_device": [P, P, U64, P, U32], "trt_trace_reads": [P, P, U64, P, U32, P],
             "trt_trace_check_fp64": [P, P, P, U64, P],
             "trt_render_ray_counts": [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
+            "trt_device_set_build": [P, U32], "trt_scene_wide_read": [P, U32, P, U64],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -228,6 +231,10 @@ class Device:
     def alloc(self, nbytes: int) -> "DeviceBuffer":
         return DeviceBuffer(self, nbytes)
 
+    def set_build(self, where: int):
+        """Where later uploads build the library's own mesh trees: BUILD_DEVICE (default) or BUILD_HOST."""
+        _check(lib().trt_device_set_build(self.h, where))
+
     def timer_start(self):
         _check(lib().trt_timer_start(self.h))
 
@@ -326,6 +333,15 @@ class Scene:
         nodes, tris, units = (int(x) for x in out)
         return {"nodes": nodes, "tris": tris, "units": units,
                 "bytes": nodes * node_b + tris * 48 + units * unit_b}
+
+    def wide_read(self, which: int) -> np.ndarray:
+        """One of the wide-tree buffers as bytes (trt_scene_wide_read): 0 nodes, 1 leaves in the trees' order,
+        2 units."""
+        st = self.stats()
+        size = {0: st["wide_bytes"], 1: st["leaf_bytes"], 2: st["unit_bytes"]}[which]
+        out = np.zeros(size, np.uint8)
+        _check(lib().trt_scene_wide_read(self.h, which, _ptr(out) if size else None, size))
+        return out
 
     def check_fp64(self, rays: DeviceBuffer, n: int, hits: DeviceBuffer, t64: DeviceBuffer):
         """Queues the FP64 re-intersection of a batch's hits (trt_trace_check_fp64): t64 receives n doubles."""

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <vector>
 #include <cstdio>
@@ -16,6 +17,7 @@
 #include "denoise_kernels.h"
 #include "render_kernels.h"
 #include "trace_kernels.h"
+#include "gbuild.h"
 #include "trt_internal.h"
 
 namespace {
@@ -66,6 +68,7 @@ struct trt_device {
     uint64_t staging_bytes = 0;
     int cus = 0;
     int blocks_wide = 1, blocks_binary = 1, blocks_render = 1;
+    uint32_t build = TRT_BUILD_DEVICE;  // where uploads build the library's own mesh trees
     hipDeviceProp_t props{};
     std::mutex mu;
     std::atomic<int> refs{1};
@@ -522,24 +525,58 @@ int trt_scene_upload(trt_device *d, const trt_scene_desc *desc, trt_scene **out)
         return fail(TRT_E_INVALID, "trt_scene_upload: a shading table of 2^32 records or more");
     const auto *nodes = static_cast<const trt::Node *>(desc->nodes);
     const auto *inst = static_cast<const trt::Instance *>(desc->instances);
-    trt::WideScene w;
+    const auto *leaves = static_cast<const trt::Leaf *>(desc->leaves);
     std::string why;
-    if (!trt::build_wide(nodes, desc->node_count, static_cast<const trt::Leaf *>(desc->leaves), inst,
-                         desc->instance_count, desc->leaf_count, desc->top_first, desc->unit_count, w, &why))
+    std::vector<trt::MeshRef> meshes;
+    if (!trt::wide_meshes(nodes, desc->node_count, inst, desc->instance_count, desc->leaf_count, desc->top_first,
+                          desc->unit_count, meshes, &why))
         return fail(TRT_E_INVALID, "trt_scene_upload: " + why);
-    if (w.nodes.size() >= (uint64_t{1} << 30)) return fail(TRT_E_INVALID, "trt_scene_upload: tree too large");
     if (desc->energy && !check_shading(desc, &why)) return fail(TRT_E_INVALID, "trt_scene_upload: " + why);
-    std::lock_guard<std::mutex> lock(d->mu);
     TRT_HIP(hipSetDevice(d->ordinal));
     auto *s = new (std::nothrow) trt_scene();
     if (!s) return fail(TRT_E_OOM, "trt_scene_upload: out of host memory");
     s->dev = d;
     d->refs++;
     uint64_t total = 0;
-    int rc = upload(d, &s->nodes, desc->nodes, desc->node_count * sizeof(trt::Node), &total);
-    if (!rc) rc = upload(d, &s->leaves, desc->leaves, desc->leaf_count * sizeof(trt::Leaf), &total);
-    if (!rc) rc = upload(d, &s->instances, desc->instances, desc->instance_count * sizeof(trt::Instance), &total);
-    if (!rc) rc = upload(d, &s->wnodes, w.nodes.data(), w.nodes.size() * sizeof(trt::WideNode), &total);
+    int rc = TRT_OK;
+    {
+        std::lock_guard<std::mutex> lock(d->mu);
+        rc = upload(d, &s->nodes, desc->nodes, desc->node_count * sizeof(trt::Node), &total);
+        if (!rc) rc = upload(d, &s->leaves, desc->leaves, desc->leaf_count * sizeof(trt::Leaf), &total);
+        if (!rc) rc = upload(d, &s->instances, desc->instances, desc->instance_count * sizeof(trt::Instance), &total);
+    }
+    // The wide trees: the mesh trees built on the device (on a stream of the call's own, so renders queued by other
+    // threads go on meanwhile) unless the device is set to build on the host; collapsed and quantised on the host.
+    const auto t_build = std::chrono::steady_clock::now();
+    trt::WideScene w;
+    std::vector<trt::OwnBuilt> pre;
+    trt::gb::BuildTimes times;
+    const bool on_device = d->build == TRT_BUILD_DEVICE && !meshes.empty();
+    if (!rc && on_device) {
+        hipStream_t bs = nullptr;
+        hipError_t e = hipStreamCreateWithFlags(&bs, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            rc = fail(TRT_E_DEVICE, std::string("trt_scene_upload: ") + hipGetErrorString(e));
+        } else {
+            if (!trt::gb::own_trees_device(static_cast<const trt::Leaf *>(s->leaves), meshes, bs, pre, &times, &why))
+                rc = fail(TRT_E_DEVICE, "trt_scene_upload: " + why);
+            hipStreamSynchronize(bs);
+            hipStreamDestroy(bs);
+        }
+    }
+    if (!rc && !trt::build_wide(nodes, desc->node_count, leaves, inst, desc->instance_count, desc->leaf_count,
+                                desc->top_first, desc->unit_count, w, &why, &meshes, on_device ? &pre : nullptr))
+        rc = fail(TRT_E_INVALID, "trt_scene_upload: " + why);
+    if (!rc && w.nodes.size() >= (uint64_t{1} << 30)) rc = fail(TRT_E_INVALID, "trt_scene_upload: tree too large");
+    const double build_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_build).count();
+    if (rc) {
+        std::string keep = g_error;
+        trt_scene_release(s);
+        return fail(rc, keep);
+    }
+    std::lock_guard<std::mutex> lock(d->mu);
+    rc = upload(d, &s->wnodes, w.nodes.data(), w.nodes.size() * sizeof(trt::WideNode), &total);
     if (!rc) rc = upload(d, &s->wunits, w.units.data(), w.units.size() * sizeof(trt::WideUnit), &total);
     if (!rc) rc = upload(d, &s->wleaves, w.wleaves.data(), w.wleaves.size() * sizeof(trt::Leaf), &total);
     if (!rc && desc->energy) {
@@ -583,7 +620,35 @@ int trt_scene_upload(trt_device *d, const trt_scene_desc *desc, trt_scene **out)
     s->stats.max_stack = w.max_stack;
     s->stats.units = desc->unit_count;
     s->stats.device_bytes = total;
+    s->stats.build_ms = float(build_ms);
+    s->stats.device_build_ms = times.device_ms;
+    for (const trt::OwnBuilt &o : pre) s->stats.device_meshes += o.nodes.empty() ? 0 : 1;
+    s->stats.host_meshes = uint32_t(meshes.size()) - s->stats.device_meshes;
     *out = s;
+    return ok();
+}
+
+int trt_device_set_build(trt_device *d, uint32_t where) {
+    if (!d) return fail(TRT_E_INVALID, "trt_device_set_build: null device");
+    if (where != TRT_BUILD_DEVICE && where != TRT_BUILD_HOST)
+        return fail(TRT_E_INVALID, "trt_device_set_build: where must be TRT_BUILD_DEVICE or TRT_BUILD_HOST");
+    d->build = where;
+    return ok();
+}
+
+int trt_scene_wide_read(trt_scene *s, uint32_t which, void *dst, uint64_t bytes) {
+    if (!s || !dst) return fail(TRT_E_INVALID, "trt_scene_wide_read: null argument");
+    const void *src = which == 0 ? s->wnodes : which == 1 ? s->wleaves : which == 2 ? s->wunits : nullptr;
+    const uint64_t size = which == 0 ? s->wnode_count * sizeof(trt::WideNode)
+                          : which == 1 ? s->leaf_count * sizeof(trt::Leaf)
+                          : which == 2 ? s->wunit_count * sizeof(trt::WideUnit) : 0;
+    if (which > 2) return fail(TRT_E_INVALID, "trt_scene_wide_read: which must be 0, 1 or 2");
+    if (bytes != size) return fail(TRT_E_INVALID, "trt_scene_wide_read: bytes is not the buffer's size");
+    if (!bytes) return ok();
+    std::lock_guard<std::mutex> lock(s->dev->mu);
+    TRT_HIP(hipSetDevice(s->dev->ordinal));
+    TRT_HIP(hipStreamSynchronize(s->dev->stream));
+    TRT_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
     return ok();
 }
 

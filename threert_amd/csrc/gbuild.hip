@@ -1,0 +1,641 @@
+// The library's own mesh trees built on the device, one level of the tree at a time over every mesh of the scene
+// together. The trees are the host build's bytes (wide.cpp OwnTree): the same binned SAH over the same float
+// arithmetic (gbuild_core.h), with the split's stable partition made from prefix sums, and the host's node
+// numbering (children allocated in pairs as a right-first depth-first walk meets their parents) recovered at the
+// end from subtree sizes.
+//
+// Each level: the bounds of every open node (min / max over its triangles' boxes and centroids, as ordered integer
+// keys), its bins on the axes it splits on (count and box of 16 bins an axis), the host's SAH sweep and leaf rule
+// one node a thread, then the prefix sums that move each triangle to its child's range. A workgroup takes 1024
+// consecutive positions; the open nodes among them are consecutive, and the first kPriv of them gather in LDS
+// before one set of global atomics, so the top levels (a few nodes over millions of triangles) do not queue on
+// the same addresses.
+//
+// A mesh the device cannot finish as the host would (a split at the median of distinct centroids: past depth 64,
+// or when every split's cost overflows) is marked and built on the host instead; so is a mesh with no finite
+// triangle.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "gbuild_core.h"
+#include "gbuild.h"
+
+namespace trt {
+namespace gb {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kPerThread = 4;
+constexpr uint32_t kChunk = kThreads * kPerThread;  // positions a workgroup takes
+constexpr int kPriv = 8;                            // open nodes a workgroup gathers in LDS
+
+struct MeshDev {
+    uint32_t first;  // its first leaf in the scene's leaf buffer
+    uint32_t base;   // its first item (and position) in the build
+    uint32_t n;      // its triangles
+    uint32_t live;   // its finite triangles (after the count)
+};
+
+__device__ inline uint32_t mesh_of(const MeshDev *m, uint32_t count, uint32_t g) {
+    uint32_t lo = 0, hi = count;  // the last mesh whose base <= g
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (m[mid].base <= g) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// ---- prefix sums (exclusive, u32): tile sums, one scan of the tile sums, then each tile again ----------------------
+constexpr uint32_t kScanTile = kThreads * 8;
+
+__device__ inline uint32_t block_exclusive(uint32_t v, uint32_t *sh, uint32_t *total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) sh[wave] = x;
+    __syncthreads();
+    uint32_t off = 0, all = 0;
+    for (int w = 0; w < kThreads / 64; w++) {
+        if (w < wave) off += sh[w];
+        all += sh[w];
+    }
+    __syncthreads();
+    *total = all;
+    return off + x - v;
+}
+
+__global__ __launch_bounds__(kThreads) void k_scan_tiles(const uint32_t *in, uint64_t n, uint32_t *tile_sum) {
+    __shared__ uint32_t sh[kThreads / 64];
+    const uint64_t base = uint64_t(blockIdx.x) * kScanTile + uint64_t(threadIdx.x) * 8;
+    uint32_t s = 0;
+    for (int k = 0; k < 8; k++)
+        if (base + k < n) s += in[base + k];
+    uint32_t total;
+    block_exclusive(s, sh, &total);
+    if (threadIdx.x == 0) tile_sum[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kThreads) void k_scan_top(uint32_t *tile_sum, uint32_t tiles, uint32_t *grand) {
+    __shared__ uint32_t sh[kThreads / 64];
+    uint32_t carry = 0;
+    for (uint32_t b = 0; b < tiles; b += kThreads) {
+        const uint32_t i = b + threadIdx.x;
+        const uint32_t v = i < tiles ? tile_sum[i] : 0;
+        uint32_t total;
+        const uint32_t ex = block_exclusive(v, sh, &total);
+        if (i < tiles) tile_sum[i] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0) *grand = carry;
+}
+
+__global__ __launch_bounds__(kThreads) void k_scan_apply(const uint32_t *in, uint64_t n, const uint32_t *tile_sum,
+                                                         uint32_t *out) {
+    __shared__ uint32_t sh[kThreads / 64];
+    const uint64_t base = uint64_t(blockIdx.x) * kScanTile + uint64_t(threadIdx.x) * 8;
+    uint32_t v[8], s = 0;
+    for (int k = 0; k < 8; k++) {
+        v[k] = base + k < n ? in[base + k] : 0;
+        s += v[k];
+    }
+    uint32_t total;
+    uint32_t run = block_exclusive(s, sh, &total) + tile_sum[blockIdx.x];
+    for (int k = 0; k < 8; k++) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+}
+
+// ---- the build's kernels ---------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void k_boxes(const Leaf *leaves, const MeshDev *meshes, uint32_t mesh_count,
+                                                    uint32_t n, GBox *box, uint32_t *finite) {
+    const uint32_t g = blockIdx.x * kThreads + threadIdx.x;
+    if (g > n) return;
+    if (g == n) {
+        finite[g] = 0;  // the scan's last entry: the total
+        return;
+    }
+    const MeshDev m = meshes[mesh_of(meshes, mesh_count, g)];
+    GBox b;
+    finite[g] = tri_box(leaves[uint64_t(m.first) + (g - m.base)], b) ? 1u : 0u;
+    box[g] = b;
+}
+
+// Positions: each mesh's finite triangles first in index order, then the others (never in a node).
+__global__ __launch_bounds__(kThreads) void k_place(const MeshDev *meshes, const int32_t *mesh_slot,
+                                                    uint32_t mesh_count, uint32_t n, const uint32_t *finite,
+                                                    const uint32_t *live_before, uint32_t *order, int32_t *pos) {
+    const uint32_t g = blockIdx.x * kThreads + threadIdx.x;
+    if (g >= n) return;
+    const uint32_t mi = mesh_of(meshes, mesh_count, g);
+    const MeshDev m = meshes[mi];
+    const uint32_t rank = live_before[g] - live_before[m.base];
+    const bool live = finite[g] != 0;
+    const uint32_t p = live ? m.base + rank : m.base + m.live + (g - m.base - rank);
+    order[p] = g;
+    pos[p] = live ? mesh_slot[mi] : -1;
+}
+
+__global__ __launch_bounds__(kThreads) void k_clear(uint32_t *bounds, uint32_t *bins, uint32_t m) {
+    const uint64_t i = uint64_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (i < uint64_t(m) * kBoundWords) {
+        const uint32_t w = uint32_t(i % kBoundWords);
+        bounds[i] = (w < 3 || (w >= 6 && w < 9)) ? kKeyMin : kKeyMax;
+    }
+    if (i < uint64_t(m) * kSlotBinWords) {
+        const uint32_t w = uint32_t(i % kBinWords);
+        bins[i] = w == 0 ? 0u : (w < 4 ? kKeyMin : kKeyMax);
+    }
+}
+
+// The first open node among a workgroup's positions (slots rise with positions).
+__device__ inline int32_t first_slot(const int32_t *pos, uint32_t base, uint32_t n, int32_t *sh) {
+    if (threadIdx.x == 0) *sh = 0x7fffffff;
+    __syncthreads();
+    int32_t mine = 0x7fffffff;
+    for (int k = 0; k < kPerThread; k++) {
+        const uint32_t j = base + threadIdx.x + k * kThreads;
+        if (j < n && pos[j] >= 0) mine = min(mine, pos[j]);
+    }
+    if (mine != 0x7fffffff) atomicMin(sh, mine);
+    __syncthreads();
+    return *sh;
+}
+
+// Node bounds: min / max keys of the boxes and centroids of each open node's triangles.
+__global__ __launch_bounds__(kThreads) void k_bounds(const uint32_t *order, const int32_t *pos, const GBox *box,
+                                                     uint32_t n, uint32_t *bounds) {
+    __shared__ uint32_t sh[kPriv * kBoundWords];
+    __shared__ int32_t a0s;
+    const uint32_t base = blockIdx.x * kChunk;
+    for (int i = threadIdx.x; i < kPriv * kBoundWords; i += kThreads) {
+        const int w = i % kBoundWords;
+        sh[i] = (w < 3 || (w >= 6 && w < 9)) ? kKeyMin : kKeyMax;
+    }
+    const int32_t a0 = first_slot(pos, base, n, &a0s);  // (its barriers order the LDS fill above)
+    for (int k = 0; k < kPerThread; k++) {
+        const uint32_t j = base + threadIdx.x + k * kThreads;
+        if (j >= n) continue;
+        const int32_t a = pos[j];
+        if (a < 0) continue;
+        const GBox b = box[order[j]];
+        uint32_t key[kBoundWords];
+        for (int q = 0; q < 3; q++) {
+            key[q] = fkey(b.lo[q]);
+            key[3 + q] = fkey(b.hi[q]);
+            key[6 + q] = key[9 + q] = fkey(centroid(b, q));
+        }
+        if (a - a0 < kPriv) {
+            uint32_t *d = sh + (a - a0) * kBoundWords;
+            for (int q = 0; q < 3; q++) {
+                atomicMin(d + q, key[q]);
+                atomicMax(d + 3 + q, key[3 + q]);
+                atomicMin(d + 6 + q, key[6 + q]);
+                atomicMax(d + 9 + q, key[9 + q]);
+            }
+        } else {
+            uint32_t *d = bounds + uint64_t(a) * kBoundWords;
+            for (int q = 0; q < 3; q++) {
+                atomicMin(d + q, key[q]);
+                atomicMax(d + 3 + q, key[3 + q]);
+                atomicMin(d + 6 + q, key[6 + q]);
+                atomicMax(d + 9 + q, key[9 + q]);
+            }
+        }
+    }
+    __syncthreads();
+    if (a0 == 0x7fffffff) return;
+    for (int i = threadIdx.x; i < kPriv * kBoundWords; i += kThreads) {
+        const int w = i % kBoundWords;
+        const bool is_min = w < 3 || (w >= 6 && w < 9);
+        const uint32_t v = sh[i];
+        if (v == (is_min ? kKeyMin : kKeyMax)) continue;  // untouched (only open nodes of this chunk touch)
+        uint32_t *d = bounds + uint64_t(a0 + i / kBoundWords) * kBoundWords + w;
+        if (is_min) atomicMin(d, v);
+        else atomicMax(d, v);
+    }
+}
+
+// Each open node's box, centroid box and binned axes; a node the device cannot split as the host would marks its
+// mesh for the host.
+__global__ __launch_bounds__(kThreads) void k_slots(TNode *tn, uint32_t level_base, uint32_t m,
+                                                    const uint32_t *bounds, Slot *slots, uint32_t *host_mesh) {
+    const uint32_t a = blockIdx.x * kThreads + threadIdx.x;
+    if (a >= m) return;
+    TNode nd = tn[level_base + a];
+    Slot s;
+    if (!slot_setup(nd, bounds + uint64_t(a) * kBoundWords, s)) {
+        host_mesh[nd.mesh] = 1;
+        s.axes = 0;  // its subtree stops here: the mesh is built again on the host
+        s.mode = kModeLeaf;
+    }
+    tn[level_base + a] = nd;
+    slots[a] = s;
+}
+
+__global__ __launch_bounds__(kThreads) void k_bins(const uint32_t *order, const int32_t *pos, const GBox *box,
+                                                   uint32_t n, const Slot *slots, uint32_t *bins) {
+    __shared__ uint32_t sh[kPriv * kSlotBinWords];
+    __shared__ int32_t a0s;
+    const uint32_t base = blockIdx.x * kChunk;
+    for (int i = threadIdx.x; i < kPriv * kSlotBinWords; i += kThreads) {
+        const int w = i % kBinWords;
+        sh[i] = w == 0 ? 0u : (w < 4 ? kKeyMin : kKeyMax);
+    }
+    const int32_t a0 = first_slot(pos, base, n, &a0s);
+    for (int k = 0; k < kPerThread; k++) {
+        const uint32_t j = base + threadIdx.x + k * kThreads;
+        if (j >= n) continue;
+        const int32_t a = pos[j];
+        if (a < 0) continue;
+        const Slot s = slots[a];
+        if (!(s.axes & 7u)) continue;
+        const GBox b = box[order[j]];
+        uint32_t lo[3], hi[3];
+        for (int q = 0; q < 3; q++) lo[q] = fkey(b.lo[q]), hi[q] = fkey(b.hi[q]);
+        uint32_t *dst = a - a0 < kPriv ? sh + (a - a0) * kSlotBinWords : bins + uint64_t(a) * kSlotBinWords;
+        for (int ax = 0; ax < 3; ax++) {
+            if (!(s.axes >> ax & 1u)) continue;
+            const int bi = bin_of(centroid(b, ax), s.clo[ax], s.scale[ax]);
+            uint32_t *d = dst + (ax * kOwnBins + bi) * kBinWords;
+            atomicAdd(d, 1u);
+            for (int q = 0; q < 3; q++) {
+                atomicMin(d + 1 + q, lo[q]);
+                atomicMax(d + 4 + q, hi[q]);
+            }
+        }
+    }
+    __syncthreads();
+    if (a0 == 0x7fffffff) return;
+    for (int i = threadIdx.x; i < kPriv * kSlotBinWords; i += kThreads) {
+        const int w = i % kBinWords;
+        const uint32_t v = sh[i];
+        if (v == (w == 0 ? 0u : (w < 4 ? kKeyMin : kKeyMax))) continue;
+        uint32_t *d = bins + uint64_t(a0 + i / kSlotBinWords) * kSlotBinWords + (i % kSlotBinWords);
+        if (w == 0) atomicAdd(d, v);
+        else if (w < 4) atomicMin(d, v);
+        else atomicMax(d, v);
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_split(const TNode *tn, uint32_t level_base, uint32_t m,
+                                                    const uint32_t *bins, Slot *slots, uint32_t *splits,
+                                                    uint32_t *host_mesh) {
+    const uint32_t a = blockIdx.x * kThreads + threadIdx.x;
+    if (a > m) return;
+    if (a == m) {
+        splits[a] = 0;
+        return;
+    }
+    const TNode nd = tn[level_base + a];
+    Slot s = slots[a];
+    int r = 0;
+    if (s.axes != 0 || nd.e - nd.b <= kOwnLeaf) {
+        r = split_decide(nd, bins + uint64_t(a) * kSlotBinWords, s);
+    }
+    if (r < 0) {
+        host_mesh[nd.mesh] = 1;
+        r = 0;
+        s.mode = kModeLeaf;
+    }
+    slots[a] = s;
+    splits[a] = r > 0 ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(kThreads) void k_children(TNode *tn, uint32_t level_base, uint32_t next_base,
+                                                       uint32_t m, Slot *slots, const uint32_t *split_before) {
+    const uint32_t a = blockIdx.x * kThreads + threadIdx.x;
+    if (a >= m) return;
+    TNode &nd = tn[level_base + a];
+    Slot &s = slots[a];
+    if (s.mode == kModeLeaf) {
+        nd.left = kLeafMark;
+        return;
+    }
+    const uint32_t c = 2 * split_before[a];
+    s.child = c;
+    nd.left = next_base + c;
+    TNode l{}, r{};
+    l.b = nd.b, l.e = s.mid, r.b = s.mid, r.e = nd.e;
+    l.depth = r.depth = nd.depth + 1;
+    l.mesh = r.mesh = nd.mesh;
+    l.left = r.left = kLeafMark;
+    tn[next_base + c] = l;
+    tn[next_base + c + 1] = r;
+}
+
+__global__ __launch_bounds__(kThreads) void k_flags(const uint32_t *order, const int32_t *pos, const GBox *box,
+                                                    uint32_t n, const Slot *slots, uint32_t *flag) {
+    const uint32_t j = blockIdx.x * kThreads + threadIdx.x;
+    if (j > n) return;
+    uint32_t f = 0;
+    if (j < n && pos[j] >= 0) f = goes_left(slots[pos[j]], box[order[j]], j) ? 1u : 0u;
+    flag[j] = f;
+}
+
+__global__ __launch_bounds__(kThreads) void k_move(const uint32_t *order, const int32_t *pos, uint32_t n,
+                                                   const TNode *tn, uint32_t level_base, const Slot *slots,
+                                                   const uint32_t *left_before, uint32_t *order2, int32_t *pos2) {
+    const uint32_t j = blockIdx.x * kThreads + threadIdx.x;
+    if (j >= n) return;
+    const int32_t a = pos[j];
+    if (a < 0 || slots[a].mode == kModeLeaf) {
+        order2[j] = order[j];
+        pos2[j] = -1;
+        return;
+    }
+    const Slot s = slots[a];
+    const TNode nd = tn[level_base + a];
+    const uint32_t lr = left_before[j] - left_before[nd.b];
+    const uint32_t nl = left_before[nd.e] - left_before[nd.b];
+    const bool left = left_before[j + 1] != left_before[j];
+    const uint32_t p = left ? nd.b + lr : nd.b + nl + (j - nd.b - lr);
+    order2[p] = order[j];
+    pos2[p] = int32_t(s.child + (left ? 0u : 1u));
+}
+
+__global__ __launch_bounds__(kThreads) void k_inner(const TNode *tn, uint32_t t0, uint32_t cnt, uint32_t *inner) {
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= cnt) return;
+    const uint32_t t = t0 + i;
+    const uint32_t l = tn[t].left;
+    inner[t] = l == kLeafMark ? 0u : 1u + inner[l] + inner[l + 1];
+}
+
+// The host's numbering: the k-th interior node a right-first depth-first walk meets has its children at 2k + 1
+// and 2k + 2. A node of rank r: its right child has rank r + 1, its left child r + 1 + (interior nodes under the
+// right child).
+__global__ __launch_bounds__(kThreads) void k_rank(const TNode *tn, uint32_t t0, uint32_t cnt, const uint32_t *inner,
+                                                   uint32_t *rank, uint32_t *index) {
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= cnt) return;
+    const uint32_t t = t0 + i;
+    const uint32_t l = tn[t].left;
+    if (l == kLeafMark) return;
+    const uint32_t r = rank[t];
+    rank[l + 1] = r + 1;
+    rank[l] = r + 1 + inner[l + 1];
+    index[l] = 2 * r + 1;
+    index[l + 1] = 2 * r + 2;
+}
+
+__global__ __launch_bounds__(kThreads) void k_emit(const TNode *tn, uint32_t count, const uint32_t *rank,
+                                                   const uint32_t *index, const MeshDev *meshes,
+                                                   const uint64_t *node_base, const uint32_t *host_mesh, Node *out) {
+    const uint32_t t = blockIdx.x * kThreads + threadIdx.x;
+    if (t >= count) return;
+    const TNode nd = tn[t];
+    if (host_mesh[nd.mesh]) return;
+    Node o;
+    for (int k = 0; k < 3; k++) o.bmin[k] = nd.lo[k], o.bmax[k] = nd.hi[k];
+    if (nd.left == kLeafMark) {
+        o.left_or_first = nd.b - meshes[nd.mesh].base;
+        o.count = nd.e - nd.b;
+    } else {
+        o.left_or_first = 2 * rank[t] + 1;
+        o.count = 0;
+    }
+    out[node_base[nd.mesh] + index[t]] = o;
+}
+
+inline uint32_t blocks(uint64_t n) { return uint32_t((n + kThreads - 1) / kThreads); }
+
+struct Scratch {
+    std::vector<void *> ptrs;
+    ~Scratch() {
+        for (void *p : ptrs) (void)hipFree(p);
+    }
+    template <class T>
+    void drop(T *p) {  // (the stream is idle)
+        for (void *&q : ptrs)
+            if (q == p) {
+                (void)hipFree(q);
+                q = nullptr;
+            }
+    }
+    template <class T>
+    hipError_t get(T **p, uint64_t count) {
+        void *v = nullptr;
+        const hipError_t e = hipMalloc(&v, std::max<uint64_t>(count * sizeof(T), 64));
+        if (e != hipSuccess) return e;
+        ptrs.push_back(v);
+        *p = static_cast<T *>(v);
+        return hipSuccess;
+    }
+};
+
+#define GB_HIP(x)                                                                        \
+    do {                                                                                 \
+        const hipError_t e_ = (x);                                                       \
+        if (e_ != hipSuccess) {                                                          \
+            *why = std::string("device build: ") + #x + ": " + hipGetErrorString(e_);   \
+            return false;                                                                \
+        }                                                                                \
+    } while (0)
+
+// Exclusive prefix sums of in[0 .. n) into out[0 .. n); out[n - 1] + in[n - 1] is not kept (callers append a 0).
+bool scan(const uint32_t *in, uint64_t n, uint32_t *out, uint32_t *tile_sum, uint32_t *grand, hipStream_t st,
+          std::string *why) {
+    const uint32_t tiles = uint32_t((n + kScanTile - 1) / kScanTile);
+    k_scan_tiles<<<tiles, kThreads, 0, st>>>(in, n, tile_sum);
+    k_scan_top<<<1, kThreads, 0, st>>>(tile_sum, tiles, grand);
+    k_scan_apply<<<tiles, kThreads, 0, st>>>(in, n, tile_sum, out);
+    GB_HIP(hipGetLastError());
+    return true;
+}
+
+}  // namespace
+
+bool own_trees_device(const Leaf *d_leaves, const std::vector<MeshRef> &meshes, hipStream_t st,
+                      std::vector<OwnBuilt> &out, BuildTimes *times, std::string *why) {
+    out.assign(meshes.size(), OwnBuilt{});
+    uint64_t n64 = 0;
+    for (const MeshRef &m : meshes) n64 += m.n;
+    if (n64 == 0) return true;
+    if (n64 >= (uint64_t{1} << 31) - kScanTile) {
+        *why = "device build: more than 2^31 triangles";
+        return false;
+    }
+    const uint32_t n = uint32_t(n64), mc = uint32_t(meshes.size());
+    hipEvent_t e0, e1;
+    GB_HIP(hipEventCreate(&e0));
+    GB_HIP(hipEventCreate(&e1));
+    GB_HIP(hipEventRecord(e0, st));
+    Scratch sc;
+    std::vector<MeshDev> hm(mc);
+    for (uint32_t i = 0, b = 0; i < mc; i++) {
+        hm[i] = MeshDev{meshes[i].first, b, meshes[i].n, 0};
+        b += meshes[i].n;
+    }
+    MeshDev *dm;
+    GBox *box;
+    uint32_t *finite, *scan_out, *tile_sum, *grand, *order, *order2, *host_mesh;
+    int32_t *pos, *pos2, *mslot;
+    const uint32_t tiles = uint32_t((uint64_t(n) + 1 + kScanTile - 1) / kScanTile) + 1;
+    GB_HIP(sc.get(&dm, mc));
+    GB_HIP(sc.get(&box, n));
+    GB_HIP(sc.get(&finite, uint64_t(n) + 1));
+    GB_HIP(sc.get(&scan_out, uint64_t(n) + 1));
+    GB_HIP(sc.get(&tile_sum, tiles));
+    GB_HIP(sc.get(&grand, 1));
+    GB_HIP(sc.get(&order, n));
+    GB_HIP(sc.get(&order2, n));
+    GB_HIP(sc.get(&pos, n));
+    GB_HIP(sc.get(&pos2, n));
+    GB_HIP(sc.get(&mslot, mc));
+    GB_HIP(sc.get(&host_mesh, mc));
+    GB_HIP(hipMemcpyAsync(dm, hm.data(), mc * sizeof(MeshDev), hipMemcpyHostToDevice, st));
+    GB_HIP(hipMemsetAsync(host_mesh, 0, mc * sizeof(uint32_t), st));
+
+    // the triangles' boxes; the finite ones of each mesh
+    k_boxes<<<blocks(uint64_t(n) + 1), kThreads, 0, st>>>(d_leaves, dm, mc, n, box, finite);
+    if (!scan(finite, uint64_t(n) + 1, scan_out, tile_sum, grand, st, why)) return false;
+    std::vector<uint32_t> before(uint64_t(n) + 1);
+    GB_HIP(hipMemcpyAsync(before.data(), scan_out, (uint64_t(n) + 1) * 4, hipMemcpyDeviceToHost, st));
+    GB_HIP(hipStreamSynchronize(st));
+    // the count of the last entry: finite[n] is 0, so before[n] is the total
+    std::vector<int32_t> hslot(mc, -1);
+    std::vector<uint32_t> roots;  // level-0 nodes: one a mesh with a finite triangle
+    for (uint32_t i = 0; i < mc; i++) {
+        hm[i].live = before[hm[i].base + hm[i].n] - before[hm[i].base];
+        if (hm[i].live) {
+            hslot[i] = int32_t(roots.size());
+            roots.push_back(i);
+        }
+    }
+    GB_HIP(hipMemcpyAsync(dm, hm.data(), mc * sizeof(MeshDev), hipMemcpyHostToDevice, st));
+    GB_HIP(hipMemcpyAsync(mslot, hslot.data(), mc * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    k_place<<<blocks(n), kThreads, 0, st>>>(dm, mslot, mc, n, finite, scan_out, order, pos);
+
+    // nodes: at most 2 live - 1 a mesh
+    uint64_t live_total = 0;
+    for (auto &m : hm) live_total += m.live;
+    const uint64_t tcap = 2 * live_total + mc;
+    TNode *tn;
+    uint32_t *inner, *rank, *index;
+    GB_HIP(sc.get(&tn, tcap));
+    GB_HIP(sc.get(&inner, tcap));
+    GB_HIP(sc.get(&rank, tcap));
+    GB_HIP(sc.get(&index, tcap));
+    {
+        std::vector<TNode> r0(roots.size());
+        for (size_t k = 0; k < roots.size(); k++) {
+            const MeshDev &m = hm[roots[k]];
+            r0[k] = TNode{};
+            r0[k].b = m.base, r0[k].e = m.base + m.live, r0[k].depth = 0, r0[k].left = kLeafMark;
+            r0[k].mesh = roots[k];
+        }
+        GB_HIP(hipMemcpyAsync(tn, r0.data(), r0.size() * sizeof(TNode), hipMemcpyHostToDevice, st));
+        std::vector<uint32_t> zero(roots.size(), 0);
+        GB_HIP(hipMemcpyAsync(rank, zero.data(), zero.size() * 4, hipMemcpyHostToDevice, st));
+        GB_HIP(hipMemcpyAsync(index, zero.data(), zero.size() * 4, hipMemcpyHostToDevice, st));
+    }
+
+    // level by level
+    uint32_t *bounds = nullptr, *bins = nullptr, *splits = nullptr, *split_before = nullptr;
+    Slot *slots = nullptr;
+    uint64_t cap = 0;
+    std::vector<std::pair<uint32_t, uint32_t>> levels;  // (first node, count)
+    uint32_t level_base = 0, m = uint32_t(roots.size());
+    uint64_t total_nodes = m;
+    while (m) {
+        if (m > cap) {  // (the stream is idle here: the last level's count was read back)
+            cap = std::max<uint64_t>(m, 2 * cap);
+            sc.drop(bounds), sc.drop(bins), sc.drop(splits), sc.drop(split_before), sc.drop(slots);
+            GB_HIP(sc.get(&bounds, cap * kBoundWords));
+            GB_HIP(sc.get(&bins, cap * kSlotBinWords));
+            GB_HIP(sc.get(&splits, cap + 1));
+            GB_HIP(sc.get(&split_before, cap + 1));
+            GB_HIP(sc.get(&slots, cap));
+        }
+        levels.push_back({level_base, m});
+        k_clear<<<blocks(uint64_t(m) * kSlotBinWords), kThreads, 0, st>>>(bounds, bins, m);
+        const uint32_t chunks = uint32_t((uint64_t(n) + kChunk - 1) / kChunk);
+        k_bounds<<<chunks, kThreads, 0, st>>>(order, pos, box, n, bounds);
+        k_slots<<<blocks(m), kThreads, 0, st>>>(tn, level_base, m, bounds, slots, host_mesh);
+        k_bins<<<chunks, kThreads, 0, st>>>(order, pos, box, n, slots, bins);
+        k_split<<<blocks(uint64_t(m) + 1), kThreads, 0, st>>>(tn, level_base, m, bins, slots, splits, host_mesh);
+        if (!scan(splits, uint64_t(m) + 1, split_before, tile_sum, grand, st, why)) return false;
+        uint32_t nsplit = 0;
+        GB_HIP(hipMemcpyAsync(&nsplit, split_before + m, 4, hipMemcpyDeviceToHost, st));
+        GB_HIP(hipStreamSynchronize(st));
+        const uint32_t next_base = level_base + m;
+        if (uint64_t(next_base) + 2 * uint64_t(nsplit) > tcap) {
+            *why = "device build: more nodes than two a triangle";
+            return false;
+        }
+        k_children<<<blocks(m), kThreads, 0, st>>>(tn, level_base, next_base, m, slots, split_before);
+        if (nsplit) {
+            k_flags<<<blocks(uint64_t(n) + 1), kThreads, 0, st>>>(order, pos, box, n, slots, finite);
+            if (!scan(finite, uint64_t(n) + 1, scan_out, tile_sum, grand, st, why)) return false;
+            k_move<<<blocks(n), kThreads, 0, st>>>(order, pos, n, tn, level_base, slots, scan_out, order2, pos2);
+            std::swap(order, order2);
+            std::swap(pos, pos2);
+        }
+        level_base = next_base;
+        m = 2 * nsplit;
+        total_nodes += m;
+        GB_HIP(hipGetLastError());
+    }
+
+    // the host's numbering, each mesh's nodes in one buffer
+    for (size_t k = levels.size(); k-- > 0;)
+        k_inner<<<blocks(levels[k].second), kThreads, 0, st>>>(tn, levels[k].first, levels[k].second, inner);
+    for (const auto &lv : levels)
+        k_rank<<<blocks(lv.second), kThreads, 0, st>>>(tn, lv.first, lv.second, inner, rank, index);
+    std::vector<uint32_t> root_inner(roots.size()), failed(mc);
+    if (!roots.empty())
+        GB_HIP(hipMemcpyAsync(root_inner.data(), inner, roots.size() * 4, hipMemcpyDeviceToHost, st));
+    GB_HIP(hipMemcpyAsync(failed.data(), host_mesh, mc * 4, hipMemcpyDeviceToHost, st));
+    GB_HIP(hipStreamSynchronize(st));
+    std::vector<uint64_t> node_base(mc, 0);
+    uint64_t out_nodes = 0;
+    for (size_t k = 0; k < roots.size(); k++) {
+        node_base[roots[k]] = out_nodes;
+        out_nodes += 2 * uint64_t(root_inner[k]) + 1;
+    }
+    uint64_t *dbase;
+    Node *dout;
+    GB_HIP(sc.get(&dbase, mc));
+    GB_HIP(sc.get(&dout, out_nodes));
+    GB_HIP(hipMemcpyAsync(dbase, node_base.data(), mc * 8, hipMemcpyHostToDevice, st));
+    k_emit<<<blocks(total_nodes), kThreads, 0, st>>>(tn, uint32_t(total_nodes), rank, index, dm, dbase, host_mesh,
+                                                     dout);
+    GB_HIP(hipGetLastError());
+    GB_HIP(hipEventRecord(e1, st));
+    std::vector<Node> hn(out_nodes);
+    std::vector<uint32_t> ho(n);
+    GB_HIP(hipMemcpyAsync(hn.data(), dout, out_nodes * sizeof(Node), hipMemcpyDeviceToHost, st));
+    GB_HIP(hipMemcpyAsync(ho.data(), order, uint64_t(n) * 4, hipMemcpyDeviceToHost, st));
+    GB_HIP(hipStreamSynchronize(st));
+    if (times) {
+        float ms = 0;
+        GB_HIP(hipEventElapsedTime(&ms, e0, e1));
+        times->device_ms = ms;
+        times->levels = uint32_t(levels.size());
+        times->nodes = out_nodes;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    for (size_t k = 0; k < roots.size(); k++) {
+        const uint32_t i = roots[k];
+        if (failed[i]) continue;  // built on the host
+        OwnBuilt &o = out[i];
+        const uint64_t cnt = 2 * uint64_t(root_inner[k]) + 1;
+        o.nodes.assign(hn.begin() + node_base[i], hn.begin() + node_base[i] + cnt);
+        o.order.resize(hm[i].n);
+        for (uint32_t q = 0; q < hm[i].n; q++) o.order[q] = ho[hm[i].base + q] - hm[i].base;
+    }
+    return true;
+}
+
+}  // namespace gb
+}  // namespace trt

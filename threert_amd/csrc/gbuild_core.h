@@ -1,0 +1,231 @@
+// The per-item and per-node steps of the device build of the library's own mesh trees (gbuild.hip), written once
+// for the device. Each step does the host build's arithmetic (wide.cpp OwnTree) in the same order, so the trees
+// are the same bytes: float min / max, the bin index int((c - lo) * (16 / ext)), the surface area
+// 2 (xy + yz + zx), the SAH cost and the leaf rule.
+#pragma once
+
+#include <cstdint>
+
+#include "trt_internal.h"
+
+#ifndef TRT_HD
+#define TRT_HD __host__ __device__
+#endif
+
+namespace trt {
+namespace gb {
+
+constexpr uint32_t kLeafMark = 0xffffffffu;  // TNode::left of a leaf
+constexpr uint32_t kKeyMin = 0xffffffffu;    // the identity of a min over keys
+constexpr uint32_t kKeyMax = 0u;             // the identity of a max over keys
+constexpr int kBinWords = 7;                 // count, low keys x3, high keys x3
+constexpr int kSlotBinWords = 3 * kOwnBins * kBinWords;
+constexpr int kBoundWords = 12;              // node low / high keys, centroid low / high keys
+
+enum Mode : uint32_t { kModeLeaf = 0, kModeSah = 1, kModeMiddle = 2 };
+constexpr uint32_t kAllEqual = 1u << 3;  // Slot::axes: every centroid of the node is the same point
+
+// A node of the build, in level order (its final index in the mesh's tree comes at the end).
+struct TNode {
+    float lo[3], hi[3];
+    uint32_t b, e;        // positions (global) of its triangles
+    uint32_t depth, left; // left: the level-order index of its left child, kLeafMark for a leaf
+    uint32_t mesh, pad;
+};
+static_assert(sizeof(TNode) == 48, "TNode is 48 bytes");
+
+// A node of the level being split.
+struct Slot {
+    float clo[3], scale[3];  // centroid box low corner; bins per unit of length, by axis
+    uint32_t axes;           // bit k: axis k is binned; kAllEqual
+    uint32_t mode, axis, bin, mid, child;
+};
+
+struct GBox {
+    float lo[3], hi[3];
+};
+
+// Floats as unsigned keys in the same order (for min / max by integer atomics). -0 never occurs: boxes are
+// made +0 when zero.
+TRT_HD inline uint32_t fkey(float f) {
+    uint32_t u;
+    __builtin_memcpy(&u, &f, 4);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+TRT_HD inline float kfloat(uint32_t k) {
+    const uint32_t u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+    float f;
+    __builtin_memcpy(&f, &u, 4);
+    return f;
+}
+
+TRT_HD inline float next_down(float f) {
+    uint32_t u;
+    __builtin_memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) == 0) u = 0x80000001u;  // +-0 -> the negative subnormal next to it
+    else if (u & 0x80000000u) u += 1;
+    else u -= 1;
+    __builtin_memcpy(&f, &u, 4);
+    return f;
+}
+TRT_HD inline float next_up(float f) {
+    uint32_t u;
+    __builtin_memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) == 0) u = 0x00000001u;
+    else if (u & 0x80000000u) u -= 1;
+    else u += 1;
+    __builtin_memcpy(&f, &u, 4);
+    return f;
+}
+// The float just at or below / above a double (wide.cpp down / up), for finite values.
+TRT_HD inline float down_of(double v) {
+    float f = static_cast<float>(v);
+    if (double(f) > v) f = next_down(f);
+    return f;
+}
+TRT_HD inline float up_of(double v) {
+    float f = static_cast<float>(v);
+    if (double(f) < v) f = next_up(f);
+    return f;
+}
+
+TRT_HD inline bool finite_d(double v) { return v - v == 0.0; }
+
+// A triangle's box (its corners from the leaf in double, rounded outwards, zeros made +0); false if a corner is
+// not finite.
+TRT_HD inline bool tri_box(const Leaf &l, GBox &b) {
+    bool finite = true;
+    for (int k = 0; k < 3; k++) {
+        const double v0 = l.r[0][k], v1 = v0 + double(l.r[1][k]), v2 = v0 + double(l.r[2][k]);
+        if (!finite_d(v0) || !finite_d(v1) || !finite_d(v2)) finite = false;
+        const double mn = v0 < (v1 < v2 ? v1 : v2) ? v0 : (v1 < v2 ? v1 : v2);
+        const double mx = v0 > (v1 > v2 ? v1 : v2) ? v0 : (v1 > v2 ? v1 : v2);
+        b.lo[k] = down_of(mn) + 0.0f;
+        b.hi[k] = up_of(mx) + 0.0f;
+    }
+    return finite;
+}
+
+TRT_HD inline float centroid(const GBox &b, int k) { return 0.5f * b.lo[k] + 0.5f * b.hi[k]; }
+
+TRT_HD inline int bin_of(float c, float lo, float scale) {
+    const int bi = int((c - lo) * scale);
+    return bi < kOwnBins - 1 ? bi : kOwnBins - 1;
+}
+
+TRT_HD inline float area3(const float *lo, const float *hi) {
+    const float x = hi[0] - lo[0], y = hi[1] - lo[1], z = hi[2] - lo[2];
+    return 2.0f * (x * y + y * z + z * x);
+}
+
+// The node's box and centroid box from its keys; which axes it bins (wide.cpp: cnt > 1, above the depth limit,
+// a finite extent > 0 with a finite scale). Returns false if the node needs the host (a split past the depth limit,
+// or a median split whose order the device does not keep).
+TRT_HD inline bool slot_setup(TNode &n, const uint32_t *bk, Slot &s) {
+    float clo[3], chi[3];
+    for (int k = 0; k < 3; k++) {
+        n.lo[k] = kfloat(bk[k]);
+        n.hi[k] = kfloat(bk[3 + k]);
+        clo[k] = kfloat(bk[6 + k]);
+        chi[k] = kfloat(bk[9 + k]);
+    }
+    const uint32_t cnt = n.e - n.b;
+    s.axes = 0;
+    s.mode = kModeLeaf;
+    s.axis = s.bin = 0;
+    s.mid = n.e;
+    s.child = 0;
+    bool all_equal = true;
+    for (int k = 0; k < 3; k++) {
+        s.clo[k] = clo[k];
+        const float ext = chi[k] - clo[k];
+        const float scale = float(kOwnBins) / ext;
+        s.scale[k] = scale;
+        if (chi[k] != clo[k]) all_equal = false;
+        if (cnt > 1 && n.depth < kOwnDepth && ext > 0.0f && ext - ext == 0.0f && scale - scale == 0.0f)
+            s.axes |= 1u << k;
+    }
+    // a node the host would split at its median: the device keeps the order of such a split only when every
+    // centroid is the same point (its triangles are then sorted by index already, and stay so)
+    if (cnt > kOwnLeaf && s.axes == 0 && !(all_equal && n.depth < kOwnDepth)) return false;
+    if (all_equal) s.axes |= kAllEqual;
+    return true;
+}
+
+// The host's SAH sweep over the node's bins and its leaf rule; fills the slot's decision. Returns 1 if the node
+// splits, 0 for a leaf, -1 if the host must build the mesh (a median split of distinct centroids, as when every
+// split's cost overflows).
+TRT_HD inline int split_decide(const TNode &n, const uint32_t *bins, Slot &s) {
+    const uint32_t cnt = n.e - n.b;
+    const float leaf_cost = float(cnt);
+    int best_axis = -1, best_bin = -1;
+    float best_cost = __builtin_inff();
+    for (int k = 0; k < 3; k++) {
+        if (!(s.axes >> k & 1u)) continue;
+        const uint32_t *bk = bins + k * kOwnBins * kBinWords;
+        float right_area[kOwnBins];
+        uint32_t right_count[kOwnBins];
+        float alo[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
+        float ahi[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+        uint32_t ac = 0;
+        for (int bi = kOwnBins - 1; bi > 0; bi--) {
+            const uint32_t *w = bk + bi * kBinWords;
+            if (w[0]) {
+                for (int j = 0; j < 3; j++) {
+                    const float l = kfloat(w[1 + j]), h = kfloat(w[4 + j]);
+                    alo[j] = l < alo[j] ? l : alo[j];
+                    ahi[j] = ahi[j] < h ? h : ahi[j];
+                }
+            }
+            ac += w[0];
+            right_area[bi] = ac ? area3(alo, ahi) : 0.0f;
+            right_count[bi] = ac;
+        }
+        for (int j = 0; j < 3; j++) alo[j] = __builtin_inff(), ahi[j] = -__builtin_inff();
+        ac = 0;
+        for (int bi = 0; bi < kOwnBins - 1; bi++) {
+            const uint32_t *w = bk + bi * kBinWords;
+            if (w[0]) {
+                for (int j = 0; j < 3; j++) {
+                    const float l = kfloat(w[1 + j]), h = kfloat(w[4 + j]);
+                    alo[j] = l < alo[j] ? l : alo[j];
+                    ahi[j] = ahi[j] < h ? h : ahi[j];
+                }
+            }
+            ac += w[0];
+            if (ac == 0 || right_count[bi + 1] == 0) continue;
+            const float cost = area3(alo, ahi) * float(ac) + right_area[bi + 1] * float(right_count[bi + 1]);
+            if (cost < best_cost) best_cost = cost, best_axis = k, best_bin = bi;
+        }
+    }
+    const float na = area3(n.lo, n.hi);
+    const float split_cost = best_axis >= 0 ? 1.0f + (na > 0.0f ? best_cost / na : float(cnt)) : __builtin_inff();
+    if (cnt <= kOwnLeaf && (cnt == 1 || !(split_cost < leaf_cost))) {
+        s.mode = kModeLeaf;
+        return 0;
+    }
+    if (best_axis >= 0) {
+        uint32_t nl = 0;
+        const uint32_t *bk = bins + best_axis * kOwnBins * kBinWords;
+        for (int bi = 0; bi <= best_bin; bi++) nl += bk[bi * kBinWords];
+        s.mode = kModeSah;
+        s.axis = uint32_t(best_axis);
+        s.bin = uint32_t(best_bin);
+        s.mid = n.b + nl;
+    } else {
+        if (!(s.axes & kAllEqual) || n.depth >= kOwnDepth) return -1;
+        s.mode = kModeMiddle;
+        s.mid = n.b + cnt / 2;
+    }
+    return 1;
+}
+
+// Whether the triangle at position j goes to the left child.
+TRT_HD inline bool goes_left(const Slot &s, const GBox &b, uint32_t j) {
+    if (s.mode == kModeSah) return uint32_t(bin_of(centroid(b, s.axis), s.clo[s.axis], s.scale[s.axis])) <= s.bin;
+    if (s.mode == kModeMiddle) return j < s.mid;
+    return false;
+}
+
+}  // namespace gb
+}  // namespace trt

@@ -126,8 +126,29 @@ void unit_box(const float *world, const float *lo, const float *hi, float *out);
 bool top_tree(const float *boxes, const uint32_t *records, uint64_t n, Tree &out, std::string *why);
 void unit_record(Instance &r, uint32_t root, bool cast_shadow);
 
-// The wide tree of an uploaded scene (wide.cpp).
+// The library's own mesh trees (wide.cpp OwnTree on the host, gbuild.hip on the device: the same bytes). Binned SAH
+// over the triangles' boxes, 16 bins an axis, leaves of at most kOwnLeaf triangles, split at the median past
+// kOwnDepth.
+constexpr uint32_t kOwnLeaf = 4;
+constexpr int kOwnBins = 16;
+constexpr uint32_t kOwnDepth = 64;
+struct OwnBuilt {
+    std::vector<Node> nodes;      // nodes[0] is the root; children at left_or_first and left_or_first + 1 (absolute)
+    std::vector<uint32_t> order;  // tree position -> the triangle's index in its mesh; leaves name positions
+};
+struct MeshRef {
+    uint32_t root, first, n;  // the reference mesh tree's root, its first leaf and its triangle count
+};
+void own_tree_host(const Leaf *leaves, uint32_t first, uint32_t n, OwnBuilt &out);
+
+// The wide tree of an uploaded scene (wide.cpp). `meshes` lists the distinct mesh trees the top tree's records
+// name, in record order, after checking the trees; `pre` (optional) holds own trees already built for them (an
+// entry with no nodes is built on the host).
+bool wide_meshes(const Node *nodes, uint64_t node_count, const Instance *inst, uint64_t inst_count,
+                 uint64_t leaf_count, uint32_t top_first, uint32_t unit_count, std::vector<MeshRef> &meshes,
+                 std::string *why);
 bool build_wide(const Node *nodes, uint64_t node_count, const Leaf *leaves, const Instance *inst, uint64_t inst_count,
-                uint64_t leaf_count, uint32_t top_first, uint32_t unit_count, WideScene &out, std::string *why);
+                uint64_t leaf_count, uint32_t top_first, uint32_t unit_count, WideScene &out, std::string *why,
+                const std::vector<MeshRef> *meshes = nullptr, const std::vector<OwnBuilt> *pre = nullptr);
 
 }  // namespace trt

@@ -182,10 +182,6 @@ bool check_walkable(const Node *nodes, uint64_t count, uint32_t root, uint32_t o
     return true;
 }
 
-constexpr uint32_t kOwnLeaf = 4;   // triangles a leaf of the library's mesh trees holds at most
-constexpr int kOwnBins = 16;
-constexpr uint32_t kOwnDepth = 64;  // past this depth a node is split at its median
-
 // The number of triangles a reference mesh tree covers (its leaves name positions 0 .. n-1 past the mesh's first).
 uint32_t mesh_size(const Node *nodes, uint32_t root) {
     uint32_t n = 0;
@@ -245,8 +241,9 @@ struct OwnTree {
             for (int k = 0; k < 3; k++) {
                 const double v0 = l.r[0][k], v1 = v0 + double(l.r[1][k]), v2 = v0 + double(l.r[2][k]);
                 if (!std::isfinite(v0) || !std::isfinite(v1) || !std::isfinite(v2)) finite = false;
-                box[i].lo[k] = down(std::min(v0, std::min(v1, v2)));
-                box[i].hi[k] = up(std::max(v0, std::max(v1, v2)));
+                // + 0.0f makes a zero bound +0: min and max then do not depend on the order they see items in
+                box[i].lo[k] = down(std::min(v0, std::min(v1, v2))) + 0.0f;
+                box[i].hi[k] = up(std::max(v0, std::max(v1, v2))) + 0.0f;
                 cen[3 * i + k] = 0.5f * box[i].lo[k] + 0.5f * box[i].hi[k];
             }
             (finite ? live : dead).push_back(i);
@@ -287,11 +284,13 @@ struct OwnTree {
             if (cnt > 1 && it.depth < kOwnDepth) {
                 for (int k = 0; k < 3; k++) {
                     const float ext = cb.hi[k] - cb.lo[k];
-                    if (!(ext > 0.0f)) continue;
+                    const float scale = float(kOwnBins) / ext;
+                    // no bins over an extent past the float range or a subnormal one: (c - lo) * scale would not
+                    // be a number, or the scale not finite
+                    if (!(ext > 0.0f) || !std::isfinite(ext) || !std::isfinite(scale)) continue;
                     Box bins[kOwnBins];
                     uint32_t counts[kOwnBins] = {};
                     for (auto &x : bins) x = empty();
-                    const float scale = float(kOwnBins) / ext;
                     for (uint32_t j = it.b; j < it.e; j++) {
                         const uint32_t p = order[j];
                         const int bi = std::min(kOwnBins - 1, int((cen[3 * p + k] - cb.lo[k]) * scale));
@@ -331,7 +330,9 @@ struct OwnTree {
                 const int k = best_axis;
                 const float scale = float(kOwnBins) / (cb.hi[k] - cb.lo[k]);
                 const float lo = cb.lo[k];
-                auto pm = std::partition(order.begin() + it.b, order.begin() + it.e, [&](uint32_t p) {
+                // stable: each side keeps the order it had, so a node's triangles stay in ascending index order
+                // above the depth limit (the device build's partition by prefix sums gives the same)
+                auto pm = std::stable_partition(order.begin() + it.b, order.begin() + it.e, [&](uint32_t p) {
                     return std::min(kOwnBins - 1, int((cen[3 * p + k] - lo) * scale)) <= best_bin;
                 });
                 mid = uint32_t(pm - order.begin());
@@ -341,10 +342,9 @@ struct OwnTree {
                 for (int a = 1; a < 3; a++)
                     if (cb.hi[a] - cb.lo[a] > cb.hi[k] - cb.lo[k]) k = a;
                 mid = it.b + cnt / 2;
-                std::nth_element(order.begin() + it.b, order.begin() + mid, order.begin() + it.e,
-                                 [&](uint32_t x, uint32_t y) {
-                                     return cen[3 * x + k] < cen[3 * y + k] || (cen[3 * x + k] == cen[3 * y + k] && x < y);
-                                 });
+                std::sort(order.begin() + it.b, order.begin() + it.e, [&](uint32_t x, uint32_t y) {
+                    return cen[3 * x + k] < cen[3 * y + k] || (cen[3 * x + k] == cen[3 * y + k] && x < y);
+                });
             }
             if (mid == it.b || mid == it.e) mid = it.b + cnt / 2;
             const uint32_t l = static_cast<uint32_t>(nodes.size());
@@ -361,73 +361,139 @@ struct OwnTree {
 
 }  // namespace
 
+void own_tree_host(const Leaf *leaves, uint32_t first, uint32_t n, OwnBuilt &out) {
+    OwnTree own;
+    own.build(leaves, first, n);
+    out.nodes = std::move(own.nodes);
+    out.order = std::move(own.order);
+}
+
+namespace {
+
+struct Plan {
+    std::vector<uint32_t> records;  // the top tree's records, in walk order
+    std::vector<MeshRef> meshes;    // distinct (root, first) pairs in record order
+    std::vector<uint32_t> mesh_of;  // by record: its index in `meshes`
+};
+
+bool plan_of(const Node *nodes, uint64_t node_count, const Instance *inst, uint64_t inst_count, uint64_t leaf_count,
+             uint32_t top_first, Plan &p, std::string *why) {
+    if (top_first >= node_count) {
+        *why = "the top tree's root is outside the node buffer";
+        return false;
+    }
+    if (!check_walkable(nodes, node_count, top_first, top_first, kTopDepth, true, inst_count, 0, &p.records, why))
+        return false;
+    std::vector<uint8_t> seen(inst_count, 0);
+    for (uint32_t r : p.records) {
+        if (seen[r]) {
+            *why = "a record is in the top tree twice";
+            return false;
+        }
+        seen[r] = 1;
+    }
+    p.mesh_of.assign(inst_count, kNone);
+    std::unordered_map<uint64_t, uint32_t> index;  // (root, first) -> mesh
+    for (uint32_t r : p.records) {
+        const Instance &u = inst[r];
+        if (to_bits(u.n[1][3]) & (kInstCutout | kInstGlass)) {
+            *why = "records with cutout or glass sections are not supported";
+            return false;
+        }
+        const uint32_t root = to_bits(u.n[0][3]);
+        if (!check_walkable(nodes, node_count, root, root, kMeshDepth, false, leaf_count, u.info[0], nullptr, why))
+            return false;
+        const uint64_t key = (uint64_t(root) << 32) | u.info[0];
+        auto it = index.find(key);
+        if (it == index.end()) {
+            it = index.emplace(key, uint32_t(p.meshes.size())).first;
+            p.meshes.push_back(MeshRef{root, u.info[0], mesh_size(nodes, root)});
+        }
+        p.mesh_of[r] = it->second;
+    }
+    // each mesh's leaves are laid out again in its own tree's order: two meshes may not share leaves
+    std::vector<std::pair<uint64_t, uint64_t>> span;
+    for (const MeshRef &m : p.meshes)
+        if (m.n) span.push_back({m.first, uint64_t(m.first) + m.n});
+    std::sort(span.begin(), span.end());
+    for (size_t k = 1; k < span.size(); k++)
+        if (span[k].first < span[k - 1].second) {
+            *why = "two mesh trees name the same leaves";
+            return false;
+        }
+    return true;
+}
+
+}  // namespace
+
+bool wide_meshes(const Node *nodes, uint64_t node_count, const Instance *inst, uint64_t inst_count,
+                 uint64_t leaf_count, uint32_t top_first, uint32_t unit_count, std::vector<MeshRef> &meshes,
+                 std::string *why) {
+    meshes.clear();
+    if (unit_count == 0) return true;
+    Plan p;
+    if (!plan_of(nodes, node_count, inst, inst_count, leaf_count, top_first, p, why)) return false;
+    meshes = std::move(p.meshes);
+    return true;
+}
+
 bool build_wide(const Node *nodes, uint64_t node_count, const Leaf *leaves, const Instance *inst, uint64_t inst_count,
-                uint64_t leaf_count, uint32_t top_first, uint32_t unit_count, WideScene &out, std::string *why) {
+                uint64_t leaf_count, uint32_t top_first, uint32_t unit_count, WideScene &out, std::string *why,
+                const std::vector<MeshRef> *meshes, const std::vector<OwnBuilt> *pre) {
     out = WideScene();
     out.units.assign(inst_count, WideUnit{});
     out.wleaves.assign(leaves, leaves + leaf_count);
     for (uint64_t i = 0; i < leaf_count; i++) out.wleaves[i].r[2][3] = from_bits(static_cast<uint32_t>(i));
     if (unit_count == 0) return true;
-    if (top_first >= node_count) {
-        *why = "the top tree's root is outside the node buffer";
+    Plan p;
+    if (!plan_of(nodes, node_count, inst, inst_count, leaf_count, top_first, p, why)) return false;
+    if (pre && (!meshes || meshes->size() != p.meshes.size() || pre->size() != p.meshes.size())) {
+        *why = "prebuilt mesh trees do not match the scene's meshes";
         return false;
-    }
-    std::vector<uint32_t> records;
-    if (!check_walkable(nodes, node_count, top_first, top_first, kTopDepth, true, inst_count, 0, &records, why))
-        return false;
-    {
-        std::vector<uint8_t> seen(inst_count, 0);
-        for (uint32_t r : records) {
-            if (seen[r]) {
-                *why = "a record is in the top tree twice";
-                return false;
-            }
-            seen[r] = 1;
-        }
     }
     Collapser col{nodes, node_count, out};
-    std::unordered_map<uint32_t, std::pair<uint32_t, uint32_t>> mesh;  // binary root -> (wide ref, bound)
+    std::vector<std::pair<uint32_t, uint32_t>> mesh(p.meshes.size());  // wide ref, stack bound
+    for (size_t i = 0; i < p.meshes.size(); i++) {
+        const MeshRef m = p.meshes[i];
+        if (pre && ((*meshes)[i].root != m.root || (*meshes)[i].first != m.first || (*meshes)[i].n != m.n)) {
+            *why = "prebuilt mesh trees do not match the scene's meshes";
+            return false;
+        }
+        OwnBuilt host;
+        const OwnBuilt *own = pre && !(*pre)[i].nodes.empty() ? &(*pre)[i] : nullptr;
+        if (!own) {
+            own_tree_host(leaves, m.first, m.n, host);
+            own = &host;
+        }
+        if (own->order.size() != m.n) {
+            *why = "a prebuilt mesh tree's order has the wrong size";
+            return false;
+        }
+        for (uint32_t j = 0; j < m.n; j++) {
+            Leaf l = leaves[uint64_t(m.first) + own->order[j]];
+            l.r[2][3] = from_bits(m.first + own->order[j]);
+            out.wleaves[uint64_t(m.first) + j] = l;
+        }
+        Collapser mc{own->nodes.data(), own->nodes.size(), out};
+        uint32_t b = 0;
+        const uint32_t ref = mc.convert(0, 0, false, b);
+        if (!mc.err.empty()) {
+            *why = mc.err;
+            return false;
+        }
+        mesh[i] = {ref, b};
+    }
     std::vector<uint32_t> bound(inst_count, 0);
-    for (uint32_t r : records) {
+    for (uint32_t r : p.records) {
         const Instance &u = inst[r];
-        const uint32_t bits = to_bits(u.n[1][3]);
-        if (bits & (kInstCutout | kInstGlass)) {
-            *why = "records with cutout or glass sections are not supported";
-            return false;
-        }
-        const uint32_t root = to_bits(u.n[0][3]);
-        auto it = mesh.find(root);
-        if (it == mesh.end()) {
-            if (!check_walkable(nodes, node_count, root, root, kMeshDepth, false, leaf_count, u.info[0], nullptr, why))
-                return false;
-            const uint32_t first = u.info[0], n = mesh_size(nodes, root);
-            OwnTree own;
-            own.build(leaves, first, n);
-            for (uint32_t j = 0; j < n; j++) {
-                Leaf l = leaves[uint64_t(first) + own.order[j]];
-                l.r[2][3] = from_bits(first + own.order[j]);
-                out.wleaves[uint64_t(first) + j] = l;
-            }
-            Collapser mc{own.nodes.data(), own.nodes.size(), out};
-            uint32_t b = 0;
-            const uint32_t ref = mc.convert(0, 0, false, b);
-            if (!mc.err.empty()) {
-                *why = mc.err;
-                return false;
-            }
-            it = mesh.emplace(root, std::make_pair(ref, b)).first;
-        } else if (!check_walkable(nodes, node_count, root, root, kMeshDepth, false, leaf_count, u.info[0], nullptr,
-                                   why)) {
-            return false;
-        }
         WideUnit w{};
         w.t[0] = u.m[0][3], w.t[1] = u.m[1][3], w.t[2] = u.m[2][3];
-        w.flags = bits;
+        w.flags = to_bits(u.n[1][3]);
         for (int k = 0; k < 3; k++) w.n0[k] = u.n[0][k], w.n1[k] = u.n[1][k], w.n2[k] = u.n[2][k];
-        w.root = it->second.first;
+        w.root = mesh[p.mesh_of[r]].first;
         w.first = u.info[0];
         out.units[r] = w;
-        bound[r] = it->second.second;
+        bound[r] = mesh[p.mesh_of[r]].second;
     }
     col.unit_bound = &bound;
     uint32_t b = 0;
