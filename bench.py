@@ -214,6 +214,26 @@ def bench_precision_render(dev) -> dict:
                         "1920x1080, 64 spp (8 launches of 8), 4 bounces, RR from depth 3"}
 
 
+def tree_build(dev, prep, sc) -> dict:
+    """The upload's wide-tree build of the 3M-triangle scene: the library's own mesh tree on the device (the
+    default) against one host thread, the same bytes; then collapse and quantisation on the host in both."""
+    first = sc.stats()
+    dev.set_build(T.BUILD_HOST)
+    try:
+        host = T.Scene(dev, *prep.desc())
+    finally:
+        dev.set_build(T.BUILD_AUTO)
+    same = all(bool((host.wide_read(k) == sc.wide_read(k)).all()) for k in (0, 1, 2))
+    hs = host.stats()
+    host.close()
+    again = T.Scene(dev, *prep.desc())
+    ds = again.stats()
+    again.close()
+    return {"host_build_ms": round(hs["build_ms"], 1), "device_build_ms": round(ds["build_ms"], 1),
+            "device_trees_ms": round(ds["device_build_ms"], 1), "first_device_build_ms": round(first["build_ms"], 1),
+            "same_bytes": same, "speedup": round(hs["build_ms"] / ds["build_ms"], 2)}
+
+
 def bench_copy(dev) -> dict:
     """The library's plain copy kernel over 1 GiB: the HBM bandwidth a streaming kernel reaches, and the
     yardstick for the HBM byte counters."""
@@ -260,6 +280,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-render", action="store_true")
     ap.add_argument("--render-frames", type=int, default=10)
+    ap.add_argument("--build-only", action="store_true",
+                    help="measure only the upload's wide-tree build of the 3M-triangle scene (device and host)")
     ap.add_argument("--serve", action="store_true", help="measure the serving workload instead (p50/p99 latency)")
     ap.add_argument("--serve-jobs", type=int, default=512)
     ap.add_argument("--serve-rate", type=float, default=40.0)
@@ -309,6 +331,14 @@ def main():
     info = dev.info()
     sc = T.Scene(dev, *prep.desc())
     st = sc.stats()
+    build = tree_build(dev, prep, sc)
+    log(f"bench: tree build {build}")
+    if args.build_only:
+        print(json.dumps({"metric": "wide-tree build of a 3M-triangle mesh", "value": build["device_build_ms"],
+                          "unit": "ms", "n_gpus": 1, "higher_is_better": False, "dtype": "f32", "data": "synthetic",
+                          "config": {"workload": "trt_scene_upload's wide-tree build: 3M-triangle clustered mesh",
+                                     "tree_build": build}}), flush=True)
+        return
     d_rays = dev.alloc(rays.nbytes)
     d_rays.upload(rays)
     d_hits = dev.alloc(N_RAYS * T.HIT.itemsize)
@@ -364,6 +394,7 @@ def main():
             "wall_s_timed": round(wall, 3),
             "fp64_check": fp64,
             "binary_walk": binary,
+            "tree_build": build,
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic()},

@@ -153,14 +153,17 @@ int trt_scene_upload(trt_device *dev, const trt_scene_desc *desc, trt_scene **ou
 int trt_scene_release(trt_scene *scene);
 int trt_scene_stats_get(trt_scene *scene, trt_scene_stats *out);
 
-/* Where later uploads on this device build the library's own mesh trees: TRT_BUILD_DEVICE (the default; binned
- * SAH one tree level at a time over every mesh together, on a stream of the upload's own) or TRT_BUILD_HOST (one
- * host thread a mesh). Both give the same bytes; a mesh the device cannot build as the host would (a median split
- * of distinct centroids past depth 64) is built on the host. No reference counterpart: the reference builds its
- * trees on the host in `prepare` (lib.rs:836-913, bvh.rs:130-239); the library's own trees replace only the walk's
- * copy of them. */
+/* Where later uploads on this device build the library's own mesh trees and their 4-wide nodes:
+ * TRT_BUILD_DEVICE (binned SAH one tree level at a time over every mesh together, then the collapse, the
+ * quantisation and the leaves' layout, on a stream of the upload's own), TRT_BUILD_HOST (one host thread), or
+ * TRT_BUILD_AUTO (the default: the device for scenes of 65,536 mesh triangles or more, where it is faster). All give
+ * the same bytes; a mesh the device cannot build as the host would (a median split of distinct centroids, past
+ * depth 64 or when every split's cost overflows) is built on the host. No reference counterpart: the reference
+ * builds its trees on the host in `prepare` (lib.rs:836-913, bvh.rs:130-239); the library's own trees replace only
+ * the walk's copy of them. */
 #define TRT_BUILD_DEVICE 0u
 #define TRT_BUILD_HOST 1u
+#define TRT_BUILD_AUTO 2u
 int trt_device_set_build(trt_device *dev, uint32_t where);
 
 /* Copies one of the scene's wide-tree buffers to host memory: which = 0 the 64-byte wide nodes

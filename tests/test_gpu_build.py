@@ -1,6 +1,6 @@
-"""The library's own mesh trees built on the device (the default, trt_device_set_build(TRT_BUILD_DEVICE)) against
-the same trees built on the host (TRT_BUILD_HOST): the wide nodes, the leaves in the trees' order and the units
-must be the same bytes. The host build is the reference for this: it is the tree every other test walks against
+"""The library's own mesh trees and their 4-wide nodes built on the device (trt_device_set_build(TRT_BUILD_DEVICE),
+and by default for large scenes) against the same built on the host (TRT_BUILD_HOST): the wide nodes, the leaves
+in the trees' order and the units must be the same bytes. The host build is the reference for this: it is the tree every other test walks against
 the oracle's hits. Cases: instanced scenes, the reference's garbage-in scenarios (bvh.rs:409-441: non-finite
 corners, coordinates near the float limit, one triangle a thousand times, a needle), a 200k-triangle clustered
 scene, and the Cornell box and a sunlit city as the renderer uploads them."""
@@ -24,12 +24,13 @@ def say(*a):
 
 def both(dev, upload):
     """(host-built scene, device-built scene) from `upload()`."""
-    dev.set_build(T.BUILD_HOST)
     try:
+        dev.set_build(T.BUILD_HOST)
         h = upload()
-    finally:
         dev.set_build(T.BUILD_DEVICE)
-    d = upload()
+        d = upload()
+    finally:
+        dev.set_build(T.BUILD_AUTO)
     return h, d
 
 

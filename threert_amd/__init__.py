@@ -31,7 +31,7 @@ assert NODE.itemsize == 32 and INSTANCE.itemsize == 160 and RAY.itemsize == 32 a
 TRT_OK, TRT_E_INVALID, TRT_E_OOM, TRT_E_DEVICE, TRT_E_GUARD = 0, 1, 2, 3, 4
 TRACE_CLOSEST, TRACE_SHADOW, TRACE_ANY = 0, 1, 2
 WALK_WIDE, WALK_BINARY = 0, 1
-BUILD_DEVICE, BUILD_HOST = 0, 1  # trt_device_set_build
+BUILD_DEVICE, BUILD_HOST, BUILD_AUTO = 0, 1, 2  # trt_device_set_build
 NO_HIT = 0xFFFFFFFF
 INF = np.float32(3.0e38)
 
@@ -232,7 +232,8 @@ class Device:
         return DeviceBuffer(self, nbytes)
 
     def set_build(self, where: int):
-        """Where later uploads build the library's own mesh trees: BUILD_DEVICE (default) or BUILD_HOST."""
+        """Where later uploads build the library's own mesh trees: BUILD_AUTO (default: the device from 65,536
+        mesh triangles), BUILD_DEVICE or BUILD_HOST."""
         _check(lib().trt_device_set_build(self.h, where))
 
     def timer_start(self):

@@ -68,7 +68,7 @@ struct trt_device {
     uint64_t staging_bytes = 0;
     int cus = 0;
     int blocks_wide = 1, blocks_binary = 1, blocks_render = 1;
-    uint32_t build = TRT_BUILD_DEVICE;  // where uploads build the library's own mesh trees
+    uint32_t build = TRT_BUILD_AUTO;  // where uploads build the library's own mesh trees
     hipDeviceProp_t props{};
     std::mutex mu;
     std::atomic<int> refs{1};
@@ -499,6 +499,10 @@ static bool check_shading(const trt_scene_desc *d, std::string *why) {
     return true;
 }
 
+// TRT_BUILD_AUTO builds on the device from this many mesh triangles: below it the device's per-level launches and
+// waits (a few milliseconds) cost more than one host thread's build.
+constexpr uint64_t kDeviceBuildMin = 65536;
+
 static int upload(trt_device *d, void **dst, const void *src, uint64_t bytes, uint64_t *total) {
     TRT_HIP(hipMalloc(dst, bytes ? bytes : 64));
     if (bytes) TRT_HIP(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
@@ -549,36 +553,61 @@ int trt_scene_upload(trt_device *d, const trt_scene_desc *desc, trt_scene **out)
     // threads go on meanwhile) unless the device is set to build on the host; collapsed and quantised on the host.
     const auto t_build = std::chrono::steady_clock::now();
     trt::WideScene w;
-    std::vector<trt::OwnBuilt> pre;
-    trt::gb::BuildTimes times;
-    const bool on_device = d->build == TRT_BUILD_DEVICE && !meshes.empty();
+    trt::gb::DeviceWide dw;
+    trt::MeshWide mw;
+    uint64_t mesh_tris = 0;
+    for (const trt::MeshRef &m : meshes) mesh_tris += m.n;
+    const bool on_device = !meshes.empty() && (d->build == TRT_BUILD_DEVICE ||
+                                               (d->build == TRT_BUILD_AUTO && mesh_tris >= kDeviceBuildMin));
     if (!rc && on_device) {
         hipStream_t bs = nullptr;
         hipError_t e = hipStreamCreateWithFlags(&bs, hipStreamNonBlocking);
         if (e != hipSuccess) {
             rc = fail(TRT_E_DEVICE, std::string("trt_scene_upload: ") + hipGetErrorString(e));
         } else {
-            if (!trt::gb::own_trees_device(static_cast<const trt::Leaf *>(s->leaves), meshes, bs, pre, &times, &why))
+            if (!trt::gb::device_wide(static_cast<const trt::Leaf *>(s->leaves), desc->leaf_count, meshes, bs, dw,
+                                      &why))
                 rc = fail(TRT_E_DEVICE, "trt_scene_upload: " + why);
             hipStreamSynchronize(bs);
             hipStreamDestroy(bs);
         }
+        if (dw.complete) mw = trt::MeshWide{dw.wnode_count, dw.mesh_ref, dw.mesh_bound};
     }
     if (!rc && !trt::build_wide(nodes, desc->node_count, leaves, inst, desc->instance_count, desc->leaf_count,
-                                desc->top_first, desc->unit_count, w, &why, &meshes, on_device ? &pre : nullptr))
+                                desc->top_first, desc->unit_count, w, &why, &meshes,
+                                on_device && !dw.complete ? &dw.trees : nullptr, dw.complete ? &mw : nullptr))
         rc = fail(TRT_E_INVALID, "trt_scene_upload: " + why);
     if (!rc && w.nodes.size() >= (uint64_t{1} << 30)) rc = fail(TRT_E_INVALID, "trt_scene_upload: tree too large");
     const double build_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_build).count();
     if (rc) {
         std::string keep = g_error;
+        hipFree(dw.d_wnodes);
+        hipFree(dw.d_wleaves);
         trt_scene_release(s);
         return fail(rc, keep);
     }
     std::lock_guard<std::mutex> lock(d->mu);
-    rc = upload(d, &s->wnodes, w.nodes.data(), w.nodes.size() * sizeof(trt::WideNode), &total);
-    if (!rc) rc = upload(d, &s->wunits, w.units.data(), w.units.size() * sizeof(trt::WideUnit), &total);
-    if (!rc) rc = upload(d, &s->wleaves, w.wleaves.data(), w.wleaves.size() * sizeof(trt::Leaf), &total);
+    if (dw.complete) {
+        // the meshes' wide nodes and the leaves are on the device already: the top tree's nodes go after them
+        const uint64_t mesh_bytes = dw.wnode_count * sizeof(trt::WideNode);
+        const uint64_t bytes = w.nodes.size() * sizeof(trt::WideNode);
+        hipError_t e = hipMalloc(&s->wnodes, bytes ? bytes : 64);
+        if (e == hipSuccess && mesh_bytes)
+            e = hipMemcpy(s->wnodes, dw.d_wnodes, mesh_bytes, hipMemcpyDeviceToDevice);
+        if (e == hipSuccess && bytes > mesh_bytes)
+            e = hipMemcpy(static_cast<char *>(s->wnodes) + mesh_bytes, w.nodes.data() + dw.wnode_count,
+                          bytes - mesh_bytes, hipMemcpyHostToDevice);
+        hipFree(dw.d_wnodes);
+        s->wleaves = dw.d_wleaves;
+        total += (bytes ? bytes : 64) + std::max<uint64_t>(desc->leaf_count * sizeof(trt::Leaf), 64);
+        if (e != hipSuccess) rc = fail(TRT_E_DEVICE, std::string("trt_scene_upload: ") + hipGetErrorString(e));
+        if (!rc) rc = upload(d, &s->wunits, w.units.data(), w.units.size() * sizeof(trt::WideUnit), &total);
+    } else {
+        rc = upload(d, &s->wnodes, w.nodes.data(), w.nodes.size() * sizeof(trt::WideNode), &total);
+        if (!rc) rc = upload(d, &s->wunits, w.units.data(), w.units.size() * sizeof(trt::WideUnit), &total);
+        if (!rc) rc = upload(d, &s->wleaves, w.wleaves.data(), w.wleaves.size() * sizeof(trt::Leaf), &total);
+    }
     if (!rc && desc->energy) {
         s->shading = true;
         s->light_count = desc->light_count;
@@ -621,8 +650,10 @@ int trt_scene_upload(trt_device *d, const trt_scene_desc *desc, trt_scene **out)
     s->stats.units = desc->unit_count;
     s->stats.device_bytes = total;
     s->stats.build_ms = float(build_ms);
-    s->stats.device_build_ms = times.device_ms;
-    for (const trt::OwnBuilt &o : pre) s->stats.device_meshes += o.nodes.empty() ? 0 : 1;
+    s->stats.device_build_ms = dw.times.device_ms;
+    if (dw.complete) s->stats.device_meshes = uint32_t(meshes.size());
+    else
+        for (const trt::OwnBuilt &o : dw.trees) s->stats.device_meshes += o.nodes.empty() ? 0 : 1;
     s->stats.host_meshes = uint32_t(meshes.size()) - s->stats.device_meshes;
     *out = s;
     return ok();
@@ -630,8 +661,8 @@ int trt_scene_upload(trt_device *d, const trt_scene_desc *desc, trt_scene **out)
 
 int trt_device_set_build(trt_device *d, uint32_t where) {
     if (!d) return fail(TRT_E_INVALID, "trt_device_set_build: null device");
-    if (where != TRT_BUILD_DEVICE && where != TRT_BUILD_HOST)
-        return fail(TRT_E_INVALID, "trt_device_set_build: where must be TRT_BUILD_DEVICE or TRT_BUILD_HOST");
+    if (where != TRT_BUILD_DEVICE && where != TRT_BUILD_HOST && where != TRT_BUILD_AUTO)
+        return fail(TRT_E_INVALID, "trt_device_set_build: where must be TRT_BUILD_DEVICE, _HOST or _AUTO");
     d->build = where;
     return ok();
 }

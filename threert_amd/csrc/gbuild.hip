@@ -406,6 +406,141 @@ __global__ __launch_bounds__(kThreads) void k_emit(const TNode *tn, uint32_t cou
     out[node_base[nd.mesh] + index[t]] = o;
 }
 
+// ---- the collapse into 4-wide nodes, one wide level at a time -------------------------------------------------
+// Wide node t (level order) stands for binary interior node wbin[t] of mesh wmesh[t]; its children are the binary
+// nodes open_kids gives, and the interior ones among them become the next level's wide nodes.
+__global__ __launch_bounds__(kThreads) void k_w_open(uint32_t t0, uint32_t cnt, const Node *own,
+                                                     const uint64_t *node_base, const uint32_t *wmesh,
+                                                     const uint32_t *wbin, uint32_t *wkid, uint32_t *wnc,
+                                                     uint32_t *inner_kids) {
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i > cnt) return;
+    if (i == cnt) {
+        inner_kids[i] = 0;
+        return;
+    }
+    const uint32_t t = t0 + i;
+    const Node *nodes = own + node_base[wmesh[t]];
+    uint32_t kid[4];
+    const int nc = open_kids(nodes, 0, wbin[t], kid);
+    uint32_t inner = 0;
+    for (int c = 0; c < 4; c++) {
+        wkid[4 * uint64_t(t) + c] = kid[c];
+        if (c < nc && nodes[kid[c]].count == 0) inner++;
+    }
+    wnc[t] = uint32_t(nc);
+    inner_kids[i] = inner;
+}
+
+__global__ __launch_bounds__(kThreads) void k_w_link(uint32_t t0, uint32_t cnt, uint32_t next_base, const Node *own,
+                                                     const uint64_t *node_base, uint32_t *wmesh, uint32_t *wbin,
+                                                     const uint32_t *wkid, const uint32_t *wnc,
+                                                     const uint32_t *inner_before, uint32_t *wchild) {
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= cnt) return;
+    const uint32_t t = t0 + i, mesh = wmesh[t];
+    const Node *nodes = own + node_base[mesh];
+    uint32_t next = next_base + inner_before[i];
+    for (uint32_t c = 0; c < 4; c++) {
+        const uint32_t k = wkid[4 * uint64_t(t) + c];
+        if (c < wnc[t] && nodes[k].count == 0) {
+            wchild[4 * uint64_t(t) + c] = next;
+            wmesh[next] = mesh;
+            wbin[next] = k;
+            next++;
+        } else {
+            wchild[4 * uint64_t(t) + c] = kNone;
+        }
+    }
+}
+
+// Wide nodes under each (itself included) and the deepest a walk's stack gets under it (wide.cpp convert).
+__global__ __launch_bounds__(kThreads) void k_w_size(uint32_t t0, uint32_t cnt, const uint32_t *wnc,
+                                                     const uint32_t *wchild, uint32_t *wsize, uint32_t *wbound) {
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= cnt) return;
+    const uint32_t t = t0 + i;
+    uint32_t size = 1, deepest = 0;
+    for (uint32_t c = 0; c < wnc[t]; c++) {
+        const uint32_t ch = wchild[4 * uint64_t(t) + c];
+        if (ch == kNone) continue;
+        size += wsize[ch];
+        deepest = max(deepest, wbound[ch]);
+    }
+    wsize[t] = size;
+    wbound[t] = deepest + wnc[t] - 1;
+}
+
+// Preorder indices: a node's children follow it, each child's subtree before the next child.
+__global__ __launch_bounds__(kThreads) void k_w_index(uint32_t t0, uint32_t cnt, const uint32_t *wnc,
+                                                      const uint32_t *wchild, const uint32_t *wsize, uint32_t *widx) {
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= cnt) return;
+    const uint32_t t = t0 + i;
+    uint32_t run = widx[t] + 1;
+    for (uint32_t c = 0; c < wnc[t]; c++) {
+        const uint32_t ch = wchild[4 * uint64_t(t) + c];
+        if (ch == kNone) continue;
+        widx[ch] = run;
+        run += wsize[ch];
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_w_emit(uint32_t count, const Node *own, const uint64_t *node_base,
+                                                     const uint32_t *wmesh, const uint32_t *wkid,
+                                                     const uint32_t *wnc, const uint32_t *wchild,
+                                                     const uint32_t *widx, uint64_t total, WideNode *out,
+                                                     uint32_t *err) {
+    const uint32_t t = blockIdx.x * kThreads + threadIdx.x;
+    if (t >= count) return;
+    if (widx[t] >= total) {  // (a numbering that does not fit: reported, never written)
+        atomicOr(err, 4u);
+        return;
+    }
+    const Node *nodes = own + node_base[wmesh[t]];
+    const uint32_t nc = wnc[t];
+    WideNode w{};
+    const Node *kp[4] = {nodes, nodes, nodes, nodes};
+    for (uint32_t c = 0; c < 4; c++) {
+        if (c >= nc) {
+            w.child[c] = kNone;
+            continue;
+        }
+        const uint32_t k = wkid[4 * uint64_t(t) + c];
+        kp[c] = nodes + k;
+        const uint32_t ch = wchild[4 * uint64_t(t) + c];
+        if (ch != kNone) {
+            w.child[c] = widx[ch];
+        } else {
+            if (nodes[k].left_or_first > kRefTriFirstMask) atomicOr(err, 2u);
+            w.child[c] = wref_tris(nodes[k].left_or_first, nodes[k].count);
+        }
+    }
+    if (!quantise_node(w, kp, int(nc))) atomicOr(err, 1u);
+    out[widx[t]] = w;
+}
+
+// Every leaf in the wide trees' order: a mesh's range laid out in its tree's order, each leaf carrying its
+// reference index in e2.w; leaves of no mesh stay where they are.
+__global__ __launch_bounds__(kThreads) void k_w_leaves(const Leaf *leaves, uint64_t leaf_count, const uint32_t *first,
+                                                       const uint32_t *count, const uint32_t *base,
+                                                       uint32_t meshes, const uint32_t *order, Leaf *out) {
+    const uint64_t i = uint64_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (i >= leaf_count) return;
+    uint32_t lo = 0, hi = meshes;  // the last mesh (by first leaf) whose first <= i
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (first[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    uint32_t src = uint32_t(i);
+    if (meshes && first[lo] <= i && i < uint64_t(first[lo]) + count[lo])
+        src = first[lo] + (order[base[lo] + (uint32_t(i) - first[lo])] - base[lo]);
+    Leaf l = leaves[src];
+    __builtin_memcpy(&l.r[2][3], &src, 4);
+    out[i] = l;
+}
+
 inline uint32_t blocks(uint64_t n) { return uint32_t((n + kThreads - 1) / kThreads); }
 
 struct Scratch {
@@ -452,10 +587,161 @@ bool scan(const uint32_t *in, uint64_t n, uint32_t *out, uint32_t *tile_sum, uin
     return true;
 }
 
+// The meshes' wide nodes and the wide leaves from the own trees in `own` (each mesh's at node_base), all on the
+// device; the caller takes the two buffers.
+bool collapse(const Leaf *d_leaves, uint64_t leaf_count, const std::vector<MeshRef> &meshes,
+              const std::vector<MeshDev> &hm, const std::vector<uint64_t> &node_base, const Node *own,
+              const uint64_t *dbase, const uint32_t *order, hipStream_t st, Scratch &sc, hipEvent_t e0, hipEvent_t e1,
+              size_t tree_levels, uint64_t own_nodes, DeviceWide &res, std::string *why) {
+    const uint32_t mc = uint32_t(meshes.size());
+    // each mesh's root: a leaf (no wide node: the mesh is one run of triangles) or the first wide node
+    std::vector<Node> roots(mc);
+    for (uint32_t i = 0; i < mc; i++)
+        GB_HIP(hipMemcpyAsync(&roots[i], own + node_base[i], sizeof(Node), hipMemcpyDeviceToHost, st));
+    GB_HIP(hipStreamSynchronize(st));
+    std::vector<uint32_t> wroot_mesh;
+    for (uint32_t i = 0; i < mc; i++)
+        if (roots[i].count == 0) wroot_mesh.push_back(i);
+    const uint64_t wcap = own_nodes / 2 + mc + 1;  // wide nodes <= interior binary nodes
+    uint32_t *wmesh, *wbin, *wkid, *wnc, *wchild, *wsize, *wbound, *widx, *inner_kids, *inner_before, *err,
+        *tile_sum, *grand;
+    GB_HIP(sc.get(&wmesh, wcap));
+    GB_HIP(sc.get(&wbin, wcap));
+    GB_HIP(sc.get(&wkid, 4 * wcap));
+    GB_HIP(sc.get(&wnc, wcap));
+    GB_HIP(sc.get(&wchild, 4 * wcap));
+    GB_HIP(sc.get(&wsize, wcap));
+    GB_HIP(sc.get(&wbound, wcap));
+    GB_HIP(sc.get(&widx, wcap));
+    GB_HIP(sc.get(&inner_kids, wcap + 1));
+    GB_HIP(sc.get(&inner_before, wcap + 1));
+    GB_HIP(sc.get(&err, 1));
+    GB_HIP(sc.get(&tile_sum, (wcap + 1) / kScanTile + 2));
+    GB_HIP(sc.get(&grand, 1));
+    GB_HIP(hipMemsetAsync(err, 0, 4, st));
+    const uint32_t r0 = uint32_t(wroot_mesh.size());
+    if (r0) {
+        std::vector<uint32_t> zero(r0, 0);
+        GB_HIP(hipMemcpyAsync(wmesh, wroot_mesh.data(), r0 * 4, hipMemcpyHostToDevice, st));
+        GB_HIP(hipMemcpyAsync(wbin, zero.data(), r0 * 4, hipMemcpyHostToDevice, st));
+    }
+    std::vector<std::pair<uint32_t, uint32_t>> levels;
+    uint32_t t0 = 0, cnt = r0;
+    while (cnt) {
+        levels.push_back({t0, cnt});
+        k_w_open<<<blocks(uint64_t(cnt) + 1), kThreads, 0, st>>>(t0, cnt, own, dbase, wmesh, wbin, wkid, wnc,
+                                                                  inner_kids);
+        if (!scan(inner_kids, uint64_t(cnt) + 1, inner_before, tile_sum, grand, st, why)) return false;
+        uint32_t next = 0;
+        GB_HIP(hipMemcpyAsync(&next, inner_before + cnt, 4, hipMemcpyDeviceToHost, st));
+        GB_HIP(hipStreamSynchronize(st));
+        if (uint64_t(t0) + cnt + next > wcap) {
+            *why = "device build: more wide nodes than interior nodes";
+            return false;
+        }
+        k_w_link<<<blocks(cnt), kThreads, 0, st>>>(t0, cnt, t0 + cnt, own, dbase, wmesh, wbin, wkid, wnc,
+                                                   inner_before, wchild);
+        t0 += cnt;
+        cnt = next;
+    }
+    const uint32_t wcount = t0;
+    for (size_t k = levels.size(); k-- > 0;)
+        k_w_size<<<blocks(levels[k].second), kThreads, 0, st>>>(levels[k].first, levels[k].second, wnc, wchild,
+                                                                  wsize, wbound);
+    std::vector<uint32_t> rsize(r0), rbound(r0);
+    if (r0) {
+        GB_HIP(hipMemcpyAsync(rsize.data(), wsize, r0 * 4, hipMemcpyDeviceToHost, st));
+        GB_HIP(hipMemcpyAsync(rbound.data(), wbound, r0 * 4, hipMemcpyDeviceToHost, st));
+    }
+    GB_HIP(hipStreamSynchronize(st));
+    // meshes' blocks in mesh order, as the host appends them
+    res.mesh_ref.assign(mc, 0);
+    res.mesh_bound.assign(mc, 0);
+    std::vector<uint32_t> ridx(r0);
+    uint64_t total = 0;
+    for (uint32_t i = 0, k = 0; i < mc; i++) {
+        if (roots[i].count > 0) {
+            if (roots[i].left_or_first > kRefTriFirstMask) {
+                *why = "a mesh has more triangles than a reference holds (2^26)";
+                return false;
+            }
+            res.mesh_ref[i] = wref_tris(roots[i].left_or_first, roots[i].count);
+            continue;
+        }
+        ridx[k] = uint32_t(total);
+        res.mesh_ref[i] = uint32_t(total);
+        res.mesh_bound[i] = rbound[k];
+        total += rsize[k];
+        k++;
+    }
+    if (total >= (uint64_t{1} << 30)) {
+        *why = "device build: tree too large";
+        return false;
+    }
+    if (r0) GB_HIP(hipMemcpyAsync(widx, ridx.data(), r0 * 4, hipMemcpyHostToDevice, st));
+    for (const auto &lv : levels)
+        k_w_index<<<blocks(lv.second), kThreads, 0, st>>>(lv.first, lv.second, wnc, wchild, wsize, widx);
+    void *wn = nullptr, *wl = nullptr;
+    GB_HIP(hipMalloc(&wn, std::max<uint64_t>(total * sizeof(WideNode), 64)));
+    res.d_wnodes = wn;
+    res.wnode_count = total;
+    GB_HIP(hipMalloc(&wl, std::max<uint64_t>(leaf_count * sizeof(Leaf), 64)));
+    res.d_wleaves = wl;
+    if (wcount)
+        k_w_emit<<<blocks(wcount), kThreads, 0, st>>>(wcount, own, dbase, wmesh, wkid, wnc, wchild, widx, total,
+                                                      static_cast<WideNode *>(wn), err);
+    // the leaves: meshes by first leaf
+    std::vector<uint32_t> by(mc);
+    for (uint32_t i = 0; i < mc; i++) by[i] = i;
+    std::sort(by.begin(), by.end(), [&](uint32_t a, uint32_t b) { return meshes[a].first < meshes[b].first; });
+    std::vector<uint32_t> hf(mc), hcnt(mc), hb(mc);
+    for (uint32_t k = 0; k < mc; k++) hf[k] = meshes[by[k]].first, hcnt[k] = meshes[by[k]].n, hb[k] = hm[by[k]].base;
+    uint32_t *df, *dc, *db;
+    GB_HIP(sc.get(&df, mc));
+    GB_HIP(sc.get(&dc, mc));
+    GB_HIP(sc.get(&db, mc));
+    GB_HIP(hipMemcpyAsync(df, hf.data(), mc * 4, hipMemcpyHostToDevice, st));
+    GB_HIP(hipMemcpyAsync(dc, hcnt.data(), mc * 4, hipMemcpyHostToDevice, st));
+    GB_HIP(hipMemcpyAsync(db, hb.data(), mc * 4, hipMemcpyHostToDevice, st));
+    if (leaf_count)
+        k_w_leaves<<<blocks(leaf_count), kThreads, 0, st>>>(d_leaves, leaf_count, df, dc, db, mc, order,
+                                                            static_cast<Leaf *>(wl));
+    GB_HIP(hipGetLastError());
+    GB_HIP(hipEventRecord(e1, st));
+    uint32_t herr = 0;
+    GB_HIP(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, st));
+    GB_HIP(hipStreamSynchronize(st));
+    if (herr & 4u) {
+        *why = "device build: wide node numbering out of range";
+        return false;
+    }
+    if (herr & 2u) {
+        *why = "a mesh has more triangles than a reference holds (2^26)";
+        return false;
+    }
+    if (herr & 1u) {
+        *why = "a node's box cannot be quantised";
+        return false;
+    }
+    float ms = 0;
+    GB_HIP(hipEventElapsedTime(&ms, e0, e1));
+    res.times.device_ms = ms;
+    res.times.levels = uint32_t(tree_levels + levels.size());
+    res.times.nodes = own_nodes;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    res.complete = true;
+    for (auto &o : res.trees) o = OwnBuilt{};
+    return true;
+}
+
 }  // namespace
 
-bool own_trees_device(const Leaf *d_leaves, const std::vector<MeshRef> &meshes, hipStream_t st,
-                      std::vector<OwnBuilt> &out, BuildTimes *times, std::string *why) {
+bool device_wide(const Leaf *d_leaves, uint64_t leaf_count, const std::vector<MeshRef> &meshes, hipStream_t st,
+                 DeviceWide &res, std::string *why) {
+    res = DeviceWide{};
+    std::vector<OwnBuilt> &out = res.trees;
+    BuildTimes *times = &res.times;
     out.assign(meshes.size(), OwnBuilt{});
     uint64_t n64 = 0;
     for (const MeshRef &m : meshes) n64 += m.n;
@@ -610,6 +896,10 @@ bool own_trees_device(const Leaf *d_leaves, const std::vector<MeshRef> &meshes, 
     k_emit<<<blocks(total_nodes), kThreads, 0, st>>>(tn, uint32_t(total_nodes), rank, index, dm, dbase, host_mesh,
                                                      dout);
     GB_HIP(hipGetLastError());
+    bool all_device = roots.size() == mc;
+    for (uint32_t i = 0; i < mc; i++) all_device = all_device && !failed[i];
+    if (all_device) return collapse(d_leaves, leaf_count, meshes, hm, node_base, dout, dbase, order, st, sc, e0, e1,
+                                    levels.size(), out_nodes, res, why);
     GB_HIP(hipEventRecord(e1, st));
     std::vector<Node> hn(out_nodes);
     std::vector<uint32_t> ho(n);

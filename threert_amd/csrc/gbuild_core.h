@@ -227,5 +227,111 @@ TRT_HD inline bool goes_left(const Slot &s, const GBox &b, uint32_t j) {
     return false;
 }
 
+// ---- the collapse into 4-wide nodes and their quantisation (wide.cpp Collapser and quantise) ---------------------
+TRT_HD inline uint32_t wref_tris(uint32_t first, uint32_t count) {
+    return (kRefTris << kRefKindShift) | ((count - 1) << kRefTriFirstBits) | first;
+}
+
+// A binary node's surface area, its extents clamped at 0 (the Collapser's measure for which child to open).
+TRT_HD inline float open_area(const Node &n) {
+    const float x0 = n.bmax[0] - n.bmin[0], y0 = n.bmax[1] - n.bmin[1], z0 = n.bmax[2] - n.bmin[2];
+    const float ex = x0 < 0.0f ? 0.0f : x0, ey = y0 < 0.0f ? 0.0f : y0, ez = z0 < 0.0f ? 0.0f : z0;
+    return 2.0f * (ex * ey + ey * ez + ez * ex);
+}
+
+// The children of the wide node made from binary interior node `b` (children at nodes[left_or_first] + {0, 1},
+// relative to `off`): while fewer than four, the interior child of largest area (the first of equals) is replaced
+// by its two children in place. Returns the count.
+TRT_HD inline int open_kids(const Node *nodes, uint32_t off, uint32_t b, uint32_t *kid) {
+    kid[0] = off + nodes[b].left_or_first;
+    kid[1] = kid[0] + 1;
+    kid[2] = kid[3] = 0;
+    int nc = 2;
+    while (nc < 4) {
+        int best = -1;
+        float best_area = -1.0f;
+        for (int c = 0; c < nc; c++) {
+            const Node &k = nodes[kid[c]];
+            if (k.count == 0 && open_area(k) > best_area) best_area = open_area(k), best = c;
+        }
+        if (best < 0) break;
+        const uint32_t l = off + nodes[kid[best]].left_or_first;
+        for (int c = nc; c > best + 1; c--) kid[c] = kid[c - 1];
+        kid[best] = l;
+        kid[best + 1] = l + 1;
+        nc++;
+    }
+    return nc;
+}
+
+// ceil(log2(y)) for a positive normal double, exactly (from its exponent and whether its mantissa is zero).
+TRT_HD inline int ceil_log2(double y) {
+    uint64_t u;
+    __builtin_memcpy(&u, &y, 8);
+    const int e = int((u >> 52) & 0x7ff) - 1023;
+    return (u & ((uint64_t{1} << 52) - 1)) ? e + 1 : e;
+}
+TRT_HD inline double pow2(int e) {  // 2^e for -1022 <= e <= 1023
+    const uint64_t u = uint64_t(e + 1023) << 52;
+    double d;
+    __builtin_memcpy(&d, &u, 8);
+    return d;
+}
+TRT_HD inline float next_down_d(double want) {  // the float at or below `want`
+    float f = static_cast<float>(want);
+    if (double(f) > want) f = next_down(f);
+    return f;
+}
+
+// Conservative 8-bit child boxes against a float origin and a power-of-two step per axis; false if an axis cannot
+// be held (wide.cpp quantise).
+TRT_HD inline bool quantise_node(WideNode &w, const Node *const *kids, int nc) {
+    for (int k = 0; k < 3; k++) {
+        double lo = __builtin_inf(), hi = -__builtin_inf();
+        for (int c = 0; c < nc; c++) {
+            const double a = kids[c]->bmin[k], b = kids[c]->bmax[k];
+            lo = a < lo ? a : lo;
+            hi = hi < b ? b : hi;
+        }
+        const double ext = hi - lo;
+        int e = -100;
+        if (ext > 0) {
+            const int c2 = ceil_log2(ext / 252.0);
+            e = c2 < -100 ? -100 : c2;
+        }
+        for (;; e++) {
+            if (e > 126) return false;
+            const double s = pow2(e);
+            const float of = next_down_d(lo - s);
+            if (!(double(of) - double(of) == 0.0)) return false;
+            bool ok = true;
+            uint32_t qlo = 0, qhi = 0;
+            for (int c = 0; c < nc && ok; c++) {
+                const double clo = kids[c]->bmin[k], chi = kids[c]->bmax[k];
+                double ql = __builtin_floor((clo - double(of)) / s) - 1.0;
+                double qh = __builtin_ceil((chi - double(of)) / s) + 1.0;
+                if (ql < 0) ql = 0;
+                while (ql > 0 && double(of) + ql * s > clo) ql -= 1.0;
+                while (double(of) + qh * s < chi) qh += 1.0;
+                if (qh > 255.0 || double(of) + ql * s > clo) {
+                    ok = false;
+                    break;
+                }
+                qlo |= uint32_t(ql) << (8 * c);
+                qhi |= uint32_t(qh) << (8 * c);
+            }
+            if (!ok) continue;
+            for (int c = nc; c < 4; c++) qlo |= 0xffu << (8 * c);  // an empty slot: low 255, high 0
+            w.origin[k] = of;
+            w.exp[k] = static_cast<uint8_t>(e + 127);
+            w.qlo[k] = qlo;
+            w.qhi[k] = qhi;
+            break;
+        }
+    }
+    w.nchild = static_cast<uint8_t>(nc);
+    return true;
+}
+
 }  // namespace gb
 }  // namespace trt

@@ -147,8 +147,16 @@ void own_tree_host(const Leaf *leaves, uint32_t first, uint32_t n, OwnBuilt &out
 bool wide_meshes(const Node *nodes, uint64_t node_count, const Instance *inst, uint64_t inst_count,
                  uint64_t leaf_count, uint32_t top_first, uint32_t unit_count, std::vector<MeshRef> &meshes,
                  std::string *why);
+// `mesh_wide` (optional): the meshes' wide nodes were made elsewhere (the device), mesh_wide->count of them ahead
+// of the top tree's; out.nodes then holds placeholders for them, out.wleaves is left empty, and each mesh's root
+// reference and stack bound come from mesh_wide.
+struct MeshWide {
+    uint64_t count = 0;
+    std::vector<uint32_t> ref, bound;
+};
 bool build_wide(const Node *nodes, uint64_t node_count, const Leaf *leaves, const Instance *inst, uint64_t inst_count,
                 uint64_t leaf_count, uint32_t top_first, uint32_t unit_count, WideScene &out, std::string *why,
-                const std::vector<MeshRef> *meshes = nullptr, const std::vector<OwnBuilt> *pre = nullptr);
+                const std::vector<MeshRef> *meshes = nullptr, const std::vector<OwnBuilt> *pre = nullptr,
+                const MeshWide *mesh_wide = nullptr);
 
 }  // namespace trt

@@ -20,59 +20,16 @@
 #include <unordered_map>
 #include <vector>
 
+#define TRT_HD
+#include "gbuild_core.h"
 #include "trt_internal.h"
 
 namespace trt {
 namespace {
 
-inline float node_area(const Node &n) {
-    float ex = std::max(n.bmax[0] - n.bmin[0], 0.0f), ey = std::max(n.bmax[1] - n.bmin[1], 0.0f),
-          ez = std::max(n.bmax[2] - n.bmin[2], 0.0f);
-    return 2.0f * (ex * ey + ey * ez + ez * ex);
-}
+inline float node_area(const Node &n) { return gb::open_area(n); }
 
-bool quantise(WideNode &w, const Node *const *kids, int nc) {
-    for (int k = 0; k < 3; k++) {
-        double lo = INFINITY, hi = -INFINITY;
-        for (int c = 0; c < nc; c++) lo = std::min(lo, double(kids[c]->bmin[k])), hi = std::max(hi, double(kids[c]->bmax[k]));
-        const double ext = hi - lo;
-        int e = -100;
-        if (ext > 0) e = std::max(-100, int(std::ceil(std::log2(ext / 252.0))));
-        for (;; e++) {
-            if (e > 126) return false;
-            const double s = std::ldexp(1.0, e);
-            const double want = lo - s;
-            float of = static_cast<float>(want);
-            if (double(of) > want) of = std::nextafter(of, -INFINITY);
-            if (!std::isfinite(of)) return false;
-            bool ok = true;
-            uint32_t qlo = 0, qhi = 0;
-            for (int c = 0; c < nc && ok; c++) {
-                const double clo = kids[c]->bmin[k], chi = kids[c]->bmax[k];
-                double ql = std::floor((clo - double(of)) / s) - 1.0;
-                double qh = std::ceil((chi - double(of)) / s) + 1.0;
-                if (ql < 0) ql = 0;
-                while (ql > 0 && double(of) + ql * s > clo) ql -= 1.0;
-                while (double(of) + qh * s < chi) qh += 1.0;
-                if (qh > 255.0 || double(of) + ql * s > clo) {
-                    ok = false;
-                    break;
-                }
-                qlo |= uint32_t(ql) << (8 * c);
-                qhi |= uint32_t(qh) << (8 * c);
-            }
-            if (!ok) continue;
-            for (int c = nc; c < 4; c++) qlo |= 0xffu << (8 * c);  // an empty slot: low 255, high 0
-            w.origin[k] = of;
-            w.exp[k] = static_cast<uint8_t>(e + 127);
-            w.qlo[k] = qlo;
-            w.qhi[k] = qhi;
-            break;
-        }
-    }
-    w.nchild = static_cast<uint8_t>(nc);
-    return true;
-}
+bool quantise(WideNode &w, const Node *const *kids, int nc) { return gb::quantise_node(w, kids, nc); }
 
 struct Collapser {
     const Node *nodes;
@@ -95,22 +52,8 @@ struct Collapser {
             if (n.left_or_first > kRefTriFirstMask) err = "a mesh has more triangles than a reference holds (2^26)";
             return ref_tris(n.left_or_first, n.count);
         }
-        uint32_t kid[4] = {off + n.left_or_first, off + n.left_or_first + 1, 0, 0};
-        int nc = 2;
-        while (nc < 4) {
-            int best = -1;
-            float best_area = -1.0f;
-            for (int c = 0; c < nc; c++) {
-                const Node &k = nodes[kid[c]];
-                if (k.count == 0 && node_area(k) > best_area) best_area = node_area(k), best = c;
-            }
-            if (best < 0) break;
-            const uint32_t l = off + nodes[kid[best]].left_or_first;
-            for (int c = nc; c > best + 1; c--) kid[c] = kid[c - 1];
-            kid[best] = l;
-            kid[best + 1] = l + 1;
-            nc++;
-        }
+        uint32_t kid[4];
+        const int nc = gb::open_kids(nodes, off, b, kid);
         const uint32_t wi = static_cast<uint32_t>(out.nodes.size());
         out.nodes.emplace_back();
         WideNode w{};
@@ -439,22 +382,36 @@ bool wide_meshes(const Node *nodes, uint64_t node_count, const Instance *inst, u
 
 bool build_wide(const Node *nodes, uint64_t node_count, const Leaf *leaves, const Instance *inst, uint64_t inst_count,
                 uint64_t leaf_count, uint32_t top_first, uint32_t unit_count, WideScene &out, std::string *why,
-                const std::vector<MeshRef> *meshes, const std::vector<OwnBuilt> *pre) {
+                const std::vector<MeshRef> *meshes, const std::vector<OwnBuilt> *pre, const MeshWide *mesh_wide) {
     out = WideScene();
     out.units.assign(inst_count, WideUnit{});
-    out.wleaves.assign(leaves, leaves + leaf_count);
-    for (uint64_t i = 0; i < leaf_count; i++) out.wleaves[i].r[2][3] = from_bits(static_cast<uint32_t>(i));
+    if (!mesh_wide) {
+        out.wleaves.assign(leaves, leaves + leaf_count);
+        for (uint64_t i = 0; i < leaf_count; i++) out.wleaves[i].r[2][3] = from_bits(static_cast<uint32_t>(i));
+    }
     if (unit_count == 0) return true;
     Plan p;
     if (!plan_of(nodes, node_count, inst, inst_count, leaf_count, top_first, p, why)) return false;
-    if (pre && (!meshes || meshes->size() != p.meshes.size() || pre->size() != p.meshes.size())) {
+    if ((pre || mesh_wide) && (!meshes || meshes->size() != p.meshes.size() ||
+                               (pre && pre->size() != p.meshes.size()) ||
+                               (mesh_wide && (mesh_wide->ref.size() != p.meshes.size() ||
+                                              mesh_wide->bound.size() != p.meshes.size())))) {
         *why = "prebuilt mesh trees do not match the scene's meshes";
         return false;
     }
     Collapser col{nodes, node_count, out};
     std::vector<std::pair<uint32_t, uint32_t>> mesh(p.meshes.size());  // wide ref, stack bound
+    if (mesh_wide) out.nodes.resize(mesh_wide->count);  // (the device's: uploaded from there)
     for (size_t i = 0; i < p.meshes.size(); i++) {
         const MeshRef m = p.meshes[i];
+        if (meshes && ((*meshes)[i].root != m.root || (*meshes)[i].first != m.first || (*meshes)[i].n != m.n)) {
+            *why = "prebuilt mesh trees do not match the scene's meshes";
+            return false;
+        }
+        if (mesh_wide) {
+            mesh[i] = {mesh_wide->ref[i], mesh_wide->bound[i]};
+            continue;
+        }
         if (pre && ((*meshes)[i].root != m.root || (*meshes)[i].first != m.first || (*meshes)[i].n != m.n)) {
             *why = "prebuilt mesh trees do not match the scene's meshes";
             return false;
