@@ -559,20 +559,26 @@ int trt_scene_upload(trt_device *d, const trt_scene_desc *desc, trt_scene **out)
     for (const trt::MeshRef &m : meshes) mesh_tris += m.n;
     const bool on_device = !meshes.empty() && (d->build == TRT_BUILD_DEVICE ||
                                                (d->build == TRT_BUILD_AUTO && mesh_tris >= kDeviceBuildMin));
+    hipStream_t bs = nullptr;  // the device build's stream, kept until its wide nodes are copied
     if (!rc && on_device) {
-        hipStream_t bs = nullptr;
         hipError_t e = hipStreamCreateWithFlags(&bs, hipStreamNonBlocking);
         if (e != hipSuccess) {
+            bs = nullptr;
             rc = fail(TRT_E_DEVICE, std::string("trt_scene_upload: ") + hipGetErrorString(e));
-        } else {
-            if (!trt::gb::device_wide(static_cast<const trt::Leaf *>(s->leaves), desc->leaf_count, meshes, bs, dw,
-                                      &why))
-                rc = fail(TRT_E_DEVICE, "trt_scene_upload: " + why);
-            hipStreamSynchronize(bs);
-            hipStreamDestroy(bs);
+        } else if (!trt::gb::device_wide(static_cast<const trt::Leaf *>(s->leaves), desc->leaf_count, meshes, bs, dw,
+                                         &why)) {
+            rc = fail(TRT_E_DEVICE, "trt_scene_upload: " + why);
         }
         if (dw.complete) mw = trt::MeshWide{dw.wnode_count, dw.mesh_ref, dw.mesh_bound};
     }
+    auto end_build_stream = [&]() {
+        if (!bs) return;
+        if (dw.d_wnodes) hipFreeAsync(dw.d_wnodes, bs);
+        dw.d_wnodes = nullptr;
+        hipStreamSynchronize(bs);
+        hipStreamDestroy(bs);
+        bs = nullptr;
+    };
     if (!rc && !trt::build_wide(nodes, desc->node_count, leaves, inst, desc->instance_count, desc->leaf_count,
                                 desc->top_first, desc->unit_count, w, &why, &meshes,
                                 on_device && !dw.complete ? &dw.trees : nullptr, dw.complete ? &mw : nullptr))
@@ -582,7 +588,7 @@ int trt_scene_upload(trt_device *d, const trt_scene_desc *desc, trt_scene **out)
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_build).count();
     if (rc) {
         std::string keep = g_error;
-        hipFree(dw.d_wnodes);
+        end_build_stream();
         hipFree(dw.d_wleaves);
         trt_scene_release(s);
         return fail(rc, keep);
@@ -594,16 +600,17 @@ int trt_scene_upload(trt_device *d, const trt_scene_desc *desc, trt_scene **out)
         const uint64_t bytes = w.nodes.size() * sizeof(trt::WideNode);
         hipError_t e = hipMalloc(&s->wnodes, bytes ? bytes : 64);
         if (e == hipSuccess && mesh_bytes)
-            e = hipMemcpy(s->wnodes, dw.d_wnodes, mesh_bytes, hipMemcpyDeviceToDevice);
+            e = hipMemcpyAsync(s->wnodes, dw.d_wnodes, mesh_bytes, hipMemcpyDeviceToDevice, bs);
         if (e == hipSuccess && bytes > mesh_bytes)
-            e = hipMemcpy(static_cast<char *>(s->wnodes) + mesh_bytes, w.nodes.data() + dw.wnode_count,
-                          bytes - mesh_bytes, hipMemcpyHostToDevice);
-        hipFree(dw.d_wnodes);
+            e = hipMemcpyAsync(static_cast<char *>(s->wnodes) + mesh_bytes, w.nodes.data() + dw.wnode_count,
+                               bytes - mesh_bytes, hipMemcpyHostToDevice, bs);
+        end_build_stream();  // (waits for the copies; frees the build's copy of the nodes in stream order)
         s->wleaves = dw.d_wleaves;
         total += (bytes ? bytes : 64) + std::max<uint64_t>(desc->leaf_count * sizeof(trt::Leaf), 64);
         if (e != hipSuccess) rc = fail(TRT_E_DEVICE, std::string("trt_scene_upload: ") + hipGetErrorString(e));
         if (!rc) rc = upload(d, &s->wunits, w.units.data(), w.units.size() * sizeof(trt::WideUnit), &total);
     } else {
+        end_build_stream();
         rc = upload(d, &s->wnodes, w.nodes.data(), w.nodes.size() * sizeof(trt::WideNode), &total);
         if (!rc) rc = upload(d, &s->wunits, w.units.data(), w.units.size() * sizeof(trt::WideUnit), &total);
         if (!rc) rc = upload(d, &s->wleaves, w.wleaves.data(), w.wleaves.size() * sizeof(trt::Leaf), &total);

@@ -19,8 +19,8 @@ struct BuildTimes {
 
 struct DeviceWide {
     // complete: every mesh was built and collapsed on the device; d_wnodes holds the meshes' wide nodes (mesh after
-    // mesh, as the host appends them: wnode_count of them) and d_wleaves every leaf in the wide trees' order; the
-    // caller owns both (hipFree). mesh_ref / mesh_bound: each mesh's root reference and stack bound.
+    // mesh, as the host appends them: wnode_count of them; stream-ordered memory of the build's stream, for
+    // hipFreeAsync there) and d_wleaves every leaf in the wide trees' order (hipFree); the caller owns both. mesh_ref / mesh_bound: each mesh's root reference and stack bound.
     bool complete = false;
     void *d_wnodes = nullptr;
     uint64_t wnode_count = 0;

@@ -543,23 +543,28 @@ __global__ __launch_bounds__(kThreads) void k_w_leaves(const Leaf *leaves, uint6
 
 inline uint32_t blocks(uint64_t n) { return uint32_t((n + kThreads - 1) / kThreads); }
 
+// The build's working memory, stream-ordered (hipMallocAsync / hipFreeAsync on the build's stream): plain
+// hipFree waits for the whole device, which would hold up the renders other threads queue meanwhile.
 struct Scratch {
+    hipStream_t st;
     std::vector<void *> ptrs;
+    explicit Scratch(hipStream_t s) : st(s) {}
     ~Scratch() {
-        for (void *p : ptrs) (void)hipFree(p);
+        for (void *p : ptrs)
+            if (p) (void)hipFreeAsync(p, st);
     }
     template <class T>
-    void drop(T *p) {  // (the stream is idle)
+    void drop(T *p) {
         for (void *&q : ptrs)
-            if (q == p) {
-                (void)hipFree(q);
+            if (q && q == p) {
+                (void)hipFreeAsync(q, st);
                 q = nullptr;
             }
     }
     template <class T>
     hipError_t get(T **p, uint64_t count) {
         void *v = nullptr;
-        const hipError_t e = hipMalloc(&v, std::max<uint64_t>(count * sizeof(T), 64));
+        const hipError_t e = hipMallocAsync(&v, std::max<uint64_t>(count * sizeof(T), 64), st);
         if (e != hipSuccess) return e;
         ptrs.push_back(v);
         *p = static_cast<T *>(v);
@@ -682,7 +687,7 @@ bool collapse(const Leaf *d_leaves, uint64_t leaf_count, const std::vector<MeshR
     for (const auto &lv : levels)
         k_w_index<<<blocks(lv.second), kThreads, 0, st>>>(lv.first, lv.second, wnc, wchild, wsize, widx);
     void *wn = nullptr, *wl = nullptr;
-    GB_HIP(hipMalloc(&wn, std::max<uint64_t>(total * sizeof(WideNode), 64)));
+    GB_HIP(hipMallocAsync(&wn, std::max<uint64_t>(total * sizeof(WideNode), 64), st));  // (freed on `st` too)
     res.d_wnodes = wn;
     res.wnode_count = total;
     GB_HIP(hipMalloc(&wl, std::max<uint64_t>(leaf_count * sizeof(Leaf), 64)));
@@ -755,7 +760,7 @@ bool device_wide(const Leaf *d_leaves, uint64_t leaf_count, const std::vector<Me
     GB_HIP(hipEventCreate(&e0));
     GB_HIP(hipEventCreate(&e1));
     GB_HIP(hipEventRecord(e0, st));
-    Scratch sc;
+    Scratch sc(st);
     std::vector<MeshDev> hm(mc);
     for (uint32_t i = 0, b = 0; i < mc; i++) {
         hm[i] = MeshDev{meshes[i].first, b, meshes[i].n, 0};
