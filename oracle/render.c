@@ -241,6 +241,7 @@ static inline uint32_t material_of(const Ctx *c, const TriAttr *a, uint32_t inst
 typedef struct {
     V p, n, ng, base, emissive;
     float roughness, metallic, specular;
+    float coat, coat_rough; /* pathtrace.wgsl:1420-1423 (F_COAT: on when any material has one; 0 otherwise) */
     int front, card;
 } Surface;
 
@@ -280,6 +281,8 @@ static Surface surface(Ctx *c, V o, V d, const or_hit *h) {
     s.specular = spec;
     s.ng = ng;
     s.n = n;
+    s.coat = clampf(m[28 + 2], 0.0f, 1.0f); /* layer.z, layer.w */
+    s.coat_rough = clampf(m[28 + 3], 0.02f, 1.0f);
     return s;
 }
 
@@ -287,6 +290,7 @@ static Surface surface(Ctx *c, V o, V d, const or_hit *h) {
 typedef struct {
     V diffuse, f0, f90, w, e_spec, kd;
     float a2, p_spec, p_coat, p_trans;
+    float coat, coat_alpha, e_coat;
 } Lobes;
 
 static void energy_node(const Ctx *c, uint32_t i, float *out) {
@@ -335,7 +339,18 @@ static Lobes lobes(const Ctx *c, const Surface *s, float nov) {
     float wd = dot(vmul(l.diffuse, l.kd), lw);
     l.p_spec = clampf(ws / fmaxf(ws + wd, 1e-6f), 0.05f, 1.0f);
     if (wd <= 0.0f) l.p_spec = 1.0f;
-    float e_coat = 0.0f, wt = 0.0f, under = 1.0f - e_coat;
+    /* pathtrace.wgsl:1675-1683: the coat's directional albedo, a single-scattering lobe of F0 0.04, F90 1 */
+    l.coat = 0.0f;
+    l.coat_alpha = 1.0f;
+    l.e_coat = 0.0f;
+    if (s->coat > 0.0f) {
+        float ec[2];
+        l.coat = s->coat;
+        energy_lookup(c, s->coat_rough, nov, ec);
+        l.e_coat = s->coat * (0.04f * ec[0] + ec[1]);
+        l.coat_alpha = s->coat_rough * s->coat_rough;
+    }
+    float e_coat = l.e_coat, wt = 0.0f, under = 1.0f - e_coat;
     float total = (ws + wd) * under + e_coat + wt;
     l.p_coat = e_coat / fmaxf(total, 1e-6f);
     l.p_trans = wt / fmaxf(total, 1e-6f);
@@ -364,6 +379,13 @@ static V eval_bsdf_k(const Surface *s, const Lobes *lb, V v, V l, float k) {
     float dv = d_ggx(lb->a2, noh) * vis_smith(lb->a2, nov, nol);
     V spec = vs(vmul(vs(f_schlick(lb->f0, lb->f90, voh), dv), lb->w), k);
     V diff = vmul(vs(lb->diffuse, INV_PI), lb->kd);
+    if (lb->coat > 0.0f) { /* pathtrace.wgsl:1750-1755 */
+        float ca2 = lb->coat_alpha * lb->coat_alpha;
+        float cf = f_schlick(v3(0.04f, 0.04f, 0.04f), v3(1, 1, 1), voh).x;
+        float coat = d_ggx(ca2, noh) * vis_smith(ca2, nov, nol) * cf * lb->coat * k;
+        diff = vs(diff, 1.0f - lb->e_coat);
+        spec = vadd(vs(spec, 1.0f - lb->e_coat), v3(coat, coat, coat));
+    }
     return vs(vadd(diff, spec), nol);
 }
 static V eval_bsdf(const Surface *s, const Lobes *lb, V v, V l) { return eval_bsdf_k(s, lb, v, l, 1.0f); }
@@ -382,6 +404,10 @@ static float pdf_bsdf(const Surface *s, const Lobes *lb, V v, V l) {
     float ps = dv / (4.0f * voh);
     float pd = nol / PI_F;
     float pc = 0.0f;
+    if (lb->coat > 0.0f) { /* pathtrace.wgsl:1783-1786 */
+        float ca2 = lb->coat_alpha * lb->coat_alpha;
+        pc = g1(ca2, nov) * d_ggx(ca2, noh) / (4.0f * nov);
+    }
     return pg + lb->p_coat * pc + (1.0f - lb->p_coat - lb->p_trans) * (lb->p_spec * ps + (1.0f - lb->p_spec) * pd);
 }
 
@@ -428,6 +454,10 @@ static V sample_bsdf(Ctx *c, const Surface *s, const Lobes *lb, V v) {
     M3 tbn = basis(s->n);
     float r[4];
     or_rng4(&c->g, or_rng_dim(&c->g, D_BSDF), r);
+    if (r[0] < lb->p_coat) { /* pathtrace.wgsl:1853-1857: the coat's lobe (p_coat is 0 without one) */
+        c->sampled_spec = 1;
+        return reflect(vneg(v), sample_vndf(v, tbn, lb->coat_alpha, r[2], r[3]));
+    }
     c->sampled_spec = r[1] < lb->p_spec;
     if (c->sampled_spec) {
         V h = sample_vndf(v, tbn, s->roughness * s->roughness, r[2], r[3]);

@@ -248,6 +248,7 @@ __device__ __forceinline__ uint32_t material_of(const Ctx &c, const TriAttr &a, 
 struct Surface {
     V p, n, ng, base, emissive;
     float roughness, metallic, specular;
+    float coat, coat_rough;  // pathtrace.wgsl:1420-1423 (0 without a coat)
     bool front, card;
 };
 
@@ -284,6 +285,9 @@ __device__ Surface surface(const Ctx &c, V o, V d, const HitRec &h) {
     s.specular = params.z;
     s.ng = ng;
     s.n = n;
+    const float4 layer = m[7];
+    s.coat = clampf(layer.z, 0.0f, 1.0f);
+    s.coat_rough = clampf(layer.w, 0.02f, 1.0f);
     return s;
 }
 
@@ -291,6 +295,7 @@ __device__ Surface surface(const Ctx &c, V o, V d, const HitRec &h) {
 struct Lobes {
     V diffuse, f0, f90, w, e_spec, kd;
     float a2, p_spec, p_coat, p_trans;
+    float coat, coat_alpha, e_coat;
 };
 
 __device__ __forceinline__ void energy_node(const Ctx &c, uint32_t i, float &e0, float &e1) {
@@ -337,7 +342,18 @@ __device__ Lobes lobes(const Ctx &c, const Surface &s, float nov) {
     const float wd = dot(vmul(l.diffuse, l.kd), lw);
     l.p_spec = clampf(ws / fmaxf(ws + wd, 1e-6f), 0.05f, 1.0f);
     if (wd <= 0.0f) l.p_spec = 1.0f;
-    const float e_coat = 0.0f, wt = 0.0f, under = 1.0f - e_coat;
+    // pathtrace.wgsl:1675-1683: the coat's directional albedo, a single-scattering lobe of F0 0.04, F90 1
+    l.coat = 0.0f;
+    l.coat_alpha = 1.0f;
+    l.e_coat = 0.0f;
+    if (s.coat > 0.0f) {
+        float ec[2];
+        l.coat = s.coat;
+        energy_lookup(c, s.coat_rough, nov, ec);
+        l.e_coat = s.coat * (0.04f * ec[0] + ec[1]);
+        l.coat_alpha = s.coat_rough * s.coat_rough;
+    }
+    const float e_coat = l.e_coat, wt = 0.0f, under = 1.0f - e_coat;
     const float total = (ws + wd) * under + e_coat + wt;
     l.p_coat = e_coat / fmaxf(total, 1e-6f);
     l.p_trans = wt / fmaxf(total, 1e-6f);
@@ -363,8 +379,15 @@ __device__ V eval_bsdf_k(const Surface &s, const Lobes &lb, V v, V l, float k) {
     const float noh = fmaxf(dot(s.n, h), 0.0f);
     const float voh = fmaxf(dot(v, h), 0.0f);
     const float dv = d_ggx(lb.a2, noh) * vis_smith(lb.a2, nov, nol);
-    const V spec = vs(vmul(vs(f_schlick(lb.f0, lb.f90, voh), dv), lb.w), k);
-    const V diff = vmul(vs(lb.diffuse, kInvPi), lb.kd);
+    V spec = vs(vmul(vs(f_schlick(lb.f0, lb.f90, voh), dv), lb.w), k);
+    V diff = vmul(vs(lb.diffuse, kInvPi), lb.kd);
+    if (lb.coat > 0.0f) {  // pathtrace.wgsl:1750-1755
+        const float ca2 = lb.coat_alpha * lb.coat_alpha;
+        const float cf = f_schlick(v3(0.04f, 0.04f, 0.04f), v3(1.0f, 1.0f, 1.0f), voh).x;
+        const float coat = d_ggx(ca2, noh) * vis_smith(ca2, nov, nol) * cf * lb.coat * k;
+        diff = vs(diff, 1.0f - lb.e_coat);
+        spec = vadd(vs(spec, 1.0f - lb.e_coat), v3(coat, coat, coat));
+    }
     return vs(vadd(diff, spec), nol);
 }
 __device__ __forceinline__ V eval_bsdf(const Surface &s, const Lobes &lb, V v, V l) { return eval_bsdf_k(s, lb, v, l, 1.0f); }
@@ -380,7 +403,11 @@ __device__ float pdf_bsdf(const Surface &s, const Lobes &lb, V v, V l) {
     const float dv = g1(lb.a2, nov) * voh * d_ggx(lb.a2, noh) / nov;
     const float ps = dv / (4.0f * voh);
     const float pd = nol / kPi;
-    const float pc = 0.0f;
+    float pc = 0.0f;
+    if (lb.coat > 0.0f) {  // pathtrace.wgsl:1783-1786
+        const float ca2 = lb.coat_alpha * lb.coat_alpha;
+        pc = g1(ca2, nov) * d_ggx(ca2, noh) / (4.0f * nov);
+    }
     return pg + lb.p_coat * pc + (1.0f - lb.p_coat - lb.p_trans) * (lb.p_spec * ps + (1.0f - lb.p_spec) * pd);
 }
 __device__ __forceinline__ float power_heuristic(float a, float b) {
@@ -424,6 +451,10 @@ __device__ V sample_bsdf(Ctx &c, const Surface &s, const Lobes &lb, V v) {
     const M3 tbn = basis(s.n);
     float r[4];
     c.g.r4(c.g.dim(D_BSDF), r);
+    if (r[0] < lb.p_coat) {  // pathtrace.wgsl:1853-1857: the coat's lobe (p_coat is 0 without one)
+        c.sampled_spec = true;
+        return reflect(vneg(v), sample_vndf(v, tbn, lb.coat_alpha, r[2], r[3]));
+    }
     c.sampled_spec = r[1] < lb.p_spec;
     if (c.sampled_spec) {
         const V h = sample_vndf(v, tbn, s.roughness * s.roughness, r[2], r[3]);

@@ -30,9 +30,10 @@ EMITTER_GUIDE = 65536
 OBJECT_STRIDE = 3
 
 
-def material(base=(0.8, 0.8, 0.8), emissive=(0.0, 0.0, 0.0), metallic=0.0, roughness=1.0, specular=0.5) -> np.ndarray:
+def material(base=(0.8, 0.8, 0.8), emissive=(0.0, 0.0, 0.0), metallic=0.0, roughness=1.0, specular=0.5,
+             clearcoat=0.0, clearcoat_roughness=0.0) -> np.ndarray:
     """scene.rs:480-491 GpuMaterial (72 words): base colour, emission, (metallic, roughness, specular, normal scale),
-    no textures, no layers."""
+    no textures; the layer's clearcoat and its roughness (scene.rs:1257, both clamped to [0, 1]), no glass."""
     m = np.zeros(72, np.float32)
     m[0:3] = base
     m[3] = 1.0
@@ -40,7 +41,7 @@ def material(base=(0.8, 0.8, 0.8), emissive=(0.0, 0.0, 0.0), metallic=0.0, rough
     m[8:12] = (metallic, roughness, specular, 1.0)
     m[12:20] = np.array([-1, -1, -1, -1, -1, -1, 0, 0], np.int32).view(np.float32)
     m[20:24] = (1.0, 1.0, 0.0, 0.0)
-    m[28:32] = (0.0, 1.5, 0.0, 0.0)
+    m[28:32] = (0.0, 1.5, min(max(clearcoat, 0.0), 1.0), min(max(clearcoat_roughness, 0.0), 1.0))
     m[64:72] = np.full(8, -1, np.int32).view(np.float32)
     return m
 
