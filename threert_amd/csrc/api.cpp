@@ -68,7 +68,7 @@ struct trt_device {
     uint64_t staging_bytes = 0;
     int cus = 0;
     int blocks_wide = 1, blocks_binary = 1, blocks_render = 1;
-    uint32_t build = TRT_BUILD_AUTO;  // where uploads build the library's own mesh trees
+    std::atomic<uint32_t> build{TRT_BUILD_AUTO};  // where uploads build the library's own mesh trees
     hipDeviceProp_t props{};
     std::mutex mu;
     std::atomic<int> refs{1};
@@ -531,9 +531,9 @@ int trt_scene_upload(trt_device *d, const trt_scene_desc *desc, trt_scene **out)
     const auto *inst = static_cast<const trt::Instance *>(desc->instances);
     const auto *leaves = static_cast<const trt::Leaf *>(desc->leaves);
     std::string why;
-    std::vector<trt::MeshRef> meshes;
-    if (!trt::wide_meshes(nodes, desc->node_count, inst, desc->instance_count, desc->leaf_count, desc->top_first,
-                          desc->unit_count, meshes, &why))
+    trt::WidePlan plan;
+    if (!trt::wide_plan(nodes, desc->node_count, inst, desc->instance_count, desc->leaf_count, desc->top_first,
+                        desc->unit_count, plan, &why))
         return fail(TRT_E_INVALID, "trt_scene_upload: " + why);
     if (desc->energy && !check_shading(desc, &why)) return fail(TRT_E_INVALID, "trt_scene_upload: " + why);
     TRT_HIP(hipSetDevice(d->ordinal));
@@ -556,9 +556,11 @@ int trt_scene_upload(trt_device *d, const trt_scene_desc *desc, trt_scene **out)
     trt::gb::DeviceWide dw;
     trt::MeshWide mw;
     uint64_t mesh_tris = 0;
+    const std::vector<trt::MeshRef> &meshes = plan.meshes;
     for (const trt::MeshRef &m : meshes) mesh_tris += m.n;
-    const bool on_device = !meshes.empty() && (d->build == TRT_BUILD_DEVICE ||
-                                               (d->build == TRT_BUILD_AUTO && mesh_tris >= kDeviceBuildMin));
+    const uint32_t where = d->build.load();
+    const bool on_device = !meshes.empty() && (where == TRT_BUILD_DEVICE ||
+                                               (where == TRT_BUILD_AUTO && mesh_tris >= kDeviceBuildMin));
     hipStream_t bs = nullptr;  // the device build's stream, kept until its wide nodes are copied
     if (!rc && on_device) {
         hipError_t e = hipStreamCreateWithFlags(&bs, hipStreamNonBlocking);
@@ -580,10 +582,11 @@ int trt_scene_upload(trt_device *d, const trt_scene_desc *desc, trt_scene **out)
         bs = nullptr;
     };
     if (!rc && !trt::build_wide(nodes, desc->node_count, leaves, inst, desc->instance_count, desc->leaf_count,
-                                desc->top_first, desc->unit_count, w, &why, &meshes,
+                                desc->top_first, desc->unit_count, w, &why, &plan,
                                 on_device && !dw.complete ? &dw.trees : nullptr, dw.complete ? &mw : nullptr))
         rc = fail(TRT_E_INVALID, "trt_scene_upload: " + why);
-    if (!rc && w.nodes.size() >= (uint64_t{1} << 30)) rc = fail(TRT_E_INVALID, "trt_scene_upload: tree too large");
+    const uint64_t wide_count = w.node_base + w.nodes.size();
+    if (!rc && wide_count >= (uint64_t{1} << 30)) rc = fail(TRT_E_INVALID, "trt_scene_upload: tree too large");
     const double build_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_build).count();
     if (rc) {
@@ -597,13 +600,13 @@ int trt_scene_upload(trt_device *d, const trt_scene_desc *desc, trt_scene **out)
     if (dw.complete) {
         // the meshes' wide nodes and the leaves are on the device already: the top tree's nodes go after them
         const uint64_t mesh_bytes = dw.wnode_count * sizeof(trt::WideNode);
-        const uint64_t bytes = w.nodes.size() * sizeof(trt::WideNode);
+        const uint64_t bytes = wide_count * sizeof(trt::WideNode);
         hipError_t e = hipMalloc(&s->wnodes, bytes ? bytes : 64);
         if (e == hipSuccess && mesh_bytes)
             e = hipMemcpyAsync(s->wnodes, dw.d_wnodes, mesh_bytes, hipMemcpyDeviceToDevice, bs);
         if (e == hipSuccess && bytes > mesh_bytes)
-            e = hipMemcpyAsync(static_cast<char *>(s->wnodes) + mesh_bytes, w.nodes.data() + dw.wnode_count,
-                               bytes - mesh_bytes, hipMemcpyHostToDevice, bs);
+            e = hipMemcpyAsync(static_cast<char *>(s->wnodes) + mesh_bytes, w.nodes.data(), bytes - mesh_bytes,
+                               hipMemcpyHostToDevice, bs);
         end_build_stream();  // (waits for the copies; frees the build's copy of the nodes in stream order)
         s->wleaves = dw.d_wleaves;
         total += (bytes ? bytes : 64) + std::max<uint64_t>(desc->leaf_count * sizeof(trt::Leaf), 64);
@@ -641,15 +644,15 @@ int trt_scene_upload(trt_device *d, const trt_scene_desc *desc, trt_scene **out)
     s->node_count = desc->node_count;
     s->leaf_count = desc->leaf_count;
     s->instance_count = desc->instance_count;
-    s->wnode_count = w.nodes.size();
+    s->wnode_count = wide_count;
     s->wunit_count = w.units.size();
     s->top_first = desc->top_first;
     s->units = desc->unit_count;
     s->top_ref = w.top_ref;
     s->max_stack = w.max_stack;
     std::memcpy(s->top_box, w.top_box, sizeof(s->top_box));
-    s->stats.wide_nodes = w.nodes.size();
-    s->stats.wide_bytes = w.nodes.size() * sizeof(trt::WideNode);
+    s->stats.wide_nodes = wide_count;
+    s->stats.wide_bytes = wide_count * sizeof(trt::WideNode);
     s->stats.binary_bytes = desc->node_count * sizeof(trt::Node);
     s->stats.leaf_bytes = desc->leaf_count * sizeof(trt::Leaf);
     s->stats.unit_bytes = w.units.size() * sizeof(trt::WideUnit);

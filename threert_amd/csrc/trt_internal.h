@@ -106,6 +106,7 @@ struct alignas(64) WideUnit {
 static_assert(sizeof(WideUnit) == 64, "a wide unit is 64 bytes");
 
 struct WideScene {
+    uint64_t node_base = 0;  // wide nodes made elsewhere (the device's mesh trees), ahead of `nodes`
     std::vector<WideNode> nodes;
     std::vector<WideUnit> units;
     // The leaves in the wide trees' own order, each mesh's within its range; e2.w holds the reference leaf index.
@@ -141,22 +142,27 @@ struct MeshRef {
 };
 void own_tree_host(const Leaf *leaves, uint32_t first, uint32_t n, OwnBuilt &out);
 
-// The wide tree of an uploaded scene (wide.cpp). `meshes` lists the distinct mesh trees the top tree's records
-// name, in record order, after checking the trees; `pre` (optional) holds own trees already built for them (an
-// entry with no nodes is built on the host).
-bool wide_meshes(const Node *nodes, uint64_t node_count, const Instance *inst, uint64_t inst_count,
-                 uint64_t leaf_count, uint32_t top_first, uint32_t unit_count, std::vector<MeshRef> &meshes,
-                 std::string *why);
+// The wide tree of an uploaded scene (wide.cpp). The plan: the top tree's records in walk order, the distinct mesh
+// trees they name (in record order) and each record's mesh, after checking the trees (units.rs check_top /
+// check_mesh on the buffers). `pre` (optional) holds own trees already built for the plan's meshes (an entry with no
+// nodes is built on the host).
+struct WidePlan {
+    std::vector<uint32_t> records;
+    std::vector<MeshRef> meshes;
+    std::vector<uint32_t> mesh_of;
+};
+bool wide_plan(const Node *nodes, uint64_t node_count, const Instance *inst, uint64_t inst_count,
+               uint64_t leaf_count, uint32_t top_first, uint32_t unit_count, WidePlan &plan, std::string *why);
 // `mesh_wide` (optional): the meshes' wide nodes were made elsewhere (the device), mesh_wide->count of them ahead
-// of the top tree's; out.nodes then holds placeholders for them, out.wleaves is left empty, and each mesh's root
-// reference and stack bound come from mesh_wide.
+// of the top tree's; out.node_base is then that count and out.nodes holds the top tree's only, out.wleaves is left
+// empty, and each mesh's root reference and stack bound come from mesh_wide.
 struct MeshWide {
     uint64_t count = 0;
     std::vector<uint32_t> ref, bound;
 };
 bool build_wide(const Node *nodes, uint64_t node_count, const Leaf *leaves, const Instance *inst, uint64_t inst_count,
                 uint64_t leaf_count, uint32_t top_first, uint32_t unit_count, WideScene &out, std::string *why,
-                const std::vector<MeshRef> *meshes = nullptr, const std::vector<OwnBuilt> *pre = nullptr,
+                const WidePlan *plan = nullptr, const std::vector<OwnBuilt> *pre = nullptr,
                 const MeshWide *mesh_wide = nullptr);
 
 }  // namespace trt

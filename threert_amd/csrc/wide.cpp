@@ -54,7 +54,7 @@ struct Collapser {
         }
         uint32_t kid[4];
         const int nc = gb::open_kids(nodes, off, b, kid);
-        const uint32_t wi = static_cast<uint32_t>(out.nodes.size());
+        const uint32_t wi = static_cast<uint32_t>(out.node_base + out.nodes.size());
         out.nodes.emplace_back();
         WideNode w{};
         const Node *kp[4];
@@ -67,7 +67,7 @@ struct Collapser {
         }
         for (int c = nc; c < 4; c++) w.child[c] = kNone;
         if (!quantise(w, kp, nc)) err = "a node's box cannot be quantised";
-        out.nodes[wi] = w;
+        out.nodes[wi - out.node_base] = w;
         bound = deepest + static_cast<uint32_t>(nc - 1);
         return ref_node(wi);
     }
@@ -313,14 +313,8 @@ void own_tree_host(const Leaf *leaves, uint32_t first, uint32_t n, OwnBuilt &out
 
 namespace {
 
-struct Plan {
-    std::vector<uint32_t> records;  // the top tree's records, in walk order
-    std::vector<MeshRef> meshes;    // distinct (root, first) pairs in record order
-    std::vector<uint32_t> mesh_of;  // by record: its index in `meshes`
-};
-
 bool plan_of(const Node *nodes, uint64_t node_count, const Instance *inst, uint64_t inst_count, uint64_t leaf_count,
-             uint32_t top_first, Plan &p, std::string *why) {
+             uint32_t top_first, WidePlan &p, std::string *why) {
     if (top_first >= node_count) {
         *why = "the top tree's root is outside the node buffer";
         return false;
@@ -369,20 +363,16 @@ bool plan_of(const Node *nodes, uint64_t node_count, const Instance *inst, uint6
 
 }  // namespace
 
-bool wide_meshes(const Node *nodes, uint64_t node_count, const Instance *inst, uint64_t inst_count,
-                 uint64_t leaf_count, uint32_t top_first, uint32_t unit_count, std::vector<MeshRef> &meshes,
-                 std::string *why) {
-    meshes.clear();
+bool wide_plan(const Node *nodes, uint64_t node_count, const Instance *inst, uint64_t inst_count,
+               uint64_t leaf_count, uint32_t top_first, uint32_t unit_count, WidePlan &plan, std::string *why) {
+    plan = WidePlan{};
     if (unit_count == 0) return true;
-    Plan p;
-    if (!plan_of(nodes, node_count, inst, inst_count, leaf_count, top_first, p, why)) return false;
-    meshes = std::move(p.meshes);
-    return true;
+    return plan_of(nodes, node_count, inst, inst_count, leaf_count, top_first, plan, why);
 }
 
 bool build_wide(const Node *nodes, uint64_t node_count, const Leaf *leaves, const Instance *inst, uint64_t inst_count,
                 uint64_t leaf_count, uint32_t top_first, uint32_t unit_count, WideScene &out, std::string *why,
-                const std::vector<MeshRef> *meshes, const std::vector<OwnBuilt> *pre, const MeshWide *mesh_wide) {
+                const WidePlan *plan, const std::vector<OwnBuilt> *pre, const MeshWide *mesh_wide) {
     out = WideScene();
     out.units.assign(inst_count, WideUnit{});
     if (!mesh_wide) {
@@ -390,31 +380,25 @@ bool build_wide(const Node *nodes, uint64_t node_count, const Leaf *leaves, cons
         for (uint64_t i = 0; i < leaf_count; i++) out.wleaves[i].r[2][3] = from_bits(static_cast<uint32_t>(i));
     }
     if (unit_count == 0) return true;
-    Plan p;
-    if (!plan_of(nodes, node_count, inst, inst_count, leaf_count, top_first, p, why)) return false;
-    if ((pre || mesh_wide) && (!meshes || meshes->size() != p.meshes.size() ||
-                               (pre && pre->size() != p.meshes.size()) ||
-                               (mesh_wide && (mesh_wide->ref.size() != p.meshes.size() ||
-                                              mesh_wide->bound.size() != p.meshes.size())))) {
+    WidePlan own_plan;
+    if (!plan) {
+        if (!plan_of(nodes, node_count, inst, inst_count, leaf_count, top_first, own_plan, why)) return false;
+        plan = &own_plan;
+    }
+    const WidePlan &p = *plan;
+    if ((pre && pre->size() != p.meshes.size()) ||
+        (mesh_wide && (mesh_wide->ref.size() != p.meshes.size() || mesh_wide->bound.size() != p.meshes.size()))) {
         *why = "prebuilt mesh trees do not match the scene's meshes";
         return false;
     }
     Collapser col{nodes, node_count, out};
     std::vector<std::pair<uint32_t, uint32_t>> mesh(p.meshes.size());  // wide ref, stack bound
-    if (mesh_wide) out.nodes.resize(mesh_wide->count);  // (the device's: uploaded from there)
+    if (mesh_wide) out.node_base = mesh_wide->count;  // (the device's, ahead of the top tree's)
     for (size_t i = 0; i < p.meshes.size(); i++) {
         const MeshRef m = p.meshes[i];
-        if (meshes && ((*meshes)[i].root != m.root || (*meshes)[i].first != m.first || (*meshes)[i].n != m.n)) {
-            *why = "prebuilt mesh trees do not match the scene's meshes";
-            return false;
-        }
         if (mesh_wide) {
             mesh[i] = {mesh_wide->ref[i], mesh_wide->bound[i]};
             continue;
-        }
-        if (pre && ((*meshes)[i].root != m.root || (*meshes)[i].first != m.first || (*meshes)[i].n != m.n)) {
-            *why = "prebuilt mesh trees do not match the scene's meshes";
-            return false;
         }
         OwnBuilt host;
         const OwnBuilt *own = pre && !(*pre)[i].nodes.empty() ? &(*pre)[i] : nullptr;
