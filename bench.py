@@ -158,9 +158,12 @@ def bench_render(dev, frames: int) -> dict:
             "unique_triangles": int(len(rs.prep.leaves))}
 
 
-def bench_cornell(dev) -> dict:
+def bench_cornell(dev, check: bool = False) -> dict:
     """Samples/s on the seeded Cornell box (36 triangles): 512 x 512, 16 spp, 4 bounces, RR from depth 3, clamp
-    off, seed 1, one launch for the 16 samples, timed with HIP events."""
+    off, seed 1, one launch for the 16 samples, timed with HIP events. The image's error against a 1024-sample
+    FP64-accumulated render (relative RMSE, sqrt(mean((x - ref)^2 / (ref^2 + 0.01)))); with `check`, its parity with
+    the oracle's render of the same seed and samples (per-pixel relative RMSE, and the share of pixels whose 20
+    accumulator words are the oracle's bits)."""
     from threert_amd import render as R
 
     rs, cam = R.cornell()
@@ -177,12 +180,33 @@ def bench_cornell(dev) -> dict:
     for _ in range(reps):
         R.render(sc, p, 0, spp, acc)
     ms = dev.timer_stop() / reps
+    acc.clear()
+    R.render(sc, p, 0, spp, acc)
+    raw = acc.read()
+    img = raw[..., 0:3] / np.maximum(raw[..., 3:4], 1.0)
+    ref = R.reference_fp64(dev, sc, p, w, h, 1024)
+    err = float(np.sqrt(np.mean((img - ref) ** 2 / (ref ** 2 + 1e-2))))
     sc.close()
     acc.close()
     sps = w * h * spp / (ms * 1e-3)
-    log(f"bench: cornell 512x512x16 {ms:.2f} ms, {sps / 1e6:.1f} Msamples/s")
-    return {"metric": "samples/s", "value": round(sps, 1), "unit": "samples/s", "ms_per_image": round(ms, 3),
-            "workload": "seeded Cornell box, 36 triangles, 512x512, 16 spp, 4 bounces, RR from depth 3, seed 1"}
+    log(f"bench: cornell 512x512x16 {ms:.2f} ms, {sps / 1e6:.1f} Msamples/s, rel RMSE vs FP64 1024 spp {err:.4f}")
+    out = {"metric": "samples/s", "value": round(sps, 1), "unit": "samples/s", "ms_per_image": round(ms, 3),
+           "rel_rmse_vs_fp64_1024spp": round(err, 5),
+           "workload": "seeded Cornell box, 36 triangles, 512x512, 16 spp, 4 bounces, RR from depth 3, seed 1"}
+    if check:
+        import oracle
+
+        t0 = time.perf_counter()
+        pr = rs.prep
+        want = oracle.RenderScene(pr.nodes, pr.leaves, pr.instances, pr.top_first, pr.units,
+                                  **rs.tables()).render(p, 0, spp)
+        cw = want[..., 0:3] / np.maximum(want[..., 3:4], 1.0)
+        rel = float(np.sqrt(np.mean(((img - cw) / np.maximum(np.abs(cw), 1e-3)) ** 2)))
+        same = float((raw.view(np.uint32) == want.view(np.uint32)).all(axis=2).mean())
+        out["oracle_parity"] = {"rel_rmse": rel, "pixels_bit_equal": round(same, 6),
+                                "oracle_s": round(time.perf_counter() - t0, 2)}
+        log(f"bench: cornell against the oracle {out['oracle_parity']}")
+    return out
 
 
 def bench_precision_render(dev) -> dict:
@@ -359,6 +383,12 @@ def main():
     dev.sync()
     wall = time.perf_counter() - w0
     ms_step = ms / args.steps
+    each = []  # the same launches one at a time, for the median of at least 5 timed runs
+    for _ in range(max(5, min(args.steps, 20))):
+        dev.timer_start()
+        sc.trace_device(d_rays, N_RAYS, d_hits, T.WALK_WIDE)
+        each.append(dev.timer_stop())
+    ms_median = float(np.median(each))
     mrays = N_RAYS / (ms_step * 1e-3) / 1e6
     hits = d_hits.download(T.HIT, N_RAYS)
     hit_rate = float((hits["tri"] != T.NO_HIT).mean())
@@ -392,6 +422,8 @@ def main():
             "reads_per_ray": {"nodes": round(reads["nodes"] / N_RAYS, 2), "tris": round(reads["tris"] / N_RAYS, 2),
                               "bytes": round(reads["bytes"] / N_RAYS, 1)},
             "wall_s_timed": round(wall, 3),
+            "ms_median_single_launch": round(ms_median, 4),
+            "mrays_s_median": round(N_RAYS / (ms_median * 1e-3) / 1e6, 2),
             "fp64_check": fp64,
             "binary_walk": binary,
             "tree_build": build,
@@ -406,7 +438,7 @@ def main():
     out["config"]["copy_1gib"] = bench_copy(dev)
     if not args.no_render:
         out["render_720p"] = bench_render(dev, args.render_frames)
-        out["render_cornell_512"] = bench_cornell(dev)
+        out["render_cornell_512"] = bench_cornell(dev, check=not args.no_cpu)
         out["render_1080p_precision"] = bench_precision_render(dev)
     if not args.no_cpu:
         log("bench: CPU baseline")

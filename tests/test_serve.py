@@ -59,13 +59,18 @@ def _bits(a):
 def test_a_job_renders_the_same_alone_and_in_a_batch(device):
     from threert_amd import render as R
 
-    print("batch of 3 jobs over 2 scenes vs each alone", flush=True)
+    print("batch of 4 jobs over 3 scenes (one with clearcoats) vs each alone", flush=True)
     rs1, cam1 = R.cornell()
     rs2, cam2, sky2 = serve.soup_scene(4000, 5)
-    s1, s2 = R.upload(device, rs1), R.upload(device, rs2)
+    rs3, _ = R.cornell()
+    rs3.materials = rs3.materials.copy()
+    rs3.materials[:, 30], rs3.materials[:, 31] = 0.7, 0.2
+    s1, s2, s3 = R.upload(device, rs1), R.upload(device, rs2), R.upload(device, rs3)
+    # a scene with a coat puts the whole launch on the kernel with the coat's terms: the others keep their bits
     jobs = [(s1, rs1.params(40, 24, cam1, seed=3, max_bounces=4, rr_depth=3), 0, 3),
             (s2, rs2.params(33, 17, cam2, seed=9, max_bounces=2, rr_depth=3, sky=sky2), 0, 2),
-            (s1, rs1.params(16, 16, cam1, seed=5, max_bounces=3, rr_depth=2), 4, 5)]
+            (s1, rs1.params(16, 16, cam1, seed=5, max_bounces=3, rr_depth=2), 4, 5),
+            (s3, rs3.params(24, 24, cam1, seed=2, max_bounces=3, rr_depth=2), 0, 2)]
     alone = []
     for sc, p, first, spp in jobs:
         acc = R.Accum(device, int(p[16]), int(p[17]))
@@ -82,6 +87,7 @@ def test_a_job_renders_the_same_alone_and_in_a_batch(device):
         a.close()
     s1.close()
     s2.close()
+    s3.close()
 
 
 @pytest.mark.gpu

@@ -85,6 +85,7 @@ struct trt_scene {
     void *vertices = nullptr, *indices = nullptr, *rows = nullptr, *materials = nullptr, *lights = nullptr,
          *emitters = nullptr, *emitter_tris = nullptr, *objects = nullptr, *energy = nullptr;
     bool shading = false;
+    bool coat = false;  // a material has a clearcoat (layer.z > 0: the reference's F_COAT, gpu.rs features_of)
     uint64_t light_count = 0, emitter_entries = 0, emitter_tri_count = 0;
     uint64_t vertex_count = 0, tri_count = 0, row_count = 0, material_count = 0, object_count = 0, emitter_vec4s = 0;
     uint64_t node_count = 0, leaf_count = 0, instance_count = 0, wnode_count = 0, wunit_count = 0;
@@ -620,6 +621,8 @@ int trt_scene_upload(trt_device *d, const trt_scene_desc *desc, trt_scene **out)
     }
     if (!rc && desc->energy) {
         s->shading = true;
+        for (uint64_t k = 0; k < desc->material_count; k++)
+            if (static_cast<const float *>(desc->materials)[72 * k + 30] > 0.0f) s->coat = true;
         s->light_count = desc->light_count;
         s->emitter_tri_count = desc->emitter_tri_count;
         s->emitter_entries = desc->emitter_vec4s ? desc->emitter_vec4s - (65537 / 4 + 1) - 1 : 0;
@@ -945,6 +948,7 @@ int trt_render_batch(trt_device *d, const trt_render_job *jobs, uint32_t n) {
     table.reserve(n);
     uint64_t tiles = 0;
     uint32_t deepest = 0;
+    bool coat = false;
     for (uint32_t k = 0; k < n; k++) {
         if (jobs[k].spp == 0) continue;
         if (!jobs[k].scene || jobs[k].scene->dev != d)
@@ -959,6 +963,7 @@ int trt_render_batch(trt_device *d, const trt_render_job *jobs, uint32_t n) {
         j.tile_first = static_cast<uint32_t>(tiles);
         tiles += uint64_t((jobs[k].acc->w + 7) / 8) * ((jobs[k].acc->h + 7) / 8);
         deepest = std::max(deepest, jobs[k].scene->max_stack);
+        coat = coat || jobs[k].scene->coat;
         table.push_back(j);
     }
     if (tiles >= (uint64_t{1} << 31)) return fail(TRT_E_INVALID, "trt_render_batch: too many tiles in one launch");
@@ -999,7 +1004,7 @@ int trt_render_batch(trt_device *d, const trt_render_job *jobs, uint32_t n) {
     TRT_HIP(hipMemcpyAsync(ring.dev[k], ring.host[k], bytes, hipMemcpyHostToDevice, d->stream));
     if (trt::launch_render(static_cast<const trt::RenderJob *>(ring.dev[k]), static_cast<uint32_t>(table.size()),
                            static_cast<uint32_t>(tiles), d->counter, d->spill, threads,
-                           static_cast<uint32_t>(entries), d->guard, d->reads + kRenderRays, grid, d->stream))
+                           static_cast<uint32_t>(entries), d->guard, d->reads + kRenderRays, coat, grid, d->stream))
         return fail(TRT_E_DEVICE, "render kernel launch failed");
     TRT_HIP(hipEventRecord(ring.ev[k], d->stream));
     return ok();

@@ -252,6 +252,9 @@ struct Surface {
     bool front, card;
 };
 
+// kCoat: the launch's scenes hold a clearcoat (the reference's F_COAT pipeline flag); without it the coat terms are
+// compiled out, as the reference's shader is specialised.
+template <bool kCoat>
 __device__ Surface surface(const Ctx &c, V o, V d, const HitRec &h) {
     V e1, e2;
     TriAttr a = tri_attr(c, h.tri, h.inst, h.part, e1, e2);
@@ -285,9 +288,13 @@ __device__ Surface surface(const Ctx &c, V o, V d, const HitRec &h) {
     s.specular = params.z;
     s.ng = ng;
     s.n = n;
-    const float4 layer = m[7];
-    s.coat = clampf(layer.z, 0.0f, 1.0f);
-    s.coat_rough = clampf(layer.w, 0.02f, 1.0f);
+    s.coat = 0.0f;
+    s.coat_rough = 1.0f;
+    if (kCoat) {
+        const float4 layer = m[7];
+        s.coat = clampf(layer.z, 0.0f, 1.0f);
+        s.coat_rough = clampf(layer.w, 0.02f, 1.0f);
+    }
     return s;
 }
 
@@ -739,6 +746,7 @@ __device__ __forceinline__ V clamp_lum(V col, float clamp_v) {
 }
 
 // One path of sample `index` at pixel (px, py): its radiance, the camera hit's albedo and normal.
+template <bool kCoat>
 __device__ void path(Ctx &c, uint32_t px, uint32_t py, uint32_t index, V &radiance, V &albedo, V &normal) {
     const uint32_t w = c.pu(SIZE, 0), h = c.pu(SIZE, 1);
     c.g.seed = hash_hp(px ^ hash_hp(py ^ hash_hp(c.pu(SIZE, 3))));
@@ -773,7 +781,7 @@ __device__ void path(Ctx &c, uint32_t px, uint32_t py, uint32_t index, V &radian
             radiance = vadd(radiance, col);
             break;
         }
-        const Surface s = surface(c, o, d, hit);
+        const Surface s = surface<kCoat>(c, o, d, hit);
         if (depth == 0u) {
             if (s.card) {
                 const float mx = fmaxf(fmaxf(s.emissive.x, fmaxf(s.emissive.y, s.emissive.z)), 1e-6f);
@@ -826,6 +834,7 @@ __device__ void path(Ctx &c, uint32_t px, uint32_t py, uint32_t index, V &radian
 // The path loop needs about 200 registers, which leaves two waves a SIMD; held to three (168 registers, about 50
 // spilled to scratch, mostly in the shading code), the kernel hides more of its memory latency: the 720p city
 // frame 2.46 -> 1.95 ms, the 3M-triangle scene 1.36x faster, the Cornell box 5 % (four waves: 2.00 ms, 1.34x, 11 %).
+template <bool kCoat>
 __global__ __launch_bounds__(kRenderBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_render(const RenderJob *jobs, uint32_t njobs, uint32_t total,
                                                          uint32_t *counter, uint32_t *spill, uint32_t spill_stride,
                                                          uint32_t spill_entries, uint32_t *guard,
@@ -860,7 +869,7 @@ __global__ __launch_bounds__(kRenderBlock) __attribute__((amdgpu_waves_per_eu(3)
             float4 col = pix[0], alb = pix[1], nrm = pix[2];
             for (uint32_t s = 0; s < a->spp; s++) {
                 V rad, al, nm;
-                path(c, px, py, a->first_sample + s, rad, al, nm);
+                path<kCoat>(c, px, py, a->first_sample + s, rad, al, nm);
                 const float lum = luminance(rad);
                 col.x += rad.x, col.y += rad.y, col.z += rad.z, col.w += 1.0f;
                 alb.x += al.x, alb.y += al.y, alb.z += al.z, alb.w += lum;
@@ -889,17 +898,21 @@ __global__ void k_reset(uint32_t *c) { *c = 0; }
 
 int render_occupancy(int *blocks_per_cu) {
     int b = 0;
-    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_render, kRenderBlock, 0);
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_render<false>, kRenderBlock, 0);
     *blocks_per_cu = b;
     return e == hipSuccess ? 0 : -1;
 }
 
 int launch_render(const RenderJob *d_jobs, uint32_t njobs, uint32_t total_tiles, uint32_t *counter,
                   uint32_t *spill, uint32_t spill_stride, uint32_t spill_entries, uint32_t *guard,
-                  unsigned long long *rays, int grid, hipStream_t s) {
+                  unsigned long long *rays, bool coat, int grid, hipStream_t s) {
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, s, counter);
-    hipLaunchKernelGGL(k_render, dim3(grid), dim3(kRenderBlock), 0, s, d_jobs, njobs, total_tiles, counter, spill,
-                       spill_stride, spill_entries, guard, rays);
+    if (coat)
+        hipLaunchKernelGGL(k_render<true>, dim3(grid), dim3(kRenderBlock), 0, s, d_jobs, njobs, total_tiles, counter,
+                           spill, spill_stride, spill_entries, guard, rays);
+    else
+        hipLaunchKernelGGL(k_render<false>, dim3(grid), dim3(kRenderBlock), 0, s, d_jobs, njobs, total_tiles, counter,
+                           spill, spill_stride, spill_entries, guard, rays);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
