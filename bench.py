@@ -294,6 +294,63 @@ def bench_serve(dev, n_jobs: int, rate: float) -> dict:
     return s
 
 
+def bench_serve_wire(dev, n_jobs: int, rate: float) -> dict:
+    """The serving workload through the `serve` wire format (threert_amd/wire.py): a client thread writes each job as
+    a JSON line at its arrival time into the server's stdin (a pipe), and times each reply line from the moment it
+    sent the job. Latency here is the client's: parsing, the reply and the pipe included."""
+    import os
+    import threading
+
+    from threert_amd import serve, wire
+
+    jobs = serve.workload(n_jobs=n_jobs, rate=rate, seed=1)
+    r_fd, w_fd = os.pipe()
+    inp, feed = os.fdopen(r_fd, "r"), os.fdopen(w_fd, "w")
+    sent, got = {}, []
+
+    class Replies:
+        def write(self, text):
+            t = time.perf_counter()
+            got.extend((t, line) for line in text.splitlines() if line)
+
+        def flush(self):
+            pass
+
+    ls = wire.LineServer(dev, inp, Replies(), budget_bytes=200 << 30)
+    t0 = time.perf_counter()
+
+    def client():
+        for j in jobs:
+            wait = t0 + j.arrival - time.perf_counter()
+            if wait > 0:
+                time.sleep(wait)
+            line = json.dumps({"id": j.id, "scene": {"kind": "soup", "triangles": j.scene[0], "seed": j.scene[1]},
+                               "width": j.width, "height": j.height, "spp": j.spp, "bounces": j.bounces,
+                               "seed": j.seed})
+            sent[j.id] = time.perf_counter()
+            feed.write(line + "\n")
+            feed.flush()
+        feed.close()
+
+    th = threading.Thread(target=client)
+    th.start()
+    ls.run(0.0, tick=lambda a, w: log(f"bench: wire {ls.replies - 1} replies, {a} rendering, {w} waiting"))
+    th.join()
+    wall = time.perf_counter() - t0
+    ls.close()
+    inp.close()
+    replies = [(t, json.loads(line)) for t, line in got]
+    done = [(t, r) for t, r in replies if r.get("ok")]
+    lat = [t - sent[r["id"]] for t, r in done]
+    samples = sum(r["stats"]["width"] * r["stats"]["height"] * r["stats"]["spp"] for _, r in done)
+    s = {"jobs": len(done), "failed": len([1 for _, r in replies if r.get("ok") is False]),
+         "p50_s": round(serve.percentile(lat, 50), 3), "p99_s": round(serve.percentile(lat, 99), 3),
+         "samples": int(samples), "wall_s": round(wall, 3), "samples_per_s": round(samples / wall, 1),
+         "ready_line": replies[0][1].get("ready", False) if replies else False}
+    log(f"bench: serving over the wire {s}")
+    return s
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -307,6 +364,8 @@ def main():
     ap.add_argument("--build-only", action="store_true",
                     help="measure only the upload's wide-tree build of the 3M-triangle scene (device and host)")
     ap.add_argument("--serve", action="store_true", help="measure the serving workload instead (p50/p99 latency)")
+    ap.add_argument("--serve-wire", action="store_true",
+                    help="the serving workload through the JSON-lines wire format, latency timed by the client")
     ap.add_argument("--serve-jobs", type=int, default=512)
     ap.add_argument("--serve-rate", type=float, default=40.0)
     ap.add_argument("--serve-equal", action="store_true",
@@ -333,6 +392,16 @@ def main():
                                                  "at most its plain error, denoised on the device, plain tracing "
                                                  "otherwise"},
                           "equal_error": r}), flush=True)
+        return
+    if args.serve_wire:
+        dev = T.Device(0)
+        s = bench_serve_wire(dev, args.serve_jobs, args.serve_rate)
+        print(json.dumps({"metric": "p99 job latency (serving over the wire format)", "value": s["p99_s"],
+                          "unit": "s", "n_gpus": 1, "higher_is_better": False, "dtype": "f32", "data": "synthetic",
+                          "config": {"workload": f"{args.serve_jobs} render jobs as JSON lines, Poisson arrivals at "
+                                                 f"{args.serve_rate}/s, seeded scenes of 10k-4M triangles, "
+                                                 "256x256-1920x1080, 1-256 spp, 2-6 bounces, 200 GB budget"},
+                          "serving": s}), flush=True)
         return
     if args.serve:
         dev = T.Device(0)

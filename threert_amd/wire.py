@@ -230,15 +230,16 @@ class LineServer:
                                 "total_ms": round((r.finished - r.arrival) * 1000.0 + write_ms, 1)}}
         self.reply({"id": jid, "ok": True, "stats": stats})
 
-    def run(self, init_ms: float = 0.0) -> None:
-        """The ready line, then jobs until the input ends and every job has its reply."""
+    def run(self, init_ms: float = 0.0, tick=None, every: float = 5.0) -> None:
+        """The ready line, then jobs until the input ends and every job has its reply; `tick(active, waiting)` at
+        most every `every` seconds."""
         info = self.dev.info()
         self.adapter = f"{info['name']} (gfx{info['gcn_arch']}, {info['compute_units']} CUs)"
         self.reply({"ready": True, "adapter": self.adapter, "init_ms": round(init_ms, 1)})
         reader = threading.Thread(target=self._read, daemon=True)
         reader.start()
         try:
-            self.srv.serve(self._arrivals, self._closed, self._finished, self._idle)
+            self.srv.serve(self._arrivals, self._closed, self._finished, self._idle, tick, every)
         finally:
             reader.join(timeout=1.0)
 
