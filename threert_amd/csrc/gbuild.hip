@@ -733,8 +733,6 @@ bool collapse(const Leaf *d_leaves, uint64_t leaf_count, const std::vector<MeshR
     res.times.device_ms = ms;
     res.times.levels = uint32_t(tree_levels + levels.size());
     res.times.nodes = own_nodes;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     res.complete = true;
     for (auto &o : res.trees) o = OwnBuilt{};
     return true;
@@ -756,9 +754,16 @@ bool device_wide(const Leaf *d_leaves, uint64_t leaf_count, const std::vector<Me
         return false;
     }
     const uint32_t n = uint32_t(n64), mc = uint32_t(meshes.size());
-    hipEvent_t e0, e1;
-    GB_HIP(hipEventCreate(&e0));
-    GB_HIP(hipEventCreate(&e1));
+    struct Events {  // destroyed on every way out
+        hipEvent_t a = nullptr, b = nullptr;
+        ~Events() {
+            if (a) (void)hipEventDestroy(a);
+            if (b) (void)hipEventDestroy(b);
+        }
+    } ev;
+    GB_HIP(hipEventCreate(&ev.a));
+    GB_HIP(hipEventCreate(&ev.b));
+    hipEvent_t e0 = ev.a, e1 = ev.b;
     GB_HIP(hipEventRecord(e0, st));
     Scratch sc(st);
     std::vector<MeshDev> hm(mc);
@@ -918,8 +923,6 @@ bool device_wide(const Leaf *d_leaves, uint64_t leaf_count, const std::vector<Me
         times->levels = uint32_t(levels.size());
         times->nodes = out_nodes;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     for (size_t k = 0; k < roots.size(); k++) {
         const uint32_t i = roots[k];
         if (failed[i]) continue;  // built on the host
