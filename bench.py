@@ -351,6 +351,35 @@ def bench_serve_wire(dev, n_jobs: int, rate: float) -> dict:
     return s
 
 
+def bench_denoise(dev, w: int = 1280, h: int = 720, reps: int = 20) -> dict:
+    """The learned denoiser's inference on one 1280 x 720 frame (the shipped weights; color, albedo and normal
+    resident in HBM; HIP events on the library's stream over `reps` frames). Flops: the convolutions'
+    sum over layers of 2 x 9 x cin x cout x pixels at the layer's level, against the dense BF16 MFMA peak."""
+    from threert_amd import unet as U
+
+    net = U.UNet(dev, w * h, U.shipped_params())
+    rng = np.random.default_rng(1)
+    img = {k: rng.random((h, w, 3)).astype(np.float32) for k in ("c", "a", "n")}
+    bufs = {k: dev.alloc(w * h * 12) for k in ("c", "a", "n", "o")}
+    for k in ("c", "a", "n"):
+        bufs[k].upload(img[k])
+    for _ in range(3):
+        net.denoise_device(1, h, w, bufs["c"], bufs["a"], bufs["n"], bufs["o"])
+    dev.sync()
+    dev.timer_start()
+    for _ in range(reps):
+        net.denoise_device(1, h, w, bufs["c"], bufs["a"], bufs["n"], bufs["o"])
+    ms = dev.timer_stop() / reps
+    for b in bufs.values():
+        b.free()
+    net.close()
+    flops = sum(2 * 9 * (c0 + c1) * co * (w >> lv) * (h >> lv) for c0, c1, co, lv in U.ARCH)
+    tf = flops / (ms * 1e-3) / 1e12
+    log(f"bench: denoise {w}x{h} {ms:.3f} ms, {flops / 1e9:.1f} GFLOP, {tf:.1f} TFLOP/s")
+    return {"ms_per_frame": round(ms, 4), "gflop_per_frame": round(flops / 1e9, 2), "tflops": round(tf, 1),
+            "frac_of_bf16_peak": round(tf / 2500.0, 4)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -363,6 +392,8 @@ def main():
     ap.add_argument("--render-frames", type=int, default=10)
     ap.add_argument("--build-only", action="store_true",
                     help="measure only the upload's wide-tree build of the 3M-triangle scene (device and host)")
+    ap.add_argument("--denoise-only", action="store_true",
+                    help="measure only the learned denoiser's inference on a 1280x720 frame")
     ap.add_argument("--serve", action="store_true", help="measure the serving workload instead (p50/p99 latency)")
     ap.add_argument("--serve-wire", action="store_true",
                     help="the serving workload through the JSON-lines wire format, latency timed by the client")
@@ -392,6 +423,16 @@ def main():
                                                  "at most its plain error, denoised on the device, plain tracing "
                                                  "otherwise"},
                           "equal_error": r}), flush=True)
+        return
+    if args.denoise_only:
+        dev = T.Device(0)
+        d = bench_denoise(dev)
+        print(json.dumps({"metric": "denoiser inference, 1280x720", "value": d["ms_per_frame"], "unit": "ms",
+                          "n_gpus": 1, "higher_is_better": False, "dtype": "bf16", "data": "synthetic",
+                          "config": {"workload": "19-layer U-Net, 3x3 convolutions on BF16 MFMA, one 1280x720 frame"},
+                          "roofline": {"bound": "mfma", "achieved": d["tflops"], "peak": 2500.0, "unit": "TFLOP/s",
+                                       "frac": d["frac_of_bf16_peak"], "traffic": None},
+                          "denoise": d}), flush=True)
         return
     if args.serve_wire:
         dev = T.Device(0)
