@@ -113,3 +113,39 @@ def test_jobs_over_the_wire_are_their_lone_renders(device, tmp_path):
     from threert_amd import output as O
 
     assert (O.read_rgb8(jobs[0]["out"]) == O.srgb8(np.fromfile(jobs[0]["hdr"], "<f4").reshape(40, 48, 3))).all()
+
+
+@pytest.mark.gpu
+def test_the_command_line_server_answers_on_stdout(tmp_path):
+    """`python -m threert_amd.wire` as a client starts it: the ready line, then one reply a job in order for a client
+    that sends one job at a time, then a clean exit when its input ends."""
+    import subprocess
+    import sys
+    import threading
+
+    print("wire: command-line server, 2 jobs and a refused one", flush=True)
+    p = subprocess.Popen([sys.executable, "-m", "threert_amd.wire", "--budget-gb", "4"], stdin=subprocess.PIPE,
+                         stdout=subprocess.PIPE, text=True, bufsize=1)
+    watchdog = threading.Timer(120.0, p.kill)  # a server that stops answering ends the test, not the run
+    watchdog.start()
+    try:
+        ready = json.loads(p.stdout.readline())
+        assert ready["ready"] is True and "gfx" in ready["adapter"]
+        replies = []
+        for job in ({"id": 1, "scene": {"kind": "cornell"}, "spp": 2, "width": 32, "height": 24,
+                     "hdr": str(tmp_path / "a.raw")},
+                    {"id": 2, "game": "bundle.js"},
+                    {"id": 3, "scene": {"kind": "soup", "triangles": 2000, "seed": 4}, "spp": 1, "width": 16,
+                     "height": 16}):
+            p.stdin.write(json.dumps(job) + "\n")
+            p.stdin.flush()
+            replies.append(json.loads(p.stdout.readline()))
+        p.stdin.close()
+        assert p.wait(timeout=60) == 0
+    finally:
+        watchdog.cancel()
+        if p.poll() is None:
+            p.kill()
+    assert [(r["id"], r["ok"]) for r in replies] == [(1, True), (2, False), (3, True)]
+    assert replies[1]["error"].startswith("unsupported")
+    assert replies[0]["stats"]["triangles"] == 36 and (tmp_path / "a.raw").stat().st_size == 32 * 24 * 12
