@@ -109,12 +109,38 @@ def test_gpu_forward_matches_the_oracle(device):
 
 
 @pytest.mark.gpu
+def test_gpu_forward_on_tall_tiles_matches_the_oracle(device):
+    """Levels 128 pixels tall or more take 16-pixel-tall tiles (the others 8): the forward pass at 144 x 64, whose
+    first level does, within the same bound."""
+    print("unet forward 1 x 144 x 64 (tall tiles at the first level)", flush=True)
+    rng = np.random.default_rng(11)
+    color, albedo, nrm, _ = _batch(rng, 1, 144, 64)
+    params = U.init_params(12)
+    params[U.layout()[-1][3]:U.layout()[-1][3] + 3] = 0.3
+    net = U.UNet(device, 144 * 64, params)
+    got = net.denoise(color, albedo, nrm)
+    ref = O.Net(params, emulate=True)
+    ref.forward(color, albedo, nrm)
+    want = ref.denoised()
+    print("forward rel L2", _rel(got, want), flush=True)
+    assert _rel(got, want) <= 1e-2
+    net.close()
+
+
+@pytest.mark.gpu
 def test_gpu_gradients_match_the_oracle(device):
-    print("unet gradients 2 x 32 x 32", flush=True)
+    print("unet gradients 2 x 32 x 32 and 1 x 128 x 16 (tall tiles at the first level)", flush=True)
     rng = np.random.default_rng(6)
     color, albedo, nrm, target = _batch(rng, 2, 32, 32)
     params = U.init_params(9)
-    net = U.UNet(device, 2 * 32 * 32, params)
+    _check_gradients(device, params, color, albedo, nrm, target)
+    color, albedo, nrm, target = _batch(rng, 1, 128, 16)
+    _check_gradients(device, params, color, albedo, nrm, target)
+
+
+def _check_gradients(device, params, color, albedo, nrm, target):
+    n, h, w = color.shape[:3]
+    net = U.UNet(device, n * h * w, params)
     loss, g = net.gradients(color, albedo, nrm, target)
     rl, rg = O.Net(params, emulate=True).loss_and_grads(color, albedo, nrm, target)
     _, xg = O.Net(params.astype(np.float64), emulate=False).loss_and_grads(color, albedo, nrm, target)

@@ -32,17 +32,21 @@ __device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float(
 __device__ __forceinline__ bf16x8 as_bf16x8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
 
 // ---- convolution ------------------------------------------------------------------------------------------------
-template <int NCO>
+// TH: the tile's height in pixels (8 or 16; the width is 8). A taller tile shares each halo row and each weight load
+// over more pixels: the full-resolution layers, whose blocks mostly wait on their halo, take 16.
+template <int NCO, int TH>
 __global__ __launch_bounds__(256) void k_conv(ConvArgs a, uint32_t tiles_x, uint32_t tiles_y) {
+    static_assert(TH == 8 || TH == 16, "tiles are 8 or 16 pixels tall");
+    constexpr int S = TH * NCO / 8;  // 16-pixel strips (two tile rows each) a wave takes
     extern __shared__ uint4 smem[];
     uint16_t *halo = reinterpret_cast<uint16_t *>(smem);
     const uint32_t cin = a.c0 + a.c1, stride = cin + 8;
     const uint32_t tile = blockIdx.x;
     const uint32_t tx = tile % tiles_x, ty = (tile / tiles_x) % tiles_y, n = tile / (tiles_x * tiles_y);
-    const int x0 = int(tx * 8) - 1, y0 = int(ty * 8) - 1;
+    const int x0 = int(tx * 8) - 1, y0 = int(ty * TH) - 1;
     const uint32_t h2 = a.h / 2, w2 = a.w / 2;
     const uint32_t chunks = cin / 8;
-    for (uint32_t q = threadIdx.x; q < 100 * chunks; q += 256) {
+    for (uint32_t q = threadIdx.x; q < (TH + 2) * 10 * chunks; q += 256) {
         const uint32_t hp = q / chunks, ch = (q % chunks) * 8;
         const int gy = y0 + int(hp / 10), gx = x0 + int(hp % 10);
         uint4 v = make_uint4(0, 0, 0, 0);
@@ -60,17 +64,17 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a, uint32_t tiles_x, uint
     const uint32_t co_tile = wave % NCO, grp = wave / NCO;
     const uint32_t co_base = blockIdx.y * (16 * NCO) + co_tile * 16;
     const uint16_t *wrow = a.wt + uint64_t(co_base + li) * 9 * cin + g * 8;
-    f32x4 acc[NCO];
+    f32x4 acc[S];
 #pragma unroll
-    for (int j = 0; j < NCO; j++) acc[j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int j = 0; j < S; j++) acc[j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     for (uint32_t tap = 0; tap < 9; tap++) {
         const uint32_t dy = tap / 3, dx = tap % 3;
         const uint16_t *hrow = halo + ((dy + (li >> 3)) * 10 + (li & 7u) + dx) * stride + g * 8;
         for (uint32_t c = 0; c < cin; c += 32) {
             const bf16x8 A = as_bf16x8(*reinterpret_cast<const uint4 *>(wrow + tap * cin + c));
 #pragma unroll
-            for (int j = 0; j < NCO; j++) {
-                const uint32_t p = grp * NCO + j;  // the strip: tile rows 2p and 2p + 1
+            for (int j = 0; j < S; j++) {
+                const uint32_t p = grp * S + j;  // the strip: tile rows 2p and 2p + 1
                 const bf16x8 B = as_bf16x8(*reinterpret_cast<const uint4 *>(hrow + 2 * p * 10 * stride + c));
                 acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[j], 0, 0, 0);
             }
@@ -82,9 +86,9 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a, uint32_t tiles_x, uint
     if (a.bias)
         for (int r = 0; r < 4; r++) b[r] = a.bias[co + r];
 #pragma unroll
-    for (int j = 0; j < NCO; j++) {
-        const uint32_t p = grp * NCO + j;
-        const uint32_t y = ty * 8 + 2 * p + (li >> 3), x = tx * 8 + (li & 7u);
+    for (int j = 0; j < S; j++) {
+        const uint32_t p = grp * S + j;
+        const uint32_t y = ty * TH + 2 * p + (li >> 3), x = tx * 8 + (li & 7u);
         if (y >= a.h || x >= a.w) continue;
         float v[4];
         for (int r = 0; r < 4; r++) {
@@ -326,19 +330,30 @@ inline int pick_nco(uint32_t cout) { return cout % 64 == 0 ? 4 : (cout % 32 == 0
 int launch_conv(const ConvArgs &a, hipStream_t s) {
     const uint32_t cin = a.c0 + a.c1;
     if (a.cout % 16 || cin % 32 || a.c0 % 32 || (a.c1 && (a.h % 2 || a.w % 2)) || cin > 512) return -1;
-    const uint32_t tx = (a.w + 7) / 8, ty = (a.h + 7) / 8;
+    const bool tall = a.h >= 128;  // 16-pixel-tall tiles for the large levels
+    const uint32_t th = tall ? 16 : 8;
+    const uint32_t tx = (a.w + 7) / 8, ty = (a.h + th - 1) / th;
     const uint64_t tiles = uint64_t(tx) * ty * a.n;
     if (tiles == 0) return 0;
     if (tiles >= (uint64_t{1} << 31)) return -1;
-    const size_t lds = size_t(100) * (cin + 8) * 2;
+    const size_t lds = size_t(th + 2) * 10 * (cin + 8) * 2;
     const int nco = pick_nco(a.cout);
     const dim3 grid(static_cast<uint32_t>(tiles), a.cout / (16 * nco));
-    if (nco == 4)
-        hipLaunchKernelGGL(k_conv<4>, grid, dim3(256), lds, s, a, tx, ty);
-    else if (nco == 2)
-        hipLaunchKernelGGL(k_conv<2>, grid, dim3(256), lds, s, a, tx, ty);
-    else
-        hipLaunchKernelGGL(k_conv<1>, grid, dim3(256), lds, s, a, tx, ty);
+    if (tall) {
+        if (nco == 4)
+            hipLaunchKernelGGL((k_conv<4, 16>), grid, dim3(256), lds, s, a, tx, ty);
+        else if (nco == 2)
+            hipLaunchKernelGGL((k_conv<2, 16>), grid, dim3(256), lds, s, a, tx, ty);
+        else
+            hipLaunchKernelGGL((k_conv<1, 16>), grid, dim3(256), lds, s, a, tx, ty);
+    } else {
+        if (nco == 4)
+            hipLaunchKernelGGL((k_conv<4, 8>), grid, dim3(256), lds, s, a, tx, ty);
+        else if (nco == 2)
+            hipLaunchKernelGGL((k_conv<2, 8>), grid, dim3(256), lds, s, a, tx, ty);
+        else
+            hipLaunchKernelGGL((k_conv<1, 8>), grid, dim3(256), lds, s, a, tx, ty);
+    }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
