@@ -7,7 +7,7 @@ samples of one render, the error of the image as it is and of the learned denois
 for n samples has a plain error e_plain(n) (log-log interpolation between the calibrated points); the denoising
 server renders it at the fewest calibrated samples k < n whose denoised error is at most e_plain(n) and denoises it,
 or, when no such k exists, renders its n samples plainly (high-sample tracing is the fallback, and its calibrated
-error the judge). Both servers replay the same arrivals; each job's image error in the denoising run is at most the
+error the yardstick). Both servers replay the same arrivals; each job's image error in the denoising run is at most the
 plain run's, by calibration.
 """
 from __future__ import annotations
