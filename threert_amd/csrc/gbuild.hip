@@ -181,18 +181,37 @@ __global__ __launch_bounds__(kThreads) void k_bounds(const uint32_t *order, cons
         sh[i] = (w < 3 || (w >= 6 && w < 9)) ? kKeyMin : kKeyMax;
     }
     const int32_t a0 = first_slot(pos, base, n, &a0s);  // (its barriers order the LDS fill above)
+    const uint32_t lane = threadIdx.x & 63u;
     for (int k = 0; k < kPerThread; k++) {
         const uint32_t j = base + threadIdx.x + k * kThreads;
-        if (j >= n) continue;
-        const int32_t a = pos[j];
-        if (a < 0) continue;
-        const GBox b = box[order[j]];
+        const int32_t a = j < n ? pos[j] : -1;
         uint32_t key[kBoundWords];
-        for (int q = 0; q < 3; q++) {
-            key[q] = fkey(b.lo[q]);
-            key[3 + q] = fkey(b.hi[q]);
-            key[6 + q] = key[9 + q] = fkey(centroid(b, q));
+        if (a >= 0) {
+            const GBox b = box[order[j]];
+            for (int q = 0; q < 3; q++) {
+                key[q] = fkey(b.lo[q]);
+                key[3 + q] = fkey(b.hi[q]);
+                key[6 + q] = key[9 + q] = fkey(centroid(b, q));
+            }
+        } else {
+            for (int q = 0; q < 3; q++) key[q] = key[6 + q] = kKeyMin, key[3 + q] = key[9 + q] = kKeyMax;
         }
+        // a wave's lanes hold consecutive positions, so a node's are a run of lanes: fold each key down the run to
+        // its first lane (after the step of distance d, a lane holds its run's next 2d lanes), which alone updates
+        // the node
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int32_t oa = __shfl_down(a, d, 64);
+            const bool take = lane + d < 64 && oa == a;
+#pragma unroll
+            for (int q = 0; q < kBoundWords; q++) {
+                const uint32_t o = __shfl_down(key[q], d, 64);
+                const bool is_min = q < 3 || (q >= 6 && q < 9);
+                if (take) key[q] = is_min ? min(key[q], o) : max(key[q], o);
+            }
+        }
+        const int32_t prev = __shfl_up(a, 1, 64);
+        if (a < 0 || (lane > 0 && prev == a)) continue;
         if (a - a0 < kPriv) {
             uint32_t *d = sh + (a - a0) * kBoundWords;
             for (int q = 0; q < 3; q++) {
