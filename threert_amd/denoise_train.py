@@ -61,13 +61,15 @@ def scene_of(kind: str, seed: int):
     if kind == "soup":
         n = int(np.exp(np.random.default_rng(seed).uniform(np.log(20_000), np.log(200_000))))
         return serve.soup_scene(n, seed)
+    if kind == "bigsoup":  # the serving workload's large scenes: sub-pixel triangles
+        n = int(np.exp(np.random.default_rng(seed).uniform(np.log(200_000), np.log(2_000_000))))
+        return serve.soup_scene(n, seed)
     rs, cam, sky = R.city(seed=seed, meshes=8, instances=96, extent=80.0)
     return rs, cam, sky
 
 
-def scene_list(count: int, first_seed: int = 1):
-    kinds = ["cornell", "soup", "city"]
-    return [(kinds[k % 3], first_seed + k) for k in range(count)]
+def scene_list(count: int, first_seed: int = 1, kinds=("cornell", "soup", "city")):
+    return [(kinds[k % len(kinds)], first_seed + k) for k in range(count)]
 
 
 def render_set(dev: Device, kind: str, seed: int, w: int, h: int, noisy: int, target_spp: int, chunk: int = 64,
@@ -224,12 +226,15 @@ def main():
     ap.add_argument("--eval-width", type=int, default=1280)
     ap.add_argument("--eval-height", type=int, default=720)
     ap.add_argument("--report", default="")
+    ap.add_argument("--kinds", default="cornell,soup,city",
+                    help="scene generators the training scenes cycle through (cornell, soup, city, bigsoup)")
     args = ap.parse_args()
     dev = Device(0)
     ins, tgs = [], []
     t0 = time.perf_counter()
     levels = tuple(int(x) for x in args.levels.split(",") if x)
-    for k, (kind, seed) in enumerate(scene_list(args.scenes)):
+    kinds = tuple(x for x in args.kinds.split(",") if x)
+    for k, (kind, seed) in enumerate(scene_list(args.scenes, kinds=kinds)):
         i, t = render_set(dev, kind, seed, args.width, args.height, args.noisy, args.target_spp, levels=levels)
         ins.append(i)
         tgs.append(t)
@@ -240,7 +245,7 @@ def main():
     losses = train(dev, net, np.stack(ins), np.stack(tgs), args.steps, args.lr, log=lambda s: print(s, flush=True))
     train_s = time.perf_counter() - t1
     np.save(args.out, net.params())
-    held_out = scene_list(args.eval_scenes, first_seed=10_000)
+    held_out = scene_list(args.eval_scenes, first_seed=10_000, kinds=kinds)
     rep = evaluate(dev, net, held_out, args.eval_width, args.eval_height, args.target_spp,
                    log=lambda s: print(s, flush=True))
     rep.update({"train_scenes": args.scenes, "train_size": [args.width, args.height],
