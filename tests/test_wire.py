@@ -149,3 +149,35 @@ def test_the_command_line_server_answers_on_stdout(tmp_path):
     assert [(r["id"], r["ok"]) for r in replies] == [(1, True), (2, False), (3, True)]
     assert replies[1]["error"].startswith("unsupported")
     assert replies[0]["stats"]["triangles"] == 36 and (tmp_path / "a.raw").stat().st_size == 32 * 24 * 12
+
+
+@pytest.mark.gpu
+def test_a_denoise_job_over_the_wire_is_the_servers_denoised_image(device, tmp_path):
+    """`denoise: true` sends the job's image through the server's U-Net: the HDR file holds what the serving
+    denoising step gives for the same accumulator (device resolve into edge-padded buffers, the network), cropped."""
+    from threert_amd import render as R
+    from threert_amd import serve as V
+    from threert_amd.unet import UNet, shipped_params
+
+    print("wire: a denoise job", flush=True)
+    job = {"id": 5, "scene": {"kind": "cornell"}, "spp": 2, "width": 40, "height": 24, "seed": 3, "denoise": True,
+           "hdr": str(tmp_path / "d.raw")}
+    net = UNet(device, 1920 * 1088, shipped_params())
+    out = io.StringIO()
+    ls = W.LineServer(device, io.StringIO(json.dumps(job) + "\n"), out, unet=net, workers=1)
+    ls.run(0.0)
+    ls.close()
+    replies = [json.loads(x) for x in out.getvalue().splitlines()]
+    assert replies[1]["ok"] and replies[1]["stats"]["denoise"] == "unet", replies
+    got = np.fromfile(job["hdr"], "<f4").reshape(24, 40, 3)
+    _, j, _ = W.parse_job(json.dumps(job), 0)
+    rs, cam, sky = W.make_scene(*j.scene)
+    sc = R.upload(device, rs)
+    acc = R.Accum(device, 40, 24)
+    R.render(sc, V.params_of(V._Resident(None, rs, cam, sky, 0), j), 0, 2, acc)
+    c, a, n, _ = acc.resolve()
+    want = V.denoise_image(net, c, a, n)
+    assert got.shape == want.shape and (got == want).all()
+    acc.close()
+    sc.close()
+    net.close()
