@@ -118,7 +118,7 @@ void levels_build(const Leaf *leaves, const std::vector<MeshRef> &meshes, std::v
             slots[a] = s;
         }
         for (uint32_t j = 0; j < n; j++) {
-            if (pos[j] < 0 || !(slots[pos[j]].axes & 7u)) continue;
+            if (pos[j] < 0 || !(slots[pos[j]].axes & 7u) || (slots[pos[j]].axes & kSmall)) continue;
             const Slot &s = slots[pos[j]];
             const GBox &b = box[order[j]];
             for (int ax = 0; ax < 3; ax++) {
@@ -136,8 +136,11 @@ void levels_build(const Leaf *leaves, const std::vector<MeshRef> &meshes, std::v
         for (uint32_t a = 0; a < m; a++) {
             const TNode &nd = tn[level_base + a];
             int r = 0;
-            if (slots[a].axes != 0 || nd.e - nd.b <= kOwnLeaf)
-                r = split_decide(nd, &bins[uint64_t(a) * kSlotBinWords], slots[a]);
+            if (slots[a].axes != 0 || nd.e - nd.b <= kOwnLeaf) {
+                if (slots[a].axes & kSmall)
+                    r = split_decide(nd, BinItems(box.data(), order.data(), nd.b, nd.e, slots[a]), slots[a]);
+                else r = split_decide(nd, BinWords{&bins[uint64_t(a) * kSlotBinWords]}, slots[a]);
+            }
             if (r < 0) host_mesh[nd.mesh] = 1, r = 0, slots[a].mode = kModeLeaf;
             splits[a] = r > 0;
         }

@@ -276,7 +276,7 @@ __global__ __launch_bounds__(kThreads) void k_bins(const uint32_t *order, const 
         const int32_t a = pos[j];
         if (a < 0) continue;
         const Slot s = slots[a];
-        if (!(s.axes & 7u)) continue;
+        if (!(s.axes & 7u) || (s.axes & kSmall)) continue;  // (a small node is binned in k_split)
         const GBox b = box[order[j]];
         uint32_t lo[3], hi[3];
         for (int q = 0; q < 3; q++) lo[q] = fkey(b.lo[q]), hi[q] = fkey(b.hi[q]);
@@ -306,8 +306,8 @@ __global__ __launch_bounds__(kThreads) void k_bins(const uint32_t *order, const 
 }
 
 __global__ __launch_bounds__(kThreads) void k_split(const TNode *tn, uint32_t level_base, uint32_t m,
-                                                    const uint32_t *bins, Slot *slots, uint32_t *splits,
-                                                    uint32_t *host_mesh) {
+                                                    const uint32_t *bins, const GBox *box, const uint32_t *order,
+                                                    Slot *slots, uint32_t *splits, uint32_t *host_mesh) {
     const uint32_t a = blockIdx.x * kThreads + threadIdx.x;
     if (a > m) return;
     if (a == m) {
@@ -318,7 +318,8 @@ __global__ __launch_bounds__(kThreads) void k_split(const TNode *tn, uint32_t le
     Slot s = slots[a];
     int r = 0;
     if (s.axes != 0 || nd.e - nd.b <= kOwnLeaf) {
-        r = split_decide(nd, bins + uint64_t(a) * kSlotBinWords, s);
+        if (s.axes & kSmall) r = split_decide(nd, BinItems(box, order, nd.b, nd.e, s), s);
+        else r = split_decide(nd, BinWords{bins + uint64_t(a) * kSlotBinWords}, s);
     }
     if (r < 0) {
         host_mesh[nd.mesh] = 1;
@@ -877,7 +878,8 @@ bool device_wide(const Leaf *d_leaves, uint64_t leaf_count, const std::vector<Me
         k_bounds<<<chunks, kThreads, 0, st>>>(order, pos, box, n, bounds);
         k_slots<<<blocks(m), kThreads, 0, st>>>(tn, level_base, m, bounds, slots, host_mesh);
         k_bins<<<chunks, kThreads, 0, st>>>(order, pos, box, n, slots, bins);
-        k_split<<<blocks(uint64_t(m) + 1), kThreads, 0, st>>>(tn, level_base, m, bins, slots, splits, host_mesh);
+        k_split<<<blocks(uint64_t(m) + 1), kThreads, 0, st>>>(tn, level_base, m, bins, box, order, slots, splits,
+                                                              host_mesh);
         if (!scan(splits, uint64_t(m) + 1, split_before, tile_sum, grand, st, why)) return false;
         uint32_t nsplit = 0;
         GB_HIP(hipMemcpyAsync(&nsplit, split_before + m, 4, hipMemcpyDeviceToHost, st));

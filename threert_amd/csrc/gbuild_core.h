@@ -24,6 +24,8 @@ constexpr int kBoundWords = 12;              // node low / high keys, centroid l
 
 enum Mode : uint32_t { kModeLeaf = 0, kModeSah = 1, kModeMiddle = 2 };
 constexpr uint32_t kAllEqual = 1u << 3;  // Slot::axes: every centroid of the node is the same point
+constexpr uint32_t kSmall = 1u << 4;     // Slot::axes: the node is small (binned from its triangles in k_split)
+constexpr uint32_t kSmallNode = 8;       // nodes of at most this many triangles are small
 
 // A node of the build, in level order (its final index in the mesh's tree comes at the end).
 struct TNode {
@@ -149,50 +151,86 @@ TRT_HD inline bool slot_setup(TNode &n, const uint32_t *bk, Slot &s) {
     // centroid is the same point (its triangles are then sorted by index already, and stay so)
     if (cnt > kOwnLeaf && s.axes == 0 && !(all_equal && n.depth < kOwnDepth)) return false;
     if (all_equal) s.axes |= kAllEqual;
+    if (cnt <= kSmallNode) s.axes |= kSmall;
     return true;
 }
+
+// Bins of a node as the sweep reads them: count and box of bin bi on axis k.
+struct BinWords {  // from the binning kernel's words
+    const uint32_t *bins;
+    TRT_HD uint32_t get(int k, int bi, float *lo, float *hi) const {
+        const uint32_t *w = bins + (k * kOwnBins + bi) * kBinWords;
+        if (w[0])
+            for (int j = 0; j < 3; j++) lo[j] = kfloat(w[1 + j]), hi[j] = kfloat(w[4 + j]);
+        return w[0];
+    }
+};
+struct BinItems {  // straight from a small node's triangles, loaded once (the same min / max, so the same floats)
+    GBox x[kSmallNode];
+    uint8_t bin[3][kSmallNode];
+    uint32_t n;
+    TRT_HD BinItems(const GBox *box, const uint32_t *order, uint32_t b, uint32_t e, const Slot &s) : n(e - b) {
+        for (uint32_t i = 0; i < n; i++) {
+            x[i] = box[order[b + i]];
+            for (int k = 0; k < 3; k++)
+                bin[k][i] = (s.axes >> k & 1u) ? uint8_t(bin_of(centroid(x[i], k), s.clo[k], s.scale[k])) : 0;
+        }
+    }
+    TRT_HD uint32_t get(int k, int bi, float *lo, float *hi) const {
+        uint32_t c = 0;
+        for (uint32_t i = 0; i < n; i++) {
+            if (bin[k][i] != bi) continue;
+            for (int q = 0; q < 3; q++) {
+                lo[q] = c == 0 || x[i].lo[q] < lo[q] ? x[i].lo[q] : lo[q];
+                hi[q] = c == 0 || hi[q] < x[i].hi[q] ? x[i].hi[q] : hi[q];
+            }
+            c++;
+        }
+        return c;
+    }
+};
 
 // The host's SAH sweep over the node's bins and its leaf rule; fills the slot's decision. Returns 1 if the node
 // splits, 0 for a leaf, -1 if the host must build the mesh (a median split of distinct centroids, as when every
 // split's cost overflows).
-TRT_HD inline int split_decide(const TNode &n, const uint32_t *bins, Slot &s) {
+template <class Bins>
+TRT_HD inline int split_decide(const TNode &n, const Bins &bins, Slot &s) {
     const uint32_t cnt = n.e - n.b;
     const float leaf_cost = float(cnt);
     int best_axis = -1, best_bin = -1;
     float best_cost = __builtin_inff();
     for (int k = 0; k < 3; k++) {
         if (!(s.axes >> k & 1u)) continue;
-        const uint32_t *bk = bins + k * kOwnBins * kBinWords;
         float right_area[kOwnBins];
         uint32_t right_count[kOwnBins];
         float alo[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
         float ahi[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
         uint32_t ac = 0;
         for (int bi = kOwnBins - 1; bi > 0; bi--) {
-            const uint32_t *w = bk + bi * kBinWords;
-            if (w[0]) {
+            float l[3], h[3];
+            const uint32_t cnt_bi = bins.get(k, bi, l, h);
+            if (cnt_bi) {
                 for (int j = 0; j < 3; j++) {
-                    const float l = kfloat(w[1 + j]), h = kfloat(w[4 + j]);
-                    alo[j] = l < alo[j] ? l : alo[j];
-                    ahi[j] = ahi[j] < h ? h : ahi[j];
+                    alo[j] = l[j] < alo[j] ? l[j] : alo[j];
+                    ahi[j] = ahi[j] < h[j] ? h[j] : ahi[j];
                 }
             }
-            ac += w[0];
+            ac += cnt_bi;
             right_area[bi] = ac ? area3(alo, ahi) : 0.0f;
             right_count[bi] = ac;
         }
         for (int j = 0; j < 3; j++) alo[j] = __builtin_inff(), ahi[j] = -__builtin_inff();
         ac = 0;
         for (int bi = 0; bi < kOwnBins - 1; bi++) {
-            const uint32_t *w = bk + bi * kBinWords;
-            if (w[0]) {
+            float l[3], h[3];
+            const uint32_t cnt_bi = bins.get(k, bi, l, h);
+            if (cnt_bi) {
                 for (int j = 0; j < 3; j++) {
-                    const float l = kfloat(w[1 + j]), h = kfloat(w[4 + j]);
-                    alo[j] = l < alo[j] ? l : alo[j];
-                    ahi[j] = ahi[j] < h ? h : ahi[j];
+                    alo[j] = l[j] < alo[j] ? l[j] : alo[j];
+                    ahi[j] = ahi[j] < h[j] ? h[j] : ahi[j];
                 }
             }
-            ac += w[0];
+            ac += cnt_bi;
             if (ac == 0 || right_count[bi + 1] == 0) continue;
             const float cost = area3(alo, ahi) * float(ac) + right_area[bi + 1] * float(right_count[bi + 1]);
             if (cost < best_cost) best_cost = cost, best_axis = k, best_bin = bi;
@@ -206,8 +244,10 @@ TRT_HD inline int split_decide(const TNode &n, const uint32_t *bins, Slot &s) {
     }
     if (best_axis >= 0) {
         uint32_t nl = 0;
-        const uint32_t *bk = bins + best_axis * kOwnBins * kBinWords;
-        for (int bi = 0; bi <= best_bin; bi++) nl += bk[bi * kBinWords];
+        for (int bi = 0; bi <= best_bin; bi++) {
+            float l[3], h[3];
+            nl += bins.get(best_axis, bi, l, h);
+        }
         s.mode = kModeSah;
         s.axis = uint32_t(best_axis);
         s.bin = uint32_t(best_bin);
