@@ -503,6 +503,14 @@ def main():
     hits = d_hits.download(T.HIT, N_RAYS)
     hit_rate = float((hits["tri"] != T.NO_HIT).mean())
     log(f"bench: {ms_step:.3f} ms/step, {mrays:.1f} Mrays/s, wall {wall:.2f} s, hit rate {hit_rate:.3f}")
+    # the boundary's host-buffer entry (trt_trace_rays): the caller's rays and hits cross PCIe each call
+    sc.trace(rays[:1024])
+    t0 = time.perf_counter()
+    host_hits = sc.trace(rays)
+    host_ms = (time.perf_counter() - t0) * 1e3
+    host_buffers = {"ms": round(host_ms, 2), "mrays_s": round(N_RAYS / (host_ms * 1e-3) / 1e6, 1),
+                    "hits_equal": bool((host_hits.view(np.uint32) == hits.view(np.uint32)).all())}
+    log(f"bench: host buffers (PCIe both ways) {host_buffers}")
     fp64 = fp64_check(dev, sc, d_rays, d_hits, hits)
     log(f"bench: FP64 check {fp64}")
     binary = binary_walk(dev, sc, d_rays, d_hits, hits)
@@ -537,6 +545,7 @@ def main():
             "fp64_check": fp64,
             "binary_walk": binary,
             "tree_build": build,
+            "host_buffers_pcie": host_buffers,
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic()},
