@@ -293,6 +293,9 @@ constexpr uint32_t kWalkDone = 0xffffffffu;   // the walk is over
 constexpr uint32_t kWalkLeave = 0xfffffffeu;  // the stack is back at the unit's entry depth: leave the unit, then pop
 constexpr uint32_t kLeafShareBatch = 3;  // ray batches (trace.hip): the other phase once a third of the lanes wait
 constexpr uint32_t kLeafShareCall = 2;   // trace_wide (the path tracer's rays): once half of them wait
+// (measured on the path tracer: 1, serving lanes as soon as they all wait, took the 720p city frame from 1.94 to
+// 1.90 ms but the 3M-triangle 1080p image from 1.36 to 2.14 s; 3 made the city 2.02 ms and the large image no
+// faster)
 
 struct WideWalk {
     F3 o, d, ro, rd, rinv;
