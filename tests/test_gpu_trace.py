@@ -253,3 +253,41 @@ def test_fp64_check_is_the_double_precision_test(device):
     rel = np.abs(hits["t"][fine].astype(np.float64) - t64[fine]) / t64[fine]
     assert rel.max() < 1e-4, rel.max()
     assert (t64[hit] == -1).mean() < 0.01
+
+
+def test_threads_share_a_device(device):
+    """The ABI is thread-safe per device: four threads, each uploading its own scene (trees built on the device)
+    and tracing batches on it while the others do, get the hits a lone thread gets."""
+    import threading
+
+    say("threads sharing the device")
+    preps = [S.instanced_scene(20 + k, meshes=4, instances=25, tris_per_mesh=(100, 30000)) for k in range(4)]
+    rays = [rays_at(p, 20000, 40 + k, T.TRACE_CLOSEST) for k, p in enumerate(preps)]
+    want = []
+    for p, r in zip(preps, rays):
+        sc = T.Scene(device, *p.desc())
+        want.append(sc.trace(r))
+        sc.close()
+    got, errors = [None] * 4, []
+
+    def work(k):
+        try:
+            sc = T.Scene(device, *preps[k].desc())
+            for _ in range(3):
+                got[k] = sc.trace(rays[k])
+            sc.close()
+        except Exception as e:  # reported below
+            errors.append((k, e))
+
+    device.set_build(T.BUILD_DEVICE)
+    try:
+        ts = [threading.Thread(target=work, args=(k,)) for k in range(4)]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join(timeout=120)
+    finally:
+        device.set_build(T.BUILD_AUTO)
+    assert not errors, errors
+    for k in range(4):
+        assert got[k] is not None and same(got[k], want[k]).all(), k
