@@ -104,6 +104,64 @@ def test_a_coat_alone_reflects_its_directional_albedo():
     assert np.all(np.abs(got - want) < 0.03 * want), (got, want)
 
 
+def test_the_emitter_guide_finds_what_the_full_search_finds():
+    """lib.rs:1459-1500 on the library's emitter table: 3000 emitter triangles over six decades of area (the
+    reference's LCG, seed 11), a ninth of them black (weight 0); the kernel's pick (pathtrace.wgsl:478-487: the guide
+    bucket's range, then the binary search) equals the search over the whole list for 200,000 random 24-bit u, the
+    bucket edges and their neighbours, and every entry's own cumulative weight. The table: the entries, the 65,537
+    guide words, the header (total weight)."""
+    seed = [11]
+
+    def rnd():
+        seed[0] = (seed[0] * 1664525 + 1013904223) & 0xFFFFFFFF
+        return seed[0]
+
+    def unit():
+        return np.float32((rnd() >> 8) / float(1 << 24))
+
+    tris = []
+    for i in range(3000):
+        c = np.array([300.0 * unit(), 5.0, 100.0 * unit()], np.float32)
+        s = np.float32(math.sqrt(2.0 * 10.0 ** (-6.0 * float(unit()))))
+        tris.append([c, c + np.array([s, 0, 0], np.float32), c + np.array([0, 0, s], np.float32)])
+    tris = np.array(tris, np.float32)
+    sections = np.array([0 if i % 9 == 0 else 1 + i % 3 for i in range(3000)], np.uint32)
+    mats = [R.material((0, 0, 0), (0.0, 0.0, 0.0))] + [R.material((0, 0, 0), (3.0 * k, k, 0.5 * k)) for k in (1, 2, 3)]
+    rs = R.build([R.RMesh(tris, sections=sections)], [R.RInstance(0, np.eye(4, dtype=np.float32), [0, 1, 2, 3])],
+                 mats)
+    t = rs.emitters
+    n = len(rs.emitter_tris)
+    assert n > 2000 and len(t) == n + (65537 + 3) // 4 + 1
+    total = t[-1, 0]
+    assert total > 0 and (t[-1, 1:] == 0).all()
+    cdf = t[:n, 1]
+    assert (np.diff(cdf) >= 0).all() and abs(float(cdf[-1]) - 1.0) < 1e-6
+    assert abs(float(np.sum(t[:n, 2].astype(np.float64))) / float(total) - 1.0) < 1e-5  # the pmf sums to 1
+    guide = t[n:n + (65537 + 3) // 4].view(np.uint32).reshape(-1)[:65537]
+
+    def search(lo, hi, u):
+        while lo < hi:
+            mid = (lo + hi) // 2
+            if cdf[mid] < u:
+                lo = mid + 1
+            else:
+                hi = mid
+        return min(lo, n - 1)
+
+    def pick(u):
+        k = min(int(np.float32(u) * np.float32(65536)), 65535)
+        return search(int(guide[k]), int(guide[k + 1]), u)
+
+    us = [rnd() >> 8 for _ in range(200_000)]
+    for k in range(65537):
+        m = k << 12
+        us.extend(x for x in (max(m - 1, 0), m, m + 1) if x < 1 << 24)
+    us.extend(min(int(np.float32(c) * np.float32(1 << 24)), (1 << 24) - 1) for c in cdf)
+    for m in us:
+        u = np.float32(m) / np.float32(1 << 24)
+        assert pick(u) == search(0, n, u), m
+
+
 def test_white_furnace_metal():
     """furnace-metal.ts: F0 = 1 (metallic, base 1), roughness 0.5, energy compensation 1: 1.0 +- 3 %."""
     rs, cam = sphere_scene((1, 1, 1), 1.0, 0.5, 0.5)
