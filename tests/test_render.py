@@ -438,3 +438,39 @@ def test_the_serving_job_that_met_a_tie_matches_the_oracle(device):
             print(f"sample {s} row {y}: {bad} pixels differ", flush=True)
             assert bad == 0
     sc.close()
+
+
+@pytest.mark.gpu
+def test_what_the_library_does_not_render_is_refused(device):
+    """Records with cutout or glass sections (scene.rs INST_CUTOUT 16, INST_GLASS 32 in the bits word), an
+    environment map (Params.env.x) and more than 64 bounces are refused with TRT_E_INVALID and a message that names
+    them; the device still renders the unchanged scene afterwards."""
+    import copy
+
+    import threert_amd as T
+
+    rs, cam = R.cornell()
+    for bit, name in ((16, "cutout"), (32, "glass")):
+        print(f"refusal: record bit {bit} ({name})", flush=True)
+        bad = copy.deepcopy(rs)
+        words = bad.prep.instances["n"][:, 1, 3].view(np.uint32)
+        words[0] |= np.uint32(bit)
+        with pytest.raises(T.ThreeRTError) as e:
+            R.upload(device, bad)
+        assert e.value.code == T.TRT_E_INVALID and "cutout or glass" in str(e.value)
+    sc = R.upload(device, rs)
+    acc = R.Accum(device, 16, 16)
+    for what, change in (("environment map", lambda p: p.view(np.float32).__setitem__(24, 1.0)),
+                         ("more than 64 bounces", lambda p: p.__setitem__(22, 65))):
+        print(f"refusal: {what}", flush=True)
+        p = rs.params(16, 16, cam)
+        change(p)
+        with pytest.raises(T.ThreeRTError) as e:
+            R.render(sc, p, 0, 1, acc)
+        assert e.value.code == T.TRT_E_INVALID and what.split()[0] in str(e.value)
+    R.render(sc, rs.params(16, 16, cam, max_bounces=64), 0, 2, acc)
+    device.sync()
+    c = acc.resolve()[0]
+    assert np.isfinite(c).all() and c.mean() > 0
+    acc.close()
+    sc.close()
