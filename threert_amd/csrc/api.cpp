@@ -193,12 +193,21 @@ int trt_device_info_get(trt_device *d, trt_device_info *out) {
 
 int trt_device_sync(trt_device *d) {
     if (!d) return fail(TRT_E_INVALID, "trt_device_sync: null device");
-    TRT_HIP(hipSetDevice(d->ordinal));
-    TRT_HIP(hipStreamSynchronize(d->stream));
     uint32_t g[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    TRT_HIP(hipMemcpy(g, d->guard, sizeof(g), hipMemcpyDeviceToHost));
+    {
+        // Read and clear the guard words under the device's lock, on its stream: no other thread's launch can land
+        // between the read and the clear, so a report reaches the first sync after the launch that raised it.
+        std::lock_guard<std::mutex> lock(d->mu);
+        TRT_HIP(hipSetDevice(d->ordinal));
+        TRT_HIP(hipStreamSynchronize(d->stream));
+        TRT_HIP(hipMemcpyAsync(g, d->guard, sizeof(g), hipMemcpyDeviceToHost, d->stream));
+        TRT_HIP(hipStreamSynchronize(d->stream));
+        if (g[0]) {
+            TRT_HIP(hipMemsetAsync(d->guard, 0, sizeof(g), d->stream));
+            TRT_HIP(hipStreamSynchronize(d->stream));
+        }
+    }
     if (g[0]) {
-        TRT_HIP(hipMemset(d->guard, 0, sizeof(g)));
         // the first out-of-range read (walk.h report_range ids)
         static const char *what[] = {"?", "binary top node", "binary node", "binary children", "binary stack",
                                      "unit record", "binary leaf run", "wide node", "wide stack", "wide unit",
