@@ -193,6 +193,15 @@ typedef struct {
 int trt_trace_rays(trt_scene *scene, const trt_ray *rays, uint64_t n, trt_hit *hits, uint32_t flags);
 /* The same with device buffers (trt_mem_alloc), queued on the device's stream; trt_device_sync waits. */
 int trt_trace_rays_device(trt_scene *scene, const trt_ray *d_rays, uint64_t n, trt_hit *d_hits, uint32_t flags);
+/* The same as trt_trace_rays_device, allocating nothing: no counterpart in the reference; for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). The
+ * walk's ray counter and its stack entries past LDS live in a fixed 32 MiB scratch array, one per GPU, held until the
+ * process ends; a tree too deep for it at full occupancy walks on fewer workgroups, and one whose stack does not fit
+ * even one workgroup is refused (TRT_E_INVALID). Every call on a GPU shares that GPU's scratch: calls through one
+ * trt_device are ordered on its stream, but calls on two handles of the same GPU that overlap race. Queued on the
+ * device's stream; trt_device_sync waits and reports the walk's guard. */
+int trt_trace_rays_device_noalloc(trt_scene *scene, const trt_ray *d_rays, uint64_t n, trt_hit *d_hits,
+                                  uint32_t flags);
 /* The same walk, counting what it reads (units.rs:922-927 Reads, units_cost.rs): reads[0] nodes fetched (64-byte
  * wide or 32-byte binary), reads[1] triangles tested (48 bytes each), reads[2] unit records read (64-byte wide
  * units or 160-byte records). Blocks. */

@@ -45,7 +45,7 @@ EXPORTS = [
     "trt_denoise_atrous", "trt_render_batch", "trt_mem_copy", "trt_trace_check_fp64", "trt_render_ray_counts",
     "trt_accum_resolve_device", "trt_accum_add_fp64", "trt_device_set_build", "trt_scene_wide_read",
     "trt_unet_create", "trt_unet_release", "trt_unet_param_count", "trt_unet_params_set", "trt_unet_params_get",
-    "trt_unet_denoise", "trt_unet_gradients", "trt_unet_train_step",
+    "trt_unet_denoise", "trt_unet_gradients", "trt_unet_train_step", "trt_trace_rays_device_noalloc",
 ]
 
 
@@ -108,7 +108,8 @@ def lib():
             "trt_build_top_tree": [P, P, U64, P],
             "trt_scene_upload": [P, P, C.POINTER(C.c_void_p)], "trt_scene_release": [P],
             "trt_scene_stats_get": [P, P], "trt_trace_rays": [P, P, U64, P, U32],
-            "trt_trace_rays_device": [P, P, U64, P, U32], "trt_trace_reads": [P, P, U64, P, U32, P],
+            "trt_trace_rays_device": [P, P, U64, P, U32], "trt_trace_rays_device_noalloc": [P, P, U64, P, U32],
+            "trt_trace_reads": [P, P, U64, P, U32, P],
             "trt_trace_check_fp64": [P, P, P, U64, P],
             "trt_render_ray_counts": [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
             "trt_device_set_build": [P, U32], "trt_scene_wide_read": [P, U32, P, U64],
@@ -324,6 +325,11 @@ class Scene:
     def trace_device(self, rays: DeviceBuffer, n: int, hits: DeviceBuffer, walk: int = WALK_WIDE):
         """Queues a batch whose rays are already in HBM; Device.sync() waits."""
         _check(lib().trt_trace_rays_device(self.h, rays.p, n, hits.p, walk))
+
+    def trace_device_noalloc(self, rays: DeviceBuffer, n: int, hits: DeviceBuffer, walk: int = WALK_WIDE):
+        """trace_device without any device allocation (trt_trace_rays_device_noalloc): the walk's counter and
+        spilled stack in the GPU's fixed scratch. Device.sync() waits."""
+        _check(lib().trt_trace_rays_device_noalloc(self.h, rays.p, n, hits.p, walk))
 
     def reads(self, rays: DeviceBuffer, n: int, hits: DeviceBuffer, walk: int = WALK_WIDE) -> dict:
         """What the walk reads over a batch (units.rs Reads): node fetches, triangle tests, unit records, and the

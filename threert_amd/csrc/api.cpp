@@ -66,6 +66,7 @@ struct trt_device {
     uint64_t denoise_bytes = 0;
     void *staging = nullptr;  // trt_trace_rays' device copies of the caller's rays and hits
     uint64_t staging_bytes = 0;
+    uint32_t *scratch = nullptr;  // this GPU's fixed scratch (trace.hip), trt_trace_rays_device_noalloc's only memory
     int cus = 0;
     int blocks_wide = 1, blocks_binary = 1, blocks_render = 1;
     std::atomic<uint32_t> build{TRT_BUILD_AUTO};  // where uploads build the library's own mesh trees
@@ -122,6 +123,12 @@ int trt_device_create(int ordinal, trt_device **out) {
         trt_device_release(d);
         return fail(TRT_E_DEVICE, std::string("trt_device_create: ") + hipGetErrorString(e));
     }
+    void *scratch = nullptr;
+    if (trt::scratch_address(ordinal, &scratch)) {
+        trt_device_release(d);
+        return fail(TRT_E_DEVICE, "trt_device_create: the fixed scratch's device address could not be found");
+    }
+    d->scratch = static_cast<uint32_t *>(scratch);
     d->cus = d->props.multiProcessorCount;
     if (trt::trace_occupancy(true, &d->blocks_wide) || d->blocks_wide < 1) d->blocks_wide = 1;
     if (trt::trace_occupancy(false, &d->blocks_binary) || d->blocks_binary < 1) d->blocks_binary = 1;
@@ -723,18 +730,36 @@ static int ensure_spill(trt_device *d, uint64_t bytes) {
     return TRT_OK;
 }
 
-static int launch(trt_scene *s, const void *d_rays, uint64_t n, void *d_hits, uint32_t flags, bool count = false) {
+// noalloc: the counter and the spilled stack entries in the GPU's fixed scratch (d->scratch), no allocation; a grid
+// smaller than the full one when a deep tree's entries would not fit it at full occupancy.
+static int launch(trt_scene *s, const void *d_rays, uint64_t n, void *d_hits, uint32_t flags, bool count = false,
+                  bool noalloc = false) {
     trt_device *d = s->dev;
     const bool wide = flags == TRT_WALK_WIDE;
     if (flags != TRT_WALK_WIDE && flags != TRT_WALK_BINARY) return fail(TRT_E_INVALID, "unknown trace flags");
     if (n >= (uint64_t{1} << 31)) return fail(TRT_E_INVALID, "a batch of 2^31 rays or more");
     if (n == 0) return ok();
-    const int grid = d->cus * (wide ? d->blocks_wide : d->blocks_binary);
-    const uint32_t threads = static_cast<uint32_t>(grid) * trt::kTraceBlock;
+    int grid = d->cus * (wide ? d->blocks_wide : d->blocks_binary);
     const uint32_t deepest = wide ? s->max_stack : trt::kRefStack;
     const uint64_t entries = deepest > trt::kLdsStack ? deepest - trt::kLdsStack : 1;
-    int rc = ensure_spill(d, entries * threads * sizeof(uint32_t));
-    if (rc) return rc;
+    uint32_t *counter = d->counter, *spill = nullptr;
+    if (noalloc) {
+        const uint64_t room = (trt::kScratchBytes - trt::kScratchCounterBytes) / sizeof(uint32_t);
+        const uint64_t blocks = room / (entries * trt::kTraceBlock);
+        if (blocks == 0)
+            return fail(TRT_E_INVALID, "the scene's walks need " + std::to_string(entries * trt::kTraceBlock * 4) +
+                                           " bytes of stack a block past LDS; the fixed scratch holds " +
+                                           std::to_string(room * 4));
+        grid = static_cast<int>(std::min<uint64_t>(static_cast<uint64_t>(grid), blocks));
+        counter = d->scratch;
+        spill = d->scratch + trt::kScratchCounterBytes / sizeof(uint32_t);
+    }
+    const uint32_t threads = static_cast<uint32_t>(grid) * trt::kTraceBlock;
+    if (!noalloc) {
+        int rc = ensure_spill(d, entries * threads * sizeof(uint32_t));
+        if (rc) return rc;
+        spill = d->spill;
+    }
     trt::TraceArgs a{};
     a.wnodes = static_cast<const uint4 *>(s->wnodes);
     a.wunits = static_cast<const float4 *>(s->wunits);
@@ -744,9 +769,9 @@ static int launch(trt_scene *s, const void *d_rays, uint64_t n, void *d_hits, ui
     a.instances = static_cast<const float4 *>(s->instances);
     a.rays = static_cast<const float4 *>(d_rays);
     a.hits = static_cast<uint32_t *>(d_hits);
-    a.counter = d->counter;
+    a.counter = counter;
     a.guard_flag = d->guard;
-    a.spill = d->spill;
+    a.spill = spill;
     a.reads = d->reads;
     a.n = n;
     a.spill_stride = threads;
@@ -1071,6 +1096,14 @@ int trt_trace_rays_device(trt_scene *s, const trt_ray *d_rays, uint64_t n, trt_h
     std::lock_guard<std::mutex> lock(s->dev->mu);
     TRT_HIP(hipSetDevice(s->dev->ordinal));
     return launch(s, d_rays, n, d_hits, flags);
+}
+
+int trt_trace_rays_device_noalloc(trt_scene *s, const trt_ray *d_rays, uint64_t n, trt_hit *d_hits, uint32_t flags) {
+    if (!s || (n && (!d_rays || !d_hits)))
+        return fail(TRT_E_INVALID, "trt_trace_rays_device_noalloc: null argument");
+    std::lock_guard<std::mutex> lock(s->dev->mu);
+    TRT_HIP(hipSetDevice(s->dev->ordinal));
+    return launch(s, d_rays, n, d_hits, flags, false, true);
 }
 
 int trt_trace_rays(trt_scene *s, const trt_ray *rays, uint64_t n, trt_hit *hits, uint32_t flags) {

@@ -11,6 +11,8 @@
 // wave per SIMD fewer (four instead of five), and was 6 % slower.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "trace_kernels.h"
 #include "walk.h"
 
@@ -143,6 +145,28 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_dyn(TraceArgs a) {
 __global__ void k_reset_counter(uint32_t *c) { *c = 0; }
 
 }  // namespace
+
+// The fixed scratch of the trace entry that allocates nothing (trt_trace_rays_device_noalloc): the ray counter in its
+// first 256 bytes, then the stack entries past the LDS ones, [entry][thread]. One copy on every GPU the library runs
+// on, held until the process ends. Host code reaches it only through the address hipGetSymbolAddress gives on that
+// GPU (the array's name is a host-side stand-in there).
+static __device__ __align__(256) uint32_t scratch[kScratchBytes / sizeof(uint32_t)];
+
+// Looked up once per GPU (with that GPU current) and kept: every handle on the GPU shares its one array.
+int scratch_address(int ordinal, void **ptr) {
+    static std::mutex mu;
+    static void *addr[kMaxDevices] = {};
+    *ptr = nullptr;
+    if (ordinal < 0 || ordinal >= kMaxDevices) return -1;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[ordinal]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return -1;
+        addr[ordinal] = p;
+    }
+    *ptr = addr[ordinal];
+    return 0;
+}
 
 int trace_occupancy(bool wide, int *blocks_per_cu) {
     int b = 0;

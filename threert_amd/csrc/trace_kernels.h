@@ -41,6 +41,14 @@ struct TraceArgs {
     float top_box[6];
 };
 
+// The fixed per-GPU scratch of trt_trace_rays_device_noalloc (trace.hip): 256 bytes of counter, then spilled stack
+// entries.
+constexpr uint64_t kScratchBytes = uint64_t{32} << 20;
+constexpr uint64_t kScratchCounterBytes = 256;
+constexpr int kMaxDevices = 64;
+// The scratch's address on GPU `ordinal`, which must be the current device; 0 on success.
+int scratch_address(int ordinal, void **ptr);
+
 int trace_occupancy(bool wide, int *blocks_per_cu);
 int launch_trace(const TraceArgs &a, bool wide, bool count, int grid, hipStream_t s);
 // n16 16-byte words from src to dst, both 16-byte aligned; at most `grid` blocks of 256
