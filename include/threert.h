@@ -101,6 +101,22 @@ void trt_unit_box(const float *world, const float *lo, const float *hi, float *o
 void trt_unit_record(void *record, uint32_t root, int cast_shadow);
 void trt_tree_free(trt_tree *t);
 
+/* ltree.rs:409-459 build (lib.rs:256-264 light_tree_of, without moving emitters or the slot for emitters added in
+ * place): the light tree over n emitter triangles, given their world vertices (n x 9, as GpuEmitterTri::vertices),
+ * centroids (n x 3) and weights (n; lib.rs:237-241 emitter_power; those not above 0 are left out). node_count 0: fewer
+ * than 256 emitters weigh something, and the kernel samples the CDF alone (ltree.rs:31 MIN_STATIC). */
+typedef struct {
+    float *nodes;         /* node_count vec4s: two packed half-float pairs (centre x, y; centre z, radius), power, word */
+    uint64_t node_count;
+    uint32_t *order;      /* entry_count: the emitter of each tree entry, in leaf order (the CDF's first entries) */
+    uint32_t *code;       /* entry_count: each entry's path code (depth + 1) << 26 | path */
+    uint64_t entry_count;
+    float power;          /* the root's */
+} trt_light_tree;
+int trt_build_light_tree(const float *verts, const float *centroids, const float *weights, uint64_t n,
+                         trt_light_tree *out);
+void trt_light_tree_free(trt_light_tree *t);
+
 /* --- scene upload (gpu.rs:2613 Gpu::upload) ---------------------------------------------------------------- */
 typedef struct {
     const void *nodes;       /* the node buffer: mesh trees, then the top tree (units.rs:532-595 Layout) */
@@ -129,6 +145,10 @@ typedef struct {
     const void *objects;      /* object table, 3 vec4 an object; the first's w is its power */
     uint64_t object_count;
     const void *energy;       /* lib.rs:1088-1104 energy_lut: 512 vec4; NULL: no shading tables */
+    const void *light_tree;   /* ltree.rs LightTree::nodes, one vec4 a node (gpu.rs:2613 `light_tree`); NULL / 0: none.
+                               * Its emitters are the CDF's first P.ltree.y entries, in leaf order; their triangles'
+                               * codes name their paths (lib.rs:419-441 emitter_order / emitter_codes) */
+    uint64_t light_tree_vec4s;
 } trt_scene_desc;
 
 typedef struct {
@@ -148,7 +168,9 @@ typedef struct {
 
 /* Copies the host buffers to HBM and builds the 4-wide quantised tree from the node buffer. The caller's memory
  * is borrowed for the call only. Refuses (TRT_E_INVALID) trees the reference's checks refuse (units.rs check_mesh /
- * check_top) and records with cutout or glass bits (not supported). */
+ * check_top, ltree.rs LightTree::check), records with cutout or glass bits, and materials that need what this core
+ * does not render (textures, normal maps, glass, alpha cutoffs, vertex colours, unlit or function materials), each
+ * named in trt_last_error. */
 int trt_scene_upload(trt_device *dev, const trt_scene_desc *desc, trt_scene **out);
 int trt_scene_release(trt_scene *scene);
 int trt_scene_stats_get(trt_scene *scene, trt_scene_stats *out);
@@ -240,7 +262,10 @@ int trt_accum_add_fp64(trt_accum *acc, double *d_sum);
 
 /* Samples first_sample .. first_sample + spp - 1 of every pixel added to `acc` (gpu.rs:4177-4196: the sample index
  * is P.size.z), on the device's stream. params: the 624-byte Params block (pathtrace.wgsl:96-140); its size.xy
- * must be the accumulator's. Needs a scene uploaded with its shading tables. */
+ * must be the accumulator's. Needs a scene uploaded with its shading tables. Refuses (TRT_E_INVALID, named) what the
+ * offline `main` would read and this core does not render: an environment map, local lights, fog, volumetric
+ * scattering, procedural geometry, the live light LOD, aerial perspective, a scene background, more than 64
+ * bounces, and light-tree candidates (P.ltree.w) for a scene uploaded without a tree. */
 int trt_render(trt_scene *scene, const void *params, uint32_t first_sample, uint32_t spp, trt_accum *acc);
 /* The rays the device's render launches traced since the last call, which resets them: closest-hit rays (camera and
  * bounce) and shadow rays (sun and emitter visibility). Waits for the device. A measurement with no counterpart in

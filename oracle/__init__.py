@@ -240,24 +240,51 @@ class _RenderScene(C.Structure):
                 ("materials", C.c_void_p), ("material_count", C.c_uint64), ("lights", C.c_void_p),
                 ("light_count", C.c_uint32), ("emitters", C.c_void_p), ("emitter_vec4s", C.c_uint64),
                 ("emitter_tris", C.c_void_p), ("emitter_tri_count", C.c_uint64), ("objects", C.c_void_p),
-                ("object_count", C.c_uint64), ("energy", C.c_void_p)]
+                ("object_count", C.c_uint64), ("energy", C.c_void_p), ("light_tree", C.c_void_p),
+                ("light_tree_count", C.c_uint64)]
 
 
 class RenderScene:
     """The kernel's buffers plus the shading tables, as the oracle's path loop reads them."""
 
     def __init__(self, nodes, leaves, instances, top, units, vertices, indices, rows, materials, lights, emitters,
-                 emitter_tris, objects, energy):
+                 emitter_tris, objects, energy, light_tree=None):
         self.geo = Scene(nodes, leaves, instances, top, units)
+        if light_tree is None:
+            light_tree = np.zeros((0, 4), np.float32)
         self.keep = [np.ascontiguousarray(x) for x in (vertices, indices, rows, materials, lights, emitters,
-                                                       emitter_tris, objects, energy)]
-        v, i, r, m, li, e, et, o, en = self.keep
+                                                       emitter_tris, objects, energy, light_tree)]
+        v, i, r, m, li, e, et, o, en, lt = self.keep
         self._s = _RenderScene(self.geo._s, _ptr(v), len(v), _ptr(i), len(i), _ptr(r), len(r), _ptr(m), len(m),
                                _ptr(li), len(li), _ptr(e), len(e), _ptr(et), len(et), _ptr(o), len(o) // 3,
-                               _ptr(en))
+                               _ptr(en), _ptr(lt) if len(lt) else None, len(lt))
         L = lib()
         L.or_render_sample.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_int]
         L.or_render_sample.restype = C.c_int64
+        L.or_emitter_draw.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.POINTER(C.c_int64)]
+        L.or_emitter_draw.restype = C.c_float
+        L.or_emitter_pmf.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]
+        L.or_emitter_pmf.restype = C.c_float
+
+    def emitter_draw(self, params: np.ndarray, x, n, u) -> tuple[int, float]:
+        """pathtrace.wgsl:547-585 emitter_draw at x (normal n) for uniforms u[3]: (CDF entry, p); entry -1: none."""
+        p = np.ascontiguousarray(params, np.uint32)
+        xs, ns, us = (np.ascontiguousarray(a, np.float32) for a in (x, n, u))
+        e = C.c_int64()
+        pr = lib().or_emitter_draw(C.byref(self._s), _ptr(p), _ptr(xs), _ptr(ns), _ptr(us), C.byref(e))
+        if pr < 0:
+            raise RuntimeError("emitter_draw read outside its tables")
+        return int(e.value), float(pr)
+
+    def emitter_pmf(self, params: np.ndarray, x, n, ei: int) -> float:
+        """pathtrace.wgsl:590-618 emitter_pmf: the probability emitter_draw at x (normal n) picks emitter ei."""
+        p = np.ascontiguousarray(params, np.uint32)
+        xs, ns = (np.ascontiguousarray(a, np.float32) for a in (x, n))
+        pr = lib().or_emitter_pmf(C.byref(self._s), _ptr(p), _ptr(xs), _ptr(ns), ei)
+        if pr < 0:
+            raise RuntimeError("emitter_pmf read outside its tables")
+        return float(pr)
 
     def render(self, params: np.ndarray, first_sample: int, spp: int, accum: np.ndarray | None = None,
                rows=None, threads=0) -> np.ndarray:

@@ -54,6 +54,8 @@ typedef struct {
     const float *objects;       /* vec4s, OBJECT_STRIDE (3) a object; [k*3].w its power */
     uint64_t object_count;
     const float *energy;        /* lib.rs energy_lut: 512 vec4 */
+    const float *light_tree;    /* ltree.rs LightTree::nodes: vec4s (NULL: none) */
+    uint64_t light_tree_count;
 } or_render_scene;
 
 /* One sample (params word 18 = P.size.z) of every pixel of rows [y0, y1), accumulated into `accum` (80 bytes a
@@ -61,6 +63,13 @@ typedef struct {
  * walks that failed (read outside their buffers) plus the table reads whose index was outside its table. */
 int64_t or_render_sample(const or_render_scene *s, const uint32_t *params, uint32_t y0, uint32_t y1, float *accum,
                          int threads);
+
+/* pathtrace.wgsl:547-618 emitter_draw for given uniforms u[3] at the point x with normal n (*entry: the CDF entry it
+ * picks, -1 none; returns its probability p) and emitter_pmf of emitter triangle ei there; -1 when a read fell outside
+ * its table. For tests of the emitter sampling. */
+float or_emitter_draw(const or_render_scene *s, const uint32_t *params, const float *x, const float *n,
+                      const float *u, int64_t *entry);
+float or_emitter_pmf(const or_render_scene *s, const uint32_t *params, const float *x, const float *n, uint32_t ei);
 
 /* denoise.rs:10-74 denoise (a-trous), and the expf it uses here. Images: 3 floats a pixel (variance: 1). */
 float or_expf(float x);

@@ -130,7 +130,7 @@ static inline float pf(const uint32_t *p, int vec, int k) {
 }
 static inline uint32_t pu(const uint32_t *p, int vec, int k) { return p[4 * vec + k]; }
 enum { CAM_POS = 0, CAM_FWD = 1, CAM_RIGHT = 2, CAM_UP = 3, SIZE = 4, COUNTS = 5, ENV = 6, MISC = 7, SKY = 8,
-       COUNTS2 = 9, LIGHT_LOD = 18 };
+       COUNTS2 = 9, LIGHT_LOD = 18, LTREE = 21 };
 
 enum { D_CAMERA = 0, D_BSDF = 1, D_RR = 2, D_SUN = 4, D_SUN_VIS = 5, D_EMIT = 7, D_EMIT_VIS = 8, D_ENV = 9,
        D_ENV_VIS = 10 };
@@ -545,20 +545,130 @@ static const float *emitter_pick_u(const Ctx *c, float u) {
     return emitter_at(c, lo < n - 1u ? lo : n - 1u);
 }
 
+/* ---- the light tree (pathtrace.wgsl:489-618) ------------------------------------------------------------------ */
+#define LT_LEAF 0x80000000u
+#define LT_PATH_BITS 26u
+static inline int lt_on(const Ctx *c) { return pf(c->P, LTREE, 3) > 0.5f; }
+static inline const float *lt_node(const Ctx *c, uint32_t i) {
+    return c->s->light_tree + 4 * ck(c, i, c->s->light_tree_count);
+}
+/* unpack2x16float: IEEE binary16 to binary32, exactly */
+static float half_to_float(uint32_t h) {
+    uint32_t sign = (h & 0x8000u) << 16, e = (h >> 10) & 0x1fu, m = h & 0x3ffu, u;
+    if (e == 0x1fu)
+        u = sign | 0x7f800000u | (m << 13);
+    else if (e == 0u) {
+        float f = (float)m * 5.9604644775390625e-8f; /* m x 2^-24, exact */
+        memcpy(&u, &f, 4);
+        u |= sign;
+    } else
+        u = sign | ((e + 112u) << 23) | (m << 13);
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+/* lt_imp (504-524): power x the best cosine any point of the node's sphere has at x / max(d^2, r^2); the offline
+ * kernel has no approximations, so no node is cut for distance */
+static float lt_imp(const float *nd, V x, V n) {
+    uint32_t a = fb(nd[0]), b = fb(nd[1]);
+    V dv = vsub(v3(half_to_float(a & 0xffffu), half_to_float(a >> 16), half_to_float(b & 0xffffu)), x);
+    float r = half_to_float(b >> 16);
+    float d2 = dot(dv, dv);
+    float r2 = r * r;
+    float cb = 1.0f;
+    if (d2 > r2 && dot(n, n) > 0.0f) {
+        float inv = 1.0f / sqrtf(d2);
+        float cos_t = dot(n, dv) * inv;
+        float sin_b = r * inv;
+        float cos_b = sqrtf(fmaxf(1.0f - sin_b * sin_b, 0.0f));
+        if (cos_t < cos_b) cb = fmaxf(cos_t * cos_b + sqrtf(fmaxf(1.0f - cos_t * cos_t, 0.0f)) * sin_b, 0.0f);
+    }
+    return nd[2] * cb / fmaxf(d2, r2);
+}
+/* emitter_pick_range (527-540) */
+static const float *emitter_pick_range(const Ctx *c, uint32_t lo, uint32_t hi, float r) {
+    float p0 = lo > 0u ? emitter_at(c, (lo > 1u ? lo : 1u) - 1u)[1] : 0.0f;
+    float p1 = emitter_at(c, hi - 1u)[1];
+    float u = fminf(p0 + r * (p1 - p0), p1);
+    uint32_t k = (uint32_t)(u * (float)EMITTER_GUIDE);
+    if (k > EMITTER_GUIDE - 1u) k = EMITTER_GUIDE - 1u;
+    uint32_t l = emitter_guide(c, k), h = emitter_guide(c, k + 1u);
+    while (l < h) {
+        uint32_t mid = (l + h) / 2u;
+        if (emitter_at(c, mid)[1] < u)
+            l = mid + 1u;
+        else
+            h = mid;
+    }
+    if (l < lo) l = lo;
+    if (l > hi - 1u) l = hi - 1u;
+    return emitter_at(c, l);
+}
+
 typedef struct {
     const float *e;
     float p, w;
 } EPick;
-static EPick emitter_draw(Ctx *c, uint32_t rd) {
-    float r3[3];
-    or_rng3(&c->g, rd, r3);
+/* emitter_draw (547-585): without the tree the CDF; with it the tail by its share, else the tree descended by
+ * importance at (x, n) to a leaf and one of its entries by weight */
+static EPick emitter_draw_u(const Ctx *c, V x, V n, const float *r3) {
     const float *tot = emitter_totals(c);
     EPick pk = {NULL, 0.0f, 0.0f};
     if (!(tot[0] > 0.0f)) return pk;
-    pk.e = emitter_pick_u(c, r3[0]);
-    pk.p = pk.e[2] / tot[0];
-    pk.w = pk.e[2];
+    if (!lt_on(c)) {
+        pk.e = emitter_pick_u(c, r3[0]);
+        pk.p = pk.e[2] / tot[0];
+        pk.w = pk.e[2];
+        return pk;
+    }
+    if (r3[0] < tot[1]) {
+        pk.e = emitter_pick_range(c, pu(c->P, LTREE, 1), emitter_count(c), r3[1]);
+        pk.p = pk.e[2] / tot[0];
+        pk.w = pk.e[2];
+        return pk;
+    }
+    float p = 1.0f - tot[1];
+    float u = r3[1];
+    uint32_t w = fb(lt_node(c, 0u)[3]);
+    for (uint32_t k = 0u; k <= LT_PATH_BITS && (w & LT_LEAF) == 0u; k++) {
+        const float *l = lt_node(c, w), *r = lt_node(c, w + 1u);
+        float il = lt_imp(l, x, n), ir = lt_imp(r, x, n);
+        if (!(il + ir > 0.0f)) return pk;
+        float pl = il / (il + ir);
+        if (u < pl) {
+            w = fb(l[3]);
+            p *= pl;
+            u = u / pl;
+        } else {
+            w = fb(r[3]);
+            p *= 1.0f - pl;
+            u = (u - pl) / (1.0f - pl);
+        }
+        u = fminf(fmaxf(u, 0.0f), 0.99999994f);
+    }
+    if ((w & LT_LEAF) == 0u) return pk;
+    uint32_t lo = w & 0x1fffffffu, hi = lo + ((w >> 29) & 3u) + 1u;
+    float wsum = 0.0f;
+    for (uint32_t j = lo; j < hi; j++) wsum += emitter_at(c, j)[2];
+    float t = r3[2] * wsum;
+    const float *e = emitter_at(c, hi - 1u);
+    for (uint32_t j = lo; j < hi; j++) {
+        const float *cc = emitter_at(c, j);
+        if (t < cc[2]) {
+            e = cc;
+            break;
+        }
+        t -= cc[2];
+    }
+    pk.e = e;
+    pk.p = p * e[2] / wsum;
+    pk.w = e[2];
     return pk;
+}
+static EPick emitter_draw(Ctx *c, V x, V n, uint32_t rd) {
+    float r3[3];
+    or_rng3(&c->g, rd, r3);
+    return emitter_draw_u(c, x, n, r3);
 }
 
 typedef struct {
@@ -578,12 +688,32 @@ static Em emitter_tri_at(const Ctx *c, uint32_t i) {
     e.flags = fb(r[13]);
     return e;
 }
-static float emitter_pmf(const Ctx *c, const Em *em) {
+/* emitter_pmf (590-618): the probability emitter_draw at (x, n) picks emitter `ei`: one of the tree's (its code names
+ * its path) by the same choices down the path and its weight in its leaf; any other by its entry's weight */
+static float emitter_pmf(const Ctx *c, uint32_t ei, const Em *em, V x, V n) {
     const float *tot = emitter_totals(c);
     if (!(tot[0] > 0.0f)) return 0.0f;
     uint32_t code = em->code & ~0x80000000u;
-    if ((code >> 26) == 0u) return emitter_at(c, code)[2] / tot[0];
-    return 0.0f; /* (light-tree codes: no tree here) */
+    if ((code >> LT_PATH_BITS) == 0u) return emitter_at(c, code)[2] / tot[0];
+    uint32_t depth = (code >> LT_PATH_BITS) - 1u;
+    float p = 1.0f - tot[1];
+    uint32_t w = fb(lt_node(c, 0u)[3]);
+    for (uint32_t k = 0u; k < depth; k++) {
+        const float *l = lt_node(c, w), *r = lt_node(c, w + 1u);
+        float il = lt_imp(l, x, n), ir = lt_imp(r, x, n);
+        if (!(il + ir > 0.0f)) return 0.0f;
+        uint32_t right = (code >> k) & 1u;
+        p *= (right == 1u ? ir : il) / (il + ir);
+        w = fb(right == 1u ? r[3] : l[3]);
+    }
+    uint32_t lo = w & 0x1fffffffu, hi = lo + ((w >> 29) & 3u) + 1u;
+    float wsum = 0.0f, we = 0.0f;
+    for (uint32_t j = lo; j < hi; j++) {
+        const float *cc = emitter_at(c, j);
+        wsum += cc[2];
+        if (fb(cc[0]) == ei) we = cc[2];
+    }
+    return p * we / wsum;
 }
 
 static V emitter_radiance(const Ctx *c, const Em *em) {
@@ -607,7 +737,7 @@ static ES sample_emitter_ris(Ctx *c, V x, V n, uint32_t m, const Surface *sf, co
     uint32_t pick_i = 0;
     for (uint32_t k = 0; k < m; k++) {
         uint32_t dk = or_rng_sub(rd, k);
-        EPick pk = emitter_draw(c, dk);
+        EPick pk = emitter_draw(c, x, n, dk);
         float bb[3];
         or_rng3(&c->g, or_rng_sub(dk, 1u), bb);
         float b1 = bb[0], b2 = bb[1];
@@ -660,9 +790,9 @@ static ES sample_emitter_ris(Ctx *c, V x, V n, uint32_t m, const Surface *sf, co
     return r;
 }
 
-static ES sample_emitter(Ctx *c, V x, uint32_t rd) {
+static ES sample_emitter(Ctx *c, V x, V n, uint32_t rd) {
     ES r = {v3(0, 0, 0), v3(0, 1, 0), 0.0f, 0.0f, 0.0f};
-    EPick pk = emitter_draw(c, rd);
+    EPick pk = emitter_draw(c, x, n, rd);
     if (pk.p <= 0.0f) return r;
     Em em = emitter_tri_at(c, fb(pk.e[0]));
     float bb[2];
@@ -739,13 +869,17 @@ static V direct_light(Ctx *c, const Surface *s, const Lobes *lb, V v) {
     if (emitters > 0u) {
         ES es;
         int seen = c->g.depth == 0u || (c->g.depth == 1u && c->mirror_path);
-        if (seen)
+        /* with the tree its candidates are already near the point: P.ltree.w of them at camera hits, one elsewhere */
+        if (lt_on(c))
+            es = sample_emitter_ris(c, o, ne, c->g.depth == 0u ? (uint32_t)pf(c->P, LTREE, 3) : 1u, s, lb, v,
+                                    or_rng_dim(&c->g, D_EMIT));
+        else if (seen)
             es = sample_emitter_ris(c, o, ne, c->g.depth == 0u ? EMIT_RIS_M : EMIT_RIS_M / 2u, s, lb, v,
                                     or_rng_dim(&c->g, D_EMIT));
         else if (pf(c->P, LIGHT_LOD, 3) > 0.0f)
             es = sample_emitter_ris(c, o, ne, 2u, s, lb, v, or_rng_dim(&c->g, D_EMIT));
         else
-            es = sample_emitter(c, o, or_rng_dim(&c->g, D_EMIT));
+            es = sample_emitter(c, o, ne, or_rng_dim(&c->g, D_EMIT));
         if (es.pdf_w > 0.0f) {
             V f = eval_bsdf(s, lb, v, es.l);
             if (any_pos(f)) {
@@ -771,7 +905,7 @@ static float emit_mis(Ctx *c, const or_hit *hit, V d, float pdf) {
     V cr = cross(em.e1, em.e2);
     float area = 0.5f * len(cr);
     float cos_l = fabsf(dot(normalize(cr), d));
-    float sel = emitter_pmf(c, &em);
+    float sel = emitter_pmf(c, entry, &em, c->mis_x, c->mis_n);
     if (!(sel > 0.0f)) return 1.0f;
     return power_heuristic(pdf, sel / fmaxf(area, 1e-12f) * hit->t * hit->t / fmaxf(cos_l, 1e-6f));
 }
@@ -798,6 +932,14 @@ static void path(Ctx *c, uint32_t px, uint32_t py, float *pix) {
     float tanh = pf(P, CAM_POS, 3), aspect = pf(P, CAM_FWD, 3);
     V d = normalize(vadd(vadd(fwd, vs(right, ndx * tanh * aspect)), vs(up, ndy * tanh)));
     V o = v3(pf(P, CAM_POS, 0), pf(P, CAM_POS, 1), pf(P, CAM_POS, 2));
+    if (pf(P, CAM_RIGHT, 3) > 0.0f) { /* the thin lens, pathtrace.wgsl:4017-4023 */
+        V focus = vadd(o, vs(d, pf(P, CAM_UP, 3) / dot(d, fwd)));
+        float r = sqrtf(cam[2]) * pf(P, CAM_RIGHT, 3);
+        float phi = TWO_PI * cam[3], sp, cp;
+        or_sincos(phi, &sp, &cp);
+        o = vadd(o, vadd(vs(right, r * cp), vs(up, r * sp)));
+        d = normalize(vsub(focus, o));
+    }
 
     V radiance = v3(0, 0, 0), thr = v3(1, 1, 1), albedo = v3(0, 0, 0), normal = v3(0, 0, 0);
     int specular_bounce = 1, behind_glass = 0, bent = 0, straight = 1;
@@ -888,4 +1030,30 @@ int64_t or_render_sample(const or_render_scene *s, const uint32_t *params, uint3
     }
     (void)threads;
     return failed + oob;
+}
+
+/* Test hooks: the emitter draw for given uniforms u[3] at (x, n) (*entry: its CDF entry, -1 none; returns its p), and
+ * the pmf of emitter `ei` there (emitter_draw, emitter_pmf above), under `params`. */
+static Ctx hook_ctx(const or_render_scene *s, const uint32_t *params, int64_t *oob) {
+    Ctx c;
+    memset(&c, 0, sizeof(c));
+    c.s = s;
+    c.P = params;
+    c.oob = oob;
+    return c;
+}
+float or_emitter_draw(const or_render_scene *s, const uint32_t *params, const float *x, const float *n,
+                      const float *u, int64_t *entry) {
+    int64_t oob = 0;
+    Ctx c = hook_ctx(s, params, &oob);
+    EPick pk = emitter_draw_u(&c, v3(x[0], x[1], x[2]), v3(n[0], n[1], n[2]), u);
+    *entry = pk.e ? (int64_t)((pk.e - s->emitters) / 4) : -1;
+    return oob ? -1.0f : pk.p;
+}
+float or_emitter_pmf(const or_render_scene *s, const uint32_t *params, const float *x, const float *n, uint32_t ei) {
+    int64_t oob = 0;
+    Ctx c = hook_ctx(s, params, &oob);
+    Em em = emitter_tri_at(&c, ei);
+    float p = emitter_pmf(&c, ei, &em, v3(x[0], x[1], x[2]), v3(n[0], n[1], n[2]));
+    return oob ? -1.0f : p;
 }

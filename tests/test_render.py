@@ -284,6 +284,45 @@ def test_cornell_matches_the_oracle(device):
 
 
 @pytest.mark.gpu
+def test_a_thin_lens_matches_the_oracle(device):
+    """camera_ray's thin lens (pathtrace.wgsl:4017-4023: lens radius cam_right.w, focus distance cam_up.w)."""
+    print("cornell through a thin lens 96x96 x 8 spp", flush=True)
+    rs, cam = R.cornell()
+    p = rs.params(96, 96, cam, seed=3, max_bounces=4, rr_depth=3, lens_radius=0.06, focus=1.6)
+    want = oracle_scene(rs).render(p, 0, 8)
+    got = gpu_render(device, rs, p, 0, 8)
+    print(f"lens: rel RMSE {rel_rmse(got, want):.2e}", flush=True)
+    assert rel_rmse(got, want) <= 1e-3
+    pin = oracle_scene(rs).render(rs.params(96, 96, cam, seed=3, max_bounces=4, rr_depth=3), 0, 8)
+    assert rel_rmse(pin, want) > 1e-2  # the lens changes the image
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("which", ["lit floor", "soup"])
+def test_light_tree_scenes_match_the_oracle(device, which):
+    """Scenes of 256 emitters or more sample them through the light tree (ltree.rs; lt_imp, emitter_draw and
+    emitter_pmf, pathtrace.wgsl:504-618; six tree candidates at camera hits): the GPU's accumulator is the oracle's.
+    The soup is the precision workload's generator at 300,000 triangles (15,000 emissive)."""
+    from threert_amd import serve
+    sys.path.insert(0, __file__.rsplit("/", 1)[0])
+    import test_light_tree as LT
+
+    if which == "soup":
+        rs, cam, sky = serve.soup_scene(300_000, 21)
+        p = rs.params(160, 96, cam, seed=5, max_bounces=4, rr_depth=3, sky=sky)
+    else:
+        rs, cam = LT.lit_floor()
+        p = rs.params(96, 64, cam, seed=5, max_bounces=3, rr_depth=2)
+    assert rs.tree_entries >= 256 and p.view(np.float32)[87] == 6.0
+    print(f"{which}: {rs.tree_entries} emitters in the tree", flush=True)
+    want = oracle_scene(rs).render(p, 0, 4, threads=8)
+    got = gpu_render(device, rs, p, 0, 4)
+    same = (got.view(np.uint32) == want.view(np.uint32)).all(axis=2).mean()
+    print(f"{which}: {same:.4f} of the pixels bit-equal, rel RMSE {rel_rmse(got, want):.2e}", flush=True)
+    assert rel_rmse(got, want) <= 1e-3 and want[..., :3].sum() > 0
+
+
+@pytest.mark.gpu
 def test_sunlit_instances_match_the_oracle(device):
     print("small city 80x48 x 2 spp", flush=True)
     rs, cam, sky = R.city(seed=3, meshes=6, instances=40, extent=60.0)
@@ -442,9 +481,11 @@ def test_the_serving_job_that_met_a_tie_matches_the_oracle(device):
 
 @pytest.mark.gpu
 def test_what_the_library_does_not_render_is_refused(device):
-    """Records with cutout or glass sections (scene.rs INST_CUTOUT 16, INST_GLASS 32 in the bits word), an
-    environment map (Params.env.x) and more than 64 bounces are refused with TRT_E_INVALID and a message that names
-    them; the device still renders the unchanged scene afterwards."""
+    """Records with cutout or glass sections (scene.rs INST_CUTOUT 16, INST_GLASS 32 in the bits word); materials with
+    textures, normal maps, unlit or function flags, transmission textures, vertex colours, alpha cutoffs or glass;
+    and Params asking for an environment map, more than 64 bounces, local lights, fog, volumetric scattering, the live
+    light LOD, procedural geometry, aerial perspective, a background or light-tree candidates without a tree: each
+    refused with TRT_E_INVALID and a message that names it. The device still renders the unchanged scene afterwards."""
     import copy
 
     import threert_amd as T
@@ -458,16 +499,33 @@ def test_what_the_library_does_not_render_is_refused(device):
         with pytest.raises(T.ThreeRTError) as e:
             R.upload(device, bad)
         assert e.value.code == T.TRT_E_INVALID and "cutout or glass" in str(e.value)
+    # materials that need what the kernel does not read (GpuMaterial words, scene.rs:480-491)
+    as_int = lambda v: np.array([v], np.int32).view(np.float32)[0]  # noqa: E731
+    for word, value, name in ((12, as_int(0), "base-colour texture"), (13, as_int(2), "normal map"),
+                              (16, as_int(1), "emissive texture"), (18, as_int(4), "unlit"),
+                              (19, as_int(1), "material function"), (22, 1.0, "transmission texture"),
+                              (23, 1.0, "vertex colours"), (27, 0.5, "alpha cutoff"), (28, 1.0, "glass")):
+        print(f"refusal: material word {word} ({name})", flush=True)
+        bad = copy.deepcopy(rs)
+        bad.materials[1, word] = value
+        with pytest.raises(T.ThreeRTError) as e:
+            R.upload(device, bad)
+        assert e.value.code == T.TRT_E_INVALID and name in str(e.value), str(e.value)
     sc = R.upload(device, rs)
     acc = R.Accum(device, 16, 16)
-    for what, change in (("environment map", lambda p: p.view(np.float32).__setitem__(24, 1.0)),
-                         ("more than 64 bounces", lambda p: p.__setitem__(22, 65))):
+    f32 = lambda k, v: (lambda p: p.view(np.float32).__setitem__(k, v))  # noqa: E731
+    for what, change in (("environment map", f32(24, 1.0)), ("more than 64 bounces", lambda p: p.__setitem__(22, 65)),
+                         ("local lights", lambda p: p.__setitem__(20, 1)), ("fog", f32(44, 0.01)),
+                         ("volumetric scattering", f32(52, 0.5)), ("live light LOD", f32(75, 10.0)),
+                         ("procedural geometry", lambda p: p.__setitem__(89, 1)),
+                         ("aerial perspective", f32(92, 1.0)), ("scene background", f32(99, 1.0)),
+                         ("light-tree candidates", f32(87, 6.0))):
         print(f"refusal: {what}", flush=True)
         p = rs.params(16, 16, cam)
         change(p)
         with pytest.raises(T.ThreeRTError) as e:
             R.render(sc, p, 0, 1, acc)
-        assert e.value.code == T.TRT_E_INVALID and what.split()[0] in str(e.value)
+        assert e.value.code == T.TRT_E_INVALID and what.split()[0] in str(e.value), str(e.value)
     R.render(sc, rs.params(16, 16, cam, max_bounces=64), 0, 2, acc)
     device.sync()
     c = acc.resolve()[0]

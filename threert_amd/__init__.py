@@ -46,6 +46,7 @@ EXPORTS = [
     "trt_accum_resolve_device", "trt_accum_add_fp64", "trt_device_set_build", "trt_scene_wide_read",
     "trt_unet_create", "trt_unet_release", "trt_unet_param_count", "trt_unet_params_set", "trt_unet_params_get",
     "trt_unet_denoise", "trt_unet_gradients", "trt_unet_train_step", "trt_trace_rays_device_noalloc",
+    "trt_build_light_tree", "trt_light_tree_free",
 ]
 
 
@@ -70,7 +71,13 @@ class _SceneDesc(C.Structure):
                 ("materials", C.c_void_p), ("material_count", C.c_uint64), ("lights", C.c_void_p),
                 ("light_count", C.c_uint64), ("emitters", C.c_void_p), ("emitter_vec4s", C.c_uint64),
                 ("emitter_tris", C.c_void_p), ("emitter_tri_count", C.c_uint64), ("objects", C.c_void_p),
-                ("object_count", C.c_uint64), ("energy", C.c_void_p)]
+                ("object_count", C.c_uint64), ("energy", C.c_void_p), ("light_tree", C.c_void_p),
+                ("light_tree_vec4s", C.c_uint64)]
+
+
+class _LightTree(C.Structure):
+    _fields_ = [("nodes", C.c_void_p), ("node_count", C.c_uint64), ("order", C.c_void_p), ("code", C.c_void_p),
+                ("entry_count", C.c_uint64), ("power", C.c_float)]
 
 
 class _SceneStats(C.Structure):
@@ -113,6 +120,7 @@ def lib():
             "trt_trace_check_fp64": [P, P, P, U64, P],
             "trt_render_ray_counts": [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
             "trt_device_set_build": [P, U32], "trt_scene_wide_read": [P, U32, P, U64],
+            "trt_build_light_tree": [P, P, P, U64, P],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -124,6 +132,8 @@ def lib():
         L.trt_unit_record.restype = None
         L.trt_tree_free.argtypes = [P]
         L.trt_tree_free.restype = None
+        L.trt_light_tree_free.argtypes = [P]
+        L.trt_light_tree_free.restype = None
         _lib = L
     return _lib
 
@@ -187,6 +197,35 @@ def top_tree(boxes: np.ndarray, records: np.ndarray):
     t = _Tree()
     _check(lib().trt_build_top_tree(_ptr(boxes), _ptr(records), len(boxes), C.byref(t)))
     return _take_tree(t)
+
+
+class LightTree:
+    """ltree.rs LightTree as the kernel and the emitter table read it: nodes (vec4s), the tree's emitters in leaf
+    order, each entry's path code, the root's power."""
+
+    def __init__(self, nodes: np.ndarray, order: np.ndarray, code: np.ndarray, power: float):
+        self.nodes, self.order, self.code, self.power = nodes, order, code, power
+
+
+def build_light_tree(verts: np.ndarray, centroids: np.ndarray, weights: np.ndarray):
+    """ltree.rs:409-459 build over emitter triangles (world vertices n x 3 x 3, centroids n x 3, weights n): a
+    LightTree, or None below 256 emitters that weigh something (the CDF alone)."""
+    verts = np.ascontiguousarray(verts, np.float32).reshape(-1, 9)
+    centroids = np.ascontiguousarray(centroids, np.float32).reshape(-1, 3)
+    weights = np.ascontiguousarray(weights, np.float32).reshape(-1)
+    n = len(weights)
+    assert len(verts) == n and len(centroids) == n
+    t = _LightTree()
+    _check(lib().trt_build_light_tree(_ptr(verts), _ptr(centroids), _ptr(weights), n, C.byref(t)))
+    try:
+        if t.node_count == 0:
+            return None
+        nodes = np.ctypeslib.as_array(C.cast(t.nodes, C.POINTER(C.c_float)), (t.node_count, 4)).copy()
+        order = np.ctypeslib.as_array(C.cast(t.order, C.POINTER(C.c_uint32)), (t.entry_count,)).copy()
+        code = np.ctypeslib.as_array(C.cast(t.code, C.POINTER(C.c_uint32)), (t.entry_count,)).copy()
+        return LightTree(nodes, order, code, float(np.float32(t.power)))
+    finally:
+        lib().trt_light_tree_free(C.byref(t))
 
 
 def unit_box(world_colmajor: np.ndarray, lo, hi) -> np.ndarray:

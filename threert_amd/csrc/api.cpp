@@ -84,7 +84,8 @@ struct trt_scene {
          *wleaves = nullptr;
     // shading tables (trt_render)
     void *vertices = nullptr, *indices = nullptr, *rows = nullptr, *materials = nullptr, *lights = nullptr,
-         *emitters = nullptr, *emitter_tris = nullptr, *objects = nullptr, *energy = nullptr;
+         *emitters = nullptr, *emitter_tris = nullptr, *objects = nullptr, *energy = nullptr, *light_tree = nullptr;
+    uint64_t light_tree_count = 0;
     bool shading = false;
     bool coat = false;  // a material has a clearcoat (layer.z > 0: the reference's F_COAT, gpu.rs features_of)
     uint64_t light_count = 0, emitter_entries = 0, emitter_tri_count = 0;
@@ -222,10 +223,10 @@ int trt_device_sync(trt_device *d) {
         static const char *table[] = {"instance", "material", "light", "emitter entry", "row", "object",
                                       "triangle indices", "vertex", "energy table", "emitter triangle (RIS candidate)",
                                       "leaf", "emitter triangle (RIS pick)", "emitter triangle (single draw)",
-                                      "emitter triangle (a hit's own)"};
+                                      "emitter triangle (a hit's own)", "light tree node"};
         std::string where;
         if (g[1]) {
-            const char *name = g[1] <= 10 ? what[g[1]] : (g[1] >= 16 && g[1] <= 29 ? table[g[1] - 16] : "?");
+            const char *name = g[1] <= 10 ? what[g[1]] : (g[1] >= 16 && g[1] <= 30 ? table[g[1] - 16] : "?");
             where = std::string(" (first: ") + name + " index " +
                     std::to_string((uint64_t(g[3]) << 32) | g[2]) + " of " + std::to_string(g[4]) + "; context " +
                     std::to_string(g[5]) + " " + std::to_string(g[6]) + " " + std::to_string(g[7]) + ")";
@@ -373,12 +374,85 @@ int trt_scene_release(trt_scene *s) {
     hipFree(s->wunits);
     hipFree(s->wleaves);
     for (void *p : {s->vertices, s->indices, s->rows, s->materials, s->lights, s->emitters, s->emitter_tris,
-                    s->objects, s->energy})
+                    s->objects, s->energy, s->light_tree})
         hipFree(p);
     trt_device *d = s->dev;
     delete s;
     device_unref(d);
     return ok();
+}
+
+// A material the path loop would render without what it asks for is refused, named: the reference's `surface`
+// (pathtrace.wgsl:1320-1425) reads these only under the F_TEX, F_VCOLOR, F_FUNC and F_UNLIT features, and its glass
+// and cutout paths (1542-1640, 1300-1310), none of which this core has. Words of the 72-word GpuMaterial
+// (scene.rs:480-491): tex0 12-15 and tex1 16-19 (texture slots, -1 none; tex1.z the flags, tex1.w a function + 1),
+// uv.z 22 (transmission texture + 1), uv.w 23 (vertex colours), uv2.w 27 (alpha cutoff), layer.x 28 (transmission).
+static bool material_rendered(const float *m, uint64_t count, std::string *why) {
+    constexpr int32_t kMatUnlit = 4;  // pathtrace.wgsl:54 MAT_UNLIT
+    static const char *slot[8] = {"a base-colour texture", "a normal map", "a roughness texture",
+                                  "a metalness texture", "an emissive texture", "an occlusion texture", "", ""};
+    for (uint64_t k = 0; k < count; k++) {
+        const float *f = m + 72 * k;
+        int32_t w[8];
+        std::memcpy(w, f + 12, sizeof(w));
+        const char *what = nullptr;
+        for (int j = 0; j < 6 && !what; j++)
+            if (w[j] >= 0) what = slot[j];
+        if (!what && w[6] == kMatUnlit) what = "an unlit material";
+        if (!what && w[7] > 0) what = "a material function";
+        if (!what && f[22] > 0.5f) what = "a transmission texture";
+        if (!what && f[23] > 0.0f) what = "vertex colours";
+        if (!what && f[27] > 0.0f) what = "an alpha cutoff";
+        if (!what && f[28] > 0.0f) what = "glass (transmission, layer.x)";
+        if (what) {
+            *why = "material " + std::to_string(k) + " needs " + what + ", which this core does not render";
+            return false;
+        }
+    }
+    return true;
+}
+
+// ltree.rs:162-233 LightTree::check, as the kernel reads the tree (pathtrace.wgsl:547-618): from node 0 every
+// interior node's children (its word and the next) are after it and inside the tree, every node is reached once and
+// no deeper than 26 levels, every leaf's 1-4 entries lie among the CDF's entries; and every emitter triangle whose
+// code has depth bits follows them through interior nodes to a leaf.
+static bool light_tree_ok(const trt_scene_desc *d, uint64_t entries, std::string *why) {
+    const uint64_t n = d->light_tree_vec4s;
+    const auto *nd = static_cast<const uint32_t *>(d->light_tree);
+    constexpr uint32_t kLeaf = 0x80000000u;
+    std::vector<uint8_t> seen(n, 0);
+    std::vector<std::pair<uint64_t, uint32_t>> st{{0, 0}};
+    while (!st.empty()) {
+        const auto [i, depth] = st.back();
+        st.pop_back();
+        if (seen[i]++) return *why = "light tree node " + std::to_string(i) + " is reached twice", false;
+        if (depth > 26) return *why = "light tree deeper than 26 levels", false;
+        const uint32_t w = nd[4 * i + 3];
+        if (w & kLeaf) {
+            if ((w & 0x1fffffffu) + ((w >> 29) & 3u) + 1u > entries)
+                return *why = "a light tree leaf names entries past the emitter table", false;
+        } else {
+            if (w <= i || uint64_t(w) + 1 >= n)
+                return *why = "light tree node " + std::to_string(i) + "'s children are not after it in the tree", false;
+            st.push_back({w, depth + 1});
+            st.push_back({uint64_t(w) + 1, depth + 1});
+        }
+    }
+    for (uint64_t i = 0; i < n; i++)
+        if (!seen[i]) return *why = "light tree node " + std::to_string(i) + " is not reached", false;
+    const auto *et = static_cast<const uint32_t *>(d->emitter_tris);
+    for (uint64_t k = 0; k < d->emitter_tri_count; k++) {
+        const uint32_t code = et[16 * k + 11] & 0x7fffffffu;
+        if ((code >> 26) == 0) continue;
+        uint64_t i = 0;
+        for (uint32_t level = 0; level + 1 < (code >> 26); level++) {
+            const uint32_t w = nd[4 * i + 3];
+            if (w & kLeaf) return *why = "an emitter's light-tree code goes past a leaf", false;
+            i = uint64_t(w) + ((code >> level) & 1u);
+        }
+        if (!(nd[4 * i + 3] & kLeaf)) return *why = "an emitter's light-tree code does not end at a leaf", false;
+    }
+    return true;
 }
 
 // The shading tables' indices, checked once so the kernel can follow them: every unit's triangles' vertices and
@@ -401,6 +475,7 @@ static bool check_shading(const trt_scene_desc *d, std::string *why) {
         *why = "an emitter table shorter than its guide and header";
         return false;
     }
+    if (!material_rendered(static_cast<const float *>(d->materials), d->material_count, why)) return false;
     const auto *inst = static_cast<const trt::Instance *>(d->instances);
     const auto *leaves = static_cast<const trt::Leaf *>(d->leaves);
     const auto *nodes = static_cast<const trt::Node *>(d->nodes);
@@ -459,6 +534,10 @@ static bool check_shading(const trt_scene_desc *d, std::string *why) {
         }
     }
     const uint64_t entries = d->emitter_vec4s ? d->emitter_vec4s - kGuideVec4s - 1 : 0;
+    if (d->light_tree_vec4s && (!d->light_tree || !light_tree_ok(d, entries, why))) {
+        if (!d->light_tree) *why = "light tree nodes without a buffer";
+        return false;
+    }
     const auto *em = static_cast<const float *>(d->emitters);
     for (uint64_t k = 0; k < entries; k++)
         if (trt::to_bits(em[4 * k]) >= d->emitter_tri_count) {
@@ -468,7 +547,9 @@ static bool check_shading(const trt_scene_desc *d, std::string *why) {
     const auto *et = static_cast<const uint32_t *>(d->emitter_tris);
     for (uint64_t k = 0; k < d->emitter_tri_count; k++) {
         const uint32_t tri = et[16 * k + 3], in = et[16 * k + 7], part = et[16 * k + 12], code = et[16 * k + 11];
-        if (tri >= d->leaf_count || in >= d->instance_count || (code & 0x7fffffffu) >= (entries ? entries : 1)) {
+        const bool tree_code = ((code & 0x7fffffffu) >> 26) != 0 && d->light_tree_vec4s;
+        if (tri >= d->leaf_count || in >= d->instance_count ||
+            (!tree_code && (code & 0x7fffffffu) >= (entries ? entries : 1))) {
             *why = "an emitter triangle names a triangle, a record or an entry that is not there";
             return false;
         }
@@ -542,7 +623,8 @@ int trt_scene_upload(trt_device *d, const trt_scene_desc *desc, trt_scene **out)
     if (desc->energy && (desc->vertex_count >= (uint64_t{1} << 32) || desc->index_count >= (uint64_t{1} << 32) ||
                          desc->row_count >= (uint64_t{1} << 32) || desc->material_count >= (uint64_t{1} << 32) ||
                          desc->light_count >= (uint64_t{1} << 32) || desc->emitter_vec4s >= (uint64_t{1} << 32) ||
-                         desc->emitter_tri_count >= (uint64_t{1} << 32) || desc->object_count >= (uint64_t{1} << 32)))
+                         desc->emitter_tri_count >= (uint64_t{1} << 32) || desc->object_count >= (uint64_t{1} << 32) ||
+                         desc->light_tree_vec4s >= (uint64_t{1} << 29)))
         return fail(TRT_E_INVALID, "trt_scene_upload: a shading table of 2^32 records or more");
     const auto *nodes = static_cast<const trt::Node *>(desc->nodes);
     const auto *inst = static_cast<const trt::Instance *>(desc->instances);
@@ -654,6 +736,8 @@ int trt_scene_upload(trt_device *d, const trt_scene_desc *desc, trt_scene **out)
         if (!rc) rc = upload(d, &s->emitter_tris, desc->emitter_tris, desc->emitter_tri_count * 64, &total);
         if (!rc) rc = upload(d, &s->objects, desc->objects, desc->object_count * 48, &total);
         if (!rc) rc = upload(d, &s->energy, desc->energy, 512 * 16, &total);
+        if (!rc) rc = upload(d, &s->light_tree, desc->light_tree, desc->light_tree_vec4s * 16, &total);
+        s->light_tree_count = desc->light_tree_vec4s;
     }
     if (rc) {
         std::string keep = g_error;
@@ -932,12 +1016,28 @@ static int job_args(const trt_render_job &j, trt::RenderArgs &a, std::string *wh
     std::memcpy(a.params, j.params, sizeof(a.params));
     const uint32_t *P = a.params;
     if (P[16] != acc->w || P[17] != acc->h) return *why = "P.size is not the accumulator's", TRT_E_INVALID;
+    if (P[20] > P[36]) return *why = "local lights (P.counts.x > P.counts2.x) are not supported", TRT_E_INVALID;
     if (P[20] > s->light_count || P[36] > P[20] || P[21] > s->emitter_entries)
         return *why = "P.counts names more lights or emitters than the scene has", TRT_E_INVALID;
     if (P[21] > 0 && P[21] != s->emitter_entries)
         return *why = "P.counts.y is not the emitter table's entry count", TRT_E_INVALID;
     if (P[22] > 64) return *why = "more than 64 bounces", TRT_E_INVALID;
+    // What the offline `main` would read and this core does not render (Params, pathtrace.wgsl:96-140), refused by
+    // name rather than rendered without it.
     if (trt::from_bits(P[24]) > 0.5f) return *why = "environment maps are not supported", TRT_E_INVALID;
+    if (trt::from_bits(P[44]) > 0.0f) return *why = "fog (P.fog.x) is not supported", TRT_E_INVALID;
+    if (trt::from_bits(P[52]) > 0.0f) return *why = "volumetric scattering (P.vol.x) is not supported", TRT_E_INVALID;
+    if (trt::from_bits(P[75]) > 0.0f) return *why = "the live light LOD (P.light_lod.w) is not supported", TRT_E_INVALID;
+    if (P[89] > 0) return *why = "procedural geometry (P.procs.y) is not supported", TRT_E_INVALID;
+    if (trt::from_bits(P[92]) > 0.0f)
+        return *why = "aerial perspective (P.aerial.x) is not supported", TRT_E_INVALID;
+    if (trt::from_bits(P[99]) > 0.5f)
+        return *why = "a scene background (P.background.w) is not supported", TRT_E_INVALID;
+    if (trt::from_bits(P[87]) > 0.5f && !s->light_tree_count)
+        return *why = "light-tree candidates (P.ltree.w) for a scene uploaded without a light tree", TRT_E_INVALID;
+    if (trt::from_bits(P[87]) > 0.5f && (P[85] > s->emitter_entries || trt::from_bits(P[87]) > 64.0f))
+        return *why = "P.ltree names more tree entries than the emitter table has, or more than 64 candidates",
+               TRT_E_INVALID;
     a.geo.wnodes = static_cast<const uint4 *>(s->wnodes);
     a.geo.wunits = static_cast<const float4 *>(s->wunits);
     a.geo.leaves = static_cast<const float4 *>(s->leaves);
@@ -961,6 +1061,8 @@ static int job_args(const trt_render_job &j, trt::RenderArgs &a, std::string *wh
     a.emitter_tris = static_cast<const float4 *>(s->emitter_tris);
     a.objects = static_cast<const float4 *>(s->objects);
     a.energy = static_cast<const float4 *>(s->energy);
+    a.light_tree = static_cast<const float4 *>(s->light_tree);
+    a.light_tree_count = static_cast<uint32_t>(s->light_tree_count);
     a.accum = static_cast<float4 *>(acc->buf);
     a.vertex_count = static_cast<uint32_t>(s->vertex_count);
     a.tri_count = static_cast<uint32_t>(s->tri_count);

@@ -8,8 +8,9 @@
 // sample_bsdf 1848-1884; env_radiance / env_pdf / sample_env 1895-1930 (constant sky); power_heuristic, luminance,
 // offset_origin 1943-1960; sky_mis / sun_disks 1963-1988; emit_mis 2009-2024; emitter_radiance /
 // sample_emitter_ris / sample_emitter 2269-2393; env_light 2431-2442; direct_light 2444-2540; emitter_draw /
-// emitter_pick_u / emitter_pmf 463-593 (no light tree). Not here: textures, glass, clearcoat, media, fog,
-// procedural geometry, the light tree, local lights, live-only paths (the upload refuses records that need them).
+// emitter_pick_u / emitter_pick_range / lt_imp / emitter_pmf 463-618 (the light tree); the clearcoat. Not here:
+// textures, glass, media, fog, procedural geometry, local lights, live-only paths (the upload and the launch refuse
+// scenes and parameters that need them).
 // normalize(v) = v / sqrt(v.v), inverseSqrt(x) = 1 / sqrt(x); sin / cos by Moshier's Cephes reduction and
 // polynomials in fmaf; every other operation in the reference's order, no contraction: the same bits as the C
 // restatement the tests hold this to.
@@ -177,6 +178,7 @@ struct Ctx {
     Rng g;
     bool mirror_path;
     bool sampled_spec;
+    V mis_x, mis_n;  // pathtrace.wgsl:438-439: the vertex direct_light last sampled from, for emit_mis
 
     // Every table index is checked against its table: one outside reads entry 0 and reports kFlagTable.
     __device__ __forceinline__ uint64_t ck(uint64_t i, uint64_t n, uint32_t table,
@@ -210,7 +212,7 @@ struct Ctx {
     }
 };
 enum { CAM_POS = 0, CAM_FWD = 1, CAM_RIGHT = 2, CAM_UP = 3, SIZE = 4, COUNTS = 5, ENV = 6, MISC = 7, SKY = 8,
-       COUNTS2 = 9, LIGHT_LOD = 18 };
+       COUNTS2 = 9, LIGHT_LOD = 18, LTREE = 21 };
 
 struct TriAttr {
     float4 n0, n1, n2, uv, x;
@@ -539,10 +541,139 @@ __device__ float4 emitter_pick_u(const Ctx &c, float u) {
     return c.emitter_at(min(lo, emitter_count(c) - 1u));
 }
 
+// ---- the light tree (pathtrace.wgsl:489-618) -----------------------------------------------------------------
+constexpr uint32_t kLtLeaf = 0x80000000u, kLtPathBits = 26;
+__device__ __forceinline__ bool lt_on(const Ctx &c) { return c.pf(LTREE, 3) > 0.5f; }
+__device__ __forceinline__ float4 lt_node(const Ctx &c, uint32_t i) {
+    return c.a->light_tree[c.ck(i, c.a->light_tree_count, 30)];
+}
+// unpack2x16float's halves: IEEE binary16 to binary32, exactly
+__device__ __forceinline__ float half_bits(uint32_t h) {
+    const uint32_t sign = (h & 0x8000u) << 16, e = (h >> 10) & 0x1fu, m = h & 0x3ffu;
+    if (e == 0x1fu) return __uint_as_float(sign | 0x7f800000u | (m << 13));
+    if (e == 0u) return __uint_as_float(__float_as_uint(static_cast<float>(m) * 5.9604644775390625e-8f) | sign);
+    return __uint_as_float(sign | ((e + 112u) << 23) | (m << 13));
+}
+// lt_imp (504-524): power x the best cosine any point of the node's sphere has at x / max(d^2, r^2) (offline: no
+// distance cut)
+__device__ float lt_imp(float4 nd, V x, V n) {
+    const uint32_t a = __float_as_uint(nd.x), b = __float_as_uint(nd.y);
+    const V dv = vsub(v3(half_bits(a & 0xffffu), half_bits(a >> 16), half_bits(b & 0xffffu)), x);
+    const float r = half_bits(b >> 16);
+    const float d2 = dot(dv, dv);
+    const float r2 = r * r;
+    float cb = 1.0f;
+    if (d2 > r2 && dot(n, n) > 0.0f) {
+        const float inv = 1.0f / sqrtf(d2);
+        const float cos_t = dot(n, dv) * inv;
+        const float sin_b = r * inv;
+        const float cos_b = sqrtf(fmaxf(1.0f - sin_b * sin_b, 0.0f));
+        if (cos_t < cos_b) cb = fmaxf(cos_t * cos_b + sqrtf(fmaxf(1.0f - cos_t * cos_t, 0.0f)) * sin_b, 0.0f);
+    }
+    return nd.z * cb / fmaxf(d2, r2);
+}
+// emitter_pick_range (527-540): an entry by weight within [lo, hi)
+__device__ float4 emitter_pick_range(const Ctx &c, uint32_t lo, uint32_t hi, float r) {
+    const float p0 = lo > 0u ? c.emitter_at(max(lo, 1u) - 1u).y : 0.0f;
+    const float p1 = c.emitter_at(hi - 1u).y;
+    const float u = fminf(p0 + r * (p1 - p0), p1);
+    const uint32_t k = min(static_cast<uint32_t>(u * static_cast<float>(kGuide)), kGuide - 1u);
+    uint32_t l = emitter_guide(c, k), h = emitter_guide(c, k + 1u);
+    while (l < h) {
+        const uint32_t mid = (l + h) / 2u;
+        if (c.emitter_at(mid).y < u)
+            l = mid + 1u;
+        else
+            h = mid;
+    }
+    return c.emitter_at(min(max(l, lo), hi - 1u));
+}
+struct EPick {
+    float4 e;
+    float p, w;
+};
+// emitter_draw (547-585): without the tree the CDF; with it the tail by its share, else the tree descended by
+// importance at (x, n) to a leaf and one of its entries by weight
+__device__ EPick emitter_draw(const Ctx &c, V x, V n, uint32_t rd) {
+    float r3[3];
+    c.g.r3(rd, r3);
+    const float4 tot = emitter_totals(c);
+    const EPick none{make_float4(0.0f, 0.0f, 0.0f, 0.0f), 0.0f, 0.0f};
+    if (!(tot.x > 0.0f)) return none;
+    if (!lt_on(c)) {
+        const float4 e = emitter_pick_u(c, r3[0]);
+        return {e, e.z / tot.x, e.z};
+    }
+    if (r3[0] < tot.y) {
+        const float4 e = emitter_pick_range(c, c.pu(LTREE, 1), emitter_count(c), r3[1]);
+        return {e, e.z / tot.x, e.z};
+    }
+    float p = 1.0f - tot.y;
+    float u = r3[1];
+    uint32_t w = __float_as_uint(lt_node(c, 0u).w);
+    for (uint32_t k = 0u; k <= kLtPathBits && (w & kLtLeaf) == 0u; k++) {
+        const float4 l = lt_node(c, w), r = lt_node(c, w + 1u);
+        const float il = lt_imp(l, x, n), ir = lt_imp(r, x, n);
+        if (!(il + ir > 0.0f)) return none;
+        const float pl = il / (il + ir);
+        if (u < pl) {
+            w = __float_as_uint(l.w);
+            p *= pl;
+            u = u / pl;
+        } else {
+            w = __float_as_uint(r.w);
+            p *= 1.0f - pl;
+            u = (u - pl) / (1.0f - pl);
+        }
+        u = fminf(fmaxf(u, 0.0f), 0.99999994f);
+    }
+    if ((w & kLtLeaf) == 0u) return none;
+    const uint32_t lo = w & 0x1fffffffu, hi = lo + ((w >> 29) & 3u) + 1u;
+    float wsum = 0.0f;
+    for (uint32_t j = lo; j < hi; j++) wsum += c.emitter_at(j).z;
+    float t = r3[2] * wsum;
+    float4 e = c.emitter_at(hi - 1u);
+    for (uint32_t j = lo; j < hi; j++) {
+        const float4 cc = c.emitter_at(j);
+        if (t < cc.z) {
+            e = cc;
+            break;
+        }
+        t -= cc.z;
+    }
+    return {e, p * e.z / wsum, e.z};
+}
+
 struct Em {
     V v0, e1, e2;
     uint32_t tri, inst, code, part, flags;
 };
+// emitter_pmf (590-618): the probability emitter_draw at (x, n) picks emitter `ei`
+__device__ float emitter_pmf(const Ctx &c, uint32_t ei, const Em &em, V x, V n) {
+    const float4 tot = emitter_totals(c);
+    if (!(tot.x > 0.0f)) return 0.0f;
+    const uint32_t code = em.code & ~0x80000000u;
+    if ((code >> kLtPathBits) == 0u) return c.emitter_at(code).z / tot.x;
+    const uint32_t depth = (code >> kLtPathBits) - 1u;
+    float p = 1.0f - tot.y;
+    uint32_t w = __float_as_uint(lt_node(c, 0u).w);
+    for (uint32_t k = 0u; k < depth; k++) {
+        const float4 l = lt_node(c, w), r = lt_node(c, w + 1u);
+        const float il = lt_imp(l, x, n), ir = lt_imp(r, x, n);
+        if (!(il + ir > 0.0f)) return 0.0f;
+        const bool right = ((code >> k) & 1u) == 1u;
+        p *= (right ? ir : il) / (il + ir);
+        w = __float_as_uint(right ? r.w : l.w);
+    }
+    const uint32_t lo = w & 0x1fffffffu, hi = lo + ((w >> 29) & 3u) + 1u;
+    float wsum = 0.0f, we = 0.0f;
+    for (uint32_t j = lo; j < hi; j++) {
+        const float4 cc = c.emitter_at(j);
+        wsum += cc.z;
+        if (__float_as_uint(cc.x) == ei) we = cc.z;
+    }
+    return p * we / wsum;
+}
 // site: which caller reads the table (25 RIS candidate, 27 RIS pick, 28 single draw, 29 a hit's own emitter)
 __device__ __forceinline__ Em emitter_tri_at(const Ctx &c, uint32_t i, uint32_t site,
                                              const uint32_t *context = nullptr) {
@@ -568,18 +699,11 @@ __device__ ES sample_emitter_ris(Ctx &c, V x, V n, uint32_t m, const Surface &sf
     ES r{v3(0.0f, 0.0f, 0.0f), v3(0.0f, 1.0f, 0.0f), 0.0f, 0.0f, 0.0f};
     float wsum = 0.0f, pick_b1 = 0.0f, pick_b2 = 0.0f, pick_ph = 0.0f, pick_pdf = 0.0f;
     uint32_t pick_i = 0;
-    const float4 tot = emitter_totals(c);
     for (uint32_t k = 0; k < m; k++) {
         const uint32_t dk = rng_sub(rd, k);
-        float r3[3];
-        c.g.r3(dk, r3);
-        float4 e = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        float pk_p = 0.0f, pk_w = 0.0f;
-        if (tot.x > 0.0f) {
-            e = emitter_pick_u(c, r3[0]);
-            pk_p = e.z / tot.x;
-            pk_w = e.z;
-        }
+        const EPick pk = emitter_draw(c, x, n, dk);
+        const float4 e = pk.e;
+        const float pk_p = pk.p, pk_w = pk.w;
         float bb[3];
         c.g.r3(rng_sub(dk, 1u), bb);
         float b1 = bb[0], b2 = bb[1];
@@ -623,14 +747,11 @@ __device__ ES sample_emitter_ris(Ctx &c, V x, V n, uint32_t m, const Surface &sf
     return r;
 }
 
-__device__ ES sample_emitter(Ctx &c, V x, uint32_t rd) {
+__device__ ES sample_emitter(Ctx &c, V x, V n, uint32_t rd) {
     ES r{v3(0.0f, 0.0f, 0.0f), v3(0.0f, 1.0f, 0.0f), 0.0f, 0.0f, 0.0f};
-    float r3[3];
-    c.g.r3(rd, r3);
-    const float4 tot = emitter_totals(c);
-    if (!(tot.x > 0.0f)) return r;
-    const float4 e = emitter_pick_u(c, r3[0]);
-    const float pk_p = e.z / tot.x;
+    const EPick pk = emitter_draw(c, x, n, rd);
+    const float4 e = pk.e;
+    const float pk_p = pk.p;
     if (pk_p <= 0.0f) return r;
     const Em em = emitter_tri_at(c, __float_as_uint(e.x), 28);
     float bb[2];
@@ -686,15 +807,21 @@ __device__ V direct_light(Ctx &c, const Surface &s, const Lobes &lb, V v) {
         }
     }
     const V ne = s.n;
+    c.mis_x = o;
+    c.mis_n = ne;
     if (emitters_on(c)) {
         ES es;
         const bool seen = c.g.depth == 0u || (c.g.depth == 1u && c.mirror_path);
-        if (seen)
+        // with the tree its candidates are already near the point: P.ltree.w of them at camera hits, one elsewhere
+        if (lt_on(c))
+            es = sample_emitter_ris(c, o, ne, c.g.depth == 0u ? static_cast<uint32_t>(c.pf(LTREE, 3)) : 1u, s, lb, v,
+                                    c.g.dim(D_EMIT));
+        else if (seen)
             es = sample_emitter_ris(c, o, ne, c.g.depth == 0u ? kEmitRisM : kEmitRisM / 2u, s, lb, v, c.g.dim(D_EMIT));
         else if (c.pf(LIGHT_LOD, 3) > 0.0f)
             es = sample_emitter_ris(c, o, ne, 2u, s, lb, v, c.g.dim(D_EMIT));
         else
-            es = sample_emitter(c, o, c.g.dim(D_EMIT));
+            es = sample_emitter(c, o, ne, c.g.dim(D_EMIT));
         if (es.pdf_w > 0.0f) {
             const V f = eval_bsdf(s, lb, v, es.l);
             if (any_pos(f)) {
@@ -729,12 +856,7 @@ __device__ float emit_mis(const Ctx &c, const HitRec &hit, V d, float pdf) {
     const V cr = cross(em.e1, em.e2);
     const float area = 0.5f * len(cr);
     const float cos_l = fabsf(dot(normalize(cr), d));
-    const float4 tot = emitter_totals(c);
-    float sel = 0.0f;
-    if (tot.x > 0.0f) {
-        const uint32_t code = em.code & ~0x80000000u;
-        if ((code >> 26) == 0u) sel = c.emitter_at(code).z / tot.x;
-    }
+    const float sel = emitter_pmf(c, entry, em, c.mis_x, c.mis_n);
     if (!(sel > 0.0f)) return 1.0f;
     return power_heuristic(pdf, sel / fmaxf(area, 1e-12f) * hit.t * hit.t / fmaxf(cos_l, 1e-6f));
 }
@@ -762,6 +884,15 @@ __device__ void path(Ctx &c, uint32_t px, uint32_t py, uint32_t index, V &radian
     const float tanh = c.pf(CAM_POS, 3), aspect = c.pf(CAM_FWD, 3);
     V d = normalize(vadd(vadd(fwd, vs(right, ndx * tanh * aspect)), vs(up, ndy * tanh)));
     V o = v3(c.pf(CAM_POS, 0), c.pf(CAM_POS, 1), c.pf(CAM_POS, 2));
+    if (c.pf(CAM_RIGHT, 3) > 0.0f) {  // the thin lens (pathtrace.wgsl:4017-4023): cam_right.w radius, cam_up.w focus
+        const V focus = vadd(o, vs(d, c.pf(CAM_UP, 3) / dot(d, fwd)));
+        const float r = sqrtf(cam[2]) * c.pf(CAM_RIGHT, 3);
+        const float phi = kTwoPi * cam[3];
+        float sp, cp;
+        sincos_c(phi, sp, cp);
+        o = vadd(o, vadd(vs(right, r * cp), vs(up, r * sp)));
+        d = normalize(vsub(focus, o));
+    }
 
     radiance = v3(0.0f, 0.0f, 0.0f);
     albedo = v3(0.0f, 0.0f, 0.0f);
@@ -862,7 +993,7 @@ __global__ __launch_bounds__(kRenderBlock) __attribute__((amdgpu_waves_per_eu(3)
         const RenderArgs *a = &job->args;
         const uint32_t t = tile - job->tile_first;
         const uint32_t w = a->params[16], h = a->params[17], tiles_x = (w + 7) / 8;
-        Ctx c{a, &st, &flags, traced, Rng{0, 0, 0}, false, false};
+        Ctx c{a, &st, &flags, traced, Rng{0, 0, 0}, false, false, v3(0.0f, 0.0f, 0.0f), v3(0.0f, 0.0f, 0.0f)};
         const uint32_t px = (t % tiles_x) * 8 + (lane & 7u), py = (t / tiles_x) * 8 + (lane >> 3);
         if (px < w && py < h) {
             float4 *pix = a->accum + 5 * (uint64_t(py) * w + px);

@@ -24,10 +24,11 @@ struct RenderArgs {
     const float4 *emitter_tris; // lib.rs GpuEmitterTri, 4 x float4
     const float4 *objects;      // 3 x float4 an object
     const float4 *energy;       // lib.rs energy_lut, 512 x float4
+    const float4 *light_tree;   // ltree.rs LightTree nodes, one float4 each
     float4 *accum;              // records.wgsl Pixel, 5 x float4 a pixel
     // the tables' sizes (in their records), against which every index is checked
     uint32_t vertex_count, tri_count, row_count, material_count, light_count, emitter_vec4s, emitter_tri_count,
-        object_count;
+        object_count, light_tree_count;
     uint32_t first_sample;
     uint32_t spp;
     uint32_t params[kParamWords];

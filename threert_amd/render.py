@@ -24,9 +24,11 @@ import numpy as np
 
 from . import (INSTANCE, Device, Scene, ThreeRTError, TRT_E_INVALID, _check, _ptr, lib)
 from . import scene as S
+from . import build_light_tree
 
 NO_EMITTER = 0xFFFFFFFF
 EMITTER_GUIDE = 65536
+RIS_OFFLINE = 6  # ltree.rs:45: light-tree candidates resampled at camera hits
 OBJECT_STRIDE = 3
 
 
@@ -153,16 +155,21 @@ class RenderScene:
     energy: np.ndarray
     total_weight: float
     extras: dict = field(default_factory=dict)
+    # ltree.rs LightTree nodes (vec4s; none: the CDF alone), its entry count (the CDF's first entries) and the share
+    # of the emitters' weight outside it (the table header's y)
+    light_tree: np.ndarray = field(default_factory=lambda: np.zeros((0, 4), np.float32))
+    tree_entries: int = 0
 
     def tables(self) -> dict:
         return {"vertices": self.vertices, "indices": self.indices, "rows": self.rows, "materials": self.materials,
                 "lights": self.lights, "emitters": self.emitters, "emitter_tris": self.emitter_tris,
-                "objects": self.objects, "energy": self.energy}
+                "objects": self.objects, "energy": self.energy, "light_tree": self.light_tree}
 
     def params(self, width: int, height: int, camera: Camera, seed: int = 1, max_bounces: int = 4, rr_depth: int = 3,
                clamp: float = 1e9, energy_compensation: float = 0.0, sky=(0.0, 0.0, 0.0), env_nee: bool = True,
-               sample: int = 0) -> np.ndarray:
-        """The 624-byte Params block (gpu.rs:129-185, lib.rs:713-756 for a camera)."""
+               sample: int = 0, lens_radius: float = 0.0, focus: float = 1.0) -> np.ndarray:
+        """The 624-byte Params block (gpu.rs:129-185, lib.rs:713-756 for a camera); lens_radius > 0 makes the camera a
+        thin lens focused at `focus` along the view direction (cam_right.w, cam_up.w; pathtrace.wgsl:4017-4023)."""
         p = np.zeros(156, np.uint32)
         f = p.view(np.float32)
         pos = np.asarray(camera.position, np.float64)
@@ -176,9 +183,9 @@ class RenderScene:
         f[4:7] = fwd
         f[7] = width / height
         f[8:11] = right
-        f[11] = 0.0
+        f[11] = lens_radius
         f[12:15] = up
-        f[15] = 1.0
+        f[15] = focus
         p[16:20] = (width, height, sample, seed)
         n_em = len(self.emitter_tris)
         p[20:24] = (len(self.lights), n_em, max_bounces, rr_depth)
@@ -191,12 +198,18 @@ class RenderScene:
         f[70] = np.uint32(self.prep.top_first).view(np.float32)
         f[71] = np.uint32(self.prep.units).view(np.float32)
         f[76] = self.total_weight
+        if self.tree_entries:  # lib.rs:673 Params::ltree: -, its entries (bits), the tail's share, RIS_OFFLINE
+            p[85] = self.tree_entries
+            f[86] = self.emitters[len(self.emitter_tris) + (EMITTER_GUIDE + 1 + 3) // 4, 1]
+            f[87] = float(RIS_OFFLINE)
         return p
 
 
 def build(meshes: list[RMesh], instances: list[RInstance], materials: list[np.ndarray],
-          lights: list[np.ndarray] = ()) -> RenderScene:
-    """Lays out a scene as prepare does: trees and records (scene.prepare), then vertices, indices, rows, emitters."""
+          lights: list[np.ndarray] = (), light_tree: bool = True) -> RenderScene:
+    """Lays out a scene as prepare does: trees and records (scene.prepare), then vertices, indices, rows, emitters.
+    With 256 emitters or more that weigh something, a light tree over them (lib.rs:650-676, on unless THREERT_LTREE=0:
+    here `light_tree=False`), whose emitters come first in the CDF in its leaf order."""
     placements = [S.Placement(i.mesh, i.world, i.cast_shadow, 0) for i in instances]
     prep = S.prepare([S.Mesh(m.tris, m.sections) for m in meshes], placements)
     mats = np.stack(materials).astype(np.float32)
@@ -223,7 +236,7 @@ def build(meshes: list[RMesh], instances: list[RInstance], materials: list[np.nd
     vertices = np.concatenate(verts) if verts else np.zeros((0, 12), np.float32)
     indices = np.concatenate(idx_parts) if idx_parts else np.zeros((0, 3), np.uint32)
     # rows, emitters
-    rows, em_tris, entries = [], [], []
+    rows, em_tris, entries, centres = [], [], [], []
     n_em_so_far = 0
     lum = lambda e: 0.2126 * e[0] + 0.7152 * e[1] + 0.0722 * e[2]  # noqa: E731
     for r, inst in enumerate(instances):
@@ -261,18 +274,40 @@ def build(meshes: list[RMesh], instances: list[RInstance], materials: list[np.nd
                 em_tris.append(rec_em)
                 a32 = area.astype(np.float32)
                 entries.append((a32 * np.float32(lum(mats[mi][4:7])), a32))
+                centres.append((p32[:, 0] + p32[:, 1] + p32[:, 2]) / np.float32(3.0))  # lib.rs:210 placed_emitter
                 n_em_so_far += len(members)
             rows.append((mi, 0, first_em, 0))
     rows = np.array(rows, np.uint32).reshape(-1, 4)
-    # lib.rs:451-481 emitter_cdf: f32 running sum in list order (np.cumsum accumulates in order, in f32)
     n_em = n_em_so_far
+    emitter_tris = (np.concatenate(em_tris).view(np.float32).reshape(-1, 16) if em_tris
+                    else np.zeros((0, 16), np.float32))
+    wgt = np.concatenate([e[0] for e in entries]).astype(np.float32) if n_em else np.zeros(0, np.float32)
+    areas = np.concatenate([e[1] for e in entries]) if n_em else np.zeros(0, np.float32)
+    # lib.rs:650-676: the light tree over the emitters (GpuEmitterTri::vertices for its bounds, the placed centroids)
+    tree = None
+    if light_tree and n_em:
+        v0 = emitter_tris[:, 0:3]
+        tv = np.stack([v0, v0 + emitter_tris[:, 4:7], v0 + emitter_tris[:, 8:11]], axis=1)
+        tree = build_light_tree(tv, np.concatenate(centres), wgt)
+    # lib.rs:419-441 emitter_order / emitter_codes: the tree's emitters in leaf order, then the rest in list order;
+    # each triangle's code its path (tree) or its entry in the CDF
+    if tree is not None:
+        placed = np.zeros(n_em, bool)
+        placed[tree.order] = True
+        order = np.concatenate([tree.order, np.nonzero(~placed)[0].astype(np.uint32)]).astype(np.uint32)
+        codes = np.empty(n_em, np.uint32)
+        codes[order] = np.arange(n_em, dtype=np.uint32)
+        codes[tree.order] = tree.code
+        emitter_tris.view(np.uint32)[:, 11] = codes
+    else:
+        order = np.arange(n_em, dtype=np.uint32)
+    # lib.rs:451-481 emitter_cdf: f32 running sum in the CDF's order (np.cumsum accumulates in order, in f32)
     cdf = np.zeros((n_em, 4), np.float32)
     if n_em:
-        wgt = np.concatenate([e[0] for e in entries]).astype(np.float32)
-        cdf[:, 0] = np.arange(n_em, dtype=np.uint32).view(np.float32)
-        cdf[:, 1] = np.cumsum(wgt, dtype=np.float32)
-        cdf[:, 2] = wgt
-        cdf[:, 3] = np.concatenate([e[1] for e in entries])
+        cdf[:, 0] = order.view(np.float32)
+        cdf[:, 1] = np.cumsum(wgt[order], dtype=np.float32)
+        cdf[:, 2] = wgt[order]
+        cdf[:, 3] = areas[order]
         total = np.float32(cdf[-1, 1])
     else:
         total = np.float32(0.0)
@@ -284,15 +319,19 @@ def build(meshes: list[RMesh], instances: list[RInstance], materials: list[np.nd
     g4 = np.zeros(((EMITTER_GUIDE + 1 + 3) // 4) * 4, np.uint32)
     g4[:EMITTER_GUIDE + 1] = guide
     header = np.array([[total, 0, 0, 0]], np.float32)
+    if tree is not None and total > 0:  # lib.rs:672-674: the share of the weight outside the tree
+        header[0, 1] = max((total - np.float32(tree.power)) / total, np.float32(0.0))
     emitters = np.concatenate([cdf, g4.view(np.float32).reshape(-1, 4), header]) if n_em else np.zeros((0, 4),
                                                                                                          np.float32)
-    emitter_tris = (np.concatenate(em_tris).view(np.float32).reshape(-1, 16) if em_tris
-                    else np.zeros((0, 16), np.float32))
     objects = np.zeros((OBJECT_STRIDE, 4), np.float32)
     objects[0, 3] = 1.0
     lights = np.stack(lights).astype(np.float32) if len(lights) else np.zeros((0, 20), np.float32)
-    return RenderScene(prep, vertices, indices, rows, mats, lights, len(lights), emitters, emitter_tris, objects,
-                       energy_lut(), float(total))
+    rs = RenderScene(prep, vertices, indices, rows, mats, lights, len(lights), emitters, emitter_tris, objects,
+                     energy_lut(), float(total))
+    if tree is not None:
+        rs.light_tree = tree.nodes.astype(np.float32)
+        rs.tree_entries = len(tree.order)
+    return rs
 
 
 # ---- device side -----------------------------------------------------------------------------------------------
@@ -304,13 +343,13 @@ def upload(dev: Device, rs: RenderScene) -> Scene:
     p = rs.prep
     keep = [np.ascontiguousarray(x) for x in (p.nodes, p.leaves, p.instances, rs.vertices, rs.indices, rs.rows,
                                               rs.materials, rs.lights, rs.emitters, rs.emitter_tris, rs.objects,
-                                              rs.energy)]
-    nodes, leaves, inst, verts, idx, rows, mats, lights, ems, emt, objs, energy = keep
+                                              rs.energy, rs.light_tree)]
+    nodes, leaves, inst, verts, idx, rows, mats, lights, ems, emt, objs, energy, ltree = keep
     null = lambda a: _ptr(a) if a.size else None  # noqa: E731
     desc = _RenderDesc(_ptr(nodes), len(nodes), _ptr(leaves), len(leaves), _ptr(inst), len(inst), p.top_first,
                        p.units, null(verts), len(verts), null(idx), len(idx), null(rows), len(rows), null(mats),
                        len(mats), null(lights), len(lights), null(ems), len(ems), null(emt), len(emt), null(objs),
-                       len(objs) // OBJECT_STRIDE, _ptr(energy))
+                       len(objs) // OBJECT_STRIDE, _ptr(energy), null(ltree), len(ltree))
     sc = Scene.__new__(Scene)
     sc.dev = dev
     h = C.c_void_p()
