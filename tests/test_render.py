@@ -223,6 +223,50 @@ def test_an_emitter_lights_a_plane_as_its_closed_form():
     assert abs(c[0] - want) / want < 0.03, (c, want)
 
 
+def sunlit_plane(elevation_deg: float, diameter_deg: float, lux: float = 1000.0):
+    """A 2 x 2 Lambert plane (albedo 1, no specular) at y = 0 under one directional light (pathtrace.wgsl:2449-2472,
+    1963-1988) at `elevation_deg` from the zenith, seen from straight above."""
+    plane = R.quad((-1, 0, -1), (-1, 0, 1), (1, 0, 1), (1, 0, -1))
+    th = math.radians(elevation_deg)
+    sun = R.directional_light((math.sin(th), -math.cos(th), 0.0), (lux, lux, lux), diameter_deg=diameter_deg)
+    rs = R.build([R.RMesh(plane)], [R.RInstance(0, np.eye(4, dtype=np.float32), [0])],
+                 [R.material((1, 1, 1), specular=0.0)], [sun])
+    return rs, R.Camera((0.0, 3.0, 0.001), (0.0, 0.0, 0.0), 20.0)
+
+
+@pytest.mark.parametrize("elevation,diameter", [(30.0, 0.5357), (30.0, 40.0), (20.0, 100.0), (55.0, 60.0)])
+def test_the_sun_lights_a_plane_as_its_closed_form(elevation, diameter):
+    """The sun's light at a Lambert plane: a cone of half-angle a around the direction at angle t0 from the normal,
+    of illuminance E across its axis, holds radiance E / (2 pi (1 - cos a)) and lights the plane with
+    E x (pi sin^2 a cos t0) / (2 pi (1 - cos a)) = E cos t0 (1 + cos a) / 2 (the cap's projected solid angle, exact
+    while the cap is above the horizon); the plane reads that over pi. Light sampling (the cone sample, its pdf, the
+    shadow ray) and BSDF rays that reach the disk (sun_disks) share it by the power heuristic: on the 0.54-degree sun
+    nearly all of it comes by light sampling, on the wide cones a good part by BSDF rays, and every case must read
+    the closed form (float64 mean over 4096 samples a pixel, within 1 %)."""
+    rs, cam = sunlit_plane(elevation, diameter)
+    p = rs.params(16, 16, cam, max_bounces=1, rr_depth=8)
+    acc = oracle_scene(rs).render(p, 0, 4096, threads=8).astype(np.float64)
+    got = (acc[..., :3] / acc[..., 3:4]).mean()
+    a = math.radians(diameter * 0.5)
+    want = 1000.0 * math.cos(math.radians(elevation)) * (1 + math.cos(a)) / 2 / math.pi
+    assert abs(got - want) < 0.01 * want, (got, want)
+
+
+def test_russian_roulette_keeps_the_expectation():
+    """Inside a closed box whose walls are Lambert of albedo 1/2 and emit 1/2 (two-sided), the radiance everywhere
+    is Le / (1 - albedo) = 1: every bounce adds half of what is left. With Russian roulette from the first bounce
+    (rr_depth 1: q = min(max throughput, 0.95), throughput / q on survival, pathtrace.wgsl:4469-4473) and 64 bounces,
+    most paths end early and the survivors carry the rest: the image reads 1 +- 2 %."""
+    box = R.box_tris((-1, -1, -1), (1, 1, 1))
+    rs = R.build([R.RMesh(box)], [R.RInstance(0, np.eye(4, dtype=np.float32), [0])],
+                 [R.material((0.5, 0.5, 0.5), (0.5, 0.5, 0.5), specular=0.0)])
+    cam = R.Camera((0.1, -0.2, -0.5), (0.3, 0.2, 1.0), 70.0)
+    p = rs.params(24, 24, cam, max_bounces=64, rr_depth=1)
+    acc = oracle_scene(rs).render(p, 0, 512, threads=8).astype(np.float64)
+    got = (acc[..., :3] / acc[..., 3:4]).mean()
+    assert abs(got - 1.0) < 0.02, got
+
+
 def test_the_sampler_holds_its_known_answers():
     """sample.rs:91-125: the direction table's last entries (Cycles'), and the first 2^m points of each 2D pair
     a (0, m, 2)-net, for the reference's seeds."""
