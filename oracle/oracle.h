@@ -110,8 +110,10 @@ typedef struct {
 #define OR_MAX_UNITS (1u << 18) /* units.rs:37 */
 
 /* Walk variants. REF: trace_units.wgsl as written (strict `<`, first found wins a tie, boxes culled at hit.t).
- * EXACT: the same walk, with ties broken to the lowest (instance, triangle) and boxes culled at hit.t widened by
- * OR_CULL_SLACK, so its answer is brute force's whatever the tree. */
+ * EXACT: the same walk, with ties broken to the lowest (instance, triangle), boxes culled at hit.t widened by
+ * OR_CULL_SLACK, and every box widened before its slab test by 2^-8 of its largest extent plus 2^-20 of its largest
+ * coordinate: a box's float slab test can miss a ray that the triangle test inside it accepts (an edge of a
+ * triangle on a flat box's face), so only a box with a margin finds what brute force finds, whatever the tree. */
 #define OR_WALK_REF 0u
 #define OR_WALK_EXACT 1u
 #define OR_CULL_SLACK (1.0f / 16384.0f)

@@ -47,6 +47,20 @@ float or_ray_aabb(const float *o, const float *inv, const float *bmin, const flo
     return tn <= tf ? tn : OR_INF;
 }
 
+/* The EXACT walk's box test: the box widened by 2^-8 of its largest extent plus 2^-20 of its largest coordinate. */
+static float ray_aabb_wide(const float *o, const float *inv, const float *bmin, const float *bmax, float tmax) {
+    float ext = fmaxf(fmaxf(bmax[0] - bmin[0], bmax[1] - bmin[1]), bmax[2] - bmin[2]);
+    float mag = 0.0f;
+    for (int k = 0; k < 3; k++) mag = fmaxf(mag, fmaxf(fabsf(bmin[k]), fabsf(bmax[k])));
+    float m = ext * 0.00390625f + mag * 9.5367431640625e-07f;
+    float lo[3] = {bmin[0] - m, bmin[1] - m, bmin[2] - m}, hi[3] = {bmax[0] + m, bmax[1] + m, bmax[2] + m};
+    return or_ray_aabb(o, inv, lo, hi, tmax);
+}
+static inline float box_test(const float *o, const float *inv, const float *bmin, const float *bmax, float tmax,
+                             uint32_t variant) {
+    return variant == OR_WALK_EXACT ? ray_aabb_wide(o, inv, bmin, bmax, tmax) : or_ray_aabb(o, inv, bmin, bmax, tmax);
+}
+
 void or_intersect_tri(const float *o, const float *d, const or_leaf *l, float tmax, float *tuv) {
     const float *v0 = l->r[0], *e1 = l->r[1], *e2 = l->r[2];
     tuv[0] = OR_INF;
@@ -126,7 +140,7 @@ int or_trace(const or_scene *s, const float *o, const float *d, float tmax_in, u
     } while (0)
     or_node whole;
     NODE(top, whole);
-    if (or_ray_aabb(o, inv_d, whole.bmin, whole.bmax, cull_t(h.t, variant)) == OR_INF) return 0;
+    if (box_test(o, inv_d, whole.bmin, whole.bmax, cull_t(h.t, variant), variant) == OR_INF) return 0;
     uint32_t stack[OR_STACK];
     uint32_t sp = 0, node = top, rec = OR_NO_INST, off = top, first = 0, base = 0;
     float ro[3], rdir[3], rinv[3];
@@ -142,8 +156,8 @@ int or_trace(const or_scene *s, const float *o, const float *d, float tmax_in, u
             or_node nl, nr;
             NODE(l, nl);
             NODE(r, nr);
-            float tl = or_ray_aabb(ro, rinv, nl.bmin, nl.bmax, cull_t(h.t, variant));
-            float tr = or_ray_aabb(ro, rinv, nr.bmin, nr.bmax, cull_t(h.t, variant));
+            float tl = box_test(ro, rinv, nl.bmin, nl.bmax, cull_t(h.t, variant), variant);
+            float tr = box_test(ro, rinv, nr.bmin, nr.bmax, cull_t(h.t, variant), variant);
             if (tl != OR_INF || tr != OR_INF) {
                 uint32_t near = l, far = r;
                 if (tr < tl) {

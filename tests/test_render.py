@@ -327,6 +327,44 @@ def test_cornell_matches_the_oracle(device):
     assert rel_rmse(got, want) <= 1e-3
 
 
+def oracle_threads() -> int:
+    import os
+
+    return min(16, os.cpu_count() or 1)
+
+
+@pytest.mark.gpu
+def test_cornell_at_full_size_matches_the_oracle(device):
+    """The plain-method workload at its size: 512 x 512, 16 samples, 4 bounces, RR from depth 3, seed 1."""
+    print("cornell 512x512 x 16 spp", flush=True)
+    rs, cam = R.cornell()
+    p = rs.params(512, 512, cam, seed=1, max_bounces=4, rr_depth=3)
+    want = oracle_scene(rs).render(p, 0, 16, threads=oracle_threads())
+    got = gpu_render(device, rs, p, 0, 16)
+    same = (got.view(np.uint32) == want.view(np.uint32)).all(axis=2).mean()
+    print(f"cornell 512: {same:.6f} of the pixels bit-equal, rel RMSE {rel_rmse(got, want):.2e}", flush=True)
+    assert rel_rmse(got, want) <= 1e-3
+
+
+@pytest.mark.gpu
+def test_the_full_city_at_720p_matches_the_oracle(device):
+    """The instanced-city workload at its size: 64 meshes (1.0M unique triangles) x 4096 instances (66M instanced),
+    metal-rough GGX, the 90,000-lux sun and a constant sky, 1280 x 720, 1 spp, 3 bounces, from a camera above the
+    skyline (a real image: most of it lit)."""
+    print("city: building 64 meshes x 4096 instances", flush=True)
+    rs, cam, sky = R.city(seed=1)
+    assert rs.extras["unique_triangles"] > 900_000 and rs.extras["instanced_triangles"] > 55_000_000
+    p = rs.params(1280, 720, cam, seed=1, max_bounces=3, rr_depth=3, sky=sky)
+    print("city: oracle render", flush=True)
+    want = oracle_scene(rs).render(p, 0, 1, threads=oracle_threads())
+    print("city: GPU render", flush=True)
+    got = gpu_render(device, rs, p, 0, 1)
+    same = (got.view(np.uint32) == want.view(np.uint32)).all(axis=2).mean()
+    print(f"city 720p: {same:.6f} of the pixels bit-equal, rel RMSE {rel_rmse(got, want):.2e}", flush=True)
+    assert rel_rmse(got, want) <= 1e-3
+    assert (want[..., :3].sum(-1) == 0).mean() < 0.2  # at one sample, 11 % (shadowed pixels whose bounce found no light)
+
+
 @pytest.mark.gpu
 def test_a_thin_lens_matches_the_oracle(device):
     """camera_ray's thin lens (pathtrace.wgsl:4017-4023: lens radius cam_right.w, focus distance cam_up.w)."""

@@ -116,6 +116,50 @@ def test_the_server_gives_each_job_its_lone_image(device):
     assert s["jobs"] == 8 and s["samples"] == sum(j.width * j.height * j.spp for j in jobs)
 
 
+@pytest.mark.gpu
+def test_a_full_scale_slice_of_the_workload_gives_each_job_its_lone_image(device):
+    """24 jobs of the serving workload at full scale (seed 1): every job on the largest scene (3.56M triangles), the
+    largest image, one more job set to 1920 x 1080, and the first jobs on four more scenes; samples capped at 4 (the
+    check is the mix, not convergence). Arriving together, they share launches; each job's image is its lone
+    render's, bit for bit."""
+    import dataclasses
+
+    from threert_amd import render as R
+
+    jobs = serve.workload(n_jobs=512, rate=40.0, seed=1)
+    biggest = max(j.scene[0] for j in jobs)
+    assert biggest >= 3_000_000
+    pick = [j for j in jobs if j.scene[0] == biggest][:6]
+    pick.append(max(jobs, key=lambda j: j.width * j.height))
+    pick.append(dataclasses.replace(jobs[0], width=1920, height=1080))
+    others = sorted({j.scene for j in jobs if j.scene[0] != biggest}, key=lambda sc: sc[0])[:4]
+    for sc in others:
+        pick += [j for j in jobs if j.scene == sc][:4]
+    pick = pick[:24]
+    assert len(pick) == 24
+    pick = [dataclasses.replace(j, id=k, spp=min(j.spp, 4), arrival=0.001 * k) for k, j in enumerate(pick)]
+    print("slice:", [(j.scene[0], j.width, j.height, j.spp, j.bounces) for j in pick], flush=True)
+    srv = serve.Server(device, serve.soup_scene, keep_images=True, workers=4)
+    res = srv.run(pick)
+    srv.close()
+    assert [r.id for r in res] == [j.id for j in pick] and srv.launches > 1
+    print(f"slice: {srv.launches} launches", flush=True)
+    scenes = {}
+    for j, r in zip(pick, res):
+        if j.scene not in scenes:
+            rs, cam, sky = serve.soup_scene(*j.scene)
+            scenes[j.scene] = (serve._Resident(None, rs, cam, sky, 0), R.upload(device, rs))
+        e, sc = scenes[j.scene]
+        acc = R.Accum(device, j.width, j.height)
+        R.render(sc, serve.params_of(e, j), 0, j.spp, acc)
+        device.sync()
+        want = acc.resolve()[0]
+        acc.close()
+        assert (_bits(r.image) == _bits(want)).all(), f"job {j.id} ({j.scene}, {j.width}x{j.height}) differs"
+    for _, sc in scenes.values():
+        sc.close()
+
+
 def test_equal_error_plan_never_raises_a_jobs_error():
     """The denoising server's plan: the fewest calibrated samples whose denoised error is at most the job's plain
     error, else the job as asked; no job's calibrated error goes up."""

@@ -4,7 +4,10 @@
 // in (0, tmax), with every triangle tested by the reference's Moller-Trumbore (pathtrace.wgsl:742-760) in the
 // reference's operation order (these files are built with -ffp-contract=off), so t, u and v are the reference's bits.
 //   trace_binary  the reference's own 32-byte binary node buffer, trace_units.wgsl:44-135 statement for statement
-//                 (64-entry stack, near child first), boxes culled at hit.t widened by kCullSlack.
+//                 (64-entry stack, near child first), boxes culled at hit.t widened by kCullSlack and each box
+//                 widened before its slab test by 2^-8 of its largest extent plus 2^-20 of its largest coordinate
+//                 (a float slab test can miss a ray the triangle test inside the box accepts: an edge of a triangle
+//                 on a flat box's face; the quantised boxes of the wide walk are wider than that by construction).
 //   trace_wide    the library's 64-byte 4-wide quantised nodes (wide.cpp): each node's four child boxes are tested
 //                 as fma(q, step / d, (origin - o) / d) per plane, children visited nearest first.
 // The stack's first kLdsStack entries live in LDS (stride = block size, so lanes never share a bank); deeper entries
@@ -45,6 +48,17 @@ __device__ __forceinline__ float ray_aabb(F3 o, F3 inv, F3 bmin, F3 bmax, float 
     const float tn = fmaxf(fmaxf(lx, ly), fmaxf(lz, 0.0f));
     const float tf = fminf(fminf(hx, hy), fminf(hz, tmax));
     return tn <= tf ? tn : kInf;
+}
+
+// The binary walk's box test: ray_aabb over the box widened by 2^-8 of its largest extent plus 2^-20 of its largest
+// coordinate.
+__device__ __forceinline__ float ray_aabb_wide(F3 o, F3 inv, F3 bmin, F3 bmax, float tmax) {
+    const float ext = fmaxf(fmaxf(bmax.x - bmin.x, bmax.y - bmin.y), bmax.z - bmin.z);
+    float mag = fmaxf(0.0f, fmaxf(fabsf(bmin.x), fabsf(bmax.x)));
+    mag = fmaxf(mag, fmaxf(fabsf(bmin.y), fabsf(bmax.y)));
+    mag = fmaxf(mag, fmaxf(fabsf(bmin.z), fabsf(bmax.z)));
+    const float m = ext * 0.00390625f + mag * 9.5367431640625e-07f;
+    return ray_aabb(o, inv, {bmin.x - m, bmin.y - m, bmin.z - m}, {bmax.x + m, bmax.y + m, bmax.z + m}, tmax);
 }
 
 // pathtrace.wgsl:742-760 intersect_tri; t = kInf for a miss.
@@ -172,7 +186,7 @@ __device__ HitRec trace_binary(const TraceArgs &a, F3 o, F3 d, float tmax_in, ui
     {
         if (kCount) cnt.nodes++;
         const float4 w0 = a.bnodes[2 * top], w1 = a.bnodes[2 * top + 1];
-        if (ray_aabb(o, inv, {w0.x, w0.y, w0.z}, {w1.x, w1.y, w1.z}, cull_t(h.t)) == kInf) return h;
+        if (ray_aabb_wide(o, inv, {w0.x, w0.y, w0.z}, {w1.x, w1.y, w1.z}, cull_t(h.t)) == kInf) return h;
     }
     st.sp = 0;
     uint32_t node = top, rec = kNone, off = top, first = 0, base = 0;
@@ -201,8 +215,8 @@ __device__ HitRec trace_binary(const TraceArgs &a, F3 o, F3 d, float tmax_in, ui
             }
             const float4 l0 = a.bnodes[2 * l], l1 = a.bnodes[2 * l + 1];
             const float4 r0 = a.bnodes[2 * r], r1 = a.bnodes[2 * r + 1];
-            float tl = ray_aabb(ro, rinv, {l0.x, l0.y, l0.z}, {l1.x, l1.y, l1.z}, cull_t(h.t));
-            float tr = ray_aabb(ro, rinv, {r0.x, r0.y, r0.z}, {r1.x, r1.y, r1.z}, cull_t(h.t));
+            float tl = ray_aabb_wide(ro, rinv, {l0.x, l0.y, l0.z}, {l1.x, l1.y, l1.z}, cull_t(h.t));
+            float tr = ray_aabb_wide(ro, rinv, {r0.x, r0.y, r0.z}, {r1.x, r1.y, r1.z}, cull_t(h.t));
             if (tl != kInf || tr != kInf) {
                 uint32_t near = l, far = r;
                 if (tr < tl) {
