@@ -9,7 +9,11 @@ stream the kernel runs on). One JSON line is printed on stdout.
 Beside the GPU number: the roofline of the walk (algorithmic bytes per launch, counted by a counting launch of the
 same walk over the same batch: 64 B per wide node fetched, 48 B per triangle tested, 64 B per unit entered), and
 the CPU baseline: the oracle's restatement of the reference's walk (trace_units.wgsl over the reference's binary
-node buffer), OpenMP over the host's cores, on a bounded sample of the same rays.
+node buffer), built on this host with -march=native, OpenMP over the host's threads and on one, on a bounded sample
+of the same rays. The other workloads follow in the same line: the instanced city at 720p (samples/s, with the path
+tracer's own roofline), the Cornell box at 512 x 512 x 16, the 3M-triangle scene at 1080p x 64, the learned
+denoiser's inference and held-out error at 1280 x 720, and the serving workload at equal image error (plain tracing
+against the denoiser with plain fallback: p50 / p99 latency, mean error).
 """
 from __future__ import annotations
 
@@ -29,6 +33,8 @@ from threert_amd import scene as S  # noqa: E402
 
 METRIC = "Mrays/s closest-hit + samples/s at 720p; p99 job latency at equal image error"
 HBM_PEAK_GBS = 8000.0
+SHADE_BYTES = 12 + 3 * 48 + 160 + 16 + 288  # a shaded hit's gathers: indices, vertices, record, row, material
+ACCUM_BYTES = 2 * 48                        # a pixel's colour, albedo and normal sums read and written
 N_TRIS = 3_000_000
 N_RAYS = 16 * 1024 * 1024
 
@@ -53,20 +59,45 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(prep, rays: np.ndarray, sample: int) -> dict:
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(prep, rays: np.ndarray, sample: int, single: int) -> dict:
+    """The oracle's restatement of the reference walk (WALK_REF: trace_units.wgsl over the reference's binary node
+    buffer), built here for this CPU (-march=native), over `sample` rays on the host's threads (OMP_NUM_THREADS: this
+    process's share of the host, else every core) and over `single` rays on one thread."""
     import oracle
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
-    threads = min(threads, 16)
-    sc = oracle.Scene(*prep.desc())
-    sub = rays[:sample].view(oracle.RAY)
-    sc.trace_batch(sub[:min(20000, sample)], oracle.WALK_REF, threads=threads)  # warm
-    t0 = time.perf_counter()
-    sc.trace_batch(sub, oracle.WALK_REF, threads=threads)
-    dt = time.perf_counter() - t0
+    nproc = os.cpu_count() or 1
+    threads = min(int(os.environ.get("OMP_NUM_THREADS", "0")) or nproc, nproc)
+    with oracle.native():
+        sc = oracle.Scene(*prep.desc())
+        sub = rays[:sample].view(oracle.RAY)
+        sc.trace_batch(sub[:min(20000, sample)], oracle.WALK_REF, threads=threads)  # warm
+        t0 = time.perf_counter()
+        sc.trace_batch(sub, oracle.WALK_REF, threads=threads)
+        dt = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        sc.trace_batch(rays[:single].view(oracle.RAY), oracle.WALK_REF, threads=1)
+        dt1 = time.perf_counter() - t0
+    model = cpu_model()
+    log(f"bench: CPU baseline {sample / dt / 1e6:.3f} Mrays/s on {threads} threads, {single / dt1 / 1e6:.3f} on one "
+        f"({model}, nproc {nproc})")
     return {"value": round(sample / dt / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": f"the first {sample} of the 16M rays, the oracle's C restatement of trace_units.wgsl over the "
-                      f"reference's binary node buffer (-O3, OpenMP, {threads} threads), {dt:.1f} s"}
+                      f"reference's binary node buffer (gcc -O3 -march=native -ffp-contract=off, OpenMP, {threads} "
+                      f"threads: this process's share of the host's {nproc}), {dt:.1f} s",
+            "single_core": {"value": round(single / dt1 / 1e6, 4), "unit": "Mrays/s", "rays": single,
+                            "seconds": round(dt1, 2)},
+            "cpu_model": model, "nproc": nproc}
 
 
 def fp64_check(dev, sc, d_rays, d_hits, hits) -> dict:
@@ -142,20 +173,33 @@ def bench_render(dev, frames: int) -> dict:
     dev.sync()
     closest, shadow = dev.render_ray_counts()
     ms_frame = ms / frames
+    # one counting launch of a frame (not timed): what its walks read and the hits it shaded
+    rd = R.render_reads(sc, p, 1, 1, acc)
     sc.close()
     acc.close()
     sps = w * h / (ms_frame * 1e-3)
+    walk_b = rd["nodes"] * 64 + rd["tris"] * 48 + rd["units"] * 64
+    frame_b = walk_b + rd["shaded"] * SHADE_BYTES + w * h * ACCUM_BYTES
+    achieved = frame_b / (ms_frame * 1e-3) / 1e9
     log(f"bench: city 720p {ms_frame:.2f} ms/frame, {sps / 1e6:.1f} Msamples/s, "
-        f"{closest / (ms * 1e-3) / 1e6:.1f} M closest + {shadow / (ms * 1e-3) / 1e6:.1f} M shadow rays/s")
+        f"{closest / (ms * 1e-3) / 1e6:.1f} M closest + {shadow / (ms * 1e-3) / 1e6:.1f} M shadow rays/s, "
+        f"{frame_b / 1e6:.0f} MB a frame, {achieved:.0f} GB/s")
     return {"metric": "samples/s at 720p", "value": round(sps, 1), "unit": "samples/s",
             "ms_per_frame": round(ms_frame, 3), "frames": frames,
             "rays": {"closest": closest, "shadow": shadow,
                      "closest_mrays_s": round(closest / (ms * 1e-3) / 1e6, 1),
                      "shadow_mrays_s": round(shadow / (ms * 1e-3) / 1e6, 1)},
+            "roofline": {"kernel": "k_render", "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                         "bytes_per_frame": int(frame_b),
+                         "reads_per_frame": rd, "bytes": "64 B a wide node + 48 B a triangle + 64 B a unit the walks "
+                         f"read, {SHADE_BYTES} B a shaded hit (indices 12, 3 vertices 144, record 160, row 16, "
+                         f"material 288), {ACCUM_BYTES} B a pixel's accumulator read and written"},
             "workload": "instanced city: 64 seeded meshes (extruded boxes, subdivided spheres, 200-40k triangles) x "
                         "4096 instances, metal-rough GGX, 90,000-lux sun + constant sky, 1280x720, 1 spp a frame, "
-                        "3 bounces, megakernel over the 4-wide quantised walk",
-            "unique_triangles": int(len(rs.prep.leaves))}
+                        "3 bounces, from above the skyline; the path tracer over the 4-wide quantised walk",
+            "unique_triangles": rs.extras["unique_triangles"],
+            "instanced_triangles": rs.extras["instanced_triangles"]}
 
 
 def bench_cornell(dev, check: bool = False) -> dict:
@@ -380,6 +424,48 @@ def bench_denoise(dev, w: int = 1280, h: int = 720, reps: int = 20) -> dict:
             "frac_of_bf16_peak": round(tf / 2500.0, 4)}
 
 
+def bench_denoise_eval(dev) -> dict:
+    """The learned denoiser at 1280 x 720 (the shipped weights): inference ms a frame and flops against the BF16 MFMA
+    peak, and the relative RMSE against 1024-sample FP64-accumulated targets on three held-out scenes (a Cornell
+    variant, a clustered soup, a small city; seeds 10000-10002, not trained on), beside plain tracing."""
+    from threert_amd import denoise_train as DT
+    from threert_amd import unet as U
+
+    inf = bench_denoise(dev)
+    net = U.UNet(dev, 1280 * 720, U.shipped_params())
+    ev = DT.evaluate(dev, net, DT.scene_list(3, first_seed=10_000), 1280, 720, 1024, log=log, timing_reps=5)
+    net.close()
+    m = {k: round(v, 4) for k, v in ev["mean"].items()}
+    log(f"bench: denoiser held-out mean rel RMSE {m}")
+    return {"ms_per_frame": inf["ms_per_frame"], "gflop_per_frame": inf["gflop_per_frame"], "tflops": inf["tflops"],
+            "roofline": {"kernel": "k_conv", "bound": "mfma", "achieved": inf["tflops"], "peak": 2500.0,
+                         "unit": "TFLOP/s", "frac": inf["frac_of_bf16_peak"], "traffic": None},
+            "held_out_rel_rmse": m, "held_out_scenes": [r["scene"] for r in ev["scenes"]],
+            "workload": "5-level U-Net (32-128 channels, 3x3 convolutions, BF16 MFMA, FP32 accumulation) on a "
+                        "1280x720 frame; error of its output over a 1-sample render (and over 4, 16, 64 samples) "
+                        "against a 1024-sample FP64-accumulated render"}
+
+
+def bench_serve_equal(dev, n_jobs: int, rate: float) -> dict:
+    """The serving workload at equal image error (threert_amd/equal_error.py): plain FP32 tracing against the learned
+    denoiser with plain tracing as the fallback, equal shares, the same arrivals."""
+    from threert_amd import equal_error
+
+    t0 = time.perf_counter()
+    r = equal_error.run(dev, n_jobs=n_jobs, rate=rate, log=log, policies=("equal",))
+    keep = ("p50_s", "p99_s", "queue_p99_s", "samples_per_s", "wall_s", "mean_rel_rmse", "denoised_jobs")
+    out = {"plain": {k: r["runs"]["plain_equal"][k] for k in keep},
+           "denoised": {k: r["runs"]["denoised_equal"][k] for k in keep},
+           "p99_ratio": r["p99_ratio"]["equal"], "p50_ratio": round(r["runs"]["denoised_equal"]["p50_s"] /
+                                                                    r["runs"]["plain_equal"]["p50_s"], 4),
+           "each_job_error_at_most_plain": r["each_job_error_at_most_plain"],
+           "calibration": r["calibration_setup"], "seconds": round(time.perf_counter() - t0, 1),
+           "workload": f"{n_jobs} render jobs, Poisson arrivals at {rate}/s, 24 seeded scenes of 10k-4M triangles, "
+                       "256x256-1920x1080, 1-256 spp, 2-6 bounces, 200 GB budget, continuous batching"}
+    log(f"bench: serving at equal error {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -390,6 +476,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-render", action="store_true")
     ap.add_argument("--render-frames", type=int, default=10)
+    ap.add_argument("--no-denoise", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--cpu-single", type=int, default=1 << 20)
     ap.add_argument("--build-only", action="store_true",
                     help="measure only the upload's wide-tree build of the 3M-triangle scene (device and host)")
     ap.add_argument("--denoise-only", action="store_true",
@@ -559,9 +648,13 @@ def main():
         out["render_720p"] = bench_render(dev, args.render_frames)
         out["render_cornell_512"] = bench_cornell(dev, check=not args.no_cpu)
         out["render_1080p_precision"] = bench_precision_render(dev)
+    if not args.no_denoise:
+        out["denoise_720p"] = bench_denoise_eval(dev)
+    if not args.no_serve:
+        out["serving_equal_error"] = bench_serve_equal(dev, args.serve_jobs, args.serve_rate)
     if not args.no_cpu:
         log("bench: CPU baseline")
-        out["cpu_baseline"] = cpu_baseline(prep, rays, min(args.cpu_sample, N_RAYS))
+        out["cpu_baseline"] = cpu_baseline(prep, rays, min(args.cpu_sample, N_RAYS), min(args.cpu_single, N_RAYS))
         out["config"]["hit_id_agreement"] = hit_agreement(prep, rays, hits, min(args.agree_sample, N_RAYS))
         log(f"bench: hit-ID agreement {out['config']['hit_id_agreement']}")
     print(json.dumps(out), flush=True)

@@ -272,6 +272,13 @@ int trt_render(trt_scene *scene, const void *params, uint32_t first_sample, uint
  * the reference, whose passes report timings only (gpu.rs:4198-4208). */
 int trt_render_ray_counts(trt_device *dev, uint64_t *closest, uint64_t *shadow);
 
+/* trt_render as one counting launch (the same samples into `acc`, the same result), blocking: reads[0] 64-byte wide
+ * nodes fetched, reads[1] triangles tested (48 bytes each), reads[2] 64-byte units entered, by every ray the launch
+ * traced; reads[3] hits shaded. A measurement with no counterpart in the reference (units.rs:922-927 Reads counts
+ * the CPU walk's reads the same way). */
+int trt_render_reads(trt_scene *scene, const void *params, uint32_t first_sample, uint32_t spp, trt_accum *acc,
+                     uint64_t *reads);
+
 /* Several render jobs in one launch (continuous batching for a render server; the reference's `serve` loop,
  * main.rs:345-398, renders one job at a time and has no counterpart): each job is trt_render's arguments; their
  * tiles share the GPU. A job's accumulator must not appear twice in one batch. Each job's result is the one

@@ -5,6 +5,7 @@ always as the checker (or the timed CPU restatement), never by the threert_amd p
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import os
 import subprocess
@@ -43,6 +44,25 @@ def build() -> str:
     if r.returncode != 0:
         raise RuntimeError(f"building the oracle failed:\n{r.stdout}\n{r.stderr}")
     return LIB_PATH
+
+
+NATIVE_PATH = os.path.join(HERE, "_native", "liboracle.so")
+
+
+@contextlib.contextmanager
+def native():
+    """The oracle built for the CPU this runs on (-march=native, contraction still off: the same bits), compiled here
+    and used for the calls inside the block: the CPU baseline's copy."""
+    global _lib, LIB_PATH
+    r = subprocess.run(["make", "-B", "-C", HERE, "native"], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"building the native oracle failed:\n{r.stdout}\n{r.stderr}")
+    saved = (_lib, LIB_PATH)
+    _lib, LIB_PATH = None, NATIVE_PATH
+    try:
+        yield lib()
+    finally:
+        _lib, LIB_PATH = saved
 
 
 _lib = None

@@ -75,8 +75,10 @@ struct trt_device {
     std::atomic<int> refs{1};
 };
 
-// d->reads: [0..2] a counting trace launch's reads, [kRenderRays], [kRenderRays + 1] rays the render launches traced
+// d->reads: [0..2] a counting trace launch's reads, [kRenderRays], [kRenderRays + 1] rays the render launches traced,
+// [kRenderReads .. + 3] a counting render launch's reads (nodes, triangles, units, shaded hits)
 constexpr int kRenderRays = 4;
+constexpr int kRenderReads = 8;
 
 struct trt_scene {
     trt_device *dev = nullptr;
@@ -1077,14 +1079,15 @@ static int job_args(const trt_render_job &j, trt::RenderArgs &a, std::string *wh
     return TRT_OK;
 }
 
-int trt_render_batch(trt_device *d, const trt_render_job *jobs, uint32_t n) {
+// reads: NULL, or the counters a counting launch adds to (the kernel with counted walks).
+static int render_batch(trt_device *d, const trt_render_job *jobs, uint32_t n, unsigned long long *reads) {
     if (!d || (n && !jobs)) return fail(TRT_E_INVALID, "trt_render_batch: null argument");
     if (n > 4096) return fail(TRT_E_INVALID, "trt_render_batch: more than 4096 jobs in one launch");
     std::vector<trt::RenderJob> table;
     table.reserve(n);
     uint64_t tiles = 0;
     uint32_t deepest = 0;
-    bool coat = false;
+    bool coat = false, tree = false;
     for (uint32_t k = 0; k < n; k++) {
         if (jobs[k].spp == 0) continue;
         if (!jobs[k].scene || jobs[k].scene->dev != d)
@@ -1100,6 +1103,7 @@ int trt_render_batch(trt_device *d, const trt_render_job *jobs, uint32_t n) {
         tiles += uint64_t((jobs[k].acc->w + 7) / 8) * ((jobs[k].acc->h + 7) / 8);
         deepest = std::max(deepest, jobs[k].scene->max_stack);
         coat = coat || jobs[k].scene->coat;
+        tree = tree || trt::from_bits(j.args.params[87]) > 0.5f;  // P.ltree.w: the job samples the light tree
         table.push_back(j);
     }
     if (tiles >= (uint64_t{1} << 31)) return fail(TRT_E_INVALID, "trt_render_batch: too many tiles in one launch");
@@ -1140,10 +1144,35 @@ int trt_render_batch(trt_device *d, const trt_render_job *jobs, uint32_t n) {
     TRT_HIP(hipMemcpyAsync(ring.dev[k], ring.host[k], bytes, hipMemcpyHostToDevice, d->stream));
     if (trt::launch_render(static_cast<const trt::RenderJob *>(ring.dev[k]), static_cast<uint32_t>(table.size()),
                            static_cast<uint32_t>(tiles), d->counter, d->spill, threads,
-                           static_cast<uint32_t>(entries), d->guard, d->reads + kRenderRays, coat, grid, d->stream))
+                           static_cast<uint32_t>(entries), d->guard, d->reads + kRenderRays, coat, tree, reads, grid,
+                           d->stream))
         return fail(TRT_E_DEVICE, "render kernel launch failed");
     TRT_HIP(hipEventRecord(ring.ev[k], d->stream));
     return ok();
+}
+
+int trt_render_batch(trt_device *d, const trt_render_job *jobs, uint32_t n) { return render_batch(d, jobs, n, nullptr); }
+
+int trt_render_reads(trt_scene *s, const void *params, uint32_t first_sample, uint32_t spp, trt_accum *acc,
+                     uint64_t *reads) {
+    if (!s || !reads) return fail(TRT_E_INVALID, "trt_render_reads: null argument");
+    trt_device *d = s->dev;
+    {
+        std::lock_guard<std::mutex> lock(d->mu);
+        TRT_HIP(hipSetDevice(d->ordinal));
+        TRT_HIP(hipMemsetAsync(d->reads + kRenderReads, 0, 4 * sizeof(unsigned long long), d->stream));
+    }
+    const trt_render_job j{s, params, first_sample, spp, acc};
+    int rc = render_batch(d, &j, 1, d->reads + kRenderReads);
+    if (rc) return rc;
+    {
+        std::lock_guard<std::mutex> lock(d->mu);
+        unsigned long long r[4] = {0, 0, 0, 0};
+        TRT_HIP(hipMemcpyAsync(r, d->reads + kRenderReads, sizeof(r), hipMemcpyDeviceToHost, d->stream));
+        TRT_HIP(hipStreamSynchronize(d->stream));
+        for (int k = 0; k < 4; k++) reads[k] = r[k];
+    }
+    return trt_device_sync(d);
 }
 
 int trt_render_ray_counts(trt_device *d, uint64_t *closest, uint64_t *shadow) {

@@ -179,6 +179,7 @@ struct Ctx {
     bool mirror_path;
     bool sampled_spec;
     V mis_x, mis_n;  // pathtrace.wgsl:438-439: the vertex direct_light last sampled from, for emit_mis
+    Reads *cnt;      // what a counting launch's walks read (and cnt->shaded: hits shaded)
 
     // Every table index is checked against its table: one outside reads entry 0 and reports kFlagTable.
     __device__ __forceinline__ uint64_t ck(uint64_t i, uint64_t n, uint32_t table,
@@ -204,11 +205,11 @@ struct Ctx {
     __device__ __forceinline__ float obj_power(uint32_t obj) const {
         return a->objects[ck(obj, a->object_count, 21) * kObjectStride].w;
     }
+    template <bool kCount>
     __device__ HitRec trace(V o, V d, float tmax, uint32_t mode) const {
-        Reads cnt;
         traced[0] += mode == kTraceClosest ? 1u : 0u;
         traced[1] += mode == kTraceClosest ? 0u : 1u;
-        return trace_wide<false>(a->geo, F3{o.x, o.y, o.z}, F3{d.x, d.y, d.z}, tmax, mode, *st, *flags, cnt);
+        return trace_wide<kCount>(a->geo, F3{o.x, o.y, o.z}, F3{d.x, d.y, d.z}, tmax, mode, *st, *flags, *cnt);
     }
 };
 enum { CAM_POS = 0, CAM_FWD = 1, CAM_RIGHT = 2, CAM_UP = 3, SIZE = 4, COUNTS = 5, ENV = 6, MISC = 7, SKY = 8,
@@ -481,8 +482,9 @@ __device__ __forceinline__ V offset_origin(V p, V n) {
     const float m = fmaxf(1.0f, fmaxf(fabsf(p.x), fmaxf(fabsf(p.y), fabsf(p.z))));
     return vadd(p, vs(n, 1e-4f * m));
 }
+template <bool kCount>
 __device__ __forceinline__ bool occluded(const Ctx &c, V o, V d, float tmax, uint32_t mode) {
-    return c.trace(o, d, tmax, mode).tri != kNone;
+    return c.trace<kCount>(o, d, tmax, mode).tri != kNone;
 }
 __device__ __forceinline__ V sky(const Ctx &c) { return v3(c.pf(SKY, 0), c.pf(SKY, 1), c.pf(SKY, 2)); }
 __device__ __forceinline__ float env_pdf() { return 1.0f / kFourPi; }
@@ -594,13 +596,16 @@ struct EPick {
 };
 // emitter_draw (547-585): without the tree the CDF; with it the tail by its share, else the tree descended by
 // importance at (x, n) to a leaf and one of its entries by weight
+// kTree: the launch has a scene whose Params ask for the light tree; without it the tree's code is compiled out (the
+// reference's pipelines are specialised to the features a scene uses), which keeps its registers from the others.
+template <bool kTree>
 __device__ EPick emitter_draw(const Ctx &c, V x, V n, uint32_t rd) {
     float r3[3];
     c.g.r3(rd, r3);
     const float4 tot = emitter_totals(c);
     const EPick none{make_float4(0.0f, 0.0f, 0.0f, 0.0f), 0.0f, 0.0f};
     if (!(tot.x > 0.0f)) return none;
-    if (!lt_on(c)) {
+    if (!kTree || !lt_on(c)) {
         const float4 e = emitter_pick_u(c, r3[0]);
         return {e, e.z / tot.x, e.z};
     }
@@ -649,11 +654,12 @@ struct Em {
     uint32_t tri, inst, code, part, flags;
 };
 // emitter_pmf (590-618): the probability emitter_draw at (x, n) picks emitter `ei`
+template <bool kTree>
 __device__ float emitter_pmf(const Ctx &c, uint32_t ei, const Em &em, V x, V n) {
     const float4 tot = emitter_totals(c);
     if (!(tot.x > 0.0f)) return 0.0f;
     const uint32_t code = em.code & ~0x80000000u;
-    if ((code >> kLtPathBits) == 0u) return c.emitter_at(code).z / tot.x;
+    if (!kTree || (code >> kLtPathBits) == 0u) return c.emitter_at(code).z / tot.x;
     const uint32_t depth = (code >> kLtPathBits) - 1u;
     float p = 1.0f - tot.y;
     uint32_t w = __float_as_uint(lt_node(c, 0u).w);
@@ -695,13 +701,14 @@ struct ES {
     float dist, pdf_w, w;
 };
 
+template <bool kTree>
 __device__ ES sample_emitter_ris(Ctx &c, V x, V n, uint32_t m, const Surface &sf, const Lobes &lb, V v, uint32_t rd) {
     ES r{v3(0.0f, 0.0f, 0.0f), v3(0.0f, 1.0f, 0.0f), 0.0f, 0.0f, 0.0f};
     float wsum = 0.0f, pick_b1 = 0.0f, pick_b2 = 0.0f, pick_ph = 0.0f, pick_pdf = 0.0f;
     uint32_t pick_i = 0;
     for (uint32_t k = 0; k < m; k++) {
         const uint32_t dk = rng_sub(rd, k);
-        const EPick pk = emitter_draw(c, x, n, dk);
+        const EPick pk = emitter_draw<kTree>(c, x, n, dk);
         const float4 e = pk.e;
         const float pk_p = pk.p, pk_w = pk.w;
         float bb[3];
@@ -747,9 +754,10 @@ __device__ ES sample_emitter_ris(Ctx &c, V x, V n, uint32_t m, const Surface &sf
     return r;
 }
 
+template <bool kTree>
 __device__ ES sample_emitter(Ctx &c, V x, V n, uint32_t rd) {
     ES r{v3(0.0f, 0.0f, 0.0f), v3(0.0f, 1.0f, 0.0f), 0.0f, 0.0f, 0.0f};
-    const EPick pk = emitter_draw(c, x, n, rd);
+    const EPick pk = emitter_draw<kTree>(c, x, n, rd);
     const float4 e = pk.e;
     const float pk_p = pk.p;
     if (pk_p <= 0.0f) return r;
@@ -777,6 +785,7 @@ __device__ ES sample_emitter(Ctx &c, V x, V n, uint32_t rd) {
     return r;
 }
 
+template <bool kTree, bool kCount>
 __device__ V direct_light(Ctx &c, const Surface &s, const Lobes &lb, V v) {
     V sum = v3(0.0f, 0.0f, 0.0f);
     const V o = offset_origin(s.p, s.ng);
@@ -798,7 +807,7 @@ __device__ V direct_light(Ctx &c, const Surface &s, const Lobes &lb, V v) {
         if (all_nonpos(col)) continue;
         const V f = eval_bsdf_k(s, lb, v, l, li[4].y);
         if (any_pos(f)) {
-            const bool vis = li[3].w >= 0.5f ? !occluded(c, o, l, kInf, kTraceShadow) : true;
+            const bool vis = li[3].w >= 0.5f ? !occluded<kCount>(c, o, l, kInf, kTraceShadow) : true;
             if (vis) {
                 const float pdf_l = 1.0f / (kTwoPi * (1.0f - cos_max));
                 const float wgt = light_mis(pdf_l, s, lb, v, l);
@@ -813,19 +822,20 @@ __device__ V direct_light(Ctx &c, const Surface &s, const Lobes &lb, V v) {
         ES es;
         const bool seen = c.g.depth == 0u || (c.g.depth == 1u && c.mirror_path);
         // with the tree its candidates are already near the point: P.ltree.w of them at camera hits, one elsewhere
-        if (lt_on(c))
-            es = sample_emitter_ris(c, o, ne, c.g.depth == 0u ? static_cast<uint32_t>(c.pf(LTREE, 3)) : 1u, s, lb, v,
+        if (kTree && lt_on(c))
+            es = sample_emitter_ris<kTree>(c, o, ne, c.g.depth == 0u ? static_cast<uint32_t>(c.pf(LTREE, 3)) : 1u, s, lb, v,
                                     c.g.dim(D_EMIT));
         else if (seen)
-            es = sample_emitter_ris(c, o, ne, c.g.depth == 0u ? kEmitRisM : kEmitRisM / 2u, s, lb, v, c.g.dim(D_EMIT));
+            es = sample_emitter_ris<kTree>(c, o, ne, c.g.depth == 0u ? kEmitRisM : kEmitRisM / 2u, s, lb, v,
+                                           c.g.dim(D_EMIT));
         else if (c.pf(LIGHT_LOD, 3) > 0.0f)
-            es = sample_emitter_ris(c, o, ne, 2u, s, lb, v, c.g.dim(D_EMIT));
+            es = sample_emitter_ris<kTree>(c, o, ne, 2u, s, lb, v, c.g.dim(D_EMIT));
         else
-            es = sample_emitter(c, o, ne, c.g.dim(D_EMIT));
+            es = sample_emitter<kTree>(c, o, ne, c.g.dim(D_EMIT));
         if (es.pdf_w > 0.0f) {
             const V f = eval_bsdf(s, lb, v, es.l);
             if (any_pos(f)) {
-                if (!occluded(c, o, es.l, es.dist * 0.999f, kTraceAny)) {
+                if (!occluded<kCount>(c, o, es.l, es.dist * 0.999f, kTraceAny)) {
                     const float wgt = light_mis(es.pdf_w, s, lb, v, es.l);
                     sum = vadd(sum, vs(vs(vmul(es.le, f), wgt), es.w));
                 }
@@ -837,13 +847,14 @@ __device__ V direct_light(Ctx &c, const Surface &s, const Lobes &lb, V v) {
         const V f = eval_bsdf(s, lb, v, l);
         if (any_pos(f)) {
             const float pdf_e = env_pdf();
-            if (pdf_e > 0.0f && !occluded(c, o, l, kInf, kTraceShadow))
+            if (pdf_e > 0.0f && !occluded<kCount>(c, o, l, kInf, kTraceShadow))
                 sum = vadd(sum, vds(vs(vmul(sky(c), f), light_mis(pdf_e, s, lb, v, l)), pdf_e));
         }
     }
     return sum;
 }
 
+template <bool kTree>
 __device__ float emit_mis(const Ctx &c, const HitRec &hit, V d, float pdf) {
     if (!emitters_on(c)) return 1.0f;
     const uint32_t rows = __float_as_uint(c.inst(hit.inst)[6].w);
@@ -856,7 +867,7 @@ __device__ float emit_mis(const Ctx &c, const HitRec &hit, V d, float pdf) {
     const V cr = cross(em.e1, em.e2);
     const float area = 0.5f * len(cr);
     const float cos_l = fabsf(dot(normalize(cr), d));
-    const float sel = emitter_pmf(c, entry, em, c.mis_x, c.mis_n);
+    const float sel = emitter_pmf<kTree>(c, entry, em, c.mis_x, c.mis_n);
     if (!(sel > 0.0f)) return 1.0f;
     return power_heuristic(pdf, sel / fmaxf(area, 1e-12f) * hit.t * hit.t / fmaxf(cos_l, 1e-6f));
 }
@@ -868,7 +879,7 @@ __device__ __forceinline__ V clamp_lum(V col, float clamp_v) {
 }
 
 // One path of sample `index` at pixel (px, py): its radiance, the camera hit's albedo and normal.
-template <bool kCoat>
+template <bool kCoat, bool kTree, bool kCount>
 __device__ void path(Ctx &c, uint32_t px, uint32_t py, uint32_t index, V &radiance, V &albedo, V &normal) {
     const uint32_t w = c.pu(SIZE, 0), h = c.pu(SIZE, 1);
     c.g.seed = hash_hp(px ^ hash_hp(py ^ hash_hp(c.pu(SIZE, 3))));
@@ -905,7 +916,7 @@ __device__ void path(Ctx &c, uint32_t px, uint32_t py, uint32_t index, V &radian
     const uint32_t last = c.pu(COUNTS, 2);
     for (uint32_t depth = 0; depth <= last; depth++) {
         c.g.depth = depth;
-        const HitRec hit = c.trace(o, d, kInf, kTraceClosest);
+        const HitRec hit = c.trace<kCount>(o, d, kInf, kTraceClosest);
         if (hit.tri == kNone) {
             V col = vmul(thr, sky_mis(c, d, depth > 0u ? last_pdf : 0.0f));
             if (depth > 1u && !straight) col = clamp_lum(col, clamp_v);
@@ -913,6 +924,7 @@ __device__ void path(Ctx &c, uint32_t px, uint32_t py, uint32_t index, V &radian
             break;
         }
         const Surface s = surface<kCoat>(c, o, d, hit);
+        if (kCount) c.cnt->shaded++;
         if (depth == 0u) {
             if (s.card) {
                 const float mx = fmaxf(fmaxf(s.emissive.x, fmaxf(s.emissive.y, s.emissive.z)), 1e-6f);
@@ -924,7 +936,7 @@ __device__ void path(Ctx &c, uint32_t px, uint32_t py, uint32_t index, V &radian
         }
         if (any_pos(s.emissive) && (s.front || !s.card)) {
             float wgt = 1.0f;
-            if (depth > 0u && !specular_bounce) wgt = emit_mis(c, hit, d, last_pdf);
+            if (depth > 0u && !specular_bounce) wgt = emit_mis<kTree>(c, hit, d, last_pdf);
             V col = vs(vmul(thr, s.emissive), wgt);
             if (depth > 1u && !straight) col = clamp_lum(col, clamp_v);
             radiance = vadd(radiance, col);
@@ -934,7 +946,7 @@ __device__ void path(Ctx &c, uint32_t px, uint32_t py, uint32_t index, V &radian
         const V v = vneg(d);
         const float nov = fmaxf(dot(s.n, v), 1e-4f);
         const Lobes lb = lobes(c, s, nov);
-        V dl = vmul(thr, direct_light(c, s, lb, v));
+        V dl = vmul(thr, direct_light<kTree, kCount>(c, s, lb, v));
         if (depth > 0u && !straight) dl = clamp_lum(dl, clamp_v);
         radiance = vadd(radiance, dl);
         const V l = sample_bsdf(c, s, lb, v);
@@ -965,17 +977,19 @@ __device__ void path(Ctx &c, uint32_t px, uint32_t py, uint32_t index, V &radian
 // The path loop needs about 200 registers, which leaves two waves a SIMD; held to three (168 registers, about 50
 // spilled to scratch, mostly in the shading code), the kernel hides more of its memory latency: the 720p city
 // frame 2.46 -> 1.95 ms, the 3M-triangle scene 1.36x faster, the Cornell box 5 % (four waves: 2.00 ms, 1.34x, 11 %).
-template <bool kCoat>
-__global__ __launch_bounds__(kRenderBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_render(const RenderJob *jobs, uint32_t njobs, uint32_t total,
-                                                         uint32_t *counter, uint32_t *spill, uint32_t spill_stride,
-                                                         uint32_t spill_entries, uint32_t *guard,
-                                                         unsigned long long *rays) {
+// kCount: a counting launch, which adds what its walks read and the hits it shades to reads[0..3] (nodes,
+// triangles, units, shaded hits).
+template <bool kCoat, bool kTree, bool kCount>
+__global__ __launch_bounds__(kRenderBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_render(
+    const RenderJob *jobs, uint32_t njobs, uint32_t total, uint32_t *counter, uint32_t *spill, uint32_t spill_stride,
+    uint32_t spill_entries, uint32_t *guard, unsigned long long *rays, unsigned long long *reads) {
     __shared__ uint32_t lds[kLdsStack * kRenderBlock];
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     Stack st{(TRT_LDS uint32_t *)(lds + threadIdx.x), spill, gtid, spill_stride, 0, kLdsStack + spill_entries};
     uint32_t flags = 0;
     uint32_t traced[2] = {0, 0};
+    Reads cnt;
     for (;;) {
         uint32_t tile = 0;
         if (lane == 0) tile = atomicAdd(counter, 1u);
@@ -993,14 +1007,14 @@ __global__ __launch_bounds__(kRenderBlock) __attribute__((amdgpu_waves_per_eu(3)
         const RenderArgs *a = &job->args;
         const uint32_t t = tile - job->tile_first;
         const uint32_t w = a->params[16], h = a->params[17], tiles_x = (w + 7) / 8;
-        Ctx c{a, &st, &flags, traced, Rng{0, 0, 0}, false, false, v3(0.0f, 0.0f, 0.0f), v3(0.0f, 0.0f, 0.0f)};
+        Ctx c{a, &st, &flags, traced, Rng{0, 0, 0}, false, false, v3(0.0f, 0.0f, 0.0f), v3(0.0f, 0.0f, 0.0f), &cnt};
         const uint32_t px = (t % tiles_x) * 8 + (lane & 7u), py = (t / tiles_x) * 8 + (lane >> 3);
         if (px < w && py < h) {
             float4 *pix = a->accum + 5 * (uint64_t(py) * w + px);
             float4 col = pix[0], alb = pix[1], nrm = pix[2];
             for (uint32_t s = 0; s < a->spp; s++) {
                 V rad, al, nm;
-                path<kCoat>(c, px, py, a->first_sample + s, rad, al, nm);
+                path<kCoat, kTree, kCount>(c, px, py, a->first_sample + s, rad, al, nm);
                 const float lum = luminance(rad);
                 col.x += rad.x, col.y += rad.y, col.z += rad.z, col.w += 1.0f;
                 alb.x += al.x, alb.y += al.y, alb.z += al.z, alb.w += lum;
@@ -1021,6 +1035,12 @@ __global__ __launch_bounds__(kRenderBlock) __attribute__((amdgpu_waves_per_eu(3)
         atomicAdd(rays, (unsigned long long)traced[0]);
         atomicAdd(rays + 1, (unsigned long long)traced[1]);
     }
+    if (kCount) {
+        atomicAdd(reads + 0, (unsigned long long)cnt.nodes);
+        atomicAdd(reads + 1, (unsigned long long)cnt.tris);
+        atomicAdd(reads + 2, (unsigned long long)cnt.units);
+        atomicAdd(reads + 3, (unsigned long long)cnt.shaded);
+    }
 }
 
 __global__ void k_reset(uint32_t *c) { *c = 0; }
@@ -1029,21 +1049,35 @@ __global__ void k_reset(uint32_t *c) { *c = 0; }
 
 int render_occupancy(int *blocks_per_cu) {
     int b = 0;
-    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_render<false>, kRenderBlock, 0);
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_render<false, false, false>, kRenderBlock, 0);
     *blocks_per_cu = b;
     return e == hipSuccess ? 0 : -1;
 }
 
 int launch_render(const RenderJob *d_jobs, uint32_t njobs, uint32_t total_tiles, uint32_t *counter,
                   uint32_t *spill, uint32_t spill_stride, uint32_t spill_entries, uint32_t *guard,
-                  unsigned long long *rays, bool coat, int grid, hipStream_t s) {
+                  unsigned long long *rays, bool coat, bool tree, unsigned long long *reads, int grid,
+                  hipStream_t s) {
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, s, counter);
-    if (coat)
-        hipLaunchKernelGGL(k_render<true>, dim3(grid), dim3(kRenderBlock), 0, s, d_jobs, njobs, total_tiles, counter,
-                           spill, spill_stride, spill_entries, guard, rays);
-    else
-        hipLaunchKernelGGL(k_render<false>, dim3(grid), dim3(kRenderBlock), 0, s, d_jobs, njobs, total_tiles, counter,
-                           spill, spill_stride, spill_entries, guard, rays);
+#define TRT_RENDER(C, T, N)                                                                                          \
+    hipLaunchKernelGGL((k_render<C, T, N>), dim3(grid), dim3(kRenderBlock), 0, s, d_jobs, njobs, total_tiles,        \
+                       counter, spill, spill_stride, spill_entries, guard, rays, reads)
+#define TRT_RENDER_N(C, T)          \
+    if (reads)                      \
+        TRT_RENDER(C, T, true);     \
+    else                            \
+        TRT_RENDER(C, T, false)
+    if (coat && tree) {
+        TRT_RENDER_N(true, true);
+    } else if (coat) {
+        TRT_RENDER_N(true, false);
+    } else if (tree) {
+        TRT_RENDER_N(false, true);
+    } else {
+        TRT_RENDER_N(false, false);
+    }
+#undef TRT_RENDER_N
+#undef TRT_RENDER
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

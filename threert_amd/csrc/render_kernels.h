@@ -49,9 +49,12 @@ int launch_resolve(const float4 *acc, uint32_t w, uint32_t h, uint32_t pw, uint3
                    float *normal, hipStream_t s);
 // d_jobs: njobs jobs in device memory, tile_first ascending from 0; total_tiles: all their tiles.
 // rays: two counters the launch adds its traced rays to (closest; shadow and any-hit); coat: some job's scene has a
-// material with a clearcoat (the kernel with the coat's terms; the same bits for scenes without one)
+// material with a clearcoat (the kernel with the coat's terms; the same bits for scenes without one); tree: some
+// job's Params ask for the light tree (the kernel with its code); reads: NULL, or four counters a counting launch adds
+// to (wide nodes, triangles, units its walks read; hits it shades)
 int launch_render(const RenderJob *d_jobs, uint32_t njobs, uint32_t total_tiles, uint32_t *counter,
                   uint32_t *spill, uint32_t spill_stride, uint32_t spill_entries, uint32_t *guard,
-                  unsigned long long *rays, bool coat, int grid, hipStream_t s);
+                  unsigned long long *rays, bool coat, bool tree, unsigned long long *reads, int grid,
+                  hipStream_t s);
 
 }  // namespace trt

@@ -167,6 +167,7 @@ __device__ __forceinline__ void store_hit(uint32_t *hits, uint64_t i, const HitR
 // What a walk read (units.rs:922-927 Reads): nodes fetched, triangles tested, unit records read.
 struct Reads {
     uint32_t nodes = 0, tris = 0, units = 0;
+    uint32_t shaded = 0;  // the path tracer's counting launches: hits shaded
 };
 
 // ---- the reference's binary walk ------------------------------------------------------------------------------

@@ -430,6 +430,15 @@ def render(scene: Scene, params: np.ndarray, first_sample: int, spp: int, acc: A
     _check(lib().trt_render(scene.h, _ptr(params), first_sample, spp, acc.handle))
 
 
+def render_reads(scene: Scene, params: np.ndarray, first_sample: int, spp: int, acc: Accum) -> dict:
+    """render as one counting launch (trt_render_reads), blocking: what its walks read (64-byte wide nodes,
+    48-byte triangles, 64-byte units) and the hits it shaded."""
+    params = np.ascontiguousarray(params, np.uint32)
+    out = np.zeros(4, np.uint64)
+    _check(lib().trt_render_reads(scene.h, _ptr(params), first_sample, spp, acc.handle, _ptr(out)))
+    return {"nodes": int(out[0]), "tris": int(out[1]), "units": int(out[2]), "shaded": int(out[3])}
+
+
 def _sig():
     L = lib()
     P, U32, I = C.c_void_p, C.c_uint32, C.c_int
@@ -441,8 +450,9 @@ def _sig():
     L.trt_accum_resolve_device.argtypes = [P, U32, U32, P, P, P]
     L.trt_accum_add_fp64.argtypes = [P, P]
     L.trt_render.argtypes = [P, P, U32, U32, P]
+    L.trt_render_reads.argtypes = [P, P, U32, U32, P, P]
     for n in ("trt_accum_create", "trt_accum_release", "trt_accum_clear", "trt_accum_read", "trt_accum_resolve",
-              "trt_accum_resolve_device", "trt_accum_add_fp64", "trt_render"):
+              "trt_accum_resolve_device", "trt_accum_add_fp64", "trt_render", "trt_render_reads"):
         getattr(L, n).restype = I
 
 
