@@ -308,6 +308,10 @@ typedef struct trt_unet trt_unet;
 int trt_unet_create(trt_device *dev, uint64_t max_pixels, trt_unet **out);
 int trt_unet_release(trt_unet *net);
 int trt_unet_param_count(trt_unet *net, uint64_t *count);
+/* The level height from which the convolutions take 16-pixel-tall tiles instead of 8 (default 128; UINT32_MAX: never);
+ * the same sums either way, so the same bits. Process-wide. A tuning and test knob with no counterpart in the
+ * reference. */
+int trt_unet_set_tall_tiles(uint32_t min_rows);
 /* Sets the parameters (host memory) and restarts the optimizer. */
 int trt_unet_params_set(trt_unet *net, const float *params, uint64_t count);
 int trt_unet_params_get(trt_unet *net, float *params, uint64_t count);

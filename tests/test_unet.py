@@ -104,8 +104,34 @@ def test_gpu_forward_matches_the_oracle(device):
     ref.forward(color, albedo, nrm)
     want = ref.denoised()
     print("forward rel L2", _rel(got, want), flush=True)
-    assert _rel(got, want) <= 1e-2
+    assert _rel(got, want) <= 3e-4  # measured 1.05e-4
     net.close()
+
+
+@pytest.mark.gpu
+def test_tall_tiles_give_the_short_tiles_bits(device):
+    """The same input and weights with 16-pixel-tall tiles at the levels 128 pixels tall or more (the default) and
+    with 8-pixel tiles everywhere: the forward outputs are the same bits. The gradients agree within 1e-5 relative
+    L2: the weight gradients are summed with FP32 atomics, whose order differs from run to run even on one tiling
+    (a slip in the tall tiles' halo or strips would move them by orders of magnitude more)."""
+    print("unet: tall against short tiles, 1 x 144 x 64 and 2 x 128 x 32", flush=True)
+    params = U.init_params(12)
+    params[U.layout()[-1][3]:U.layout()[-1][3] + 3] = 0.3
+    for n, h, w in ((1, 144, 64), (2, 128, 32)):
+        color, albedo, nrm, target = _batch(np.random.default_rng(h), n, h, w)
+        out = {}
+        for rows in (128, 0xFFFFFFFF):
+            U.set_tall_tiles(rows)
+            try:
+                net = U.UNet(device, n * h * w, params)
+                out[rows] = (net.denoise(color, albedo, nrm), net.gradients(color, albedo, nrm, target))
+                net.close()
+            finally:
+                U.set_tall_tiles(128)
+        (d0, (l0, g0)), (d1, (l1, g1)) = out[128], out[0xFFFFFFFF]
+        assert (d0.view(np.uint32) == d1.view(np.uint32)).all(), (n, h, w)
+        assert abs(l0 - l1) <= 1e-6 * abs(l0)
+        assert _rel(g0, g1) <= 1e-5, (n, h, w, _rel(g0, g1))
 
 
 @pytest.mark.gpu
@@ -123,7 +149,7 @@ def test_gpu_forward_on_tall_tiles_matches_the_oracle(device):
     ref.forward(color, albedo, nrm)
     want = ref.denoised()
     print("forward rel L2", _rel(got, want), flush=True)
-    assert _rel(got, want) <= 1e-2
+    assert _rel(got, want) <= 8e-3  # measured 3.96e-3
     net.close()
 
 

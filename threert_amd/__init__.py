@@ -46,7 +46,7 @@ EXPORTS = [
     "trt_accum_resolve_device", "trt_accum_add_fp64", "trt_device_set_build", "trt_scene_wide_read",
     "trt_unet_create", "trt_unet_release", "trt_unet_param_count", "trt_unet_params_set", "trt_unet_params_get",
     "trt_unet_denoise", "trt_unet_gradients", "trt_unet_train_step", "trt_trace_rays_device_noalloc",
-    "trt_build_light_tree", "trt_light_tree_free", "trt_render_reads",
+    "trt_build_light_tree", "trt_light_tree_free", "trt_render_reads", "trt_unet_set_tall_tiles",
 ]
 
 

@@ -11,6 +11,8 @@
 // so every MFMA operand is one aligned 16-byte LDS read, then adds its FP32 partial sums to dW with atomics.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include <algorithm>
 
 #include "unet_kernels.h"
@@ -327,10 +329,14 @@ inline int pick_nco(uint32_t cout) { return cout % 64 == 0 ? 4 : (cout % 32 == 0
 
 }  // namespace
 
+// The level height from which convolutions take 16-pixel-tall tiles (the same sums as 8-pixel tiles, so the same bits).
+static std::atomic<uint32_t> g_tall_min{128};
+void set_conv_tall_min(uint32_t rows) { g_tall_min = rows; }
+
 int launch_conv(const ConvArgs &a, hipStream_t s) {
     const uint32_t cin = a.c0 + a.c1;
     if (a.cout % 16 || cin % 32 || a.c0 % 32 || (a.c1 && (a.h % 2 || a.w % 2)) || cin > 512) return -1;
-    const bool tall = a.h >= 128;  // 16-pixel-tall tiles for the large levels
+    const bool tall = a.h >= g_tall_min.load() && a.h >= 16;  // 16-pixel-tall tiles for the large levels
     const uint32_t th = tall ? 16 : 8;
     const uint32_t tx = (a.w + 7) / 8, ty = (a.h + th - 1) / th;
     const uint64_t tiles = uint64_t(tx) * ty * a.n;

@@ -21,6 +21,8 @@ struct ConvArgs {
     uint32_t c0, c1, cout, n, h, w, relu;
 };
 int launch_conv(const ConvArgs &a, hipStream_t s);
+// Levels at least `rows` tall take 16-pixel-tall convolution tiles (default 128; UINT32_MAX: none).
+void set_conv_tall_min(uint32_t rows);
 
 // dw[co][tap][ci] += sum over pixels of dy[p][co] * x[p + tap][ci], x = concat(in0, up2(in1)) zero-padded;
 // db[co] += sum over pixels of dy[p][co]. FP32 atomics into dw / db (zero them first).

@@ -306,6 +306,11 @@ int trt_unet_release(trt_unet *u) {
     return trt_ok();
 }
 
+int trt_unet_set_tall_tiles(uint32_t min_rows) {
+    trt::set_conv_tall_min(min_rows);
+    return trt_ok();
+}
+
 int trt_unet_param_count(trt_unet *u, uint64_t *count) {
     if (!u || !count) return trt_fail(TRT_E_INVALID, "trt_unet_param_count: null argument");
     *count = u->nparam;

@@ -70,12 +70,19 @@ def _sig():
     L.trt_unet_denoise.argtypes = [P, U32, U32, U32, P, P, P, P]
     L.trt_unet_gradients.argtypes = [P, U32, U32, U32, P, P, P, P, P, P]
     L.trt_unet_train_step.argtypes = [P, U32, U32, U32, P, P, P, P, C.c_float, P]
+    L.trt_unet_set_tall_tiles.argtypes = [U32]
     for n in ("trt_unet_create", "trt_unet_release", "trt_unet_param_count", "trt_unet_params_set",
-              "trt_unet_params_get", "trt_unet_denoise", "trt_unet_gradients", "trt_unet_train_step"):
+              "trt_unet_params_get", "trt_unet_denoise", "trt_unet_gradients", "trt_unet_train_step",
+              "trt_unet_set_tall_tiles"):
         getattr(L, n).restype = I
 
 
 _sig()
+
+
+def set_tall_tiles(min_rows: int):
+    """Levels at least min_rows tall take 16-pixel-tall convolution tiles (default 128; 2^32 - 1: never)."""
+    _check(lib().trt_unet_set_tall_tiles(min_rows))
 
 
 def _as_nhwc3(a) -> np.ndarray:
