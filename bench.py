@@ -115,7 +115,56 @@ def fp64_check(dev, sc, d_rays, d_hits, hits) -> dict:
     q = np.quantile(rel, [0.5, 0.99, 0.999]) if len(rel) else [0.0, 0.0, 0.0]
     return {"hits": int(hit.sum()), "disputed": int((t64[hit] == -1).sum()), "invalid": int((t64 == -2).sum()),
             "t_rel_err": {"p50": float(q[0]), "p99": float(q[1]), "p999": float(q[2]),
-                          "max": float(rel.max()) if len(rel) else 0.0}}
+                          "max": float(rel.max()) if len(rel) else 0.0},
+            "disputed_rays": np.nonzero(hit & (t64 == -1))[0]}
+
+
+def watertight_walk(dev, sc, prep, rays, d_rays, d_hits, mt_hits, disputed, launches: int = 5) -> dict:
+    """The same batch through the wide walk with the watertight triangle test (TRT_WALK_WATERTIGHT): rays/s, the FP64
+    check of its hits, and on the rays whose Moller-Trumbore hit the FP64 test disputes, both tests' answers against
+    the double-precision brute force (every triangle, ray placement and Moller-Trumbore in double): the same
+    triangle, or one sharing a vertex with it (a ray within rounding of an edge)."""
+    import oracle
+
+    sc.trace_device(d_rays, N_RAYS, d_hits, T.WALK_WATERTIGHT)
+    dev.sync()
+    dev.timer_start()
+    for _ in range(launches):
+        sc.trace_device(d_rays, N_RAYS, d_hits, T.WALK_WATERTIGHT)
+    ms = dev.timer_stop() / launches
+    wt = d_hits.download(T.HIT, N_RAYS)
+    check = fp64_check(dev, sc, d_rays, d_hits, wt)
+    idx = disputed
+    f64 = oracle.Scene(*prep.desc()).brute64_batch(rays[idx].view(oracle.RAY),
+                                                   threads=int(os.environ.get("OMP_NUM_THREADS", "0")))
+    leaves = prep.leaves.reshape(-1, 3, 4)[:, :, :3].astype(np.float64)
+
+    def corners(t):
+        v0, e1, e2 = leaves[t]
+        return np.stack([v0, v0 + e1, v0 + e2])
+
+    def verdict(h):
+        same = touch = 0
+        for k, i in enumerate(idx):
+            a, b = int(h["tri"][i]), int(f64["tri"][k])
+            if b == 0xFFFFFFFF or a == 0xFFFFFFFF:
+                same += a == b
+                continue
+            if a == b:
+                same += 1
+                continue
+            ca, cb = corners(a), corners(b)
+            touch += bool((np.abs(ca[:, None] - cb[None]).sum(-1) < 1e-6 * (1 + np.abs(cb).max())).any())
+        return {"same_triangle": int(same), "a_neighbour": int(touch), "other": int(len(idx) - same - touch)}
+
+    out = {"mrays_s": round(N_RAYS / (ms * 1e-3) / 1e6, 1), "ms_per_launch": round(ms, 3),
+           "fp64_check": {k: v for k, v in check.items() if k != "disputed_rays"},
+           "same_hits_as_moller_trumbore": round(float((wt.view(np.uint32).reshape(-1, 6) ==
+                                                        mt_hits.view(np.uint32).reshape(-1, 6)).all(1).mean()), 6),
+           "on_the_mt_disputed_rays": {"rays": int(len(idx)), "fp64_misses": int((f64["tri"] == 0xFFFFFFFF).sum()),
+                                       "moller_trumbore": verdict(mt_hits), "watertight": verdict(wt)}}
+    log(f"bench: watertight walk {out}")
+    return out
 
 
 def binary_walk(dev, sc, d_rays, d_hits, wide_hits, launches: int = 3) -> dict:
@@ -601,7 +650,9 @@ def main():
                     "hits_equal": bool((host_hits.view(np.uint32) == hits.view(np.uint32)).all())}
     log(f"bench: host buffers (PCIe both ways) {host_buffers}")
     fp64 = fp64_check(dev, sc, d_rays, d_hits, hits)
+    disputed = fp64.pop("disputed_rays")
     log(f"bench: FP64 check {fp64}")
+    watertight = watertight_walk(dev, sc, prep, rays, d_rays, d_hits, hits, disputed)
     binary = binary_walk(dev, sc, d_rays, d_hits, hits)
     log(f"bench: binary walk {binary}")
 
@@ -632,6 +683,7 @@ def main():
             "ms_median_single_launch": round(ms_median, 4),
             "mrays_s_median": round(N_RAYS / (ms_median * 1e-3) / 1e6, 2),
             "fp64_check": fp64,
+            "watertight_walk": watertight,
             "binary_walk": binary,
             "tree_build": build,
             "host_buffers_pcie": host_buffers,

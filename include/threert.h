@@ -37,6 +37,9 @@ extern "C" {
 /* trt_trace_rays flags: which node format the kernel walks. */
 #define TRT_WALK_WIDE 0u   /* the library's 64-byte 4-wide quantised tree (default)            */
 #define TRT_WALK_BINARY 1u /* the reference's own 32-byte binary node buffer, as uploaded      */
+#define TRT_WALK_WATERTIGHT 2u /* the 4-wide walk with the watertight triangle test (Woop, Benthin and Wald 2013)
+                                * in place of the reference's Moller-Trumbore: a ray on an edge two triangles share
+                                * meets one of them. No counterpart in the reference (pathtrace.wgsl:742-760 is MT) */
 
 #define TRT_NO_HIT 0xffffffffu /* trt_hit.tri / trt_hit.inst of a ray that hit nothing */
 

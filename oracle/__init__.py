@@ -19,11 +19,13 @@ NODE = np.dtype([("bmin", "<f4", 3), ("left_or_first", "<u4"), ("bmax", "<f4", 3
 INSTANCE = np.dtype([("m", "<f4", (3, 4)), ("n", "<f4", (3, 4)), ("info", "<u4", 4), ("p", "<f4", (3, 4))])
 RAY = np.dtype([("o", "<f4", 3), ("tmax", "<f4"), ("d", "<f4", 3), ("mode", "<u4")])
 HIT = np.dtype([("t", "<f4"), ("tri", "<u4"), ("u", "<f4"), ("v", "<f4"), ("inst", "<u4"), ("part", "<u4")])
+HIT64 = np.dtype([("t", "<f8"), ("u", "<f8"), ("v", "<f8"), ("tri", "<u4"), ("inst", "<u4")])
 READS = np.dtype([("top_nodes", "<u8"), ("mesh_nodes", "<u8"), ("mesh_verts", "<u8"), ("records", "<u8"),
                   ("steps", "<u8"), ("deepest", "<u8")])
 
 WALK_REF = 0
 WALK_EXACT = 1
+WALK_WT = 2  # EXACT with the watertight triangle test
 INF = np.float32(3.0e38)
 NO_HIT = 0xFFFFFFFF
 
@@ -243,6 +245,18 @@ class Scene:
         if failed:
             raise RuntimeError(f"{failed} walks read outside their buffers or did not stop")
         return hits, reads[0]
+
+    def brute64_batch(self, rays: np.ndarray, threads=0) -> np.ndarray:
+        """The closest hit in double precision by brute force (or_brute64_batch): HIT64 records, t -1 and tri
+        NO_TRI for a miss; ties to the lowest (instance, triangle)."""
+        rays = np.ascontiguousarray(rays, dtype=RAY)
+        hits = np.zeros(len(rays), HIT64)
+        L = lib()
+        L.or_brute64_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_int, C.c_void_p]
+        L.or_brute64_batch.restype = C.c_int64
+        if L.or_brute64_batch(C.byref(self._s), _ptr(rays), len(rays), threads, _ptr(hits)):
+            raise RuntimeError("FP64 brute force read outside its buffers")
+        return hits
 
     def brute_batch(self, rays: np.ndarray, variant=WALK_EXACT, threads=0):
         rays = np.ascontiguousarray(rays, dtype=RAY)

@@ -116,6 +116,8 @@ typedef struct {
  * triangle on a flat box's face), so only a box with a margin finds what brute force finds, whatever the tree. */
 #define OR_WALK_REF 0u
 #define OR_WALK_EXACT 1u
+/* WT: EXACT with the watertight triangle test (Woop, Benthin and Wald 2013) in place of Moller-Trumbore. */
+#define OR_WALK_WT 2u
 #define OR_CULL_SLACK (1.0f / 16384.0f)
 
 void or_tree_free(or_tree *t);
@@ -161,6 +163,15 @@ int or_brute(const or_scene *s, const float *o, const float *d, float tmax, uint
 /* A batch through or_trace on `threads` OpenMP threads (0: all). Returns the number of failed rays. */
 int64_t or_trace_batch(const or_scene *s, const or_ray *rays, uint64_t n, uint32_t variant, int threads,
                        or_hit *hits, or_reads *reads_total);
+/* The closest hit in double precision by brute force: every triangle of every unit (the ray placed in the unit's
+ * space as the walk places it, then the stored v0, e1, e2 by Moller-Trumbore, every operation in double); ties to the
+ * lowest (instance, triangle). t64 = -1 and tri = OR_NO_TRI for a miss. The judge of rays whose float tests disagree. */
+typedef struct {
+    double t, u, v;
+    uint32_t tri, inst;
+} or_hit64;
+int64_t or_brute64_batch(const or_scene *s, const or_ray *rays, uint64_t n, int threads, or_hit64 *hits);
+
 int64_t or_brute_batch(const or_scene *s, const or_ray *rays, uint64_t n, uint32_t variant, int threads,
                        or_hit *hits);
 

@@ -56,9 +56,10 @@ static float ray_aabb_wide(const float *o, const float *inv, const float *bmin, 
     float lo[3] = {bmin[0] - m, bmin[1] - m, bmin[2] - m}, hi[3] = {bmax[0] + m, bmax[1] + m, bmax[2] + m};
     return or_ray_aabb(o, inv, lo, hi, tmax);
 }
+static inline int exact(uint32_t variant) { return variant != OR_WALK_REF; }
 static inline float box_test(const float *o, const float *inv, const float *bmin, const float *bmax, float tmax,
                              uint32_t variant) {
-    return variant == OR_WALK_EXACT ? ray_aabb_wide(o, inv, bmin, bmax, tmax) : or_ray_aabb(o, inv, bmin, bmax, tmax);
+    return exact(variant) ? ray_aabb_wide(o, inv, bmin, bmax, tmax) : or_ray_aabb(o, inv, bmin, bmax, tmax);
 }
 
 void or_intersect_tri(const float *o, const float *d, const or_leaf *l, float tmax, float *tuv) {
@@ -82,6 +83,60 @@ void or_intersect_tri(const float *o, const float *d, const or_leaf *l, float tm
     tuv[0] = t;
     tuv[1] = u;
     tuv[2] = v;
+}
+
+/* The watertight test (Woop, Benthin and Wald, JCGT 2013): the vertices v0, v0 + e1, v0 + e2 translated to the
+ * ray's origin and sheared so the ray runs along +z (kz its largest component, the first of equals; kx, ky the next
+ * two, swapped when d[kz] < 0), the edge functions U, V, W of their projections (again in double where one rounds to
+ * exactly 0), the same signs or a miss, t = T / det; u = V / det and v = W / det weigh v1 and v2. */
+void or_intersect_tri_wt(const float *o, const float *d, const or_leaf *l, float tmax, float *tuv) {
+    tuv[0] = OR_INF;
+    tuv[1] = tuv[2] = 0.0f;
+    float ax_ = fabsf(d[0]), ay_ = fabsf(d[1]), az_ = fabsf(d[2]);
+    int kz = ax_ >= ay_ ? (ax_ >= az_ ? 0 : 2) : (ay_ >= az_ ? 1 : 2);
+    int kx = kz == 2 ? 0 : kz + 1, ky = kx == 2 ? 0 : kx + 1;
+    if (d[kz] < 0.0f) {
+        int t = kx;
+        kx = ky;
+        ky = t;
+    }
+    float sx = d[kx] / d[kz], sy = d[ky] / d[kz], sz = 1.0f / d[kz];
+    const float *v0 = l->r[0], *e1 = l->r[1], *e2 = l->r[2];
+    float v1[3] = {v0[0] + e1[0], v0[1] + e1[1], v0[2] + e1[2]};
+    float v2[3] = {v0[0] + e2[0], v0[1] + e2[1], v0[2] + e2[2]};
+    float A[3], B[3], C[3];
+    for (int k = 0; k < 3; k++) {
+        A[k] = v0[k] - o[k];
+        B[k] = v1[k] - o[k];
+        C[k] = v2[k] - o[k];
+    }
+    float ax = A[kx] - sx * A[kz], ay = A[ky] - sy * A[kz];
+    float bx = B[kx] - sx * B[kz], by = B[ky] - sy * B[kz];
+    float cx = C[kx] - sx * C[kz], cy = C[ky] - sy * C[kz];
+    float U = cx * by - cy * bx, V = ax * cy - ay * cx, W = bx * ay - by * ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        U = (float)((double)cx * by - (double)cy * bx);
+        V = (float)((double)ax * cy - (double)ay * cx);
+        W = (float)((double)bx * ay - (double)by * ax);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return;
+    float det = U + V + W;
+    if (det == 0.0f) return;
+    float T = U * (sz * A[kz]) + V * (sz * B[kz]) + W * (sz * C[kz]);
+    float rcp = 1.0f / det;
+    float t = T * rcp;
+    if (!(t > 0.0f) || t >= tmax) return;
+    tuv[0] = t;
+    tuv[1] = V * rcp;
+    tuv[2] = W * rcp;
+}
+
+static inline void tri_test(uint32_t variant, const float *o, const float *d, const or_leaf *l, float tmax,
+                            float *tuv) {
+    if (variant == OR_WALK_WT)
+        or_intersect_tri_wt(o, d, l, tmax, tuv);
+    else
+        or_intersect_tri(o, d, l, tmax, tuv);
 }
 
 void or_inv(const float *d, float *inv) {
@@ -108,12 +163,12 @@ static inline void unit_ray(const or_instance *u, const float *o, const float *d
 }
 
 static inline float clamp_tmax(float t) { return t < OR_INF ? t : OR_INF; }
-static inline float cull_t(float t, uint32_t variant) { return variant == OR_WALK_EXACT ? t + t * OR_CULL_SLACK : t; }
+static inline float cull_t(float t, uint32_t variant) { return exact(variant) ? t + t * OR_CULL_SLACK : t; }
 
 /* Whether a candidate (t, inst, tri), already inside (0, tmax), replaces the hit. */
 static inline int accept(const or_hit *h, float x, uint32_t inst, uint32_t tri, uint32_t variant) {
     if (x < h->t) return 1;
-    if (variant == OR_WALK_EXACT && x == h->t && h->tri != OR_NO_TRI)
+    if (exact(variant) && x == h->t && h->tri != OR_NO_TRI)
         return inst < h->inst || (inst == h->inst && tri < h->tri);
     return 0;
 }
@@ -194,7 +249,7 @@ int or_trace(const or_scene *s, const float *o, const float *d, float tmax_in, u
                 if ((uint64_t)ti >= s->leaf_count) return -1;
                 reads->mesh_verts += 3;
                 float x[3];
-                or_intersect_tri(ro, rdir, &s->leaves[ti], variant == OR_WALK_EXACT ? tmax : h.t, x);
+                tri_test(variant, ro, rdir, &s->leaves[ti], exact(variant) ? tmax : h.t, x);
                 if (x[0] < tmax && accept(&h, x[0], rec, ti, variant)) {
                     h.t = x[0];
                     h.tri = ti;
@@ -312,7 +367,7 @@ int or_brute(const or_scene *s, const float *o, const float *d, float tmax_in, u
                     goto fail;
                 }
                 float x[3];
-                or_intersect_tri(ro, rd, &s->leaves[ti], variant == OR_WALK_EXACT ? tmax : h.t, x);
+                tri_test(variant, ro, rd, &s->leaves[ti], exact(variant) ? tmax : h.t, x);
                 if (x[0] < tmax && accept(&h, x[0], rec, ti, variant)) {
                     h.t = x[0];
                     h.tri = ti;
@@ -376,6 +431,115 @@ int64_t or_brute_batch(const or_scene *s, const or_ray *rays, uint64_t n, uint32
     for (int64_t i = 0; i < (int64_t)n; i++) {
         if (or_brute(s, rays[i].o, rays[i].d, rays[i].tmax, rays[i].mode, variant, &hits[i])) failed++;
     }
+    (void)threads;
+    return failed;
+}
+
+/* ---- the FP64 brute force (or_brute64_batch) ------------------------------------------------------------------ */
+static int brute64(const or_scene *s, const float *of, const float *df, float tmax_in, or_hit64 *hit) {
+    or_hit64 h = {-1.0, 0.0, 0.0, OR_NO_TRI, OR_NO_INST};
+    double tmax = tmax_in < OR_INF ? (double)tmax_in : (double)OR_INF;
+    /* every record the top tree names */
+    uint32_t cap = 64 + 2 * (uint32_t)s->units;
+    uint32_t *st = (uint32_t *)malloc(sizeof(uint32_t) * cap);
+    uint32_t sp = 0;
+    if (s->units) st[sp++] = s->top;
+    while (sp) {
+        uint32_t i = st[--sp];
+        if (i >= s->node_count) {
+            free(st);
+            return -1;
+        }
+        const or_node *n = &s->nodes[i];
+        if (n->count == 0) {
+            if (sp + 2 > cap) {
+                free(st);
+                return -1;
+            }
+            st[sp++] = s->top + n->left_or_first;
+            st[sp++] = s->top + n->left_or_first + 1;
+            continue;
+        }
+        uint32_t rec = n->left_or_first;
+        if (rec >= s->instance_count) {
+            free(st);
+            return -1;
+        }
+        const or_instance *u = &s->instances[rec];
+        double o[3], d[3];
+        double t3[3] = {(double)of[0] - u->m[0][3], (double)of[1] - u->m[1][3], (double)of[2] - u->m[2][3]};
+        if ((fbits(u->n[1][3]) & OR_UNIT_SHIFTED) == 0) {
+            for (int k = 0; k < 3; k++) {
+                o[k] = (double)u->n[0][k] * t3[0] + (double)u->n[1][k] * t3[1] + (double)u->n[2][k] * t3[2];
+                d[k] = (double)u->n[0][k] * df[0] + (double)u->n[1][k] * df[1] + (double)u->n[2][k] * df[2];
+            }
+        } else {
+            for (int k = 0; k < 3; k++) o[k] = t3[k], d[k] = df[k];
+        }
+        /* the unit's triangles: its mesh tree's leaves */
+        uint32_t root = fbits(u->n[0][3]), first = u->info[0];
+        uint32_t ms[256]; /* a mesh tree is at most OR_MESH_DEPTH deep: at most one pending child a level */
+        uint64_t msp = 0;
+        ms[msp++] = root;
+        while (msp) {
+            uint32_t j = ms[--msp];
+            if (j >= s->node_count) {
+                free(st);
+                return -1;
+            }
+            const or_node *m = &s->nodes[j];
+            if (m->count == 0) {
+                if (msp + 2 > 256) {
+                    free(st);
+                    return -1;
+                }
+                ms[msp++] = root + m->left_or_first;
+                ms[msp++] = root + m->left_or_first + 1;
+                continue;
+            }
+            for (uint32_t k = 0; k < m->count; k++) {
+                uint32_t ti = first + m->left_or_first + k;
+                if ((uint64_t)ti >= s->leaf_count) {
+                    free(st);
+                    return -1;
+                }
+                const or_leaf *l = &s->leaves[ti];
+                double v0[3] = {l->r[0][0], l->r[0][1], l->r[0][2]}, e1[3] = {l->r[1][0], l->r[1][1], l->r[1][2]},
+                       e2[3] = {l->r[2][0], l->r[2][1], l->r[2][2]};
+                double pv[3] = {d[1] * e2[2] - e2[1] * d[2], d[2] * e2[0] - e2[2] * d[0], d[0] * e2[1] - e2[0] * d[1]};
+                double det = e1[0] * pv[0] + e1[1] * pv[1] + e1[2] * pv[2];
+                if (det == 0.0) continue;
+                double inv = 1.0 / det;
+                double tv[3] = {o[0] - v0[0], o[1] - v0[1], o[2] - v0[2]};
+                double uu = (tv[0] * pv[0] + tv[1] * pv[1] + tv[2] * pv[2]) * inv;
+                if (!(uu >= 0.0 && uu <= 1.0)) continue;
+                double qv[3] = {tv[1] * e1[2] - e1[1] * tv[2], tv[2] * e1[0] - e1[2] * tv[0],
+                                tv[0] * e1[1] - e1[0] * tv[1]};
+                double vv = (d[0] * qv[0] + d[1] * qv[1] + d[2] * qv[2]) * inv;
+                if (vv < 0.0 || uu + vv > 1.0) continue;
+                double tt = (e2[0] * qv[0] + e2[1] * qv[1] + e2[2] * qv[2]) * inv;
+                if (!(tt > 0.0) || tt >= tmax) continue;
+                int better = h.tri == OR_NO_TRI || tt < h.t ||
+                             (tt == h.t && (rec < h.inst || (rec == h.inst && ti < h.tri)));
+                if (better) {
+                    h.t = tt, h.u = uu, h.v = vv, h.tri = ti, h.inst = rec;
+                }
+            }
+        }
+    }
+    free(st);
+    *hit = h;
+    return 0;
+}
+
+int64_t or_brute64_batch(const or_scene *s, const or_ray *rays, uint64_t n, int threads, or_hit64 *hits) {
+    int64_t failed = 0;
+#ifdef _OPENMP
+    if (threads <= 0) threads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads) reduction(+ : failed)
+#endif
+    for (int64_t i = 0; i < (int64_t)n; i++)
+        if (brute64(s, rays[i].o, rays[i].d, rays[i].tmax, &hits[i])) failed++;
     (void)threads;
     return failed;
 }

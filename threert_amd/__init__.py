@@ -31,6 +31,7 @@ assert NODE.itemsize == 32 and INSTANCE.itemsize == 160 and RAY.itemsize == 32 a
 TRT_OK, TRT_E_INVALID, TRT_E_OOM, TRT_E_DEVICE, TRT_E_GUARD = 0, 1, 2, 3, 4
 TRACE_CLOSEST, TRACE_SHADOW, TRACE_ANY = 0, 1, 2
 WALK_WIDE, WALK_BINARY = 0, 1
+WALK_WATERTIGHT = 2  # the 4-wide walk with the watertight triangle test (Woop, Benthin and Wald 2013)
 BUILD_DEVICE, BUILD_HOST, BUILD_AUTO = 0, 1, 2  # trt_device_set_build
 NO_HIT = 0xFFFFFFFF
 INF = np.float32(3.0e38)

@@ -821,8 +821,10 @@ static int ensure_spill(trt_device *d, uint64_t bytes) {
 static int launch(trt_scene *s, const void *d_rays, uint64_t n, void *d_hits, uint32_t flags, bool count = false,
                   bool noalloc = false) {
     trt_device *d = s->dev;
-    const bool wide = flags == TRT_WALK_WIDE;
-    if (flags != TRT_WALK_WIDE && flags != TRT_WALK_BINARY) return fail(TRT_E_INVALID, "unknown trace flags");
+    const bool wide = flags != TRT_WALK_BINARY;
+    const bool watertight = flags == TRT_WALK_WATERTIGHT;
+    if (flags != TRT_WALK_WIDE && flags != TRT_WALK_BINARY && flags != TRT_WALK_WATERTIGHT)
+        return fail(TRT_E_INVALID, "unknown trace flags (TRT_WALK_WIDE, TRT_WALK_BINARY or TRT_WALK_WATERTIGHT)");
     if (n >= (uint64_t{1} << 31)) return fail(TRT_E_INVALID, "a batch of 2^31 rays or more");
     if (n == 0) return ok();
     int grid = d->cus * (wide ? d->blocks_wide : d->blocks_binary);
@@ -871,7 +873,8 @@ static int launch(trt_scene *s, const void *d_rays, uint64_t n, void *d_hits, ui
     a.top_first = s->top_first;
     a.units = s->units;
     std::memcpy(a.top_box, s->top_box, sizeof(a.top_box));
-    if (trt::launch_trace(a, wide, count, grid, d->stream)) return fail(TRT_E_DEVICE, "trace kernel launch failed");
+    if (trt::launch_trace(a, wide, count, grid, d->stream, watertight))
+        return fail(TRT_E_DEVICE, "trace kernel launch failed");
     return ok();
 }
 

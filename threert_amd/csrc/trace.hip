@@ -55,7 +55,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
 constexpr uint32_t kRefill = 24;
 constexpr uint32_t kChunk = 64;  // at least a wave: one refill switches chunks at most once
 
-template <bool kCount>
+template <bool kCount, bool kWatertight>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_dyn(TraceArgs a) {
     __shared__ uint32_t lds[kLdsStack * kTraceBlock];
     const uint32_t lane = threadIdx.x & 63u;
@@ -127,7 +127,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_dyn(TraceArgs a) {
         }
         // The other phase: every waiting lane tests its run of triangles, or enters or leaves a unit.
         if (active && wide_waits(w)) {
-            wide_other<kCount>(a, w, st, flags, cnt);
+            wide_other<kCount, kWatertight>(a, w, st, flags, cnt);
             if (wide_over(w)) {
                 store_hit(a.hits, ray, w.h);
                 active = false;
@@ -170,18 +170,22 @@ int scratch_address(int ordinal, void **ptr) {
 
 int trace_occupancy(bool wide, int *blocks_per_cu) {
     int b = 0;
-    hipError_t e = wide ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_trace_dyn<false>, kTraceBlock, 0)
+    hipError_t e = wide ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_trace_dyn<false, false>, kTraceBlock, 0)
                         : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_trace<false, false>, kTraceBlock, 0);
     *blocks_per_cu = b;
     return e == hipSuccess ? 0 : -1;
 }
 
-int launch_trace(const TraceArgs &a, bool wide, bool count, int grid, hipStream_t s) {
+int launch_trace(const TraceArgs &a, bool wide, bool count, int grid, hipStream_t s, bool watertight) {
     hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, s, a.counter);
-    if (wide && count)
-        hipLaunchKernelGGL((k_trace_dyn<true>), dim3(grid), dim3(kTraceBlock), 0, s, a);
+    if (wide && watertight && count)
+        hipLaunchKernelGGL((k_trace_dyn<true, true>), dim3(grid), dim3(kTraceBlock), 0, s, a);
+    else if (wide && watertight)
+        hipLaunchKernelGGL((k_trace_dyn<false, true>), dim3(grid), dim3(kTraceBlock), 0, s, a);
+    else if (wide && count)
+        hipLaunchKernelGGL((k_trace_dyn<true, false>), dim3(grid), dim3(kTraceBlock), 0, s, a);
     else if (wide)
-        hipLaunchKernelGGL((k_trace_dyn<false>), dim3(grid), dim3(kTraceBlock), 0, s, a);
+        hipLaunchKernelGGL((k_trace_dyn<false, false>), dim3(grid), dim3(kTraceBlock), 0, s, a);
     else if (count)
         hipLaunchKernelGGL((k_trace<false, true>), dim3(grid), dim3(kTraceBlock), 0, s, a);
     else

@@ -50,7 +50,8 @@ constexpr int kMaxDevices = 64;
 int scratch_address(int ordinal, void **ptr);
 
 int trace_occupancy(bool wide, int *blocks_per_cu);
-int launch_trace(const TraceArgs &a, bool wide, bool count, int grid, hipStream_t s);
+// watertight (wide walk only): triangles by the watertight test instead of Moller-Trumbore
+int launch_trace(const TraceArgs &a, bool wide, bool count, int grid, hipStream_t s, bool watertight = false);
 // n16 16-byte words from src to dst, both 16-byte aligned; at most `grid` blocks of 256
 int launch_copy(void *dst, const void *src, uint64_t n16, int grid, hipStream_t s);
 // FP64 re-intersection of n hits (check64.hip); at most `grid` blocks of 256

@@ -81,6 +81,58 @@ __device__ __forceinline__ void intersect_tri(F3 o, F3 d, float4 l0, float4 l1, 
     t = tt, u = uu, v = vv;
 }
 
+// The watertight triangle test (Woop, Benthin and Wald, "Watertight Ray/Triangle Intersection", JCGT 2013): the
+// triangle's vertices translated to the ray's origin and sheared so the ray runs along +z, then the 2-D edge functions
+// U, V, W of the projected vertices (the same float operations for an edge however a triangle names it, so a ray on
+// an edge shared by two triangles meets one of them); an edge function that rounds to exactly 0 is evaluated again
+// in double. The vertices are v0, v0 + e1, v0 + e2 from the leaf. u = V / det, v = W / det weigh v1 and v2, as
+// Moller-Trumbore's u and v do.
+struct Shear {
+    uint32_t kx, ky, kz;
+    float sx, sy, sz;
+};
+__device__ __forceinline__ float comp(F3 v, uint32_t k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+__device__ __forceinline__ Shear shear_of(F3 d) {
+    const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    Shear s;
+    s.kz = ax >= ay ? (ax >= az ? 0u : 2u) : (ay >= az ? 1u : 2u);  // the largest component, the first of equals
+    s.kx = s.kz == 2u ? 0u : s.kz + 1u;
+    s.ky = s.kx == 2u ? 0u : s.kx + 1u;
+    if (comp(d, s.kz) < 0.0f) {
+        const uint32_t t = s.kx;
+        s.kx = s.ky, s.ky = t;
+    }
+    s.sx = comp(d, s.kx) / comp(d, s.kz);
+    s.sy = comp(d, s.ky) / comp(d, s.kz);
+    s.sz = 1.0f / comp(d, s.kz);
+    return s;
+}
+__device__ __forceinline__ void intersect_tri_wt(F3 o, const Shear &k, float4 l0, float4 l1, float4 l2, float tmax,
+                                                 float &t, float &u, float &v) {
+    t = kInf;
+    const F3 v0{l0.x, l0.y, l0.z};
+    const F3 v1{l0.x + l1.x, l0.y + l1.y, l0.z + l1.z}, v2{l0.x + l2.x, l0.y + l2.y, l0.z + l2.z};
+    const F3 A = sub3(v0, o), B = sub3(v1, o), C = sub3(v2, o);
+    const float Az0 = comp(A, k.kz), Bz0 = comp(B, k.kz), Cz0 = comp(C, k.kz);
+    const float ax = comp(A, k.kx) - k.sx * Az0, ay = comp(A, k.ky) - k.sy * Az0;
+    const float bx = comp(B, k.kx) - k.sx * Bz0, by = comp(B, k.ky) - k.sy * Bz0;
+    const float cx = comp(C, k.kx) - k.sx * Cz0, cy = comp(C, k.ky) - k.sy * Cz0;
+    float U = cx * by - cy * bx, V = ax * cy - ay * cx, W = bx * ay - by * ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        U = static_cast<float>(static_cast<double>(cx) * by - static_cast<double>(cy) * bx);
+        V = static_cast<float>(static_cast<double>(ax) * cy - static_cast<double>(ay) * cx);
+        W = static_cast<float>(static_cast<double>(bx) * ay - static_cast<double>(by) * ax);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return;
+    const float det = U + V + W;
+    if (det == 0.0f) return;
+    const float T = U * (k.sz * Az0) + V * (k.sz * Bz0) + W * (k.sz * Cz0);
+    const float rcp = 1.0f / det;
+    const float tt = T * rcp;
+    if (!(tt > 0.0f) || tt >= tmax) return;
+    t = tt, u = V * rcp, v = W * rcp;
+}
+
 // The first out-of-range read of a launch, for the host's error message: guard word 1 = which check (a kRange id),
 // words 2-3 = the index (low, high), word 4 = the size it was checked against. Word 0 holds the kFlag bits.
 // Words 5-7 may carry three values of the reporting thread's context (ctx non-null). Returns whether this thread's
@@ -444,7 +496,8 @@ __device__ __forceinline__ void wide_node(const TraceArgs &a, WideWalk &w, Stack
 
 // One step of the other phase (wide_waits holds): a run of triangles tested, a unit entered or left. Shadow and
 // any-hit walks end at the first hit they take.
-template <bool kCount>
+// kWatertight: triangles by intersect_tri_wt (TRT_WALK_WATERTIGHT) instead of the reference's Moller-Trumbore.
+template <bool kCount, bool kWatertight = false>
 __device__ __forceinline__ void wide_other(const TraceArgs &a, WideWalk &w, Stack &st, uint32_t &flags, Reads &cnt) {
     const uint32_t node = w.node;
     if (++w.guard > kGuardSteps) {
@@ -504,13 +557,18 @@ __device__ __forceinline__ void wide_other(const TraceArgs &a, WideWalk &w, Stac
         return;
     }
     bool taken = false;
+    Shear sh{};
+    if (kWatertight) sh = shear_of(w.rd);
     for (uint32_t k = 0; k < count; k++) {
         if (kCount) cnt.tris++;
         const float4 *l = a.wleaves + 3 * (f + k);
         const float4 lv0 = l[0], lv1 = l[1], lv2 = l[2];
         const uint32_t ti = __float_as_uint(lv2.w);  // the reference leaf index: what hits report and ties order
         float x, uu, vv;
-        intersect_tri(w.ro, w.rd, lv0, lv1, lv2, w.tmax, x, uu, vv);
+        if (kWatertight)
+            intersect_tri_wt(w.ro, sh, lv0, lv1, lv2, w.tmax, x, uu, vv);
+        else
+            intersect_tri(w.ro, w.rd, lv0, lv1, lv2, w.tmax, x, uu, vv);
         const bool take = x < w.tmax && accept(w.h, x, w.rec, ti);
         take_hit(w.h, take, x, ti, uu, vv, w.rec, __float_as_uint(lv0.w));
         taken = taken || take;
