@@ -295,6 +295,19 @@ typedef struct {
 } trt_render_job;
 int trt_render_batch(trt_device *dev, const trt_render_job *jobs, uint32_t n);
 
+/* How the device's later render launches (trt_render, trt_render_batch) run their paths: TRT_RENDER_MEGAKERNEL (a
+ * thread follows one pixel's path through every bounce, tracing as it goes), TRT_RENDER_WAVEFRONT (up to 4,194,304
+ * paths at a time, bounce by bounce: one walk launch traces every path's next ray and the previous bounce's shadow
+ * rays from a compacted queue, then a shade launch resolves the shadow rays, shades the hits and queues the next
+ * rays; about 600 bytes of HBM a path), or TRT_RENDER_AUTO (the default: the megakernel). Both give the same bits.
+ * A batch of jobs over more than one scene, a job with more than 3 directional lights, and trt_render_reads run as
+ * the megakernel whatever the setting. No reference counterpart: the reference's offline pass is one megakernel
+ * dispatch (pathtrace.wgsl:4106 `main`). */
+#define TRT_RENDER_AUTO 0u
+#define TRT_RENDER_MEGAKERNEL 1u
+#define TRT_RENDER_WAVEFRONT 2u
+int trt_device_set_render(trt_device *dev, uint32_t how);
+
 /* --- denoising (denoise.rs:10-74 denoise: edge-avoiding a-trous guided by albedo and normal) ---------------- */
 /* Device buffers (trt_mem_alloc): color, albedo, normal 3 floats a pixel, variance 1 (trt_accum_resolve's
  * outputs), out 3 floats a pixel. Queued on the device's stream. */

@@ -33,6 +33,7 @@ TRACE_CLOSEST, TRACE_SHADOW, TRACE_ANY = 0, 1, 2
 WALK_WIDE, WALK_BINARY = 0, 1
 WALK_WATERTIGHT = 2  # the 4-wide walk with the watertight triangle test (Woop, Benthin and Wald 2013)
 BUILD_DEVICE, BUILD_HOST, BUILD_AUTO = 0, 1, 2  # trt_device_set_build
+RENDER_AUTO, RENDER_MEGAKERNEL, RENDER_WAVEFRONT = 0, 1, 2  # trt_device_set_render
 NO_HIT = 0xFFFFFFFF
 INF = np.float32(3.0e38)
 
@@ -44,7 +45,7 @@ EXPORTS = [
     "trt_scene_stats_get", "trt_trace_rays", "trt_trace_rays_device", "trt_trace_reads",
     "trt_accum_create", "trt_accum_release", "trt_accum_clear", "trt_accum_read", "trt_accum_resolve", "trt_render",
     "trt_denoise_atrous", "trt_render_batch", "trt_mem_copy", "trt_trace_check_fp64", "trt_render_ray_counts",
-    "trt_accum_resolve_device", "trt_accum_add_fp64", "trt_device_set_build", "trt_scene_wide_read",
+    "trt_accum_resolve_device", "trt_accum_add_fp64", "trt_device_set_build", "trt_device_set_render", "trt_scene_wide_read",
     "trt_unet_create", "trt_unet_release", "trt_unet_param_count", "trt_unet_params_set", "trt_unet_params_get",
     "trt_unet_denoise", "trt_unet_gradients", "trt_unet_train_step", "trt_trace_rays_device_noalloc",
     "trt_build_light_tree", "trt_light_tree_free", "trt_render_reads", "trt_unet_set_tall_tiles",
@@ -120,7 +121,7 @@ def lib():
             "trt_trace_reads": [P, P, U64, P, U32, P],
             "trt_trace_check_fp64": [P, P, P, U64, P],
             "trt_render_ray_counts": [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
-            "trt_device_set_build": [P, U32], "trt_scene_wide_read": [P, U32, P, U64],
+            "trt_device_set_build": [P, U32], "trt_device_set_render": [P, U32], "trt_scene_wide_read": [P, U32, P, U64],
             "trt_build_light_tree": [P, P, P, U64, P],
         }
         for name, args in sig.items():
@@ -271,6 +272,11 @@ class Device:
 
     def alloc(self, nbytes: int) -> "DeviceBuffer":
         return DeviceBuffer(self, nbytes)
+
+    def set_render(self, how: int):
+        """How later render launches run their paths: RENDER_AUTO (default: the megakernel), RENDER_MEGAKERNEL or
+        RENDER_WAVEFRONT (bounce by bounce over queued rays). The same bits either way."""
+        _check(lib().trt_device_set_render(self.h, how))
 
     def set_build(self, where: int):
         """Where later uploads build the library's own mesh trees: BUILD_AUTO (default: the device from 65,536

@@ -68,8 +68,11 @@ struct trt_device {
     uint64_t staging_bytes = 0;
     uint32_t *scratch = nullptr;  // this GPU's fixed scratch (trace.hip), trt_trace_rays_device_noalloc's only memory
     int cus = 0;
-    int blocks_wide = 1, blocks_binary = 1, blocks_render = 1;
+    int blocks_wide = 1, blocks_binary = 1, blocks_render = 1, blocks_shade = 1;
     std::atomic<uint32_t> build{TRT_BUILD_AUTO};  // where uploads build the library's own mesh trees
+    std::atomic<uint32_t> render{TRT_RENDER_AUTO};  // how render launches run (trt_device_set_render)
+    void *wf = nullptr;    // the wavefront's path state, rays, hits and queues for wf_stride paths
+    uint64_t wf_stride = 0;
     hipDeviceProp_t props{};
     std::mutex mu;
     std::atomic<int> refs{1};
@@ -136,6 +139,7 @@ int trt_device_create(int ordinal, trt_device **out) {
     if (trt::trace_occupancy(true, &d->blocks_wide) || d->blocks_wide < 1) d->blocks_wide = 1;
     if (trt::trace_occupancy(false, &d->blocks_binary) || d->blocks_binary < 1) d->blocks_binary = 1;
     if (trt::render_occupancy(&d->blocks_render) || d->blocks_render < 1) d->blocks_render = 1;
+    if (trt::wf_shade_occupancy(&d->blocks_shade) || d->blocks_shade < 1) d->blocks_shade = 1;
     *out = d;
     return ok();
 }
@@ -151,6 +155,7 @@ static void device_unref(trt_device *d) {
     hipFree(d->denoise_scratch);
     hipFree(d->spill);
     hipFree(d->staging);
+    hipFree(d->wf);
     for (int k = 0; k < kJobRing; k++) {
         hipHostFree(d->jobs.host[k]);
         hipFree(d->jobs.dev[k]);
@@ -774,6 +779,14 @@ int trt_scene_upload(trt_device *d, const trt_scene_desc *desc, trt_scene **out)
     return ok();
 }
 
+int trt_device_set_render(trt_device *d, uint32_t how) {
+    if (!d) return fail(TRT_E_INVALID, "trt_device_set_render: null device");
+    if (how != TRT_RENDER_AUTO && how != TRT_RENDER_MEGAKERNEL && how != TRT_RENDER_WAVEFRONT)
+        return fail(TRT_E_INVALID, "trt_device_set_render: how must be TRT_RENDER_AUTO, _MEGAKERNEL or _WAVEFRONT");
+    d->render = how;
+    return ok();
+}
+
 int trt_device_set_build(trt_device *d, uint32_t where) {
     if (!d) return fail(TRT_E_INVALID, "trt_device_set_build: null device");
     if (where != TRT_BUILD_DEVICE && where != TRT_BUILD_HOST && where != TRT_BUILD_AUTO)
@@ -1082,6 +1095,140 @@ static int job_args(const trt_render_job &j, trt::RenderArgs &a, std::string *wh
     return TRT_OK;
 }
 
+// ---- the wavefront path tracer's plan ------------------------------------------------------------------------------
+// A batch's paths (pixel x sample) go through the wavefront in chunks of at most kWfMaxPaths. A job whose image fits
+// a chunk goes in whole rounds (every pixel, several samples); a larger one in pixel ranges of one sample. A chunk
+// never holds one pixel twice, and a pixel's samples come in sample order, so that the accumulation adds them as the
+// megakernel does.
+constexpr uint64_t kWfMaxPaths = uint64_t{1} << 22;
+constexpr uint64_t kWfBytesPerPath = 24 * 16 + (6 * 6 + 2 * 6 + 2) * 4;  // 24 float4 arrays, hits, lists, actives
+struct WfChunk {
+    uint32_t sub_first, nsubs, npaths, pixels;
+};
+
+static void wf_plan(const std::vector<trt::RenderJob> &table, uint64_t stride, std::vector<trt::WfSub> &subs,
+                    std::vector<WfChunk> &chunks) {
+    WfChunk cur{0, 0, 0, 0};
+    auto close = [&]() {
+        if (cur.npaths) chunks.push_back(cur);
+        cur = WfChunk{static_cast<uint32_t>(subs.size()), 0, 0, 0};
+    };
+    auto add = [&](uint32_t job, uint32_t pixel_first, uint32_t count, uint32_t sample_first, uint32_t rounds,
+                   uint32_t width) {
+        subs.push_back(trt::WfSub{job, pixel_first, count, sample_first, rounds, cur.npaths, cur.pixels, width});
+        cur.nsubs++;
+        cur.npaths += count * rounds;
+        cur.pixels += count;
+    };
+    for (uint32_t j = 0; j < table.size(); j++) {
+        const trt::RenderArgs &a = table[j].args;
+        const uint32_t w = a.params[16], spp = a.spp;
+        const uint64_t npix = uint64_t(a.params[16]) * a.params[17];
+        uint32_t r = 0;
+        uint64_t pix = 0;
+        while (r < spp) {
+            const uint64_t room = stride - cur.npaths;
+            if (npix <= stride) {
+                if (room < npix) {
+                    close();
+                    continue;
+                }
+                const uint32_t rounds = static_cast<uint32_t>(std::min<uint64_t>(spp - r, room / npix));
+                add(j, 0, static_cast<uint32_t>(npix), a.first_sample + r, rounds, w);
+                r += rounds;
+            } else {
+                if (room == 0) {
+                    close();
+                    continue;
+                }
+                const uint32_t cnt = static_cast<uint32_t>(std::min<uint64_t>(npix - pix, room));
+                add(j, static_cast<uint32_t>(pix), cnt, a.first_sample + r, 1, w);
+                pix += cnt;
+                if (pix == npix) pix = 0, r++;
+            }
+        }
+    }
+    close();
+}
+
+static trt::WfArgs wf_layout(void *base, uint64_t stride) {
+    trt::WfArgs w{};
+    auto *f = static_cast<float4 *>(base);
+    w.stride = static_cast<uint32_t>(stride);
+    w.thr = f;
+    w.rad = f + stride;
+    w.alb = f + 2 * stride;
+    w.nrm = f + 3 * stride;
+    w.misx = f + 4 * stride;
+    w.misn = f + 5 * stride;
+    w.thr_dl = f + 6 * stride;
+    w.contrib = f + 7 * stride;                // kWfShadow slots
+    w.rays = f + (7 + trt::kWfShadow) * stride;  // kWfSlots slots of 2 float4
+    auto *u = reinterpret_cast<uint32_t *>(f + 24 * stride);
+    w.hits = u;
+    u += 6 * trt::kWfSlots * stride;
+    w.list[0] = u;
+    u += trt::kWfSlots * stride;
+    w.list[1] = u;
+    u += trt::kWfSlots * stride;
+    w.active[0] = u;
+    u += stride;
+    w.active[1] = u;
+    u += stride;
+    w.counts = u;
+    return w;
+}
+static_assert(7 + trt::kWfShadow + 2 * trt::kWfSlots == 24, "the float4 arrays of wf_layout");
+
+static int ensure_wf(trt_device *d, uint64_t stride) {
+    if (stride <= d->wf_stride) return TRT_OK;
+    TRT_HIP(hipStreamSynchronize(d->stream));
+    TRT_HIP(hipFree(d->wf));
+    d->wf = nullptr;
+    d->wf_stride = 0;
+    TRT_HIP(hipMalloc(&d->wf, stride * kWfBytesPerPath + 256));
+    d->wf_stride = stride;
+    return TRT_OK;
+}
+
+// The wavefront's launches for a batch whose jobs share one scene: per chunk, the camera rays, then per bounce the
+// walk of the queued rays and the shade kernel, then the accumulation. `last`: the jobs' largest last bounce.
+static int wf_launch(trt_device *d, const trt::RenderJob *d_jobs, const trt::WfSub *d_subs,
+                     const std::vector<WfChunk> &chunks, uint64_t stride, const trt::TraceArgs &geo, uint32_t last,
+                     bool coat, bool tree) {
+    trt::WfArgs w = wf_layout(d->wf, stride);
+    w.jobs = d_jobs;
+    w.guard = d->guard;
+    w.rays_traced = d->reads + kRenderRays;
+    trt::TraceArgs t = geo;
+    t.rays = w.rays;
+    t.hits = w.hits;
+    t.n = trt::kWfSlots * stride;
+    const int trace_grid = d->cus * d->blocks_wide;
+    for (const WfChunk &c : chunks) {
+        w.subs = d_subs + c.sub_first;
+        w.nsubs = c.nsubs;
+        w.npaths = c.npaths;
+        const int blocks = static_cast<int>((c.npaths + trt::kRenderBlock - 1) / trt::kRenderBlock);
+        const int small = std::min(blocks, d->cus * 8);
+        const int shade_grid = std::min(blocks, d->cus * d->blocks_shade);
+        if (trt::launch_wf_begin(w, small, d->stream)) return fail(TRT_E_DEVICE, "wavefront launch failed");
+        for (uint32_t k = 0; k <= last; k++) {
+            t.ray_index = w.list[k & 1u];
+            t.n_dev = w.counts + k % 3u;
+            if (trt::launch_trace(t, true, false, trace_grid, d->stream))
+                return fail(TRT_E_DEVICE, "wavefront trace launch failed");
+            if (trt::launch_wf_shade(w, k, coat, tree, shade_grid, d->stream))
+                return fail(TRT_E_DEVICE, "wavefront shade launch failed");
+        }
+        const int acc_grid = std::min(static_cast<int>((c.pixels + trt::kRenderBlock - 1) / trt::kRenderBlock),
+                                      d->cus * 8);
+        if (trt::launch_wf_accum(w, c.pixels, acc_grid, d->stream))
+            return fail(TRT_E_DEVICE, "wavefront accumulation launch failed");
+    }
+    return ok();
+}
+
 // reads: NULL, or the counters a counting launch adds to (the kernel with counted walks).
 static int render_batch(trt_device *d, const trt_render_job *jobs, uint32_t n, unsigned long long *reads) {
     if (!d || (n && !jobs)) return fail(TRT_E_INVALID, "trt_render_batch: null argument");
@@ -1111,13 +1258,37 @@ static int render_batch(trt_device *d, const trt_render_job *jobs, uint32_t n, u
     }
     if (tiles >= (uint64_t{1} << 31)) return fail(TRT_E_INVALID, "trt_render_batch: too many tiles in one launch");
     if (table.empty()) return ok();
+    // The wavefront takes batches of one scene (its walk launches take one scene's trees) whose jobs have at most
+    // kWfMaxSuns directional lights; counting launches stay on the megakernel.
+    bool wf = reads == nullptr && d->render.load() == TRT_RENDER_WAVEFRONT;
+    uint64_t paths = 0;
+    uint32_t last = 0;
+    const trt_scene *first_scene = nullptr;
+    for (uint32_t k = 0; k < n && wf; k++) {
+        if (jobs[k].spp == 0) continue;
+        if (!first_scene) first_scene = jobs[k].scene;
+        wf = jobs[k].scene == first_scene;
+    }
+    for (const auto &j : table) {
+        wf = wf && j.args.params[36] <= trt::kWfMaxSuns;  // P.counts2.x: the directional lights
+        paths += uint64_t(j.args.params[16]) * j.args.params[17] * j.args.spp;
+        last = std::max(last, j.args.params[22]);  // P.counts.z: the last bounce
+    }
+    std::vector<trt::WfSub> subs;
+    std::vector<WfChunk> chunks;
+    const uint64_t stride = (std::min(paths, kWfMaxPaths) + 255) / 256 * 256;
+    if (wf) wf_plan(table, stride, subs, chunks);
     std::lock_guard<std::mutex> lock(d->mu);
     TRT_HIP(hipSetDevice(d->ordinal));
-    const int grid = d->cus * d->blocks_render;
-    const uint32_t threads = static_cast<uint32_t>(grid) * trt::kRenderBlock;
+    const int grid = wf ? d->cus * d->blocks_wide : d->cus * d->blocks_render;
+    const uint32_t threads = static_cast<uint32_t>(grid) * (wf ? trt::kTraceBlock : trt::kRenderBlock);
     const uint64_t entries = deepest > trt::kLdsStack ? deepest - trt::kLdsStack : 1;
     int rc = ensure_spill(d, entries * threads * sizeof(uint32_t));
     if (rc) return rc;
+    if (wf) {
+        rc = ensure_wf(d, stride);
+        if (rc) return rc;
+    }
     for (auto &j : table) {
         j.args.geo.counter = d->counter;
         j.args.geo.guard_flag = d->guard;
@@ -1132,7 +1303,8 @@ static int render_batch(trt_device *d, const trt_render_job *jobs, uint32_t n, u
     const int k = ring.next;
     ring.next = (ring.next + 1) % kJobRing;
     if (ring.ev[k]) TRT_HIP(hipEventSynchronize(ring.ev[k]));
-    const uint64_t bytes = table.size() * sizeof(trt::RenderJob);
+    const uint64_t job_bytes = (table.size() * sizeof(trt::RenderJob) + 255) / 256 * 256;
+    const uint64_t bytes = job_bytes + subs.size() * sizeof(trt::WfSub);
     if (bytes > ring.cap[k]) {
         TRT_HIP(hipHostFree(ring.host[k]));
         TRT_HIP(hipFree(ring.dev[k]));
@@ -1143,9 +1315,16 @@ static int render_batch(trt_device *d, const trt_render_job *jobs, uint32_t n, u
         ring.cap[k] = cap;
     }
     if (!ring.ev[k]) TRT_HIP(hipEventCreateWithFlags(&ring.ev[k], hipEventDisableTiming));
-    std::memcpy(ring.host[k], table.data(), bytes);
+    std::memcpy(ring.host[k], table.data(), table.size() * sizeof(trt::RenderJob));
+    if (!subs.empty())
+        std::memcpy(static_cast<char *>(ring.host[k]) + job_bytes, subs.data(), subs.size() * sizeof(trt::WfSub));
     TRT_HIP(hipMemcpyAsync(ring.dev[k], ring.host[k], bytes, hipMemcpyHostToDevice, d->stream));
-    if (trt::launch_render(static_cast<const trt::RenderJob *>(ring.dev[k]), static_cast<uint32_t>(table.size()),
+    if (wf) {
+        const auto *d_jobs = static_cast<const trt::RenderJob *>(ring.dev[k]);
+        const auto *d_subs = reinterpret_cast<const trt::WfSub *>(static_cast<const char *>(ring.dev[k]) + job_bytes);
+        rc = wf_launch(d, d_jobs, d_subs, chunks, stride, table[0].args.geo, last, coat, tree);
+        if (rc) return rc;
+    } else if (trt::launch_render(static_cast<const trt::RenderJob *>(ring.dev[k]), static_cast<uint32_t>(table.size()),
                            static_cast<uint32_t>(tiles), d->counter, d->spill, threads,
                            static_cast<uint32_t>(entries), d->guard, d->reads + kRenderRays, coat, tree, reads, grid,
                            d->stream))

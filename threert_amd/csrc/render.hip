@@ -785,8 +785,32 @@ __device__ ES sample_emitter(Ctx &c, V x, V n, uint32_t rd) {
     return r;
 }
 
-template <bool kTree, bool kCount>
-__device__ V direct_light(Ctx &c, const Surface &s, const Lobes &lb, V v) {
+// Where the wavefront's direct light goes: contribution j as if visible, and (when it waits on one) the shadow ray
+// that decides it in ray slot j + 1. The shade kernel of the next bounce adds the visible ones, in j's order.
+struct WfShadows {
+    const WfArgs *w;
+    uint32_t p;        // the path
+    uint32_t n = 0;    // contributions so far
+    uint32_t present = 0, traced = 0;  // bit j: contribution j is there; it counts only if shadow ray j + 1 misses
+    __device__ __forceinline__ void put(V col, bool trace, V o, V d, float tmax, uint32_t mode) {
+        if (n >= kWfShadow) return;  // the host sends jobs with more than kWfMaxSuns suns to the megakernel
+        const uint64_t st = w->stride;
+        w->contrib[n * st + p] = make_float4(col.x, col.y, col.z, 0.0f);
+        if (trace) {
+            float4 *r = w->rays + 2 * ((n + 1) * st + p);
+            r[0] = make_float4(o.x, o.y, o.z, tmax);
+            r[1] = make_float4(d.x, d.y, d.z, __uint_as_float(mode));
+            traced |= 1u << n;
+        }
+        present |= 1u << n;
+        n++;
+    }
+};
+
+// pathtrace.wgsl direct_light: the sum of what the suns, one emitter sample and one environment sample send toward v.
+// kDefer (the wavefront): each contribution goes to `q` with its shadow ray, untraced, and the sum returned is 0.
+template <bool kTree, bool kCount, bool kDefer = false>
+__device__ V direct_light(Ctx &c, const Surface &s, const Lobes &lb, V v, WfShadows *q = nullptr) {
     V sum = v3(0.0f, 0.0f, 0.0f);
     const V o = offset_origin(s.p, s.ng);
     for (uint32_t i = 0; i < c.pu(COUNTS2, 0); i++) {
@@ -806,7 +830,10 @@ __device__ V direct_light(Ctx &c, const Surface &s, const Lobes &lb, V v) {
         const V col = xyz(li[2]);
         if (all_nonpos(col)) continue;
         const V f = eval_bsdf_k(s, lb, v, l, li[4].y);
-        if (any_pos(f)) {
+        if (kDefer && any_pos(f)) {
+            const float pdf_l = 1.0f / (kTwoPi * (1.0f - cos_max));
+            q->put(vs(vmul(col, f), light_mis(pdf_l, s, lb, v, l)), li[3].w >= 0.5f, o, l, kInf, kTraceShadow);
+        } else if (any_pos(f)) {
             const bool vis = li[3].w >= 0.5f ? !occluded<kCount>(c, o, l, kInf, kTraceShadow) : true;
             if (vis) {
                 const float pdf_l = 1.0f / (kTwoPi * (1.0f - cos_max));
@@ -834,7 +861,10 @@ __device__ V direct_light(Ctx &c, const Surface &s, const Lobes &lb, V v) {
             es = sample_emitter<kTree>(c, o, ne, c.g.dim(D_EMIT));
         if (es.pdf_w > 0.0f) {
             const V f = eval_bsdf(s, lb, v, es.l);
-            if (any_pos(f)) {
+            if (kDefer && any_pos(f)) {
+                q->put(vs(vs(vmul(es.le, f), light_mis(es.pdf_w, s, lb, v, es.l)), es.w), true, o, es.l,
+                       es.dist * 0.999f, kTraceAny);
+            } else if (any_pos(f)) {
                 if (!occluded<kCount>(c, o, es.l, es.dist * 0.999f, kTraceAny)) {
                     const float wgt = light_mis(es.pdf_w, s, lb, v, es.l);
                     sum = vadd(sum, vs(vs(vmul(es.le, f), wgt), es.w));
@@ -845,7 +875,11 @@ __device__ V direct_light(Ctx &c, const Surface &s, const Lobes &lb, V v) {
     if (env_sampled(c)) {
         const V l = sample_env(c, c.g.dim(D_ENV));
         const V f = eval_bsdf(s, lb, v, l);
-        if (any_pos(f)) {
+        if (kDefer && any_pos(f)) {
+            const float pdf_e = env_pdf();
+            if (pdf_e > 0.0f)
+                q->put(vds(vs(vmul(sky(c), f), light_mis(pdf_e, s, lb, v, l)), pdf_e), true, o, l, kInf, kTraceShadow);
+        } else if (any_pos(f)) {
             const float pdf_e = env_pdf();
             if (pdf_e > 0.0f && !occluded<kCount>(c, o, l, kInf, kTraceShadow))
                 sum = vadd(sum, vds(vs(vmul(sky(c), f), light_mis(pdf_e, s, lb, v, l)), pdf_e));
@@ -878,9 +912,8 @@ __device__ __forceinline__ V clamp_lum(V col, float clamp_v) {
     return col;
 }
 
-// One path of sample `index` at pixel (px, py): its radiance, the camera hit's albedo and normal.
-template <bool kCoat, bool kTree, bool kCount>
-__device__ void path(Ctx &c, uint32_t px, uint32_t py, uint32_t index, V &radiance, V &albedo, V &normal) {
+// camera_ray (pathtrace.wgsl:4002-4024) for sample `index` at pixel (px, py), with the pixel's sampler set up.
+__device__ __forceinline__ void camera_ray(Ctx &c, uint32_t px, uint32_t py, uint32_t index, V &o_out, V &d_out) {
     const uint32_t w = c.pu(SIZE, 0), h = c.pu(SIZE, 1);
     c.g.seed = hash_hp(px ^ hash_hp(py ^ hash_hp(c.pu(SIZE, 3))));
     c.g.index = index;
@@ -904,7 +937,14 @@ __device__ void path(Ctx &c, uint32_t px, uint32_t py, uint32_t index, V &radian
         o = vadd(o, vadd(vs(right, r * cp), vs(up, r * sp)));
         d = normalize(vsub(focus, o));
     }
+    o_out = o, d_out = d;
+}
 
+// One path of sample `index` at pixel (px, py): its radiance, the camera hit's albedo and normal.
+template <bool kCoat, bool kTree, bool kCount>
+__device__ void path(Ctx &c, uint32_t px, uint32_t py, uint32_t index, V &radiance, V &albedo, V &normal) {
+    V o, d;
+    camera_ray(c, px, py, index, o, d);
     radiance = v3(0.0f, 0.0f, 0.0f);
     albedo = v3(0.0f, 0.0f, 0.0f);
     normal = v3(0.0f, 0.0f, 0.0f);
@@ -1043,6 +1083,315 @@ __global__ __launch_bounds__(kRenderBlock) __attribute__((amdgpu_waves_per_eu(3)
     }
 }
 
+// ---- the wavefront path tracer ---------------------------------------------------------------------------------
+// The path loop above, cut at its traces: k_wf_begin queues every path's camera ray, the wide walk (k_trace_dyn)
+// traces the queued rays of a bounce, and k_wf_shade resolves the previous bounce's shadow rays, shades the hit and
+// queues the next rays. A path's state between kernels: throughput and last pdf (thr), radiance and these flag bits
+// (rad.w), the camera hit's albedo and normal, direct light's last vertex (emit_mis), and the pending direct light.
+enum : uint32_t {
+    kWfDepth = 0xffu,        // the bounce of the ray in slot 0
+    kWfHasRay = 1u << 8,     // slot 0 holds a traced closest ray to shade
+    kWfSpecular = 1u << 9,   // the path's specular_bounce
+    kWfStraight = 1u << 10,  // the path's straight
+    kWfMirror = 1u << 11,    // the path's mirror_path
+    kWfDl = 1u << 12,        // a bounce's direct light waits: its visible contributions x thr_dl go to the radiance
+    kWfClampDl = 1u << 13,   // ... clamped (clamp_lum)
+    kWfPresentShift = 14,    // 5 bits: contribution j is there
+    kWfTracedShift = 19,     // 5 bits: contribution j counts only if shadow ray j + 1 missed
+};
+static_assert(kWfShadow == 5, "the flag words hold 5 contributions");
+
+struct WfPath {
+    uint32_t job, px, py, index;
+};
+// path p of the chunk: its run (a binary search over the runs' first paths), pixel and sample
+__device__ __forceinline__ WfPath wf_path(const WfArgs &a, uint32_t p) {
+    uint32_t lo = 0, hi = a.nsubs;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (a.subs[mid].path_first <= p)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const WfSub &s = a.subs[lo];
+    const uint32_t off = p - s.path_first;
+    const uint32_t pixel = s.pixel_first + off % s.count;
+    return {s.job, pixel % s.width, pixel / s.width, s.sample_first + off / s.count};
+}
+__device__ __forceinline__ Ctx wf_ctx(const RenderArgs *ra, uint32_t *flags, uint32_t *traced, Reads *cnt) {
+    return Ctx{ra, nullptr, flags, traced, Rng{0, 0, 0}, false, false, v3(0.0f, 0.0f, 0.0f), v3(0.0f, 0.0f, 0.0f),
+               cnt};
+}
+// The wave's lanes take their own jobs one job at a time, so that a job's arguments are the same for the whole wave
+// (scalar loads) however the wave's paths are spread over jobs. `body(ra)` runs once in every lane with `valid`.
+template <typename F>
+__device__ __forceinline__ void wf_by_job(const WfArgs &a, bool valid, uint32_t job, F &&body) {
+    bool done = !valid;
+    for (uint64_t todo = __ballot(!done); todo != 0; todo = __ballot(!done)) {
+        const uint32_t j = __builtin_amdgcn_readfirstlane(__shfl(job, __ffsll((unsigned long long)todo) - 1, 64));
+        if (!done && job == j) {
+            body(&a.jobs[j].args);
+            done = true;
+        }
+    }
+}
+// The wave's queued rays (bit s of `slots`: ray slot s of path p) appended to `list`, one atomic a wave, and its
+// paths still running to `active`.
+// out: the counts' ring entry of the next bounce; parity: its lists
+__device__ __forceinline__ void wf_append(const WfArgs &a, uint32_t out, uint32_t parity, uint32_t p, uint32_t slots,
+                                          bool alive, uint32_t lane, unsigned long long &closest,
+                                          unsigned long long &shadow) {
+    const uint64_t below = (uint64_t{1} << lane) - 1;
+    uint32_t total = 0;
+    for (uint32_t s = 0; s < kWfSlots; s++) {
+        const uint32_t m = __popcll(__ballot((slots >> s) & 1u));
+        total += m;
+        if (s == 0) closest += m;
+        else shadow += m;
+    }
+    uint32_t first = 0;
+    if (lane == 0 && total) first = atomicAdd(a.counts + out, total);
+    first = __builtin_amdgcn_readfirstlane(__shfl(first, 0, 64));
+    const uint64_t st = a.stride;
+    for (uint32_t s = 0; s < kWfSlots; s++) {
+        const uint64_t b = __ballot((slots >> s) & 1u);
+        if ((slots >> s) & 1u) a.list[parity][first + __popcll(b & below)] = uint32_t(s * st + p);
+        first += __popcll(b);
+    }
+    const uint64_t live = __ballot(alive);
+    uint32_t afirst = 0;
+    if (lane == 0 && live) afirst = atomicAdd(a.counts + 3 + out, uint32_t(__popcll(live)));
+    afirst = __builtin_amdgcn_readfirstlane(__shfl(afirst, 0, 64));
+    if (alive) a.active[parity][afirst + __popcll(live & below)] = p;
+}
+__device__ __forceinline__ void wf_flush(const WfArgs &a, uint32_t flags, uint32_t lane, unsigned long long closest,
+                                         unsigned long long shadow) {
+    if (flags) atomicOr(a.guard, flags);
+    if (lane == 0 && (closest | shadow)) {
+        atomicAdd(a.rays_traced, closest);
+        atomicAdd(a.rays_traced + 1, shadow);
+    }
+}
+
+// Every path's camera ray in slot 0 and its state reset; list 0 and active 0 hold every path.
+__global__ __launch_bounds__(kRenderBlock) void k_wf_begin(WfArgs a) {
+    const uint32_t lane = threadIdx.x & 63u;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.counts[0] = a.counts[3] = a.npaths;
+        a.counts[1] = a.counts[4] = 0;
+    }
+    uint32_t flags = 0, traced[2] = {0, 0};
+    Reads cnt;
+    unsigned long long closest = 0;
+    const uint32_t waves = gridDim.x * (kRenderBlock / 64);
+    for (uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) / 64 * 64; base < a.npaths; base += waves * 64) {
+        const uint32_t p = base + lane;
+        const bool valid = p < a.npaths;
+        closest += __popcll(__ballot(valid));
+        const WfPath id = valid ? wf_path(a, p) : WfPath{0, 0, 0, 0};
+        wf_by_job(a, valid, id.job, [&](const RenderArgs *ra) {
+            Ctx c = wf_ctx(ra, &flags, traced, &cnt);
+            V o, d;
+            camera_ray(c, id.px, id.py, id.index, o, d);
+            a.rays[2 * uint64_t(p)] = make_float4(o.x, o.y, o.z, kInf);
+            a.rays[2 * uint64_t(p) + 1] = make_float4(d.x, d.y, d.z, __uint_as_float(kTraceClosest));
+            a.thr[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+            a.rad[p] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(kWfHasRay | kWfSpecular | kWfStraight));
+            a.alb[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            a.nrm[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            a.list[0][p] = p;
+            a.active[0][p] = p;
+        });
+    }
+    wf_flush(a, flags, lane, closest, 0);
+}
+
+// One bounce of path p (the loop body of `path`), from what the walk left in its ray slots: the pending direct light
+// resolved first, then the hit in slot 0 shaded. Returns the ray slots it queued (bit s: slot s); `alive` says
+// whether the path waits on anything.
+template <bool kCoat, bool kTree>
+__device__ uint32_t wf_bounce(Ctx &c, const WfArgs &a, uint32_t p, const WfPath &id, bool &alive) {
+    const uint64_t st = a.stride;
+    const float4 rd = a.rad[p];
+    uint32_t f = __float_as_uint(rd.w);
+    V radiance = xyz(rd);
+    const float clamp_v = c.pf(MISC, 0);
+    if (f & kWfDl) {
+        V sum = v3(0.0f, 0.0f, 0.0f);
+        for (uint32_t j = 0; j < kWfShadow; j++) {
+            if (!((f >> (kWfPresentShift + j)) & 1u)) continue;
+            if ((f >> (kWfTracedShift + j)) & 1u) {
+                const uint32_t tri = a.hits[6 * ((j + 1) * st + p) + 1];
+                if (tri != kNone) continue;
+            }
+            sum = vadd(sum, xyz(a.contrib[j * st + p]));
+        }
+        V dl = vmul(xyz(a.thr_dl[p]), sum);
+        if (f & kWfClampDl) dl = clamp_lum(dl, clamp_v);
+        radiance = vadd(radiance, dl);
+        f &= ~(kWfDl | kWfClampDl | (0x1fu << kWfPresentShift) | (0x1fu << kWfTracedShift));
+    }
+    uint32_t slots = 0;
+    if (f & kWfHasRay) {
+        f &= ~kWfHasRay;
+        const uint32_t depth = f & kWfDepth;
+        c.g.seed = hash_hp(id.px ^ hash_hp(id.py ^ hash_hp(c.pu(SIZE, 3))));
+        c.g.index = id.index;
+        c.g.depth = depth;
+        c.mirror_path = (f & kWfMirror) != 0;
+        const bool specular_bounce = (f & kWfSpecular) != 0, straight = (f & kWfStraight) != 0;
+        const float4 t4 = a.thr[p];
+        V thr = xyz(t4);
+        const float last_pdf = t4.w;
+        const float4 r0 = a.rays[2 * uint64_t(p)], r1 = a.rays[2 * uint64_t(p) + 1];
+        const V o = xyz(r0), d = xyz(r1);
+        const uint32_t *hw = a.hits + 6 * uint64_t(p);
+        HitRec hit;
+        hit.t = __uint_as_float(hw[0]);
+        hit.tri = hw[1];
+        hit.u = __uint_as_float(hw[2]);
+        hit.v = __uint_as_float(hw[3]);
+        hit.inst = hw[4];
+        hit.part = hw[5];
+        const uint32_t last = c.pu(COUNTS, 2);
+        if (hit.tri == kNone) {
+            V col = vmul(thr, sky_mis(c, d, depth > 0u ? last_pdf : 0.0f));
+            if (depth > 1u && !straight) col = clamp_lum(col, clamp_v);
+            radiance = vadd(radiance, col);
+        } else {
+            const Surface s = surface<kCoat>(c, o, d, hit);
+            if (depth == 0u) {
+                V albedo;
+                if (s.card) {
+                    const float mx = fmaxf(fmaxf(s.emissive.x, fmaxf(s.emissive.y, s.emissive.z)), 1e-6f);
+                    albedo = vds(s.emissive, mx);
+                } else {
+                    albedo = s.base;
+                }
+                a.alb[p] = make_float4(albedo.x, albedo.y, albedo.z, 0.0f);
+                a.nrm[p] = make_float4(s.n.x, s.n.y, s.n.z, 0.0f);
+            }
+            if (any_pos(s.emissive) && (s.front || !s.card)) {
+                float wgt = 1.0f;
+                if (depth > 0u && !specular_bounce) {
+                    c.mis_x = xyz(a.misx[p]);
+                    c.mis_n = xyz(a.misn[p]);
+                    wgt = emit_mis<kTree>(c, hit, d, last_pdf);
+                }
+                V col = vs(vmul(thr, s.emissive), wgt);
+                if (depth > 1u && !straight) col = clamp_lum(col, clamp_v);
+                radiance = vadd(radiance, col);
+            }
+            if (!s.card && depth != last) {
+                const V v = vneg(d);
+                const float nov = fmaxf(dot(s.n, v), 1e-4f);
+                const Lobes lb = lobes(c, s, nov);
+                WfShadows q{&a, p};
+                direct_light<kTree, false, true>(c, s, lb, v, &q);
+                a.thr_dl[p] = make_float4(thr.x, thr.y, thr.z, 0.0f);
+                a.misx[p] = make_float4(c.mis_x.x, c.mis_x.y, c.mis_x.z, 0.0f);
+                a.misn[p] = make_float4(c.mis_n.x, c.mis_n.y, c.mis_n.z, 0.0f);
+                f |= kWfDl | (depth > 0u && !straight ? kWfClampDl : 0u) | (q.present << kWfPresentShift) |
+                     (q.traced << kWfTracedShift);
+                slots |= q.traced << 1;
+                const V l = sample_bsdf(c, s, lb, v);
+                const float pdf = pdf_bsdf(s, lb, v, l);
+                const V wgt = vds(eval_bsdf(s, lb, v, l), fmaxf(pdf, 1e-20f));
+                bool go = !(pdf <= 0.0f || all_nonpos(wgt));
+                if (go) {
+                    thr = vmul(thr, wgt);
+                    const bool mirror = (depth == 0u || c.mirror_path) && s.roughness < kMirrorRoughness &&
+                                        c.sampled_spec;
+                    if (depth >= c.pu(COUNTS, 3)) {
+                        const float qq = fminf(fmaxf(thr.x, fmaxf(thr.y, thr.z)), 0.95f);
+                        if (c.g.r1(c.g.dim(D_RR)) > qq)
+                            go = false;
+                        else
+                            thr = vds(thr, qq);
+                    }
+                    if (go) {
+                        const V no = offset_origin(s.p, dot(l, s.ng) < 0.0f ? vneg(s.ng) : s.ng);
+                        a.rays[2 * uint64_t(p)] = make_float4(no.x, no.y, no.z, kInf);
+                        a.rays[2 * uint64_t(p) + 1] = make_float4(l.x, l.y, l.z, __uint_as_float(kTraceClosest));
+                        a.thr[p] = make_float4(thr.x, thr.y, thr.z, pdf);
+                        f = (f & ~(kWfDepth | kWfSpecular | kWfStraight | kWfMirror)) | (depth + 1u) | kWfHasRay |
+                            (mirror ? kWfMirror : 0u);
+                        slots |= 1u;
+                    }
+                }
+            }
+        }
+    }
+    a.rad[p] = make_float4(radiance.x, radiance.y, radiance.z, __uint_as_float(f));
+    alive = (f & (kWfHasRay | kWfDl)) != 0;
+    return slots;
+}
+
+// Bounce k's shading: the paths of active list k % 2 (their count at counts[3 + k % 3]); the rays it queues go to
+// list (k + 1) % 2, counted at counts[(k + 1) % 3], which the launch of bounce k - 1 cleared.
+template <bool kCoat, bool kTree>
+__global__ __launch_bounds__(kRenderBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_wf_shade(WfArgs a,
+                                                                                                   uint32_t k) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t in = k % 3u, out = (k + 1u) % 3u, clear = (k + 2u) % 3u;
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.counts[clear] = a.counts[3 + clear] = 0;
+    const uint32_t n = a.counts[3 + in];
+    uint32_t flags = 0, traced[2] = {0, 0};
+    Reads cnt;
+    unsigned long long closest = 0, shadow = 0;
+    const uint32_t waves = gridDim.x * (kRenderBlock / 64);
+    for (uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) / 64 * 64; base < n; base += waves * 64) {
+        const uint32_t i = base + lane;
+        const bool valid = i < n;
+        const uint32_t p = valid ? a.active[k & 1u][i] : 0u;
+        const WfPath id = valid ? wf_path(a, p) : WfPath{0, 0, 0, 0};
+        uint32_t slots = 0;
+        bool alive = false;
+        wf_by_job(a, valid, id.job, [&](const RenderArgs *ra) {
+            Ctx c = wf_ctx(ra, &flags, traced, &cnt);
+            slots = wf_bounce<kCoat, kTree>(c, a, p, id, alive);
+        });
+        wf_append(a, out, (k + 1u) & 1u, p, slots, alive, lane, closest, shadow);
+    }
+    wf_flush(a, flags, lane, closest, shadow);
+}
+
+// Every path's radiance, albedo and normal added to its pixel's record, as k_render adds a pixel's samples: in the
+// order of their sample index (a run holds a pixel's samples `count` paths apart; the runs of one pixel come in
+// sample order, chunk after chunk).
+__global__ __launch_bounds__(kRenderBlock) void k_wf_accum(WfArgs a, uint32_t pixels) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < pixels; i += gridDim.x * blockDim.x) {
+        uint32_t lo = 0, hi = a.nsubs;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (a.subs[mid].acc_first <= i)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        const WfSub s = a.subs[lo];
+        const RenderArgs *ra = &a.jobs[s.job].args;
+        const uint32_t j = i - s.acc_first, pixel = s.pixel_first + j;
+        float4 *pix = ra->accum + 5 * uint64_t(pixel);
+        float4 col = pix[0], alb = pix[1], nrm = pix[2];
+        for (uint32_t r = 0; r < s.rounds; r++) {
+            const uint32_t p = s.path_first + r * s.count + j;
+            V rad = xyz(a.rad[p]), al = xyz(a.alb[p]), nm = xyz(a.nrm[p]);
+            if (non_finite(rad)) rad = v3(0.0f, 0.0f, 0.0f);
+            if (non_finite(al)) al = v3(0.0f, 0.0f, 0.0f);
+            if (non_finite(nm)) nm = v3(0.0f, 1.0f, 0.0f);
+            const float lum = luminance(rad);
+            col.x += rad.x, col.y += rad.y, col.z += rad.z, col.w += 1.0f;
+            alb.x += al.x, alb.y += al.y, alb.z += al.z, alb.w += lum;
+            nrm.x += nm.x, nrm.y += nm.y, nrm.z += nm.z, nrm.w += lum * lum;
+        }
+        pix[0] = col;
+        pix[1] = alb;
+        pix[2] = nrm;
+    }
+}
+
 __global__ void k_reset(uint32_t *c) { *c = 0; }
 
 }  // namespace
@@ -1052,6 +1401,35 @@ int render_occupancy(int *blocks_per_cu) {
     const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_render<false, false, false>, kRenderBlock, 0);
     *blocks_per_cu = b;
     return e == hipSuccess ? 0 : -1;
+}
+
+int wf_shade_occupancy(int *blocks_per_cu) {
+    int b = 0;
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_wf_shade<false, false>, kRenderBlock, 0);
+    *blocks_per_cu = b;
+    return e == hipSuccess ? 0 : -1;
+}
+
+int launch_wf_begin(const WfArgs &a, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_wf_begin, dim3(grid), dim3(kRenderBlock), 0, s, a);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_wf_shade(const WfArgs &a, uint32_t k, bool coat, bool tree, int grid, hipStream_t s) {
+    if (coat && tree)
+        hipLaunchKernelGGL((k_wf_shade<true, true>), dim3(grid), dim3(kRenderBlock), 0, s, a, k);
+    else if (coat)
+        hipLaunchKernelGGL((k_wf_shade<true, false>), dim3(grid), dim3(kRenderBlock), 0, s, a, k);
+    else if (tree)
+        hipLaunchKernelGGL((k_wf_shade<false, true>), dim3(grid), dim3(kRenderBlock), 0, s, a, k);
+    else
+        hipLaunchKernelGGL((k_wf_shade<false, false>), dim3(grid), dim3(kRenderBlock), 0, s, a, k);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_wf_accum(const WfArgs &a, uint32_t pixels, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_wf_accum, dim3(grid), dim3(kRenderBlock), 0, s, a, pixels);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int launch_render(const RenderJob *d_jobs, uint32_t njobs, uint32_t total_tiles, uint32_t *counter,

@@ -41,6 +41,47 @@ struct RenderJob {
     uint32_t pad[3];
 };
 
+// ---- the wavefront path tracer (render.hip k_wf_*) ----------------------------------------------------------------
+// A chunk's paths (one per pixel and sample) live in structure-of-arrays state; each bounce the walk kernel traces
+// every path's next closest ray and the previous bounce's shadow rays together, from a compacted list of ray slots,
+// and the shade kernel resolves those shadow rays, shades the hit and queues the next rays.
+constexpr uint32_t kWfShadow = 5;             // shadow rays one bounce can leave pending (directional lights + 2)
+constexpr uint32_t kWfSlots = kWfShadow + 1;  // a path's ray slots: 0 the next closest ray, then its shadow rays
+constexpr uint32_t kWfMaxSuns = kWfShadow - 2;  // directional lights the wavefront takes (more: the megakernel)
+
+// A run of a chunk's paths: `rounds` samples (sample_first on) of pixels [pixel_first, pixel_first + count) of one job,
+// path path_first + r x count + j for sample r and pixel j; acc_first: the run's first pixel among the chunk's runs'.
+struct WfSub {
+    uint32_t job, pixel_first, count, sample_first, rounds, path_first, acc_first, width;
+};
+
+struct WfArgs {
+    const RenderJob *jobs;
+    const WfSub *subs;
+    uint32_t nsubs, npaths;  // the chunk's runs and paths
+    uint32_t stride;         // entries in each state array and each ray slot
+    float4 *thr;             // xyz throughput, w the last BSDF pdf
+    float4 *rad;             // xyz radiance, w the path's flag bits
+    float4 *alb, *nrm;       // xyz the camera hit's albedo and normal
+    float4 *misx, *misn;     // xyz the vertex direct light last sampled from and its normal (emit_mis)
+    float4 *thr_dl;          // xyz the throughput the pending shadow rays' contributions are weighed by
+    float4 *contrib;         // [kWfShadow][stride]: xyz a pending direct-light contribution
+    float4 *rays;            // [kWfSlots][stride] trt_ray
+    uint32_t *hits;          // [kWfSlots][stride] trt_hit
+    uint32_t *list[2];       // compacted ray-slot indices (slot x stride + path), by bounce parity
+    uint32_t *active[2];     // the paths the shade kernel takes, by bounce parity
+    uint32_t *counts;        // rings of 3 by bounce: [k % 3] list k's length, [3 + k % 3] active list k's
+    uint32_t *guard;
+    unsigned long long *rays_traced;  // closest, shadow
+};
+int wf_shade_occupancy(int *blocks_per_cu);
+// the chunk's counts set, and every path's camera ray queued (list 0, active 0)
+int launch_wf_begin(const WfArgs &a, int grid, hipStream_t s);
+// bounce k's shade kernel, after the walk of list k % 2: reads active list k % 2, queues into the lists (k + 1) % 2
+int launch_wf_shade(const WfArgs &a, uint32_t k, bool coat, bool tree, int grid, hipStream_t s);
+// every path's radiance, albedo and normal added to its pixel's record, sample after sample; pixels: the runs' pixels
+int launch_wf_accum(const WfArgs &a, uint32_t pixels, int grid, hipStream_t s);
+
 int render_occupancy(int *blocks_per_cu);
 // accum.hip: an accumulator's color sums (non-finite ones as zero) and count added to an FP64 image, 4 doubles a pixel
 int launch_add64(const float4 *acc, uint64_t n, double *sum, hipStream_t s);

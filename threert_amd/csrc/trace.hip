@@ -58,6 +58,7 @@ constexpr uint32_t kChunk = 64;  // at least a wave: one refill switches chunks 
 template <bool kCount, bool kWatertight>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_dyn(TraceArgs a) {
     __shared__ uint32_t lds[kLdsStack * kTraceBlock];
+    const uint64_t n = a.n_dev ? uint64_t(*a.n_dev) : a.n;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     Stack st{(TRT_LDS uint32_t *)(lds + threadIdx.x), a.spill, gtid, a.spill_stride, 0, kLdsStack + a.spill_entries};
@@ -92,13 +93,13 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_dyn(TraceArgs a) {
                 if (lane == 0) next_v = atomicAdd(a.counter, kChunk);
             }
             if (!active && !exhausted) {
-                if (i < a.n) {
-                    ray = uint32_t(i);
-                    const float4 r0 = a.rays[2 * i], r1 = a.rays[2 * i + 1];
+                if (i < n) {
+                    ray = a.ray_index ? a.ray_index[i] : uint32_t(i);
+                    const float4 r0 = a.rays[2 * uint64_t(ray)], r1 = a.rays[2 * uint64_t(ray) + 1];
                     if (wide_begin(a, F3{r0.x, r0.y, r0.z}, F3{r1.x, r1.y, r1.z}, r0.w, __float_as_uint(r1.w), w, st))
                         active = true;
                     else
-                        store_hit(a.hits, i, w.h);
+                        store_hit(a.hits, ray, w.h);
                 } else {
                     exhausted = true;
                 }

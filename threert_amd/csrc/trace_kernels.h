@@ -31,6 +31,8 @@ struct TraceArgs {
     uint32_t *guard_flag;     // kFlag* bits of the walks that stopped and reported
     uint32_t *spill;          // stack entries past kLdsStack, [entry][thread]
     unsigned long long *reads;  // counting launches: nodes, triangles, units
+    const uint32_t *ray_index;  // NULL: rays[i] for i < n; else rays[ray_index[i]] (hits stored at the same index)
+    const uint32_t *n_dev;      // NULL: n; else the batch's size read from device memory (the wavefront's queues)
     uint64_t n;
     uint32_t spill_stride;    // threads in the grid
     uint32_t spill_entries;   // stack entries a thread has past kLdsStack
