@@ -62,6 +62,7 @@ struct trt_device {
     uint32_t *spill = nullptr;
     uint64_t spill_bytes = 0;
     unsigned long long *reads = nullptr;
+    unsigned long long *ray_counts = nullptr;  // kShards pairs (closest, shadow), kRayShardGap apart
     void *denoise_scratch = nullptr;  // trt_denoise_atrous' intermediate images
     uint64_t denoise_bytes = 0;
     void *staging = nullptr;  // trt_trace_rays' device copies of the caller's rays and hits
@@ -78,9 +79,8 @@ struct trt_device {
     std::atomic<int> refs{1};
 };
 
-// d->reads: [0..2] a counting trace launch's reads, [kRenderRays], [kRenderRays + 1] rays the render launches traced,
-// [kRenderReads .. + 3] a counting render launch's reads (nodes, triangles, units, shaded hits)
-constexpr int kRenderRays = 4;
+// d->reads: [0..2] a counting trace launch's reads, [kRenderReads .. + 3] a counting render launch's reads (nodes,
+// triangles, units, shaded hits). The rays the render launches traced are in d->ray_counts, sharded (kRayShardGap).
 constexpr int kRenderReads = 8;
 
 struct trt_scene {
@@ -120,11 +120,13 @@ int trt_device_create(int ordinal, trt_device **out) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&d->ev0);
     if (e == hipSuccess) e = hipEventCreate(&d->ev1);
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&d->counter), 256);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&d->counter), trt::kShardBytes);
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&d->guard), 256);
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&d->reads), 256);
     if (e == hipSuccess) e = hipMemset(d->guard, 0, 256);
     if (e == hipSuccess) e = hipMemset(d->reads, 0, 256);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&d->ray_counts), trt::kRayCountBytes);
+    if (e == hipSuccess) e = hipMemset(d->ray_counts, 0, trt::kRayCountBytes);
     if (e != hipSuccess) {
         trt_device_release(d);
         return fail(TRT_E_DEVICE, std::string("trt_device_create: ") + hipGetErrorString(e));
@@ -152,6 +154,7 @@ static void device_unref(trt_device *d) {
     hipFree(d->counter);
     hipFree(d->guard);
     hipFree(d->reads);
+    hipFree(d->ray_counts);
     hipFree(d->denoise_scratch);
     hipFree(d->spill);
     hipFree(d->staging);
@@ -1102,6 +1105,8 @@ static int job_args(const trt_render_job &j, trt::RenderArgs &a, std::string *wh
 // megakernel does.
 constexpr uint64_t kWfMaxPaths = uint64_t{1} << 22;
 constexpr uint64_t kWfBytesPerPath = 24 * 16 + (6 * 6 + 2 * 6 + 2) * 4;  // 24 float4 arrays, hits, lists, actives
+constexpr uint64_t kWfCountBytes = 3 * 2 * trt::kShardBytes;               // the queues' segment counts
+constexpr uint64_t kWfStrideStep = uint64_t{trt::kShards} * 64;          // a segment holds whole waves of paths
 struct WfChunk {
     uint32_t sub_first, nsubs, npaths, pixels;
 };
@@ -1155,6 +1160,7 @@ static trt::WfArgs wf_layout(void *base, uint64_t stride) {
     trt::WfArgs w{};
     auto *f = static_cast<float4 *>(base);
     w.stride = static_cast<uint32_t>(stride);
+    w.cap = static_cast<uint32_t>(stride / trt::kShards);
     w.thr = f;
     w.rad = f + stride;
     w.alb = f + 2 * stride;
@@ -1186,7 +1192,7 @@ static int ensure_wf(trt_device *d, uint64_t stride) {
     TRT_HIP(hipFree(d->wf));
     d->wf = nullptr;
     d->wf_stride = 0;
-    TRT_HIP(hipMalloc(&d->wf, stride * kWfBytesPerPath + 256));
+    TRT_HIP(hipMalloc(&d->wf, stride * kWfBytesPerPath + kWfCountBytes));
     d->wf_stride = stride;
     return TRT_OK;
 }
@@ -1199,11 +1205,12 @@ static int wf_launch(trt_device *d, const trt::RenderJob *d_jobs, const trt::WfS
     trt::WfArgs w = wf_layout(d->wf, stride);
     w.jobs = d_jobs;
     w.guard = d->guard;
-    w.rays_traced = d->reads + kRenderRays;
+    w.rays_traced = d->ray_counts;
     trt::TraceArgs t = geo;
     t.rays = w.rays;
     t.hits = w.hits;
     t.n = trt::kWfSlots * stride;
+    t.seg_cap = trt::kWfSlots * w.cap;
     const int trace_grid = d->cus * d->blocks_wide;
     for (const WfChunk &c : chunks) {
         w.subs = d_subs + c.sub_first;
@@ -1211,11 +1218,12 @@ static int wf_launch(trt_device *d, const trt::RenderJob *d_jobs, const trt::WfS
         w.npaths = c.npaths;
         const int blocks = static_cast<int>((c.npaths + trt::kRenderBlock - 1) / trt::kRenderBlock);
         const int small = std::min(blocks, d->cus * 8);
-        const int shade_grid = std::min(blocks, d->cus * d->blocks_shade);
+        // every queue segment needs a block: at least kShards
+        const int shade_grid = std::max(static_cast<int>(trt::kShards), std::min(blocks, d->cus * d->blocks_shade));
         if (trt::launch_wf_begin(w, small, d->stream)) return fail(TRT_E_DEVICE, "wavefront launch failed");
         for (uint32_t k = 0; k <= last; k++) {
             t.ray_index = w.list[k & 1u];
-            t.n_dev = w.counts + k % 3u;
+            t.seg_counts = w.counts + (k % 3u) * 2 * trt::kShards * trt::kShardGap;  // ring entry k % 3's ray list
             if (trt::launch_trace(t, true, false, trace_grid, d->stream))
                 return fail(TRT_E_DEVICE, "wavefront trace launch failed");
             if (trt::launch_wf_shade(w, k, coat, tree, shade_grid, d->stream))
@@ -1276,7 +1284,7 @@ static int render_batch(trt_device *d, const trt_render_job *jobs, uint32_t n, u
     }
     std::vector<trt::WfSub> subs;
     std::vector<WfChunk> chunks;
-    const uint64_t stride = (std::min(paths, kWfMaxPaths) + 255) / 256 * 256;
+    const uint64_t stride = (std::min(paths, kWfMaxPaths) + kWfStrideStep - 1) / kWfStrideStep * kWfStrideStep;
     if (wf) wf_plan(table, stride, subs, chunks);
     std::lock_guard<std::mutex> lock(d->mu);
     TRT_HIP(hipSetDevice(d->ordinal));
@@ -1326,7 +1334,7 @@ static int render_batch(trt_device *d, const trt_render_job *jobs, uint32_t n, u
         if (rc) return rc;
     } else if (trt::launch_render(static_cast<const trt::RenderJob *>(ring.dev[k]), static_cast<uint32_t>(table.size()),
                            static_cast<uint32_t>(tiles), d->counter, d->spill, threads,
-                           static_cast<uint32_t>(entries), d->guard, d->reads + kRenderRays, coat, tree, reads, grid,
+                           static_cast<uint32_t>(entries), d->guard, d->ray_counts, coat, tree, reads, grid,
                            d->stream))
         return fail(TRT_E_DEVICE, "render kernel launch failed");
     TRT_HIP(hipEventRecord(ring.ev[k], d->stream));
@@ -1362,11 +1370,13 @@ int trt_render_ray_counts(trt_device *d, uint64_t *closest, uint64_t *shadow) {
     {
         std::lock_guard<std::mutex> lock(d->mu);
         TRT_HIP(hipSetDevice(d->ordinal));
-        unsigned long long r[2] = {0, 0};
-        TRT_HIP(hipMemcpyAsync(r, d->reads + kRenderRays, sizeof(r), hipMemcpyDeviceToHost, d->stream));
-        TRT_HIP(hipMemsetAsync(d->reads + kRenderRays, 0, sizeof(r), d->stream));
+        std::vector<unsigned long long> r(trt::kRayCountBytes / sizeof(unsigned long long));
+        TRT_HIP(hipMemcpyAsync(r.data(), d->ray_counts, trt::kRayCountBytes, hipMemcpyDeviceToHost, d->stream));
+        TRT_HIP(hipMemsetAsync(d->ray_counts, 0, trt::kRayCountBytes, d->stream));
         TRT_HIP(hipStreamSynchronize(d->stream));
-        *closest = r[0], *shadow = r[1];
+        uint64_t c = 0, sh = 0;
+        for (uint32_t k = 0; k < trt::kShards; k++) c += r[k * trt::kRayShardGap], sh += r[k * trt::kRayShardGap + 1];
+        *closest = c, *shadow = sh;
     }
     return trt_device_sync(d);
 }

@@ -1030,11 +1030,13 @@ __global__ __launch_bounds__(kRenderBlock) __attribute__((amdgpu_waves_per_eu(3)
     uint32_t flags = 0;
     uint32_t traced[2] = {0, 0};
     Reads cnt;
+    const ShardSet sh{counter, nullptr, total, 0};
+    uint32_t shard = blockIdx.x % kShards, closed = 0;
     for (;;) {
-        uint32_t tile = 0;
-        if (lane == 0) tile = atomicAdd(counter, 1u);
-        tile = __builtin_amdgcn_readfirstlane(__shfl(tile, 0, 64));
-        if (tile >= total) break;
+        uint32_t req = 0;
+        if (lane == 0) req = atomicAdd(sh.counter(shard), 1u);
+        uint32_t tile, t1;
+        if (!shard_take(sh, shard, closed, req, 1u, lane, tile, t1)) break;
         uint32_t lo = 0, hi = njobs;
         while (hi - lo > 1) {
             const uint32_t mid = (lo + hi) / 2;
@@ -1072,8 +1074,9 @@ __global__ __launch_bounds__(kRenderBlock) __attribute__((amdgpu_waves_per_eu(3)
         traced[1] += __shfl_xor(traced[1], k, 64);
     }
     if (lane == 0) {
-        atomicAdd(rays, (unsigned long long)traced[0]);
-        atomicAdd(rays + 1, (unsigned long long)traced[1]);
+        unsigned long long *r = rays + (blockIdx.x % kShards) * kRayShardGap;
+        atomicAdd(r, (unsigned long long)traced[0]);
+        atomicAdd(r + 1, (unsigned long long)traced[1]);
     }
     if (kCount) {
         atomicAdd(reads + 0, (unsigned long long)cnt.nodes);
@@ -1136,11 +1139,16 @@ __device__ __forceinline__ void wf_by_job(const WfArgs &a, bool valid, uint32_t 
         }
     }
 }
-// The wave's queued rays (bit s of `slots`: ray slot s of path p) appended to `list`, one atomic a wave, and its
-// paths still running to `active`.
-// out: the counts' ring entry of the next bounce; parity: its lists
-__device__ __forceinline__ void wf_append(const WfArgs &a, uint32_t out, uint32_t parity, uint32_t p, uint32_t slots,
-                                          bool alive, uint32_t lane, unsigned long long &closest,
+// The queues are kShards segments (shards.h), each with its own counts: a path stays in its segment (p / cap) from
+// bounce to bounce, the shade kernel's blocks of segment s (blockIdx % kShards == s) take its active paths and append
+// to it, and the walk kernel takes the segments as its shards.
+__device__ __forceinline__ uint32_t *wf_count(const WfArgs &a, uint32_t ring, uint32_t which, uint32_t seg) {
+    return a.counts + ((ring * 2 + which) * kShards + seg) * kShardGap;  // which: 0 the ray list, 1 the active paths
+}
+// The wave's queued rays (bit s of `slots`: ray slot s of path p) appended to segment `seg` of ray list (ring, parity),
+// one atomic a wave, and its paths still running to the segment of the active list.
+__device__ __forceinline__ void wf_append(const WfArgs &a, uint32_t ring, uint32_t parity, uint32_t seg, uint32_t p,
+                                          uint32_t slots, bool alive, uint32_t lane, unsigned long long &closest,
                                           unsigned long long &shadow) {
     const uint64_t below = (uint64_t{1} << lane) - 1;
     uint32_t total = 0;
@@ -1151,44 +1159,48 @@ __device__ __forceinline__ void wf_append(const WfArgs &a, uint32_t out, uint32_
         else shadow += m;
     }
     uint32_t first = 0;
-    if (lane == 0 && total) first = atomicAdd(a.counts + out, total);
+    if (lane == 0 && total) first = atomicAdd(wf_count(a, ring, 0, seg), total);
     first = __builtin_amdgcn_readfirstlane(__shfl(first, 0, 64));
     const uint64_t st = a.stride;
+    uint32_t *list = a.list[parity] + uint64_t(seg) * kWfSlots * a.cap;
     for (uint32_t s = 0; s < kWfSlots; s++) {
         const uint64_t b = __ballot((slots >> s) & 1u);
-        if ((slots >> s) & 1u) a.list[parity][first + __popcll(b & below)] = uint32_t(s * st + p);
+        if ((slots >> s) & 1u) list[first + __popcll(b & below)] = uint32_t(s * st + p);
         first += __popcll(b);
     }
     const uint64_t live = __ballot(alive);
     uint32_t afirst = 0;
-    if (lane == 0 && live) afirst = atomicAdd(a.counts + 3 + out, uint32_t(__popcll(live)));
+    if (lane == 0 && live) afirst = atomicAdd(wf_count(a, ring, 1, seg), uint32_t(__popcll(live)));
     afirst = __builtin_amdgcn_readfirstlane(__shfl(afirst, 0, 64));
-    if (alive) a.active[parity][afirst + __popcll(live & below)] = p;
+    if (alive) a.active[parity][uint64_t(seg) * a.cap + afirst + __popcll(live & below)] = p;
 }
 __device__ __forceinline__ void wf_flush(const WfArgs &a, uint32_t flags, uint32_t lane, unsigned long long closest,
                                          unsigned long long shadow) {
     if (flags) atomicOr(a.guard, flags);
     if (lane == 0 && (closest | shadow)) {
-        atomicAdd(a.rays_traced, closest);
-        atomicAdd(a.rays_traced + 1, shadow);
+        unsigned long long *r = a.rays_traced + (blockIdx.x % kShards) * kRayShardGap;
+        atomicAdd(r, closest);
+        atomicAdd(r + 1, shadow);
     }
 }
 
-// Every path's camera ray in slot 0 and its state reset; list 0 and active 0 hold every path.
+// Every path's camera ray in slot 0 and its state reset; ray list 0 and active list 0 hold every path, path p in
+// segment p / cap.
 __global__ __launch_bounds__(kRenderBlock) void k_wf_begin(WfArgs a) {
     const uint32_t lane = threadIdx.x & 63u;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        a.counts[0] = a.counts[3] = a.npaths;
-        a.counts[1] = a.counts[4] = 0;
+    if (blockIdx.x == 0 && threadIdx.x < kShards) {
+        const uint32_t seg = threadIdx.x, lo = seg * a.cap;
+        const uint32_t n = a.npaths > lo ? min(a.npaths - lo, a.cap) : 0u;
+        *wf_count(a, 0, 0, seg) = *wf_count(a, 0, 1, seg) = n;
+        *wf_count(a, 1, 0, seg) = *wf_count(a, 1, 1, seg) = 0;
+        if (seg == 0) atomicAdd(a.rays_traced, (unsigned long long)a.npaths);  // the camera rays
     }
     uint32_t flags = 0, traced[2] = {0, 0};
     Reads cnt;
-    unsigned long long closest = 0;
     const uint32_t waves = gridDim.x * (kRenderBlock / 64);
     for (uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) / 64 * 64; base < a.npaths; base += waves * 64) {
         const uint32_t p = base + lane;
         const bool valid = p < a.npaths;
-        closest += __popcll(__ballot(valid));
         const WfPath id = valid ? wf_path(a, p) : WfPath{0, 0, 0, 0};
         wf_by_job(a, valid, id.job, [&](const RenderArgs *ra) {
             Ctx c = wf_ctx(ra, &flags, traced, &cnt);
@@ -1200,11 +1212,12 @@ __global__ __launch_bounds__(kRenderBlock) void k_wf_begin(WfArgs a) {
             a.rad[p] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(kWfHasRay | kWfSpecular | kWfStraight));
             a.alb[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             a.nrm[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            a.list[0][p] = p;
+            const uint32_t seg = p / a.cap;
+            a.list[0][uint64_t(seg) * kWfSlots * a.cap + (p - seg * a.cap)] = p;
             a.active[0][p] = p;
         });
     }
-    wf_flush(a, flags, lane, closest, 0);
+    wf_flush(a, flags, lane, 0, 0);
 }
 
 // One bounce of path p (the loop body of `path`), from what the walk left in its ray slots: the pending direct light
@@ -1328,23 +1341,26 @@ __device__ uint32_t wf_bounce(Ctx &c, const WfArgs &a, uint32_t p, const WfPath 
     return slots;
 }
 
-// Bounce k's shading: the paths of active list k % 2 (their count at counts[3 + k % 3]); the rays it queues go to
-// list (k + 1) % 2, counted at counts[(k + 1) % 3], which the launch of bounce k - 1 cleared.
+// Bounce k's shading: the paths of active list k % 2 (its counts in ring entry k % 3); the rays and paths it queues go
+// to the lists (k + 1) % 2, counted in ring entry (k + 1) % 3, which the launch of bounce k - 1 cleared. Segment s is
+// the work of the blocks with blockIdx % kShards == s (the grid has at least kShards blocks).
 template <bool kCoat, bool kTree>
 __global__ __launch_bounds__(kRenderBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_wf_shade(WfArgs a,
                                                                                                    uint32_t k) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t in = k % 3u, out = (k + 1u) % 3u, clear = (k + 2u) % 3u;
-    if (blockIdx.x == 0 && threadIdx.x == 0) a.counts[clear] = a.counts[3 + clear] = 0;
-    const uint32_t n = a.counts[3 + in];
+    if (blockIdx.x == 0 && threadIdx.x < 2 * kShards) *wf_count(a, clear, threadIdx.x / kShards, threadIdx.x % kShards) = 0;
+    const uint32_t seg = blockIdx.x % kShards;
+    const uint32_t nb = (gridDim.x - seg + kShards - 1) / kShards, bi = blockIdx.x / kShards;
+    const uint32_t n = *wf_count(a, in, 1, seg);
+    const uint32_t *act = a.active[k & 1u] + uint64_t(seg) * a.cap;
     uint32_t flags = 0, traced[2] = {0, 0};
     Reads cnt;
     unsigned long long closest = 0, shadow = 0;
-    const uint32_t waves = gridDim.x * (kRenderBlock / 64);
-    for (uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) / 64 * 64; base < n; base += waves * 64) {
+    for (uint32_t base = bi * kRenderBlock + threadIdx.x / 64 * 64; base < n; base += nb * kRenderBlock) {
         const uint32_t i = base + lane;
         const bool valid = i < n;
-        const uint32_t p = valid ? a.active[k & 1u][i] : 0u;
+        const uint32_t p = valid ? act[i] : 0u;
         const WfPath id = valid ? wf_path(a, p) : WfPath{0, 0, 0, 0};
         uint32_t slots = 0;
         bool alive = false;
@@ -1352,7 +1368,7 @@ __global__ __launch_bounds__(kRenderBlock) __attribute__((amdgpu_waves_per_eu(3)
             Ctx c = wf_ctx(ra, &flags, traced, &cnt);
             slots = wf_bounce<kCoat, kTree>(c, a, p, id, alive);
         });
-        wf_append(a, out, (k + 1u) & 1u, p, slots, alive, lane, closest, shadow);
+        wf_append(a, out, (k + 1u) & 1u, seg, p, slots, alive, lane, closest, shadow);
     }
     wf_flush(a, flags, lane, closest, shadow);
 }
@@ -1392,7 +1408,7 @@ __global__ __launch_bounds__(kRenderBlock) void k_wf_accum(WfArgs a, uint32_t pi
     }
 }
 
-__global__ void k_reset(uint32_t *c) { *c = 0; }
+__global__ void k_reset(uint32_t *c) { c[threadIdx.x * kShardGap] = 0; }
 
 }  // namespace
 
@@ -1436,7 +1452,7 @@ int launch_render(const RenderJob *d_jobs, uint32_t njobs, uint32_t total_tiles,
                   uint32_t *spill, uint32_t spill_stride, uint32_t spill_entries, uint32_t *guard,
                   unsigned long long *rays, bool coat, bool tree, unsigned long long *reads, int grid,
                   hipStream_t s) {
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, s, counter);
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(kShards), 0, s, counter);
 #define TRT_RENDER(C, T, N)                                                                                          \
     hipLaunchKernelGGL((k_render<C, T, N>), dim3(grid), dim3(kRenderBlock), 0, s, d_jobs, njobs, total_tiles,        \
                        counter, spill, spill_stride, spill_entries, guard, rays, reads)

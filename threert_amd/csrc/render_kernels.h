@@ -10,6 +10,10 @@
 namespace trt {
 
 constexpr int kRenderBlock = 256;
+// The render launches' ray counts (closest, shadow), one pair a shard (shards.h): shard s's pair at
+// counts[s x kRayShardGap], 256 bytes apart; the total is their sum.
+constexpr uint32_t kRayShardGap = 32;
+constexpr uint64_t kRayCountBytes = uint64_t{kShards} * kRayShardGap * sizeof(unsigned long long);
 constexpr int kParamWords = 156;  // Params: 39 vec4 (gpu.rs:129-185)
 
 struct RenderArgs {
@@ -60,6 +64,7 @@ struct WfArgs {
     const WfSub *subs;
     uint32_t nsubs, npaths;  // the chunk's runs and paths
     uint32_t stride;         // entries in each state array and each ray slot
+    uint32_t cap;            // paths in a queue segment (stride / kShards): path p is in segment p / cap
     float4 *thr;             // xyz throughput, w the last BSDF pdf
     float4 *rad;             // xyz radiance, w the path's flag bits
     float4 *alb, *nrm;       // xyz the camera hit's albedo and normal
@@ -68,11 +73,11 @@ struct WfArgs {
     float4 *contrib;         // [kWfShadow][stride]: xyz a pending direct-light contribution
     float4 *rays;            // [kWfSlots][stride] trt_ray
     uint32_t *hits;          // [kWfSlots][stride] trt_hit
-    uint32_t *list[2];       // compacted ray-slot indices (slot x stride + path), by bounce parity
-    uint32_t *active[2];     // the paths the shade kernel takes, by bounce parity
-    uint32_t *counts;        // rings of 3 by bounce: [k % 3] list k's length, [3 + k % 3] active list k's
+    uint32_t *list[2];       // ray-slot indices (slot x stride + path) by bounce parity, segment s from s x 6 x cap
+    uint32_t *active[2];     // the paths the shade kernel takes, by bounce parity, segment s from s x cap
+    uint32_t *counts;        // the segments' lengths: [bounce % 3][ray list, active list][kShards][kShardGap]
     uint32_t *guard;
-    unsigned long long *rays_traced;  // closest, shadow
+    unsigned long long *rays_traced;  // the sharded ray counts (kRayShardGap)
 };
 int wf_shade_occupancy(int *blocks_per_cu);
 // the chunk's counts set, and every path's camera ray queued (list 0, active 0)
@@ -88,8 +93,8 @@ int launch_add64(const float4 *acc, uint64_t n, double *sum, hipStream_t s);
 // accum.hip: the accumulator's color / albedo / normal means into pw x ph images (edge rows and columns repeated)
 int launch_resolve(const float4 *acc, uint32_t w, uint32_t h, uint32_t pw, uint32_t ph, float *color, float *albedo,
                    float *normal, hipStream_t s);
-// d_jobs: njobs jobs in device memory, tile_first ascending from 0; total_tiles: all their tiles.
-// rays: two counters the launch adds its traced rays to (closest; shadow and any-hit); coat: some job's scene has a
+// d_jobs: njobs jobs in device memory, tile_first ascending from 0; total_tiles: all their tiles; counter: the kShards
+// tile counters. rays: the sharded ray counts the launch adds its traced rays to (closest; shadow and any-hit); coat: some job's scene has a
 // material with a clearcoat (the kernel with the coat's terms; the same bits for scenes without one); tree: some
 // job's Params ask for the light tree (the kernel with its code); reads: NULL, or four counters a counting launch adds
 // to (wide nodes, triangles, units its walks read; hits it shades)

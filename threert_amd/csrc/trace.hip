@@ -1,10 +1,10 @@
 // Ray-batch traversal kernels for gfx950 (wave64): the walks of walk.h over a batch of rays in HBM.
-// Binary walk: persistent waves take 64 rays at a time from a counter and walk them to the end.
+// Binary walk: persistent waves take 64 rays at a time from the shards' counters (shards.h) and walk them to the end.
 // Wide walk: a lane whose ray is finished takes a new one while the rest of its wave walks on. Incoherent rays differ
 // several-fold in length (on the 3M-triangle workload a wave's longest ray takes about four times its mean), so a
 // wave that waited for its longest ray would leave three lanes in four idle; instead, once kRefill lanes are idle
 // (or all are), the wave hands out that many ray indices by lane rank from a chunk it reserved ahead with one
-// atomic. Between refills the wave alternates the two phases of its walks (walk.h WideWalk). A refill stops the
+// atomic on its shard's counter. Between refills the wave alternates the two phases of its walks (walk.h WideWalk). A refill stops the
 // whole wave on the new rays' loads, so refills are batched (on the 3M-triangle workload, every 16 or 24 lanes ran
 // 4 % faster than every 32 and 1 % faster than every 8); with an atomic per refill as well, the best batch was 32
 // and 3.5 % slower. Loading each lane's next ray ahead of time hid the loads too, but took eight more registers, one
@@ -29,13 +29,16 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
     Stack st{(TRT_LDS uint32_t *)(lds + threadIdx.x), a.spill, gtid, a.spill_stride, 0, kLdsStack + a.spill_entries};
     uint32_t flags = 0;
     Reads cnt;
+    const ShardSet sh{a.counter, a.seg_counts, static_cast<uint32_t>(a.n), a.seg_cap};
+    uint32_t s = blockIdx.x % kShards, closed = 0;
     for (;;) {
-        uint32_t first_ray = 0;
-        if (lane == 0) first_ray = atomicAdd(a.counter, 64u);
-        first_ray = __shfl(first_ray, 0, 64);
-        if (uint64_t(first_ray) >= a.n) break;
-        const uint64_t i = uint64_t(first_ray) + lane;
-        if (i < a.n) {
+        uint32_t req = 0;
+        if (lane == 0) req = atomicAdd(sh.counter(s), 64u);
+        uint32_t a0, a1;
+        if (!shard_take(sh, s, closed, req, 64u, lane, a0, a1)) break;
+        const uint32_t k = a0 + lane;
+        if (k < a1) {
+            const uint64_t i = a.ray_index ? a.ray_index[k] : k;
             const float4 r0 = a.rays[2 * i], r1 = a.rays[2 * i + 1];
             const F3 o{r0.x, r0.y, r0.z}, d{r1.x, r1.y, r1.z};
             const uint32_t mode = __float_as_uint(r1.w);
@@ -55,10 +58,12 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
 constexpr uint32_t kRefill = 24;
 constexpr uint32_t kChunk = 64;  // at least a wave: one refill switches chunks at most once
 
-template <bool kCount, bool kWatertight>
+// kShard: the rays come from the shards' counters (the wavefront's queue segments, shards.h); else from one counter,
+// [0, n), which keeps two fewer scalars and a few registers live (five waves a SIMD instead of four: 9 % faster on
+// the 3M-triangle batch, whose 16.8M rays take a quarter of one counter's atomic rate).
+template <bool kCount, bool kWatertight, bool kShard>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_dyn(TraceArgs a) {
     __shared__ uint32_t lds[kLdsStack * kTraceBlock];
-    const uint64_t n = a.n_dev ? uint64_t(*a.n_dev) : a.n;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     Stack st{(TRT_LDS uint32_t *)(lds + threadIdx.x), a.spill, gtid, a.spill_stride, 0, kLdsStack + a.spill_entries};
@@ -69,12 +74,24 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_dyn(TraceArgs a) {
     bool active = false, exhausted = false;
     const uint64_t below = (uint64_t{1} << lane) - 1;
     // Ray indices come to a wave in chunks of kChunk: [res, res_end) is being handed out, and the chunk after it was
-    // requested when this one was taken (its start arrives in lane 0's next_v), so a refill waits on no atomic.
-    uint32_t next_v = 0;
-    if (lane == 0) next_v = atomicAdd(a.counter, 2 * kChunk);
-    uint32_t res = __builtin_amdgcn_readfirstlane(__shfl(next_v, 0, 64));
-    uint32_t res_end = res + kChunk;
-    next_v = res + kChunk;
+    // requested when this one was taken (lane 0's req), so a refill waits on no atomic (until a shard runs out).
+    // `more`: a shard may still hold rays. Every refill that needs more than its chunk holds either hands out rays,
+    // closes a shard, or finds every shard spent (its idle lanes then stop): a lane left idle while `more` holds would
+    // stop the node phase below (kRefill idle lanes end it) without a new ray.
+    const ShardSet sh{a.counter, a.seg_counts, static_cast<uint32_t>(a.n), a.seg_cap};
+    uint32_t s = blockIdx.x % kShards, closed = 0, res = 0, res_end = 0;
+    uint32_t req = 0;
+    bool more = kShard;  // one counter: a lane past n is past every ray
+    if (kShard) {
+        if (lane == 0) req = atomicAdd(sh.counter(s), kChunk);
+        more = shard_take(sh, s, closed, req, kChunk, lane, res, res_end);
+        if (more && lane == 0) req = atomicAdd(sh.counter(s), kChunk);
+    } else {
+        if (lane == 0) req = atomicAdd(a.counter, 2 * kChunk);
+        res = __builtin_amdgcn_readfirstlane(__shfl(req, 0, 64));
+        res_end = res + kChunk;
+        req = res + kChunk;
+    }
     for (;;) {
         // lanes without a walk take new rays, kRefill at a time (or all, when no lane walks)
         const uint64_t idle = __ballot(!active && !exhausted);
@@ -83,24 +100,47 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_dyn(TraceArgs a) {
             const uint32_t need = uint32_t(__popcll(idle)), avail = res_end - res;
             const uint32_t rank = uint32_t(__popcll(idle & below));
             uint32_t i = res + rank;
-            if (need <= avail) {
+            bool got = rank < avail;
+            if (!kShard) {
+                // one counter: chunks run past n, and the lanes past it are done
+                if (need > avail) {
+                    const uint32_t nx = __builtin_amdgcn_readfirstlane(__shfl(req, 0, 64));
+                    i = got ? i : nx + (rank - avail);
+                    res = nx + (need - avail);
+                    res_end = nx + kChunk;
+                    if (lane == 0) req = atomicAdd(a.counter, kChunk);
+                } else {
+                    res += need;
+                }
+                got = i < a.n;
+            } else if (need <= avail) {
                 res += need;
             } else {
-                const uint32_t nx = __builtin_amdgcn_readfirstlane(__shfl(next_v, 0, 64));
-                i = rank < avail ? i : nx + (rank - avail);
-                res = nx + (need - avail);
-                res_end = nx + kChunk;
-                if (lane == 0) next_v = atomicAdd(a.counter, kChunk);
+                res = res_end;
+                uint32_t b0 = 0, b1 = 0;
+                if (more && shard_take(sh, s, closed, req, kChunk, lane, b0, b1)) {
+                    // the lanes past the old chunk take from the new one (shorter at a shard's end: the rest wait)
+                    const uint32_t extra = need - avail < b1 - b0 ? need - avail : b1 - b0;
+                    if (!got && rank - avail < extra) {
+                        i = b0 + (rank - avail);
+                        got = true;
+                    }
+                    res = b0 + extra;
+                    res_end = b1;
+                    if (lane == 0) req = atomicAdd(sh.counter(s), kChunk);
+                } else {
+                    more = false;
+                }
             }
             if (!active && !exhausted) {
-                if (i < n) {
-                    ray = a.ray_index ? a.ray_index[i] : uint32_t(i);
+                if (got) {
+                    ray = a.ray_index ? a.ray_index[i] : i;
                     const float4 r0 = a.rays[2 * uint64_t(ray)], r1 = a.rays[2 * uint64_t(ray) + 1];
                     if (wide_begin(a, F3{r0.x, r0.y, r0.z}, F3{r1.x, r1.y, r1.z}, r0.w, __float_as_uint(r1.w), w, st))
                         active = true;
                     else
                         store_hit(a.hits, ray, w.h);
-                } else {
+                } else if (!more) {
                     exhausted = true;
                 }
             }
@@ -143,12 +183,12 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_dyn(TraceArgs a) {
     }
 }
 
-__global__ void k_reset_counter(uint32_t *c) { *c = 0; }
+__global__ void k_reset_counter(uint32_t *c) { c[threadIdx.x * kShardGap] = 0; }
 
 }  // namespace
 
-// The fixed scratch of the trace entry that allocates nothing (trt_trace_rays_device_noalloc): the ray counter in its
-// first 256 bytes, then the stack entries past the LDS ones, [entry][thread]. One copy on every GPU the library runs
+// The fixed scratch of the trace entry that allocates nothing (trt_trace_rays_device_noalloc): the shards' ray counters
+// in its first kScratchCounterBytes, then the stack entries past the LDS ones, [entry][thread]. One copy on every GPU the library runs
 // on, held until the process ends. Host code reaches it only through the address hipGetSymbolAddress gives on that
 // GPU (the array's name is a host-side stand-in there).
 static __device__ __align__(256) uint32_t scratch[kScratchBytes / sizeof(uint32_t)];
@@ -169,24 +209,31 @@ int scratch_address(int ordinal, void **ptr) {
     return 0;
 }
 
+int launch_shards_reset(uint32_t *ctr, hipStream_t s) {
+    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(kShards), 0, s, ctr);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 int trace_occupancy(bool wide, int *blocks_per_cu) {
     int b = 0;
-    hipError_t e = wide ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_trace_dyn<false, false>, kTraceBlock, 0)
+    hipError_t e = wide ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_trace_dyn<false, false, false>, kTraceBlock, 0)
                         : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_trace<false, false>, kTraceBlock, 0);
     *blocks_per_cu = b;
     return e == hipSuccess ? 0 : -1;
 }
 
 int launch_trace(const TraceArgs &a, bool wide, bool count, int grid, hipStream_t s, bool watertight) {
-    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, s, a.counter);
-    if (wide && watertight && count)
-        hipLaunchKernelGGL((k_trace_dyn<true, true>), dim3(grid), dim3(kTraceBlock), 0, s, a);
+    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(kShards), 0, s, a.counter);
+    if (wide && a.seg_counts)  // the wavefront's queues (no counting or watertight variant)
+        hipLaunchKernelGGL((k_trace_dyn<false, false, true>), dim3(grid), dim3(kTraceBlock), 0, s, a);
+    else if (wide && watertight && count)
+        hipLaunchKernelGGL((k_trace_dyn<true, true, false>), dim3(grid), dim3(kTraceBlock), 0, s, a);
     else if (wide && watertight)
-        hipLaunchKernelGGL((k_trace_dyn<false, true>), dim3(grid), dim3(kTraceBlock), 0, s, a);
+        hipLaunchKernelGGL((k_trace_dyn<false, true, false>), dim3(grid), dim3(kTraceBlock), 0, s, a);
     else if (wide && count)
-        hipLaunchKernelGGL((k_trace_dyn<true, false>), dim3(grid), dim3(kTraceBlock), 0, s, a);
+        hipLaunchKernelGGL((k_trace_dyn<true, false, false>), dim3(grid), dim3(kTraceBlock), 0, s, a);
     else if (wide)
-        hipLaunchKernelGGL((k_trace_dyn<false, false>), dim3(grid), dim3(kTraceBlock), 0, s, a);
+        hipLaunchKernelGGL((k_trace_dyn<false, false, false>), dim3(grid), dim3(kTraceBlock), 0, s, a);
     else if (count)
         hipLaunchKernelGGL((k_trace<false, true>), dim3(grid), dim3(kTraceBlock), 0, s, a);
     else

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 
+#include "shards.h"
 #include "trt_internal.h"
 
 namespace trt {
@@ -27,12 +28,13 @@ struct TraceArgs {
     const float4 *instances;  // Instance, 10 x float4 each
     const float4 *rays;       // trt_ray, 2 x float4 each
     uint32_t *hits;           // trt_hit, 6 words each
-    uint32_t *counter;        // rays handed out so far
+    uint32_t *counter;        // kShards counters (shards.h): the rays each shard has handed out
     uint32_t *guard_flag;     // kFlag* bits of the walks that stopped and reported
     uint32_t *spill;          // stack entries past kLdsStack, [entry][thread]
     unsigned long long *reads;  // counting launches: nodes, triangles, units
-    const uint32_t *ray_index;  // NULL: rays[i] for i < n; else rays[ray_index[i]] (hits stored at the same index)
-    const uint32_t *n_dev;      // NULL: n; else the batch's size read from device memory (the wavefront's queues)
+    const uint32_t *ray_index;  // NULL: rays[i]; else rays[ray_index[i]] (hits stored at the same index)
+    const uint32_t *seg_counts; // NULL: i in [0, n); else kShards segments of ray_index (the wavefront's queues):
+    uint32_t seg_cap;           // segment s from s x seg_cap, seg_counts[s x kShardGap] entries
     uint64_t n;
     uint32_t spill_stride;    // threads in the grid
     uint32_t spill_entries;   // stack entries a thread has past kLdsStack
@@ -43,10 +45,10 @@ struct TraceArgs {
     float top_box[6];
 };
 
-// The fixed per-GPU scratch of trt_trace_rays_device_noalloc (trace.hip): 256 bytes of counter, then spilled stack
+// The fixed per-GPU scratch of trt_trace_rays_device_noalloc (trace.hip): the shards' counters, then spilled stack
 // entries.
 constexpr uint64_t kScratchBytes = uint64_t{32} << 20;
-constexpr uint64_t kScratchCounterBytes = 256;
+constexpr uint64_t kScratchCounterBytes = kShardBytes;
 constexpr int kMaxDevices = 64;
 // The scratch's address on GPU `ordinal`, which must be the current device; 0 on success.
 int scratch_address(int ordinal, void **ptr);
