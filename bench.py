@@ -199,6 +199,20 @@ def hit_agreement(prep, rays: np.ndarray, hits: np.ndarray, sample: int) -> dict
     return {"rays": sample, "same": int(same.sum()), "fraction": float(same.mean())}
 
 
+def under(dev, how, fn):
+    """fn() with the device's render form set to `how` (then back to the default, TRT_RENDER_AUTO)."""
+    dev.set_render(how)
+    try:
+        return fn()
+    finally:
+        dev.set_render(T.RENDER_AUTO)
+
+
+def auto_form(leaves: int, paths: int) -> str:
+    """The form TRT_RENDER_AUTO takes for a one-scene launch (include/threert.h trt_device_set_render)."""
+    return "wavefront" if leaves >= 1024 or paths >= 1 << 22 else "megakernel"
+
+
 def bench_render(dev, frames: int) -> dict:
     """Samples/s at 720p: the instanced city (64 seeded meshes x 4096 instances), 1280 x 720, 1 spp a frame,
     3 bounces, sun + constant sky; `frames` frames (one sample each) timed with HIP events."""
@@ -222,6 +236,19 @@ def bench_render(dev, frames: int) -> dict:
     dev.sync()
     closest, shadow = dev.render_ray_counts()
     ms_frame = ms / frames
+    form = auto_form(len(rs.prep.leaves), w * h)
+
+    def frames_ms():
+        R.render(sc, p, 0, 1, acc)
+        dev.sync()
+        dev.timer_start()
+        for f in range(frames):
+            R.render(sc, p, 1 + f, 1, acc)
+        return round(dev.timer_stop() / frames, 3)
+
+    forms = {"wavefront": under(dev, T.RENDER_WAVEFRONT, frames_ms),
+             "megakernel": under(dev, T.RENDER_MEGAKERNEL, frames_ms)}
+    dev.render_ray_counts()
     # one counting launch of a frame (not timed): what its walks read and the hits it shaded
     rd = R.render_reads(sc, p, 1, 1, acc)
     sc.close()
@@ -230,15 +257,16 @@ def bench_render(dev, frames: int) -> dict:
     walk_b = rd["nodes"] * 64 + rd["tris"] * 48 + rd["units"] * 64
     frame_b = walk_b + rd["shaded"] * SHADE_BYTES + w * h * ACCUM_BYTES
     achieved = frame_b / (ms_frame * 1e-3) / 1e9
-    log(f"bench: city 720p {ms_frame:.2f} ms/frame, {sps / 1e6:.1f} Msamples/s, "
+    log(f"bench: city 720p {ms_frame:.2f} ms/frame ({form}; ms a frame by form {forms}), {sps / 1e6:.1f} Msamples/s, "
         f"{closest / (ms * 1e-3) / 1e6:.1f} M closest + {shadow / (ms * 1e-3) / 1e6:.1f} M shadow rays/s, "
         f"{frame_b / 1e6:.0f} MB a frame, {achieved:.0f} GB/s")
+    kernel = "k_trace_dyn (sharded) + k_wf_shade" if form == "wavefront" else "k_render"
     return {"metric": "samples/s at 720p", "value": round(sps, 1), "unit": "samples/s",
-            "ms_per_frame": round(ms_frame, 3), "frames": frames,
+            "ms_per_frame": round(ms_frame, 3), "frames": frames, "form": form, "ms_per_frame_by_form": forms,
             "rays": {"closest": closest, "shadow": shadow,
                      "closest_mrays_s": round(closest / (ms * 1e-3) / 1e6, 1),
                      "shadow_mrays_s": round(shadow / (ms * 1e-3) / 1e6, 1)},
-            "roofline": {"kernel": "k_render", "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "roofline": {"kernel": kernel, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                          "bytes_per_frame": int(frame_b),
                          "reads_per_frame": rd, "bytes": "64 B a wide node + 48 B a triangle + 64 B a unit the walks "
@@ -273,6 +301,17 @@ def bench_cornell(dev, check: bool = False) -> dict:
     for _ in range(reps):
         R.render(sc, p, 0, spp, acc)
     ms = dev.timer_stop() / reps
+
+    def image_ms():
+        R.render(sc, p, 0, spp, acc)
+        dev.sync()
+        dev.timer_start()
+        for _ in range(reps):
+            R.render(sc, p, 0, spp, acc)
+        return round(dev.timer_stop() / reps, 3)
+
+    forms = {"wavefront": under(dev, T.RENDER_WAVEFRONT, image_ms),
+             "megakernel": under(dev, T.RENDER_MEGAKERNEL, image_ms)}
     acc.clear()
     R.render(sc, p, 0, spp, acc)
     raw = acc.read()
@@ -282,8 +321,11 @@ def bench_cornell(dev, check: bool = False) -> dict:
     sc.close()
     acc.close()
     sps = w * h * spp / (ms * 1e-3)
-    log(f"bench: cornell 512x512x16 {ms:.2f} ms, {sps / 1e6:.1f} Msamples/s, rel RMSE vs FP64 1024 spp {err:.4f}")
+    form = auto_form(len(rs.prep.leaves), w * h * spp)
+    log(f"bench: cornell 512x512x16 {ms:.2f} ms ({form}; by form {forms}), {sps / 1e6:.1f} Msamples/s, "
+        f"rel RMSE vs FP64 1024 spp {err:.4f}")
     out = {"metric": "samples/s", "value": round(sps, 1), "unit": "samples/s", "ms_per_image": round(ms, 3),
+           "form": form, "ms_per_image_by_form": forms,
            "rel_rmse_vs_fp64_1024spp": round(err, 5),
            "workload": "seeded Cornell box, 36 triangles, 512x512, 16 spp, 4 bounces, RR from depth 3, seed 1"}
     if check:
@@ -317,16 +359,25 @@ def bench_precision_render(dev) -> dict:
     R.render(sc, p, 0, 1, acc)  # warm
     dev.sync()
     acc.clear()
-    dev.timer_start()
-    for s in range(0, spp, per):
-        R.render(sc, p, s, per, acc)
-    ms = dev.timer_stop()
-    dev.sync()
+
+    def image_ms():
+        dev.timer_start()
+        for s in range(0, spp, per):
+            R.render(sc, p, s, per, acc)
+        t = dev.timer_stop()
+        dev.sync()
+        return t
+
+    ms = image_ms()
+    forms = {"wavefront": round(under(dev, T.RENDER_WAVEFRONT, image_ms), 2),
+             "megakernel": round(under(dev, T.RENDER_MEGAKERNEL, image_ms), 2)}
     sc.close()
     acc.close()
     sps = w * h * spp / (ms * 1e-3)
-    log(f"bench: precision 1080p x 64 spp {ms:.1f} ms, {sps / 1e6:.1f} Msamples/s")
+    form = auto_form(len(rs.prep.leaves), w * h * per)
+    log(f"bench: precision 1080p x 64 spp {ms:.1f} ms ({form}; by form {forms}), {sps / 1e6:.1f} Msamples/s")
     return {"metric": "samples/s", "value": round(sps, 1), "unit": "samples/s", "ms_per_image": round(ms, 2),
+            "form": form, "ms_per_image_by_form": forms,
             "workload": "3M clustered triangles (5 % emissive, 150k emitter triangles), sun + constant sky, "
                         "1920x1080, 64 spp (8 launches of 8), 4 bounces, RR from depth 3"}
 

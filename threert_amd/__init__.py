@@ -274,7 +274,8 @@ class Device:
         return DeviceBuffer(self, nbytes)
 
     def set_render(self, how: int):
-        """How later render launches run their paths: RENDER_AUTO (default: the megakernel), RENDER_MEGAKERNEL or
+        """How later render launches run their paths: RENDER_AUTO (default: the wavefront for scenes of 1,024
+        triangles or more or launches of 4,194,304 paths or more, else the megakernel), RENDER_MEGAKERNEL or
         RENDER_WAVEFRONT (bounce by bounce over queued rays). The same bits either way."""
         _check(lib().trt_device_set_render(self.h, how))
 

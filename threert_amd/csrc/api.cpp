@@ -1104,6 +1104,11 @@ static int job_args(const trt_render_job &j, trt::RenderArgs &a, std::string *wh
 // never holds one pixel twice, and a pixel's samples come in sample order, so that the accumulation adds them as the
 // megakernel does.
 constexpr uint64_t kWfMaxPaths = uint64_t{1} << 22;
+// TRT_RENDER_AUTO's choice: on MI355X the wavefront renders scenes of 3,000 to 3M triangles 1.4-4.4x as fast as the
+// megakernel (clustered soups at 320x180 to 1280x720, 1-4 spp; the 66M-triangle instanced city 1.4-2.0x); the
+// 36-triangle Cornell box, whose bounces cost less than the path state the wavefront moves, 0.7-0.8x as fast at
+// 1280x720 and 1-4 spp (up to 3.7M paths) but 1.15x at 512x512 x 16 spp (a full chunk of 4.2M paths).
+constexpr uint64_t kWfAutoLeaves = 1024;
 constexpr uint64_t kWfBytesPerPath = 24 * 16 + (6 * 6 + 2 * 6 + 2) * 4;  // 24 float4 arrays, hits, lists, actives
 constexpr uint64_t kWfCountBytes = 3 * 2 * trt::kShardBytes;               // the queues' segment counts
 constexpr uint64_t kWfStrideStep = uint64_t{trt::kShards} * 64;          // a segment holds whole waves of paths
@@ -1267,8 +1272,10 @@ static int render_batch(trt_device *d, const trt_render_job *jobs, uint32_t n, u
     if (tiles >= (uint64_t{1} << 31)) return fail(TRT_E_INVALID, "trt_render_batch: too many tiles in one launch");
     if (table.empty()) return ok();
     // The wavefront takes batches of one scene (its walk launches take one scene's trees) whose jobs have at most
-    // kWfMaxSuns directional lights; counting launches stay on the megakernel.
-    bool wf = reads == nullptr && d->render.load() == TRT_RENDER_WAVEFRONT;
+    // kWfMaxSuns directional lights; counting launches stay on the megakernel. TRT_RENDER_AUTO takes it for scenes
+    // of kWfAutoLeaves triangles or more and for launches of a whole chunk of paths or more.
+    const uint32_t how = d->render.load();
+    bool wf = reads == nullptr && how != TRT_RENDER_MEGAKERNEL;
     uint64_t paths = 0;
     uint32_t last = 0;
     const trt_scene *first_scene = nullptr;
@@ -1277,9 +1284,11 @@ static int render_batch(trt_device *d, const trt_render_job *jobs, uint32_t n, u
         if (!first_scene) first_scene = jobs[k].scene;
         wf = jobs[k].scene == first_scene;
     }
+    for (const auto &j : table) paths += uint64_t(j.args.params[16]) * j.args.params[17] * j.args.spp;
+    if (wf && how == TRT_RENDER_AUTO)
+        wf = first_scene && (first_scene->leaf_count >= kWfAutoLeaves || paths >= kWfMaxPaths);
     for (const auto &j : table) {
         wf = wf && j.args.params[36] <= trt::kWfMaxSuns;  // P.counts2.x: the directional lights
-        paths += uint64_t(j.args.params[16]) * j.args.params[17] * j.args.spp;
         last = std::max(last, j.args.params[22]);  // P.counts.z: the last bounce
     }
     std::vector<trt::WfSub> subs;
