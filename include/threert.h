@@ -303,9 +303,9 @@ int trt_render_batch(trt_device *dev, const trt_render_job *jobs, uint32_t n);
  * or more, where it renders 1.4-4.4x as fast, and for launches of 4,194,304 paths or more; the megakernel for
  * smaller launches of smaller scenes, where the path state costs more than it saves).
  * Both give the same bits.
- * A batch of jobs over more than one scene, a job with more than 3 directional lights, and trt_render_reads run as
- * the megakernel whatever the setting. No reference counterpart: the reference's offline pass is one megakernel
- * dispatch (pathtrace.wgsl:4106 `main`). */
+ * The wavefront takes a batch one scene at a time (AUTO: when every scene has 1,024 triangles or more). A batch with a
+ * job of more than 3 directional lights, and trt_render_reads, run as the megakernel whatever the setting. No
+ * reference counterpart: the reference's offline pass is one megakernel dispatch (pathtrace.wgsl:4106 `main`). */
 #define TRT_RENDER_AUTO 0u
 #define TRT_RENDER_MEGAKERNEL 1u
 #define TRT_RENDER_WAVEFRONT 2u

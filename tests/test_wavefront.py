@@ -147,8 +147,9 @@ def test_a_frame_larger_than_a_chunk(device):
 
 @pytest.mark.gpu
 def test_batches(device):
-    """trt_render_batch under the wavefront: three jobs of one scene (two sizes, sample counts and seeds) equal each
-    job rendered alone by the megakernel; a batch over two scenes runs as the megakernel and equals it too."""
+    """trt_render_batch under the wavefront: three jobs of one scene (two sizes, sample counts and seeds), then the
+    same jobs with one of a second scene between them (the chunks take one scene at a time), each equal to its job
+    rendered alone by the megakernel."""
     from threert_amd import serve
 
     print("wavefront: batches", flush=True)
@@ -156,12 +157,13 @@ def test_batches(device):
     rs2, cam2 = lit_floor()
     sc, sc2 = R.upload(device, rs), R.upload(device, rs2)
     specs = [(sc, rs.params(64, 48, cam, seed=1), 0, 3), (sc, rs.params(130, 70, cam, seed=9), 5, 2),
-             (sc, rs.params(33, 17, cam, seed=4, max_bounces=6, rr_depth=2), 1, 7)]
+             (sc, rs.params(33, 17, cam, seed=4, max_bounces=6, rr_depth=2), 1, 7),
+             (sc2, rs2.params(40, 30, cam2, seed=2, max_bounces=3), 0, 2)]
     alone = [render_as(device, T.RENDER_MEGAKERNEL, s, p, f, n) for s, p, f, n in specs]
-    for how, jobs in ((T.RENDER_WAVEFRONT, specs),
-                      (T.RENDER_WAVEFRONT, specs + [(sc2, rs2.params(40, 30, cam2, seed=2, max_bounces=3), 0, 2)])):
+    for order in ((0, 1, 2), (0, 3, 1, 2)):
+        jobs = [specs[k] for k in order]
         accs = [R.Accum(device, int(p[16]), int(p[17])) for _, p, _, _ in jobs]
-        device.set_render(how)
+        device.set_render(T.RENDER_WAVEFRONT)
         try:
             serve.render_batch(device, [(s, p, f, n, a) for (s, p, f, n), a in zip(jobs, accs)])
             got = [a.read() for a in accs]
@@ -169,11 +171,8 @@ def test_batches(device):
             device.set_render(T.RENDER_AUTO)
         for a in accs:
             a.close()
-        for k in range(len(specs)):
-            assert equal(got[k], alone[k]) == 1.0, k
-        if len(jobs) > len(specs):
-            s, p, f, n = jobs[-1]
-            assert equal(got[-1], render_as(device, T.RENDER_MEGAKERNEL, s, p, f, n)) == 1.0
+        for k, g in zip(order, got):
+            assert equal(g, alone[k]) == 1.0, (order, k)
     sc.close()
     sc2.close()
 

@@ -1099,7 +1099,8 @@ static int job_args(const trt_render_job &j, trt::RenderArgs &a, std::string *wh
 }
 
 // ---- the wavefront path tracer's plan ------------------------------------------------------------------------------
-// A batch's paths (pixel x sample) go through the wavefront in chunks of at most kWfMaxPaths. A job whose image fits
+// A batch's paths (pixel x sample) go through the wavefront in chunks of at most kWfMaxPaths, the jobs of one scene
+// after another (a chunk holds one scene's jobs: its walk launches take that scene's trees). A job whose image fits
 // a chunk goes in whole rounds (every pixel, several samples); a larger one in pixel ranges of one sample. A chunk
 // never holds one pixel twice, and a pixel's samples come in sample order, so that the accumulation adds them as the
 // megakernel does.
@@ -1114,23 +1115,39 @@ constexpr uint64_t kWfCountBytes = 3 * 2 * trt::kShardBytes;               // th
 constexpr uint64_t kWfStrideStep = uint64_t{trt::kShards} * 64;          // a segment holds whole waves of paths
 struct WfChunk {
     uint32_t sub_first, nsubs, npaths, pixels;
+    uint32_t job;   // a job of the chunk (table index): the scene whose trees its walks take
+    uint32_t last;  // the last bounce of its jobs
 };
 
-static void wf_plan(const std::vector<trt::RenderJob> &table, uint64_t stride, std::vector<trt::WfSub> &subs,
-                    std::vector<WfChunk> &chunks) {
-    WfChunk cur{0, 0, 0, 0};
+// scene_of[j]: table entry j's scene
+static void wf_plan(const std::vector<trt::RenderJob> &table, const std::vector<const trt_scene *> &scene_of,
+                    uint64_t stride, std::vector<trt::WfSub> &subs, std::vector<WfChunk> &chunks) {
+    WfChunk cur{0, 0, 0, 0, 0, 0};
     auto close = [&]() {
         if (cur.npaths) chunks.push_back(cur);
-        cur = WfChunk{static_cast<uint32_t>(subs.size()), 0, 0, 0};
+        cur = WfChunk{static_cast<uint32_t>(subs.size()), 0, 0, 0, 0, 0};
     };
     auto add = [&](uint32_t job, uint32_t pixel_first, uint32_t count, uint32_t sample_first, uint32_t rounds,
                    uint32_t width) {
         subs.push_back(trt::WfSub{job, pixel_first, count, sample_first, rounds, cur.npaths, cur.pixels, width});
+        if (cur.nsubs == 0) cur.job = job;
+        cur.last = std::max(cur.last, table[job].args.params[22]);  // P.counts.z
         cur.nsubs++;
         cur.npaths += count * rounds;
         cur.pixels += count;
     };
-    for (uint32_t j = 0; j < table.size(); j++) {
+    // the jobs grouped by scene, in the order the scenes first appear
+    std::vector<uint32_t> order(table.size());
+    for (uint32_t j = 0; j < table.size(); j++) order[j] = j;
+    std::vector<const trt_scene *> firsts;
+    for (const trt_scene *sc : scene_of)
+        if (std::find(firsts.begin(), firsts.end(), sc) == firsts.end()) firsts.push_back(sc);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+        return std::find(firsts.begin(), firsts.end(), scene_of[x]) < std::find(firsts.begin(), firsts.end(), scene_of[y]);
+    });
+    for (uint32_t k = 0; k < order.size(); k++) {
+        const uint32_t j = order[k];
+        if (k > 0 && scene_of[order[k - 1]] != scene_of[j]) close();  // a new scene: a new chunk
         const trt::RenderArgs &a = table[j].args;
         const uint32_t w = a.params[16], spp = a.spp;
         const uint64_t npix = uint64_t(a.params[16]) * a.params[17];
@@ -1205,19 +1222,20 @@ static int ensure_wf(trt_device *d, uint64_t stride) {
 // The wavefront's launches for a batch whose jobs share one scene: per chunk, the camera rays, then per bounce the
 // walk of the queued rays and the shade kernel, then the accumulation. `last`: the jobs' largest last bounce.
 static int wf_launch(trt_device *d, const trt::RenderJob *d_jobs, const trt::WfSub *d_subs,
-                     const std::vector<WfChunk> &chunks, uint64_t stride, const trt::TraceArgs &geo, uint32_t last,
+                     const std::vector<WfChunk> &chunks, uint64_t stride, const std::vector<trt::RenderJob> &table,
                      bool coat, bool tree) {
     trt::WfArgs w = wf_layout(d->wf, stride);
     w.jobs = d_jobs;
     w.guard = d->guard;
     w.rays_traced = d->ray_counts;
-    trt::TraceArgs t = geo;
-    t.rays = w.rays;
-    t.hits = w.hits;
-    t.n = trt::kWfSlots * stride;
-    t.seg_cap = trt::kWfSlots * w.cap;
     const int trace_grid = d->cus * d->blocks_wide;
     for (const WfChunk &c : chunks) {
+        trt::TraceArgs t = table[c.job].args.geo;  // the chunk's scene
+        t.rays = w.rays;
+        t.hits = w.hits;
+        t.n = trt::kWfSlots * stride;
+        t.seg_cap = trt::kWfSlots * w.cap;
+        const uint32_t last = c.last;
         w.subs = d_subs + c.sub_first;
         w.nsubs = c.nsubs;
         w.npaths = c.npaths;
@@ -1271,30 +1289,28 @@ static int render_batch(trt_device *d, const trt_render_job *jobs, uint32_t n, u
     }
     if (tiles >= (uint64_t{1} << 31)) return fail(TRT_E_INVALID, "trt_render_batch: too many tiles in one launch");
     if (table.empty()) return ok();
-    // The wavefront takes batches of one scene (its walk launches take one scene's trees) whose jobs have at most
-    // kWfMaxSuns directional lights; counting launches stay on the megakernel. TRT_RENDER_AUTO takes it for scenes
-    // of kWfAutoLeaves triangles or more and for launches of a whole chunk of paths or more.
+    // The wavefront takes batches whose jobs have at most kWfMaxSuns directional lights; counting launches stay on
+    // the megakernel. TRT_RENDER_AUTO takes it when every scene has kWfAutoLeaves triangles or more, or for launches
+    // of a whole chunk of paths or more.
     const uint32_t how = d->render.load();
     bool wf = reads == nullptr && how != TRT_RENDER_MEGAKERNEL;
     uint64_t paths = 0;
-    uint32_t last = 0;
-    const trt_scene *first_scene = nullptr;
-    for (uint32_t k = 0; k < n && wf; k++) {
+    bool large = true;
+    std::vector<const trt_scene *> scene_of;
+    for (uint32_t k = 0; k < n; k++) {
         if (jobs[k].spp == 0) continue;
-        if (!first_scene) first_scene = jobs[k].scene;
-        wf = jobs[k].scene == first_scene;
+        scene_of.push_back(jobs[k].scene);
+        large = large && jobs[k].scene->leaf_count >= kWfAutoLeaves;
     }
-    for (const auto &j : table) paths += uint64_t(j.args.params[16]) * j.args.params[17] * j.args.spp;
-    if (wf && how == TRT_RENDER_AUTO)
-        wf = first_scene && (first_scene->leaf_count >= kWfAutoLeaves || paths >= kWfMaxPaths);
     for (const auto &j : table) {
+        paths += uint64_t(j.args.params[16]) * j.args.params[17] * j.args.spp;
         wf = wf && j.args.params[36] <= trt::kWfMaxSuns;  // P.counts2.x: the directional lights
-        last = std::max(last, j.args.params[22]);  // P.counts.z: the last bounce
     }
+    if (wf && how == TRT_RENDER_AUTO) wf = large || paths >= kWfMaxPaths;
     std::vector<trt::WfSub> subs;
     std::vector<WfChunk> chunks;
     const uint64_t stride = (std::min(paths, kWfMaxPaths) + kWfStrideStep - 1) / kWfStrideStep * kWfStrideStep;
-    if (wf) wf_plan(table, stride, subs, chunks);
+    if (wf) wf_plan(table, scene_of, stride, subs, chunks);
     std::lock_guard<std::mutex> lock(d->mu);
     TRT_HIP(hipSetDevice(d->ordinal));
     const int grid = wf ? d->cus * d->blocks_wide : d->cus * d->blocks_render;
@@ -1339,7 +1355,7 @@ static int render_batch(trt_device *d, const trt_render_job *jobs, uint32_t n, u
     if (wf) {
         const auto *d_jobs = static_cast<const trt::RenderJob *>(ring.dev[k]);
         const auto *d_subs = reinterpret_cast<const trt::WfSub *>(static_cast<const char *>(ring.dev[k]) + job_bytes);
-        rc = wf_launch(d, d_jobs, d_subs, chunks, stride, table[0].args.geo, last, coat, tree);
+        rc = wf_launch(d, d_jobs, d_subs, chunks, stride, table, coat, tree);
         if (rc) return rc;
     } else if (trt::launch_render(static_cast<const trt::RenderJob *>(ring.dev[k]), static_cast<uint32_t>(table.size()),
                            static_cast<uint32_t>(tiles), d->counter, d->spill, threads,
