@@ -1036,7 +1036,7 @@ __global__ __launch_bounds__(kRenderBlock) __attribute__((amdgpu_waves_per_eu(3)
         uint32_t req = 0;
         if (lane == 0) req = atomicAdd(sh.counter(shard), 1u);
         uint32_t tile, t1;
-        if (!shard_take(sh, shard, closed, req, 1u, lane, tile, t1)) break;
+        if (!shard_take<false>(sh, shard, closed, req, 1u, lane, tile, t1)) break;
         uint32_t lo = 0, hi = njobs;
         while (hi - lo > 1) {
             const uint32_t mid = (lo + hi) / 2;

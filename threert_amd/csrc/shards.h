@@ -27,8 +27,9 @@ struct ShardSet {
     uint32_t *ctr;
     const uint32_t *seg_counts;
     uint32_t n, seg_cap;
+    template <bool kSeg>  // the segments (else [0, n) split evenly)
     __device__ __forceinline__ void range(uint32_t s, uint32_t &lo, uint32_t &hi) const {
-        if (seg_counts) {
+        if (kSeg) {
             lo = s * seg_cap;
             hi = lo + seg_counts[s * kShardGap];
         } else {
@@ -39,16 +40,17 @@ struct ShardSet {
     __device__ __forceinline__ uint32_t *counter(uint32_t s) const { return ctr + s * kShardGap; }
 };
 
-// Called by the whole wave (its control flow uniform): lane 0's `req` is what its atomicAdd of `chunk` on shard s
-// returned; the items [a0, a1) it names, at most `chunk` and at least one. When shard s is spent the wave moves on to
-// the next open shard (s changes; closed gathers the shards it found spent) and takes a chunk there. False when every
-// shard is spent.
+// Called by the whole wave, every lane active: lane 0's `req` is what its atomicAdd of `chunk` on shard s returned;
+// the items [a0, a1) it names, at most `chunk` and at least one. When shard s is spent the wave moves on to the next
+// open shard (s changes; closed gathers the shards it found spent) and takes a chunk there. False when every shard is
+// spent. kSeg: the shards are ShardSet's segments.
+template <bool kSeg>
 __device__ __forceinline__ bool shard_take(const ShardSet &sh, uint32_t &s, uint32_t &closed, uint32_t req,
                                            uint32_t chunk, uint32_t lane, uint32_t &a0, uint32_t &a1) {
-    uint32_t c = __builtin_amdgcn_readfirstlane(__shfl(req, 0, 64));
+    uint32_t c = __builtin_amdgcn_readfirstlane(req);  // lane 0's: every lane is active
     for (;;) {
         uint32_t lo, hi;
-        sh.range(s, lo, hi);
+        sh.range<kSeg>(s, lo, hi);
         lo = __builtin_amdgcn_readfirstlane(lo);
         hi = __builtin_amdgcn_readfirstlane(hi);
         if (c < hi - lo) {
@@ -60,7 +62,7 @@ __device__ __forceinline__ bool shard_take(const ShardSet &sh, uint32_t &s, uint
         bool open = false;
         if (lane < kShards && !((closed >> lane) & 1u)) {
             uint32_t ql, qh;
-            sh.range(lane, ql, qh);
+            sh.range<kSeg>(lane, ql, qh);
             const uint32_t v = __hip_atomic_load(sh.counter(lane), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             open = v < qh - ql;
         }
@@ -74,7 +76,7 @@ __device__ __forceinline__ bool shard_take(const ShardSet &sh, uint32_t &s, uint
         s = __builtin_amdgcn_readfirstlane(t);
         uint32_t v = 0;
         if (lane == 0) v = atomicAdd(sh.counter(s), chunk);
-        c = __builtin_amdgcn_readfirstlane(__shfl(v, 0, 64));
+        c = __builtin_amdgcn_readfirstlane(v);
     }
 }
 

@@ -35,7 +35,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
         uint32_t req = 0;
         if (lane == 0) req = atomicAdd(sh.counter(s), 64u);
         uint32_t a0, a1;
-        if (!shard_take(sh, s, closed, req, 64u, lane, a0, a1)) break;
+        if (!shard_take<false>(sh, s, closed, req, 64u, lane, a0, a1)) break;
         const uint32_t k = a0 + lane;
         if (k < a1) {
             const uint64_t i = a.ray_index ? a.ray_index[k] : k;
@@ -58,11 +58,13 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
 constexpr uint32_t kRefill = 24;
 constexpr uint32_t kChunk = 64;  // at least a wave: one refill switches chunks at most once
 
-// kShard: the rays come from the shards' counters (the wavefront's queue segments, shards.h); else from one counter,
-// [0, n), which keeps two fewer scalars and a few registers live (five waves a SIMD instead of four: 9 % faster on
-// the 3M-triangle batch, whose 16.8M rays take a quarter of one counter's atomic rate).
+// kShard: the rays come from the shards' counters (the wavefront's queue segments, shards.h) through the queues' ray
+// slot lists; else from one counter, [0, n), which keeps two fewer scalars and a few registers live (93 registers
+// without a hint; the sharded form needed 102, four waves a SIMD, 9 % slower on the 3M-triangle batch, whose 16.8M
+// rays take a third of one counter's atomic rate). The sharded form is its own kernel, k_trace_queue, held to five
+// waves a SIMD (94 registers, no spills).
 template <bool kCount, bool kWatertight, bool kShard>
-__global__ __launch_bounds__(kTraceBlock) void k_trace_dyn(TraceArgs a) {
+__device__ __forceinline__ void trace_dyn(const TraceArgs &a) {
     __shared__ uint32_t lds[kLdsStack * kTraceBlock];
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -84,7 +86,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_dyn(TraceArgs a) {
     bool more = kShard;  // one counter: a lane past n is past every ray
     if (kShard) {
         if (lane == 0) req = atomicAdd(sh.counter(s), kChunk);
-        more = shard_take(sh, s, closed, req, kChunk, lane, res, res_end);
+        more = shard_take<true>(sh, s, closed, req, kChunk, lane, res, res_end);
         if (more && lane == 0) req = atomicAdd(sh.counter(s), kChunk);
     } else {
         if (lane == 0) req = atomicAdd(a.counter, 2 * kChunk);
@@ -118,7 +120,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_dyn(TraceArgs a) {
             } else {
                 res = res_end;
                 uint32_t b0 = 0, b1 = 0;
-                if (more && shard_take(sh, s, closed, req, kChunk, lane, b0, b1)) {
+                if (more && shard_take<true>(sh, s, closed, req, kChunk, lane, b0, b1)) {
                     // the lanes past the old chunk take from the new one (shorter at a shard's end: the rest wait)
                     const uint32_t extra = need - avail < b1 - b0 ? need - avail : b1 - b0;
                     if (!got && rank - avail < extra) {
@@ -134,7 +136,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_dyn(TraceArgs a) {
             }
             if (!active && !exhausted) {
                 if (got) {
-                    ray = a.ray_index ? a.ray_index[i] : i;
+                    ray = kShard ? a.ray_index[i] : i;  // the queues name ray slots
                     const float4 r0 = a.rays[2 * uint64_t(ray)], r1 = a.rays[2 * uint64_t(ray) + 1];
                     if (wide_begin(a, F3{r0.x, r0.y, r0.z}, F3{r1.x, r1.y, r1.z}, r0.w, __float_as_uint(r1.w), w, st))
                         active = true;
@@ -183,6 +185,14 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_dyn(TraceArgs a) {
     }
 }
 
+template <bool kCount, bool kWatertight>
+__global__ __launch_bounds__(kTraceBlock) void k_trace_dyn(TraceArgs a) {
+    trace_dyn<kCount, kWatertight, false>(a);
+}
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(5))) void k_trace_queue(TraceArgs a) {
+    trace_dyn<false, false, true>(a);
+}
+
 __global__ void k_reset_counter(uint32_t *c) { c[threadIdx.x * kShardGap] = 0; }
 
 }  // namespace
@@ -216,7 +226,7 @@ int launch_shards_reset(uint32_t *ctr, hipStream_t s) {
 
 int trace_occupancy(bool wide, int *blocks_per_cu) {
     int b = 0;
-    hipError_t e = wide ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_trace_dyn<false, false, false>, kTraceBlock, 0)
+    hipError_t e = wide ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_trace_dyn<false, false>, kTraceBlock, 0)
                         : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_trace<false, false>, kTraceBlock, 0);
     *blocks_per_cu = b;
     return e == hipSuccess ? 0 : -1;
@@ -225,15 +235,15 @@ int trace_occupancy(bool wide, int *blocks_per_cu) {
 int launch_trace(const TraceArgs &a, bool wide, bool count, int grid, hipStream_t s, bool watertight) {
     hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(kShards), 0, s, a.counter);
     if (wide && a.seg_counts)  // the wavefront's queues (no counting or watertight variant)
-        hipLaunchKernelGGL((k_trace_dyn<false, false, true>), dim3(grid), dim3(kTraceBlock), 0, s, a);
+        hipLaunchKernelGGL(k_trace_queue, dim3(grid), dim3(kTraceBlock), 0, s, a);
     else if (wide && watertight && count)
-        hipLaunchKernelGGL((k_trace_dyn<true, true, false>), dim3(grid), dim3(kTraceBlock), 0, s, a);
+        hipLaunchKernelGGL((k_trace_dyn<true, true>), dim3(grid), dim3(kTraceBlock), 0, s, a);
     else if (wide && watertight)
-        hipLaunchKernelGGL((k_trace_dyn<false, true, false>), dim3(grid), dim3(kTraceBlock), 0, s, a);
+        hipLaunchKernelGGL((k_trace_dyn<false, true>), dim3(grid), dim3(kTraceBlock), 0, s, a);
     else if (wide && count)
-        hipLaunchKernelGGL((k_trace_dyn<true, false, false>), dim3(grid), dim3(kTraceBlock), 0, s, a);
+        hipLaunchKernelGGL((k_trace_dyn<true, false>), dim3(grid), dim3(kTraceBlock), 0, s, a);
     else if (wide)
-        hipLaunchKernelGGL((k_trace_dyn<false, false, false>), dim3(grid), dim3(kTraceBlock), 0, s, a);
+        hipLaunchKernelGGL((k_trace_dyn<false, false>), dim3(grid), dim3(kTraceBlock), 0, s, a);
     else if (count)
         hipLaunchKernelGGL((k_trace<false, true>), dim3(grid), dim3(kTraceBlock), 0, s, a);
     else
