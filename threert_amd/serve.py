@@ -105,19 +105,36 @@ class _Resident:
 class SceneStore:
     """Scenes resident in HBM by key, least recently used first out. A worker pool builds each scene on the host and
     uploads it (the library builds its wide trees during the upload, seconds for millions of triangles), so the
-    serving loop never waits for a build; the budget counts a scene from its first job's admission."""
+    serving loop seldom waits for a build. A worker uploads only while the upload fits what `headroom()` says is
+    left of the budget (the bytes it reserves count at once, `pending`); past that it leaves the upload to the
+    scene's first admission, which makes room first."""
 
-    def __init__(self, dev: Device, make_scene, workers: int = 4):
+    def __init__(self, dev: Device, make_scene, workers: int = 4, headroom=None):
         self.dev = dev
         self.make_scene = make_scene
+        self.headroom = headroom  # bytes of the budget not yet taken by admitted jobs (None: no limit)
         self.resident: "OrderedDict[tuple, _Resident]" = OrderedDict()
         self.host: dict = {}
+        self.pending = 0  # bytes of scenes uploaded by the workers (or being uploaded) and not yet resident
         self.pool = ThreadPoolExecutor(max_workers=workers)
         self.lock = threading.Lock()
 
     def _build(self, key):
+        """[rs, cam, sky, the upload or None, the bytes it holds in `pending`]"""
         rs, cam, sky = self.make_scene(*key)
-        return [rs, cam, sky, R.upload(self.dev, rs)]
+        b = scene_bytes(rs)
+        with self.lock:
+            ok = self.headroom is None or self.pending + b <= self.headroom()
+            if ok:
+                self.pending += b
+        if not ok:
+            return [rs, cam, sky, None, 0]
+        try:
+            return [rs, cam, sky, R.upload(self.dev, rs), b]
+        except Exception:
+            with self.lock:
+                self.pending -= b
+            raise
 
     def prefetch(self, key):
         with self.lock:
@@ -133,20 +150,49 @@ class SceneStore:
         rs = self.host[key].result()[0]
         return scene_bytes(rs)
 
+    def pending_except(self, key) -> int:
+        """`pending` without the bytes `key`'s own upload holds there."""
+        with self.lock:
+            own = 0
+            f = self.host.get(key)
+            if f is not None and f.done() and f.exception() is None and key not in self.resident:
+                own = f.result()[4]
+            return self.pending - own
+
     def acquire(self, key) -> _Resident:
         if key in self.resident:
             e = self.resident.pop(key)
             self.resident[key] = e
         else:
             built = self.host[key].result()
-            rs, cam, sky, sc = built
-            if sc is None:  # evicted since: upload again
+            rs, cam, sky, sc = built[:4]
+            if sc is None:  # never uploaded, or dropped since: upload now
                 sc = R.upload(self.dev, rs)
-            built[3] = None  # the upload now belongs to the resident entry
+            with self.lock:
+                self.pending -= built[4]
+                built[3], built[4] = None, 0  # the upload now belongs to the resident entry
             e = _Resident(sc, rs, cam, sky, scene_bytes(rs))
             self.resident[key] = e
         e.users += 1
         return e
+
+    def drop_uploads(self, need: int, keep) -> int:
+        """Frees workers' uploads not yet resident (other than `keep`'s) until `need` bytes are free; their scenes
+        are uploaded again at admission. Returns the bytes freed."""
+        freed = 0
+        with self.lock:
+            for key, f in self.host.items():
+                if freed >= need:
+                    break
+                if key == keep or key in self.resident or not f.done() or f.exception() is not None:
+                    continue
+                built = f.result()
+                if built[3] is not None:
+                    built[3].close()
+                    freed += built[4]
+                    self.pending -= built[4]
+                    built[3], built[4] = None, 0
+        return freed
 
     def release(self, e: _Resident):
         e.users -= 1
@@ -176,6 +222,7 @@ class SceneStore:
             if f.done() and f.exception() is None and f.result()[3] is not None:
                 f.result()[3].close()  # built and uploaded, never used
         self.host.clear()
+        self.pending = 0
 
 
 ACCUM_BYTES = 80  # records.wgsl Pixel: five vec4 per pixel
@@ -216,8 +263,9 @@ class Server:
             raise ValueError(f"unknown policy {policy!r}")
         self.policy = policy
         self._dn = None  # device buffers of the denoising step
-        self.store = SceneStore(dev, make_scene, workers)
         self.budget = budget_bytes
+        self.used = 0  # bytes of admitted jobs' accumulators and of resident scenes (kept across serve calls, as they are)
+        self.store = SceneStore(dev, make_scene, workers, headroom=lambda: self.budget - self.used)
         self.launch_samples = launch_samples  # path samples one launch aims at (all jobs together)
         self.keep_images = keep_images
         self.max_jobs = max_jobs
@@ -295,7 +343,6 @@ class Server:
         `every` seconds. Returns when the source is closed and every job has finished."""
         waiting: deque[Job] = deque()
         active: list[_Active] = []
-        used = 0
         self.t0 = time.perf_counter()
         now = lambda: time.perf_counter() - self.t0  # noqa: E731
         told = 0.0
@@ -322,21 +369,46 @@ class Server:
                     waiting.remove(j)
                     emit(Result(j.id, j.arrival, now(), now(), 0, None, False, f"{type(e).__name__}: {e}"))
                     continue
-                if used + need > self.budget:
-                    used -= self.store.evict_for(need, used, self.budget)
-                    if used + need > self.budget:
+                # the workers' uploads of other scenes count against the budget too
+                pending = self.store.pending_except(j.scene)
+                if self.used + pending + need > self.budget:
+                    self.used -= self.store.evict_for(need + pending, self.used, self.budget)
+                    if self.used + pending + need > self.budget:
+                        pending -= self.store.drop_uploads(self.used + pending + need - self.budget, j.scene)
+                    if self.used + pending + need > self.budget:
                         break
                 waiting.remove(j)
-                e = self.store.acquire(j.scene)
-                used += need
-                acc = R.Accum(self.dev, j.width, j.height)
+                accum = j.width * j.height * ACCUM_BYTES
+                try:  # an upload here (a scene dropped or never uploaded) may fail: the job fails
+                    e = self.store.acquire(j.scene)
+                except Exception as ex:
+                    emit(Result(j.id, j.arrival, now(), now(), 0, None, False, f"{type(ex).__name__}: {ex}"))
+                    continue
+                self.used += need - accum  # the scene, resident now if it was not
+                try:
+                    acc = R.Accum(self.dev, j.width, j.height)
+                except Exception as ex:
+                    self.store.release(e)
+                    emit(Result(j.id, j.arrival, now(), now(), 0, None, False, f"{type(ex).__name__}: {ex}"))
+                    continue
+                self.used += accum
                 active.append(_Active(j, e, acc, params_of(e, j), 0, now()))
             if not active:
                 idle(now())
                 continue
             ks = self._slices(active, now())
-            render_batch(self.dev, [(a.entry.scene, a.params, a.done, k, a.acc) for a, k in zip(active, ks)])
-            self.dev.sync()
+            try:
+                render_batch(self.dev, [(a.entry.scene, a.params, a.done, k, a.acc) for a, k in zip(active, ks)])
+                self.dev.sync()
+            except Exception as ex:  # the launch failed (or a walk stopped and reported): its jobs fail, serving goes on
+                for a in active:
+                    emit(Result(a.job.id, a.job.arrival, a.admitted, now(), 0, None, False,
+                                f"{type(ex).__name__}: {ex}"))
+                    a.acc.close()
+                    self.used -= a.job.width * a.job.height * ACCUM_BYTES
+                    self.store.release(a.entry)
+                active = []
+                continue
             self.launches += 1
             still = []
             for a, k in zip(active, ks):
@@ -352,7 +424,7 @@ class Server:
                     emit(Result(a.job.id, a.job.arrival, a.admitted, now(), a.job.spp * a.job.width * a.job.height,
                                 img if keep else None, a.job.denoise))
                     a.acc.close()
-                    used -= a.job.width * a.job.height * ACCUM_BYTES
+                    self.used -= a.job.width * a.job.height * ACCUM_BYTES
                     self.store.release(a.entry)
                 else:
                     still.append(a)
