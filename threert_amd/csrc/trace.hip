@@ -154,11 +154,14 @@ __device__ __forceinline__ void trace_dyn(const TraceArgs &a) {
         }
         // Node phase: until a kLeafShareBatch-th of the walking lanes wait, no lane has a node left, or kRefill
         // lanes have finished.
+        // (Two ballots a step: the lanes at a node and the lanes walking; the waiting and the idle lanes follow.)
         const uint32_t walking = __popcll(busy);
+        const uint64_t live = __ballot(!exhausted);
         for (;;) {
             const bool at_node = active && wide_at_node(w);
-            if (__ballot(at_node) == 0 || kLeafShareBatch * __popcll(__ballot(active && wide_waits(w))) >= walking ||
-                __popcll(__ballot(!active && !exhausted)) >= kRefill)
+            const uint64_t m_node = __ballot(at_node), m_walk = __ballot(active);
+            if (m_node == 0 || kLeafShareBatch * __popcll(m_walk & ~m_node) >= walking ||
+                __popcll(live & ~m_walk) >= kRefill)
                 break;
             if (at_node) {
                 wide_node<kCount>(a, w, st, flags, cnt);

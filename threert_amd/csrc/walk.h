@@ -358,7 +358,7 @@ __device__ __forceinline__ float qbyte(uint32_t w, int c) { return float((w >> (
 // a lone walk's.
 constexpr uint32_t kWalkDone = 0xffffffffu;   // the walk is over
 constexpr uint32_t kWalkLeave = 0xfffffffeu;  // the stack is back at the unit's entry depth: leave the unit, then pop
-constexpr uint32_t kLeafShareBatch = 3;  // ray batches (trace.hip): the other phase once a third of the lanes wait
+constexpr uint32_t kLeafShareBatch = 2;  // ray batches (trace.hip): the other phase once half of the lanes wait
 constexpr uint32_t kLeafShareCall = 2;   // trace_wide (the path tracer's rays): once half of them wait
 // (measured on the path tracer: 1, serving lanes as soon as they all wait, took the 720p city frame from 1.94 to
 // 1.90 ms but the 3M-triangle 1080p image from 1.36 to 2.14 s; 3 made the city 2.02 ms and the large image no
@@ -462,32 +462,43 @@ __device__ __forceinline__ void wide_node(const TraceArgs &a, WideWalk &w, Stack
         TRT_CSWAP(1, 3);
         TRT_CSWAP(1, 2);
 #undef TRT_CSWAP
-        if (key[0] != kNone) {
-            // the children to queue are ref[1..np]; they go on the stack farthest first, so the second nearest is
-            // on top
+        if (st.sp + 3 <= kLdsStack) {
+            // The common case, with no branch: the three push slots and the pop slot are all in LDS, so the lane
+            // writes its three candidates (counting only the queued ones) and reads the top entry whether it then
+            // moves to its nearest child or pops. (As two branches, the lanes that push and the lanes that pop ran
+            // one after the other: 2.5 % of the 3M-triangle batch's time.)
+            const bool any = key[0] != kNone;
             const uint32_t np = uint32_t(key[1] != kNone) + uint32_t(key[2] != kNone) + uint32_t(key[3] != kNone);
-            if (st.sp + 3 <= kLdsStack) {
-                // all three slots in LDS: store three entries unconditionally, count only np of them
-                uint32_t s0 = ref[1];
-                s0 = np == 2 ? ref[2] : s0;
-                s0 = np == 3 ? ref[3] : s0;
-                const uint32_t s1 = np == 3 ? ref[2] : ref[1];
-                TRT_LDS uint32_t *top = st.lds + st.sp * kTraceBlock;
-                top[0] = s0;
-                top[kTraceBlock] = s1;
-                top[2 * kTraceBlock] = ref[1];
-                st.sp += np;
-            } else {
-                if (st.sp + np > st.cap) {
-                    flags |= kFlagWalk;
-                    report_range(a.guard_flag, 8, st.sp + np, st.cap);
-                    w.node = kWalkDone;
-                    return;
-                }
-                if (key[3] != kNone) st.push(ref[3]);
-                if (key[2] != kNone) st.push(ref[2]);
-                if (key[1] != kNone) st.push(ref[1]);
+            uint32_t s0 = ref[1];
+            s0 = np == 2 ? ref[2] : s0;
+            s0 = np == 3 ? ref[3] : s0;
+            const uint32_t s1 = np == 3 ? ref[2] : ref[1];
+            const uint32_t sp = st.sp;
+            TRT_LDS uint32_t *top = st.lds + sp * kTraceBlock;
+            const uint32_t popped = st.lds[(sp > 0 ? sp - 1 : 0) * kTraceBlock];
+            top[0] = s0;
+            top[kTraceBlock] = s1;
+            top[2 * kTraceBlock] = ref[1];
+            // the pop (wide_pop): leave the unit at its entry depth, end at an empty stack, else the top entry
+            const bool leave = w.rec != kNone && sp == w.base;
+            const uint32_t next = leave ? kWalkLeave : (sp == 0 ? kWalkDone : popped);
+            w.node = any ? ref[0] : next;
+            st.sp = any ? sp + np : ((leave || sp == 0) ? sp : sp - 1);
+            return;
+        }
+        if (key[0] != kNone) {
+            // near the LDS part's end: the children to queue are ref[1..np]; they go on the stack farthest first, so
+            // the second nearest is on top
+            const uint32_t np = uint32_t(key[1] != kNone) + uint32_t(key[2] != kNone) + uint32_t(key[3] != kNone);
+            if (st.sp + np > st.cap) {
+                flags |= kFlagWalk;
+                report_range(a.guard_flag, 8, st.sp + np, st.cap);
+                w.node = kWalkDone;
+                return;
             }
+            if (key[3] != kNone) st.push(ref[3]);
+            if (key[2] != kNone) st.push(ref[2]);
+            if (key[1] != kNone) st.push(ref[1]);
             w.node = ref[0];
             return;
         }
@@ -507,8 +518,12 @@ __device__ __forceinline__ void wide_other(const TraceArgs &a, WideWalk &w, Stac
     }
     if (node == kWalkLeave) {
         w.rec = kNone;
+        if (st.sp == 0) {  // nothing left to walk: the world ray is not needed again
+            w.node = kWalkDone;
+            return;
+        }
         w.ro = w.o, w.rd = w.d, w.rinv = inv3(w.d);  // the same bits as at the start: kept no register for it
-        w.node = st.sp == 0 ? kWalkDone : st.pop();
+        w.node = st.pop();
         return;
     }
     if (ref_kind(node) == kRefUnit) {
