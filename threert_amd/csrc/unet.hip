@@ -107,6 +107,106 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a, uint32_t tiles_x, uint
     }
 }
 
+// k_conv for cin = 32 * CH (CH = 1 .. 7, not 6) with every size fixed at compile time: the halo's loads all issued
+// before its stores, its indices and the operands' LDS offsets constants, the outputs rounded to bf16 by the
+// hardware's v_cvt_pk_bf16_f32 (round to nearest even, as to_bf16). k_conv spent its time in index arithmetic and
+// software rounding (the full-resolution layers were VALU-bound: 743 VALU instructions a wave against 36 MFMAs).
+// Per output the same MFMAs in the same order, so the same bits. Inference's last layer writes k_output's colour
+// instead of its 32 FP32 channels (ConvArgs out_color).
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) {
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2_t{lo, hi}, bf16x2_t));
+}
+
+template <int NCO, int TH, int CH>
+__global__ __launch_bounds__(256) void k_conv_c(ConvArgs a, uint32_t tiles_x, uint32_t tiles_y) {
+    static_assert(TH == 8 || TH == 16, "tiles are 8 or 16 pixels tall");
+    constexpr int S = TH * NCO / 8;  // 16-pixel strips (two tile rows each) a wave takes
+    constexpr uint32_t cin = CH * 32, stride = cin + 8, C4 = CH * 4;
+    constexpr uint32_t items = (TH + 2) * 10 * C4, per = (items + 255) / 256;
+    extern __shared__ uint4 smem[];
+    uint16_t *halo = reinterpret_cast<uint16_t *>(smem);
+    const uint32_t tile = blockIdx.x;
+    const uint32_t tx = tile % tiles_x, ty = (tile / tiles_x) % tiles_y, n = tile / (tiles_x * tiles_y);
+    const int x0 = int(tx * 8) - 1, y0 = int(ty * TH) - 1;
+    const uint32_t h2 = a.h / 2, w2 = a.w / 2, c0q = a.c0 / 8;
+    const uint16_t *in0 = a.in0 + uint64_t(n) * a.h * a.w * a.c0;
+    const uint16_t *in1 = a.in1 ? a.in1 + uint64_t(n) * h2 * w2 * a.c1 : nullptr;
+    uint4 v[per];
+#pragma unroll
+    for (uint32_t k = 0; k < per; k++) {
+        const uint32_t q = threadIdx.x + k * 256;
+        const uint32_t hp = q / C4, ck = q % C4;
+        const uint32_t hy = hp / 10, hx = hp - hy * 10;
+        const int gy = y0 + int(hy), gx = x0 + int(hx);
+        v[k] = make_uint4(0, 0, 0, 0);
+        if (((k + 1) * 256 <= items || q < items) && uint32_t(gy) < a.h && uint32_t(gx) < a.w) {
+            const uint32_t pix = uint32_t(gy) * a.w + uint32_t(gx);
+            if (ck < c0q)
+                v[k] = *reinterpret_cast<const uint4 *>(in0 + pix * a.c0 + ck * 8);
+            else
+                v[k] = *reinterpret_cast<const uint4 *>(in1 + (uint32_t(gy / 2) * w2 + uint32_t(gx / 2)) * a.c1 +
+                                                        (ck - c0q) * 8);
+        }
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < per; k++) {
+        const uint32_t q = threadIdx.x + k * 256;
+        if ((k + 1) * 256 <= items || q < items)
+            *reinterpret_cast<uint4 *>(halo + (q / C4) * stride + (q % C4) * 8) = v[k];
+    }
+    __syncthreads();
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u, li = lane & 15u, g = lane >> 4;
+    const uint32_t co_tile = wave % NCO, grp = wave / NCO;
+    const uint32_t co_base = blockIdx.y * (16 * NCO) + co_tile * 16;
+    const uint16_t *wrow = a.wt + uint64_t(co_base + li) * 9 * cin + g * 8;
+    const uint16_t *hbase = halo + ((li >> 3) * 10 + (li & 7u) + 2 * grp * S * 10) * stride + g * 8;
+    f32x4 acc[S];
+#pragma unroll
+    for (int j = 0; j < S; j++) acc[j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (uint32_t tap = 0; tap < 9; tap++) {
+        const uint32_t dy = tap / 3, dx = tap % 3;
+        const uint16_t *hrow = hbase + (dy * 10 + dx) * stride;
+#pragma unroll
+        for (uint32_t c = 0; c < cin; c += 32) {
+            const bf16x8 A = as_bf16x8(*reinterpret_cast<const uint4 *>(wrow + tap * cin + c));
+#pragma unroll
+            for (int j = 0; j < S; j++) {
+                const bf16x8 B = as_bf16x8(*reinterpret_cast<const uint4 *>(hrow + 2 * j * 10 * stride + c));
+                acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[j], 0, 0, 0);
+            }
+        }
+    }
+    // D[co][pixel]: this lane holds channels co .. co + 3 of pixel li of each strip
+    const uint32_t co = co_base + 4 * g;
+    if (a.out_color && co != 0) return;  // the colour is channels 0-2
+    float b[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (a.bias)
+        for (int r = 0; r < 4; r++) b[r] = a.bias[co + r];
+#pragma unroll
+    for (int j = 0; j < S; j++) {
+        const uint32_t p = grp * S + j;
+        const uint32_t y = ty * TH + 2 * p + (li >> 3), x = tx * 8 + (li & 7u);
+        if (y >= a.h || x >= a.w) continue;
+        float r4[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            r4[r] = acc[j][r] + b[r];
+            if (a.relu) r4[r] = fmaxf(r4[r], 0.0f);
+        }
+        const uint64_t pix = (uint64_t(n) * a.h + y) * a.w + x;
+        if (a.out_color) {
+            for (int r = 0; r < 3; r++) a.out_color[3 * pix + r] = expm1f(fmaxf(r4[r], 0.0f));
+            continue;
+        }
+        const uint64_t o = pix * a.cout + co;
+        if (a.out_f32) *reinterpret_cast<float4 *>(a.out_f32 + o) = make_float4(r4[0], r4[1], r4[2], r4[3]);
+        if (a.out_bf16)
+            *reinterpret_cast<uint2 *>(a.out_bf16 + o) = make_uint2(pack_bf16(r4[0], r4[1]), pack_bf16(r4[2], r4[3]));
+    }
+}
+
 // ---- weight gradient --------------------------------------------------------------------------------------------
 template <int NCO>
 __global__ __launch_bounds__(256) void k_wgrad(WgradArgs a, uint32_t tiles_x, uint32_t tiles_y, uint32_t tiles,
@@ -345,6 +445,40 @@ int launch_conv(const ConvArgs &a, hipStream_t s) {
     const size_t lds = size_t(th + 2) * 10 * (cin + 8) * 2;
     const int nco = pick_nco(a.cout);
     const dim3 grid(static_cast<uint32_t>(tiles), a.cout / (16 * nco));
+    const uint32_t ch = cin / 32;
+    if (a.out_color && (cin != 32 || a.c1)) return -1;  // the fused colour output takes cin = 32 only
+    // (64 output channels over 64 inputs on levels of 256 rows or more stay on k_conv: k_conv_c<4, 16, 2> needs
+    // 220 registers, one wave a SIMD, and ran those layers 20 % slower)
+    if (ch >= 1 && ch <= 7 && ch != 6 && !(nco == 4 && ch == 2 && a.h >= 256)) {
+#define TRT_C(N, T)                                                                                    \
+    switch (ch) {                                                                                      \
+    case 1: hipLaunchKernelGGL((k_conv_c<N, T, 1>), grid, dim3(256), lds, s, a, tx, ty); break;      \
+    case 2: hipLaunchKernelGGL((k_conv_c<N, T, 2>), grid, dim3(256), lds, s, a, tx, ty); break;      \
+    case 3: hipLaunchKernelGGL((k_conv_c<N, T, 3>), grid, dim3(256), lds, s, a, tx, ty); break;      \
+    case 4: hipLaunchKernelGGL((k_conv_c<N, T, 4>), grid, dim3(256), lds, s, a, tx, ty); break;      \
+    case 5: hipLaunchKernelGGL((k_conv_c<N, T, 5>), grid, dim3(256), lds, s, a, tx, ty); break;      \
+    default: hipLaunchKernelGGL((k_conv_c<N, T, 7>), grid, dim3(256), lds, s, a, tx, ty); break;     \
+    }
+        if (tall) {
+            if (nco == 4) {
+                TRT_C(4, 16)
+            } else if (nco == 2) {
+                TRT_C(2, 16)
+            } else {
+                TRT_C(1, 16)
+            }
+        } else {
+            if (nco == 4) {
+                TRT_C(4, 8)
+            } else if (nco == 2) {
+                TRT_C(2, 8)
+            } else {
+                TRT_C(1, 8)
+            }
+        }
+#undef TRT_C
+        return hipGetLastError() == hipSuccess ? 0 : -1;
+    }
     if (tall) {
         if (nco == 4)
             hipLaunchKernelGGL((k_conv<4, 16>), grid, dim3(256), lds, s, a, tx, ty);

@@ -19,6 +19,9 @@ struct ConvArgs {
     uint16_t *out_bf16;   // (n, h, w, cout), or null
     float *out_f32;       // (n, h, w, cout), or null
     uint32_t c0, c1, cout, n, h, w, relu;
+    // Inference's last layer (cin = 32): with out_color set, the only output is k_output's colour,
+    // expm1(max(y, 0)) of channels 0-2, into this (n, h, w, 3) image.
+    float *out_color = nullptr;
 };
 int launch_conv(const ConvArgs &a, hipStream_t s);
 // Levels at least `rows` tall take 16-pixel-tall convolution tiles (default 128; UINT32_MAX: none).

@@ -139,10 +139,11 @@ int wgrad(trt_unet *u, int l, const uint16_t *in0, const uint16_t *in1, const ui
     return TRT_OK;
 }
 
+// out_color null: training's forward (the FP32 output kept for the loss); else inference, the last convolution
+// writing the colour straight to out_color.
 int forward(trt_unet *u, uint32_t n, uint32_t h, uint32_t w, const float *color, const float *albedo,
-            const float *normal, hipStream_t s) {
-    const uint64_t p0 = uint64_t(n) * h * w;
-    UNET_LAUNCH(trt::launch_features(color, albedo, normal, u->feat, p0, s));
+            const float *normal, hipStream_t s, float *out_color = nullptr) {
+    UNET_LAUNCH(trt::launch_features(color, albedo, normal, u->feat, uint64_t(n) * h * w, s));
     int rc = conv(u, E0A, u->feat, nullptr, u->act[E0A], nullptr, n, h, w, s);
     if (!rc) rc = conv(u, E0B, u->act[E0A], nullptr, u->act[E0B], nullptr, n, h, w, s);
     const int enc_a[4] = {E1A, E2A, E3A, E4A}, enc_b[4] = {E1B, E2B, E3B, E4B}, prev_b[4] = {E0B, E1B, E2B, E3B};
@@ -158,7 +159,16 @@ int forward(trt_unet *u, uint32_t n, uint32_t h, uint32_t w, const float *color,
         rc = conv(u, dec_a[k], u->act[skip[k]], u->act[below[k]], u->act[dec_a[k]], nullptr, n, h, w, s);
         if (!rc) rc = conv(u, dec_b[k], u->act[dec_a[k]], nullptr, u->act[dec_b[k]], nullptr, n, h, w, s);
     }
-    if (!rc) rc = conv(u, OUT, u->act[D0B], nullptr, nullptr, u->y, n, h, w, s);
+    if (!rc && out_color) {
+        const Spec &sp = kSpec[OUT];
+        trt::ConvArgs a{};
+        a.in0 = u->act[D0B], a.in1 = nullptr, a.wt = u->wf + u->hoff[OUT], a.bias = u->params + u->boff[OUT];
+        a.out_bf16 = nullptr, a.out_f32 = nullptr, a.out_color = out_color;
+        a.c0 = sp.c0, a.c1 = 0, a.cout = sp.cout, a.n = n, a.h = h, a.w = w, a.relu = 0;
+        UNET_LAUNCH(trt::launch_conv(a, s));
+    } else if (!rc) {
+        rc = conv(u, OUT, u->act[D0B], nullptr, nullptr, u->y, n, h, w, s);
+    }
     return rc;
 }
 
@@ -352,9 +362,7 @@ int trt_unet_denoise(trt_unet *u, uint32_t n, uint32_t h, uint32_t w, const floa
     trt_device_lock(u->dev, true);
     hipStream_t s = trt_device_stream(u->dev);
     hipSetDevice(trt_device_ordinal(u->dev));
-    rc = forward(u, n, h, w, color, albedo, normal, s);
-    if (!rc && trt::launch_output(u->y, out, uint64_t(n) * h * w, s) != 0)
-        rc = trt_fail(TRT_E_DEVICE, "trt_unet_denoise: launch failed");
+    rc = forward(u, n, h, w, color, albedo, normal, s, out);  // the colour written by the last convolution
     trt_device_lock(u->dev, false);
     return rc ? rc : trt_ok();
 }
