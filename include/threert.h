@@ -152,6 +152,9 @@ typedef struct {
                                * Its emitters are the CDF's first P.ltree.y entries, in leaf order; their triangles'
                                * codes name their paths (lib.rs:419-441 emitter_order / emitter_codes) */
     uint64_t light_tree_vec4s;
+    const void *glass_energy; /* lib.rs:1254 glass_energy_lut: 1024 vec4, R + T of the rough dielectric over 16 sqrt(cos)
+                               * x 16 roughness x 8 sqrt((ior - 1) / (ior + 1)) nodes, entering then leaving. Needed
+                               * (and read) only when a material has glass (layer.x > 0, the reference's F_GLASS) */
 } trt_scene_desc;
 
 typedef struct {
@@ -171,9 +174,10 @@ typedef struct {
 
 /* Copies the host buffers to HBM and builds the 4-wide quantised tree from the node buffer. The caller's memory
  * is borrowed for the call only. Refuses (TRT_E_INVALID) trees the reference's checks refuse (units.rs check_mesh /
- * check_top, ltree.rs LightTree::check), records with cutout or glass bits, and materials that need what this core
- * does not render (textures, normal maps, glass, alpha cutoffs, vertex colours, unlit or function materials), each
- * named in trt_last_error. */
+ * check_top, ltree.rs LightTree::check), records with cutout bits or with glass among shadow-casting sections (the
+ * reference's F_IGLASS; a record whose sections are all glass casts no shadow and is taken), materials that need
+ * what this core does not render (textures, normal maps, alpha cutoffs, vertex colours, unlit or function
+ * materials), and glass without the glass energy table, each named in trt_last_error. */
 int trt_scene_upload(trt_device *dev, const trt_scene_desc *desc, trt_scene **out);
 int trt_scene_release(trt_scene *scene);
 int trt_scene_stats_get(trt_scene *scene, trt_scene_stats *out);

@@ -275,23 +275,28 @@ class _RenderScene(C.Structure):
                 ("light_count", C.c_uint32), ("emitters", C.c_void_p), ("emitter_vec4s", C.c_uint64),
                 ("emitter_tris", C.c_void_p), ("emitter_tri_count", C.c_uint64), ("objects", C.c_void_p),
                 ("object_count", C.c_uint64), ("energy", C.c_void_p), ("light_tree", C.c_void_p),
-                ("light_tree_count", C.c_uint64)]
+                ("light_tree_count", C.c_uint64), ("glass_energy", C.c_void_p)]
 
 
 class RenderScene:
     """The kernel's buffers plus the shading tables, as the oracle's path loop reads them."""
 
     def __init__(self, nodes, leaves, instances, top, units, vertices, indices, rows, materials, lights, emitters,
-                 emitter_tris, objects, energy, light_tree=None):
+                 emitter_tris, objects, energy, light_tree=None, glass_energy=None):
         self.geo = Scene(nodes, leaves, instances, top, units)
         if light_tree is None:
             light_tree = np.zeros((0, 4), np.float32)
+        if glass_energy is None:
+            glass_energy = np.zeros((0, 4), np.float32)
+        elif np.asarray(glass_energy).shape != (1024, 4):
+            raise ValueError("glass_energy must be the (1024, 4) float32 table")
         self.keep = [np.ascontiguousarray(x) for x in (vertices, indices, rows, materials, lights, emitters,
                                                        emitter_tris, objects, energy, light_tree)]
-        v, i, r, m, li, e, et, o, en, lt = self.keep
+        self.keep.append(np.ascontiguousarray(glass_energy, np.float32))
+        v, i, r, m, li, e, et, o, en, lt, ge = self.keep
         self._s = _RenderScene(self.geo._s, _ptr(v), len(v), _ptr(i), len(i), _ptr(r), len(r), _ptr(m), len(m),
                                _ptr(li), len(li), _ptr(e), len(e), _ptr(et), len(et), _ptr(o), len(o) // 3,
-                               _ptr(en), _ptr(lt) if len(lt) else None, len(lt))
+                               _ptr(en), _ptr(lt) if len(lt) else None, len(lt), _ptr(ge) if len(ge) else None)
         L = lib()
         L.or_render_sample.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_int]
         L.or_render_sample.restype = C.c_int64
@@ -300,6 +305,10 @@ class RenderScene:
         L.or_emitter_draw.restype = C.c_float
         L.or_emitter_pmf.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]
         L.or_emitter_pmf.restype = C.c_float
+        L.or_glass_energy.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_int]
+        L.or_glass_energy.restype = C.c_float
+        L.or_glass_share.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_float]
+        L.or_glass_share.restype = C.c_float
 
     def emitter_draw(self, params: np.ndarray, x, n, u) -> tuple[int, float]:
         """pathtrace.wgsl:547-585 emitter_draw at x (normal n) for uniforms u[3]: (CDF entry, p); entry -1: none."""
@@ -319,6 +328,23 @@ class RenderScene:
         if pr < 0:
             raise RuntimeError("emitter_pmf read outside its tables")
         return float(pr)
+
+    def glass_energy(self, nov: float, roughness: float, ior: float, leaving: bool) -> float:
+        """pathtrace.wgsl:1582-1603 glass_energy: R + T of the rough interface, interpolated in the table."""
+        if len(self.keep[-1]) == 0:
+            raise ValueError("the scene has no glass table")
+        r = lib().or_glass_energy(C.byref(self._s), nov, roughness, ior, int(bool(leaving)))
+        if r < 0:
+            raise RuntimeError("glass_energy read outside its table")
+        return float(r)
+
+    def glass_share(self, params: np.ndarray, material: int, cos_d: float) -> float:
+        """pathtrace.wgsl:930-970 glass_at's share for `material` at |cos| cos_d, under `params`."""
+        p = np.ascontiguousarray(params, np.uint32)
+        r = lib().or_glass_share(C.byref(self._s), _ptr(p), material, cos_d)
+        if r < 0:
+            raise RuntimeError("glass_share read outside its tables")
+        return float(r)
 
     def render(self, params: np.ndarray, first_sample: int, spp: int, accum: np.ndarray | None = None,
                rows=None, threads=0) -> np.ndarray:

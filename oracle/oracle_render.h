@@ -56,6 +56,9 @@ typedef struct {
     const float *energy;        /* lib.rs energy_lut: 512 vec4 */
     const float *light_tree;    /* ltree.rs LightTree::nodes: vec4s (NULL: none) */
     uint64_t light_tree_count;
+    /* lib.rs glass_energy_lut: 1024 vec4, entering then leaving. Non-NULL: a material has glass (the reference's F_GLASS,
+     * gpu.rs features_of: layer.x > 0), and the glass paths run */
+    const float *glass_energy;
 } or_render_scene;
 
 /* One sample (params word 18 = P.size.z) of every pixel of rows [y0, y1), accumulated into `accum` (80 bytes a
@@ -70,6 +73,10 @@ int64_t or_render_sample(const or_render_scene *s, const uint32_t *params, uint3
 float or_emitter_draw(const or_render_scene *s, const uint32_t *params, const float *x, const float *n,
                       const float *u, int64_t *entry);
 float or_emitter_pmf(const or_render_scene *s, const uint32_t *params, const float *x, const float *n, uint32_t ei);
+/* glass_energy (pathtrace.wgsl:1582-1603) in the scene's table (-1: read outside it); glass_at's share (930-970) of
+ * a material at |cos| cos_d under `params` */
+float or_glass_energy(const or_render_scene *s, float nov, float roughness, float ior, int leaving);
+float or_glass_share(const or_render_scene *s, const uint32_t *params, uint32_t material, float cos_d);
 
 /* denoise.rs:10-74 denoise (a-trous), and the expf it uses here. Images: 3 floats a pixel (variance: 1). */
 float or_expf(float x);

@@ -150,6 +150,7 @@ typedef struct {
     int mirror_path;
     V mis_x, mis_n;
     int sampled_spec;
+    int sampled_trans; /* the last BSDF sample went through the surface (pathtrace.wgsl:449) */
     int failed;
     int64_t *oob; /* table reads outside their table (each reads entry 0 instead) */
 } Ctx;
@@ -242,6 +243,7 @@ typedef struct {
     V p, n, ng, base, emissive;
     float roughness, metallic, specular;
     float coat, coat_rough; /* pathtrace.wgsl:1420-1423 (F_COAT: on when any material has one; 0 otherwise) */
+    float transmission, ior; /* pathtrace.wgsl:1413-1419 (F_GLASS: a material has glass) */
     int front, card;
 } Surface;
 
@@ -283,6 +285,12 @@ static Surface surface(Ctx *c, V o, V d, const or_hit *h) {
     s.n = n;
     s.coat = clampf(m[28 + 2], 0.0f, 1.0f); /* layer.z, layer.w */
     s.coat_rough = clampf(m[28 + 3], 0.02f, 1.0f);
+    s.ior = 1.5f;
+    s.transmission = 0.0f;
+    if (c->s->glass_energy) { /* layer.x, layer.y */
+        s.transmission = clampf(m[28 + 0], 0.0f, 1.0f);
+        s.ior = fmaxf(m[28 + 1], 1.0001f);
+    }
     return s;
 }
 
@@ -291,7 +299,59 @@ typedef struct {
     V diffuse, f0, f90, w, e_spec, kd;
     float a2, p_spec, p_coat, p_trans;
     float coat, coat_alpha, e_coat;
+    /* the dielectric (pathtrace.wgsl:1535-1538): its share x compensation, tint, n(far) / n(view), rough or a delta
+     * transmission, its directional reflection albedo */
+    V tint;
+    float t, eta, e_glass;
+    int rough_t;
 } Lobes;
+
+/* ---- the dielectric (pathtrace.wgsl:1542-1640) ---------------------------------------------------------------- */
+#define GLASS_DELTA 0.05f
+#define GLASS_E_MU 16u
+#define GLASS_E_ROUGH 16u
+#define GLASS_E_IOR 8u
+#define GLASS_E_ZMAX 0.70710678f
+static inline int glass_on(const Ctx *c) { return c->s->glass_energy != NULL; }
+/* 1559-1566 */
+static float fresnel_dielectric(float c, float eta) {
+    float g2 = eta * eta - 1.0f + c * c;
+    if (g2 <= 0.0f) return 1.0f;
+    float g = sqrtf(g2);
+    float a = (g - c) / (g + c);
+    float b = (c * (g + c) - 1.0f) / (c * (g - c) + 1.0f);
+    return 0.5f * a * a * (1.0f + b * b);
+}
+/* 1569-1572 */
+static inline float lambda_ggx(float a2, float c) {
+    float c2 = c * c;
+    return 0.5f * (sqrtf(1.0f + a2 * (1.0f - c2) / fmaxf(c2, 1e-8f)) - 1.0f);
+}
+/* 1582-1603 */
+static float glass_e_at(const Ctx *c, uint32_t i) {
+    const float *v = c->s->glass_energy + 4 * ck(c, i / 4u, 1024);
+    return v[i & 3u];
+}
+static float glass_e_row(const Ctx *c, uint32_t row, float x) {
+    uint32_t x0 = (uint32_t)x;
+    uint32_t x1 = x0 + 1u < GLASS_E_MU - 1u ? x0 + 1u : GLASS_E_MU - 1u;
+    return mixf(glass_e_at(c, row * GLASS_E_MU + x0), glass_e_at(c, row * GLASS_E_MU + x1), x - (float)x0);
+}
+static float glass_energy(const Ctx *c, float nov, float roughness, float ior, int leaving) {
+    float x = clampf(sqrtf(nov), 0.0f, 1.0f) * (float)(GLASS_E_MU - 1u);
+    float y = clampf(roughness, 0.0f, 1.0f) * (float)(GLASS_E_ROUGH - 1u);
+    float z = clampf(sqrtf(fmaxf(ior - 1.0f, 0.0f) / (ior + 1.0f)) / GLASS_E_ZMAX, 0.0f, 1.0f) * (float)(GLASS_E_IOR - 1u);
+    uint32_t y0 = (uint32_t)y;
+    uint32_t y1 = y0 + 1u < GLASS_E_ROUGH - 1u ? y0 + 1u : GLASS_E_ROUGH - 1u;
+    uint32_t side = leaving ? GLASS_E_IOR : 0u, zi = (uint32_t)z;
+    uint32_t z0 = side + zi;
+    uint32_t z1 = side + (zi + 1u < GLASS_E_IOR - 1u ? zi + 1u : GLASS_E_IOR - 1u);
+    float e0 = mixf(glass_e_row(c, z0 * GLASS_E_ROUGH + y0, x), glass_e_row(c, z0 * GLASS_E_ROUGH + y1, x), y - (float)y0);
+    float e1 = mixf(glass_e_row(c, z1 * GLASS_E_ROUGH + y0, x), glass_e_row(c, z1 * GLASS_E_ROUGH + y1, x), y - (float)y0);
+    return mixf(e0, e1, z - floorf(z));
+}
+/* 1606 */
+static inline int two_sided(const Ctx *c, const Lobes *lb) { return glass_on(c) && lb->rough_t; }
 
 static void energy_node(const Ctx *c, uint32_t i, float *out) {
     const float *v = c->s->energy + 4 * ck(c, i >> 1, 512);
@@ -316,9 +376,13 @@ static void energy_lookup(const Ctx *c, float roughness, float nov, float *e) {
 
 static Lobes lobes(const Ctx *c, const Surface *s, float nov) {
     Lobes l;
-    float t = 0.0f;
+    float t = s->transmission * (1.0f - s->metallic);
     l.diffuse = vs(vs(s->base, 1.0f - s->metallic), 1.0f - t);
     float f0 = 0.08f * s->specular;
+    if (t > 0.0f) { /* glass: F0 from the index of refraction (pathtrace.wgsl:1654-1658) */
+        float ri = (s->ior - 1.0f) / (s->ior + 1.0f);
+        f0 = ri * ri;
+    }
     V f0v = v3(f0, f0, f0);
     l.f0 = v3(mixf(f0v.x, s->base.x, s->metallic), mixf(f0v.y, s->base.y, s->metallic),
               mixf(f0v.z, s->base.z, s->metallic));
@@ -332,6 +396,10 @@ static Lobes lobes(const Ctx *c, const Surface *s, float nov) {
     l.e_spec = vmul(vadd(vs(l.f0, e[0]), v3(f90 * e[1], f90 * e[1], f90 * e[1])), l.w);
     l.f90 = v3(f90, f90, f90);
     l.kd = vsub(v3(1, 1, 1), l.e_spec);
+    if (glass_on(c) && t > 0.0f) { /* 1669-1673: the share that is not the dielectric keeps the specular lobe */
+        l.w = vs(l.w, 1.0f - t);
+        l.e_spec = vs(l.e_spec, 1.0f - t);
+    }
     float r = s->roughness;
     l.a2 = r * r * r * r;
     V lw = v3(0.3f, 0.6f, 0.1f);
@@ -351,6 +419,21 @@ static Lobes lobes(const Ctx *c, const Surface *s, float nov) {
         l.coat_alpha = s->coat_rough * s->coat_rough;
     }
     float e_coat = l.e_coat, wt = 0.0f, under = 1.0f - e_coat;
+    l.t = 0.0f;
+    l.eta = 0.0f;
+    l.rough_t = 0;
+    l.tint = v3(0, 0, 0);
+    l.e_glass = 0.0f;
+    if (glass_on(c) && t > 0.0f) { /* 1688-1697 */
+        l.eta = s->front ? s->ior : 1.0f / s->ior;
+        l.rough_t = s->roughness > GLASS_DELTA && fabsf(s->ior - 1.0f) > 1e-3f;
+        float gc = 1.0f;
+        if (l.rough_t) gc = mixf(1.0f, 1.0f / glass_energy(c, nov, s->roughness, s->ior, !s->front), comp);
+        l.t = t * under * gc;
+        l.tint = vs(s->base, l.t);
+        l.e_glass = t * fresnel_dielectric(nov, l.eta);
+        wt = under * (l.e_glass + (t - l.e_glass) * dot(s->base, lw));
+    }
     float total = (ws + wd) * under + e_coat + wt;
     l.p_coat = e_coat / fmaxf(total, 1e-6f);
     l.p_trans = wt / fmaxf(total, 1e-6f);
@@ -369,15 +452,52 @@ static inline V f_schlick(V f0, V f90, float voh) {
     return vadd(f0, vs(vsub(f90, f0), k));
 }
 
-static V eval_bsdf_k(const Surface *s, const Lobes *lb, V v, V l, float k) {
+static inline float g1(float a2, float nov) { return 2.0f * nov / (nov + sqrtf(a2 + (1.0f - a2) * nov * nov)); }
+/* The dielectric's transmission toward l (the far side) x |n.l|, Walter 2007 eq. 21 (pathtrace.wgsl:1610-1622) */
+static V dielectric_t(const Surface *s, const Lobes *lb, V v, V l, float nov, float nol) {
+    if (!lb->rough_t) return v3(0, 0, 0);
+    V h = normalize(vadd(v, vs(l, lb->eta)));
+    if (dot(h, s->n) < 0.0f) h = vneg(h);
+    float voh = dot(v, h), loh = dot(l, h);
+    if (voh <= 0.0f || loh >= 0.0f) return v3(0, 0, 0);
+    float den = voh + lb->eta * loh;
+    float g2 = 1.0f / (1.0f + lambda_ggx(lb->a2, nov) + lambda_ggx(lb->a2, nol));
+    float f = (1.0f - fresnel_dielectric(voh, lb->eta)) * d_ggx(lb->a2, fmaxf(dot(s->n, h), 0.0f)) * g2 * voh * lb->eta *
+              lb->eta * -loh / (nov * fmaxf(den * den, 1e-12f));
+    return vs(lb->tint, f);
+}
+/* The density of l under the dielectric's sampling x its share of the samples (1626-1640) */
+static float pdf_dielectric(const Surface *s, const Lobes *lb, V v, V l) {
+    float nov = fmaxf(dot(s->n, v), 1e-4f);
+    if (dot(s->n, l) > 0.0f) {
+        V h = normalize(vadd(v, l));
+        float voh = fmaxf(dot(v, h), 1e-6f);
+        float f = fresnel_dielectric(lb->rough_t ? voh : nov, lb->eta);
+        return lb->p_trans * f * g1(lb->a2, nov) * d_ggx(lb->a2, fmaxf(dot(s->n, h), 0.0f)) / (4.0f * nov);
+    }
+    if (!lb->rough_t) return 0.0f;
+    V h = normalize(vadd(v, vs(l, lb->eta)));
+    if (dot(h, s->n) < 0.0f) h = vneg(h);
+    float voh = dot(v, h), loh = dot(l, h);
+    if (voh <= 0.0f || loh >= 0.0f) return 0.0f;
+    float den = voh + lb->eta * loh;
+    return lb->p_trans * (1.0f - fresnel_dielectric(voh, lb->eta)) * g1(lb->a2, nov) * voh *
+           d_ggx(lb->a2, fmaxf(dot(s->n, h), 0.0f)) * lb->eta * lb->eta * -loh / (nov * fmaxf(den * den, 1e-12f));
+}
+
+static V eval_bsdf_k(const Ctx *c, const Surface *s, const Lobes *lb, V v, V l, float k) {
     float nol = dot(s->n, l);
     float nov = fmaxf(dot(s->n, v), 1e-4f);
+    if (two_sided(c, lb) && nol < 0.0f && dot(s->ng, l) < 0.0f) return dielectric_t(s, lb, v, l, nov, -nol);
     if (nol <= 0.0f || dot(s->ng, l) <= 0.0f) return v3(0, 0, 0);
     V h = normalize(vadd(v, l));
     float noh = fmaxf(dot(s->n, h), 0.0f);
     float voh = fmaxf(dot(v, h), 0.0f);
     float dv = d_ggx(lb->a2, noh) * vis_smith(lb->a2, nov, nol);
     V spec = vs(vmul(vs(f_schlick(lb->f0, lb->f90, voh), dv), lb->w), k);
+    /* the dielectric's reflection (1747-1748): lb->t holds the coat's share already, so it joins after the coat */
+    float glass = 0.0f;
+    if (glass_on(c) && lb->t > 0.0f) glass = lb->t * fresnel_dielectric(voh, lb->eta) * dv * k;
     V diff = vmul(vs(lb->diffuse, INV_PI), lb->kd);
     if (lb->coat > 0.0f) { /* pathtrace.wgsl:1750-1755 */
         float ca2 = lb->coat_alpha * lb->coat_alpha;
@@ -386,15 +506,16 @@ static V eval_bsdf_k(const Surface *s, const Lobes *lb, V v, V l, float k) {
         diff = vs(diff, 1.0f - lb->e_coat);
         spec = vadd(vs(spec, 1.0f - lb->e_coat), v3(coat, coat, coat));
     }
+    spec = vadd(spec, v3(glass, glass, glass));
     return vs(vadd(diff, spec), nol);
 }
-static V eval_bsdf(const Surface *s, const Lobes *lb, V v, V l) { return eval_bsdf_k(s, lb, v, l, 1.0f); }
+static V eval_bsdf(const Ctx *c, const Surface *s, const Lobes *lb, V v, V l) { return eval_bsdf_k(c, s, lb, v, l, 1.0f); }
 
-static inline float g1(float a2, float nov) { return 2.0f * nov / (nov + sqrtf(a2 + (1.0f - a2) * nov * nov)); }
 
-static float pdf_bsdf(const Surface *s, const Lobes *lb, V v, V l) {
+static float pdf_bsdf(const Ctx *c, const Surface *s, const Lobes *lb, V v, V l) {
     float nol = dot(s->n, l);
     float pg = 0.0f;
+    if (glass_on(c) && lb->p_trans > 0.0f) pg = pdf_dielectric(s, lb, v, l);
     if (nol <= 0.0f) return pg;
     float nov = fmaxf(dot(s->n, v), 1e-4f);
     V h = normalize(vadd(v, l));
@@ -415,8 +536,8 @@ static inline float power_heuristic(float a, float b) {
     float a2 = a * a;
     return a2 / fmaxf(a2 + b * b, 1e-20f);
 }
-static float light_mis(float pdf_l, const Surface *s, const Lobes *lb, V v, V l) {
-    return power_heuristic(pdf_l, pdf_bsdf(s, lb, v, l));
+static float light_mis(const Ctx *c, float pdf_l, const Surface *s, const Lobes *lb, V v, V l) {
+    return power_heuristic(pdf_l, pdf_bsdf(c, s, lb, v, l));
 }
 
 static M3 basis(V n) {
@@ -449,14 +570,38 @@ static V sample_vndf(V v, M3 tbn, float alpha, float u1, float u2) {
     return mmul(tbn, hl);
 }
 static inline V reflect(V e1, V e2) { return vsub(e1, vs(e2, 2.0f * dot(e2, e1))); }
+/* WGSL refract: the zero vector at total internal reflection */
+static inline V refract(V e1, V e2, float eta) {
+    float d = dot(e2, e1);
+    float k = 1.0f - eta * eta * (1.0f - d * d);
+    if (k < 0.0f) return v3(0, 0, 0);
+    return vsub(vs(e1, eta), vs(e2, eta * d + sqrtf(k)));
+}
 
 static V sample_bsdf(Ctx *c, const Surface *s, const Lobes *lb, V v) {
     M3 tbn = basis(s->n);
     float r[4];
     or_rng4(&c->g, or_rng_dim(&c->g, D_BSDF), r);
+    c->sampled_trans = 0;
     if (r[0] < lb->p_coat) { /* pathtrace.wgsl:1853-1857: the coat's lobe (p_coat is 0 without one) */
         c->sampled_spec = 1;
         return reflect(vneg(v), sample_vndf(v, tbn, lb->coat_alpha, r[2], r[3]));
+    }
+    if (r[0] < lb->p_coat + lb->p_trans) {
+        /* 1858-1874, the dielectric: a visible microfacet normal (smooth glass refracts through the shading normal),
+         * reflected or refracted by the Fresnel reflectance there; a reflection under the surface or a refraction
+         * above it is no sample */
+        V h = sample_vndf(v, tbn, s->roughness * s->roughness, r[2], r[3]);
+        V m = lb->rough_t ? h : s->n;
+        V tr = refract(vneg(v), m, 1.0f / lb->eta);
+        c->sampled_spec = r[0] - lb->p_coat < fresnel_dielectric(fmaxf(dot(v, m), 1e-4f), lb->eta) * lb->p_trans ||
+                          (tr.x == 0.0f && tr.y == 0.0f && tr.z == 0.0f);
+        if (c->sampled_spec) {
+            V rl = reflect(vneg(v), h);
+            return dot(rl, s->n) <= 0.0f ? v3(0, 0, 0) : rl;
+        }
+        c->sampled_trans = 1;
+        return dot(tr, s->n) >= 0.0f ? v3(0, 0, 0) : tr;
     }
     c->sampled_spec = r[1] < lb->p_spec;
     if (c->sampled_spec) {
@@ -481,6 +626,85 @@ static int occluded(Ctx *c, V o, V d, float tmax, uint32_t mode) {
     float of[3] = {o.x, o.y, o.z}, df[3] = {d.x, d.y, d.z};
     if (or_trace(&c->s->geo, of, df, tmax, mode, OR_WALK_EXACT, 0, &h, NULL)) c->failed++;
     return h.tri != OR_NO_TRI;
+}
+
+/* ---- glass along shadow rays (pathtrace.wgsl:846-977) ------------------------------------------------------- */
+typedef struct {
+    V ng, tint;
+    float share;
+} GlassAt;
+/* the share of a ray a glass material lets through at |cos| cs (glass_at, pathtrace.wgsl:955-969) */
+static float glass_share(const Ctx *c, const float *m, float cs) {
+    float tr = clampf(m[28 + 0], 0.0f, 1.0f) * (1.0f - clampf(m[8 + 0], 0.0f, 1.0f));
+    if (tr <= 0.0f) return 0.0f;
+    float ior = fmaxf(m[28 + 1], 1.0001f);
+    float ri = (ior - 1.0f) / (ior + 1.0f);
+    float f0 = ri * ri;
+    float e[2];
+    energy_lookup(c, clampf(m[8 + 1], 0.02f, 1.0f), cs, e);
+    /* spec_terms (1521-1525) at this F0: the red channel of its directional albedo */
+    const float k503 = 50.0f / 3.0f;
+    float f90 = clampf(dot(v3(f0, f0, f0), v3(k503, k503, k503)), 0.0f, 1.0f);
+    float w = 1.0f + f0 * (pf(c->P, MISC, 2) * (1.0f / fmaxf(e[0] + e[1], 1e-3f) - 1.0f));
+    return tr * (1.0f - (f0 * e[0] + f90 * e[1]) * w);
+}
+/* glass_at (930-970): a hit's geometric normal, the share of a ray its transmission lets through at that angle
+ * (transmission x (1 - the specular lobe's directional albedo there); 0: not glass) and its tint */
+static GlassAt glass_at(const Ctx *c, const or_hit *h, V d) {
+    TriData td = tri_data(c, h->tri, h->inst, h->part);
+    V e1 = td.e1, e2 = td.e2;
+    TriAttr a = td.a;
+    float side = 1.0f;
+    if (h->inst != OR_NO_INST) {
+        const or_instance *r = inst_at(c, h->inst);
+        e1 = inst_vec(r, e1);
+        e2 = inst_vec(r, e2);
+        inst_attr(r, &a);
+        side = inst_sign(r);
+    }
+    const float *m = material_at(c, material_of(c, &a, h->inst));
+    GlassAt g;
+    g.ng = vs(normalize(cross(e1, e2)), side);
+    g.share = 0.0f;
+    g.tint = v3(1, 1, 1);
+    if (!(clampf(m[28 + 0], 0.0f, 1.0f) * (1.0f - clampf(m[8 + 0], 0.0f, 1.0f)) > 0.0f)) return g;
+    g.tint = ld3(m);
+    g.share = glass_share(c, m, fabsf(dot(g.ng, d)));
+    return g;
+}
+/* glass_tr (885-906): past every opaque caster, what the glass a shadow ray crosses lets through, the ray kept
+ * straight; toward the sky an odd number of glass surfaces lets nothing through */
+static V glass_tr(Ctx *c, V o, V d, float tmax, int sky) {
+    float left = tmax;
+    V tr = v3(1, 1, 1);
+    int odd = 0;
+    for (uint32_t k = 0; k < 8u; k++) {
+        or_hit h;
+        float of[3] = {o.x, o.y, o.z}, df[3] = {d.x, d.y, d.z};
+        if (or_trace(&c->s->geo, of, df, left, OR_TRACE_CLOSEST, OR_WALK_EXACT, 0, &h, NULL)) c->failed++;
+        if (h.tri == OR_NO_TRI) break;
+        GlassAt g = glass_at(c, &h, d);
+        if (g.share > 0.0f) {
+            tr = vmul(tr, vs(g.tint, g.share));
+            odd = !odd;
+        }
+        o = offset_origin(vadd(o, vs(d, h.t)), dot(g.ng, d) < 0.0f ? vneg(g.ng) : g.ng);
+        left -= h.t;
+        if (left <= 0.0f || all_nonpos(tr)) break;
+    }
+    if (sky && odd) return v3(0, 0, 0);
+    return tr;
+}
+/* shadow_tr (846-855): 0 past a shadow caster; else the glass's share (objects that cast no shadow are glass) */
+static V shadow_tr(Ctx *c, V o, V d, float tmax, int sky) {
+    if (occluded(c, o, d, tmax, OR_TRACE_SHADOW)) return v3(0, 0, 0);
+    if (!glass_on(c)) return v3(1, 1, 1);
+    return glass_tr(c, o, d, tmax, sky);
+}
+/* light_origin (2153-2156): a shadow ray's origin on the side l leaves the surface from */
+static inline V light_origin(const Ctx *c, const Surface *s, V o, V l) {
+    if (glass_on(c) && dot(l, s->ng) < 0.0f) return offset_origin(s->p, vneg(s->ng));
+    return o;
 }
 
 static V env_radiance(const Ctx *c, V d) {
@@ -757,7 +981,7 @@ static ES sample_emitter_ris(Ctx *c, V x, V n, uint32_t m, const Surface *sf, co
         float d2 = fmaxf(dot(dl, dl), 1e-8f);
         V l = vs(dl, 1.0f / sqrtf(d2));
         float g = fabsf(dot(ln, l)) * (dot(n, n) == 0.0f ? 1.0f : fmaxf(dot(n, l), 0.0f)) / d2;
-        if (m > 1u) g = fabsf(dot(ln, l)) * luminance(eval_bsdf(sf, lb, v, l)) / d2;
+        if (m > 1u) g = fabsf(dot(ln, l)) * luminance(eval_bsdf(c, sf, lb, v, l)) / d2;
         float ph = pk.w / (pk.e[3] * 1.0f) * g;
         float pdf_a = pk.p / (0.5f * a2);
         float w = ph / pdf_a;
@@ -823,12 +1047,15 @@ static ES sample_emitter(Ctx *c, V x, V n, uint32_t rd) {
 
 static V env_light(Ctx *c, const Surface *s, const Lobes *lb, V v, V o) {
     V l = sample_env(c, or_rng_dim(&c->g, D_ENV));
-    V f = eval_bsdf(s, lb, v, l);
+    V f = eval_bsdf(c, s, lb, v, l);
     if (any_pos(f)) {
         float pdf_e = env_pdf();
         if (pdf_e > 0.0f) {
-            if (!occluded(c, o, l, OR_INF, OR_TRACE_SHADOW))
-                return vds(vs(vmul(env_radiance(c, l), f), light_mis(pdf_e, s, lb, v, l)), pdf_e);
+            V vis = shadow_tr(c, light_origin(c, s, o, l), l, OR_INF, 1); /* sky_tr (975-977) */
+            if (any_pos(vis)) {
+                V e = vds(vs(vmul(env_radiance(c, l), f), light_mis(c, pdf_e, s, lb, v, l)), pdf_e);
+                return glass_on(c) ? vmul(e, vis) : e;
+            }
         }
     }
     return v3(0, 0, 0);
@@ -852,17 +1079,19 @@ static V direct_light(Ctx *c, const Surface *s, const Lobes *lb, V v) {
         V l = mmul(tb, v3(sin_t * cp, sin_t * sp, cos_t));
         V col = ld3(li + 8);
         if (all_nonpos(col)) continue;
-        V f = eval_bsdf_k(s, lb, v, l, li[17]);
+        V f = eval_bsdf_k(c, s, lb, v, l, li[17]);
         if (any_pos(f)) {
-            int vis = li[15] >= 0.5f ? !occluded(c, o, l, OR_INF, OR_TRACE_SHADOW) : 1;
-            if (vis) {
+            V vis = li[15] >= 0.5f ? shadow_tr(c, light_origin(c, s, o, l), l, OR_INF, 0) : v3(1, 1, 1);
+            if (any_pos(vis)) {
                 float pdf_l = 1.0f / (TWO_PI * (1.0f - cos_max));
-                float wgt = light_mis(pdf_l, s, lb, v, l);
-                sum = vadd(sum, vs(vmul(col, f), wgt));
+                float wgt = light_mis(c, pdf_l, s, lb, v, l);
+                V e = vs(vmul(col, f), wgt);
+                sum = vadd(sum, glass_on(c) ? vmul(e, vis) : e);
             }
         }
     }
-    V ne = s->n;
+    /* rough glass takes emitters from both sides, as a point without a hemisphere (2488-2491) */
+    V ne = two_sided(c, lb) ? v3(0, 0, 0) : s->n;
     c->mis_x = o;
     c->mis_n = ne;
     uint32_t emitters = emitters_on(c) ? emitter_count(c) : 0u;
@@ -881,10 +1110,10 @@ static V direct_light(Ctx *c, const Surface *s, const Lobes *lb, V v) {
         else
             es = sample_emitter(c, o, ne, or_rng_dim(&c->g, D_EMIT));
         if (es.pdf_w > 0.0f) {
-            V f = eval_bsdf(s, lb, v, es.l);
+            V f = eval_bsdf(c, s, lb, v, es.l);
             if (any_pos(f)) {
-                if (!occluded(c, o, es.l, es.dist * 0.999f, OR_TRACE_ANY)) {
-                    float wgt = light_mis(es.pdf_w, s, lb, v, es.l);
+                if (!occluded(c, light_origin(c, s, o, es.l), es.l, es.dist * 0.999f, OR_TRACE_ANY)) {
+                    float wgt = light_mis(c, es.pdf_w, s, lb, v, es.l);
                     sum = vadd(sum, vs(vs(vmul(es.le, f), wgt), es.w));
                 }
             }
@@ -984,16 +1213,33 @@ static void path(Ctx *c, uint32_t px, uint32_t py, float *pix) {
         if (depth > 0u && !straight) dl = clamp_lum(dl, clamp_v);
         radiance = vadd(radiance, dl);
         V l = sample_bsdf(c, &s, &lb, v);
-        float pdf = pdf_bsdf(&s, &lb, v, l);
-        V wgt = vds(eval_bsdf(&s, &lb, v, l), fmaxf(pdf, 1e-20f));
-        if (pdf <= 0.0f || all_nonpos(wgt)) break;
+        /* 4414-4447: through smooth glass a delta lobe, weight tint / its sampling share, no MIS */
+        int through = glass_on(c) && c->sampled_trans && !lb.rough_t;
+        float pdf = 0.0f;
+        V wgt = vds(lb.tint, fmaxf(lb.p_trans, 1e-6f));
+        if (!through) {
+            pdf = pdf_bsdf(c, &s, &lb, v, l);
+            wgt = vds(eval_bsdf(c, &s, &lb, v, l), fmaxf(pdf, 1e-20f));
+        }
+        if ((!through && pdf <= 0.0f) || all_nonpos(wgt)) break;
         thr = vmul(thr, wgt);
-        last_pdf = pdf;
-        specular_bounce = 0;
-        behind_glass = 0;
-        bent = 0;
-        straight = 0;
-        c->mirror_path = (depth == 0u || c->mirror_path) && s.roughness < MIRROR_ROUGHNESS && c->sampled_spec;
+        if (through && dot(l, s.ng) < 0.0f) { /* 4449-4465: glass passed since the last scattering vertex */
+            behind_glass = 1;
+            bent = !bent;
+        } else if (through) {
+            last_pdf = 0.0f;
+            specular_bounce = 1;
+            behind_glass = 0;
+            bent = 0;
+        } else {
+            last_pdf = pdf;
+            specular_bounce = 0;
+            behind_glass = 0;
+            bent = 0;
+        }
+        straight = straight && through;
+        c->mirror_path = (depth == 0u || c->mirror_path) && s.roughness < MIRROR_ROUGHNESS &&
+                         (c->sampled_spec || c->sampled_trans);
         if (depth >= pu(P, COUNTS, 3)) {
             float q = fminf(fmaxf(thr.x, fmaxf(thr.y, thr.z)), 0.95f);
             if (or_rng1(&c->g, or_rng_dim(&c->g, D_RR)) > q) break;
@@ -1056,4 +1302,17 @@ float or_emitter_pmf(const or_render_scene *s, const uint32_t *params, const flo
     Em em = emitter_tri_at(&c, ei);
     float p = emitter_pmf(&c, ei, &em, v3(x[0], x[1], x[2]), v3(n[0], n[1], n[2]));
     return oob ? -1.0f : p;
+}
+
+float or_glass_energy(const or_render_scene *s, float nov, float roughness, float ior, int leaving) {
+    int64_t oob = 0;
+    Ctx c = hook_ctx(s, NULL, &oob);
+    float e = glass_energy(&c, nov, roughness, ior, leaving);
+    return oob ? -1.0f : e;
+}
+float or_glass_share(const or_render_scene *s, const uint32_t *params, uint32_t material, float cos_d) {
+    int64_t oob = 0;
+    Ctx c = hook_ctx(s, params, &oob);
+    float e = glass_share(&c, material_at(&c, material), cos_d);
+    return oob ? -1.0f : e;
 }

@@ -32,7 +32,7 @@ def test_the_structs_match_the_header_sizes():
     assert T.RAY.itemsize == 32 and T.HIT.itemsize == 24
     from threert_amd import render as R
 
-    assert ctypes.sizeof(R._RenderDesc) == 8 * 6 + 8 + 8 * 17 + 16
+    assert ctypes.sizeof(R._RenderDesc) == 8 * 6 + 8 + 8 * 17 + 16 + 8  # ... light_tree, light_tree_vec4s, glass_energy
     assert ctypes.sizeof(T._Tree) == 8 * 4 + 8 + 24
 
 

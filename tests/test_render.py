@@ -563,8 +563,9 @@ def test_the_serving_job_that_met_a_tie_matches_the_oracle(device):
 
 @pytest.mark.gpu
 def test_what_the_library_does_not_render_is_refused(device):
-    """Records with cutout or glass sections (scene.rs INST_CUTOUT 16, INST_GLASS 32 in the bits word); materials with
-    textures, normal maps, unlit or function flags, transmission textures, vertex colours, alpha cutoffs or glass;
+    """Records with cutout sections or with glass among shadow-casting sections (scene.rs INST_CUTOUT 16, INST_GLASS 32
+    in the bits word; the Cornell box's records cast shadows); materials with textures, normal maps, unlit or function
+    flags, transmission textures, vertex colours, alpha cutoffs, or glass without the glass energy table;
     and Params asking for an environment map, more than 64 bounces, local lights, fog, volumetric scattering, the live
     light LOD, procedural geometry, aerial perspective, a background or light-tree candidates without a tree: each
     refused with TRT_E_INVALID and a message that names it. The device still renders the unchanged scene afterwards."""
@@ -573,20 +574,21 @@ def test_what_the_library_does_not_render_is_refused(device):
     import threert_amd as T
 
     rs, cam = R.cornell()
-    for bit, name in ((16, "cutout"), (32, "glass")):
+    for bit, name in ((16, "cutout sections"), (32, "glass among shadow-casting sections")):
         print(f"refusal: record bit {bit} ({name})", flush=True)
         bad = copy.deepcopy(rs)
         words = bad.prep.instances["n"][:, 1, 3].view(np.uint32)
         words[0] |= np.uint32(bit)
         with pytest.raises(T.ThreeRTError) as e:
             R.upload(device, bad)
-        assert e.value.code == T.TRT_E_INVALID and "cutout or glass" in str(e.value)
+        assert e.value.code == T.TRT_E_INVALID and name in str(e.value), str(e.value)
     # materials that need what the kernel does not read (GpuMaterial words, scene.rs:480-491)
     as_int = lambda v: np.array([v], np.int32).view(np.float32)[0]  # noqa: E731
     for word, value, name in ((12, as_int(0), "base-colour texture"), (13, as_int(2), "normal map"),
                               (16, as_int(1), "emissive texture"), (18, as_int(4), "unlit"),
                               (19, as_int(1), "material function"), (22, 1.0, "transmission texture"),
-                              (23, 1.0, "vertex colours"), (27, 0.5, "alpha cutoff"), (28, 1.0, "glass")):
+                              (23, 1.0, "vertex colours"), (27, 0.5, "alpha cutoff"),
+                              (28, 1.0, "no glass energy table")):
         print(f"refusal: material word {word} ({name})", flush=True)
         bad = copy.deepcopy(rs)
         bad.materials[1, word] = value

@@ -74,7 +74,7 @@ class _SceneDesc(C.Structure):
                 ("light_count", C.c_uint64), ("emitters", C.c_void_p), ("emitter_vec4s", C.c_uint64),
                 ("emitter_tris", C.c_void_p), ("emitter_tri_count", C.c_uint64), ("objects", C.c_void_p),
                 ("object_count", C.c_uint64), ("energy", C.c_void_p), ("light_tree", C.c_void_p),
-                ("light_tree_vec4s", C.c_uint64)]
+                ("light_tree_vec4s", C.c_uint64), ("glass_energy", C.c_void_p)]
 
 
 class _LightTree(C.Structure):

@@ -89,10 +89,12 @@ struct trt_scene {
          *wleaves = nullptr;
     // shading tables (trt_render)
     void *vertices = nullptr, *indices = nullptr, *rows = nullptr, *materials = nullptr, *lights = nullptr,
-         *emitters = nullptr, *emitter_tris = nullptr, *objects = nullptr, *energy = nullptr, *light_tree = nullptr;
+         *emitters = nullptr, *emitter_tris = nullptr, *objects = nullptr, *energy = nullptr, *light_tree = nullptr,
+         *glass_energy = nullptr;
     uint64_t light_tree_count = 0;
     bool shading = false;
-    bool coat = false;  // a material has a clearcoat (layer.z > 0: the reference's F_COAT, gpu.rs features_of)
+    bool coat = false;   // a material has a clearcoat (layer.z > 0: the reference's F_COAT, gpu.rs features_of)
+    bool glass = false;  // a material has glass (layer.x > 0: the reference's F_GLASS, gpu.rs:2030-2032)
     uint64_t light_count = 0, emitter_entries = 0, emitter_tri_count = 0;
     uint64_t vertex_count = 0, tri_count = 0, row_count = 0, material_count = 0, object_count = 0, emitter_vec4s = 0;
     uint64_t node_count = 0, leaf_count = 0, instance_count = 0, wnode_count = 0, wunit_count = 0;
@@ -233,10 +235,10 @@ int trt_device_sync(trt_device *d) {
         static const char *table[] = {"instance", "material", "light", "emitter entry", "row", "object",
                                       "triangle indices", "vertex", "energy table", "emitter triangle (RIS candidate)",
                                       "leaf", "emitter triangle (RIS pick)", "emitter triangle (single draw)",
-                                      "emitter triangle (a hit's own)", "light tree node"};
+                                      "emitter triangle (a hit's own)", "light tree node", "glass energy table"};
         std::string where;
         if (g[1]) {
-            const char *name = g[1] <= 10 ? what[g[1]] : (g[1] >= 16 && g[1] <= 30 ? table[g[1] - 16] : "?");
+            const char *name = g[1] <= 10 ? what[g[1]] : (g[1] >= 16 && g[1] <= 31 ? table[g[1] - 16] : "?");
             where = std::string(" (first: ") + name + " index " +
                     std::to_string((uint64_t(g[3]) << 32) | g[2]) + " of " + std::to_string(g[4]) + "; context " +
                     std::to_string(g[5]) + " " + std::to_string(g[6]) + " " + std::to_string(g[7]) + ")";
@@ -384,7 +386,7 @@ int trt_scene_release(trt_scene *s) {
     hipFree(s->wunits);
     hipFree(s->wleaves);
     for (void *p : {s->vertices, s->indices, s->rows, s->materials, s->lights, s->emitters, s->emitter_tris,
-                    s->objects, s->energy, s->light_tree})
+                    s->objects, s->energy, s->light_tree, s->glass_energy})
         hipFree(p);
     trt_device *d = s->dev;
     delete s;
@@ -393,11 +395,12 @@ int trt_scene_release(trt_scene *s) {
 }
 
 // A material the path loop would render without what it asks for is refused, named: the reference's `surface`
-// (pathtrace.wgsl:1320-1425) reads these only under the F_TEX, F_VCOLOR, F_FUNC and F_UNLIT features, and its glass
-// and cutout paths (1542-1640, 1300-1310), none of which this core has. Words of the 72-word GpuMaterial
-// (scene.rs:480-491): tex0 12-15 and tex1 16-19 (texture slots, -1 none; tex1.z the flags, tex1.w a function + 1),
-// uv.z 22 (transmission texture + 1), uv.w 23 (vertex colours), uv2.w 27 (alpha cutoff), layer.x 28 (transmission).
-static bool material_rendered(const float *m, uint64_t count, std::string *why) {
+// (pathtrace.wgsl:1320-1425) reads these only under the F_TEX, F_VCOLOR, F_FUNC and F_UNLIT features, and its cutout
+// paths (1300-1310), none of which this core has; glass (1542-1640) needs the glass energy table. Words of the
+// 72-word GpuMaterial (scene.rs:480-491): tex0 12-15 and tex1 16-19 (texture slots, -1 none; tex1.z the flags, tex1.w
+// a function + 1), uv.z 22 (transmission texture + 1), uv.w 23 (vertex colours), uv2.w 27 (alpha cutoff), layer.x 28
+// (transmission).
+static bool material_rendered(const float *m, uint64_t count, bool glass_table, std::string *why) {
     constexpr int32_t kMatUnlit = 4;  // pathtrace.wgsl:54 MAT_UNLIT
     static const char *slot[8] = {"a base-colour texture", "a normal map", "a roughness texture",
                                   "a metalness texture", "an emissive texture", "an occlusion texture", "", ""};
@@ -413,9 +416,13 @@ static bool material_rendered(const float *m, uint64_t count, std::string *why) 
         if (!what && f[22] > 0.5f) what = "a transmission texture";
         if (!what && f[23] > 0.0f) what = "vertex colours";
         if (!what && f[27] > 0.0f) what = "an alpha cutoff";
-        if (!what && f[28] > 0.0f) what = "glass (transmission, layer.x)";
         if (what) {
             *why = "material " + std::to_string(k) + " needs " + what + ", which this core does not render";
+            return false;
+        }
+        if (f[28] > 0.0f && !glass_table) {
+            *why = "material " + std::to_string(k) + " is glass (transmission, layer.x) and the scene has no glass "
+                   "energy table";
             return false;
         }
     }
@@ -485,7 +492,9 @@ static bool check_shading(const trt_scene_desc *d, std::string *why) {
         *why = "an emitter table shorter than its guide and header";
         return false;
     }
-    if (!material_rendered(static_cast<const float *>(d->materials), d->material_count, why)) return false;
+    if (!material_rendered(static_cast<const float *>(d->materials), d->material_count, d->glass_energy != nullptr,
+                           why))
+        return false;
     const auto *inst = static_cast<const trt::Instance *>(d->instances);
     const auto *leaves = static_cast<const trt::Leaf *>(d->leaves);
     const auto *nodes = static_cast<const trt::Node *>(d->nodes);
@@ -729,8 +738,10 @@ int trt_scene_upload(trt_device *d, const trt_scene_desc *desc, trt_scene **out)
     }
     if (!rc && desc->energy) {
         s->shading = true;
-        for (uint64_t k = 0; k < desc->material_count; k++)
+        for (uint64_t k = 0; k < desc->material_count; k++) {
             if (static_cast<const float *>(desc->materials)[72 * k + 30] > 0.0f) s->coat = true;
+            if (static_cast<const float *>(desc->materials)[72 * k + 28] > 0.0f) s->glass = true;
+        }
         s->light_count = desc->light_count;
         s->emitter_tri_count = desc->emitter_tri_count;
         s->emitter_entries = desc->emitter_vec4s ? desc->emitter_vec4s - (65537 / 4 + 1) - 1 : 0;
@@ -748,6 +759,7 @@ int trt_scene_upload(trt_device *d, const trt_scene_desc *desc, trt_scene **out)
         if (!rc) rc = upload(d, &s->energy, desc->energy, 512 * 16, &total);
         if (!rc) rc = upload(d, &s->light_tree, desc->light_tree, desc->light_tree_vec4s * 16, &total);
         s->light_tree_count = desc->light_tree_vec4s;
+        if (!rc && s->glass) rc = upload(d, &s->glass_energy, desc->glass_energy, 1024 * 16, &total);
     }
     if (rc) {
         std::string keep = g_error;
@@ -1084,6 +1096,7 @@ static int job_args(const trt_render_job &j, trt::RenderArgs &a, std::string *wh
     a.energy = static_cast<const float4 *>(s->energy);
     a.light_tree = static_cast<const float4 *>(s->light_tree);
     a.light_tree_count = static_cast<uint32_t>(s->light_tree_count);
+    a.glass_energy = static_cast<const float4 *>(s->glass_energy);
     a.accum = static_cast<float4 *>(acc->buf);
     a.vertex_count = static_cast<uint32_t>(s->vertex_count);
     a.tri_count = static_cast<uint32_t>(s->tri_count);
@@ -1268,7 +1281,7 @@ static int render_batch(trt_device *d, const trt_render_job *jobs, uint32_t n, u
     table.reserve(n);
     uint64_t tiles = 0;
     uint32_t deepest = 0;
-    bool coat = false, tree = false;
+    bool coat = false, tree = false, glass = false;
     for (uint32_t k = 0; k < n; k++) {
         if (jobs[k].spp == 0) continue;
         if (!jobs[k].scene || jobs[k].scene->dev != d)
@@ -1284,16 +1297,17 @@ static int render_batch(trt_device *d, const trt_render_job *jobs, uint32_t n, u
         tiles += uint64_t((jobs[k].acc->w + 7) / 8) * ((jobs[k].acc->h + 7) / 8);
         deepest = std::max(deepest, jobs[k].scene->max_stack);
         coat = coat || jobs[k].scene->coat;
+        glass = glass || jobs[k].scene->glass;
         tree = tree || trt::from_bits(j.args.params[87]) > 0.5f;  // P.ltree.w: the job samples the light tree
         table.push_back(j);
     }
     if (tiles >= (uint64_t{1} << 31)) return fail(TRT_E_INVALID, "trt_render_batch: too many tiles in one launch");
     if (table.empty()) return ok();
-    // The wavefront takes batches whose jobs have at most kWfMaxSuns directional lights; counting launches stay on
-    // the megakernel. TRT_RENDER_AUTO takes it when every scene has kWfAutoLeaves triangles or more, or for launches
-    // of a whole chunk of paths or more.
+    // The wavefront takes batches whose jobs have at most kWfMaxSuns directional lights and no glass; counting
+    // launches stay on the megakernel. TRT_RENDER_AUTO takes it when every scene has kWfAutoLeaves triangles or more,
+    // or for launches of a whole chunk of paths or more.
     const uint32_t how = d->render.load();
-    bool wf = reads == nullptr && how != TRT_RENDER_MEGAKERNEL;
+    bool wf = reads == nullptr && how != TRT_RENDER_MEGAKERNEL && !glass;
     uint64_t paths = 0;
     bool large = true;
     std::vector<const trt_scene *> scene_of;
@@ -1359,7 +1373,7 @@ static int render_batch(trt_device *d, const trt_render_job *jobs, uint32_t n, u
         if (rc) return rc;
     } else if (trt::launch_render(static_cast<const trt::RenderJob *>(ring.dev[k]), static_cast<uint32_t>(table.size()),
                            static_cast<uint32_t>(tiles), d->counter, d->spill, threads,
-                           static_cast<uint32_t>(entries), d->guard, d->ray_counts, coat, tree, reads, grid,
+                           static_cast<uint32_t>(entries), d->guard, d->ray_counts, coat, tree, glass, reads, grid,
                            d->stream))
         return fail(TRT_E_DEVICE, "render kernel launch failed");
     TRT_HIP(hipEventRecord(ring.ev[k], d->stream));

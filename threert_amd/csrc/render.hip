@@ -8,9 +8,12 @@
 // sample_bsdf 1848-1884; env_radiance / env_pdf / sample_env 1895-1930 (constant sky); power_heuristic, luminance,
 // offset_origin 1943-1960; sky_mis / sun_disks 1963-1988; emit_mis 2009-2024; emitter_radiance /
 // sample_emitter_ris / sample_emitter 2269-2393; env_light 2431-2442; direct_light 2444-2540; emitter_draw /
-// emitter_pick_u / emitter_pick_range / lt_imp / emitter_pmf 463-618 (the light tree); the clearcoat. Not here:
-// textures, glass, media, fog, procedural geometry, local lights, live-only paths (the upload and the launch refuse
-// scenes and parameters that need them).
+// emitter_pick_u / emitter_pick_range / lt_imp / emitter_pmf 463-618 (the light tree); the clearcoat; glass (kGlass:
+// the dielectric 1542-1640 with fresnel_dielectric, lambda_ggx, glass_energy, dielectric_t, pdf_dielectric; its
+// parts of lobes, eval_bsdf_k, pdf_bsdf and sample_bsdf; shadow_tr / glass_tr / glass_at / sky_tr 846-977;
+// light_origin 2153-2156; the path loop's `through`, behind_glass and bent 4156-4465). Not here: textures, media,
+// fog, procedural geometry, local lights, glass among a shadow caster's sections, live-only paths (the upload and
+// the launch refuse scenes and parameters that need them).
 // normalize(v) = v / sqrt(v.v), inverseSqrt(x) = 1 / sqrt(x); sin / cos by Moshier's Cephes reduction and
 // polynomials in fmaf; every other operation in the reference's order, no contraction: the same bits as the C
 // restatement the tests hold this to.
@@ -178,6 +181,7 @@ struct Ctx {
     Rng g;
     bool mirror_path;
     bool sampled_spec;
+    bool sampled_trans;  // the last BSDF sample went through the surface (pathtrace.wgsl:449)
     V mis_x, mis_n;  // pathtrace.wgsl:438-439: the vertex direct_light last sampled from, for emit_mis
     Reads *cnt;      // what a counting launch's walks read (and cnt->shaded: hits shaded)
 
@@ -251,13 +255,14 @@ __device__ __forceinline__ uint32_t material_of(const Ctx &c, const TriAttr &a, 
 struct Surface {
     V p, n, ng, base, emissive;
     float roughness, metallic, specular;
-    float coat, coat_rough;  // pathtrace.wgsl:1420-1423 (0 without a coat)
+    float coat, coat_rough;     // pathtrace.wgsl:1420-1423 (0 without a coat)
+    float transmission, ior;    // pathtrace.wgsl:1413-1419 (0 and 1.5 without glass)
     bool front, card;
 };
 
 // kCoat: the launch's scenes hold a clearcoat (the reference's F_COAT pipeline flag); without it the coat terms are
-// compiled out, as the reference's shader is specialised.
-template <bool kCoat>
+// compiled out, as the reference's shader is specialised. kGlass likewise for glass (F_GLASS).
+template <bool kCoat, bool kGlass = false>
 __device__ Surface surface(const Ctx &c, V o, V d, const HitRec &h) {
     V e1, e2;
     TriAttr a = tri_attr(c, h.tri, h.inst, h.part, e1, e2);
@@ -298,6 +303,13 @@ __device__ Surface surface(const Ctx &c, V o, V d, const HitRec &h) {
         s.coat = clampf(layer.z, 0.0f, 1.0f);
         s.coat_rough = clampf(layer.w, 0.02f, 1.0f);
     }
+    s.ior = 1.5f;
+    s.transmission = 0.0f;
+    if (kGlass) {
+        const float4 layer = m[7];
+        s.transmission = clampf(layer.x, 0.0f, 1.0f);
+        s.ior = fmaxf(layer.y, 1.0001f);
+    }
     return s;
 }
 
@@ -306,7 +318,56 @@ struct Lobes {
     V diffuse, f0, f90, w, e_spec, kd;
     float a2, p_spec, p_coat, p_trans;
     float coat, coat_alpha, e_coat;
+    // the dielectric (pathtrace.wgsl:1535-1538): its share x compensation, tint, n(far) / n(view), its directional
+    // reflection albedo, and whether its transmission has a density (rough) or is a delta lobe
+    V tint;
+    float t, eta, e_glass;
+    bool rough_t;
 };
+
+// ---- the dielectric (pathtrace.wgsl:1542-1640) --------------------------------------------------------------------
+constexpr float kGlassDelta = 0.05f;
+constexpr uint32_t kGlassEMu = 16, kGlassERough = 16, kGlassEIor = 8;
+constexpr float kGlassEZmax = 0.70710678f;
+// Unpolarised Fresnel reflectance, c the cosine on the incident side, eta = n(far) / n(incident); 1 at total
+// internal reflection (Walter et al. 2007, eq. 22)
+__device__ float fresnel_dielectric(float c, float eta) {
+    const float g2 = eta * eta - 1.0f + c * c;
+    if (g2 <= 0.0f) return 1.0f;
+    const float g = sqrtf(g2);
+    const float a = (g - c) / (g + c);
+    const float b = (c * (g + c) - 1.0f) / (c * (g - c) + 1.0f);
+    return 0.5f * a * a * (1.0f + b * b);
+}
+__device__ __forceinline__ float lambda_ggx(float a2, float c) {
+    const float c2 = c * c;
+    return 0.5f * (sqrtf(1.0f + a2 * (1.0f - c2) / fmaxf(c2, 1e-8f)) - 1.0f);
+}
+// R + T of the rough interface (the glass energy table: 16 sqrt(cos) x 16 roughness x 8 sqrt((ior - 1) / (ior + 1))
+// nodes, entering then leaving, four a float4), trilinear
+__device__ __forceinline__ float glass_e_at(const Ctx &c, uint32_t i) {
+    const float4 v = c.a->glass_energy[c.ck(i / 4u, 1024, 31)];
+    const uint32_t k = i & 3u;
+    return k == 0u ? v.x : (k == 1u ? v.y : (k == 2u ? v.z : v.w));
+}
+__device__ __forceinline__ float glass_e_row(const Ctx &c, uint32_t row, float x) {
+    const uint32_t x0 = static_cast<uint32_t>(x);
+    const uint32_t x1 = min(x0 + 1u, kGlassEMu - 1u);
+    return mixf(glass_e_at(c, row * kGlassEMu + x0), glass_e_at(c, row * kGlassEMu + x1), x - static_cast<float>(x0));
+}
+__device__ float glass_energy(const Ctx &c, float nov, float roughness, float ior, bool leaving) {
+    const float x = clampf(sqrtf(nov), 0.0f, 1.0f) * static_cast<float>(kGlassEMu - 1u);
+    const float y = clampf(roughness, 0.0f, 1.0f) * static_cast<float>(kGlassERough - 1u);
+    const float z = clampf(sqrtf(fmaxf(ior - 1.0f, 0.0f) / (ior + 1.0f)) / kGlassEZmax, 0.0f, 1.0f) *
+                    static_cast<float>(kGlassEIor - 1u);
+    const uint32_t y0 = static_cast<uint32_t>(y), y1 = min(y0 + 1u, kGlassERough - 1u);
+    const uint32_t side = leaving ? kGlassEIor : 0u, zi = static_cast<uint32_t>(z);
+    const uint32_t z0 = side + zi, z1 = side + min(zi + 1u, kGlassEIor - 1u);
+    const float fy = y - static_cast<float>(y0);
+    const float e0 = mixf(glass_e_row(c, z0 * kGlassERough + y0, x), glass_e_row(c, z0 * kGlassERough + y1, x), fy);
+    const float e1 = mixf(glass_e_row(c, z1 * kGlassERough + y0, x), glass_e_row(c, z1 * kGlassERough + y1, x), fy);
+    return mixf(e0, e1, z - floorf(z));
+}
 
 __device__ __forceinline__ void energy_node(const Ctx &c, uint32_t i, float &e0, float &e1) {
     const float4 v = c.a->energy[c.ck(i >> 1, 512, 24)];
@@ -329,11 +390,16 @@ __device__ void energy_lookup(const Ctx &c, float roughness, float nov, float *e
     e[1] = mixf(mixf(a1, b1, fx), mixf(c1, d1, fx), fy);
 }
 
+template <bool kGlass = false>
 __device__ Lobes lobes(const Ctx &c, const Surface &s, float nov) {
     Lobes l;
-    const float t = 0.0f;
+    const float t = kGlass ? s.transmission * (1.0f - s.metallic) : 0.0f;
     l.diffuse = vs(vs(s.base, 1.0f - s.metallic), 1.0f - t);
-    const float f0 = 0.08f * s.specular;
+    float f0 = 0.08f * s.specular;
+    if (kGlass && t > 0.0f) {  // glass: F0 from the index of refraction (pathtrace.wgsl:1654-1658)
+        const float ri = (s.ior - 1.0f) / (s.ior + 1.0f);
+        f0 = ri * ri;
+    }
     l.f0 = v3(mixf(f0, s.base.x, s.metallic), mixf(f0, s.base.y, s.metallic), mixf(f0, s.base.z, s.metallic));
     float e[2];
     energy_lookup(c, s.roughness, nov, e);
@@ -345,6 +411,10 @@ __device__ Lobes lobes(const Ctx &c, const Surface &s, float nov) {
     l.e_spec = vmul(vadd(vs(l.f0, e[0]), v3(f90 * e[1], f90 * e[1], f90 * e[1])), l.w);
     l.f90 = v3(f90, f90, f90);
     l.kd = vsub(v3(1.0f, 1.0f, 1.0f), l.e_spec);
+    if (kGlass && t > 0.0f) {  // 1669-1673: the share that is not the dielectric keeps the specular lobe
+        l.w = vs(l.w, 1.0f - t);
+        l.e_spec = vs(l.e_spec, 1.0f - t);
+    }
     const float r = s.roughness;
     l.a2 = r * r * r * r;
     const V lw = v3(0.3f, 0.6f, 0.1f);
@@ -363,7 +433,23 @@ __device__ Lobes lobes(const Ctx &c, const Surface &s, float nov) {
         l.e_coat = s.coat * (0.04f * ec[0] + ec[1]);
         l.coat_alpha = s.coat_rough * s.coat_rough;
     }
-    const float e_coat = l.e_coat, wt = 0.0f, under = 1.0f - e_coat;
+    const float e_coat = l.e_coat, under = 1.0f - e_coat;
+    float wt = 0.0f;
+    l.t = 0.0f;
+    l.eta = 0.0f;
+    l.rough_t = false;
+    l.tint = v3(0.0f, 0.0f, 0.0f);
+    l.e_glass = 0.0f;
+    if (kGlass && t > 0.0f) {  // 1688-1697
+        l.eta = s.front ? s.ior : 1.0f / s.ior;
+        l.rough_t = s.roughness > kGlassDelta && fabsf(s.ior - 1.0f) > 1e-3f;
+        float gc = 1.0f;
+        if (l.rough_t) gc = mixf(1.0f, 1.0f / glass_energy(c, nov, s.roughness, s.ior, !s.front), comp);
+        l.t = t * under * gc;
+        l.tint = vs(s.base, l.t);
+        l.e_glass = t * fresnel_dielectric(nov, l.eta);
+        wt = under * (l.e_glass + (t - l.e_glass) * dot(s.base, lw));
+    }
     const float total = (ws + wd) * under + e_coat + wt;
     l.p_coat = e_coat / fmaxf(total, 1e-6f);
     l.p_trans = wt / fmaxf(total, 1e-6f);
@@ -381,15 +467,56 @@ __device__ __forceinline__ V f_schlick(V f0, V f90, float voh) {
     const float c = 1.0f - voh, c2 = c * c, k = c2 * c2 * c;
     return vadd(f0, vs(vsub(f90, f0), k));
 }
+__device__ __forceinline__ float g1(float a2, float nov) { return 2.0f * nov / (nov + sqrtf(a2 + (1.0f - a2) * nov * nov)); }
+// The dielectric's transmission toward l (the far side) x |n.l|, Walter 2007 eq. 21 (pathtrace.wgsl:1610-1622)
+__device__ V dielectric_t(const Surface &s, const Lobes &lb, V v, V l, float nov, float nol) {
+    if (!lb.rough_t) return v3(0.0f, 0.0f, 0.0f);
+    V h = normalize(vadd(v, vs(l, lb.eta)));
+    if (dot(h, s.n) < 0.0f) h = vneg(h);
+    const float voh = dot(v, h), loh = dot(l, h);
+    if (voh <= 0.0f || loh >= 0.0f) return v3(0.0f, 0.0f, 0.0f);
+    const float den = voh + lb.eta * loh;
+    const float g2 = 1.0f / (1.0f + lambda_ggx(lb.a2, nov) + lambda_ggx(lb.a2, nol));
+    const float f = (1.0f - fresnel_dielectric(voh, lb.eta)) * d_ggx(lb.a2, fmaxf(dot(s.n, h), 0.0f)) * g2 * voh *
+                    lb.eta * lb.eta * -loh / (nov * fmaxf(den * den, 1e-12f));
+    return vs(lb.tint, f);
+}
+// The density of l under the dielectric's sampling x its share of the samples (1626-1640)
+__device__ float pdf_dielectric(const Surface &s, const Lobes &lb, V v, V l) {
+    const float nov = fmaxf(dot(s.n, v), 1e-4f);
+    if (dot(s.n, l) > 0.0f) {
+        const V h = normalize(vadd(v, l));
+        const float voh = fmaxf(dot(v, h), 1e-6f);
+        const float f = fresnel_dielectric(lb.rough_t ? voh : nov, lb.eta);
+        return lb.p_trans * f * g1(lb.a2, nov) * d_ggx(lb.a2, fmaxf(dot(s.n, h), 0.0f)) / (4.0f * nov);
+    }
+    if (!lb.rough_t) return 0.0f;
+    V h = normalize(vadd(v, vs(l, lb.eta)));
+    if (dot(h, s.n) < 0.0f) h = vneg(h);
+    const float voh = dot(v, h), loh = dot(l, h);
+    if (voh <= 0.0f || loh >= 0.0f) return 0.0f;
+    const float den = voh + lb.eta * loh;
+    return lb.p_trans * (1.0f - fresnel_dielectric(voh, lb.eta)) * g1(lb.a2, nov) * voh *
+           d_ggx(lb.a2, fmaxf(dot(s.n, h), 0.0f)) * lb.eta * lb.eta * -loh / (nov * fmaxf(den * den, 1e-12f));
+}
+// Whether lights on both sides of the surface reach it: rough glass (1606)
+template <bool kGlass>
+__device__ __forceinline__ bool two_sided(const Lobes &lb) { return kGlass && lb.rough_t; }
+
+template <bool kGlass = false>
 __device__ V eval_bsdf_k(const Surface &s, const Lobes &lb, V v, V l, float k) {
     const float nol = dot(s.n, l);
     const float nov = fmaxf(dot(s.n, v), 1e-4f);
+    if (two_sided<kGlass>(lb) && nol < 0.0f && dot(s.ng, l) < 0.0f) return dielectric_t(s, lb, v, l, nov, -nol);
     if (nol <= 0.0f || dot(s.ng, l) <= 0.0f) return v3(0.0f, 0.0f, 0.0f);
     const V h = normalize(vadd(v, l));
     const float noh = fmaxf(dot(s.n, h), 0.0f);
     const float voh = fmaxf(dot(v, h), 0.0f);
     const float dv = d_ggx(lb.a2, noh) * vis_smith(lb.a2, nov, nol);
     V spec = vs(vmul(vs(f_schlick(lb.f0, lb.f90, voh), dv), lb.w), k);
+    // the dielectric's reflection (1747-1748): lb.t holds the coat's share already, so it joins after the coat
+    float glass = 0.0f;
+    if (kGlass && lb.t > 0.0f) glass = lb.t * fresnel_dielectric(voh, lb.eta) * dv * k;
     V diff = vmul(vs(lb.diffuse, kInvPi), lb.kd);
     if (lb.coat > 0.0f) {  // pathtrace.wgsl:1750-1755
         const float ca2 = lb.coat_alpha * lb.coat_alpha;
@@ -398,13 +525,18 @@ __device__ V eval_bsdf_k(const Surface &s, const Lobes &lb, V v, V l, float k) {
         diff = vs(diff, 1.0f - lb.e_coat);
         spec = vadd(vs(spec, 1.0f - lb.e_coat), v3(coat, coat, coat));
     }
+    if (kGlass) spec = vadd(spec, v3(glass, glass, glass));
     return vs(vadd(diff, spec), nol);
 }
-__device__ __forceinline__ V eval_bsdf(const Surface &s, const Lobes &lb, V v, V l) { return eval_bsdf_k(s, lb, v, l, 1.0f); }
-__device__ __forceinline__ float g1(float a2, float nov) { return 2.0f * nov / (nov + sqrtf(a2 + (1.0f - a2) * nov * nov)); }
+template <bool kGlass = false>
+__device__ __forceinline__ V eval_bsdf(const Surface &s, const Lobes &lb, V v, V l) {
+    return eval_bsdf_k<kGlass>(s, lb, v, l, 1.0f);
+}
+template <bool kGlass = false>
 __device__ float pdf_bsdf(const Surface &s, const Lobes &lb, V v, V l) {
     const float nol = dot(s.n, l);
-    const float pg = 0.0f;
+    float pg = 0.0f;
+    if (kGlass && lb.p_trans > 0.0f) pg = pdf_dielectric(s, lb, v, l);
     if (nol <= 0.0f) return pg;
     const float nov = fmaxf(dot(s.n, v), 1e-4f);
     const V h = normalize(vadd(v, l));
@@ -424,8 +556,9 @@ __device__ __forceinline__ float power_heuristic(float a, float b) {
     const float a2 = a * a;
     return a2 / fmaxf(a2 + b * b, 1e-20f);
 }
+template <bool kGlass = false>
 __device__ __forceinline__ float light_mis(float pdf_l, const Surface &s, const Lobes &lb, V v, V l) {
-    return power_heuristic(pdf_l, pdf_bsdf(s, lb, v, l));
+    return power_heuristic(pdf_l, pdf_bsdf<kGlass>(s, lb, v, l));
 }
 __device__ __forceinline__ M3 basis(V n) {
     const float sgn = n.z >= 0.0f ? 1.0f : -1.0f;
@@ -456,14 +589,39 @@ __device__ V sample_vndf(V v, const M3 &tbn, float alpha, float u1, float u2) {
     return mmul(tbn, hl);
 }
 __device__ __forceinline__ V reflect(V e1, V e2) { return vsub(e1, vs(e2, 2.0f * dot(e2, e1))); }
+// WGSL refract: the zero vector at total internal reflection
+__device__ __forceinline__ V refract(V e1, V e2, float eta) {
+    const float d = dot(e2, e1);
+    const float k = 1.0f - eta * eta * (1.0f - d * d);
+    if (k < 0.0f) return v3(0.0f, 0.0f, 0.0f);
+    return vsub(vs(e1, eta), vs(e2, eta * d + sqrtf(k)));
+}
 
+template <bool kGlass = false>
 __device__ V sample_bsdf(Ctx &c, const Surface &s, const Lobes &lb, V v) {
     const M3 tbn = basis(s.n);
     float r[4];
     c.g.r4(c.g.dim(D_BSDF), r);
+    c.sampled_trans = false;
     if (r[0] < lb.p_coat) {  // pathtrace.wgsl:1853-1857: the coat's lobe (p_coat is 0 without one)
         c.sampled_spec = true;
         return reflect(vneg(v), sample_vndf(v, tbn, lb.coat_alpha, r[2], r[3]));
+    }
+    if (kGlass && r[0] < lb.p_coat + lb.p_trans) {
+        // 1858-1874, the dielectric: a visible microfacet normal (smooth glass refracts through the shading normal),
+        // reflected or refracted by the Fresnel reflectance there; a reflection under the surface or a refraction
+        // above it is no sample (the zero vector)
+        const V h = sample_vndf(v, tbn, s.roughness * s.roughness, r[2], r[3]);
+        const V m = lb.rough_t ? h : s.n;
+        const V tr = refract(vneg(v), m, 1.0f / lb.eta);
+        c.sampled_spec = r[0] - lb.p_coat < fresnel_dielectric(fmaxf(dot(v, m), 1e-4f), lb.eta) * lb.p_trans ||
+                         (tr.x == 0.0f && tr.y == 0.0f && tr.z == 0.0f);
+        if (c.sampled_spec) {
+            const V rl = reflect(vneg(v), h);
+            return dot(rl, s.n) <= 0.0f ? v3(0.0f, 0.0f, 0.0f) : rl;
+        }
+        c.sampled_trans = true;
+        return dot(tr, s.n) >= 0.0f ? v3(0.0f, 0.0f, 0.0f) : tr;
     }
     c.sampled_spec = r[1] < lb.p_spec;
     if (c.sampled_spec) {
@@ -518,6 +676,71 @@ __device__ V sky_mis(const Ctx &c, V d, float pdf) {
         col = vadd(col, sum);
     }
     return col;
+}
+
+// ---- glass along shadow rays (pathtrace.wgsl:846-977) -------------------------------------------------------------
+// The share of a ray a glass material lets through at |cos| cs: transmission x (1 - the specular lobe's directional
+// albedo there, spec_terms 1521-1525 at the glass's F0); 0: not glass (glass_at 955-969)
+__device__ float glass_share(const Ctx &c, const float4 *m, float cs) {
+    const float4 params = m[2], layer = m[7];
+    const float tr = clampf(layer.x, 0.0f, 1.0f) * (1.0f - clampf(params.x, 0.0f, 1.0f));
+    if (tr <= 0.0f) return 0.0f;
+    const float ior = fmaxf(layer.y, 1.0001f);
+    const float ri = (ior - 1.0f) / (ior + 1.0f);
+    const float f0 = ri * ri;
+    float e[2];
+    energy_lookup(c, clampf(params.y, 0.02f, 1.0f), cs, e);
+    const float k503 = 50.0f / 3.0f;
+    const float f90 = clampf(dot(v3(f0, f0, f0), v3(k503, k503, k503)), 0.0f, 1.0f);
+    const float w = 1.0f + f0 * (c.pf(MISC, 2) * (1.0f / fmaxf(e[0] + e[1], 1e-3f) - 1.0f));
+    return tr * (1.0f - (f0 * e[0] + f90 * e[1]) * w);
+}
+// glass_tr (885-906): past every opaque caster, what the glass a shadow ray crosses lets through (up to 8 surfaces,
+// the ray kept straight: no refraction); toward the sky an odd number of glass surfaces lets nothing through
+template <bool kCount>
+__device__ V glass_tr(const Ctx &c, V o, V d, float tmax, bool to_sky) {
+    float left = tmax;
+    V tr = v3(1.0f, 1.0f, 1.0f);
+    bool odd = false;
+    for (uint32_t k = 0; k < 8u; k++) {
+        const HitRec h = c.trace<kCount>(o, d, left, kTraceClosest);
+        if (h.tri == kNone) break;
+        // glass_at (930-970): the hit's geometric normal, its material's share at this angle and its tint
+        V e1, e2;
+        TriAttr a = tri_attr(c, h.tri, h.inst, h.part, e1, e2);
+        float side = 1.0f;
+        if (h.inst != kNone) {
+            const float4 *r = c.inst(h.inst);
+            e1 = inst_vec(r, e1);
+            e2 = inst_vec(r, e2);
+            side = (__float_as_uint(r[4].w) & kInstMirrored) ? -1.0f : 1.0f;
+        }
+        const float4 *m = c.material(material_of(c, a, h.inst));
+        const V ng = vs(normalize(cross(e1, e2)), side);
+        const float share = glass_share(c, m, fabsf(dot(ng, d)));
+        if (share > 0.0f) {
+            tr = vmul(tr, vs(xyz(m[0]), share));
+            odd = !odd;
+        }
+        o = offset_origin(vadd(o, vs(d, h.t)), dot(ng, d) < 0.0f ? vneg(ng) : ng);
+        left -= h.t;
+        if (left <= 0.0f || all_nonpos(tr)) break;
+    }
+    if (to_sky && odd) return v3(0.0f, 0.0f, 0.0f);
+    return tr;
+}
+// shadow_tr (846-855): 0 past a shadow caster; else what the glass lets through (objects that cast no shadow are
+// glass); sky_tr (975-977) with to_sky
+template <bool kCount>
+__device__ V shadow_tr(const Ctx &c, V o, V d, float tmax, bool to_sky) {
+    if (c.trace<kCount>(o, d, tmax, kTraceShadow).tri != kNone) return v3(0.0f, 0.0f, 0.0f);
+    return glass_tr<kCount>(c, o, d, tmax, to_sky);
+}
+// light_origin (2153-2156): a shadow ray's origin on the side l leaves the surface from
+template <bool kGlass>
+__device__ __forceinline__ V light_origin(const Surface &s, V o, V l) {
+    if (kGlass && dot(l, s.ng) < 0.0f) return offset_origin(s.p, vneg(s.ng));
+    return o;
 }
 
 __device__ __forceinline__ uint32_t emitter_count(const Ctx &c) { return c.pu(COUNTS, 1); }
@@ -701,7 +924,7 @@ struct ES {
     float dist, pdf_w, w;
 };
 
-template <bool kTree>
+template <bool kTree, bool kGlass = false>
 __device__ ES sample_emitter_ris(Ctx &c, V x, V n, uint32_t m, const Surface &sf, const Lobes &lb, V v, uint32_t rd) {
     ES r{v3(0.0f, 0.0f, 0.0f), v3(0.0f, 1.0f, 0.0f), 0.0f, 0.0f, 0.0f};
     float wsum = 0.0f, pick_b1 = 0.0f, pick_b2 = 0.0f, pick_ph = 0.0f, pick_pdf = 0.0f;
@@ -727,7 +950,7 @@ __device__ ES sample_emitter_ris(Ctx &c, V x, V n, uint32_t m, const Surface &sf
         const float d2 = fmaxf(dot(dl, dl), 1e-8f);
         const V l = vs(dl, 1.0f / sqrtf(d2));
         float g = fabsf(dot(ln, l)) * (dot(n, n) == 0.0f ? 1.0f : fmaxf(dot(n, l), 0.0f)) / d2;
-        if (m > 1u) g = fabsf(dot(ln, l)) * luminance(eval_bsdf(sf, lb, v, l)) / d2;
+        if (m > 1u) g = fabsf(dot(ln, l)) * luminance(eval_bsdf<kGlass>(sf, lb, v, l)) / d2;
         const float ph = pk_w / (e.w * 1.0f) * g;
         const float pdf_a = pk_p / (0.5f * a2);
         const float w = ph / pdf_a;
@@ -809,8 +1032,11 @@ struct WfShadows {
 
 // pathtrace.wgsl direct_light: the sum of what the suns, one emitter sample and one environment sample send toward v.
 // kDefer (the wavefront): each contribution goes to `q` with its shadow ray, untraced, and the sum returned is 0.
-template <bool kTree, bool kCount, bool kDefer = false>
+// kGlass (the megakernel alone): shadow rays leave from the far side toward lights behind rough glass, the suns' and
+// the sky's pass glass at its share (shadow_tr), and rough glass samples emitters from both sides.
+template <bool kTree, bool kCount, bool kDefer = false, bool kGlass = false>
 __device__ V direct_light(Ctx &c, const Surface &s, const Lobes &lb, V v, WfShadows *q = nullptr) {
+    static_assert(!(kDefer && kGlass), "the wavefront does not render glass");
     V sum = v3(0.0f, 0.0f, 0.0f);
     const V o = offset_origin(s.p, s.ng);
     for (uint32_t i = 0; i < c.pu(COUNTS2, 0); i++) {
@@ -829,10 +1055,18 @@ __device__ V direct_light(Ctx &c, const Surface &s, const Lobes &lb, V v, WfShad
         const V l = mmul(tb, v3(sin_t * cp, sin_t * sp, cos_t));
         const V col = xyz(li[2]);
         if (all_nonpos(col)) continue;
-        const V f = eval_bsdf_k(s, lb, v, l, li[4].y);
+        const V f = eval_bsdf_k<kGlass>(s, lb, v, l, li[4].y);
         if (kDefer && any_pos(f)) {
             const float pdf_l = 1.0f / (kTwoPi * (1.0f - cos_max));
             q->put(vs(vmul(col, f), light_mis(pdf_l, s, lb, v, l)), li[3].w >= 0.5f, o, l, kInf, kTraceShadow);
+        } else if (kGlass && any_pos(f)) {
+            const V vis = li[3].w >= 0.5f ? shadow_tr<kCount>(c, light_origin<kGlass>(s, o, l), l, kInf, false)
+                                          : v3(1.0f, 1.0f, 1.0f);
+            if (any_pos(vis)) {
+                const float pdf_l = 1.0f / (kTwoPi * (1.0f - cos_max));
+                const float wgt = light_mis<kGlass>(pdf_l, s, lb, v, l);
+                sum = vadd(sum, vmul(vs(vmul(col, f), wgt), vis));
+            }
         } else if (any_pos(f)) {
             const bool vis = li[3].w >= 0.5f ? !occluded<kCount>(c, o, l, kInf, kTraceShadow) : true;
             if (vis) {
@@ -842,7 +1076,8 @@ __device__ V direct_light(Ctx &c, const Surface &s, const Lobes &lb, V v, WfShad
             }
         }
     }
-    const V ne = s.n;
+    // rough glass takes emitters from both sides, as a point without a hemisphere (2488-2491)
+    const V ne = two_sided<kGlass>(lb) ? v3(0.0f, 0.0f, 0.0f) : s.n;
     c.mis_x = o;
     c.mis_n = ne;
     if (emitters_on(c)) {
@@ -850,23 +1085,24 @@ __device__ V direct_light(Ctx &c, const Surface &s, const Lobes &lb, V v, WfShad
         const bool seen = c.g.depth == 0u || (c.g.depth == 1u && c.mirror_path);
         // with the tree its candidates are already near the point: P.ltree.w of them at camera hits, one elsewhere
         if (kTree && lt_on(c))
-            es = sample_emitter_ris<kTree>(c, o, ne, c.g.depth == 0u ? static_cast<uint32_t>(c.pf(LTREE, 3)) : 1u, s, lb, v,
-                                    c.g.dim(D_EMIT));
+            es = sample_emitter_ris<kTree, kGlass>(c, o, ne, c.g.depth == 0u ? static_cast<uint32_t>(c.pf(LTREE, 3)) : 1u,
+                                                   s, lb, v, c.g.dim(D_EMIT));
         else if (seen)
-            es = sample_emitter_ris<kTree>(c, o, ne, c.g.depth == 0u ? kEmitRisM : kEmitRisM / 2u, s, lb, v,
-                                           c.g.dim(D_EMIT));
+            es = sample_emitter_ris<kTree, kGlass>(c, o, ne, c.g.depth == 0u ? kEmitRisM : kEmitRisM / 2u, s, lb, v,
+                                                   c.g.dim(D_EMIT));
         else if (c.pf(LIGHT_LOD, 3) > 0.0f)
-            es = sample_emitter_ris<kTree>(c, o, ne, 2u, s, lb, v, c.g.dim(D_EMIT));
+            es = sample_emitter_ris<kTree, kGlass>(c, o, ne, 2u, s, lb, v, c.g.dim(D_EMIT));
         else
             es = sample_emitter<kTree>(c, o, ne, c.g.dim(D_EMIT));
         if (es.pdf_w > 0.0f) {
-            const V f = eval_bsdf(s, lb, v, es.l);
+            const V f = eval_bsdf<kGlass>(s, lb, v, es.l);
             if (kDefer && any_pos(f)) {
                 q->put(vs(vs(vmul(es.le, f), light_mis(es.pdf_w, s, lb, v, es.l)), es.w), true, o, es.l,
                        es.dist * 0.999f, kTraceAny);
             } else if (any_pos(f)) {
-                if (!occluded<kCount>(c, o, es.l, es.dist * 0.999f, kTraceAny)) {
-                    const float wgt = light_mis(es.pdf_w, s, lb, v, es.l);
+                // emitter_tr (1066-1070): any surface stops it, glass too
+                if (!occluded<kCount>(c, light_origin<kGlass>(s, o, es.l), es.l, es.dist * 0.999f, kTraceAny)) {
+                    const float wgt = light_mis<kGlass>(es.pdf_w, s, lb, v, es.l);
                     sum = vadd(sum, vs(vs(vmul(es.le, f), wgt), es.w));
                 }
             }
@@ -874,11 +1110,18 @@ __device__ V direct_light(Ctx &c, const Surface &s, const Lobes &lb, V v, WfShad
     }
     if (env_sampled(c)) {
         const V l = sample_env(c, c.g.dim(D_ENV));
-        const V f = eval_bsdf(s, lb, v, l);
+        const V f = eval_bsdf<kGlass>(s, lb, v, l);
         if (kDefer && any_pos(f)) {
             const float pdf_e = env_pdf();
             if (pdf_e > 0.0f)
                 q->put(vds(vs(vmul(sky(c), f), light_mis(pdf_e, s, lb, v, l)), pdf_e), true, o, l, kInf, kTraceShadow);
+        } else if (kGlass && any_pos(f)) {
+            const float pdf_e = env_pdf();
+            if (pdf_e > 0.0f) {
+                const V vis = shadow_tr<kCount>(c, light_origin<kGlass>(s, o, l), l, kInf, true);
+                if (any_pos(vis))
+                    sum = vadd(sum, vmul(vds(vs(vmul(sky(c), f), light_mis<kGlass>(pdf_e, s, lb, v, l)), pdf_e), vis));
+            }
         } else if (any_pos(f)) {
             const float pdf_e = env_pdf();
             if (pdf_e > 0.0f && !occluded<kCount>(c, o, l, kInf, kTraceShadow))
@@ -941,7 +1184,7 @@ __device__ __forceinline__ void camera_ray(Ctx &c, uint32_t px, uint32_t py, uin
 }
 
 // One path of sample `index` at pixel (px, py): its radiance, the camera hit's albedo and normal.
-template <bool kCoat, bool kTree, bool kCount>
+template <bool kCoat, bool kTree, bool kCount, bool kGlass>
 __device__ void path(Ctx &c, uint32_t px, uint32_t py, uint32_t index, V &radiance, V &albedo, V &normal) {
     V o, d;
     camera_ray(c, px, py, index, o, d);
@@ -950,6 +1193,8 @@ __device__ void path(Ctx &c, uint32_t px, uint32_t py, uint32_t index, V &radian
     normal = v3(0.0f, 0.0f, 0.0f);
     V thr = v3(1.0f, 1.0f, 1.0f);
     bool specular_bounce = true, straight = true;
+    // glass passed since the last scattering vertex (pathtrace.wgsl:4156-4162): any, and an odd number of surfaces
+    bool behind_glass = false, bent = false;
     float last_pdf = 0.0f;
     const float clamp_v = c.pf(MISC, 0);
     c.mirror_path = false;
@@ -958,12 +1203,12 @@ __device__ void path(Ctx &c, uint32_t px, uint32_t py, uint32_t index, V &radian
         c.g.depth = depth;
         const HitRec hit = c.trace<kCount>(o, d, kInf, kTraceClosest);
         if (hit.tri == kNone) {
-            V col = vmul(thr, sky_mis(c, d, depth > 0u ? last_pdf : 0.0f));
+            V col = vmul(thr, sky_mis(c, d, (depth > 0u && !(kGlass && bent)) ? last_pdf : 0.0f));
             if (depth > 1u && !straight) col = clamp_lum(col, clamp_v);
             radiance = vadd(radiance, col);
             break;
         }
-        const Surface s = surface<kCoat>(c, o, d, hit);
+        const Surface s = surface<kCoat, kGlass>(c, o, d, hit);
         if (kCount) c.cnt->shaded++;
         if (depth == 0u) {
             if (s.card) {
@@ -976,7 +1221,7 @@ __device__ void path(Ctx &c, uint32_t px, uint32_t py, uint32_t index, V &radian
         }
         if (any_pos(s.emissive) && (s.front || !s.card)) {
             float wgt = 1.0f;
-            if (depth > 0u && !specular_bounce) wgt = emit_mis<kTree>(c, hit, d, last_pdf);
+            if (depth > 0u && !specular_bounce && !(kGlass && behind_glass)) wgt = emit_mis<kTree>(c, hit, d, last_pdf);
             V col = vs(vmul(thr, s.emissive), wgt);
             if (depth > 1u && !straight) col = clamp_lum(col, clamp_v);
             radiance = vadd(radiance, col);
@@ -985,19 +1230,43 @@ __device__ void path(Ctx &c, uint32_t px, uint32_t py, uint32_t index, V &radian
         if (depth == last) break;
         const V v = vneg(d);
         const float nov = fmaxf(dot(s.n, v), 1e-4f);
-        const Lobes lb = lobes(c, s, nov);
-        V dl = vmul(thr, direct_light<kTree, kCount>(c, s, lb, v));
+        const Lobes lb = lobes<kGlass>(c, s, nov);
+        V dl = vmul(thr, direct_light<kTree, kCount, false, kGlass>(c, s, lb, v));
         if (depth > 0u && !straight) dl = clamp_lum(dl, clamp_v);
         radiance = vadd(radiance, dl);
-        const V l = sample_bsdf(c, s, lb, v);
-        const float pdf = pdf_bsdf(s, lb, v, l);
-        const V wgt = vds(eval_bsdf(s, lb, v, l), fmaxf(pdf, 1e-20f));
-        if (pdf <= 0.0f || all_nonpos(wgt)) break;
-        thr = vmul(thr, wgt);
-        last_pdf = pdf;
-        specular_bounce = false;
-        straight = false;
-        c.mirror_path = (depth == 0u || c.mirror_path) && s.roughness < kMirrorRoughness && c.sampled_spec;
+        const V l = sample_bsdf<kGlass>(c, s, lb, v);
+        if (kGlass) {
+            // 4414-4465: through smooth glass a delta lobe, weight tint / its sampling share, no MIS
+            const bool through = c.sampled_trans && !lb.rough_t;
+            float pdf = 0.0f;
+            V wgt = vds(lb.tint, fmaxf(lb.p_trans, 1e-6f));
+            if (!through) {
+                pdf = pdf_bsdf<kGlass>(s, lb, v, l);
+                wgt = vds(eval_bsdf<kGlass>(s, lb, v, l), fmaxf(pdf, 1e-20f));
+            }
+            if ((!through && pdf <= 0.0f) || all_nonpos(wgt)) break;
+            thr = vmul(thr, wgt);
+            if (through && dot(l, s.ng) < 0.0f) {
+                behind_glass = true;
+                bent = !bent;
+            } else {
+                last_pdf = through ? 0.0f : pdf;
+                specular_bounce = through;
+                behind_glass = false;
+                bent = false;
+            }
+            straight = straight && through;
+        } else {
+            const float pdf = pdf_bsdf(s, lb, v, l);
+            const V wgt = vds(eval_bsdf(s, lb, v, l), fmaxf(pdf, 1e-20f));
+            if (pdf <= 0.0f || all_nonpos(wgt)) break;
+            thr = vmul(thr, wgt);
+            last_pdf = pdf;
+            specular_bounce = false;
+            straight = false;
+        }
+        c.mirror_path = (depth == 0u || c.mirror_path) && s.roughness < kMirrorRoughness &&
+                        (c.sampled_spec || (kGlass && c.sampled_trans));
         if (depth >= c.pu(COUNTS, 3)) {
             const float q = fminf(fmaxf(thr.x, fmaxf(thr.y, thr.z)), 0.95f);
             if (c.g.r1(c.g.dim(D_RR)) > q) break;
@@ -1019,7 +1288,7 @@ __device__ void path(Ctx &c, uint32_t px, uint32_t py, uint32_t index, V &radian
 // frame 2.46 -> 1.95 ms, the 3M-triangle scene 1.36x faster, the Cornell box 5 % (four waves: 2.00 ms, 1.34x, 11 %).
 // kCount: a counting launch, which adds what its walks read and the hits it shades to reads[0..3] (nodes,
 // triangles, units, shaded hits).
-template <bool kCoat, bool kTree, bool kCount>
+template <bool kCoat, bool kTree, bool kCount, bool kGlass>
 __global__ __launch_bounds__(kRenderBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_render(
     const RenderJob *jobs, uint32_t njobs, uint32_t total, uint32_t *counter, uint32_t *spill, uint32_t spill_stride,
     uint32_t spill_entries, uint32_t *guard, unsigned long long *rays, unsigned long long *reads) {
@@ -1049,14 +1318,14 @@ __global__ __launch_bounds__(kRenderBlock) __attribute__((amdgpu_waves_per_eu(3)
         const RenderArgs *a = &job->args;
         const uint32_t t = tile - job->tile_first;
         const uint32_t w = a->params[16], h = a->params[17], tiles_x = (w + 7) / 8;
-        Ctx c{a, &st, &flags, traced, Rng{0, 0, 0}, false, false, v3(0.0f, 0.0f, 0.0f), v3(0.0f, 0.0f, 0.0f), &cnt};
+        Ctx c{a, &st, &flags, traced, Rng{0, 0, 0}, false, false, false, v3(0.0f, 0.0f, 0.0f), v3(0.0f, 0.0f, 0.0f), &cnt};
         const uint32_t px = (t % tiles_x) * 8 + (lane & 7u), py = (t / tiles_x) * 8 + (lane >> 3);
         if (px < w && py < h) {
             float4 *pix = a->accum + 5 * (uint64_t(py) * w + px);
             float4 col = pix[0], alb = pix[1], nrm = pix[2];
             for (uint32_t s = 0; s < a->spp; s++) {
                 V rad, al, nm;
-                path<kCoat, kTree, kCount>(c, px, py, a->first_sample + s, rad, al, nm);
+                path<kCoat, kTree, kCount, kGlass>(c, px, py, a->first_sample + s, rad, al, nm);
                 const float lum = luminance(rad);
                 col.x += rad.x, col.y += rad.y, col.z += rad.z, col.w += 1.0f;
                 alb.x += al.x, alb.y += al.y, alb.z += al.z, alb.w += lum;
@@ -1123,7 +1392,7 @@ __device__ __forceinline__ WfPath wf_path(const WfArgs &a, uint32_t p) {
     return {s.job, pixel % s.width, pixel / s.width, s.sample_first + off / s.count};
 }
 __device__ __forceinline__ Ctx wf_ctx(const RenderArgs *ra, uint32_t *flags, uint32_t *traced, Reads *cnt) {
-    return Ctx{ra, nullptr, flags, traced, Rng{0, 0, 0}, false, false, v3(0.0f, 0.0f, 0.0f), v3(0.0f, 0.0f, 0.0f),
+    return Ctx{ra, nullptr, flags, traced, Rng{0, 0, 0}, false, false, false, v3(0.0f, 0.0f, 0.0f), v3(0.0f, 0.0f, 0.0f),
                cnt};
 }
 // The wave's lanes take their own jobs one job at a time, so that a job's arguments are the same for the whole wave
@@ -1414,7 +1683,8 @@ __global__ void k_reset(uint32_t *c) { c[threadIdx.x * kShardGap] = 0; }
 
 int render_occupancy(int *blocks_per_cu) {
     int b = 0;
-    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_render<false, false, false>, kRenderBlock, 0);
+    const hipError_t e =
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_render<false, false, false, false>, kRenderBlock, 0);
     *blocks_per_cu = b;
     return e == hipSuccess ? 0 : -1;
 }
@@ -1450,18 +1720,29 @@ int launch_wf_accum(const WfArgs &a, uint32_t pixels, int grid, hipStream_t s) {
 
 int launch_render(const RenderJob *d_jobs, uint32_t njobs, uint32_t total_tiles, uint32_t *counter,
                   uint32_t *spill, uint32_t spill_stride, uint32_t spill_entries, uint32_t *guard,
-                  unsigned long long *rays, bool coat, bool tree, unsigned long long *reads, int grid,
+                  unsigned long long *rays, bool coat, bool tree, bool glass, unsigned long long *reads, int grid,
                   hipStream_t s) {
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(kShards), 0, s, counter);
-#define TRT_RENDER(C, T, N)                                                                                          \
-    hipLaunchKernelGGL((k_render<C, T, N>), dim3(grid), dim3(kRenderBlock), 0, s, d_jobs, njobs, total_tiles,        \
+#define TRT_RENDER(C, T, N, G)                                                                                       \
+    hipLaunchKernelGGL((k_render<C, T, N, G>), dim3(grid), dim3(kRenderBlock), 0, s, d_jobs, njobs, total_tiles,     \
                        counter, spill, spill_stride, spill_entries, guard, rays, reads)
-#define TRT_RENDER_N(C, T)          \
-    if (reads)                      \
-        TRT_RENDER(C, T, true);     \
-    else                            \
-        TRT_RENDER(C, T, false)
-    if (coat && tree) {
+#define TRT_RENDER_N(C, T)            \
+    if (reads)                        \
+        TRT_RENDER(C, T, true, false); \
+    else                              \
+        TRT_RENDER(C, T, false, false)
+    // glass scenes take the kernel with the coat's terms as well (the same bits for materials without a coat)
+    if (glass && tree) {
+        if (reads)
+            TRT_RENDER(true, true, true, true);
+        else
+            TRT_RENDER(true, true, false, true);
+    } else if (glass) {
+        if (reads)
+            TRT_RENDER(true, false, true, true);
+        else
+            TRT_RENDER(true, false, false, true);
+    } else if (coat && tree) {
         TRT_RENDER_N(true, true);
     } else if (coat) {
         TRT_RENDER_N(true, false);

@@ -333,8 +333,12 @@ bool plan_of(const Node *nodes, uint64_t node_count, const Instance *inst, uint6
     std::unordered_map<uint64_t, uint32_t> index;  // (root, first) -> mesh
     for (uint32_t r : p.records) {
         const Instance &u = inst[r];
-        if (to_bits(u.n[1][3]) & (kInstCutout | kInstGlass)) {
-            *why = "records with cutout or glass sections are not supported";
+        // A record whose sections are all glass casts no shadow (scene.rs:127 rows_cast) and its walks need no row
+        // flags; glass among a shadow caster's sections (the reference's F_IGLASS) and cutouts do.
+        const uint32_t bits = to_bits(u.n[1][3]);
+        if ((bits & kInstCutout) || ((bits & kInstGlass) && (bits & kUnitShadow))) {
+            *why = (bits & kInstCutout) ? "records with cutout sections are not supported"
+                                        : "records with glass among shadow-casting sections are not supported";
             return false;
         }
         const uint32_t root = to_bits(u.n[0][3]);

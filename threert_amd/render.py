@@ -30,12 +30,15 @@ NO_EMITTER = 0xFFFFFFFF
 EMITTER_GUIDE = 65536
 RIS_OFFLINE = 6  # ltree.rs:45: light-tree candidates resampled at camera hits
 OBJECT_STRIDE = 3
+INST_GLASS = 32  # scene.rs:263: an instance with glass rows
+ROW_GLASS = 1 << 25  # scene.rs:103: a row's class bit for glass (mesh_class 2)
 
 
 def material(base=(0.8, 0.8, 0.8), emissive=(0.0, 0.0, 0.0), metallic=0.0, roughness=1.0, specular=0.5,
-             clearcoat=0.0, clearcoat_roughness=0.0) -> np.ndarray:
+             clearcoat=0.0, clearcoat_roughness=0.0, transmission=0.0, ior=1.5) -> np.ndarray:
     """scene.rs:480-491 GpuMaterial (72 words): base colour, emission, (metallic, roughness, specular, normal scale),
-    no textures; the layer's clearcoat and its roughness (scene.rs:1257, both clamped to [0, 1]), no glass."""
+    no textures; the layer (scene.rs:1257): transmission (glass, clamped to [0, 1]), index of refraction, clearcoat
+    and its roughness (both clamped to [0, 1])."""
     m = np.zeros(72, np.float32)
     m[0:3] = base
     m[3] = 1.0
@@ -43,7 +46,8 @@ def material(base=(0.8, 0.8, 0.8), emissive=(0.0, 0.0, 0.0), metallic=0.0, rough
     m[8:12] = (metallic, roughness, specular, 1.0)
     m[12:20] = np.array([-1, -1, -1, -1, -1, -1, 0, 0], np.int32).view(np.float32)
     m[20:24] = (1.0, 1.0, 0.0, 0.0)
-    m[28:32] = (0.0, 1.5, min(max(clearcoat, 0.0), 1.0), min(max(clearcoat_roughness, 0.0), 1.0))
+    m[28:32] = (min(max(transmission, 0.0), 1.0), ior, min(max(clearcoat, 0.0), 1.0),
+                min(max(clearcoat_roughness, 0.0), 1.0))
     m[64:72] = np.full(8, -1, np.int32).view(np.float32)
     return m
 
@@ -116,6 +120,88 @@ def energy_lut() -> np.ndarray:
     return _LUT
 
 
+# ---- lib.rs:1186-1254 glass_energy / glass_energy_lut, in f64 ---------------------------------------------------
+GLASS_E = (16, 16, 8)                 # nodes over sqrt(cos), roughness, sqrt((ior - 1) / (ior + 1)) (pathtrace.wgsl)
+GLASS_E_ZMAX = 1.0 / math.sqrt(2.0)   # ior 3
+
+
+def fresnel_dielectric(c, eta):
+    """Unpolarised Fresnel reflectance, `c` the cosine on the incident side, `eta` = n_t / n_i; 1 at total internal
+    reflection (Walter et al. 2007, eq. 22)."""
+    c = np.asarray(c, np.float64)
+    g2 = eta * eta - 1.0 + c * c
+    g = np.sqrt(np.maximum(g2, 0.0))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        a = (g - c) / (g + c)
+        b = (c * (g + c) - 1.0) / (c * (g - c) + 1.0)
+        f = 0.5 * a * a * (1.0 + b * b)
+    return np.where(g2 <= 0.0, 1.0, f)
+
+
+def glass_energy(mu: float, alpha: float, eta: float, n: int = 24) -> float:
+    """R + T of a rough dielectric interface (GGX, single scattering) toward a view at cosine `mu`, without the tint:
+    the mean over n x n stratified visible normals (Heitz 2018) of G2 / G1 for the ray reflected or refracted by the
+    Fresnel reflectance at the normal; reflection with the kernel's approximate joint visibility, refraction with the
+    height-correlated Smith term (Walter eq. 40). `alpha` = roughness^2, `eta` = n(far) / n(view)."""
+    a2 = alpha * alpha
+
+    def lam(c):
+        c = np.minimum(np.abs(c), 1.0)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            return 0.5 * (-1.0 + np.sqrt(1.0 + a2 * (1.0 - c * c) / (c * c)))
+
+    v0, v2 = math.sqrt(max(1.0 - mu * mu, 0.0)), mu
+    k = math.sqrt(alpha * alpha * v0 * v0 + v2 * v2)
+    vh0, vh2 = alpha * v0 / k, v2 / k
+    t20, t22 = -vh2, vh0
+    lv, s = float(lam(mu)), 0.5 * (1.0 + vh2)
+    i = np.arange(n, dtype=np.float64)
+    r = np.sqrt((i + 0.5) / n)[:, None]
+    phi = (2.0 * math.pi * (i + 0.5) / n)[None, :]
+    p1 = r * np.cos(phi)
+    p2 = (1.0 - s) * np.sqrt(np.maximum(1.0 - p1 * p1, 0.0)) + s * r * np.sin(phi)
+    pz = np.sqrt(np.maximum(1.0 - p1 * p1 - p2 * p2, 0.0))
+    n0, n1, n2 = p2 * t20 + pz * vh0, p1, p2 * t22 + pz * vh2
+    m0, m1, m2 = alpha * n0, alpha * n1, np.maximum(n2, 0.0)
+    ln = np.sqrt(m0 * m0 + m1 * m1 + m2 * m2)
+    m0, m1, m2 = m0 / ln, m1 / ln, m2 / ln
+    vm = v0 * m0 + v2 * m2
+    f = fresnel_dielectric(vm, eta)
+    nol = 2.0 * vm * m2 - v2
+    g2r = 2.0 * mu * nol / np.maximum(nol * (mu * (1.0 - alpha) + alpha) + mu * (nol * (1.0 - alpha) + alpha), 1e-7)
+    refl = np.where(nol > 0.0, f * g2r * (1.0 + lv), 0.0)
+    c2 = 1.0 - (1.0 - vm * vm) / (eta * eta)
+    lz = (vm / eta - np.sqrt(np.maximum(c2, 0.0))) * m2 - v2 / eta
+    ok = (f < 1.0) & (c2 > 0.0) & (lz < 0.0)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        tr = np.where(ok, (1.0 - f) * (1.0 + lv) / (1.0 + lv + lam(lz)), 0.0)
+    return float((refl + tr).sum() / (n * n))
+
+
+_GLASS_LUT = None
+
+
+def glass_energy_lut() -> np.ndarray:
+    """glass_energy over the table's nodes, entering then leaving (ior, then its inverse), four values a vec4: the
+    kernel's glass energy table (1024 vec4)."""
+    global _GLASS_LUT
+    if _GLASS_LUT is None:
+        nm, nr, nz = GLASS_E
+        vals = []
+        for side in range(2):
+            for iz in range(nz):
+                z = max(iz / (nz - 1) * GLASS_E_ZMAX, 0.02)
+                ior = (1.0 + z * z) / (1.0 - z * z)
+                for ir in range(nr):
+                    r = max(ir / (nr - 1), 0.02)
+                    for im in range(nm):
+                        y = im / (nm - 1)
+                        e = glass_energy(max(y * y, 1e-3), r * r, ior if side == 0 else 1.0 / ior, 24)
+                        vals.append(min(max(e, 0.05), 1.0))
+        _GLASS_LUT = np.array(vals, np.float32).reshape(-1, 4)
+    return _GLASS_LUT
+
+
 # ---- the render scene -------------------------------------------------------------------------------------------
 @dataclass
 class RMesh:
@@ -159,11 +245,14 @@ class RenderScene:
     # of the emitters' weight outside it (the table header's y)
     light_tree: np.ndarray = field(default_factory=lambda: np.zeros((0, 4), np.float32))
     tree_entries: int = 0
+    # lib.rs glass_energy_lut (1024 vec4) when a material has glass (gpu.rs features_of: layer.x > 0); None: no glass
+    glass_energy: np.ndarray | None = None
 
     def tables(self) -> dict:
         return {"vertices": self.vertices, "indices": self.indices, "rows": self.rows, "materials": self.materials,
                 "lights": self.lights, "emitters": self.emitters, "emitter_tris": self.emitter_tris,
-                "objects": self.objects, "energy": self.energy, "light_tree": self.light_tree}
+                "objects": self.objects, "energy": self.energy, "light_tree": self.light_tree,
+                "glass_energy": self.glass_energy}
 
     def params(self, width: int, height: int, camera: Camera, seed: int = 1, max_bounces: int = 4, rr_depth: int = 3,
                clamp: float = 1e9, energy_compensation: float = 0.0, sky=(0.0, 0.0, 0.0), env_nee: bool = True,
@@ -209,10 +298,20 @@ def build(meshes: list[RMesh], instances: list[RInstance], materials: list[np.nd
           lights: list[np.ndarray] = (), light_tree: bool = True) -> RenderScene:
     """Lays out a scene as prepare does: trees and records (scene.prepare), then vertices, indices, rows, emitters.
     With 256 emitters or more that weigh something, a light tree over them (lib.rs:650-676, on unless THREERT_LTREE=0:
-    here `light_tree=False`), whose emitters come first in the CDF in its leaf order."""
-    placements = [S.Placement(i.mesh, i.world, i.cast_shadow, 0) for i in instances]
-    prep = S.prepare([S.Mesh(m.tris, m.sections) for m in meshes], placements)
+    here `light_tree=False`), whose emitters come first in the CDF in its leaf order. Glass (a material with
+    transmission) is a class of its rows (scene.rs:2472 mesh_class): an instance whose sections are all glass casts no
+    shadow (scene.rs:127 rows_cast) and carries INST_GLASS (rows_bits); glass beside other sections of a shadow-casting
+    instance (the reference's F_IGLASS) is refused."""
     mats = np.stack(materials).astype(np.float32)
+    placements = []
+    for k, i in enumerate(instances):
+        glass = [bool(mats[m, 28] > 0.0) for m in i.materials]
+        cast = i.cast_shadow and not all(glass)
+        if any(glass) and cast:
+            raise ValueError(f"instance {k}: glass beside other sections of a shadow-casting instance is not "
+                             "supported (the reference's F_IGLASS)")
+        placements.append(S.Placement(i.mesh, i.world, cast, 0, INST_GLASS if any(glass) else 0))
+    prep = S.prepare([S.Mesh(m.tris, m.sections) for m in meshes], placements)
     # vertices: three a triangle, in the mesh's original order; indices in the tree order
     verts, vfirst, idx_parts = [], [], []
     at = 0
@@ -276,7 +375,7 @@ def build(meshes: list[RMesh], instances: list[RInstance], materials: list[np.nd
                 entries.append((a32 * np.float32(lum(mats[mi][4:7])), a32))
                 centres.append((p32[:, 0] + p32[:, 1] + p32[:, 2]) / np.float32(3.0))  # lib.rs:210 placed_emitter
                 n_em_so_far += len(members)
-            rows.append((mi, 0, first_em, 0))
+            rows.append((mi | (ROW_GLASS if mats[mi, 28] > 0.0 else 0), 0, first_em, 0))
     rows = np.array(rows, np.uint32).reshape(-1, 4)
     n_em = n_em_so_far
     emitter_tris = (np.concatenate(em_tris).view(np.float32).reshape(-1, 16) if em_tris
@@ -331,6 +430,8 @@ def build(meshes: list[RMesh], instances: list[RInstance], materials: list[np.nd
     if tree is not None:
         rs.light_tree = tree.nodes.astype(np.float32)
         rs.tree_entries = len(tree.order)
+    if np.any(mats[:, 28] > 0.0):
+        rs.glass_energy = glass_energy_lut()
     return rs
 
 
@@ -341,15 +442,18 @@ from . import _SceneDesc as _RenderDesc  # noqa: E402  (trt_scene_desc with its 
 def upload(dev: Device, rs: RenderScene) -> Scene:
     """gpu.rs:2613 Gpu::upload with the shading tables."""
     p = rs.prep
+    glass = np.zeros((0, 4), np.float32) if rs.glass_energy is None else rs.glass_energy
     keep = [np.ascontiguousarray(x) for x in (p.nodes, p.leaves, p.instances, rs.vertices, rs.indices, rs.rows,
                                               rs.materials, rs.lights, rs.emitters, rs.emitter_tris, rs.objects,
-                                              rs.energy, rs.light_tree)]
-    nodes, leaves, inst, verts, idx, rows, mats, lights, ems, emt, objs, energy, ltree = keep
+                                              rs.energy, rs.light_tree, np.asarray(glass, np.float32))]
+    nodes, leaves, inst, verts, idx, rows, mats, lights, ems, emt, objs, energy, ltree, gl = keep
+    if gl.size and gl.shape != (1024, 4):
+        raise ValueError("glass_energy must be the (1024, 4) float32 table")
     null = lambda a: _ptr(a) if a.size else None  # noqa: E731
     desc = _RenderDesc(_ptr(nodes), len(nodes), _ptr(leaves), len(leaves), _ptr(inst), len(inst), p.top_first,
                        p.units, null(verts), len(verts), null(idx), len(idx), null(rows), len(rows), null(mats),
                        len(mats), null(lights), len(lights), null(ems), len(ems), null(emt), len(emt), null(objs),
-                       len(objs) // OBJECT_STRIDE, _ptr(energy), null(ltree), len(ltree))
+                       len(objs) // OBJECT_STRIDE, _ptr(energy), null(ltree), len(ltree), null(gl))
     sc = Scene.__new__(Scene)
     sc.dev = dev
     h = C.c_void_p()

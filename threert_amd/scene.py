@@ -29,6 +29,7 @@ class Placement:
     world: np.ndarray                # (4, 4) float32, row-major: p_world = world @ (p, 1)
     cast_shadow: bool = True
     object: int = 0
+    bits: int = 0                    # scene.rs Instance::bits: INST_CUTOUT (16) | INST_GLASS (32) of its rows
 
 
 @dataclass
@@ -69,7 +70,7 @@ def section_rank(sections: np.ndarray) -> np.ndarray:
     return rank
 
 
-def instance_record(world: np.ndarray, first: int, obj: int = 0) -> np.ndarray:
+def instance_record(world: np.ndarray, first: int, obj: int = 0, bits: int = 0) -> np.ndarray:
     """scene.rs:239-246 Instance::gpu: m rows, normal rows (M3^-1)^T with the bits words, info, p rows.
     (The inverse is taken in f64 and rounded; glam's f32 inverse is not reproduced.)"""
     w = np.asarray(world, np.float64)
@@ -78,7 +79,7 @@ def instance_record(world: np.ndarray, first: int, obj: int = 0) -> np.ndarray:
     nm = np.linalg.inv(w[:3, :3]).T
     r["n"][0, :, :3] = nm.astype(np.float32)
     mirrored = 4 if np.linalg.det(w[:3, :3]) < 0 else 0
-    r["n"][0, 1, 3] = np.array([mirrored], np.uint32).view(np.float32)[0]
+    r["n"][0, 1, 3] = np.array([mirrored | (bits & (16 | 32))], np.uint32).view(np.float32)[0]
     r["info"][0] = (first, obj, 0, 0)
     r["p"][0] = r["m"][0]
     return r
@@ -112,7 +113,7 @@ def prepare(meshes: list[Mesh], placements: list[Placement], mesh_room: int | No
     instances = np.zeros(max(1, units), INSTANCE)
     boxes = np.zeros((units, 6), np.float32)
     for k, p in enumerate(placements):
-        rec = instance_record(p.world, firsts[p.mesh], p.object)
+        rec = instance_record(p.world, firsts[p.mesh], p.object, p.bits)
         unit_record(rec, roots[p.mesh], p.cast_shadow)
         instances[k] = rec[0]
         t = trees[p.mesh]
