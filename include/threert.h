@@ -155,6 +155,16 @@ typedef struct {
     const void *glass_energy; /* lib.rs:1254 glass_energy_lut: 1024 vec4, R + T of the rough dielectric over 16 sqrt(cos)
                                * x 16 roughness x 8 sqrt((ior - 1) / (ior + 1)) nodes, entering then leaving. Needed
                                * (and read) only when a material has glass (layer.x > 0, the reference's F_GLASS) */
+    /* texpack.rs Packed (gpu.rs:2433-2446 uploads it as an Rgba8Unorm 2D array of 5 levels): tex_layers layers of
+     * RGBA8 texels, each layer its levels 0..4 one after the other (level l is tex_side >> l square, rows top to
+     * bottom), layer after layer; tex_side a power of two >= 16. tex_info: Atlas::info, two vec4 a texture ((u0, v0,
+     * du, dv) of its rect in page UV; (layer, wrap, width, height)). NULL tex_info: no textures (the reference's F_TEX
+     * off); then no material may name a texture slot */
+    const void *tex_pages;
+    uint32_t tex_side;
+    uint32_t tex_layers;
+    const void *tex_info;
+    uint64_t tex_info_vec4s;
 } trt_scene_desc;
 
 typedef struct {

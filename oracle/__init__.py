@@ -275,14 +275,17 @@ class _RenderScene(C.Structure):
                 ("light_count", C.c_uint32), ("emitters", C.c_void_p), ("emitter_vec4s", C.c_uint64),
                 ("emitter_tris", C.c_void_p), ("emitter_tri_count", C.c_uint64), ("objects", C.c_void_p),
                 ("object_count", C.c_uint64), ("energy", C.c_void_p), ("light_tree", C.c_void_p),
-                ("light_tree_count", C.c_uint64), ("glass_energy", C.c_void_p)]
+                ("light_tree_count", C.c_uint64), ("glass_energy", C.c_void_p), ("tex_pages", C.c_void_p),
+                ("tex_side", C.c_uint32), ("tex_layers", C.c_uint32), ("tex_info", C.c_void_p),
+                ("tex_info_vec4s", C.c_uint64)]
 
 
 class RenderScene:
     """The kernel's buffers plus the shading tables, as the oracle's path loop reads them."""
 
     def __init__(self, nodes, leaves, instances, top, units, vertices, indices, rows, materials, lights, emitters,
-                 emitter_tris, objects, energy, light_tree=None, glass_energy=None):
+                 emitter_tris, objects, energy, light_tree=None, glass_energy=None, tex_pages=None, tex_side=0,
+                 tex_layers=0, tex_info=None):
         self.geo = Scene(nodes, leaves, instances, top, units)
         if light_tree is None:
             light_tree = np.zeros((0, 4), np.float32)
@@ -293,10 +296,20 @@ class RenderScene:
         self.keep = [np.ascontiguousarray(x) for x in (vertices, indices, rows, materials, lights, emitters,
                                                        emitter_tris, objects, energy, light_tree)]
         self.keep.append(np.ascontiguousarray(glass_energy, np.float32))
-        v, i, r, m, li, e, et, o, en, lt, ge = self.keep
+        if tex_info is None:
+            tex_pages, tex_info = np.zeros(0, np.uint8), np.zeros((0, 4), np.float32)
+        else:
+            layer = sum((tex_side >> k) ** 2 * 4 for k in range(5))
+            if np.asarray(tex_pages).nbytes != layer * tex_layers:
+                raise ValueError("tex_pages must hold tex_layers layers of 5 RGBA8 levels of tex_side >> level")
+        self.keep += [np.ascontiguousarray(tex_pages, np.uint8).reshape(-1),
+                      np.ascontiguousarray(tex_info, np.float32).reshape(-1, 4)]
+        v, i, r, m, li, e, et, o, en, lt, ge, tp, ti = self.keep
         self._s = _RenderScene(self.geo._s, _ptr(v), len(v), _ptr(i), len(i), _ptr(r), len(r), _ptr(m), len(m),
                                _ptr(li), len(li), _ptr(e), len(e), _ptr(et), len(et), _ptr(o), len(o) // 3,
-                               _ptr(en), _ptr(lt) if len(lt) else None, len(lt), _ptr(ge) if len(ge) else None)
+                               _ptr(en), _ptr(lt) if len(lt) else None, len(lt), _ptr(ge) if len(ge) else None,
+                               _ptr(tp) if len(ti) else None, tex_side, tex_layers, _ptr(ti) if len(ti) else None,
+                               len(ti))
         L = lib()
         L.or_render_sample.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_int]
         L.or_render_sample.restype = C.c_int64
@@ -309,6 +322,8 @@ class RenderScene:
         L.or_glass_energy.restype = C.c_float
         L.or_glass_share.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_float]
         L.or_glass_share.restype = C.c_float
+        L.or_tex.argtypes = [C.c_void_p, C.c_int32, C.c_float, C.c_float, C.c_float, C.c_void_p]
+        L.or_tex.restype = C.c_int
 
     def emitter_draw(self, params: np.ndarray, x, n, u) -> tuple[int, float]:
         """pathtrace.wgsl:547-585 emitter_draw at x (normal n) for uniforms u[3]: (CDF entry, p); entry -1: none."""
@@ -331,12 +346,19 @@ class RenderScene:
 
     def glass_energy(self, nov: float, roughness: float, ior: float, leaving: bool) -> float:
         """pathtrace.wgsl:1582-1603 glass_energy: R + T of the rough interface, interpolated in the table."""
-        if len(self.keep[-1]) == 0:
+        if len(self.keep[10]) == 0:
             raise ValueError("the scene has no glass table")
         r = lib().or_glass_energy(C.byref(self._s), nov, roughness, ior, int(bool(leaving)))
         if r < 0:
             raise RuntimeError("glass_energy read outside its table")
         return float(r)
+
+    def tex(self, tex_id: int, u: float, v: float, lod: float) -> np.ndarray:
+        """pathtrace.wgsl:1114-1117 tex: texture tex_id at (u, v), lod = log2 of the footprint in its UV units."""
+        out = np.zeros(4, np.float32)
+        if lib().or_tex(C.byref(self._s), tex_id, u, v, lod, _ptr(out)) < 0:
+            raise RuntimeError("tex read outside its tables")
+        return out
 
     def glass_share(self, params: np.ndarray, material: int, cos_d: float) -> float:
         """pathtrace.wgsl:930-970 glass_at's share for `material` at |cos| cos_d, under `params`."""
@@ -360,6 +382,20 @@ class RenderScene:
             if failed:
                 raise RuntimeError(f"{failed} walks read outside their buffers")
         return accum
+
+
+def log2(x: float) -> float:
+    """The kernel's log2 (Cephes log2f, render.c or_log2)."""
+    L = lib()
+    L.or_log2.argtypes, L.or_log2.restype = [C.c_float], C.c_float
+    return float(L.or_log2(x))
+
+
+def exp2(x: float) -> float:
+    """The kernel's exp2 (Cephes exp2f, render.c or_exp2)."""
+    L = lib()
+    L.or_exp2.argtypes, L.or_exp2.restype = [C.c_float], C.c_float
+    return float(L.or_exp2(x))
 
 
 def rng4_at(pixel_seed: int, index: int, d: int) -> np.ndarray:

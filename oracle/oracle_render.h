@@ -59,6 +59,14 @@ typedef struct {
     /* lib.rs glass_energy_lut: 1024 vec4, entering then leaving. Non-NULL: a material has glass (the reference's F_GLASS,
      * gpu.rs features_of: layer.x > 0), and the glass paths run */
     const float *glass_energy;
+    /* texpack.rs Packed: the texture pages, RGBA8, tex_layers layers of MIPS = 5 levels (level l is tex_side >> l
+     * square), level after level within a layer, layer after layer; and Atlas::info, two vec4 a texture ((u0, v0, du,
+     * dv) of its rect in page UV; (layer, wrap, width, height)). tex_info NULL: no textures (the reference's F_TEX
+     * off) */
+    const uint8_t *tex_pages;
+    uint32_t tex_side, tex_layers;
+    const float *tex_info;
+    uint64_t tex_info_vec4s;
 } or_render_scene;
 
 /* One sample (params word 18 = P.size.z) of every pixel of rows [y0, y1), accumulated into `accum` (80 bytes a
@@ -77,6 +85,11 @@ float or_emitter_pmf(const or_render_scene *s, const uint32_t *params, const flo
  * a material at |cos| cos_d under `params` */
 float or_glass_energy(const or_render_scene *s, float nov, float roughness, float ior, int leaving);
 float or_glass_share(const or_render_scene *s, const uint32_t *params, uint32_t material, float cos_d);
+/* The kernel's log2 and exp2 (Cephes log2f / exp2f); tex (pathtrace.wgsl:1107-1117) of texture `id` at uv with LOD
+ * `lod` into out[4] (returns -1 when it read outside its tables) */
+float or_log2(float x);
+float or_exp2(float x);
+int or_tex(const or_render_scene *s, int32_t id, float u, float v, float lod, float *out);
 
 /* denoise.rs:10-74 denoise (a-trous), and the expf it uses here. Images: 3 floats a pixel (variance: 1). */
 float or_expf(float x);

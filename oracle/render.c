@@ -35,6 +35,57 @@ static const float INV_PI = 1.0f / PI_F;
 static const float TWO_PI = 2.0f * PI_F;
 static const float FOUR_PI = 4.0f * PI_F;
 
+/* ---- log2 / exp2: Cephes log2f and exp2f (S. L. Moshier): the mantissa in [sqrt(1/2), sqrt(2)) and a minimax
+ * polynomial in it; exp2 by its integer part and a polynomial on [-1/2, 1/2], both in fmaf. The kernel has the same
+ * steps, so the same bits. */
+float or_log2(float xx) {
+    if (!(xx > 0.0f)) return xx == 0.0f ? -INFINITY : NAN;
+    if (xx == INFINITY) return INFINITY;
+    int e;
+    float x = frexpf(xx, &e);
+    if (x < 0.70710678118654752f) {
+        e -= 1;
+        x = x + x - 1.0f;
+    } else {
+        x = x - 1.0f;
+    }
+    float z = x * x;
+    float y = fmaf(7.0376836292e-2f, x, -1.1514610310e-1f);
+    y = fmaf(y, x, 1.1676998740e-1f);
+    y = fmaf(y, x, -1.2420140846e-1f);
+    y = fmaf(y, x, 1.4249322787e-1f);
+    y = fmaf(y, x, -1.6668057665e-1f);
+    y = fmaf(y, x, 2.0000714765e-1f);
+    y = fmaf(y, x, -2.4999993993e-1f);
+    y = fmaf(y, x, 3.3333331174e-1f);
+    y = y * x * z;
+    y = fmaf(-0.5f, z, y);
+    const float LOG2EA = 0.44269504088896340736f;
+    float r = y * LOG2EA;
+    r = fmaf(x, LOG2EA, r);
+    r = r + y;
+    r = r + x;
+    return r + (float)e;
+}
+float or_exp2(float xx) {
+    if (xx != xx) return xx;
+    if (xx > 127.0f) return INFINITY;
+    if (xx < -127.0f) return 0.0f;
+    float i0 = floorf(xx);
+    float x = xx - i0;
+    if (x > 0.5f) {
+        i0 += 1.0f;
+        x -= 1.0f;
+    }
+    float p = fmaf(1.535336188319500e-4f, x, 1.339887440266574e-3f);
+    p = fmaf(p, x, 9.618437357674640e-3f);
+    p = fmaf(p, x, 5.550332471162809e-2f);
+    p = fmaf(p, x, 2.402264791363012e-1f);
+    p = fmaf(p, x, 6.931472028550421e-1f);
+    p = fmaf(p, x, 1.0f);
+    return ldexpf(p, (int)i0);
+}
+
 /* ---- sin / cos: Cephes sinf/cosf (S. L. Moshier): octant reduction by three-part pi/4, minimax polynomials. */
 void or_sincos(float x, float *s, float *c) {
     const float DP1 = 0.78515625f, DP2 = 2.4187564849853515625e-4f, DP3 = 3.77489497744594108e-8f;
@@ -187,6 +238,7 @@ typedef struct {
 typedef struct {
     TriAttr a;
     V v0, e1, e2;
+    uint64_t vi[3]; /* its vertices (the record's first added) */
 } TriData;
 
 /* pathtrace.wgsl:330-360 tri_data (no particles, no deforming meshes). */
@@ -199,6 +251,7 @@ static TriData tri_data(const Ctx *c, uint32_t ti, uint32_t inst, uint32_t part)
     const float *p2 = c->s->vertices + 12 * ck(c, (uint64_t)ix[2] + base, c->s->vertex_count);
     const float *n0 = p0 + 4, *n1 = p1 + 4, *n2 = p2 + 4;
     TriData t;
+    t.vi[0] = (uint64_t)ix[0] + base, t.vi[1] = (uint64_t)ix[1] + base, t.vi[2] = (uint64_t)ix[2] + base;
     t.v0 = ld3(p0);
     t.e1 = vsub(ld3(p1), ld3(p0));
     t.e2 = vsub(ld3(p2), ld3(p0));
@@ -238,6 +291,110 @@ static inline uint32_t material_of(const Ctx *c, const TriAttr *a, uint32_t inst
     return row_word(c, inst_at(c, inst)->info[3], fb(a->uv[3]), 0) & 0xffffffu;
 }
 
+/* ---- textures (pathtrace.wgsl:1102-1140, texpack.rs) ---------------------------------------------------------- */
+#define TEX_MIPS 5u
+#define TEX_MAX_LEVEL 4.0f
+#define LOD_BIAS 0.5f
+static inline int tex_on(const Ctx *c) { return c->s->tex_info != NULL; }
+static inline float fractf(float x) { return x - floorf(x); }
+static inline int32_t as_i32(float f) {
+    int32_t i;
+    memcpy(&i, &f, 4);
+    return i;
+}
+/* 1102-1104: each channel (c + 0.055) / 1.055 to the power 2.4, or c / 12.92 at or below 0.04045 */
+static inline float srgb1(float c) {
+    return c <= 0.04045f ? c / 12.92f : or_exp2(2.4f * or_log2((c + 0.055f) / 1.055f));
+}
+static inline V srgb_to_linear(V c) { return v3(srgb1(c.x), srgb1(c.y), srgb1(c.z)); }
+/* The pages' level `level` of layer `layer`, bilinear at page uv (pu, pv) with the reference's sampler (gpu.rs:2690:
+ * repeat, linear): texel centres at (i + 0.5) / n, four texels weighed by the fractions, RGBA8 as unorm. */
+static void tex_bilinear(const Ctx *c, uint32_t layer, uint32_t level, float pu, float pv, float *out) {
+    const uint64_t side = c->s->tex_side;
+    uint64_t layer_bytes = 0;
+    for (uint32_t l = 0; l < TEX_MIPS; l++) layer_bytes += (side >> l) * (side >> l) * 4u;
+    uint64_t off = (uint64_t)ck(c, layer, c->s->tex_layers) * layer_bytes;
+    for (uint32_t l = 0; l < level; l++) off += (side >> l) * (side >> l) * 4u;
+    const uint32_t n = (uint32_t)(side >> level);
+    float x = pu * (float)n - 0.5f, y = pv * (float)n - 0.5f;
+    float x0f = floorf(x), y0f = floorf(y);
+    float fx = x - x0f, fy = y - y0f;
+    if (!(x0f >= -1048576.0f && x0f <= 1048576.0f)) x0f = 0.0f;
+    if (!(y0f >= -1048576.0f && y0f <= 1048576.0f)) y0f = 0.0f;
+    int32_t xi = (int32_t)x0f, yi = (int32_t)y0f, ni = (int32_t)n;
+    uint32_t xa = (uint32_t)(((xi % ni) + ni) % ni), ya = (uint32_t)(((yi % ni) + ni) % ni);
+    uint32_t xb = xa + 1u == n ? 0u : xa + 1u, yb = ya + 1u == n ? 0u : ya + 1u;
+    const uint8_t *pg = c->s->tex_pages + off;
+    const uint8_t *t00 = pg + ((uint64_t)ya * n + xa) * 4u, *t10 = pg + ((uint64_t)ya * n + xb) * 4u;
+    const uint8_t *t01 = pg + ((uint64_t)yb * n + xa) * 4u, *t11 = pg + ((uint64_t)yb * n + xb) * 4u;
+    const float k = 1.0f / 255.0f;
+    for (int j = 0; j < 4; j++)
+        out[j] = mixf(mixf((float)t00[j] * k, (float)t10[j] * k, fx), mixf((float)t01[j] * k, (float)t11[j] * k, fx), fy);
+}
+/* tex_level (1107-1112): the texture's rect in its page (wrapped or clamped uv), trilinear between the two levels
+ * about `level` (clamped to the pages' 0..4) */
+static void tex_level(const Ctx *c, int32_t id, float u, float v, float level, float *out) {
+    const float *a = c->s->tex_info + 4 * ck(c, (uint64_t)(uint32_t)id * 2u, c->s->tex_info_vec4s);
+    const float *b = c->s->tex_info + 4 * ck(c, (uint64_t)(uint32_t)id * 2u + 1u, c->s->tex_info_vec4s);
+    float fu, fv;
+    if (b[1] > 0.5f) {
+        fu = fractf(u), fv = fractf(v);
+    } else {
+        fu = clampf(u, 0.0f, 1.0f), fv = clampf(v, 0.0f, 1.0f);
+    }
+    const float pu = a[0] + fu * a[2], pv = a[1] + fv * a[3];
+    const uint32_t layer = (uint32_t)(int32_t)b[0];
+    const float lv = clampf(level, 0.0f, TEX_MAX_LEVEL);
+    const uint32_t l0 = (uint32_t)lv;
+    const float fl = lv - (float)l0;
+    tex_bilinear(c, layer, l0, pu, pv, out);
+    if (fl > 0.0f) {
+        float o1[4];
+        tex_bilinear(c, layer, l0 + 1u, pu, pv, o1);
+        for (int j = 0; j < 4; j++) out[j] = mixf(out[j], o1[j], fl);
+    }
+}
+/* tex (1114-1117): `lod` is log2 of the footprint in texture-UV units; the level adds the texture's own size */
+static void tex(const Ctx *c, int32_t id, float u, float v, float lod, float *out) {
+    const float *b = c->s->tex_info + 4 * ck(c, (uint64_t)(uint32_t)id * 2u + 1u, c->s->tex_info_vec4s);
+    tex_level(c, id, u, v, lod + 0.5f * or_log2(b[2] * b[3]), out);
+}
+/* mesh_uv (1119-1122): the hit's uv x the material's repeat + its offset */
+static void mesh_uv(const TriAttr *a, const float *m, float u, float v, float *uv) {
+    const float w = 1.0f - u - v;
+    uv[0] = (a->n0[3] * w + a->n2[3] * u + a->uv[1] * v) * m[20] + m[24];
+    uv[1] = (a->n1[3] * w + a->uv[0] * u + a->uv[2] * v) * m[21] + m[25];
+}
+/* mesh_density (1127-1137): (UV area x repeat, area) of the triangle as it stands; (repeat, 1) where either has none */
+static void mesh_density(V e1, V e2, const TriAttr *a, const float *m, float *den) {
+    const float d1x = a->n2[3] - a->n0[3], d1y = a->uv[0] - a->n1[3];
+    const float d2x = a->uv[1] - a->n0[3], d2y = a->uv[2] - a->n1[3];
+    const float uv_area = fabsf(d1x * d2y - d2x * d1y);
+    const float area = len(cross(e1, e2));
+    const float rep = fmaxf(fabsf(m[20] * m[21]), 1e-12f);
+    const int sized = area > 2e-12f && uv_area > 2e-14f;
+    den[0] = sized ? uv_area * rep : rep;
+    den[1] = sized ? area : 1.0f;
+}
+/* mesh_lod (1140) */
+static inline float mesh_lod(const float *den, float fp2, float over) {
+    return 0.5f * or_log2(fmaxf(den[0] * fp2 / (den[1] * over), 1e-30f));
+}
+/* emission (1312-1317): the factor x the emissive map */
+static V emission(const Ctx *c, const float *m, const float *uv, float lod) {
+    V em = ld3(m + 4);
+    if (all_nonpos(em)) return em;
+    if (tex_on(c) && as_i32(m[16]) >= 0) {
+        float tx[4];
+        tex(c, as_i32(m[16]), uv[0], uv[1], lod, tx);
+        em = vmul(em, srgb_to_linear(v3(tx[0], tx[1], tx[2])));
+    }
+    return em;
+}
+static inline float lod_height(const Ctx *c) {
+    return (float)(pu(c->P, COUNTS2, 2) > 0u ? pu(c->P, COUNTS2, 2) : pu(c->P, SIZE, 1));
+}
+
 /* ---- surface -------------------------------------------------------------------------------------------------- */
 typedef struct {
     V p, n, ng, base, emissive;
@@ -247,7 +404,50 @@ typedef struct {
     int front, card;
 } Surface;
 
-static Surface surface(Ctx *c, V o, V d, const or_hit *h) {
+/* hit_tangent (366-394): the vertices' tangents (hand of the first vertex; 0: none, then one from the uv edges, or
+ * any vector at right angles to the normal where the uvs have no area), placed by the record */
+static inline V unit_or_zero(V v) {
+    float dd = dot(v, v);
+    if (dd > 0.0f && isfinite(dd)) return normalize(v);
+    return v3(0, 0, 0);
+}
+static void hit_tangent(const Ctx *c, const TriData *td, uint32_t inst, float u, float v, float *tw) {
+    const float *t0 = c->s->vertices + 12 * ck(c, td->vi[0], c->s->vertex_count) + 8;
+    const float *t1 = c->s->vertices + 12 * ck(c, td->vi[1], c->s->vertex_count) + 8;
+    const float *t2 = c->s->vertices + 12 * ck(c, td->vi[2], c->s->vertex_count) + 8;
+    V t = vadd(vadd(vs(ld3(t0), 1.0f - u - v), vs(ld3(t1), u)), vs(ld3(t2), v));
+    float hand = t0[3];
+    if (t0[3] == 0.0f) {
+        const TriAttr *a = &td->a;
+        const float d1x = a->n2[3] - a->n0[3], d1y = a->uv[0] - a->n1[3];
+        const float d2x = a->uv[1] - a->n0[3], d2y = a->uv[2] - a->n1[3];
+        const float det = d1x * d2y - d2x * d1y;
+        if (fabsf(det) < 1e-12f) {
+            V any = v3(0.0f, 1.0f, 0.0f);
+            if (fabsf(a->n0[0]) < 0.9f) any = v3(1.0f, 0.0f, 0.0f);
+            t = normalize(cross(ld3(a->n0), any));
+            hand = 1.0f;
+        } else {
+            const float r = 1.0f / det;
+            const V tu = vs(vsub(vs(td->e1, d2y), vs(td->e2, d1y)), r);
+            const float l = len(tu);
+            V tn = v3(0, 0, 0);
+            if (l > 0.0f && isfinite(l)) tn = vds(tu, l);
+            const V bt = vs(vsub(vs(td->e2, d1x), vs(td->e1, d2x)), r);
+            hand = dot(cross(ld3(a->n0), tn), bt) < 0.0f ? 1.0f : -1.0f;
+            t = tn;
+        }
+    }
+    if (inst != OR_NO_INST) {
+        const or_instance *r = inst_at(c, inst);
+        t = unit_or_zero(inst_vec(r, t));
+        hand = hand * inst_sign(r);
+    }
+    tw[0] = t.x, tw[1] = t.y, tw[2] = t.z, tw[3] = hand;
+}
+
+/* surface (1320-1425): `width` is the ray cone's footprint (m) at the hit, for the textures' level */
+static Surface surface(Ctx *c, V o, V d, const or_hit *h, float width) {
     TriData td = tri_data(c, h->tri, h->inst, h->part);
     V e1 = td.e1, e2 = td.e2;
     TriAttr a = td.a;
@@ -269,14 +469,49 @@ static Surface surface(Ctx *c, V o, V d, const or_hit *h) {
     Surface s;
     s.p = vadd(o, vs(d, h->t));
     s.front = front;
-    int32_t tex1z;
-    memcpy(&tex1z, &m[4 * 4 + 2], 4);
-    s.card = tex1z == MAT_EMITTER_CARD;
+    float uv[2] = {0.0f, 0.0f}, lod = 0.0f;
+    if (tex_on(c)) { /* 1344-1348 */
+        float den[2];
+        mesh_uv(&a, m, h->u, h->v, uv);
+        mesh_density(e1, e2, &a, m, den);
+        const float fp = fmaxf(width, 1e-7f);
+        lod = mesh_lod(den, fp * fp, fmaxf(fabsf(dot(ng, d)), 1e-3f)) - LOD_BIAS;
+    }
+    s.card = as_i32(m[4 * 4 + 2]) == MAT_EMITTER_CARD;
     V base = ld3(m);
     float rough = m[8 + 1], metal = m[8 + 0], spec = m[8 + 2];
-    V em = ld3(m + 4);
-    if (!all_nonpos(em)) em = em; /* emission(): no emissive map */
-    s.emissive = vs(em, obj_power(c, object_of(&a)));
+    if (tex_on(c)) { /* 1360-1364 */
+        float tx[4];
+        if (as_i32(m[12]) >= 0) {
+            tex(c, as_i32(m[12]), uv[0], uv[1], lod, tx);
+            base = vmul(base, srgb_to_linear(v3(tx[0], tx[1], tx[2])));
+        }
+        if (as_i32(m[14]) >= 0) {
+            tex(c, as_i32(m[14]), uv[0], uv[1], lod, tx);
+            rough *= tx[1];
+        }
+        if (as_i32(m[15]) >= 0) {
+            tex(c, as_i32(m[15]), uv[0], uv[1], lod, tx);
+            metal *= tx[2];
+        }
+    }
+    s.emissive = vs(emission(c, m, uv, lod), obj_power(c, object_of(&a)));
+    /* 1376-1387: the normal map where the surface is seen directly or in a mirror-like reflection */
+    const int near = c->g.depth == 0u || c->mirror_path;
+    if (tex_on(c) && as_i32(m[13]) >= 0 && near) {
+        float tx[4], tw[4];
+        tex(c, as_i32(m[13]), uv[0], uv[1], lod, tx);
+        const V tn = v3(tx[0] * 2.0f - 1.0f, tx[1] * 2.0f - 1.0f, tx[2] * 2.0f - 1.0f);
+        hit_tangent(c, &td, h->inst, h->u, h->v, tw);
+        const V twv = ld3(tw);
+        const V tp = vsub(twv, vs(n, dot(n, twv)));
+        if (dot(tp, tp) > 1e-12f) {
+            const V tt = normalize(tp);
+            const V b = vs(cross(n, tt), tw[3]);
+            const V pn = vadd(vadd(vs(tt, tn.x * m[11]), vs(b, tn.y * m[11])), vs(n, tn.z));
+            if (dot(pn, pn) > 1e-12f && dot(pn, ng) > 0.0f) n = normalize(pn);
+        }
+    }
     s.base = base;
     s.roughness = clampf(rough, 0.02f, 1.0f);
     s.metallic = clampf(metal, 0.0f, 1.0f);
@@ -287,8 +522,13 @@ static Surface surface(Ctx *c, V o, V d, const or_hit *h) {
     s.coat_rough = clampf(m[28 + 3], 0.02f, 1.0f);
     s.ior = 1.5f;
     s.transmission = 0.0f;
-    if (c->s->glass_energy) { /* layer.x, layer.y */
+    if (c->s->glass_energy) { /* layer.x, layer.y; the transmission map (uv.z: its texture + 1) */
         s.transmission = clampf(m[28 + 0], 0.0f, 1.0f);
+        if (tex_on(c) && m[22] > 0.5f) {
+            float tx[4];
+            tex(c, (int32_t)m[22] - 1, uv[0], uv[1], lod, tx);
+            s.transmission *= tx[0];
+        }
         s.ior = fmaxf(m[28 + 1], 1.0001f);
     }
     return s;
@@ -634,9 +874,10 @@ typedef struct {
     float share;
 } GlassAt;
 /* the share of a ray a glass material lets through at |cos| cs (glass_at, pathtrace.wgsl:955-969) */
-static float glass_share(const Ctx *c, const float *m, float cs) {
+static float glass_share(const Ctx *c, const float *m, float cs, float tr_map) {
     float tr = clampf(m[28 + 0], 0.0f, 1.0f) * (1.0f - clampf(m[8 + 0], 0.0f, 1.0f));
     if (tr <= 0.0f) return 0.0f;
+    tr = tr * tr_map; /* the transmission map's red, 1 without one */
     float ior = fmaxf(m[28 + 1], 1.0001f);
     float ri = (ior - 1.0f) / (ior + 1.0f);
     float f0 = ri * ri;
@@ -649,8 +890,9 @@ static float glass_share(const Ctx *c, const float *m, float cs) {
     return tr * (1.0f - (f0 * e[0] + f90 * e[1]) * w);
 }
 /* glass_at (930-970): a hit's geometric normal, the share of a ray its transmission lets through at that angle
- * (transmission x (1 - the specular lobe's directional albedo there); 0: not glass) and its tint */
-static GlassAt glass_at(const Ctx *c, const or_hit *h, V d) {
+ * (transmission x (1 - the specular lobe's directional albedo there); 0: not glass) and its tint; `t` the distance
+ * from the shadow ray's start (the maps' level, as a camera ray's footprint there) */
+static GlassAt glass_at(const Ctx *c, const or_hit *h, V d, float t) {
     TriData td = tri_data(c, h->tri, h->inst, h->part);
     V e1 = td.e1, e2 = td.e2;
     TriAttr a = td.a;
@@ -667,9 +909,25 @@ static GlassAt glass_at(const Ctx *c, const or_hit *h, V d) {
     g.ng = vs(normalize(cross(e1, e2)), side);
     g.share = 0.0f;
     g.tint = v3(1, 1, 1);
-    if (!(clampf(m[28 + 0], 0.0f, 1.0f) * (1.0f - clampf(m[8 + 0], 0.0f, 1.0f)) > 0.0f)) return g;
+    if (clampf(m[28 + 0], 0.0f, 1.0f) * (1.0f - clampf(m[8 + 0], 0.0f, 1.0f)) <= 0.0f) return g;
     g.tint = ld3(m);
-    g.share = glass_share(c, m, fabsf(dot(g.ng, d)));
+    float tr_map = 1.0f;
+    if (tex_on(c) && (as_i32(m[12]) >= 0 || m[22] > 0.5f)) { /* 950-956 */
+        float uv[2], den[2], tx[4];
+        mesh_uv(&a, m, h->u, h->v, uv);
+        const float fp = fmaxf(t * 2.0f * pf(c->P, CAM_POS, 3) / lod_height(c), 1e-7f);
+        mesh_density(e1, e2, &a, m, den);
+        const float lod = mesh_lod(den, fp * fp, 1.0f) - LOD_BIAS;
+        if (as_i32(m[12]) >= 0) {
+            tex(c, as_i32(m[12]), uv[0], uv[1], lod, tx);
+            g.tint = vmul(g.tint, srgb_to_linear(v3(tx[0], tx[1], tx[2])));
+        }
+        if (m[22] > 0.5f) {
+            tex(c, (int32_t)m[22] - 1, uv[0], uv[1], lod, tx);
+            tr_map = tx[0];
+        }
+    }
+    g.share = glass_share(c, m, fabsf(dot(g.ng, d)), tr_map);
     return g;
 }
 /* glass_tr (885-906): past every opaque caster, what the glass a shadow ray crosses lets through, the ray kept
@@ -678,12 +936,14 @@ static V glass_tr(Ctx *c, V o, V d, float tmax, int sky) {
     float left = tmax;
     V tr = v3(1, 1, 1);
     int odd = 0;
+    float t0 = 0.0f;
     for (uint32_t k = 0; k < 8u; k++) {
         or_hit h;
         float of[3] = {o.x, o.y, o.z}, df[3] = {d.x, d.y, d.z};
         if (or_trace(&c->s->geo, of, df, left, OR_TRACE_CLOSEST, OR_WALK_EXACT, 0, &h, NULL)) c->failed++;
         if (h.tri == OR_NO_TRI) break;
-        GlassAt g = glass_at(c, &h, d);
+        t0 += h.t;
+        GlassAt g = glass_at(c, &h, d, t0);
         if (g.share > 0.0f) {
             tr = vmul(tr, vs(g.tint, g.share));
             odd = !odd;
@@ -940,14 +1200,22 @@ static float emitter_pmf(const Ctx *c, uint32_t ei, const Em *em, V x, V n) {
     return p * we / wsum;
 }
 
-static V emitter_radiance(const Ctx *c, const Em *em) {
+/* emitter_radiance (2269-2283): the emission at the sample (b1, b2), its map read at a 5 cm footprint */
+static V emitter_radiance(const Ctx *c, const Em *em, float b1, float b2) {
     TriAttr a = tri_data(c, em->tri, em->inst, em->part).a;
     if (em->inst != OR_NO_INST) {
         uint32_t obj = inst_at(c, em->inst)->info[1] + fb(a.x[1]);
         memcpy(&a.x[1], &obj, 4);
     }
     const float *m = material_at(c, material_of(c, &a, em->inst));
-    return vs(ld3(m + 4), obj_power(c, object_of(&a)));
+    float uv[2] = {0.0f, 0.0f}, lod = 0.0f;
+    if (tex_on(c)) {
+        float den[2];
+        mesh_uv(&a, m, b1, b2, uv);
+        mesh_density(em->e1, em->e2, &a, m, den);
+        lod = mesh_lod(den, 0.0025f, 1.0f);
+    }
+    return vs(emission(c, m, uv, lod), obj_power(c, object_of(&a)));
 }
 
 typedef struct {
@@ -1006,7 +1274,7 @@ static ES sample_emitter_ris(Ctx *c, V x, V n, uint32_t m, const Surface *sf, co
     float cos_l = dot(ln, vneg(l));
     if (!(em.flags & EMITTER_CARD)) cos_l = fabsf(cos_l);
     if (cos_l <= 0.0f) return r;
-    r.le = emitter_radiance(c, &em);
+    r.le = emitter_radiance(c, &em, pick_b1, pick_b2);
     r.l = l;
     r.dist = dist;
     r.pdf_w = pick_pdf * d2 / cos_l;
@@ -1037,7 +1305,7 @@ static ES sample_emitter(Ctx *c, V x, V n, uint32_t rd) {
     float cos_l = dot(ln, vneg(l));
     if (!(em.flags & EMITTER_CARD)) cos_l = fabsf(cos_l);
     if (cos_l <= 0.0f) return r;
-    r.le = emitter_radiance(c, &em);
+    r.le = emitter_radiance(c, &em, b1, b2);
     r.l = l;
     r.dist = dist;
     r.pdf_w = pk.p / area * d2 / cos_l;
@@ -1173,6 +1441,9 @@ static void path(Ctx *c, uint32_t px, uint32_t py, float *pix) {
     V radiance = v3(0, 0, 0), thr = v3(1, 1, 1), albedo = v3(0, 0, 0), normal = v3(0, 0, 0);
     int specular_bounce = 1, behind_glass = 0, bent = 0, straight = 1;
     float last_pdf = 0.0f;
+    /* the ray cone (Akenine-Moller et al. 2019) for the textures' level: width at the origin, spread angle
+     * (pathtrace.wgsl:4165-4167) */
+    float cone_w = 0.0f, cone_a = 2.0f * pf(P, CAM_POS, 3) / lod_height(c);
     float clamp_v = pf(P, MISC, 0);
     c->mirror_path = 0;
     uint32_t last = pu(P, COUNTS, 2);
@@ -1187,7 +1458,8 @@ static void path(Ctx *c, uint32_t px, uint32_t py, float *pix) {
             radiance = vadd(radiance, col);
             break;
         }
-        Surface s = surface(c, o, d, &hit);
+        const float width = cone_w + cone_a * hit.t;
+        Surface s = surface(c, o, d, &hit, width);
         if (depth == 0u) {
             if (s.card) {
                 float mx = fmaxf(fmaxf(s.emissive.x, fmaxf(s.emissive.y, s.emissive.z)), 1e-6f);
@@ -1238,6 +1510,8 @@ static void path(Ctx *c, uint32_t px, uint32_t py, float *pix) {
             bent = 0;
         }
         straight = straight && through;
+        cone_w = width;
+        cone_a += 2.0f * s.roughness * s.roughness;
         c->mirror_path = (depth == 0u || c->mirror_path) && s.roughness < MIRROR_ROUGHNESS &&
                          (c->sampled_spec || c->sampled_trans);
         if (depth >= pu(P, COUNTS, 3)) {
@@ -1313,6 +1587,13 @@ float or_glass_energy(const or_render_scene *s, float nov, float roughness, floa
 float or_glass_share(const or_render_scene *s, const uint32_t *params, uint32_t material, float cos_d) {
     int64_t oob = 0;
     Ctx c = hook_ctx(s, params, &oob);
-    float e = glass_share(&c, material_at(&c, material), cos_d);
+    float e = glass_share(&c, material_at(&c, material), cos_d, 1.0f);
     return oob ? -1.0f : e;
+}
+
+int or_tex(const or_render_scene *s, int32_t id, float u, float v, float lod, float *out) {
+    int64_t oob = 0;
+    Ctx c = hook_ctx(s, NULL, &oob);
+    tex(&c, id, u, v, lod, out);
+    return oob ? -1 : 0;
 }

@@ -586,7 +586,7 @@ def test_what_the_library_does_not_render_is_refused(device):
     as_int = lambda v: np.array([v], np.int32).view(np.float32)[0]  # noqa: E731
     for word, value, name in ((12, as_int(0), "base-colour texture"), (13, as_int(2), "normal map"),
                               (16, as_int(1), "emissive texture"), (18, as_int(4), "unlit"),
-                              (19, as_int(1), "material function"), (22, 1.0, "transmission texture"),
+                              (19, as_int(1), "material function"), (22, 1.0, "transmission map"),
                               (23, 1.0, "vertex colours"), (27, 0.5, "alpha cutoff"),
                               (28, 1.0, "no glass energy table")):
         print(f"refusal: material word {word} ({name})", flush=True)

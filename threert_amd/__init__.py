@@ -74,7 +74,9 @@ class _SceneDesc(C.Structure):
                 ("light_count", C.c_uint64), ("emitters", C.c_void_p), ("emitter_vec4s", C.c_uint64),
                 ("emitter_tris", C.c_void_p), ("emitter_tri_count", C.c_uint64), ("objects", C.c_void_p),
                 ("object_count", C.c_uint64), ("energy", C.c_void_p), ("light_tree", C.c_void_p),
-                ("light_tree_vec4s", C.c_uint64), ("glass_energy", C.c_void_p)]
+                ("light_tree_vec4s", C.c_uint64), ("glass_energy", C.c_void_p), ("tex_pages", C.c_void_p),
+                ("tex_side", C.c_uint32), ("tex_layers", C.c_uint32), ("tex_info", C.c_void_p),
+                ("tex_info_vec4s", C.c_uint64)]
 
 
 class _LightTree(C.Structure):

@@ -90,8 +90,10 @@ struct trt_scene {
     // shading tables (trt_render)
     void *vertices = nullptr, *indices = nullptr, *rows = nullptr, *materials = nullptr, *lights = nullptr,
          *emitters = nullptr, *emitter_tris = nullptr, *objects = nullptr, *energy = nullptr, *light_tree = nullptr,
-         *glass_energy = nullptr;
-    uint64_t light_tree_count = 0;
+         *glass_energy = nullptr, *tex_pages = nullptr, *tex_info = nullptr;
+    uint64_t light_tree_count = 0, tex_info_vec4s = 0;
+    uint32_t tex_side = 0, tex_layers = 0;
+    bool tex = false;    // the scene has texture pages (the reference's F_TEX)
     bool shading = false;
     bool coat = false;   // a material has a clearcoat (layer.z > 0: the reference's F_COAT, gpu.rs features_of)
     bool glass = false;  // a material has glass (layer.x > 0: the reference's F_GLASS, gpu.rs:2030-2032)
@@ -235,10 +237,11 @@ int trt_device_sync(trt_device *d) {
         static const char *table[] = {"instance", "material", "light", "emitter entry", "row", "object",
                                       "triangle indices", "vertex", "energy table", "emitter triangle (RIS candidate)",
                                       "leaf", "emitter triangle (RIS pick)", "emitter triangle (single draw)",
-                                      "emitter triangle (a hit's own)", "light tree node", "glass energy table"};
+                                      "emitter triangle (a hit's own)", "light tree node", "glass energy table",
+                                      "texture page layer", "texture info"};
         std::string where;
         if (g[1]) {
-            const char *name = g[1] <= 10 ? what[g[1]] : (g[1] >= 16 && g[1] <= 31 ? table[g[1] - 16] : "?");
+            const char *name = g[1] <= 10 ? what[g[1]] : (g[1] >= 16 && g[1] <= 33 ? table[g[1] - 16] : "?");
             where = std::string(" (first: ") + name + " index " +
                     std::to_string((uint64_t(g[3]) << 32) | g[2]) + " of " + std::to_string(g[4]) + "; context " +
                     std::to_string(g[5]) + " " + std::to_string(g[6]) + " " + std::to_string(g[7]) + ")";
@@ -386,7 +389,7 @@ int trt_scene_release(trt_scene *s) {
     hipFree(s->wunits);
     hipFree(s->wleaves);
     for (void *p : {s->vertices, s->indices, s->rows, s->materials, s->lights, s->emitters, s->emitter_tris,
-                    s->objects, s->energy, s->light_tree, s->glass_energy})
+                    s->objects, s->energy, s->light_tree, s->glass_energy, s->tex_pages, s->tex_info})
         hipFree(p);
     trt_device *d = s->dev;
     delete s;
@@ -395,25 +398,32 @@ int trt_scene_release(trt_scene *s) {
 }
 
 // A material the path loop would render without what it asks for is refused, named: the reference's `surface`
-// (pathtrace.wgsl:1320-1425) reads these only under the F_TEX, F_VCOLOR, F_FUNC and F_UNLIT features, and its cutout
-// paths (1300-1310), none of which this core has; glass (1542-1640) needs the glass energy table. Words of the
-// 72-word GpuMaterial (scene.rs:480-491): tex0 12-15 and tex1 16-19 (texture slots, -1 none; tex1.z the flags, tex1.w
-// a function + 1), uv.z 22 (transmission texture + 1), uv.w 23 (vertex colours), uv2.w 27 (alpha cutoff), layer.x 28
-// (transmission).
-static bool material_rendered(const float *m, uint64_t count, bool glass_table, std::string *why) {
+// (pathtrace.wgsl:1320-1425) reads these only under the F_VCOLOR, F_FUNC and F_UNLIT features, and its cutout paths
+// (1300-1310), none of which this core has; glass (1542-1640) needs the glass energy table, a texture slot (F_TEX,
+// 1360-1425) a texture of the scene's pages. (The occlusion slot, tex1.y, is read by no path of the kernel: it is
+// taken and ignored, as the reference's.) Words of the 72-word GpuMaterial (scene.rs:480-491): tex0 12-15 and tex1
+// 16-19 (texture slots, -1 none; tex1.z the flags, tex1.w a function + 1), uv.z 22 (transmission texture + 1), uv.w
+// 23 (vertex colours), uv2.w 27 (alpha cutoff), layer.x 28 (transmission).
+static bool material_rendered(const float *m, uint64_t count, bool glass_table, uint64_t textures, std::string *why) {
     constexpr int32_t kMatUnlit = 4;  // pathtrace.wgsl:54 MAT_UNLIT
-    static const char *slot[8] = {"a base-colour texture", "a normal map", "a roughness texture",
-                                  "a metalness texture", "an emissive texture", "an occlusion texture", "", ""};
+    static const char *slot[5] = {"base-colour texture", "normal map", "roughness texture", "metalness texture",
+                                  "emissive texture"};
     for (uint64_t k = 0; k < count; k++) {
         const float *f = m + 72 * k;
         int32_t w[8];
         std::memcpy(w, f + 12, sizeof(w));
+        // texture slots: each a texture of the pages (the transmission map's slot + 1 in uv.z)
+        const int64_t tslot = f[22] > 0.5f ? (f[22] < 2147483648.0f ? int64_t(f[22]) - 1 : -2) : -1;
+        for (int j = 0; j < 6; j++) {
+            const int64_t id = j < 5 ? w[j] : tslot;
+            if (id == -1 || (j < 5 && id < 0) || (id >= 0 && uint64_t(id) < textures)) continue;
+            *why = "material " + std::to_string(k) + "'s " + (j < 5 ? slot[j] : "transmission map") + " names texture " +
+                   std::to_string(id) + ", and the scene's pages hold " + std::to_string(textures);
+            return false;
+        }
         const char *what = nullptr;
-        for (int j = 0; j < 6 && !what; j++)
-            if (w[j] >= 0) what = slot[j];
-        if (!what && w[6] == kMatUnlit) what = "an unlit material";
+        if (w[6] == kMatUnlit) what = "an unlit material";
         if (!what && w[7] > 0) what = "a material function";
-        if (!what && f[22] > 0.5f) what = "a transmission texture";
         if (!what && f[23] > 0.0f) what = "vertex colours";
         if (!what && f[27] > 0.0f) what = "an alpha cutoff";
         if (what) {
@@ -474,6 +484,33 @@ static bool light_tree_ok(const trt_scene_desc *d, uint64_t entries, std::string
 
 // The shading tables' indices, checked once so the kernel can follow them: every unit's triangles' vertices and
 // section rows, the rows' materials, the emitter entries' triangles and the emitters' triangles, instances, rows.
+// texpack.rs pages as the kernel reads them: a power-of-two side of 16 or more (5 levels), at least one layer, every
+// texture's rect on a layer the pages have and inside its page
+static uint64_t page_layer_bytes(uint32_t side) {
+    uint64_t n = 0;
+    for (uint32_t l = 0; l < 5; l++) n += uint64_t(side >> l) * (side >> l) * 4;
+    return n;
+}
+static bool pages_ok(const trt_scene_desc *d, std::string *why) {
+    const uint32_t side = d->tex_side;
+    if (!d->tex_pages || side < 16 || side > 16384 || (side & (side - 1)) || d->tex_layers == 0 ||
+        d->tex_layers > 2048 || d->tex_info_vec4s < 2 || (d->tex_info_vec4s & 1) || d->tex_info_vec4s >= (1u << 30)) {
+        *why = "texture pages need a power-of-two side from 16 to 16384, 1 to 2048 layers and 2 info rows a texture";
+        return false;
+    }
+    const float *info = static_cast<const float *>(d->tex_info);
+    for (uint64_t i = 0; i < d->tex_info_vec4s / 2; i++) {
+        const float *a = info + 8 * i, *b = a + 4;
+        bool ok = b[0] >= 0.0f && b[0] < float(d->tex_layers) && b[2] >= 0.0f && b[3] >= 0.0f;
+        for (int j = 0; j < 8 && ok; j++) ok = std::isfinite(a[j]);
+        if (!ok) {
+            *why = "texture " + std::to_string(i) + "'s info names no layer of the pages, or is not finite";
+            return false;
+        }
+    }
+    return true;
+}
+
 static bool check_shading(const trt_scene_desc *d, std::string *why) {
     constexpr uint64_t kGuideVec4s = 65537 / 4 + 1;
     if (!d->vertices || !d->indices || !d->rows || !d->materials || !d->objects || d->object_count == 0) {
@@ -492,9 +529,11 @@ static bool check_shading(const trt_scene_desc *d, std::string *why) {
         *why = "an emitter table shorter than its guide and header";
         return false;
     }
+    const uint64_t textures = d->tex_info ? d->tex_info_vec4s / 2 : 0;
     if (!material_rendered(static_cast<const float *>(d->materials), d->material_count, d->glass_energy != nullptr,
-                           why))
+                           textures, why))
         return false;
+    if (d->tex_info && !pages_ok(d, why)) return false;
     const auto *inst = static_cast<const trt::Instance *>(d->instances);
     const auto *leaves = static_cast<const trt::Leaf *>(d->leaves);
     const auto *nodes = static_cast<const trt::Node *>(d->nodes);
@@ -760,6 +799,12 @@ int trt_scene_upload(trt_device *d, const trt_scene_desc *desc, trt_scene **out)
         if (!rc) rc = upload(d, &s->light_tree, desc->light_tree, desc->light_tree_vec4s * 16, &total);
         s->light_tree_count = desc->light_tree_vec4s;
         if (!rc && s->glass) rc = upload(d, &s->glass_energy, desc->glass_energy, 1024 * 16, &total);
+        if (!rc && desc->tex_info) {
+            s->tex = true;
+            s->tex_side = desc->tex_side, s->tex_layers = desc->tex_layers, s->tex_info_vec4s = desc->tex_info_vec4s;
+            rc = upload(d, &s->tex_pages, desc->tex_pages, page_layer_bytes(desc->tex_side) * desc->tex_layers, &total);
+            if (!rc) rc = upload(d, &s->tex_info, desc->tex_info, desc->tex_info_vec4s * 16, &total);
+        }
     }
     if (rc) {
         std::string keep = g_error;
@@ -1097,6 +1142,10 @@ static int job_args(const trt_render_job &j, trt::RenderArgs &a, std::string *wh
     a.light_tree = static_cast<const float4 *>(s->light_tree);
     a.light_tree_count = static_cast<uint32_t>(s->light_tree_count);
     a.glass_energy = static_cast<const float4 *>(s->glass_energy);
+    a.tex_pages = static_cast<const uint32_t *>(s->tex_pages);
+    a.tex_info = static_cast<const float4 *>(s->tex_info);
+    a.tex_info_count = s->tex_info_vec4s;
+    a.tex_side = s->tex_side, a.tex_layers = s->tex_layers;
     a.accum = static_cast<float4 *>(acc->buf);
     a.vertex_count = static_cast<uint32_t>(s->vertex_count);
     a.tri_count = static_cast<uint32_t>(s->tri_count);
@@ -1281,7 +1330,7 @@ static int render_batch(trt_device *d, const trt_render_job *jobs, uint32_t n, u
     table.reserve(n);
     uint64_t tiles = 0;
     uint32_t deepest = 0;
-    bool coat = false, tree = false, glass = false;
+    bool coat = false, tree = false, glass = false, tex = false;
     for (uint32_t k = 0; k < n; k++) {
         if (jobs[k].spp == 0) continue;
         if (!jobs[k].scene || jobs[k].scene->dev != d)
@@ -1298,16 +1347,17 @@ static int render_batch(trt_device *d, const trt_render_job *jobs, uint32_t n, u
         deepest = std::max(deepest, jobs[k].scene->max_stack);
         coat = coat || jobs[k].scene->coat;
         glass = glass || jobs[k].scene->glass;
+        tex = tex || jobs[k].scene->tex;
         tree = tree || trt::from_bits(j.args.params[87]) > 0.5f;  // P.ltree.w: the job samples the light tree
         table.push_back(j);
     }
     if (tiles >= (uint64_t{1} << 31)) return fail(TRT_E_INVALID, "trt_render_batch: too many tiles in one launch");
     if (table.empty()) return ok();
-    // The wavefront takes batches whose jobs have at most kWfMaxSuns directional lights and no glass; counting
+    // The wavefront takes batches whose jobs have at most kWfMaxSuns directional lights, no glass and no textures; counting
     // launches stay on the megakernel. TRT_RENDER_AUTO takes it when every scene has kWfAutoLeaves triangles or more,
     // or for launches of a whole chunk of paths or more.
     const uint32_t how = d->render.load();
-    bool wf = reads == nullptr && how != TRT_RENDER_MEGAKERNEL && !glass;
+    bool wf = reads == nullptr && how != TRT_RENDER_MEGAKERNEL && !glass && !tex;
     uint64_t paths = 0;
     bool large = true;
     std::vector<const trt_scene *> scene_of;
@@ -1373,7 +1423,7 @@ static int render_batch(trt_device *d, const trt_render_job *jobs, uint32_t n, u
         if (rc) return rc;
     } else if (trt::launch_render(static_cast<const trt::RenderJob *>(ring.dev[k]), static_cast<uint32_t>(table.size()),
                            static_cast<uint32_t>(tiles), d->counter, d->spill, threads,
-                           static_cast<uint32_t>(entries), d->guard, d->ray_counts, coat, tree, glass, reads, grid,
+                           static_cast<uint32_t>(entries), d->guard, d->ray_counts, coat, tree, glass, tex, reads, grid,
                            d->stream))
         return fail(TRT_E_DEVICE, "render kernel launch failed");
     TRT_HIP(hipEventRecord(ring.ev[k], d->stream));

@@ -11,9 +11,13 @@
 // emitter_pick_u / emitter_pick_range / lt_imp / emitter_pmf 463-618 (the light tree); the clearcoat; glass (kGlass:
 // the dielectric 1542-1640 with fresnel_dielectric, lambda_ggx, glass_energy, dielectric_t, pdf_dielectric; its
 // parts of lobes, eval_bsdf_k, pdf_bsdf and sample_bsdf; shadow_tr / glass_tr / glass_at / sky_tr 846-977;
-// light_origin 2153-2156; the path loop's `through`, behind_glass and bent 4156-4465). Not here: textures, media,
-// fog, procedural geometry, local lights, glass among a shadow caster's sections, live-only paths (the upload and
-// the launch refuse scenes and parameters that need them).
+// light_origin 2153-2156; the path loop's `through`, behind_glass and bent 4156-4465); textures (kTex: the pages'
+// trilinear read tex / tex_level 1107-1117 with srgb_to_linear 1102-1104, mesh_uv / mesh_density / mesh_lod
+// 1119-1140, emission 1312-1317, the colour, roughness, metalness, emissive, normal and transmission maps of
+// surface 1344-1425 with hit_tangent 366-394, glass_at's maps 950-956, emitter_radiance 2269-2283, the path loop's
+// ray cone 4165-4167 and 4451-4452). Not here: media, fog, procedural geometry, local lights, cutouts, glass among a
+// shadow caster's sections, live-only paths (the upload and the launch refuse scenes and parameters that need
+// them).
 // normalize(v) = v / sqrt(v.v), inverseSqrt(x) = 1 / sqrt(x); sin / cos by Moshier's Cephes reduction and
 // polynomials in fmaf; every other operation in the reference's order, no contraction: the same bits as the C
 // restatement the tests hold this to.
@@ -172,6 +176,56 @@ constexpr int kMatEmitterCard = 2;
 constexpr float kMirrorRoughness = 0.3f;
 constexpr uint32_t kObjectStride = 3;
 
+// Cephes log2f and exp2f (S. L. Moshier): the mantissa in [sqrt(1/2), sqrt(2)) and a minimax polynomial in it; exp2
+// by its integer part and a polynomial on [-1/2, 1/2], in fmaf as the C restatement has them.
+__device__ float log2_c(float xx) {
+    if (!(xx > 0.0f)) return xx == 0.0f ? -__builtin_inff() : __builtin_nanf("");
+    if (xx == __builtin_inff()) return xx;
+    int e;
+    float x = frexpf(xx, &e);
+    if (x < 0.70710678118654752f) {
+        e -= 1;
+        x = x + x - 1.0f;
+    } else {
+        x = x - 1.0f;
+    }
+    const float z = x * x;
+    float y = fmaf(7.0376836292e-2f, x, -1.1514610310e-1f);
+    y = fmaf(y, x, 1.1676998740e-1f);
+    y = fmaf(y, x, -1.2420140846e-1f);
+    y = fmaf(y, x, 1.4249322787e-1f);
+    y = fmaf(y, x, -1.6668057665e-1f);
+    y = fmaf(y, x, 2.0000714765e-1f);
+    y = fmaf(y, x, -2.4999993993e-1f);
+    y = fmaf(y, x, 3.3333331174e-1f);
+    y = y * x * z;
+    y = fmaf(-0.5f, z, y);
+    const float LOG2EA = 0.44269504088896340736f;
+    float r = y * LOG2EA;
+    r = fmaf(x, LOG2EA, r);
+    r = r + y;
+    r = r + x;
+    return r + static_cast<float>(e);
+}
+__device__ float exp2_c(float xx) {
+    if (xx != xx) return xx;
+    if (xx > 127.0f) return __builtin_inff();
+    if (xx < -127.0f) return 0.0f;
+    float i0 = floorf(xx);
+    float x = xx - i0;
+    if (x > 0.5f) {
+        i0 += 1.0f;
+        x -= 1.0f;
+    }
+    float p = fmaf(1.535336188319500e-4f, x, 1.339887440266574e-3f);
+    p = fmaf(p, x, 9.618437357674640e-3f);
+    p = fmaf(p, x, 5.550332471162809e-2f);
+    p = fmaf(p, x, 2.402264791363012e-1f);
+    p = fmaf(p, x, 6.931472028550421e-1f);
+    p = fmaf(p, x, 1.0f);
+    return ldexpf(p, static_cast<int>(i0));
+}
+
 // ---- the kernel's context ---------------------------------------------------------------------------------------
 struct Ctx {
     const RenderArgs *a;
@@ -252,6 +306,77 @@ __device__ __forceinline__ uint32_t material_of(const Ctx &c, const TriAttr &a, 
     return c.row_word(__float_as_uint(c.inst(inst)[6].w), __float_as_uint(a.uv.w), 0) & 0xffffffu;
 }
 
+// ---- textures (pathtrace.wgsl:1102-1140; texpack.rs pages: RGBA8, 5 levels a layer) -------------------------------
+constexpr uint32_t kTexMips = 5;
+constexpr float kTexMaxLevel = 4.0f, kLodBias = 0.5f;
+__device__ __forceinline__ float fractf(float x) { return x - floorf(x); }
+__device__ __forceinline__ float srgb1(float c) {
+    return c <= 0.04045f ? c / 12.92f : exp2_c(2.4f * log2_c((c + 0.055f) / 1.055f));
+}
+__device__ __forceinline__ V srgb_to_linear(V c) { return v3(srgb1(c.x), srgb1(c.y), srgb1(c.z)); }
+// Level `level` of page layer `layer`, bilinear at page uv with the reference's sampler (repeat, linear): texel
+// centres at (i + 0.5) / n, four texels weighed by the fractions, RGBA8 as unorm.
+__device__ float4 tex_bilinear(const Ctx &c, uint32_t layer, uint32_t level, float pu, float pv) {
+    const uint64_t side = c.a->tex_side;
+    uint64_t layer_words = 0;
+    for (uint32_t l = 0; l < kTexMips; l++) layer_words += (side >> l) * (side >> l);
+    uint64_t off = c.ck(layer, c.a->tex_layers, 32) * layer_words;
+    for (uint32_t l = 0; l < level; l++) off += (side >> l) * (side >> l);
+    const uint32_t n = static_cast<uint32_t>(side >> level);
+    const float x = pu * static_cast<float>(n) - 0.5f, y = pv * static_cast<float>(n) - 0.5f;
+    float x0f = floorf(x), y0f = floorf(y);
+    const float fx = x - x0f, fy = y - y0f;
+    if (!(x0f >= -1048576.0f && x0f <= 1048576.0f)) x0f = 0.0f;
+    if (!(y0f >= -1048576.0f && y0f <= 1048576.0f)) y0f = 0.0f;
+    const int32_t xi = static_cast<int32_t>(x0f), yi = static_cast<int32_t>(y0f), ni = static_cast<int32_t>(n);
+    const uint32_t xa = static_cast<uint32_t>(((xi % ni) + ni) % ni), ya = static_cast<uint32_t>(((yi % ni) + ni) % ni);
+    const uint32_t xb = xa + 1u == n ? 0u : xa + 1u, yb = ya + 1u == n ? 0u : ya + 1u;
+    const uint32_t *pg = c.a->tex_pages + off;
+    const uint32_t w00 = pg[uint64_t(ya) * n + xa], w10 = pg[uint64_t(ya) * n + xb];
+    const uint32_t w01 = pg[uint64_t(yb) * n + xa], w11 = pg[uint64_t(yb) * n + xb];
+    const float k = 1.0f / 255.0f;
+    float o[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const float a00 = static_cast<float>((w00 >> (8 * j)) & 0xffu) * k, a10 = static_cast<float>((w10 >> (8 * j)) & 0xffu) * k;
+        const float a01 = static_cast<float>((w01 >> (8 * j)) & 0xffu) * k, a11 = static_cast<float>((w11 >> (8 * j)) & 0xffu) * k;
+        o[j] = mixf(mixf(a00, a10, fx), mixf(a01, a11, fx), fy);
+    }
+    return make_float4(o[0], o[1], o[2], o[3]);
+}
+// tex_level (1107-1112): the texture's rect in its page (wrapped or clamped uv), trilinear between the two levels
+// about `level` (clamped to the pages' 0..4)
+__device__ float4 tex_level(const Ctx &c, int32_t id, float u, float v, float level) {
+    const float4 a = c.a->tex_info[c.ck(uint64_t(uint32_t(id)) * 2u, c.a->tex_info_count, 33)];
+    const float4 b = c.a->tex_info[c.ck(uint64_t(uint32_t(id)) * 2u + 1u, c.a->tex_info_count, 33)];
+    float fu, fv;
+    if (b.y > 0.5f) {
+        fu = fractf(u), fv = fractf(v);
+    } else {
+        fu = clampf(u, 0.0f, 1.0f), fv = clampf(v, 0.0f, 1.0f);
+    }
+    const float pu = a.x + fu * a.z, pv = a.y + fv * a.w;
+    const uint32_t layer = static_cast<uint32_t>(static_cast<int32_t>(b.x));
+    const float lv = clampf(level, 0.0f, kTexMaxLevel);
+    const uint32_t l0 = static_cast<uint32_t>(lv);
+    const float fl = lv - static_cast<float>(l0);
+    float4 r = tex_bilinear(c, layer, l0, pu, pv);
+    if (fl > 0.0f) {
+        const float4 r1 = tex_bilinear(c, layer, l0 + 1u, pu, pv);
+        r = make_float4(mixf(r.x, r1.x, fl), mixf(r.y, r1.y, fl), mixf(r.z, r1.z, fl), mixf(r.w, r1.w, fl));
+    }
+    return r;
+}
+// tex (1114-1117): `lod` is log2 of the footprint in texture-UV units; the level adds the texture's own size
+__device__ float4 tex(const Ctx &c, int32_t id, float u, float v, float lod) {
+    const float4 b = c.a->tex_info[c.ck(uint64_t(uint32_t(id)) * 2u + 1u, c.a->tex_info_count, 33)];
+    return tex_level(c, id, u, v, lod + 0.5f * log2_c(b.z * b.w));
+}
+__device__ __forceinline__ int32_t as_i32(float f) { return static_cast<int32_t>(__float_as_uint(f)); }
+__device__ __forceinline__ float lod_height(const Ctx &c) {
+    return static_cast<float>(c.pu(COUNTS2, 2) > 0u ? c.pu(COUNTS2, 2) : c.pu(SIZE, 1));
+}
+
 struct Surface {
     V p, n, ng, base, emissive;
     float roughness, metallic, specular;
@@ -262,10 +387,92 @@ struct Surface {
 
 // kCoat: the launch's scenes hold a clearcoat (the reference's F_COAT pipeline flag); without it the coat terms are
 // compiled out, as the reference's shader is specialised. kGlass likewise for glass (F_GLASS).
-template <bool kCoat, bool kGlass = false>
-__device__ Surface surface(const Ctx &c, V o, V d, const HitRec &h) {
+// mesh_uv (1119-1122): the hit's uv x the material's repeat + its offset
+__device__ __forceinline__ void mesh_uv(const TriAttr &a, const float4 *m, float u, float v, float &uu, float &vv) {
+    const float w = 1.0f - u - v;
+    const float4 rep = m[5], off = m[6];
+    uu = (a.n0.w * w + a.n2.w * u + a.uv.y * v) * rep.x + off.x;
+    vv = (a.n1.w * w + a.uv.x * u + a.uv.z * v) * rep.y + off.y;
+}
+// mesh_density (1127-1137): (UV area x repeat, area) of the triangle as it stands; (repeat, 1) where either has none
+__device__ __forceinline__ void mesh_density(V e1, V e2, const TriAttr &a, const float4 *m, float &d0, float &d1) {
+    const float d1x = a.n2.w - a.n0.w, d1y = a.uv.x - a.n1.w;
+    const float d2x = a.uv.y - a.n0.w, d2y = a.uv.z - a.n1.w;
+    const float uv_area = fabsf(d1x * d2y - d2x * d1y);
+    const float area = len(cross(e1, e2));
+    const float4 rep = m[5];
+    const float r = fmaxf(fabsf(rep.x * rep.y), 1e-12f);
+    const bool sized = area > 2e-12f && uv_area > 2e-14f;
+    d0 = sized ? uv_area * r : r;
+    d1 = sized ? area : 1.0f;
+}
+// mesh_lod (1140)
+__device__ __forceinline__ float mesh_lod(float d0, float d1, float fp2, float over) {
+    return 0.5f * log2_c(fmaxf(d0 * fp2 / (d1 * over), 1e-30f));
+}
+// emission (1312-1317): the factor x the emissive map
+template <bool kTex>
+__device__ V emission(const Ctx &c, const float4 *m, float u, float v, float lod) {
+    V em = xyz(m[1]);
+    if (all_nonpos(em)) return em;
+    if (kTex && as_i32(m[4].x) >= 0) {
+        const float4 tx = tex(c, as_i32(m[4].x), u, v, lod);
+        em = vmul(em, srgb_to_linear(v3(tx.x, tx.y, tx.z)));
+    }
+    return em;
+}
+__device__ __forceinline__ V unit_or_zero(V v) {
+    const float dd = dot(v, v);
+    if (dd > 0.0f && isfinite(dd)) return normalize(v);
+    return v3(0.0f, 0.0f, 0.0f);
+}
+// hit_tangent (366-394): the vertices' tangents (hand of the first vertex; 0: none, then one from the uv edges, or
+// any vector at right angles to the normal where the uvs have no area), placed by the record. `a`, e1, e2: the
+// triangle's attributes and edges in its mesh's space.
+__device__ float4 hit_tangent(const Ctx &c, const TriAttr &a, V e1, V e2, uint32_t ti, uint32_t inst, float u, float v) {
+    uint32_t base = 0;
+    if (inst != kNone) base = __float_as_uint(c.inst(inst)[6].z);
+    const uint32_t *ix = c.a->indices + 3 * c.ck(ti, c.a->tri_count, 22);
+    const float4 t0 = c.a->vertices[3 * c.ck(uint64_t(ix[0]) + base, c.a->vertex_count, 23) + 2];
+    const float4 t1 = c.a->vertices[3 * c.ck(uint64_t(ix[1]) + base, c.a->vertex_count, 23) + 2];
+    const float4 t2 = c.a->vertices[3 * c.ck(uint64_t(ix[2]) + base, c.a->vertex_count, 23) + 2];
+    V t = vadd(vadd(vs(xyz(t0), 1.0f - u - v), vs(xyz(t1), u)), vs(xyz(t2), v));
+    float hand = t0.w;
+    if (t0.w == 0.0f) {
+        const float d1x = a.n2.w - a.n0.w, d1y = a.uv.x - a.n1.w;
+        const float d2x = a.uv.y - a.n0.w, d2y = a.uv.z - a.n1.w;
+        const float det = d1x * d2y - d2x * d1y;
+        if (fabsf(det) < 1e-12f) {
+            V any = v3(0.0f, 1.0f, 0.0f);
+            if (fabsf(a.n0.x) < 0.9f) any = v3(1.0f, 0.0f, 0.0f);
+            t = normalize(cross(xyz(a.n0), any));
+            hand = 1.0f;
+        } else {
+            const float r = 1.0f / det;
+            const V tu = vs(vsub(vs(e1, d2y), vs(e2, d1y)), r);
+            const float l = len(tu);
+            V tn = v3(0.0f, 0.0f, 0.0f);
+            if (l > 0.0f && isfinite(l)) tn = vds(tu, l);
+            const V bt = vs(vsub(vs(e2, d1x), vs(e1, d2x)), r);
+            hand = dot(cross(xyz(a.n0), tn), bt) < 0.0f ? 1.0f : -1.0f;
+            t = tn;
+        }
+    }
+    if (inst != kNone) {
+        const float4 *r = c.inst(inst);
+        t = unit_or_zero(inst_vec(r, t));
+        hand = hand * ((__float_as_uint(r[4].w) & kInstMirrored) ? -1.0f : 1.0f);
+    }
+    return make_float4(t.x, t.y, t.z, hand);
+}
+
+// `width`: the ray cone's footprint (m) at the hit, for the textures' level (kTex)
+template <bool kCoat, bool kGlass = false, bool kTex = false>
+__device__ Surface surface(const Ctx &c, V o, V d, const HitRec &h, float width = 0.0f) {
     V e1, e2;
     TriAttr a = tri_attr(c, h.tri, h.inst, h.part, e1, e2);
+    const TriAttr a_mesh = a;  // the mesh's own attributes and edges (hit_tangent)
+    const V e1_mesh = e1, e2_mesh = e2;
     float side = 1.0f;
     if (h.inst != kNone) {
         const float4 *r = c.inst(h.inst);
@@ -287,12 +494,45 @@ __device__ Surface surface(const Ctx &c, V o, V d, const HitRec &h) {
     Surface s;
     s.p = vadd(o, vs(d, h.t));
     s.front = front;
+    float uu = 0.0f, vv = 0.0f, lod = 0.0f;
+    if (kTex) {  // pathtrace.wgsl:1344-1348
+        float d0, d1;
+        mesh_uv(a, m, h.u, h.v, uu, vv);
+        mesh_density(e1, e2, a, m, d0, d1);
+        const float fp = fmaxf(width, 1e-7f);
+        lod = mesh_lod(d0, d1, fp * fp, fmaxf(fabsf(dot(ng, d)), 1e-3f)) - kLodBias;
+    }
     s.card = static_cast<int>(__float_as_uint(m[4].z)) == kMatEmitterCard;
     const float4 params = m[2];
-    s.emissive = vs(xyz(m[1]), c.obj_power(__float_as_uint(a.x.y)));
-    s.base = xyz(m[0]);
-    s.roughness = clampf(params.y, 0.02f, 1.0f);
-    s.metallic = clampf(params.x, 0.0f, 1.0f);
+    V base = xyz(m[0]);
+    float rough = params.y, metal = params.x;
+    if (kTex) {  // 1360-1364: the colour (sRGB), roughness (green) and metalness (blue) maps
+        const float4 t0 = m[3];
+        if (as_i32(t0.x) >= 0) {
+            const float4 tx = tex(c, as_i32(t0.x), uu, vv, lod);
+            base = vmul(base, srgb_to_linear(v3(tx.x, tx.y, tx.z)));
+        }
+        if (as_i32(t0.z) >= 0) rough *= tex(c, as_i32(t0.z), uu, vv, lod).y;
+        if (as_i32(t0.w) >= 0) metal *= tex(c, as_i32(t0.w), uu, vv, lod).z;
+    }
+    s.emissive = vs(emission<kTex>(c, m, uu, vv, lod), c.obj_power(__float_as_uint(a.x.y)));
+    // 1376-1387: the normal map where the surface is seen directly or in a mirror-like reflection
+    if (kTex && as_i32(m[3].y) >= 0 && (c.g.depth == 0u || c.mirror_path)) {
+        const float4 tx = tex(c, as_i32(m[3].y), uu, vv, lod);
+        const V tn = v3(tx.x * 2.0f - 1.0f, tx.y * 2.0f - 1.0f, tx.z * 2.0f - 1.0f);
+        const float4 tw = hit_tangent(c, a_mesh, e1_mesh, e2_mesh, h.tri, h.inst, h.u, h.v);
+        const V twv = xyz(tw);
+        const V tp = vsub(twv, vs(n, dot(n, twv)));
+        if (dot(tp, tp) > 1e-12f) {
+            const V tt = normalize(tp);
+            const V b = vs(cross(n, tt), tw.w);
+            const V pn = vadd(vadd(vs(tt, tn.x * params.w), vs(b, tn.y * params.w)), vs(n, tn.z));
+            if (dot(pn, pn) > 1e-12f && dot(pn, ng) > 0.0f) n = normalize(pn);
+        }
+    }
+    s.base = base;
+    s.roughness = clampf(rough, 0.02f, 1.0f);
+    s.metallic = clampf(metal, 0.0f, 1.0f);
     s.specular = params.z;
     s.ng = ng;
     s.n = n;
@@ -305,9 +545,10 @@ __device__ Surface surface(const Ctx &c, V o, V d, const HitRec &h) {
     }
     s.ior = 1.5f;
     s.transmission = 0.0f;
-    if (kGlass) {
+    if (kGlass) {  // layer.x, layer.y; the transmission map (uv.z: its texture + 1)
         const float4 layer = m[7];
         s.transmission = clampf(layer.x, 0.0f, 1.0f);
+        if (kTex && m[5].z > 0.5f) s.transmission *= tex(c, static_cast<int32_t>(m[5].z) - 1, uu, vv, lod).x;
         s.ior = fmaxf(layer.y, 1.0001f);
     }
     return s;
@@ -681,10 +922,11 @@ __device__ V sky_mis(const Ctx &c, V d, float pdf) {
 // ---- glass along shadow rays (pathtrace.wgsl:846-977) -------------------------------------------------------------
 // The share of a ray a glass material lets through at |cos| cs: transmission x (1 - the specular lobe's directional
 // albedo there, spec_terms 1521-1525 at the glass's F0); 0: not glass (glass_at 955-969)
-__device__ float glass_share(const Ctx &c, const float4 *m, float cs) {
+__device__ float glass_share(const Ctx &c, const float4 *m, float cs, float tr_map) {
     const float4 params = m[2], layer = m[7];
-    const float tr = clampf(layer.x, 0.0f, 1.0f) * (1.0f - clampf(params.x, 0.0f, 1.0f));
+    float tr = clampf(layer.x, 0.0f, 1.0f) * (1.0f - clampf(params.x, 0.0f, 1.0f));
     if (tr <= 0.0f) return 0.0f;
+    tr = tr * tr_map;  // the transmission map's red, 1 without one
     const float ior = fmaxf(layer.y, 1.0001f);
     const float ri = (ior - 1.0f) / (ior + 1.0f);
     const float f0 = ri * ri;
@@ -697,14 +939,16 @@ __device__ float glass_share(const Ctx &c, const float4 *m, float cs) {
 }
 // glass_tr (885-906): past every opaque caster, what the glass a shadow ray crosses lets through (up to 8 surfaces,
 // the ray kept straight: no refraction); toward the sky an odd number of glass surfaces lets nothing through
-template <bool kCount>
+template <bool kCount, bool kTex>
 __device__ V glass_tr(const Ctx &c, V o, V d, float tmax, bool to_sky) {
     float left = tmax;
     V tr = v3(1.0f, 1.0f, 1.0f);
     bool odd = false;
+    float t0 = 0.0f;  // the distance from the ray's start (the maps' level, as a camera ray's footprint there)
     for (uint32_t k = 0; k < 8u; k++) {
         const HitRec h = c.trace<kCount>(o, d, left, kTraceClosest);
         if (h.tri == kNone) break;
+        t0 += h.t;
         // glass_at (930-970): the hit's geometric normal, its material's share at this angle and its tint
         V e1, e2;
         TriAttr a = tri_attr(c, h.tri, h.inst, h.part, e1, e2);
@@ -717,9 +961,24 @@ __device__ V glass_tr(const Ctx &c, V o, V d, float tmax, bool to_sky) {
         }
         const float4 *m = c.material(material_of(c, a, h.inst));
         const V ng = vs(normalize(cross(e1, e2)), side);
-        const float share = glass_share(c, m, fabsf(dot(ng, d)));
+        V tint = xyz(m[0]);
+        float tr_map = 1.0f;
+        if (kTex && clampf(m[7].x, 0.0f, 1.0f) * (1.0f - clampf(m[2].x, 0.0f, 1.0f)) > 0.0f &&
+            (as_i32(m[3].x) >= 0 || m[5].z > 0.5f)) {  // 950-956
+            float uu, vv, d0, d1;
+            mesh_uv(a, m, h.u, h.v, uu, vv);
+            const float fp = fmaxf(t0 * 2.0f * c.pf(CAM_POS, 3) / lod_height(c), 1e-7f);
+            mesh_density(e1, e2, a, m, d0, d1);
+            const float lod = mesh_lod(d0, d1, fp * fp, 1.0f) - kLodBias;
+            if (as_i32(m[3].x) >= 0) {
+                const float4 tx = tex(c, as_i32(m[3].x), uu, vv, lod);
+                tint = vmul(tint, srgb_to_linear(v3(tx.x, tx.y, tx.z)));
+            }
+            if (m[5].z > 0.5f) tr_map = tex(c, static_cast<int32_t>(m[5].z) - 1, uu, vv, lod).x;
+        }
+        const float share = glass_share(c, m, fabsf(dot(ng, d)), tr_map);
         if (share > 0.0f) {
-            tr = vmul(tr, vs(xyz(m[0]), share));
+            tr = vmul(tr, vs(tint, share));
             odd = !odd;
         }
         o = offset_origin(vadd(o, vs(d, h.t)), dot(ng, d) < 0.0f ? vneg(ng) : ng);
@@ -731,10 +990,10 @@ __device__ V glass_tr(const Ctx &c, V o, V d, float tmax, bool to_sky) {
 }
 // shadow_tr (846-855): 0 past a shadow caster; else what the glass lets through (objects that cast no shadow are
 // glass); sky_tr (975-977) with to_sky
-template <bool kCount>
+template <bool kCount, bool kTex>
 __device__ V shadow_tr(const Ctx &c, V o, V d, float tmax, bool to_sky) {
     if (c.trace<kCount>(o, d, tmax, kTraceShadow).tri != kNone) return v3(0.0f, 0.0f, 0.0f);
-    return glass_tr<kCount>(c, o, d, tmax, to_sky);
+    return glass_tr<kCount, kTex>(c, o, d, tmax, to_sky);
 }
 // light_origin (2153-2156): a shadow ray's origin on the side l leaves the surface from
 template <bool kGlass>
@@ -911,12 +1170,21 @@ __device__ __forceinline__ Em emitter_tri_at(const Ctx &c, uint32_t i, uint32_t 
     return {xyz(r0), xyz(r1), xyz(r2), __float_as_uint(r0.w), __float_as_uint(r1.w), __float_as_uint(r2.w),
             __float_as_uint(r3.x), __float_as_uint(r3.y)};
 }
-__device__ V emitter_radiance(const Ctx &c, const Em &em) {
+// emitter_radiance (2269-2283): the emission at the sample (b1, b2), its map read at a 5 cm footprint
+template <bool kTex>
+__device__ V emitter_radiance(const Ctx &c, const Em &em, float b1, float b2) {
     V e1, e2;
     TriAttr a = tri_attr(c, em.tri, em.inst, em.part, e1, e2);
     if (em.inst != kNone) a.x.y = __uint_as_float(__float_as_uint(c.inst(em.inst)[6].y) + __float_as_uint(a.x.y));
     const float4 *m = c.material(material_of(c, a, em.inst));
-    return vs(xyz(m[1]), c.obj_power(__float_as_uint(a.x.y)));
+    float uu = 0.0f, vv = 0.0f, lod = 0.0f;
+    if (kTex) {
+        float d0, d1;
+        mesh_uv(a, m, b1, b2, uu, vv);
+        mesh_density(em.e1, em.e2, a, m, d0, d1);
+        lod = mesh_lod(d0, d1, 0.0025f, 1.0f);
+    }
+    return vs(emission<kTex>(c, m, uu, vv, lod), c.obj_power(__float_as_uint(a.x.y)));
 }
 
 struct ES {
@@ -924,7 +1192,7 @@ struct ES {
     float dist, pdf_w, w;
 };
 
-template <bool kTree, bool kGlass = false>
+template <bool kTree, bool kGlass = false, bool kTex = false>
 __device__ ES sample_emitter_ris(Ctx &c, V x, V n, uint32_t m, const Surface &sf, const Lobes &lb, V v, uint32_t rd) {
     ES r{v3(0.0f, 0.0f, 0.0f), v3(0.0f, 1.0f, 0.0f), 0.0f, 0.0f, 0.0f};
     float wsum = 0.0f, pick_b1 = 0.0f, pick_b2 = 0.0f, pick_ph = 0.0f, pick_pdf = 0.0f;
@@ -969,7 +1237,7 @@ __device__ ES sample_emitter_ris(Ctx &c, V x, V n, uint32_t m, const Surface &sf
     float cos_l = dot(ln, vneg(l));
     if (!(em.flags & kEmitterCard)) cos_l = fabsf(cos_l);
     if (cos_l <= 0.0f) return r;
-    r.le = emitter_radiance(c, em);
+    r.le = emitter_radiance<kTex>(c, em, pick_b1, pick_b2);
     r.l = l;
     r.dist = dist;
     r.pdf_w = pick_pdf * d2 / cos_l;
@@ -977,7 +1245,7 @@ __device__ ES sample_emitter_ris(Ctx &c, V x, V n, uint32_t m, const Surface &sf
     return r;
 }
 
-template <bool kTree>
+template <bool kTree, bool kTex = false>
 __device__ ES sample_emitter(Ctx &c, V x, V n, uint32_t rd) {
     ES r{v3(0.0f, 0.0f, 0.0f), v3(0.0f, 1.0f, 0.0f), 0.0f, 0.0f, 0.0f};
     const EPick pk = emitter_draw<kTree>(c, x, n, rd);
@@ -1000,7 +1268,7 @@ __device__ ES sample_emitter(Ctx &c, V x, V n, uint32_t rd) {
     float cos_l = dot(ln, vneg(l));
     if (!(em.flags & kEmitterCard)) cos_l = fabsf(cos_l);
     if (cos_l <= 0.0f) return r;
-    r.le = emitter_radiance(c, em);
+    r.le = emitter_radiance<kTex>(c, em, b1, b2);
     r.l = l;
     r.dist = dist;
     r.pdf_w = pk_p / area * d2 / cos_l;
@@ -1034,7 +1302,7 @@ struct WfShadows {
 // kDefer (the wavefront): each contribution goes to `q` with its shadow ray, untraced, and the sum returned is 0.
 // kGlass (the megakernel alone): shadow rays leave from the far side toward lights behind rough glass, the suns' and
 // the sky's pass glass at its share (shadow_tr), and rough glass samples emitters from both sides.
-template <bool kTree, bool kCount, bool kDefer = false, bool kGlass = false>
+template <bool kTree, bool kCount, bool kDefer = false, bool kGlass = false, bool kTex = false>
 __device__ V direct_light(Ctx &c, const Surface &s, const Lobes &lb, V v, WfShadows *q = nullptr) {
     static_assert(!(kDefer && kGlass), "the wavefront does not render glass");
     V sum = v3(0.0f, 0.0f, 0.0f);
@@ -1060,7 +1328,7 @@ __device__ V direct_light(Ctx &c, const Surface &s, const Lobes &lb, V v, WfShad
             const float pdf_l = 1.0f / (kTwoPi * (1.0f - cos_max));
             q->put(vs(vmul(col, f), light_mis(pdf_l, s, lb, v, l)), li[3].w >= 0.5f, o, l, kInf, kTraceShadow);
         } else if (kGlass && any_pos(f)) {
-            const V vis = li[3].w >= 0.5f ? shadow_tr<kCount>(c, light_origin<kGlass>(s, o, l), l, kInf, false)
+            const V vis = li[3].w >= 0.5f ? shadow_tr<kCount, kTex>(c, light_origin<kGlass>(s, o, l), l, kInf, false)
                                           : v3(1.0f, 1.0f, 1.0f);
             if (any_pos(vis)) {
                 const float pdf_l = 1.0f / (kTwoPi * (1.0f - cos_max));
@@ -1085,15 +1353,15 @@ __device__ V direct_light(Ctx &c, const Surface &s, const Lobes &lb, V v, WfShad
         const bool seen = c.g.depth == 0u || (c.g.depth == 1u && c.mirror_path);
         // with the tree its candidates are already near the point: P.ltree.w of them at camera hits, one elsewhere
         if (kTree && lt_on(c))
-            es = sample_emitter_ris<kTree, kGlass>(c, o, ne, c.g.depth == 0u ? static_cast<uint32_t>(c.pf(LTREE, 3)) : 1u,
-                                                   s, lb, v, c.g.dim(D_EMIT));
+            es = sample_emitter_ris<kTree, kGlass, kTex>(
+                c, o, ne, c.g.depth == 0u ? static_cast<uint32_t>(c.pf(LTREE, 3)) : 1u, s, lb, v, c.g.dim(D_EMIT));
         else if (seen)
-            es = sample_emitter_ris<kTree, kGlass>(c, o, ne, c.g.depth == 0u ? kEmitRisM : kEmitRisM / 2u, s, lb, v,
-                                                   c.g.dim(D_EMIT));
+            es = sample_emitter_ris<kTree, kGlass, kTex>(c, o, ne, c.g.depth == 0u ? kEmitRisM : kEmitRisM / 2u, s, lb,
+                                                         v, c.g.dim(D_EMIT));
         else if (c.pf(LIGHT_LOD, 3) > 0.0f)
-            es = sample_emitter_ris<kTree, kGlass>(c, o, ne, 2u, s, lb, v, c.g.dim(D_EMIT));
+            es = sample_emitter_ris<kTree, kGlass, kTex>(c, o, ne, 2u, s, lb, v, c.g.dim(D_EMIT));
         else
-            es = sample_emitter<kTree>(c, o, ne, c.g.dim(D_EMIT));
+            es = sample_emitter<kTree, kTex>(c, o, ne, c.g.dim(D_EMIT));
         if (es.pdf_w > 0.0f) {
             const V f = eval_bsdf<kGlass>(s, lb, v, es.l);
             if (kDefer && any_pos(f)) {
@@ -1118,7 +1386,7 @@ __device__ V direct_light(Ctx &c, const Surface &s, const Lobes &lb, V v, WfShad
         } else if (kGlass && any_pos(f)) {
             const float pdf_e = env_pdf();
             if (pdf_e > 0.0f) {
-                const V vis = shadow_tr<kCount>(c, light_origin<kGlass>(s, o, l), l, kInf, true);
+                const V vis = shadow_tr<kCount, kTex>(c, light_origin<kGlass>(s, o, l), l, kInf, true);
                 if (any_pos(vis))
                     sum = vadd(sum, vmul(vds(vs(vmul(sky(c), f), light_mis<kGlass>(pdf_e, s, lb, v, l)), pdf_e), vis));
             }
@@ -1184,7 +1452,7 @@ __device__ __forceinline__ void camera_ray(Ctx &c, uint32_t px, uint32_t py, uin
 }
 
 // One path of sample `index` at pixel (px, py): its radiance, the camera hit's albedo and normal.
-template <bool kCoat, bool kTree, bool kCount, bool kGlass>
+template <bool kCoat, bool kTree, bool kCount, bool kGlass, bool kTex>
 __device__ void path(Ctx &c, uint32_t px, uint32_t py, uint32_t index, V &radiance, V &albedo, V &normal) {
     V o, d;
     camera_ray(c, px, py, index, o, d);
@@ -1196,6 +1464,8 @@ __device__ void path(Ctx &c, uint32_t px, uint32_t py, uint32_t index, V &radian
     // glass passed since the last scattering vertex (pathtrace.wgsl:4156-4162): any, and an odd number of surfaces
     bool behind_glass = false, bent = false;
     float last_pdf = 0.0f;
+    // the ray cone (Akenine-Moller et al. 2019) for the textures' level: width at the origin, spread angle (4165-4167)
+    float cone_w = 0.0f, cone_a = kTex ? 2.0f * c.pf(CAM_POS, 3) / lod_height(c) : 0.0f;
     const float clamp_v = c.pf(MISC, 0);
     c.mirror_path = false;
     const uint32_t last = c.pu(COUNTS, 2);
@@ -1208,7 +1478,8 @@ __device__ void path(Ctx &c, uint32_t px, uint32_t py, uint32_t index, V &radian
             radiance = vadd(radiance, col);
             break;
         }
-        const Surface s = surface<kCoat, kGlass>(c, o, d, hit);
+        const float width = kTex ? cone_w + cone_a * hit.t : 0.0f;
+        const Surface s = surface<kCoat, kGlass, kTex>(c, o, d, hit, width);
         if (kCount) c.cnt->shaded++;
         if (depth == 0u) {
             if (s.card) {
@@ -1231,7 +1502,7 @@ __device__ void path(Ctx &c, uint32_t px, uint32_t py, uint32_t index, V &radian
         const V v = vneg(d);
         const float nov = fmaxf(dot(s.n, v), 1e-4f);
         const Lobes lb = lobes<kGlass>(c, s, nov);
-        V dl = vmul(thr, direct_light<kTree, kCount, false, kGlass>(c, s, lb, v));
+        V dl = vmul(thr, direct_light<kTree, kCount, false, kGlass, kTex>(c, s, lb, v));
         if (depth > 0u && !straight) dl = clamp_lum(dl, clamp_v);
         radiance = vadd(radiance, dl);
         const V l = sample_bsdf<kGlass>(c, s, lb, v);
@@ -1256,6 +1527,10 @@ __device__ void path(Ctx &c, uint32_t px, uint32_t py, uint32_t index, V &radian
                 bent = false;
             }
             straight = straight && through;
+            if (kTex) {
+                cone_w = width;
+                cone_a += 2.0f * s.roughness * s.roughness;
+            }
         } else {
             const float pdf = pdf_bsdf(s, lb, v, l);
             const V wgt = vds(eval_bsdf(s, lb, v, l), fmaxf(pdf, 1e-20f));
@@ -1264,6 +1539,10 @@ __device__ void path(Ctx &c, uint32_t px, uint32_t py, uint32_t index, V &radian
             last_pdf = pdf;
             specular_bounce = false;
             straight = false;
+            if (kTex) {
+                cone_w = width;
+                cone_a += 2.0f * s.roughness * s.roughness;
+            }
         }
         c.mirror_path = (depth == 0u || c.mirror_path) && s.roughness < kMirrorRoughness &&
                         (c.sampled_spec || (kGlass && c.sampled_trans));
@@ -1288,7 +1567,7 @@ __device__ void path(Ctx &c, uint32_t px, uint32_t py, uint32_t index, V &radian
 // frame 2.46 -> 1.95 ms, the 3M-triangle scene 1.36x faster, the Cornell box 5 % (four waves: 2.00 ms, 1.34x, 11 %).
 // kCount: a counting launch, which adds what its walks read and the hits it shades to reads[0..3] (nodes,
 // triangles, units, shaded hits).
-template <bool kCoat, bool kTree, bool kCount, bool kGlass>
+template <bool kCoat, bool kTree, bool kCount, bool kGlass, bool kTex>
 __global__ __launch_bounds__(kRenderBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_render(
     const RenderJob *jobs, uint32_t njobs, uint32_t total, uint32_t *counter, uint32_t *spill, uint32_t spill_stride,
     uint32_t spill_entries, uint32_t *guard, unsigned long long *rays, unsigned long long *reads) {
@@ -1325,7 +1604,7 @@ __global__ __launch_bounds__(kRenderBlock) __attribute__((amdgpu_waves_per_eu(3)
             float4 col = pix[0], alb = pix[1], nrm = pix[2];
             for (uint32_t s = 0; s < a->spp; s++) {
                 V rad, al, nm;
-                path<kCoat, kTree, kCount, kGlass>(c, px, py, a->first_sample + s, rad, al, nm);
+                path<kCoat, kTree, kCount, kGlass, kTex>(c, px, py, a->first_sample + s, rad, al, nm);
                 const float lum = luminance(rad);
                 col.x += rad.x, col.y += rad.y, col.z += rad.z, col.w += 1.0f;
                 alb.x += al.x, alb.y += al.y, alb.z += al.z, alb.w += lum;
@@ -1684,7 +1963,7 @@ __global__ void k_reset(uint32_t *c) { c[threadIdx.x * kShardGap] = 0; }
 int render_occupancy(int *blocks_per_cu) {
     int b = 0;
     const hipError_t e =
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_render<false, false, false, false>, kRenderBlock, 0);
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_render<false, false, false, false, false>, kRenderBlock, 0);
     *blocks_per_cu = b;
     return e == hipSuccess ? 0 : -1;
 }
@@ -1720,36 +1999,39 @@ int launch_wf_accum(const WfArgs &a, uint32_t pixels, int grid, hipStream_t s) {
 
 int launch_render(const RenderJob *d_jobs, uint32_t njobs, uint32_t total_tiles, uint32_t *counter,
                   uint32_t *spill, uint32_t spill_stride, uint32_t spill_entries, uint32_t *guard,
-                  unsigned long long *rays, bool coat, bool tree, bool glass, unsigned long long *reads, int grid,
-                  hipStream_t s) {
+                  unsigned long long *rays, bool coat, bool tree, bool glass, bool tex, unsigned long long *reads,
+                  int grid, hipStream_t s) {
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(kShards), 0, s, counter);
-#define TRT_RENDER(C, T, N, G)                                                                                       \
-    hipLaunchKernelGGL((k_render<C, T, N, G>), dim3(grid), dim3(kRenderBlock), 0, s, d_jobs, njobs, total_tiles,     \
+#define TRT_RENDER(C, T, N, G, X)                                                                                    \
+    hipLaunchKernelGGL((k_render<C, T, N, G, X>), dim3(grid), dim3(kRenderBlock), 0, s, d_jobs, njobs, total_tiles,  \
                        counter, spill, spill_stride, spill_entries, guard, rays, reads)
-#define TRT_RENDER_N(C, T)            \
-    if (reads)                        \
-        TRT_RENDER(C, T, true, false); \
-    else                              \
-        TRT_RENDER(C, T, false, false)
-    // glass scenes take the kernel with the coat's terms as well (the same bits for materials without a coat)
-    if (glass && tree) {
-        if (reads)
-            TRT_RENDER(true, true, true, true);
-        else
-            TRT_RENDER(true, true, false, true);
+#define TRT_RENDER_N(C, T, G, X)          \
+    if (reads)                            \
+        TRT_RENDER(C, T, true, G, X);     \
+    else                                  \
+        TRT_RENDER(C, T, false, G, X)
+    // Glass and textured scenes take the kernel with the coat's terms as well (the same bits for materials without
+    // a coat), textured ones with or without the glass code.
+    if (tex && glass && tree) {
+        TRT_RENDER_N(true, true, true, true);
+    } else if (tex && glass) {
+        TRT_RENDER_N(true, false, true, true);
+    } else if (tex && tree) {
+        TRT_RENDER_N(true, true, false, true);
+    } else if (tex) {
+        TRT_RENDER_N(true, false, false, true);
+    } else if (glass && tree) {
+        TRT_RENDER_N(true, true, true, false);
     } else if (glass) {
-        if (reads)
-            TRT_RENDER(true, false, true, true);
-        else
-            TRT_RENDER(true, false, false, true);
+        TRT_RENDER_N(true, false, true, false);
     } else if (coat && tree) {
-        TRT_RENDER_N(true, true);
+        TRT_RENDER_N(true, true, false, false);
     } else if (coat) {
-        TRT_RENDER_N(true, false);
+        TRT_RENDER_N(true, false, false, false);
     } else if (tree) {
-        TRT_RENDER_N(false, true);
+        TRT_RENDER_N(false, true, false, false);
     } else {
-        TRT_RENDER_N(false, false);
+        TRT_RENDER_N(false, false, false, false);
     }
 #undef TRT_RENDER_N
 #undef TRT_RENDER

@@ -30,6 +30,10 @@ struct RenderArgs {
     const float4 *energy;       // lib.rs energy_lut, 512 x float4
     const float4 *light_tree;   // ltree.rs LightTree nodes, one float4 each
     const float4 *glass_energy; // lib.rs glass_energy_lut, 1024 x float4 (a glass scene's; NULL otherwise)
+    const uint32_t *tex_pages;  // texpack.rs pages: RGBA8 texels, per layer levels 0..4 (a textured scene's)
+    const float4 *tex_info;     // texpack.rs Atlas::info, 2 x float4 a texture
+    uint64_t tex_info_count;    // its float4s
+    uint32_t tex_side, tex_layers;
     float4 *accum;              // records.wgsl Pixel, 5 x float4 a pixel
     // the tables' sizes (in their records), against which every index is checked
     uint32_t vertex_count, tri_count, row_count, material_count, light_count, emitter_vec4s, emitter_tri_count,
@@ -98,11 +102,12 @@ int launch_resolve(const float4 *acc, uint32_t w, uint32_t h, uint32_t pw, uint3
 // tile counters. rays: the sharded ray counts the launch adds its traced rays to (closest; shadow and any-hit); coat: some job's scene has a
 // material with a clearcoat (the kernel with the coat's terms; the same bits for scenes without one); tree: some
 // job's Params ask for the light tree (the kernel with its code); glass: some job's scene has glass (the kernel with
-// the dielectric and glass along shadow rays; every such job's glass_energy set); reads: NULL, or four counters a
-// counting launch adds to (wide nodes, triangles, units its walks read; hits it shades)
+// the dielectric and glass along shadow rays; every such job's glass_energy set); tex: some job's scene has textures
+// (the kernel with the maps and the ray cone; every such job's tex_* set); reads: NULL, or four counters a counting
+// launch adds to (wide nodes, triangles, units its walks read; hits it shades)
 int launch_render(const RenderJob *d_jobs, uint32_t njobs, uint32_t total_tiles, uint32_t *counter,
                   uint32_t *spill, uint32_t spill_stride, uint32_t spill_entries, uint32_t *guard,
-                  unsigned long long *rays, bool coat, bool tree, bool glass, unsigned long long *reads, int grid,
-                  hipStream_t s);
+                  unsigned long long *rays, bool coat, bool tree, bool glass, bool tex, unsigned long long *reads,
+                  int grid, hipStream_t s);
 
 }  // namespace trt

@@ -35,17 +35,23 @@ ROW_GLASS = 1 << 25  # scene.rs:103: a row's class bit for glass (mesh_class 2)
 
 
 def material(base=(0.8, 0.8, 0.8), emissive=(0.0, 0.0, 0.0), metallic=0.0, roughness=1.0, specular=0.5,
-             clearcoat=0.0, clearcoat_roughness=0.0, transmission=0.0, ior=1.5) -> np.ndarray:
-    """scene.rs:480-491 GpuMaterial (72 words): base colour, emission, (metallic, roughness, specular, normal scale),
-    no textures; the layer (scene.rs:1257): transmission (glass, clamped to [0, 1]), index of refraction, clearcoat
-    and its roughness (both clamped to [0, 1])."""
+             clearcoat=0.0, clearcoat_roughness=0.0, transmission=0.0, ior=1.5, base_map=-1, normal_map=-1,
+             roughness_map=-1, metalness_map=-1, emissive_map=-1, transmission_map=-1, normal_scale=1.0,
+             repeat=(1.0, 1.0), offset=(0.0, 0.0)) -> np.ndarray:
+    """scene.rs:480-491 GpuMaterial (72 words; scene.rs:1238-1262 build_material): base colour, emission, (metallic,
+    roughness, specular, normal scale); the texture slots (tex0: base colour (sRGB), normal, roughness (green),
+    metalness (blue); tex1.x emissive (sRGB); uv.z the transmission map's slot + 1 (red); -1 none), the uv repeat and
+    offset; the layer (scene.rs:1257): transmission (glass, clamped to [0, 1]), index of refraction, clearcoat and its
+    roughness (both clamped to [0, 1])."""
     m = np.zeros(72, np.float32)
     m[0:3] = base
     m[3] = 1.0
     m[4:7] = emissive
-    m[8:12] = (metallic, roughness, specular, 1.0)
-    m[12:20] = np.array([-1, -1, -1, -1, -1, -1, 0, 0], np.int32).view(np.float32)
-    m[20:24] = (1.0, 1.0, 0.0, 0.0)
+    m[8:12] = (metallic, roughness, specular, normal_scale)
+    m[12:20] = np.array([base_map, normal_map, roughness_map, metalness_map, emissive_map, -1, 0, 0],
+                        np.int32).view(np.float32)
+    m[20:24] = (repeat[0], repeat[1], float(transmission_map) + 1.0, 0.0)
+    m[24:26] = offset
     m[28:32] = (min(max(transmission, 0.0), 1.0), ior, min(max(clearcoat, 0.0), 1.0),
                 min(max(clearcoat_roughness, 0.0), 1.0))
     m[64:72] = np.full(8, -1, np.int32).view(np.float32)
@@ -208,6 +214,100 @@ class RMesh:
     tris: np.ndarray                   # (n, 3, 3) model space
     normals: np.ndarray | None = None  # (n, 3, 3) per corner; None: the triangle's own
     sections: np.ndarray | None = None  # (n,) section of each triangle
+    uvs: np.ndarray | None = None      # (n, 3, 2) per corner; None: zeros
+    tangents: np.ndarray | None = None  # (n, 3, 4) per corner (xyz, hand); None: zeros (hand 0: derived from the uvs)
+
+
+# ---- texpack.rs: texture pages ----------------------------------------------------------------------------------
+TEX_GUTTER, TEX_MIPS, TEX_ALIGN, TEX_MAX_PAGE, TEX_MAX_DIM = 16, 5, 16, 4096, 2048
+
+
+@dataclass
+class Texture:
+    rgba: np.ndarray   # (h, w, 4) uint8, as stored (sRGB-encoded colour maps are decoded by the kernel)
+    wrap: bool = True  # repeat (else clamp) outside [0, 1]
+
+
+def _content_size(w: int, h: int) -> tuple[int, int]:
+    """texpack.rs:48-60 content_size."""
+    cw, ch = min(w, TEX_MAX_DIM), min(h, TEX_MAX_DIM)
+    k = min(np.float32(cw) / np.float32(w), np.float32(ch) / np.float32(h))
+    cw, ch = max(int(np.float32(w) * k), 1), max(int(np.float32(h) * k), 1)
+    if cw >= 512 and cw & (cw - 1) == 0:
+        cw -= 2 * TEX_GUTTER
+    if ch >= 512 and ch & (ch - 1) == 0:
+        ch -= 2 * TEX_GUTTER
+    return cw, ch
+
+
+def _mip_chain(level0: np.ndarray) -> list:
+    """texpack.rs:396-412 mip_chain: 2 x 2 boxes of the stored values, rounded ((a + b + c + d + 2) / 4)."""
+    chain = [level0]
+    for _ in range(1, TEX_MIPS):
+        p = chain[-1].astype(np.uint32)
+        h, w = max(p.shape[0] // 2, 1), max(p.shape[1] // 2, 1)
+        s = p[0:2 * h:2, 0:2 * w:2] + p[0:2 * h:2, 1:2 * w:2] + p[1:2 * h:2, 0:2 * w:2] + p[1:2 * h:2, 1:2 * w:2]
+        chain.append(((s + 2) // 4).astype(np.uint8))
+    return chain
+
+
+def pack_textures(textures: list) -> tuple[np.ndarray, int, int, np.ndarray]:
+    """texpack.rs:298-393 pack and Atlas::info (72-86): every texture shelf-packed into square RGBA8 layers with
+    16-texel gutters of wrapped (or clamped) texels, tallest first, each layer's 2 x 2 box mip chain. Returns (pages:
+    per layer levels 0..4 one after the other, side, layers, info: 2 vec4 a texture). Textures whose size the
+    reference's packer changes (wider than 2048, or a power of two from 512) are refused: its resize
+    (texpack.rs:415 resize_triangle) is not restated here; the library takes any pages."""
+    src = []
+    for k, tx in enumerate(textures):
+        img = np.ascontiguousarray(tx.rgba, np.uint8)
+        if img.ndim != 3 or img.shape[2] != 4:
+            raise ValueError(f"texture {k}: rgba must be (h, w, 4) uint8")
+        h, w = img.shape[:2]
+        if _content_size(w, h) != (w, h):
+            raise ValueError(f"texture {k}: {w} x {h} would be resized by the reference's packer (not restated)")
+        src.append(img)
+    up = lambda v: (v + TEX_ALIGN - 1) // TEX_ALIGN * TEX_ALIGN  # noqa: E731
+    padded = [(up(i.shape[1] + 2 * TEX_GUTTER), up(i.shape[0] + 2 * TEX_GUTTER)) for i in src]
+    order = sorted(range(len(src)), key=lambda i: (-padded[i][1], -padded[i][0]))
+    area = sum(pw * ph for pw, ph in padded)
+    widest = max((max(p) for p in padded), default=1)
+    side = max(1 << (widest - 1).bit_length(), 256)
+    while side < TEX_MAX_PAGE and side * side < area:
+        side *= 2
+    place = [(0, 0, 0)] * len(src)
+    layer = x = y = shelf = 0
+    for i in order:
+        pw, ph = padded[i]
+        if x + pw > side:
+            x, y, shelf = 0, y + shelf, 0
+        if y + ph > side:
+            layer, x, y, shelf = layer + 1, 0, 0, 0
+        place[i] = (layer, x, y)
+        x += pw
+        shelf = max(shelf, ph)
+    pages = []
+    for lyr in range(layer + 1):
+        page = np.zeros((side, side, 4), np.uint8)
+        for i in (j for j in range(len(src)) if place[j][0] == lyr):
+            _, px, py = place[i]
+            pw, ph = padded[i]
+            h, w = src[i].shape[:2]
+            if textures[i].wrap:
+                rows, cols = np.arange(ph) - TEX_GUTTER, np.arange(pw) - TEX_GUTTER
+                rows, cols = rows % h, cols % w
+            else:
+                rows = np.clip(np.arange(ph) - TEX_GUTTER, 0, h - 1)
+                cols = np.clip(np.arange(pw) - TEX_GUTTER, 0, w - 1)
+            page[py:py + ph, px:px + pw] = src[i][rows[:, None], cols[None, :]]
+        pages.append(np.concatenate([lv.reshape(-1) for lv in _mip_chain(page)]))
+    s = np.float32(side)
+    info = np.zeros((max(len(src), 1) * 2, 4), np.float32)
+    for i, (lyr, px, py) in enumerate(place):
+        h, w = src[i].shape[:2]
+        info[2 * i] = (np.float32(px + TEX_GUTTER) / s, np.float32(py + TEX_GUTTER) / s, np.float32(w) / s,
+                       np.float32(h) / s)
+        info[2 * i + 1] = (lyr, 1.0 if textures[i].wrap else 0.0, w, h)
+    return np.concatenate(pages), side, layer + 1, info
 
 
 @dataclass
@@ -247,12 +347,18 @@ class RenderScene:
     tree_entries: int = 0
     # lib.rs glass_energy_lut (1024 vec4) when a material has glass (gpu.rs features_of: layer.x > 0); None: no glass
     glass_energy: np.ndarray | None = None
+    # texpack.rs pages (per layer, levels 0..4), their side and layers, and the info table; None: no textures
+    tex_pages: np.ndarray | None = None
+    tex_side: int = 0
+    tex_layers: int = 0
+    tex_info: np.ndarray | None = None
 
     def tables(self) -> dict:
         return {"vertices": self.vertices, "indices": self.indices, "rows": self.rows, "materials": self.materials,
                 "lights": self.lights, "emitters": self.emitters, "emitter_tris": self.emitter_tris,
                 "objects": self.objects, "energy": self.energy, "light_tree": self.light_tree,
-                "glass_energy": self.glass_energy}
+                "glass_energy": self.glass_energy, "tex_pages": self.tex_pages, "tex_side": self.tex_side,
+                "tex_layers": self.tex_layers, "tex_info": self.tex_info}
 
     def params(self, width: int, height: int, camera: Camera, seed: int = 1, max_bounces: int = 4, rr_depth: int = 3,
                clamp: float = 1e9, energy_compensation: float = 0.0, sky=(0.0, 0.0, 0.0), env_nee: bool = True,
@@ -295,7 +401,7 @@ class RenderScene:
 
 
 def build(meshes: list[RMesh], instances: list[RInstance], materials: list[np.ndarray],
-          lights: list[np.ndarray] = (), light_tree: bool = True) -> RenderScene:
+          lights: list[np.ndarray] = (), light_tree: bool = True, textures: list = ()) -> RenderScene:
     """Lays out a scene as prepare does: trees and records (scene.prepare), then vertices, indices, rows, emitters.
     With 256 emitters or more that weigh something, a light tree over them (lib.rs:650-676, on unless THREERT_LTREE=0:
     here `light_tree=False`), whose emitters come first in the CDF in its leaf order. Glass (a material with
@@ -327,6 +433,11 @@ def build(meshes: list[RMesh], instances: list[RInstance], materials: list[np.nd
         v = np.zeros((n * 3, 12), np.float32)
         v[:, 0:3] = tris.reshape(-1, 3)
         v[:, 4:7] = nrm.reshape(-1, 3)
+        if m.uvs is not None:  # indexed.rs Vertex: (position, u), (normal, v), (tangent, hand)
+            uv = np.asarray(m.uvs, np.float32).reshape(-1, 2)
+            v[:, 3], v[:, 7] = uv[:, 0], uv[:, 1]
+        if m.tangents is not None:
+            v[:, 8:12] = np.asarray(m.tangents, np.float32).reshape(-1, 4)
         verts.append(v)
         vfirst.append(at)
         at += n * 3
@@ -432,6 +543,8 @@ def build(meshes: list[RMesh], instances: list[RInstance], materials: list[np.nd
         rs.tree_entries = len(tree.order)
     if np.any(mats[:, 28] > 0.0):
         rs.glass_energy = glass_energy_lut()
+    if len(textures):
+        rs.tex_pages, rs.tex_side, rs.tex_layers, rs.tex_info = pack_textures(list(textures))
     return rs
 
 
@@ -454,6 +567,12 @@ def upload(dev: Device, rs: RenderScene) -> Scene:
                        p.units, null(verts), len(verts), null(idx), len(idx), null(rows), len(rows), null(mats),
                        len(mats), null(lights), len(lights), null(ems), len(ems), null(emt), len(emt), null(objs),
                        len(objs) // OBJECT_STRIDE, _ptr(energy), null(ltree), len(ltree), null(gl))
+    if rs.tex_info is not None:
+        pages = np.ascontiguousarray(rs.tex_pages, np.uint8)
+        info = np.ascontiguousarray(rs.tex_info, np.float32)
+        keep += [pages, info]
+        desc.tex_pages, desc.tex_side, desc.tex_layers = _ptr(pages), rs.tex_side, rs.tex_layers
+        desc.tex_info, desc.tex_info_vec4s = _ptr(info), len(info)
     sc = Scene.__new__(Scene)
     sc.dev = dev
     h = C.c_void_p()

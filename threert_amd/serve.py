@@ -246,7 +246,7 @@ def params_of(e: _Resident, j: Job) -> np.ndarray:
 def scene_bytes(rs: R.RenderScene) -> int:
     p = rs.prep
     b = p.nodes.nbytes + p.leaves.nbytes + p.instances.nbytes
-    b += sum(a.nbytes for a in rs.tables().values() if a is not None)
+    b += sum(a.nbytes for a in rs.tables().values() if isinstance(a, np.ndarray))
     # the wide tree is at most twice the binary buffer; the leaves are copied once more in the wide trees' order
     return int(b + p.nodes.nbytes * 2 + p.leaves.nbytes)
 
