@@ -207,6 +207,115 @@ __global__ __launch_bounds__(256) void k_conv_c(ConvArgs a, uint32_t tiles_x, ui
     }
 }
 
+// k_conv_c for a decoder's concatenation, cin = 32 * (C0Q + C1Q): the second input (half resolution, upsampled by 2)
+// is staged in LDS once, at its own resolution ((TH / 2 + 2) x 6 pixels for the tile's (TH + 2) x 10 halo), and each
+// lane reads the half-resolution pixel under its full-resolution one. k_conv_c staged the upsampled copy, four times
+// the pixels: twice the LDS, fewer blocks a CU, and four reads of every half-resolution pixel. Per output the same
+// MFMAs in the same order (a tap's first-input channels, then its second input's), so the same bits.
+template <int NCO, int TH, int C0Q, int C1Q>
+__global__ __launch_bounds__(256) void k_conv_up(ConvArgs a, uint32_t tiles_x, uint32_t tiles_y) {
+    static_assert(TH == 8 || TH == 16, "tiles are 8 or 16 pixels tall");
+    constexpr int S = TH * NCO / 8;
+    constexpr uint32_t c0 = C0Q * 32, c1 = C1Q * 32, cin = c0 + c1, st0 = c0 + 8, st1 = c1 + 8;
+    constexpr uint32_t R1 = TH / 2 + 2, W1 = 6;                 // the half-resolution halo's rows and columns
+    constexpr uint32_t q0 = c0 / 8, q1 = c1 / 8;                // 16-byte chunks a pixel
+    constexpr uint32_t n0 = (TH + 2) * 10 * q0, items = n0 + R1 * W1 * q1, per = (items + 255) / 256;
+    extern __shared__ uint4 smem[];
+    uint16_t *halo0 = reinterpret_cast<uint16_t *>(smem);
+    uint16_t *halo1 = halo0 + (TH + 2) * 10 * st0;
+    const uint32_t tile = blockIdx.x;
+    const uint32_t tx = tile % tiles_x, ty = (tile / tiles_x) % tiles_y, n = tile / (tiles_x * tiles_y);
+    const int x0 = int(tx * 8) - 1, y0 = int(ty * TH) - 1;       // the full-resolution halo's origin
+    const int hx0 = int(tx * 4) - 1, hy0 = int(ty * TH / 2) - 1;  // the half-resolution halo's
+    const uint32_t h2 = a.h / 2, w2 = a.w / 2;
+    const uint16_t *in0 = a.in0 + uint64_t(n) * a.h * a.w * c0;
+    const uint16_t *in1 = a.in1 + uint64_t(n) * h2 * w2 * c1;
+    uint4 v[per];
+#pragma unroll
+    for (uint32_t k = 0; k < per; k++) {
+        const uint32_t q = threadIdx.x + k * 256;
+        v[k] = make_uint4(0, 0, 0, 0);
+        if (q < n0) {
+            const uint32_t hp = q / q0, ck = q % q0;
+            const uint32_t hy = hp / 10, hx = hp - hy * 10;
+            const int gy = y0 + int(hy), gx = x0 + int(hx);
+            if (uint32_t(gy) < a.h && uint32_t(gx) < a.w)
+                v[k] = *reinterpret_cast<const uint4 *>(in0 + (uint32_t(gy) * a.w + uint32_t(gx)) * c0 + ck * 8);
+        } else if ((k + 1) * 256 <= items || q < items) {
+            const uint32_t r = q - n0, hp = r / q1, ck = r % q1;
+            const uint32_t hy = hp / W1, hx = hp - hy * W1;
+            const int gy = hy0 + int(hy), gx = hx0 + int(hx);
+            if (uint32_t(gy) < h2 && uint32_t(gx) < w2)
+                v[k] = *reinterpret_cast<const uint4 *>(in1 + (uint32_t(gy) * w2 + uint32_t(gx)) * c1 + ck * 8);
+        }
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < per; k++) {
+        const uint32_t q = threadIdx.x + k * 256;
+        if (q < n0) {
+            *reinterpret_cast<uint4 *>(halo0 + (q / q0) * st0 + (q % q0) * 8) = v[k];
+        } else if ((k + 1) * 256 <= items || q < items) {
+            const uint32_t r = q - n0;
+            *reinterpret_cast<uint4 *>(halo1 + (r / q1) * st1 + (r % q1) * 8) = v[k];
+        }
+    }
+    __syncthreads();
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u, li = lane & 15u, g = lane >> 4;
+    const uint32_t co_tile = wave % NCO, grp = wave / NCO;
+    const uint32_t co_base = blockIdx.y * (16 * NCO) + co_tile * 16;
+    const uint16_t *wrow = a.wt + uint64_t(co_base + li) * 9 * cin + g * 8;
+    const uint16_t *hbase0 = halo0 + ((li >> 3) * 10 + (li & 7u) + 2 * grp * S * 10) * st0 + g * 8;
+    f32x4 acc[S];
+#pragma unroll
+    for (int j = 0; j < S; j++) acc[j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (uint32_t tap = 0; tap < 9; tap++) {
+        const uint32_t dy = tap / 3, dx = tap % 3;
+        const uint16_t *hrow0 = hbase0 + (dy * 10 + dx) * st0;
+        // the lane's full-resolution halo pixel (row 2 (grp S + j) + li / 8 + dy, column li % 8 + dx) lies on
+        // half-resolution halo pixel (grp S + j + (li / 8 + dy + 1) / 2, (li % 8 + dx + 1) / 2)
+        const uint32_t rr = ((li >> 3) + dy + 1) >> 1, cc = ((li & 7u) + dx + 1) >> 1;
+        const uint16_t *hrow1 = halo1 + ((grp * S + rr) * W1 + cc) * st1 + g * 8;
+#pragma unroll
+        for (uint32_t c = 0; c < c0; c += 32) {
+            const bf16x8 A = as_bf16x8(*reinterpret_cast<const uint4 *>(wrow + tap * cin + c));
+#pragma unroll
+            for (int j = 0; j < S; j++) {
+                const bf16x8 B = as_bf16x8(*reinterpret_cast<const uint4 *>(hrow0 + 2 * j * 10 * st0 + c));
+                acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[j], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (uint32_t c = 0; c < c1; c += 32) {
+            const bf16x8 A = as_bf16x8(*reinterpret_cast<const uint4 *>(wrow + tap * cin + c0 + c));
+#pragma unroll
+            for (int j = 0; j < S; j++) {
+                const bf16x8 B = as_bf16x8(*reinterpret_cast<const uint4 *>(hrow1 + j * W1 * st1 + c));
+                acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[j], 0, 0, 0);
+            }
+        }
+    }
+    const uint32_t co = co_base + 4 * g;
+    float b[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (a.bias)
+        for (int r = 0; r < 4; r++) b[r] = a.bias[co + r];
+#pragma unroll
+    for (int j = 0; j < S; j++) {
+        const uint32_t p = grp * S + j;
+        const uint32_t y = ty * TH + 2 * p + (li >> 3), x = tx * 8 + (li & 7u);
+        if (y >= a.h || x >= a.w) continue;
+        float r4[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            r4[r] = acc[j][r] + b[r];
+            if (a.relu) r4[r] = fmaxf(r4[r], 0.0f);
+        }
+        const uint64_t o = ((uint64_t(n) * a.h + y) * a.w + x) * a.cout + co;
+        if (a.out_f32) *reinterpret_cast<float4 *>(a.out_f32 + o) = make_float4(r4[0], r4[1], r4[2], r4[3]);
+        if (a.out_bf16)
+            *reinterpret_cast<uint2 *>(a.out_bf16 + o) = make_uint2(pack_bf16(r4[0], r4[1]), pack_bf16(r4[2], r4[3]));
+    }
+}
+
 // ---- weight gradient --------------------------------------------------------------------------------------------
 template <int NCO>
 __global__ __launch_bounds__(256) void k_wgrad(WgradArgs a, uint32_t tiles_x, uint32_t tiles_y, uint32_t tiles,
@@ -447,6 +556,27 @@ int launch_conv(const ConvArgs &a, hipStream_t s) {
     const dim3 grid(static_cast<uint32_t>(tiles), a.cout / (16 * nco));
     const uint32_t ch = cin / 32;
     if (a.out_color && (cin != 32 || a.c1)) return -1;  // the fused colour output takes cin = 32 only
+    // the decoder's concatenations (unet.py ARCH: 32 + 64, 64 + 64, 64 + 96, 96 + 128) stage their second input at half
+    // resolution
+    const int up = a.c1 == 0 ? 0 : (a.c0 == 32 && a.c1 == 64 && nco == 2) ? 1 : (a.c0 == 64 && a.c1 == 64 && nco == 4) ? 2
+                 : (a.c0 == 64 && a.c1 == 96 && nco == 4) ? 3 : (a.c0 == 96 && a.c1 == 128 && nco == 2) ? 4 : 0;
+    if (up) {
+        const size_t lds_up = (size_t(th + 2) * 10 * (a.c0 + 8) + size_t(th / 2 + 2) * 6 * (a.c1 + 8)) * 2;
+#define TRT_U(T)                                                                                                        \
+    switch (up) {                                                                                                       \
+    case 1: hipLaunchKernelGGL((k_conv_up<2, T, 1, 2>), grid, dim3(256), lds_up, s, a, tx, ty); break;                \
+    case 2: hipLaunchKernelGGL((k_conv_up<4, T, 2, 2>), grid, dim3(256), lds_up, s, a, tx, ty); break;                \
+    case 3: hipLaunchKernelGGL((k_conv_up<4, T, 2, 3>), grid, dim3(256), lds_up, s, a, tx, ty); break;                \
+    default: hipLaunchKernelGGL((k_conv_up<2, T, 3, 4>), grid, dim3(256), lds_up, s, a, tx, ty); break;               \
+    }
+        if (tall) {
+            TRT_U(16)
+        } else {
+            TRT_U(8)
+        }
+#undef TRT_U
+        return hipGetLastError() == hipSuccess ? 0 : -1;
+    }
     // (64 output channels over 64 inputs on levels of 256 rows or more stay on k_conv: k_conv_c<4, 16, 2> needs
     // 220 registers, one wave a SIMD, and ran those layers 20 % slower)
     if (ch >= 1 && ch <= 7 && ch != 6 && !(nco == 4 && ch == 2 && a.h >= 256)) {
