@@ -34,6 +34,15 @@ __device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float(
 __device__ __forceinline__ bf16x8 as_bf16x8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
 
 // ---- convolution ------------------------------------------------------------------------------------------------
+// The halo's rows in LDS: `width` pixels of `stride` elements (a multiple of 8: 16-byte units, an odd number of them),
+// the row padded to a pitch of 8 units modulo 16. A B operand's 16 lanes read 8 pixels of one halo row and the 8
+// under them; with an odd pixel stride the 8 land on 8 distinct 16-byte bank groups of the 16 and, a row 8 groups
+// further on, the other 8 land on the other 8: each ds_read_b128 is conflict-free (measured before: two thirds of
+// the LDS cycles of the full-resolution layers were bank conflicts).
+__host__ __device__ constexpr uint32_t row_pitch(uint32_t stride, uint32_t width) {
+    return (width * (stride / 8) + (8u + 16u - (width * (stride / 8)) % 16u) % 16u) * 8u;
+}
+
 // TH: the tile's height in pixels (8 or 16; the width is 8). A taller tile shares each halo row and each weight load
 // over more pixels: the full-resolution layers, whose blocks mostly wait on their halo, take 16.
 template <int NCO, int TH>
@@ -42,7 +51,7 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a, uint32_t tiles_x, uint
     constexpr int S = TH * NCO / 8;  // 16-pixel strips (two tile rows each) a wave takes
     extern __shared__ uint4 smem[];
     uint16_t *halo = reinterpret_cast<uint16_t *>(smem);
-    const uint32_t cin = a.c0 + a.c1, stride = cin + 8;
+    const uint32_t cin = a.c0 + a.c1, stride = cin + 8, rp = row_pitch(stride, 10);
     const uint32_t tile = blockIdx.x;
     const uint32_t tx = tile % tiles_x, ty = (tile / tiles_x) % tiles_y, n = tile / (tiles_x * tiles_y);
     const int x0 = int(tx * 8) - 1, y0 = int(ty * TH) - 1;
@@ -59,7 +68,7 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a, uint32_t tiles_x, uint
                 v = *reinterpret_cast<const uint4 *>(a.in1 + ((uint64_t(n) * h2 + gy / 2) * w2 + gx / 2) * a.c1 +
                                                      (ch - a.c0));
         }
-        *reinterpret_cast<uint4 *>(halo + hp * stride + ch) = v;
+        *reinterpret_cast<uint4 *>(halo + (hp / 10) * rp + (hp % 10) * stride + ch) = v;
     }
     __syncthreads();
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u, li = lane & 15u, g = lane >> 4;
@@ -71,13 +80,13 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a, uint32_t tiles_x, uint
     for (int j = 0; j < S; j++) acc[j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     for (uint32_t tap = 0; tap < 9; tap++) {
         const uint32_t dy = tap / 3, dx = tap % 3;
-        const uint16_t *hrow = halo + ((dy + (li >> 3)) * 10 + (li & 7u) + dx) * stride + g * 8;
+        const uint16_t *hrow = halo + (dy + (li >> 3)) * rp + ((li & 7u) + dx) * stride + g * 8;
         for (uint32_t c = 0; c < cin; c += 32) {
             const bf16x8 A = as_bf16x8(*reinterpret_cast<const uint4 *>(wrow + tap * cin + c));
 #pragma unroll
             for (int j = 0; j < S; j++) {
                 const uint32_t p = grp * S + j;  // the strip: tile rows 2p and 2p + 1
-                const bf16x8 B = as_bf16x8(*reinterpret_cast<const uint4 *>(hrow + 2 * p * 10 * stride + c));
+                const bf16x8 B = as_bf16x8(*reinterpret_cast<const uint4 *>(hrow + 2 * p * rp + c));
                 acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[j], 0, 0, 0);
             }
         }
@@ -123,7 +132,7 @@ template <int NCO, int TH, int CH>
 __global__ __launch_bounds__(256) void k_conv_c(ConvArgs a, uint32_t tiles_x, uint32_t tiles_y) {
     static_assert(TH == 8 || TH == 16, "tiles are 8 or 16 pixels tall");
     constexpr int S = TH * NCO / 8;  // 16-pixel strips (two tile rows each) a wave takes
-    constexpr uint32_t cin = CH * 32, stride = cin + 8, C4 = CH * 4;
+    constexpr uint32_t cin = CH * 32, stride = cin + 8, C4 = CH * 4, RP = row_pitch(stride, 10);
     constexpr uint32_t items = (TH + 2) * 10 * C4, per = (items + 255) / 256;
     extern __shared__ uint4 smem[];
     uint16_t *halo = reinterpret_cast<uint16_t *>(smem);
@@ -154,26 +163,26 @@ __global__ __launch_bounds__(256) void k_conv_c(ConvArgs a, uint32_t tiles_x, ui
     for (uint32_t k = 0; k < per; k++) {
         const uint32_t q = threadIdx.x + k * 256;
         if ((k + 1) * 256 <= items || q < items)
-            *reinterpret_cast<uint4 *>(halo + (q / C4) * stride + (q % C4) * 8) = v[k];
+            *reinterpret_cast<uint4 *>(halo + (q / C4 / 10) * RP + (q / C4 % 10) * stride + (q % C4) * 8) = v[k];
     }
     __syncthreads();
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u, li = lane & 15u, g = lane >> 4;
     const uint32_t co_tile = wave % NCO, grp = wave / NCO;
     const uint32_t co_base = blockIdx.y * (16 * NCO) + co_tile * 16;
     const uint16_t *wrow = a.wt + uint64_t(co_base + li) * 9 * cin + g * 8;
-    const uint16_t *hbase = halo + ((li >> 3) * 10 + (li & 7u) + 2 * grp * S * 10) * stride + g * 8;
+    const uint16_t *hbase = halo + ((li >> 3) + 2 * grp * S) * RP + (li & 7u) * stride + g * 8;
     f32x4 acc[S];
 #pragma unroll
     for (int j = 0; j < S; j++) acc[j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     for (uint32_t tap = 0; tap < 9; tap++) {
         const uint32_t dy = tap / 3, dx = tap % 3;
-        const uint16_t *hrow = hbase + (dy * 10 + dx) * stride;
+        const uint16_t *hrow = hbase + dy * RP + dx * stride;
 #pragma unroll
         for (uint32_t c = 0; c < cin; c += 32) {
             const bf16x8 A = as_bf16x8(*reinterpret_cast<const uint4 *>(wrow + tap * cin + c));
 #pragma unroll
             for (int j = 0; j < S; j++) {
-                const bf16x8 B = as_bf16x8(*reinterpret_cast<const uint4 *>(hrow + 2 * j * 10 * stride + c));
+                const bf16x8 B = as_bf16x8(*reinterpret_cast<const uint4 *>(hrow + 2 * j * RP + c));
                 acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[j], 0, 0, 0);
             }
         }
@@ -217,12 +226,13 @@ __global__ __launch_bounds__(256) void k_conv_up(ConvArgs a, uint32_t tiles_x, u
     static_assert(TH == 8 || TH == 16, "tiles are 8 or 16 pixels tall");
     constexpr int S = TH * NCO / 8;
     constexpr uint32_t c0 = C0Q * 32, c1 = C1Q * 32, cin = c0 + c1, st0 = c0 + 8, st1 = c1 + 8;
+    constexpr uint32_t RP0 = row_pitch(st0, 10), RP1 = row_pitch(st1, 6);
     constexpr uint32_t R1 = TH / 2 + 2, W1 = 6;                 // the half-resolution halo's rows and columns
     constexpr uint32_t q0 = c0 / 8, q1 = c1 / 8;                // 16-byte chunks a pixel
     constexpr uint32_t n0 = (TH + 2) * 10 * q0, items = n0 + R1 * W1 * q1, per = (items + 255) / 256;
     extern __shared__ uint4 smem[];
     uint16_t *halo0 = reinterpret_cast<uint16_t *>(smem);
-    uint16_t *halo1 = halo0 + (TH + 2) * 10 * st0;
+    uint16_t *halo1 = halo0 + (TH + 2) * RP0;
     const uint32_t tile = blockIdx.x;
     const uint32_t tx = tile % tiles_x, ty = (tile / tiles_x) % tiles_y, n = tile / (tiles_x * tiles_y);
     const int x0 = int(tx * 8) - 1, y0 = int(ty * TH) - 1;       // the full-resolution halo's origin
@@ -253,10 +263,10 @@ __global__ __launch_bounds__(256) void k_conv_up(ConvArgs a, uint32_t tiles_x, u
     for (uint32_t k = 0; k < per; k++) {
         const uint32_t q = threadIdx.x + k * 256;
         if (q < n0) {
-            *reinterpret_cast<uint4 *>(halo0 + (q / q0) * st0 + (q % q0) * 8) = v[k];
+            *reinterpret_cast<uint4 *>(halo0 + (q / q0 / 10) * RP0 + (q / q0 % 10) * st0 + (q % q0) * 8) = v[k];
         } else if ((k + 1) * 256 <= items || q < items) {
             const uint32_t r = q - n0;
-            *reinterpret_cast<uint4 *>(halo1 + (r / q1) * st1 + (r % q1) * 8) = v[k];
+            *reinterpret_cast<uint4 *>(halo1 + (r / q1 / W1) * RP1 + (r / q1 % W1) * st1 + (r % q1) * 8) = v[k];
         }
     }
     __syncthreads();
@@ -264,23 +274,23 @@ __global__ __launch_bounds__(256) void k_conv_up(ConvArgs a, uint32_t tiles_x, u
     const uint32_t co_tile = wave % NCO, grp = wave / NCO;
     const uint32_t co_base = blockIdx.y * (16 * NCO) + co_tile * 16;
     const uint16_t *wrow = a.wt + uint64_t(co_base + li) * 9 * cin + g * 8;
-    const uint16_t *hbase0 = halo0 + ((li >> 3) * 10 + (li & 7u) + 2 * grp * S * 10) * st0 + g * 8;
+    const uint16_t *hbase0 = halo0 + ((li >> 3) + 2 * grp * S) * RP0 + (li & 7u) * st0 + g * 8;
     f32x4 acc[S];
 #pragma unroll
     for (int j = 0; j < S; j++) acc[j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     for (uint32_t tap = 0; tap < 9; tap++) {
         const uint32_t dy = tap / 3, dx = tap % 3;
-        const uint16_t *hrow0 = hbase0 + (dy * 10 + dx) * st0;
+        const uint16_t *hrow0 = hbase0 + dy * RP0 + dx * st0;
         // the lane's full-resolution halo pixel (row 2 (grp S + j) + li / 8 + dy, column li % 8 + dx) lies on
         // half-resolution halo pixel (grp S + j + (li / 8 + dy + 1) / 2, (li % 8 + dx + 1) / 2)
         const uint32_t rr = ((li >> 3) + dy + 1) >> 1, cc = ((li & 7u) + dx + 1) >> 1;
-        const uint16_t *hrow1 = halo1 + ((grp * S + rr) * W1 + cc) * st1 + g * 8;
+        const uint16_t *hrow1 = halo1 + (grp * S + rr) * RP1 + cc * st1 + g * 8;
 #pragma unroll
         for (uint32_t c = 0; c < c0; c += 32) {
             const bf16x8 A = as_bf16x8(*reinterpret_cast<const uint4 *>(wrow + tap * cin + c));
 #pragma unroll
             for (int j = 0; j < S; j++) {
-                const bf16x8 B = as_bf16x8(*reinterpret_cast<const uint4 *>(hrow0 + 2 * j * 10 * st0 + c));
+                const bf16x8 B = as_bf16x8(*reinterpret_cast<const uint4 *>(hrow0 + 2 * j * RP0 + c));
                 acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[j], 0, 0, 0);
             }
         }
@@ -289,7 +299,7 @@ __global__ __launch_bounds__(256) void k_conv_up(ConvArgs a, uint32_t tiles_x, u
             const bf16x8 A = as_bf16x8(*reinterpret_cast<const uint4 *>(wrow + tap * cin + c0 + c));
 #pragma unroll
             for (int j = 0; j < S; j++) {
-                const bf16x8 B = as_bf16x8(*reinterpret_cast<const uint4 *>(hrow1 + j * W1 * st1 + c));
+                const bf16x8 B = as_bf16x8(*reinterpret_cast<const uint4 *>(hrow1 + j * RP1 + c));
                 acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[j], 0, 0, 0);
             }
         }
@@ -551,7 +561,7 @@ int launch_conv(const ConvArgs &a, hipStream_t s) {
     const uint64_t tiles = uint64_t(tx) * ty * a.n;
     if (tiles == 0) return 0;
     if (tiles >= (uint64_t{1} << 31)) return -1;
-    const size_t lds = size_t(th + 2) * 10 * (cin + 8) * 2;
+    const size_t lds = size_t(th + 2) * row_pitch(cin + 8, 10) * 2;
     const int nco = pick_nco(a.cout);
     const dim3 grid(static_cast<uint32_t>(tiles), a.cout / (16 * nco));
     const uint32_t ch = cin / 32;
@@ -561,7 +571,7 @@ int launch_conv(const ConvArgs &a, hipStream_t s) {
     const int up = a.c1 == 0 ? 0 : (a.c0 == 32 && a.c1 == 64 && nco == 2) ? 1 : (a.c0 == 64 && a.c1 == 64 && nco == 4) ? 2
                  : (a.c0 == 64 && a.c1 == 96 && nco == 4) ? 3 : (a.c0 == 96 && a.c1 == 128 && nco == 2) ? 4 : 0;
     if (up) {
-        const size_t lds_up = (size_t(th + 2) * 10 * (a.c0 + 8) + size_t(th / 2 + 2) * 6 * (a.c1 + 8)) * 2;
+        const size_t lds_up = (size_t(th + 2) * row_pitch(a.c0 + 8, 10) + size_t(th / 2 + 2) * row_pitch(a.c1 + 8, 6)) * 2;
 #define TRT_U(T)                                                                                                        \
     switch (up) {                                                                                                       \
     case 1: hipLaunchKernelGGL((k_conv_up<2, T, 1, 2>), grid, dim3(256), lds_up, s, a, tx, ty); break;                \
