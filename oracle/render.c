@@ -183,7 +183,7 @@ static inline uint32_t pu(const uint32_t *p, int vec, int k) { return p[4 * vec 
 enum { CAM_POS = 0, CAM_FWD = 1, CAM_RIGHT = 2, CAM_UP = 3, SIZE = 4, COUNTS = 5, ENV = 6, MISC = 7, SKY = 8,
        COUNTS2 = 9, LIGHT_LOD = 18, LTREE = 21 };
 
-enum { D_CAMERA = 0, D_BSDF = 1, D_RR = 2, D_SUN = 4, D_SUN_VIS = 5, D_EMIT = 7, D_EMIT_VIS = 8, D_ENV = 9,
+enum { D_CAMERA = 0, D_BSDF = 1, D_RR = 2, D_SUN = 4, D_SUN_VIS = 5, D_LOCAL = 6, D_EMIT = 7, D_EMIT_VIS = 8, D_ENV = 9,
        D_ENV_VIS = 10 };
 #define EMIT_RIS_M 6u
 #define EMITTER_GUIDE 65536u
@@ -1329,6 +1329,152 @@ static V env_light(Ctx *c, const Surface *s, const Lobes *lb, V v, V o) {
     return v3(0, 0, 0);
 }
 
+/* ---- local lights (pathtrace.wgsl:2039-2262) --------------------------------------------------------------- */
+#define RIS_EXACT 8u
+#define RIS_M 8u
+typedef struct {
+    V c, l, sl;
+    float dist;
+} LS;
+static inline float smoothstepf(float e0, float e1, float x) {
+    float t = clampf((x - e0) / (e1 - e0), 0.0f, 1.0f);
+    return t * t * (3.0f - 2.0f * t);
+}
+/* local_light (2042-2104): a point, spot or rect light seen from x: its intensity x attenuation (unshadowed), the
+ * direction to its centre (or to the rect's sampled point), and a shadow ray toward a point of it (u: 2 uniforms) */
+static LS local_light(const float *li, V x, const float *u) {
+    LS r;
+    const V pos = ld3(li);
+    const float range = li[11];
+    if ((uint32_t)li[3] == 3u) { /* the rect: a uniform point on it, luminance x area x cos_light / d^2 */
+        const V n = ld3(li + 4), ux = ld3(li + 12);
+        const V uy = vs(normalize(cross(n, ux)), li[7]);
+        const V lp = vadd(vadd(pos, vs(ux, 2.0f * u[0] - 1.0f)), vs(uy, 2.0f * u[1] - 1.0f));
+        const V dl = vsub(lp, x);
+        const float d2 = fmaxf(dot(dl, dl), 1e-8f);
+        const float dist = sqrtf(d2);
+        const V l = vds(dl, dist);
+        const float cos_l = dot(n, vneg(l));
+        float k = 0.0f;
+        if (cos_l > 0.0f) {
+            const float area = 4.0f * len(ux) * li[7];
+            k = area * cos_l / d2 * smoothstepf(li[18] - 0.04f, li[18] + 0.04f, cos_l);
+            if (range > 0.0f) {
+                const float q = d2 / (range * range);
+                const float win = clampf(1.0f - q * q, 0.0f, 1.0f);
+                k *= win * win;
+            }
+        }
+        r.c = vs(ld3(li + 8), k), r.l = l, r.sl = l, r.dist = dist;
+        return r;
+    }
+    V lp = pos;
+    const float radius = li[7];
+    if (radius > 0.0f) {
+        const float z = 1.0f - 2.0f * u[0];
+        const float rr = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+        float sp, cp;
+        or_sincos(TWO_PI * u[1], &sp, &cp);
+        lp = vadd(lp, vs(v3(rr * cp, rr * sp, z), radius));
+    }
+    const V dl = vsub(pos, x);
+    const float d2 = fmaxf(dot(dl, dl), 1e-8f);
+    const V l = vs(dl, 1.0f / sqrtf(d2));
+    float atten = 1.0f / d2;
+    if (range > 0.0f) {
+        const float q = d2 / (range * range);
+        const float win = clampf(1.0f - q * q, 0.0f, 1.0f);
+        atten *= win * win;
+    }
+    if ((uint32_t)li[3] == 2u) { /* the cone: glTF KHR_lights_punctual's angular attenuation */
+        const float cone = clampf((dot(vneg(l), ld3(li + 4)) - li[12]) * li[13], 0.0f, 1.0f);
+        atten *= cone * cone;
+    }
+    const V ls = vsub(lp, x);
+    const float dist = fmaxf(len(ls), 1e-6f);
+    r.c = vs(ld3(li + 8), atten), r.l = l, r.sl = vds(ls, dist), r.dist = dist;
+    return r;
+}
+/* light_vis (908-915): a light that casts no shadow is seen (no ray) */
+static V light_vis(Ctx *c, int shadows, V o, V d, float tmax) {
+    if (!shadows) return v3(1, 1, 1);
+    return shadow_tr(c, o, d, tmax, 0);
+}
+/* local_lights (2158-2262, offline: no light tiles, no volume): at points seen directly, or with few lights, every
+ * unshadowed light and, when few, every shadowed one, exactly (one shadow ray each); otherwise one of RIS_M (3 past
+ * the first bounce) uniform candidates resampled by unshadowed contribution, one shadow ray. (The reference skips
+ * whole groups of 16 lights a point is outside the reach of: the same sum, every light of such a group being
+ * rejected by its own range test below.) */
+static V local_lights(Ctx *c, const Surface *s, const Lobes *lb, V v, V o, uint32_t rd) {
+    const uint32_t nd = pu(c->P, COUNTS2, 0), nall = pu(c->P, COUNTS, 0);
+    const uint32_t nl = nall - nd;
+    V sum = v3(0, 0, 0);
+    if (nl == 0u) return sum;
+    const V x = s->p;
+    const int seen = c->g.depth == 0u || (c->g.depth == 1u && c->mirror_path);
+    const int glass = glass_on(c);
+    uint32_t first = nd, count = nl;
+    if (seen || nl <= RIS_EXACT) {
+        const uint32_t ns = pu(c->P, COUNTS2, 3), n_sh = nall - ns;
+        const uint32_t end = (n_sh <= RIS_EXACT || nl <= RIS_EXACT) ? nall : ns;
+        for (uint32_t i = nd; i < end; i++) {
+            const float *li = light_at(c, i);
+            if (all_nonpos(ld3(li + 8))) continue;
+            const V dc = vsub(ld3(li), x);
+            const float reach = li[11] + ((uint32_t)li[3] == 3u ? len(ld3(li + 12)) + li[7] : li[7]);
+            if (li[11] > 0.0f && dot(dc, dc) > reach * reach) continue;
+            if ((uint32_t)li[3] != 3u && dot(dc, s->ng) < -li[7] && !two_sided(c, lb)) continue;
+            float u[2];
+            or_rng2(&c->g, or_rng_sub(rd, i), u);
+            const LS ls = local_light(li, x, u);
+            if (all_nonpos(ls.c)) continue;
+            const V f = eval_bsdf_k(c, s, lb, v, ls.l, li[17]);
+            if (any_pos(f)) {
+                const V vis = light_vis(c, li[15] >= 0.5f, light_origin(c, s, o, ls.l), ls.sl, ls.dist * 0.999f);
+                if (any_pos(vis)) {
+                    const V e = vmul(ls.c, f);
+                    sum = vadd(sum, glass ? vmul(e, vis) : e);
+                }
+            }
+        }
+        if (end == nall) return sum;
+        first = ns;
+        count = n_sh;
+    }
+    const uint32_t m_cand = seen ? RIS_M : 3u;
+    float wsum = 0.0f, pick_ph = 0.0f;
+    uint32_t pick = 0;
+    LS pick_ls;
+    memset(&pick_ls, 0, sizeof pick_ls);
+    for (uint32_t k = 0; k < m_cand; k++) {
+        float r4[4];
+        or_rng4(&c->g, or_rng_sub(rd ^ 0x80000000u, k), r4);
+        uint32_t j = (uint32_t)(r4[0] * (float)count);
+        const uint32_t idx = first + (j < count - 1u ? j : count - 1u);
+        const LS ls = local_light(light_at(c, idx), x, r4 + 1);
+        float ph = luminance(ls.c);
+        if (two_sided(c, lb))
+            ph *= fabsf(dot(s->n, ls.l));
+        else
+            ph *= fmaxf(dot(s->n, ls.l), 0.0f) + 0.05f * fmaxf(dot(s->ng, ls.l), 0.0f);
+        if (ph <= 0.0f) continue;
+        const float w = ph * 1.0f;
+        wsum += w;
+        if (r4[3] * wsum < w) pick = idx, pick_ph = ph, pick_ls = ls;
+    }
+    if (pick_ph > 0.0f) {
+        const float *lk = light_at(c, pick);
+        const V cc = vmul(pick_ls.c, eval_bsdf_k(c, s, lb, v, pick_ls.l, lk[17]));
+        if (any_pos(cc)) {
+            const V vis = light_vis(c, lk[15] >= 0.5f, light_origin(c, s, o, pick_ls.l), pick_ls.sl,
+                                    pick_ls.dist * 0.999f);
+            if (any_pos(vis))
+                sum = vadd(sum, vs(glass ? vmul(cc, vis) : cc, wsum * (float)count / ((float)m_cand * pick_ph)));
+        }
+    }
+    return sum;
+}
+
 static V direct_light(Ctx *c, const Surface *s, const Lobes *lb, V v) {
     V sum = v3(0, 0, 0);
     V o = offset_origin(s->p, s->ng);
@@ -1358,6 +1504,7 @@ static V direct_light(Ctx *c, const Surface *s, const Lobes *lb, V v) {
             }
         }
     }
+    if (pu(c->P, COUNTS, 0) > pu(c->P, COUNTS2, 0)) sum = vadd(sum, local_lights(c, s, lb, v, o, or_rng_dim(&c->g, D_LOCAL)));
     /* rough glass takes emitters from both sides, as a point without a hemisphere (2488-2491) */
     V ne = two_sided(c, lb) ? v3(0, 0, 0) : s->n;
     c->mis_x = o;

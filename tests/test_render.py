@@ -566,9 +566,9 @@ def test_what_the_library_does_not_render_is_refused(device):
     """Records with cutout sections or with glass among shadow-casting sections (scene.rs INST_CUTOUT 16, INST_GLASS 32
     in the bits word; the Cornell box's records cast shadows); materials with textures, normal maps, unlit or function
     flags, transmission textures, vertex colours, alpha cutoffs, or glass without the glass energy table;
-    and Params asking for an environment map, more than 64 bounces, local lights, fog, volumetric scattering, the live
-    light LOD, procedural geometry, aerial perspective, a background or light-tree candidates without a tree: each
-    refused with TRT_E_INVALID and a message that names it. The device still renders the unchanged scene afterwards."""
+    and Params asking for an environment map, more than 64 bounces, more lights than the scene has, fog, volumetric
+    scattering, the live light LOD, procedural geometry, aerial perspective, a background or light-tree candidates
+    without a tree: each refused with TRT_E_INVALID and a message that names it. The device still renders the unchanged scene afterwards."""
     import copy
 
     import threert_amd as T
@@ -599,7 +599,7 @@ def test_what_the_library_does_not_render_is_refused(device):
     acc = R.Accum(device, 16, 16)
     f32 = lambda k, v: (lambda p: p.view(np.float32).__setitem__(k, v))  # noqa: E731
     for what, change in (("environment map", f32(24, 1.0)), ("more than 64 bounces", lambda p: p.__setitem__(22, 65)),
-                         ("local lights", lambda p: p.__setitem__(20, 1)), ("fog", f32(44, 0.01)),
+                         ("P.counts names more lights", lambda p: p.__setitem__(20, 99)), ("fog", f32(44, 0.01)),
                          ("volumetric scattering", f32(52, 0.5)), ("live light LOD", f32(75, 10.0)),
                          ("procedural geometry", lambda p: p.__setitem__(89, 1)),
                          ("aerial perspective", f32(92, 1.0)), ("scene background", f32(99, 1.0)),

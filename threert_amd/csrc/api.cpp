@@ -97,7 +97,7 @@ struct trt_scene {
     bool shading = false;
     bool coat = false;   // a material has a clearcoat (layer.z > 0: the reference's F_COAT, gpu.rs features_of)
     bool glass = false;  // a material has glass (layer.x > 0: the reference's F_GLASS, gpu.rs:2030-2032)
-    uint64_t light_count = 0, emitter_entries = 0, emitter_tri_count = 0;
+    uint64_t light_count = 0, emitter_entries = 0, emitter_tri_count = 0, n_dir = 0;  // n_dir: directional lights
     uint64_t vertex_count = 0, tri_count = 0, row_count = 0, material_count = 0, object_count = 0, emitter_vec4s = 0;
     uint64_t node_count = 0, leaf_count = 0, instance_count = 0, wnode_count = 0, wunit_count = 0;
     uint32_t top_first = 0, units = 0, top_ref = trt::kNone, max_stack = 0;
@@ -525,6 +525,15 @@ static bool check_shading(const trt_scene_desc *d, std::string *why) {
         *why = "lights without a buffer";
         return false;
     }
+    // lib.rs:367-380 light_order: the directional lights first, then point (1), spot (2) and rect (3) lights
+    for (uint64_t k = 0, local = 0; k < d->light_count; k++) {
+        const float type = static_cast<const float *>(d->lights)[20 * k + 3];
+        if (!(type == 0.0f || type == 1.0f || type == 2.0f || type == 3.0f) || (type == 0.0f && local)) {
+            *why = "light " + std::to_string(k) + ": a type other than 0-3, or a directional light after a local one";
+            return false;
+        }
+        local = local || type != 0.0f;
+    }
     if (d->emitter_vec4s && (d->emitter_vec4s < kGuideVec4s + 1 || !d->emitters)) {
         *why = "an emitter table shorter than its guide and header";
         return false;
@@ -782,6 +791,9 @@ int trt_scene_upload(trt_device *d, const trt_scene_desc *desc, trt_scene **out)
             if (static_cast<const float *>(desc->materials)[72 * k + 28] > 0.0f) s->glass = true;
         }
         s->light_count = desc->light_count;
+        s->n_dir = 0;
+        while (s->n_dir < desc->light_count && static_cast<const float *>(desc->lights)[20 * s->n_dir + 3] == 0.0f)
+            s->n_dir++;
         s->emitter_tri_count = desc->emitter_tri_count;
         s->emitter_entries = desc->emitter_vec4s ? desc->emitter_vec4s - (65537 / 4 + 1) - 1 : 0;
         s->vertex_count = desc->vertex_count, s->tri_count = desc->index_count, s->row_count = desc->row_count;
@@ -1094,7 +1106,12 @@ static int job_args(const trt_render_job &j, trt::RenderArgs &a, std::string *wh
     std::memcpy(a.params, j.params, sizeof(a.params));
     const uint32_t *P = a.params;
     if (P[16] != acc->w || P[17] != acc->h) return *why = "P.size is not the accumulator's", TRT_E_INVALID;
-    if (P[20] > P[36]) return *why = "local lights (P.counts.x > P.counts2.x) are not supported", TRT_E_INVALID;
+    // P.counts2.x the directional lights (the scene's leading ones); past them, local lights with P.counts2.w the
+    // first shadowed one (lib.rs:700)
+    if (P[36] > s->n_dir || (P[20] > P[36] && P[36] != s->n_dir))
+        return *why = "P.counts2.x is not the scene's directional light count", TRT_E_INVALID;
+    if (P[20] > P[36] && (P[39] < P[36] || P[39] > P[20]))
+        return *why = "P.counts2.w (the first shadowed local light) is not among the local lights", TRT_E_INVALID;
     if (P[20] > s->light_count || P[36] > P[20] || P[21] > s->emitter_entries)
         return *why = "P.counts names more lights or emitters than the scene has", TRT_E_INVALID;
     if (P[21] > 0 && P[21] != s->emitter_entries)
@@ -1330,7 +1347,7 @@ static int render_batch(trt_device *d, const trt_render_job *jobs, uint32_t n, u
     table.reserve(n);
     uint64_t tiles = 0;
     uint32_t deepest = 0;
-    bool coat = false, tree = false, glass = false, tex = false;
+    bool coat = false, tree = false, glass = false, tex = false, local = false;
     for (uint32_t k = 0; k < n; k++) {
         if (jobs[k].spp == 0) continue;
         if (!jobs[k].scene || jobs[k].scene->dev != d)
@@ -1348,16 +1365,17 @@ static int render_batch(trt_device *d, const trt_render_job *jobs, uint32_t n, u
         coat = coat || jobs[k].scene->coat;
         glass = glass || jobs[k].scene->glass;
         tex = tex || jobs[k].scene->tex;
+        local = local || j.args.params[20] > j.args.params[36];  // P.counts.x > P.counts2.x: local lights
         tree = tree || trt::from_bits(j.args.params[87]) > 0.5f;  // P.ltree.w: the job samples the light tree
         table.push_back(j);
     }
     if (tiles >= (uint64_t{1} << 31)) return fail(TRT_E_INVALID, "trt_render_batch: too many tiles in one launch");
     if (table.empty()) return ok();
-    // The wavefront takes batches whose jobs have at most kWfMaxSuns directional lights, no glass and no textures; counting
-    // launches stay on the megakernel. TRT_RENDER_AUTO takes it when every scene has kWfAutoLeaves triangles or more,
+    // The wavefront takes batches whose jobs have at most kWfMaxSuns directional lights, no glass, no textures and no
+    // local lights; counting launches stay on the megakernel. TRT_RENDER_AUTO takes it when every scene has kWfAutoLeaves triangles or more,
     // or for launches of a whole chunk of paths or more.
     const uint32_t how = d->render.load();
-    bool wf = reads == nullptr && how != TRT_RENDER_MEGAKERNEL && !glass && !tex;
+    bool wf = reads == nullptr && how != TRT_RENDER_MEGAKERNEL && !glass && !tex && !local;
     uint64_t paths = 0;
     bool large = true;
     std::vector<const trt_scene *> scene_of;
@@ -1423,7 +1441,7 @@ static int render_batch(trt_device *d, const trt_render_job *jobs, uint32_t n, u
         if (rc) return rc;
     } else if (trt::launch_render(static_cast<const trt::RenderJob *>(ring.dev[k]), static_cast<uint32_t>(table.size()),
                            static_cast<uint32_t>(tiles), d->counter, d->spill, threads,
-                           static_cast<uint32_t>(entries), d->guard, d->ray_counts, coat, tree, glass, tex, reads, grid,
+                           static_cast<uint32_t>(entries), d->guard, d->ray_counts, coat, tree, glass || tex || local, reads, grid,
                            d->stream))
         return fail(TRT_E_DEVICE, "render kernel launch failed");
     TRT_HIP(hipEventRecord(ring.ev[k], d->stream));

@@ -8,10 +8,10 @@
 // sample_bsdf 1848-1884; env_radiance / env_pdf / sample_env 1895-1930 (constant sky); power_heuristic, luminance,
 // offset_origin 1943-1960; sky_mis / sun_disks 1963-1988; emit_mis 2009-2024; emitter_radiance /
 // sample_emitter_ris / sample_emitter 2269-2393; env_light 2431-2442; direct_light 2444-2540; emitter_draw /
-// emitter_pick_u / emitter_pick_range / lt_imp / emitter_pmf 463-618 (the light tree); the clearcoat; glass (kGlass:
+// emitter_pick_u / emitter_pick_range / lt_imp / emitter_pmf 463-618 (the light tree); the clearcoat; glass (kExt:
 // the dielectric 1542-1640 with fresnel_dielectric, lambda_ggx, glass_energy, dielectric_t, pdf_dielectric; its
 // parts of lobes, eval_bsdf_k, pdf_bsdf and sample_bsdf; shadow_tr / glass_tr / glass_at / sky_tr 846-977;
-// light_origin 2153-2156; the path loop's `through`, behind_glass and bent 4156-4465); textures (kTex: the pages'
+// light_origin 2153-2156; the path loop's `through`, behind_glass and bent 4156-4465); textures (kExt: the pages'
 // trilinear read tex / tex_level 1107-1117 with srgb_to_linear 1102-1104, mesh_uv / mesh_density / mesh_lod
 // 1119-1140, emission 1312-1317, the colour, roughness, metalness, emissive, normal and transmission maps of
 // surface 1344-1425 with hit_tangent 366-394, glass_at's maps 950-956, emitter_radiance 2269-2283, the path loop's
@@ -170,7 +170,7 @@ __device__ __forceinline__ uint32_t rng_sub(uint32_t d, uint32_t k) {
     return hash_hp(d ^ hash_hp(k + 0x9e3779b9u)) | 0x80000000u;
 }
 
-enum : uint32_t { D_CAMERA = 0, D_BSDF = 1, D_RR = 2, D_SUN = 4, D_EMIT = 7, D_ENV = 9 };
+enum : uint32_t { D_CAMERA = 0, D_BSDF = 1, D_RR = 2, D_SUN = 4, D_LOCAL = 6, D_EMIT = 7, D_ENV = 9 };
 constexpr uint32_t kEmitRisM = 6, kGuide = 65536, kNoEmitter = 0xffffffffu, kInstMirrored = 4, kEmitterCard = 1;
 constexpr int kMatEmitterCard = 2;
 constexpr float kMirrorRoughness = 0.3f;
@@ -376,6 +376,13 @@ __device__ __forceinline__ int32_t as_i32(float f) { return static_cast<int32_t>
 __device__ __forceinline__ float lod_height(const Ctx &c) {
     return static_cast<float>(c.pu(COUNTS2, 2) > 0u ? c.pu(COUNTS2, 2) : c.pu(SIZE, 1));
 }
+// kExt: the kernel has the code of the reference's optional features (its F_GLASS and F_TEX pipelines, local lights);
+// which of them a job's scene has is read from its arguments, as the oracle reads it (no glass table: no glass; no
+// texture info: no textures). Without kExt the code is compiled out.
+template <bool kExt>
+__device__ __forceinline__ bool glass_on(const Ctx &c) { return kExt && c.a->glass_energy != nullptr; }
+template <bool kExt>
+__device__ __forceinline__ bool tex_on(const Ctx &c) { return kExt && c.a->tex_info != nullptr; }
 
 struct Surface {
     V p, n, ng, base, emissive;
@@ -386,7 +393,7 @@ struct Surface {
 };
 
 // kCoat: the launch's scenes hold a clearcoat (the reference's F_COAT pipeline flag); without it the coat terms are
-// compiled out, as the reference's shader is specialised. kGlass likewise for glass (F_GLASS).
+// compiled out, as the reference's shader is specialised. kExt likewise for glass and textures.
 // mesh_uv (1119-1122): the hit's uv x the material's repeat + its offset
 __device__ __forceinline__ void mesh_uv(const TriAttr &a, const float4 *m, float u, float v, float &uu, float &vv) {
     const float w = 1.0f - u - v;
@@ -411,11 +418,11 @@ __device__ __forceinline__ float mesh_lod(float d0, float d1, float fp2, float o
     return 0.5f * log2_c(fmaxf(d0 * fp2 / (d1 * over), 1e-30f));
 }
 // emission (1312-1317): the factor x the emissive map
-template <bool kTex>
+template <bool kExt>
 __device__ V emission(const Ctx &c, const float4 *m, float u, float v, float lod) {
     V em = xyz(m[1]);
     if (all_nonpos(em)) return em;
-    if (kTex && as_i32(m[4].x) >= 0) {
+    if (tex_on<kExt>(c) && as_i32(m[4].x) >= 0) {
         const float4 tx = tex(c, as_i32(m[4].x), u, v, lod);
         em = vmul(em, srgb_to_linear(v3(tx.x, tx.y, tx.z)));
     }
@@ -466,8 +473,8 @@ __device__ float4 hit_tangent(const Ctx &c, const TriAttr &a, V e1, V e2, uint32
     return make_float4(t.x, t.y, t.z, hand);
 }
 
-// `width`: the ray cone's footprint (m) at the hit, for the textures' level (kTex)
-template <bool kCoat, bool kGlass = false, bool kTex = false>
+// `width`: the ray cone's footprint (m) at the hit, for the textures' level
+template <bool kCoat, bool kExt = false>
 __device__ Surface surface(const Ctx &c, V o, V d, const HitRec &h, float width = 0.0f) {
     V e1, e2;
     TriAttr a = tri_attr(c, h.tri, h.inst, h.part, e1, e2);
@@ -495,7 +502,7 @@ __device__ Surface surface(const Ctx &c, V o, V d, const HitRec &h, float width 
     s.p = vadd(o, vs(d, h.t));
     s.front = front;
     float uu = 0.0f, vv = 0.0f, lod = 0.0f;
-    if (kTex) {  // pathtrace.wgsl:1344-1348
+    if (tex_on<kExt>(c)) {  // pathtrace.wgsl:1344-1348
         float d0, d1;
         mesh_uv(a, m, h.u, h.v, uu, vv);
         mesh_density(e1, e2, a, m, d0, d1);
@@ -506,7 +513,7 @@ __device__ Surface surface(const Ctx &c, V o, V d, const HitRec &h, float width 
     const float4 params = m[2];
     V base = xyz(m[0]);
     float rough = params.y, metal = params.x;
-    if (kTex) {  // 1360-1364: the colour (sRGB), roughness (green) and metalness (blue) maps
+    if (tex_on<kExt>(c)) {  // 1360-1364: the colour (sRGB), roughness (green) and metalness (blue) maps
         const float4 t0 = m[3];
         if (as_i32(t0.x) >= 0) {
             const float4 tx = tex(c, as_i32(t0.x), uu, vv, lod);
@@ -515,9 +522,9 @@ __device__ Surface surface(const Ctx &c, V o, V d, const HitRec &h, float width 
         if (as_i32(t0.z) >= 0) rough *= tex(c, as_i32(t0.z), uu, vv, lod).y;
         if (as_i32(t0.w) >= 0) metal *= tex(c, as_i32(t0.w), uu, vv, lod).z;
     }
-    s.emissive = vs(emission<kTex>(c, m, uu, vv, lod), c.obj_power(__float_as_uint(a.x.y)));
+    s.emissive = vs(emission<kExt>(c, m, uu, vv, lod), c.obj_power(__float_as_uint(a.x.y)));
     // 1376-1387: the normal map where the surface is seen directly or in a mirror-like reflection
-    if (kTex && as_i32(m[3].y) >= 0 && (c.g.depth == 0u || c.mirror_path)) {
+    if (tex_on<kExt>(c) && as_i32(m[3].y) >= 0 && (c.g.depth == 0u || c.mirror_path)) {
         const float4 tx = tex(c, as_i32(m[3].y), uu, vv, lod);
         const V tn = v3(tx.x * 2.0f - 1.0f, tx.y * 2.0f - 1.0f, tx.z * 2.0f - 1.0f);
         const float4 tw = hit_tangent(c, a_mesh, e1_mesh, e2_mesh, h.tri, h.inst, h.u, h.v);
@@ -545,10 +552,10 @@ __device__ Surface surface(const Ctx &c, V o, V d, const HitRec &h, float width 
     }
     s.ior = 1.5f;
     s.transmission = 0.0f;
-    if (kGlass) {  // layer.x, layer.y; the transmission map (uv.z: its texture + 1)
+    if (glass_on<kExt>(c)) {  // layer.x, layer.y; the transmission map (uv.z: its texture + 1)
         const float4 layer = m[7];
         s.transmission = clampf(layer.x, 0.0f, 1.0f);
-        if (kTex && m[5].z > 0.5f) s.transmission *= tex(c, static_cast<int32_t>(m[5].z) - 1, uu, vv, lod).x;
+        if (tex_on<kExt>(c) && m[5].z > 0.5f) s.transmission *= tex(c, static_cast<int32_t>(m[5].z) - 1, uu, vv, lod).x;
         s.ior = fmaxf(layer.y, 1.0001f);
     }
     return s;
@@ -631,13 +638,13 @@ __device__ void energy_lookup(const Ctx &c, float roughness, float nov, float *e
     e[1] = mixf(mixf(a1, b1, fx), mixf(c1, d1, fx), fy);
 }
 
-template <bool kGlass = false>
+template <bool kExt = false>
 __device__ Lobes lobes(const Ctx &c, const Surface &s, float nov) {
     Lobes l;
-    const float t = kGlass ? s.transmission * (1.0f - s.metallic) : 0.0f;
+    const float t = kExt ? s.transmission * (1.0f - s.metallic) : 0.0f;
     l.diffuse = vs(vs(s.base, 1.0f - s.metallic), 1.0f - t);
     float f0 = 0.08f * s.specular;
-    if (kGlass && t > 0.0f) {  // glass: F0 from the index of refraction (pathtrace.wgsl:1654-1658)
+    if (kExt && t > 0.0f) {  // glass: F0 from the index of refraction (pathtrace.wgsl:1654-1658)
         const float ri = (s.ior - 1.0f) / (s.ior + 1.0f);
         f0 = ri * ri;
     }
@@ -652,7 +659,7 @@ __device__ Lobes lobes(const Ctx &c, const Surface &s, float nov) {
     l.e_spec = vmul(vadd(vs(l.f0, e[0]), v3(f90 * e[1], f90 * e[1], f90 * e[1])), l.w);
     l.f90 = v3(f90, f90, f90);
     l.kd = vsub(v3(1.0f, 1.0f, 1.0f), l.e_spec);
-    if (kGlass && t > 0.0f) {  // 1669-1673: the share that is not the dielectric keeps the specular lobe
+    if (kExt && t > 0.0f) {  // 1669-1673: the share that is not the dielectric keeps the specular lobe
         l.w = vs(l.w, 1.0f - t);
         l.e_spec = vs(l.e_spec, 1.0f - t);
     }
@@ -681,7 +688,7 @@ __device__ Lobes lobes(const Ctx &c, const Surface &s, float nov) {
     l.rough_t = false;
     l.tint = v3(0.0f, 0.0f, 0.0f);
     l.e_glass = 0.0f;
-    if (kGlass && t > 0.0f) {  // 1688-1697
+    if (kExt && t > 0.0f) {  // 1688-1697
         l.eta = s.front ? s.ior : 1.0f / s.ior;
         l.rough_t = s.roughness > kGlassDelta && fabsf(s.ior - 1.0f) > 1e-3f;
         float gc = 1.0f;
@@ -741,14 +748,14 @@ __device__ float pdf_dielectric(const Surface &s, const Lobes &lb, V v, V l) {
            d_ggx(lb.a2, fmaxf(dot(s.n, h), 0.0f)) * lb.eta * lb.eta * -loh / (nov * fmaxf(den * den, 1e-12f));
 }
 // Whether lights on both sides of the surface reach it: rough glass (1606)
-template <bool kGlass>
-__device__ __forceinline__ bool two_sided(const Lobes &lb) { return kGlass && lb.rough_t; }
+template <bool kExt>
+__device__ __forceinline__ bool two_sided(const Lobes &lb) { return kExt && lb.rough_t; }
 
-template <bool kGlass = false>
+template <bool kExt = false>
 __device__ V eval_bsdf_k(const Surface &s, const Lobes &lb, V v, V l, float k) {
     const float nol = dot(s.n, l);
     const float nov = fmaxf(dot(s.n, v), 1e-4f);
-    if (two_sided<kGlass>(lb) && nol < 0.0f && dot(s.ng, l) < 0.0f) return dielectric_t(s, lb, v, l, nov, -nol);
+    if (two_sided<kExt>(lb) && nol < 0.0f && dot(s.ng, l) < 0.0f) return dielectric_t(s, lb, v, l, nov, -nol);
     if (nol <= 0.0f || dot(s.ng, l) <= 0.0f) return v3(0.0f, 0.0f, 0.0f);
     const V h = normalize(vadd(v, l));
     const float noh = fmaxf(dot(s.n, h), 0.0f);
@@ -757,7 +764,7 @@ __device__ V eval_bsdf_k(const Surface &s, const Lobes &lb, V v, V l, float k) {
     V spec = vs(vmul(vs(f_schlick(lb.f0, lb.f90, voh), dv), lb.w), k);
     // the dielectric's reflection (1747-1748): lb.t holds the coat's share already, so it joins after the coat
     float glass = 0.0f;
-    if (kGlass && lb.t > 0.0f) glass = lb.t * fresnel_dielectric(voh, lb.eta) * dv * k;
+    if (kExt && lb.t > 0.0f) glass = lb.t * fresnel_dielectric(voh, lb.eta) * dv * k;
     V diff = vmul(vs(lb.diffuse, kInvPi), lb.kd);
     if (lb.coat > 0.0f) {  // pathtrace.wgsl:1750-1755
         const float ca2 = lb.coat_alpha * lb.coat_alpha;
@@ -766,18 +773,18 @@ __device__ V eval_bsdf_k(const Surface &s, const Lobes &lb, V v, V l, float k) {
         diff = vs(diff, 1.0f - lb.e_coat);
         spec = vadd(vs(spec, 1.0f - lb.e_coat), v3(coat, coat, coat));
     }
-    if (kGlass) spec = vadd(spec, v3(glass, glass, glass));
+    if (kExt) spec = vadd(spec, v3(glass, glass, glass));
     return vs(vadd(diff, spec), nol);
 }
-template <bool kGlass = false>
+template <bool kExt = false>
 __device__ __forceinline__ V eval_bsdf(const Surface &s, const Lobes &lb, V v, V l) {
-    return eval_bsdf_k<kGlass>(s, lb, v, l, 1.0f);
+    return eval_bsdf_k<kExt>(s, lb, v, l, 1.0f);
 }
-template <bool kGlass = false>
+template <bool kExt = false>
 __device__ float pdf_bsdf(const Surface &s, const Lobes &lb, V v, V l) {
     const float nol = dot(s.n, l);
     float pg = 0.0f;
-    if (kGlass && lb.p_trans > 0.0f) pg = pdf_dielectric(s, lb, v, l);
+    if (kExt && lb.p_trans > 0.0f) pg = pdf_dielectric(s, lb, v, l);
     if (nol <= 0.0f) return pg;
     const float nov = fmaxf(dot(s.n, v), 1e-4f);
     const V h = normalize(vadd(v, l));
@@ -797,9 +804,9 @@ __device__ __forceinline__ float power_heuristic(float a, float b) {
     const float a2 = a * a;
     return a2 / fmaxf(a2 + b * b, 1e-20f);
 }
-template <bool kGlass = false>
+template <bool kExt = false>
 __device__ __forceinline__ float light_mis(float pdf_l, const Surface &s, const Lobes &lb, V v, V l) {
-    return power_heuristic(pdf_l, pdf_bsdf<kGlass>(s, lb, v, l));
+    return power_heuristic(pdf_l, pdf_bsdf<kExt>(s, lb, v, l));
 }
 __device__ __forceinline__ M3 basis(V n) {
     const float sgn = n.z >= 0.0f ? 1.0f : -1.0f;
@@ -838,7 +845,7 @@ __device__ __forceinline__ V refract(V e1, V e2, float eta) {
     return vsub(vs(e1, eta), vs(e2, eta * d + sqrtf(k)));
 }
 
-template <bool kGlass = false>
+template <bool kExt = false>
 __device__ V sample_bsdf(Ctx &c, const Surface &s, const Lobes &lb, V v) {
     const M3 tbn = basis(s.n);
     float r[4];
@@ -848,7 +855,7 @@ __device__ V sample_bsdf(Ctx &c, const Surface &s, const Lobes &lb, V v) {
         c.sampled_spec = true;
         return reflect(vneg(v), sample_vndf(v, tbn, lb.coat_alpha, r[2], r[3]));
     }
-    if (kGlass && r[0] < lb.p_coat + lb.p_trans) {
+    if (kExt && r[0] < lb.p_coat + lb.p_trans) {
         // 1858-1874, the dielectric: a visible microfacet normal (smooth glass refracts through the shading normal),
         // reflected or refracted by the Fresnel reflectance there; a reflection under the surface or a refraction
         // above it is no sample (the zero vector)
@@ -939,7 +946,7 @@ __device__ float glass_share(const Ctx &c, const float4 *m, float cs, float tr_m
 }
 // glass_tr (885-906): past every opaque caster, what the glass a shadow ray crosses lets through (up to 8 surfaces,
 // the ray kept straight: no refraction); toward the sky an odd number of glass surfaces lets nothing through
-template <bool kCount, bool kTex>
+template <bool kCount, bool kExt>
 __device__ V glass_tr(const Ctx &c, V o, V d, float tmax, bool to_sky) {
     float left = tmax;
     V tr = v3(1.0f, 1.0f, 1.0f);
@@ -963,7 +970,7 @@ __device__ V glass_tr(const Ctx &c, V o, V d, float tmax, bool to_sky) {
         const V ng = vs(normalize(cross(e1, e2)), side);
         V tint = xyz(m[0]);
         float tr_map = 1.0f;
-        if (kTex && clampf(m[7].x, 0.0f, 1.0f) * (1.0f - clampf(m[2].x, 0.0f, 1.0f)) > 0.0f &&
+        if (tex_on<kExt>(c) && clampf(m[7].x, 0.0f, 1.0f) * (1.0f - clampf(m[2].x, 0.0f, 1.0f)) > 0.0f &&
             (as_i32(m[3].x) >= 0 || m[5].z > 0.5f)) {  // 950-956
             float uu, vv, d0, d1;
             mesh_uv(a, m, h.u, h.v, uu, vv);
@@ -990,15 +997,14 @@ __device__ V glass_tr(const Ctx &c, V o, V d, float tmax, bool to_sky) {
 }
 // shadow_tr (846-855): 0 past a shadow caster; else what the glass lets through (objects that cast no shadow are
 // glass); sky_tr (975-977) with to_sky
-template <bool kCount, bool kTex>
+template <bool kCount, bool kExt>
 __device__ V shadow_tr(const Ctx &c, V o, V d, float tmax, bool to_sky) {
     if (c.trace<kCount>(o, d, tmax, kTraceShadow).tri != kNone) return v3(0.0f, 0.0f, 0.0f);
-    return glass_tr<kCount, kTex>(c, o, d, tmax, to_sky);
+    return glass_tr<kCount, kExt>(c, o, d, tmax, to_sky);
 }
 // light_origin (2153-2156): a shadow ray's origin on the side l leaves the surface from
-template <bool kGlass>
-__device__ __forceinline__ V light_origin(const Surface &s, V o, V l) {
-    if (kGlass && dot(l, s.ng) < 0.0f) return offset_origin(s.p, vneg(s.ng));
+__device__ __forceinline__ V light_origin(const Surface &s, V o, V l, bool glass) {
+    if (glass && dot(l, s.ng) < 0.0f) return offset_origin(s.p, vneg(s.ng));
     return o;
 }
 
@@ -1171,20 +1177,20 @@ __device__ __forceinline__ Em emitter_tri_at(const Ctx &c, uint32_t i, uint32_t 
             __float_as_uint(r3.x), __float_as_uint(r3.y)};
 }
 // emitter_radiance (2269-2283): the emission at the sample (b1, b2), its map read at a 5 cm footprint
-template <bool kTex>
+template <bool kExt>
 __device__ V emitter_radiance(const Ctx &c, const Em &em, float b1, float b2) {
     V e1, e2;
     TriAttr a = tri_attr(c, em.tri, em.inst, em.part, e1, e2);
     if (em.inst != kNone) a.x.y = __uint_as_float(__float_as_uint(c.inst(em.inst)[6].y) + __float_as_uint(a.x.y));
     const float4 *m = c.material(material_of(c, a, em.inst));
     float uu = 0.0f, vv = 0.0f, lod = 0.0f;
-    if (kTex) {
+    if (tex_on<kExt>(c)) {
         float d0, d1;
         mesh_uv(a, m, b1, b2, uu, vv);
         mesh_density(em.e1, em.e2, a, m, d0, d1);
         lod = mesh_lod(d0, d1, 0.0025f, 1.0f);
     }
-    return vs(emission<kTex>(c, m, uu, vv, lod), c.obj_power(__float_as_uint(a.x.y)));
+    return vs(emission<kExt>(c, m, uu, vv, lod), c.obj_power(__float_as_uint(a.x.y)));
 }
 
 struct ES {
@@ -1192,7 +1198,7 @@ struct ES {
     float dist, pdf_w, w;
 };
 
-template <bool kTree, bool kGlass = false, bool kTex = false>
+template <bool kTree, bool kExt = false>
 __device__ ES sample_emitter_ris(Ctx &c, V x, V n, uint32_t m, const Surface &sf, const Lobes &lb, V v, uint32_t rd) {
     ES r{v3(0.0f, 0.0f, 0.0f), v3(0.0f, 1.0f, 0.0f), 0.0f, 0.0f, 0.0f};
     float wsum = 0.0f, pick_b1 = 0.0f, pick_b2 = 0.0f, pick_ph = 0.0f, pick_pdf = 0.0f;
@@ -1218,7 +1224,7 @@ __device__ ES sample_emitter_ris(Ctx &c, V x, V n, uint32_t m, const Surface &sf
         const float d2 = fmaxf(dot(dl, dl), 1e-8f);
         const V l = vs(dl, 1.0f / sqrtf(d2));
         float g = fabsf(dot(ln, l)) * (dot(n, n) == 0.0f ? 1.0f : fmaxf(dot(n, l), 0.0f)) / d2;
-        if (m > 1u) g = fabsf(dot(ln, l)) * luminance(eval_bsdf<kGlass>(sf, lb, v, l)) / d2;
+        if (m > 1u) g = fabsf(dot(ln, l)) * luminance(eval_bsdf<kExt>(sf, lb, v, l)) / d2;
         const float ph = pk_w / (e.w * 1.0f) * g;
         const float pdf_a = pk_p / (0.5f * a2);
         const float w = ph / pdf_a;
@@ -1237,7 +1243,7 @@ __device__ ES sample_emitter_ris(Ctx &c, V x, V n, uint32_t m, const Surface &sf
     float cos_l = dot(ln, vneg(l));
     if (!(em.flags & kEmitterCard)) cos_l = fabsf(cos_l);
     if (cos_l <= 0.0f) return r;
-    r.le = emitter_radiance<kTex>(c, em, pick_b1, pick_b2);
+    r.le = emitter_radiance<kExt>(c, em, pick_b1, pick_b2);
     r.l = l;
     r.dist = dist;
     r.pdf_w = pick_pdf * d2 / cos_l;
@@ -1245,7 +1251,7 @@ __device__ ES sample_emitter_ris(Ctx &c, V x, V n, uint32_t m, const Surface &sf
     return r;
 }
 
-template <bool kTree, bool kTex = false>
+template <bool kTree, bool kExt = false>
 __device__ ES sample_emitter(Ctx &c, V x, V n, uint32_t rd) {
     ES r{v3(0.0f, 0.0f, 0.0f), v3(0.0f, 1.0f, 0.0f), 0.0f, 0.0f, 0.0f};
     const EPick pk = emitter_draw<kTree>(c, x, n, rd);
@@ -1268,7 +1274,7 @@ __device__ ES sample_emitter(Ctx &c, V x, V n, uint32_t rd) {
     float cos_l = dot(ln, vneg(l));
     if (!(em.flags & kEmitterCard)) cos_l = fabsf(cos_l);
     if (cos_l <= 0.0f) return r;
-    r.le = emitter_radiance<kTex>(c, em, b1, b2);
+    r.le = emitter_radiance<kExt>(c, em, b1, b2);
     r.l = l;
     r.dist = dist;
     r.pdf_w = pk_p / area * d2 / cos_l;
@@ -1298,13 +1304,166 @@ struct WfShadows {
     }
 };
 
+// ---- local lights (pathtrace.wgsl:2039-2262) ---------------------------------------------------------------------
+constexpr uint32_t kRisExact = 8, kRisM = 8;
+struct LS {
+    V c, l, sl;
+    float dist;
+};
+__device__ __forceinline__ float smoothstepf(float e0, float e1, float x) {
+    const float t = clampf((x - e0) / (e1 - e0), 0.0f, 1.0f);
+    return t * t * (3.0f - 2.0f * t);
+}
+// local_light (2042-2104): a point, spot or rect light seen from x: its intensity x attenuation (unshadowed), the
+// direction to its centre (or to the rect's sampled point), and a shadow ray toward a point of it
+__device__ LS local_light(const float4 *li, V x, float u0, float u1) {
+    LS r;
+    const float4 p4 = li[0], d4 = li[1], c4 = li[2], e4 = li[3];
+    const V pos = xyz(p4);
+    const float range = c4.w;
+    if (static_cast<uint32_t>(p4.w) == 3u) {  // the rect: a uniform point on it, luminance x area x cos_light / d^2
+        const V n = xyz(d4), ux = xyz(e4);
+        const V uy = vs(normalize(cross(n, ux)), d4.w);
+        const V lp = vadd(vadd(pos, vs(ux, 2.0f * u0 - 1.0f)), vs(uy, 2.0f * u1 - 1.0f));
+        const V dl = vsub(lp, x);
+        const float d2 = fmaxf(dot(dl, dl), 1e-8f);
+        const float dist = sqrtf(d2);
+        const V l = vds(dl, dist);
+        const float cos_l = dot(n, vneg(l));
+        float k = 0.0f;
+        if (cos_l > 0.0f) {
+            const float area = 4.0f * len(ux) * d4.w;
+            const float bd = li[4].z;
+            k = area * cos_l / d2 * smoothstepf(bd - 0.04f, bd + 0.04f, cos_l);
+            if (range > 0.0f) {
+                const float q = d2 / (range * range);
+                const float win = clampf(1.0f - q * q, 0.0f, 1.0f);
+                k *= win * win;
+            }
+        }
+        r.c = vs(xyz(c4), k), r.l = l, r.sl = l, r.dist = dist;
+        return r;
+    }
+    V lp = pos;
+    const float radius = d4.w;
+    if (radius > 0.0f) {
+        const float z = 1.0f - 2.0f * u0;
+        const float rr = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+        float sp, cp;
+        sincos_c(kTwoPi * u1, sp, cp);
+        lp = vadd(lp, vs(v3(rr * cp, rr * sp, z), radius));
+    }
+    const V dl = vsub(pos, x);
+    const float d2 = fmaxf(dot(dl, dl), 1e-8f);
+    const V l = vs(dl, 1.0f / sqrtf(d2));
+    float atten = 1.0f / d2;
+    if (range > 0.0f) {
+        const float q = d2 / (range * range);
+        const float win = clampf(1.0f - q * q, 0.0f, 1.0f);
+        atten *= win * win;
+    }
+    if (static_cast<uint32_t>(p4.w) == 2u) {  // the cone: glTF KHR_lights_punctual's angular attenuation
+        const float cone = clampf((dot(vneg(l), xyz(d4)) - e4.x) * e4.y, 0.0f, 1.0f);
+        atten *= cone * cone;
+    }
+    const V ls = vsub(lp, x);
+    const float dist = fmaxf(len(ls), 1e-6f);
+    r.c = vs(xyz(c4), atten), r.l = l, r.sl = vds(ls, dist), r.dist = dist;
+    return r;
+}
+// local_lights (2158-2262, offline: no light tiles, no volume): where the point is seen directly, or with few lights,
+// every unshadowed light and, when few, every shadowed one exactly (one shadow ray each); otherwise one of kRisM (3
+// past the first bounce) uniform candidates resampled by unshadowed contribution, one shadow ray. (The reference skips
+// whole groups of 16 lights a point is outside the reach of: the same sum, every light of such a group being
+// rejected by its own range test here.)
+template <bool kCount, bool kExt>
+__device__ V local_lights(Ctx &c, const Surface &s, const Lobes &lb, V v, V o, uint32_t rd) {
+    const uint32_t nd = c.pu(COUNTS2, 0), nall = c.pu(COUNTS, 0);
+    const uint32_t nl = nall - nd;
+    V sum = v3(0.0f, 0.0f, 0.0f);
+    const V x = s.p;
+    const bool seen = c.g.depth == 0u || (c.g.depth == 1u && c.mirror_path);
+    const bool glass = glass_on<kExt>(c);
+    uint32_t first = nd, count = nl;
+    if (seen || nl <= kRisExact) {
+        const uint32_t ns = c.pu(COUNTS2, 3), n_sh = nall - ns;
+        const uint32_t end = (n_sh <= kRisExact || nl <= kRisExact) ? nall : ns;
+        for (uint32_t i = nd; i < end; i++) {
+            const float4 *li = c.light(i);
+            const float4 p4 = li[0], d4 = li[1], c4 = li[2], e4 = li[3];
+            if (all_nonpos(xyz(c4))) continue;
+            const V dc = vsub(xyz(p4), x);
+            const float reach = c4.w + (static_cast<uint32_t>(p4.w) == 3u ? len(xyz(e4)) + d4.w : d4.w);
+            if (c4.w > 0.0f && dot(dc, dc) > reach * reach) continue;
+            if (static_cast<uint32_t>(p4.w) != 3u && dot(dc, s.ng) < -d4.w && !two_sided<kExt>(lb)) continue;
+            float u[2];
+            c.g.r2(rng_sub(rd, i), u);
+            const LS ls = local_light(li, x, u[0], u[1]);
+            if (all_nonpos(ls.c)) continue;
+            const V f = eval_bsdf_k<kExt>(s, lb, v, ls.l, li[4].y);
+            if (any_pos(f)) {
+                const V lo = light_origin(s, o, ls.l, glass);
+                const V vis = e4.w >= 0.5f ? (glass ? shadow_tr<kCount, kExt>(c, lo, ls.sl, ls.dist * 0.999f, false)
+                                                    : (occluded<kCount>(c, lo, ls.sl, ls.dist * 0.999f, kTraceShadow)
+                                                           ? v3(0.0f, 0.0f, 0.0f)
+                                                           : v3(1.0f, 1.0f, 1.0f)))
+                                           : v3(1.0f, 1.0f, 1.0f);
+                if (any_pos(vis)) {
+                    const V e = vmul(ls.c, f);
+                    sum = vadd(sum, glass ? vmul(e, vis) : e);
+                }
+            }
+        }
+        if (end == nall) return sum;
+        first = ns;
+        count = n_sh;
+    }
+    const uint32_t m_cand = seen ? kRisM : 3u;
+    float wsum = 0.0f, pick_ph = 0.0f;
+    uint32_t pick = 0;
+    LS pick_ls{v3(0.0f, 0.0f, 0.0f), v3(0.0f, 0.0f, 0.0f), v3(0.0f, 0.0f, 0.0f), 0.0f};
+    for (uint32_t k = 0; k < m_cand; k++) {
+        float r4[4];
+        c.g.r4(rng_sub(rd ^ 0x80000000u, k), r4);
+        const uint32_t idx = first + min(static_cast<uint32_t>(r4[0] * static_cast<float>(count)), count - 1u);
+        const LS ls = local_light(c.light(idx), x, r4[1], r4[2]);
+        float ph = luminance(ls.c);
+        if (two_sided<kExt>(lb))
+            ph *= fabsf(dot(s.n, ls.l));
+        else
+            ph *= fmaxf(dot(s.n, ls.l), 0.0f) + 0.05f * fmaxf(dot(s.ng, ls.l), 0.0f);
+        if (ph <= 0.0f) continue;
+        const float w = ph * 1.0f;
+        wsum += w;
+        if (r4[3] * wsum < w) pick = idx, pick_ph = ph, pick_ls = ls;
+    }
+    if (pick_ph > 0.0f) {
+        const float4 *lk = c.light(pick);
+        const V cc = vmul(pick_ls.c, eval_bsdf_k<kExt>(s, lb, v, pick_ls.l, lk[4].y));
+        if (any_pos(cc)) {
+            const V lo = light_origin(s, o, pick_ls.l, glass);
+            const float tmax = pick_ls.dist * 0.999f;
+            const V vis = lk[3].w >= 0.5f ? (glass ? shadow_tr<kCount, kExt>(c, lo, pick_ls.sl, tmax, false)
+                                                   : (occluded<kCount>(c, lo, pick_ls.sl, tmax, kTraceShadow)
+                                                          ? v3(0.0f, 0.0f, 0.0f)
+                                                          : v3(1.0f, 1.0f, 1.0f)))
+                                          : v3(1.0f, 1.0f, 1.0f);
+            if (any_pos(vis))
+                sum = vadd(sum, vs(glass ? vmul(cc, vis) : cc,
+                                   wsum * static_cast<float>(count) / (static_cast<float>(m_cand) * pick_ph)));
+        }
+    }
+    return sum;
+}
+
 // pathtrace.wgsl direct_light: the sum of what the suns, one emitter sample and one environment sample send toward v.
 // kDefer (the wavefront): each contribution goes to `q` with its shadow ray, untraced, and the sum returned is 0.
-// kGlass (the megakernel alone): shadow rays leave from the far side toward lights behind rough glass, the suns' and
-// the sky's pass glass at its share (shadow_tr), and rough glass samples emitters from both sides.
-template <bool kTree, bool kCount, bool kDefer = false, bool kGlass = false, bool kTex = false>
+// kExt (the megakernel alone): with glass, shadow rays leave from the far side toward lights behind rough glass, the
+// suns' and the sky's pass glass at its share (shadow_tr), and rough glass samples emitters from both sides.
+template <bool kTree, bool kCount, bool kDefer = false, bool kExt = false>
 __device__ V direct_light(Ctx &c, const Surface &s, const Lobes &lb, V v, WfShadows *q = nullptr) {
-    static_assert(!(kDefer && kGlass), "the wavefront does not render glass");
+    static_assert(!(kDefer && kExt), "the wavefront does not render glass, textures or local lights");
+    const bool glass = glass_on<kExt>(c);
     V sum = v3(0.0f, 0.0f, 0.0f);
     const V o = offset_origin(s.p, s.ng);
     for (uint32_t i = 0; i < c.pu(COUNTS2, 0); i++) {
@@ -1323,16 +1482,16 @@ __device__ V direct_light(Ctx &c, const Surface &s, const Lobes &lb, V v, WfShad
         const V l = mmul(tb, v3(sin_t * cp, sin_t * sp, cos_t));
         const V col = xyz(li[2]);
         if (all_nonpos(col)) continue;
-        const V f = eval_bsdf_k<kGlass>(s, lb, v, l, li[4].y);
+        const V f = eval_bsdf_k<kExt>(s, lb, v, l, li[4].y);
         if (kDefer && any_pos(f)) {
             const float pdf_l = 1.0f / (kTwoPi * (1.0f - cos_max));
             q->put(vs(vmul(col, f), light_mis(pdf_l, s, lb, v, l)), li[3].w >= 0.5f, o, l, kInf, kTraceShadow);
-        } else if (kGlass && any_pos(f)) {
-            const V vis = li[3].w >= 0.5f ? shadow_tr<kCount, kTex>(c, light_origin<kGlass>(s, o, l), l, kInf, false)
+        } else if (glass && any_pos(f)) {
+            const V vis = li[3].w >= 0.5f ? shadow_tr<kCount, kExt>(c, light_origin(s, o, l, glass), l, kInf, false)
                                           : v3(1.0f, 1.0f, 1.0f);
             if (any_pos(vis)) {
                 const float pdf_l = 1.0f / (kTwoPi * (1.0f - cos_max));
-                const float wgt = light_mis<kGlass>(pdf_l, s, lb, v, l);
+                const float wgt = light_mis<kExt>(pdf_l, s, lb, v, l);
                 sum = vadd(sum, vmul(vs(vmul(col, f), wgt), vis));
             }
         } else if (any_pos(f)) {
@@ -1344,8 +1503,10 @@ __device__ V direct_light(Ctx &c, const Surface &s, const Lobes &lb, V v, WfShad
             }
         }
     }
+    if (kExt && c.pu(COUNTS, 0) > c.pu(COUNTS2, 0))
+        sum = vadd(sum, local_lights<kCount, kExt>(c, s, lb, v, o, c.g.dim(D_LOCAL)));
     // rough glass takes emitters from both sides, as a point without a hemisphere (2488-2491)
-    const V ne = two_sided<kGlass>(lb) ? v3(0.0f, 0.0f, 0.0f) : s.n;
+    const V ne = two_sided<kExt>(lb) ? v3(0.0f, 0.0f, 0.0f) : s.n;
     c.mis_x = o;
     c.mis_n = ne;
     if (emitters_on(c)) {
@@ -1353,24 +1514,24 @@ __device__ V direct_light(Ctx &c, const Surface &s, const Lobes &lb, V v, WfShad
         const bool seen = c.g.depth == 0u || (c.g.depth == 1u && c.mirror_path);
         // with the tree its candidates are already near the point: P.ltree.w of them at camera hits, one elsewhere
         if (kTree && lt_on(c))
-            es = sample_emitter_ris<kTree, kGlass, kTex>(
+            es = sample_emitter_ris<kTree, kExt>(
                 c, o, ne, c.g.depth == 0u ? static_cast<uint32_t>(c.pf(LTREE, 3)) : 1u, s, lb, v, c.g.dim(D_EMIT));
         else if (seen)
-            es = sample_emitter_ris<kTree, kGlass, kTex>(c, o, ne, c.g.depth == 0u ? kEmitRisM : kEmitRisM / 2u, s, lb,
+            es = sample_emitter_ris<kTree, kExt>(c, o, ne, c.g.depth == 0u ? kEmitRisM : kEmitRisM / 2u, s, lb,
                                                          v, c.g.dim(D_EMIT));
         else if (c.pf(LIGHT_LOD, 3) > 0.0f)
-            es = sample_emitter_ris<kTree, kGlass, kTex>(c, o, ne, 2u, s, lb, v, c.g.dim(D_EMIT));
+            es = sample_emitter_ris<kTree, kExt>(c, o, ne, 2u, s, lb, v, c.g.dim(D_EMIT));
         else
-            es = sample_emitter<kTree, kTex>(c, o, ne, c.g.dim(D_EMIT));
+            es = sample_emitter<kTree, kExt>(c, o, ne, c.g.dim(D_EMIT));
         if (es.pdf_w > 0.0f) {
-            const V f = eval_bsdf<kGlass>(s, lb, v, es.l);
+            const V f = eval_bsdf<kExt>(s, lb, v, es.l);
             if (kDefer && any_pos(f)) {
                 q->put(vs(vs(vmul(es.le, f), light_mis(es.pdf_w, s, lb, v, es.l)), es.w), true, o, es.l,
                        es.dist * 0.999f, kTraceAny);
             } else if (any_pos(f)) {
                 // emitter_tr (1066-1070): any surface stops it, glass too
-                if (!occluded<kCount>(c, light_origin<kGlass>(s, o, es.l), es.l, es.dist * 0.999f, kTraceAny)) {
-                    const float wgt = light_mis<kGlass>(es.pdf_w, s, lb, v, es.l);
+                if (!occluded<kCount>(c, light_origin(s, o, es.l, glass), es.l, es.dist * 0.999f, kTraceAny)) {
+                    const float wgt = light_mis<kExt>(es.pdf_w, s, lb, v, es.l);
                     sum = vadd(sum, vs(vs(vmul(es.le, f), wgt), es.w));
                 }
             }
@@ -1378,17 +1539,17 @@ __device__ V direct_light(Ctx &c, const Surface &s, const Lobes &lb, V v, WfShad
     }
     if (env_sampled(c)) {
         const V l = sample_env(c, c.g.dim(D_ENV));
-        const V f = eval_bsdf<kGlass>(s, lb, v, l);
+        const V f = eval_bsdf<kExt>(s, lb, v, l);
         if (kDefer && any_pos(f)) {
             const float pdf_e = env_pdf();
             if (pdf_e > 0.0f)
                 q->put(vds(vs(vmul(sky(c), f), light_mis(pdf_e, s, lb, v, l)), pdf_e), true, o, l, kInf, kTraceShadow);
-        } else if (kGlass && any_pos(f)) {
+        } else if (glass && any_pos(f)) {
             const float pdf_e = env_pdf();
             if (pdf_e > 0.0f) {
-                const V vis = shadow_tr<kCount, kTex>(c, light_origin<kGlass>(s, o, l), l, kInf, true);
+                const V vis = shadow_tr<kCount, kExt>(c, light_origin(s, o, l, glass), l, kInf, true);
                 if (any_pos(vis))
-                    sum = vadd(sum, vmul(vds(vs(vmul(sky(c), f), light_mis<kGlass>(pdf_e, s, lb, v, l)), pdf_e), vis));
+                    sum = vadd(sum, vmul(vds(vs(vmul(sky(c), f), light_mis<kExt>(pdf_e, s, lb, v, l)), pdf_e), vis));
             }
         } else if (any_pos(f)) {
             const float pdf_e = env_pdf();
@@ -1452,7 +1613,7 @@ __device__ __forceinline__ void camera_ray(Ctx &c, uint32_t px, uint32_t py, uin
 }
 
 // One path of sample `index` at pixel (px, py): its radiance, the camera hit's albedo and normal.
-template <bool kCoat, bool kTree, bool kCount, bool kGlass, bool kTex>
+template <bool kCoat, bool kTree, bool kCount, bool kExt>
 __device__ void path(Ctx &c, uint32_t px, uint32_t py, uint32_t index, V &radiance, V &albedo, V &normal) {
     V o, d;
     camera_ray(c, px, py, index, o, d);
@@ -1465,7 +1626,7 @@ __device__ void path(Ctx &c, uint32_t px, uint32_t py, uint32_t index, V &radian
     bool behind_glass = false, bent = false;
     float last_pdf = 0.0f;
     // the ray cone (Akenine-Moller et al. 2019) for the textures' level: width at the origin, spread angle (4165-4167)
-    float cone_w = 0.0f, cone_a = kTex ? 2.0f * c.pf(CAM_POS, 3) / lod_height(c) : 0.0f;
+    float cone_w = 0.0f, cone_a = kExt ? 2.0f * c.pf(CAM_POS, 3) / lod_height(c) : 0.0f;
     const float clamp_v = c.pf(MISC, 0);
     c.mirror_path = false;
     const uint32_t last = c.pu(COUNTS, 2);
@@ -1473,13 +1634,13 @@ __device__ void path(Ctx &c, uint32_t px, uint32_t py, uint32_t index, V &radian
         c.g.depth = depth;
         const HitRec hit = c.trace<kCount>(o, d, kInf, kTraceClosest);
         if (hit.tri == kNone) {
-            V col = vmul(thr, sky_mis(c, d, (depth > 0u && !(kGlass && bent)) ? last_pdf : 0.0f));
+            V col = vmul(thr, sky_mis(c, d, (depth > 0u && !(kExt && bent)) ? last_pdf : 0.0f));
             if (depth > 1u && !straight) col = clamp_lum(col, clamp_v);
             radiance = vadd(radiance, col);
             break;
         }
-        const float width = kTex ? cone_w + cone_a * hit.t : 0.0f;
-        const Surface s = surface<kCoat, kGlass, kTex>(c, o, d, hit, width);
+        const float width = kExt ? cone_w + cone_a * hit.t : 0.0f;
+        const Surface s = surface<kCoat, kExt>(c, o, d, hit, width);
         if (kCount) c.cnt->shaded++;
         if (depth == 0u) {
             if (s.card) {
@@ -1492,7 +1653,7 @@ __device__ void path(Ctx &c, uint32_t px, uint32_t py, uint32_t index, V &radian
         }
         if (any_pos(s.emissive) && (s.front || !s.card)) {
             float wgt = 1.0f;
-            if (depth > 0u && !specular_bounce && !(kGlass && behind_glass)) wgt = emit_mis<kTree>(c, hit, d, last_pdf);
+            if (depth > 0u && !specular_bounce && !(kExt && behind_glass)) wgt = emit_mis<kTree>(c, hit, d, last_pdf);
             V col = vs(vmul(thr, s.emissive), wgt);
             if (depth > 1u && !straight) col = clamp_lum(col, clamp_v);
             radiance = vadd(radiance, col);
@@ -1501,19 +1662,19 @@ __device__ void path(Ctx &c, uint32_t px, uint32_t py, uint32_t index, V &radian
         if (depth == last) break;
         const V v = vneg(d);
         const float nov = fmaxf(dot(s.n, v), 1e-4f);
-        const Lobes lb = lobes<kGlass>(c, s, nov);
-        V dl = vmul(thr, direct_light<kTree, kCount, false, kGlass, kTex>(c, s, lb, v));
+        const Lobes lb = lobes<kExt>(c, s, nov);
+        V dl = vmul(thr, direct_light<kTree, kCount, false, kExt>(c, s, lb, v));
         if (depth > 0u && !straight) dl = clamp_lum(dl, clamp_v);
         radiance = vadd(radiance, dl);
-        const V l = sample_bsdf<kGlass>(c, s, lb, v);
-        if (kGlass) {
+        const V l = sample_bsdf<kExt>(c, s, lb, v);
+        if (kExt) {
             // 4414-4465: through smooth glass a delta lobe, weight tint / its sampling share, no MIS
             const bool through = c.sampled_trans && !lb.rough_t;
             float pdf = 0.0f;
             V wgt = vds(lb.tint, fmaxf(lb.p_trans, 1e-6f));
             if (!through) {
-                pdf = pdf_bsdf<kGlass>(s, lb, v, l);
-                wgt = vds(eval_bsdf<kGlass>(s, lb, v, l), fmaxf(pdf, 1e-20f));
+                pdf = pdf_bsdf<kExt>(s, lb, v, l);
+                wgt = vds(eval_bsdf<kExt>(s, lb, v, l), fmaxf(pdf, 1e-20f));
             }
             if ((!through && pdf <= 0.0f) || all_nonpos(wgt)) break;
             thr = vmul(thr, wgt);
@@ -1527,7 +1688,7 @@ __device__ void path(Ctx &c, uint32_t px, uint32_t py, uint32_t index, V &radian
                 bent = false;
             }
             straight = straight && through;
-            if (kTex) {
+            if (kExt) {
                 cone_w = width;
                 cone_a += 2.0f * s.roughness * s.roughness;
             }
@@ -1539,13 +1700,13 @@ __device__ void path(Ctx &c, uint32_t px, uint32_t py, uint32_t index, V &radian
             last_pdf = pdf;
             specular_bounce = false;
             straight = false;
-            if (kTex) {
+            if (kExt) {
                 cone_w = width;
                 cone_a += 2.0f * s.roughness * s.roughness;
             }
         }
         c.mirror_path = (depth == 0u || c.mirror_path) && s.roughness < kMirrorRoughness &&
-                        (c.sampled_spec || (kGlass && c.sampled_trans));
+                        (c.sampled_spec || (kExt && c.sampled_trans));
         if (depth >= c.pu(COUNTS, 3)) {
             const float q = fminf(fmaxf(thr.x, fmaxf(thr.y, thr.z)), 0.95f);
             if (c.g.r1(c.g.dim(D_RR)) > q) break;
@@ -1567,7 +1728,7 @@ __device__ void path(Ctx &c, uint32_t px, uint32_t py, uint32_t index, V &radian
 // frame 2.46 -> 1.95 ms, the 3M-triangle scene 1.36x faster, the Cornell box 5 % (four waves: 2.00 ms, 1.34x, 11 %).
 // kCount: a counting launch, which adds what its walks read and the hits it shades to reads[0..3] (nodes,
 // triangles, units, shaded hits).
-template <bool kCoat, bool kTree, bool kCount, bool kGlass, bool kTex>
+template <bool kCoat, bool kTree, bool kCount, bool kExt>
 __global__ __launch_bounds__(kRenderBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_render(
     const RenderJob *jobs, uint32_t njobs, uint32_t total, uint32_t *counter, uint32_t *spill, uint32_t spill_stride,
     uint32_t spill_entries, uint32_t *guard, unsigned long long *rays, unsigned long long *reads) {
@@ -1604,7 +1765,7 @@ __global__ __launch_bounds__(kRenderBlock) __attribute__((amdgpu_waves_per_eu(3)
             float4 col = pix[0], alb = pix[1], nrm = pix[2];
             for (uint32_t s = 0; s < a->spp; s++) {
                 V rad, al, nm;
-                path<kCoat, kTree, kCount, kGlass, kTex>(c, px, py, a->first_sample + s, rad, al, nm);
+                path<kCoat, kTree, kCount, kExt>(c, px, py, a->first_sample + s, rad, al, nm);
                 const float lum = luminance(rad);
                 col.x += rad.x, col.y += rad.y, col.z += rad.z, col.w += 1.0f;
                 alb.x += al.x, alb.y += al.y, alb.z += al.z, alb.w += lum;
@@ -1963,7 +2124,7 @@ __global__ void k_reset(uint32_t *c) { c[threadIdx.x * kShardGap] = 0; }
 int render_occupancy(int *blocks_per_cu) {
     int b = 0;
     const hipError_t e =
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_render<false, false, false, false, false>, kRenderBlock, 0);
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_render<false, false, false, false>, kRenderBlock, 0);
     *blocks_per_cu = b;
     return e == hipSuccess ? 0 : -1;
 }
@@ -1999,39 +2160,31 @@ int launch_wf_accum(const WfArgs &a, uint32_t pixels, int grid, hipStream_t s) {
 
 int launch_render(const RenderJob *d_jobs, uint32_t njobs, uint32_t total_tiles, uint32_t *counter,
                   uint32_t *spill, uint32_t spill_stride, uint32_t spill_entries, uint32_t *guard,
-                  unsigned long long *rays, bool coat, bool tree, bool glass, bool tex, unsigned long long *reads,
-                  int grid, hipStream_t s) {
+                  unsigned long long *rays, bool coat, bool tree, bool ext, unsigned long long *reads, int grid,
+                  hipStream_t s) {
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(kShards), 0, s, counter);
-#define TRT_RENDER(C, T, N, G, X)                                                                                    \
-    hipLaunchKernelGGL((k_render<C, T, N, G, X>), dim3(grid), dim3(kRenderBlock), 0, s, d_jobs, njobs, total_tiles,  \
+#define TRT_RENDER(C, T, N, E)                                                                                       \
+    hipLaunchKernelGGL((k_render<C, T, N, E>), dim3(grid), dim3(kRenderBlock), 0, s, d_jobs, njobs, total_tiles,     \
                        counter, spill, spill_stride, spill_entries, guard, rays, reads)
-#define TRT_RENDER_N(C, T, G, X)          \
-    if (reads)                            \
-        TRT_RENDER(C, T, true, G, X);     \
-    else                                  \
-        TRT_RENDER(C, T, false, G, X)
-    // Glass and textured scenes take the kernel with the coat's terms as well (the same bits for materials without
-    // a coat), textured ones with or without the glass code.
-    if (tex && glass && tree) {
-        TRT_RENDER_N(true, true, true, true);
-    } else if (tex && glass) {
-        TRT_RENDER_N(true, false, true, true);
-    } else if (tex && tree) {
-        TRT_RENDER_N(true, true, false, true);
-    } else if (tex) {
-        TRT_RENDER_N(true, false, false, true);
-    } else if (glass && tree) {
-        TRT_RENDER_N(true, true, true, false);
-    } else if (glass) {
-        TRT_RENDER_N(true, false, true, false);
+#define TRT_RENDER_N(C, T, E)          \
+    if (reads)                         \
+        TRT_RENDER(C, T, true, E);     \
+    else                               \
+        TRT_RENDER(C, T, false, E)
+    // Scenes with glass, textures or local lights take the kernel with their code and the coat's terms (the same
+    // bits for materials without a coat); which of them a job's scene has, the kernel reads from its arguments.
+    if (ext && tree) {
+        TRT_RENDER_N(true, true, true);
+    } else if (ext) {
+        TRT_RENDER_N(true, false, true);
     } else if (coat && tree) {
-        TRT_RENDER_N(true, true, false, false);
+        TRT_RENDER_N(true, true, false);
     } else if (coat) {
-        TRT_RENDER_N(true, false, false, false);
+        TRT_RENDER_N(true, false, false);
     } else if (tree) {
-        TRT_RENDER_N(false, true, false, false);
+        TRT_RENDER_N(false, true, false);
     } else {
-        TRT_RENDER_N(false, false, false, false);
+        TRT_RENDER_N(false, false, false);
     }
 #undef TRT_RENDER_N
 #undef TRT_RENDER

@@ -101,13 +101,13 @@ int launch_resolve(const float4 *acc, uint32_t w, uint32_t h, uint32_t pw, uint3
 // d_jobs: njobs jobs in device memory, tile_first ascending from 0; total_tiles: all their tiles; counter: the kShards
 // tile counters. rays: the sharded ray counts the launch adds its traced rays to (closest; shadow and any-hit); coat: some job's scene has a
 // material with a clearcoat (the kernel with the coat's terms; the same bits for scenes without one); tree: some
-// job's Params ask for the light tree (the kernel with its code); glass: some job's scene has glass (the kernel with
-// the dielectric and glass along shadow rays; every such job's glass_energy set); tex: some job's scene has textures
-// (the kernel with the maps and the ray cone; every such job's tex_* set); reads: NULL, or four counters a counting
-// launch adds to (wide nodes, triangles, units its walks read; hits it shades)
+// job's Params ask for the light tree (the kernel with its code); ext: some job's scene has glass, textures or local
+// lights (the kernel with their code; a job's glass_energy and tex_info, null without them, and its Params say which it
+// has); reads: NULL, or four counters a counting launch adds to (wide nodes, triangles, units its walks read; hits it
+// shades)
 int launch_render(const RenderJob *d_jobs, uint32_t njobs, uint32_t total_tiles, uint32_t *counter,
                   uint32_t *spill, uint32_t spill_stride, uint32_t spill_entries, uint32_t *guard,
-                  unsigned long long *rays, bool coat, bool tree, bool glass, bool tex, unsigned long long *reads,
-                  int grid, hipStream_t s);
+                  unsigned long long *rays, bool coat, bool tree, bool ext, unsigned long long *reads, int grid,
+                  hipStream_t s);
 
 }  // namespace trt

@@ -71,6 +71,70 @@ def directional_light(direction, lux, diameter_deg=0.5357, cast_shadow=True) -> 
     return li
 
 
+def spot_cone(angle: float, penumbra: float) -> tuple[float, float]:
+    """scene.rs:1309-1313 spot_cone: the outer cone's cosine and 1 / (inner cosine - outer cosine), the difference no
+    less than 0.001; the inner cone angle x (1 - penumbra)."""
+    a = np.float32(min(max(angle, 0.0), math.pi))
+    co = np.float32(math.cos(a))
+    ci = np.float32(math.cos(np.float32(a * np.float32(1.0 - min(max(penumbra, 0.0), 1.0)))))
+    return float(co), float(np.float32(1.0) / np.float32(max(ci - co, np.float32(1e-3))))
+
+
+def point_light(position, candela, radius=0.0, range_=0.0, cast_shadow=True, specular_scale=1.0) -> np.ndarray:
+    """scene.rs:1332-1341 GpuLight of a point light: candela rgb, a sphere of `radius` for soft shadows, `range_` the
+    distance its windowed attenuation reaches (0: none)."""
+    li = np.zeros(20, np.float32)
+    li[0:4] = (*position, 1.0)
+    li[4:8] = (0.0, 0.0, -1.0, radius)
+    li[8:11] = candela
+    li[11] = range_
+    li[15] = 1.0 if cast_shadow else 0.0
+    li[17] = specular_scale
+    return li
+
+
+def spot_light(position, direction, candela, angle, penumbra=0.0, radius=0.0, range_=0.0, cast_shadow=True,
+               specular_scale=1.0) -> np.ndarray:
+    """scene.rs:1342-1353 GpuLight of a spot light: `direction` its axis, `angle` its outer half angle (radians),
+    `penumbra` the share of it over which it fades."""
+    d = np.asarray(direction, np.float64)
+    d = d / np.linalg.norm(d)
+    li = point_light(position, candela, radius, range_, cast_shadow, specular_scale)
+    li[3] = 2.0
+    li[4:7] = d
+    li[12:14] = spot_cone(angle, penumbra)
+    return li
+
+
+def rect_light(center, normal, width_axis, width, height, nits, range_=0.0, barn_door_deg=90.0, cast_shadow=True,
+               specular_scale=1.0) -> np.ndarray:
+    """scene.rs:1355-1367 GpuLight of a hidden rect light (UE RectLight): a one-sided Lambertian rectangle of
+    luminance `nits` facing `normal`, `width` along `width_axis` and `height` across it, cut beyond the barn-door
+    angle."""
+    n = np.asarray(normal, np.float64)
+    n = n / np.linalg.norm(n)
+    ux = np.asarray(width_axis, np.float64)
+    ux = ux / np.linalg.norm(ux) * (width * 0.5)
+    li = np.zeros(20, np.float32)
+    li[0:4] = (*center, 3.0)
+    li[4:8] = (*n, height * 0.5)
+    li[8:11] = nits
+    li[11] = range_
+    li[12:15] = ux
+    li[15] = 1.0 if cast_shadow else 0.0
+    li[17] = specular_scale
+    li[18] = math.cos(math.radians(min(max(barn_door_deg, 0.0), 90.0)))
+    return li
+
+
+def light_order(lights: list) -> list:
+    """lib.rs:367-380 light_order, as the kernel needs it: directional lights first, then local lights that cast no
+    shadow, then shadowed ones (the reference also sorts each class spatially for its light groups, which only skip
+    work; the kernel here reads no groups and the order within a class is kept)."""
+    cls = lambda li: 0 if li[3] == 0.0 else (1 if li[15] < 0.5 else 2)  # noqa: E731
+    return sorted(range(len(lights)), key=lambda i: cls(lights[i]))
+
+
 # ---- lib.rs:1026-1104 specular_albedo / energy_lut, in f64 ------------------------------------------------------
 def specular_albedo(mu: float, alpha: float, nr: int = 48, nphi: int = 8) -> tuple[float, float]:
     mu = min(max(mu, 1e-4), 1.0)
@@ -388,7 +452,10 @@ class RenderScene:
         f[28:32] = (clamp, self.total_weight, energy_compensation, 0.0)
         f[32:35] = sky
         env_on = 1 if (env_nee and (0.2126 * sky[0] + 0.7152 * sky[1] + 0.0722 * sky[2]) > 0) else 0
-        p[36:40] = (self.n_directional, env_on, 0, 0)
+        # lib.rs:725 counts2: the directional lights, the sky's sampling, -, the first shadowed local light
+        first_shadowed = next((k for k in range(len(self.lights)) if self.lights[k, 3] != 0.0 and
+                               self.lights[k, 15] >= 0.5), len(self.lights))
+        p[36:40] = (self.n_directional, env_on, 0, first_shadowed)
         f[68] = 0.0
         f[70] = np.uint32(self.prep.top_first).view(np.float32)
         f[71] = np.uint32(self.prep.units).view(np.float32)
@@ -535,8 +602,10 @@ def build(meshes: list[RMesh], instances: list[RInstance], materials: list[np.nd
                                                                                                          np.float32)
     objects = np.zeros((OBJECT_STRIDE, 4), np.float32)
     objects[0, 3] = 1.0
-    lights = np.stack(lights).astype(np.float32) if len(lights) else np.zeros((0, 20), np.float32)
-    rs = RenderScene(prep, vertices, indices, rows, mats, lights, len(lights), emitters, emitter_tris, objects,
+    lights = np.stack([lights[k] for k in light_order(lights)]).astype(np.float32) if len(lights) else \
+        np.zeros((0, 20), np.float32)
+    n_dir = int((lights[:, 3] == 0.0).sum())
+    rs = RenderScene(prep, vertices, indices, rows, mats, lights, n_dir, emitters, emitter_tris, objects,
                      energy_lut(), float(total))
     if tree is not None:
         rs.light_tree = tree.nodes.astype(np.float32)
