@@ -139,7 +139,10 @@ typedef struct {
     uint64_t row_count;
     const void *materials;    /* scene.rs:480-491 GpuMaterial, 288 B */
     uint64_t material_count;
-    const void *lights;       /* scene.rs:495-501 GpuLight, 80 B (directional lights first) */
+    const void *lights;       /* scene.rs:495-501 GpuLight, 80 B: lib.rs:367-380's order, directional lights (type
+                               * pos.w 0) first, then point (1), spot (2) and rect (3) lights, those that cast no
+                               * shadow (ext.w 0) before those that do; other types, or a directional light after a
+                               * local one, are refused */
     uint64_t light_count;
     const void *emitters;     /* lib.rs:451-481 emitter_cdf: entries, 16385 guide vec4s, header vec4 */
     uint64_t emitter_vec4s;
@@ -280,9 +283,11 @@ int trt_accum_add_fp64(trt_accum *acc, double *d_sum);
 /* Samples first_sample .. first_sample + spp - 1 of every pixel added to `acc` (gpu.rs:4177-4196: the sample index
  * is P.size.z), on the device's stream. params: the 624-byte Params block (pathtrace.wgsl:96-140); its size.xy
  * must be the accumulator's. Needs a scene uploaded with its shading tables. Refuses (TRT_E_INVALID, named) what the
- * offline `main` would read and this core does not render: an environment map, local lights, fog, volumetric
- * scattering, procedural geometry, the live light LOD, aerial perspective, a scene background, more than 64
- * bounces, and light-tree candidates (P.ltree.w) for a scene uploaded without a tree. */
+ * offline `main` would read and this core does not render: an environment map, fog, volumetric scattering,
+ * procedural geometry, the live light LOD, aerial perspective, a scene background, more than 64 bounces, and
+ * light-tree candidates (P.ltree.w) for a scene uploaded without a tree; and light counts the scene does not hold:
+ * P.counts.x past its lights, P.counts2.x other than its directional lights when local lights follow, P.counts2.w
+ * (the first shadow-casting local light, lib.rs:700) outside the local lights. */
 int trt_render(trt_scene *scene, const void *params, uint32_t first_sample, uint32_t spp, trt_accum *acc);
 /* The rays the device's render launches traced since the last call, which resets them: closest-hit rays (camera and
  * bounce) and shadow rays (sun and emitter visibility). Waits for the device. A measurement with no counterpart in

@@ -1,23 +1,26 @@
 /*
  * Oracle (test infrastructure only): the offline path loop, pathtrace.wgsl `main` (LIVE = false) restated for the
  * features the workloads use: pinhole camera, instanced triangle meshes with per-section materials (Lambert + GGX
- * metal-rough, emission), directional lights (soft cone), emissive triangles from the CDF (RIS at camera hits), a
- * constant sky with light sampling, MIS, Russian roulette, the indirect clamp, FP32 accumulation. Not restated:
- * textures, glass, clearcoat, media, fog, procedural geometry, the light tree, local lights, live-only paths.
+ * metal-rough, emission, clearcoat, glass, textures), directional lights (soft cone), point, spot and rect lights,
+ * emissive triangles from the CDF or the light tree (RIS at camera hits), a constant sky with light sampling, MIS,
+ * Russian roulette, the indirect clamp, FP32 accumulation. Not restated: media, fog, procedural geometry, cutouts,
+ * live-only paths.
  *
  *   pathtrace.wgsl:4002-4024 camera_ray       4106-4557 main (offline parts)
  *   pathtrace.wgsl:279-283 inst_vec/inst_sign  286-288 inst_normal  310-313 material_of  330-360 tri_data
  *   pathtrace.wgsl:406-413 inst_attr          430-432 seen_directly
  *   pathtrace.wgsl:463-487 emitter totals / pick   547-560 emitter_draw (no tree)   590-593 emitter_pmf (no tree)
  *   pathtrace.wgsl:846-855 shadow_tr          909-915 light_vis   975-977 sky_tr   1066-1070 emitter_tr
- *   pathtrace.wgsl:1312-1317 emission         1320-1425 surface (no textures / functions / glass / coat)
- *   pathtrace.wgsl:1492-1505 energy_lookup    1521-1525 spec_terms   1642-1699 lobes (no glass / coat)
+ *   pathtrace.wgsl:1312-1317 emission         1320-1425 surface (no functions)
+ *   pathtrace.wgsl:1492-1505 energy_lookup    1521-1525 spec_terms   1642-1699 lobes
  *   pathtrace.wgsl:1707-1730 d_ggx / vis_smith_joint_approx / f_schlick   1733-1764 eval_bsdf_k   1766-1789 g1,
  *   pdf_bsdf   1797-1817 light_mis   1819-1844 basis / sample_vndf   1848-1884 sample_bsdf
  *   pathtrace.wgsl:1895-1930 env_radiance / env_pdf / sample_env (constant sky)   1943-1960 power_heuristic,
  *   luminance, offset_origin   1963-1988 sky_mis / sun_disks   2009-2024 emit_mis
  *   pathtrace.wgsl:2269-2393 emitter_radiance / sample_emitter_ris / sample_emitter   2431-2442 env_light
- *   pathtrace.wgsl:2444-2540 direct_light (no local lights, no light_pick)
+ *   pathtrace.wgsl:2042-2104 local_light    2158-2262 local_lights (offline: no light tiles, no volumes)
+ *   pathtrace.wgsl:2444-2540 direct_light (no light_pick)
+ *   textures, glass, the clearcoat and the light tree: cited where each is restated below
  * WGSL leaves normalize, inverseSqrt, sin and cos to the driver; here normalize(v) = v / sqrt(v.v),
  * inverseSqrt(x) = 1 / sqrt(x), sin/cos by or_sincos. Everything else rounds in the order the reference writes it.
  */

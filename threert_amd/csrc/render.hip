@@ -15,9 +15,9 @@
 // trilinear read tex / tex_level 1107-1117 with srgb_to_linear 1102-1104, mesh_uv / mesh_density / mesh_lod
 // 1119-1140, emission 1312-1317, the colour, roughness, metalness, emissive, normal and transmission maps of
 // surface 1344-1425 with hit_tangent 366-394, glass_at's maps 950-956, emitter_radiance 2269-2283, the path loop's
-// ray cone 4165-4167 and 4451-4452). Not here: media, fog, procedural geometry, local lights, cutouts, glass among a
-// shadow caster's sections, live-only paths (the upload and the launch refuse scenes and parameters that need
-// them).
+// ray cone 4165-4167 and 4451-4452); local lights (kExt: local_light 2042-2104, local_lights 2158-2262 offline, with
+// no light tiles or volumes). Not here: media, fog, procedural geometry, cutouts, glass among a shadow caster's
+// sections, live-only paths (the upload and the launch refuse scenes and parameters that need them).
 // normalize(v) = v / sqrt(v.v), inverseSqrt(x) = 1 / sqrt(x); sin / cos by Moshier's Cephes reduction and
 // polynomials in fmaf; every other operation in the reference's order, no contraction: the same bits as the C
 // restatement the tests hold this to.
