@@ -200,6 +200,22 @@ def test_shadowless_lights_pass_blockers_and_few_lights_are_summed():
     assert np.all(np.abs(m - want) < 0.01 * want), (m, want)
 
 
+def test_a_soft_lights_half_shadow_under_a_blockers_edge():
+    """A sphere light of radius 0.3 m 2 m over the patch, an opaque slab at y = 1 over x >= 0: the shadow ray aims at
+    a uniform point of the sphere, so under the slab's edge exactly the points with x > 0 are hidden: half the
+    unshadowed value (the patch's spread in x averages out), +- 2 %; a hard light (radius 1 mm) lights the patch
+    fully 0.3 m to the -x side and not at all 0.3 m to the +x side."""
+    slab = (R.box_tris((0.0, 0.975, -6.0), (6.0, 1.025, 6.0)), R.material((0.5, 0.5, 0.5), specular=0.0))
+    one = 0.5 * 1000 / 4 / math.pi
+    m = render_patch(plane_scene([R.point_light((0, 2, 0), 1000.0, radius=0.3)], [slab]),
+                     patch_camera((0, 0.6, 3), (0, 0, 0)), 256)
+    assert np.all(np.abs(m - 0.5 * one) < 0.02 * 0.5 * one), (m, 0.5 * one)
+    for x, lit in ((-0.3, True), (0.3, False)):
+        rs = plane_scene([R.point_light((x, 2, 0), 1000.0, radius=0.001)], [slab])
+        m = render_patch(rs, patch_camera((x, 0.6, 3), (x, 0, 0)), 4)
+        assert (np.all(np.abs(m - one) < 0.03 * one) if lit else np.all(m == 0.0)), (x, m)
+
+
 def test_a_pane_tints_a_point_lights_shadow():
     """shadow-glass-tint.ts as written, with its point light: 0.5 I / 4 / pi x (0.96 x 0.5)^2 +- 2 %."""
     pane = (R.box_tris((-0.5, 0.975, -0.5), (0.5, 1.025, 0.5)),
