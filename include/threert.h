@@ -168,6 +168,15 @@ typedef struct {
     uint32_t tex_layers;
     const void *tex_info;
     uint64_t tex_info_vec4s;
+    /* The environment map, as Gpu::upload takes it (gpu.rs:2613 `env`, `env_cdf`): env, an equirectangular map of
+     * env_width x env_height RGBA float texels (lib.rs SkyBaked's EnvMap; row 0 the top, +y), kept as RGBA half floats
+     * of min(v, 65000) (gpu.rs:3552-3561 write_env_texels); env_cdf, lib.rs:916-942 build_env_cdf's env_height +
+     * env_width x env_height floats. Rendered where Params' env.x is set (env.yz its size); NULL env: none */
+    const void *env;
+    uint32_t env_width;
+    uint32_t env_height;
+    const void *env_cdf;
+    uint64_t env_cdf_count;
 } trt_scene_desc;
 
 typedef struct {

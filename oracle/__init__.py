@@ -277,7 +277,8 @@ class _RenderScene(C.Structure):
                 ("object_count", C.c_uint64), ("energy", C.c_void_p), ("light_tree", C.c_void_p),
                 ("light_tree_count", C.c_uint64), ("glass_energy", C.c_void_p), ("tex_pages", C.c_void_p),
                 ("tex_side", C.c_uint32), ("tex_layers", C.c_uint32), ("tex_info", C.c_void_p),
-                ("tex_info_vec4s", C.c_uint64)]
+                ("tex_info_vec4s", C.c_uint64), ("env", C.c_void_p), ("env_w", C.c_uint32), ("env_h", C.c_uint32),
+                ("env_cdf", C.c_void_p), ("env_cdf_count", C.c_uint64)]
 
 
 class RenderScene:
@@ -285,7 +286,7 @@ class RenderScene:
 
     def __init__(self, nodes, leaves, instances, top, units, vertices, indices, rows, materials, lights, emitters,
                  emitter_tris, objects, energy, light_tree=None, glass_energy=None, tex_pages=None, tex_side=0,
-                 tex_layers=0, tex_info=None):
+                 tex_layers=0, tex_info=None, env=None, env_cdf=None):
         self.geo = Scene(nodes, leaves, instances, top, units)
         if light_tree is None:
             light_tree = np.zeros((0, 4), np.float32)
@@ -304,12 +305,23 @@ class RenderScene:
                 raise ValueError("tex_pages must hold tex_layers layers of 5 RGBA8 levels of tex_side >> level")
         self.keep += [np.ascontiguousarray(tex_pages, np.uint8).reshape(-1),
                       np.ascontiguousarray(tex_info, np.float32).reshape(-1, 4)]
-        v, i, r, m, li, e, et, o, en, lt, ge, tp, ti = self.keep
+        # env: (h, w, 4) uint16 half-float texels; env_cdf: h + w h floats
+        if env is None:
+            env, env_cdf = np.zeros((0, 0, 4), np.uint16), np.zeros(0, np.float32)
+        else:
+            env = np.asarray(env)
+            if env.dtype != np.uint16 or env.ndim != 3 or env.shape[2] != 4:
+                raise ValueError("env must be (h, w, 4) uint16 half-float texels")
+            if np.asarray(env_cdf).size != env.shape[0] * (1 + env.shape[1]):
+                raise ValueError("env_cdf must hold h + w h floats")
+        self.keep += [np.ascontiguousarray(env, np.uint16), np.ascontiguousarray(env_cdf, np.float32).reshape(-1)]
+        v, i, r, m, li, e, et, o, en, lt, ge, tp, ti, ev, ec = self.keep
         self._s = _RenderScene(self.geo._s, _ptr(v), len(v), _ptr(i), len(i), _ptr(r), len(r), _ptr(m), len(m),
                                _ptr(li), len(li), _ptr(e), len(e), _ptr(et), len(et), _ptr(o), len(o) // 3,
                                _ptr(en), _ptr(lt) if len(lt) else None, len(lt), _ptr(ge) if len(ge) else None,
                                _ptr(tp) if len(ti) else None, tex_side, tex_layers, _ptr(ti) if len(ti) else None,
-                               len(ti))
+                               len(ti), _ptr(ev) if ev.size else None, ev.shape[1], ev.shape[0],
+                               _ptr(ec) if ev.size else None, len(ec))
         L = lib()
         L.or_render_sample.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_int]
         L.or_render_sample.restype = C.c_int64
@@ -324,6 +336,8 @@ class RenderScene:
         L.or_glass_share.restype = C.c_float
         L.or_tex.argtypes = [C.c_void_p, C.c_int32, C.c_float, C.c_float, C.c_float, C.c_void_p]
         L.or_tex.restype = C.c_int
+        L.or_env.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.or_env.restype = C.c_int
 
     def emitter_draw(self, params: np.ndarray, x, n, u) -> tuple[int, float]:
         """pathtrace.wgsl:547-585 emitter_draw at x (normal n) for uniforms u[3]: (CDF entry, p); entry -1: none."""
@@ -359,6 +373,16 @@ class RenderScene:
         if lib().or_tex(C.byref(self._s), tex_id, u, v, lod, _ptr(out)) < 0:
             raise RuntimeError("tex read outside its tables")
         return out
+
+    def env(self, params: np.ndarray, d, u=(0.5, 0.5, 0.5, 0.5)) -> tuple[np.ndarray, float, np.ndarray]:
+        """pathtrace.wgsl:1895-1941 under `params`: env_radiance and env_pdf along d, and sample_env's direction for the
+        uniforms u[4]."""
+        p = np.ascontiguousarray(params, np.uint32)
+        ds, us = (np.ascontiguousarray(a, np.float32) for a in (d, u))
+        rad, smp, pdf = np.zeros(3, np.float32), np.zeros(3, np.float32), C.c_float()
+        if lib().or_env(C.byref(self._s), _ptr(p), _ptr(ds), _ptr(us), _ptr(rad), C.byref(pdf), _ptr(smp)) < 0:
+            raise RuntimeError("the environment's reads fell outside its tables")
+        return rad, float(pdf.value), smp
 
     def glass_share(self, params: np.ndarray, material: int, cos_d: float) -> float:
         """pathtrace.wgsl:930-970 glass_at's share for `material` at |cos| cos_d, under `params`."""
@@ -447,3 +471,17 @@ def sincos(x: float):
     s, c = C.c_float(), C.c_float()
     L.or_sincos(C.c_float(x), C.byref(s), C.byref(c))
     return s.value, c.value
+
+
+def atan2(y: float, x: float) -> float:
+    """The kernel's atan2 (Cephes atanf, render.c or_atan2)."""
+    L = lib()
+    L.or_atan2.argtypes, L.or_atan2.restype = [C.c_float, C.c_float], C.c_float
+    return float(L.or_atan2(y, x))
+
+
+def acos(x: float) -> float:
+    """The kernel's acos (Cephes asinf, render.c or_acos)."""
+    L = lib()
+    L.or_acos.argtypes, L.or_acos.restype = [C.c_float], C.c_float
+    return float(L.or_acos(x))

@@ -67,6 +67,14 @@ typedef struct {
     uint32_t tex_side, tex_layers;
     const float *tex_info;
     uint64_t tex_info_vec4s;
+    /* The environment map as gpu.rs:3537-3561 env_texture holds it: RGBA half floats (each value min(v, 65000)
+     * rounded to half), env_w x env_h texels, row after row from the top (v = 0, +y); and lib.rs:916-942
+     * build_env_cdf, env_h + env_w x env_h floats (the rows' CDF, then each row's). env NULL: none (P.env.x must be
+     * 0) */
+    const uint16_t *env;
+    uint32_t env_w, env_h;
+    const float *env_cdf;
+    uint64_t env_cdf_count;
 } or_render_scene;
 
 /* One sample (params word 18 = P.size.z) of every pixel of rows [y0, y1), accumulated into `accum` (80 bytes a
@@ -88,6 +96,13 @@ float or_glass_share(const or_render_scene *s, const uint32_t *params, uint32_t 
 /* The kernel's log2 and exp2 (Cephes log2f / exp2f); tex (pathtrace.wgsl:1107-1117) of texture `id` at uv with LOD
  * `lod` into out[4] (returns -1 when it read outside its tables) */
 float or_log2(float x);
+/* The kernel's atan2 and acos (Cephes atanf / asinf); env_radiance, env_pdf and sample_env (pathtrace.wgsl:1889-1941)
+ * of the scene's environment under `params` (sample_env for the uniforms u[4]); -1 / NaN when a read fell outside
+ * its table */
+float or_atan2(float y, float x);
+float or_acos(float x);
+int or_env(const or_render_scene *s, const uint32_t *params, const float *d, const float *u, float *radiance,
+           float *pdf, float *sample);
 float or_exp2(float x);
 int or_tex(const or_render_scene *s, int32_t id, float u, float v, float lod, float *out);
 

@@ -160,6 +160,12 @@ static inline uint32_t fb(float f) {
     memcpy(&u, &f, 4);
     return u;
 }
+static inline float bits_f(uint32_t u) {
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+static inline uint32_t umin(uint32_t a, uint32_t b) { return a < b ? a : b; }
 static inline int non_finite(V v) {
     return (fb(v.x) & 0x7fffffffu) >= 0x7f800000u || (fb(v.y) & 0x7fffffffu) >= 0x7f800000u ||
            (fb(v.z) & 0x7fffffffu) >= 0x7f800000u;
@@ -970,24 +976,165 @@ static inline V light_origin(const Ctx *c, const Surface *s, V o, V l) {
     return o;
 }
 
-static V env_radiance(const Ctx *c, V d) {
-    (void)d;
-    return v3(pf(c->P, SKY, 0), pf(c->P, SKY, 1), pf(c->P, SKY, 2));
+/* unpack2x16float: IEEE binary16 to binary32, exactly */
+static float half_to_float(uint32_t h) {
+    uint32_t sign = (h & 0x8000u) << 16, e = (h >> 10) & 0x1fu, m = h & 0x3ffu, u;
+    if (e == 0x1fu)
+        u = sign | 0x7f800000u | (m << 13);
+    else if (e == 0u) {
+        float f = (float)m * 5.9604644775390625e-8f; /* m x 2^-24, exact */
+        memcpy(&u, &f, 4);
+        u |= sign;
+    } else
+        u = sign | ((e + 112u) << 23) | (m << 13);
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
 }
-static float env_pdf(void) { return 1.0f / FOUR_PI; }
+/* ---- the environment map (pathtrace.wgsl:1889-1941) ------------------------------------------------------------
+ * atan2 and acos by Cephes atanf / asinf (S. L. Moshier): the argument reduced to [0, tan(pi/8)] or [0, 1/2] and a
+ * minimax polynomial there, in fmaf; the kernel has the same steps, so the same bits. */
+static float atan_red(float x) { /* atan of x >= 0 */
+    float y = 0.0f;
+    if (x > 2.414213562373095f) {
+        y = 1.5707963267948966f;
+        x = -(1.0f / x);
+    } else if (x > 0.4142135623730950f) {
+        y = 0.7853981633974483f;
+        x = (x - 1.0f) / (x + 1.0f);
+    }
+    const float z = x * x;
+    float p = fmaf(8.05374449538e-2f, z, -1.38776856032e-1f);
+    p = fmaf(p, z, 1.99777106478e-1f);
+    p = fmaf(p, z, -3.33329491539e-1f);
+    return y + fmaf(p * z, x, x);
+}
+float or_atan2(float y, float x) {
+    if (x != x || y != y) return x + y;
+    if (x == 0.0f) return y > 0.0f ? 1.5707963267948966f : (y < 0.0f ? -1.5707963267948966f : 0.0f);
+    float w = 0.0f;
+    if (x < 0.0f) w = y < 0.0f ? -3.141592653589793f : 3.141592653589793f;
+    const float q = y / x;
+    const float a = q < 0.0f ? -atan_red(-q) : atan_red(q);
+    return w + a;
+}
+static float asin_red(float a) { /* asin of a in [0, 1] */
+    float x, z;
+    int flag = 0;
+    if (a > 0.5f) {
+        z = 0.5f * (1.0f - a);
+        x = sqrtf(z);
+        flag = 1;
+    } else {
+        x = a;
+        z = x * x;
+    }
+    float r = x;
+    if (!(a < 1e-4f)) {
+        float p = fmaf(4.2163199048e-2f, z, 2.4181311049e-2f);
+        p = fmaf(p, z, 4.5470025998e-2f);
+        p = fmaf(p, z, 7.4953002686e-2f);
+        p = fmaf(p, z, 1.6666752422e-1f);
+        r = fmaf(p * z, x, x);
+    }
+    if (flag) r = 1.5707963267948966f - (r + r);
+    return r;
+}
+float or_acos(float x) {
+    if (!(x >= -1.0f && x <= 1.0f)) return NAN;
+    if (x < -0.5f) return 3.141592653589793f - 2.0f * asin_red(sqrtf(0.5f * (1.0f + x)));
+    if (x > 0.5f) return 2.0f * asin_red(sqrtf(0.5f * (1.0f - x)));
+    return 1.5707963267948966f - (x < 0.0f ? -asin_red(-x) : asin_red(x));
+}
+/* WGSL's float to u32: saturating, NaN to 0 */
+static inline uint32_t u32_sat(float x) { return x > 0.0f ? (x < 4294967040.0f ? (uint32_t)x : 0xffffff00u) : 0u; }
+/* env_uv (1889-1893) */
+static void env_uv(V d, float *u, float *v) {
+    *u = or_atan2(d.x, -d.z) / TWO_PI + 0.5f;
+    *v = or_acos(clampf(d.y, -1.0f, 1.0f)) / PI_F;
+}
+static inline float env_cdf_at(const Ctx *c, uint64_t i) { return c->s->env_cdf[ck(c, i, c->s->env_cdf_count)]; }
+/* env_radiance (1895-1898): the map's texture read at level 0 as the reference's sampler defines it (repeat in u
+ * and v, linear), with float weights */
+static V env_radiance(const Ctx *c, V d) {
+    if (pf(c->P, ENV, 0) < 0.5f) return v3(pf(c->P, SKY, 0), pf(c->P, SKY, 1), pf(c->P, SKY, 2));
+    float u, v;
+    env_uv(d, &u, &v);
+    const uint32_t w = c->s->env_w, h = c->s->env_h;
+    float x = u * (float)w - 0.5f, y = v * (float)h - 0.5f;
+    float x0f = floorf(x), y0f = floorf(y);
+    const float fx = x - x0f, fy = y - y0f;
+    if (!(x0f >= -1048576.0f && x0f <= 1048576.0f)) x0f = 0.0f;
+    if (!(y0f >= -1048576.0f && y0f <= 1048576.0f)) y0f = 0.0f;
+    const int32_t xi = (int32_t)x0f, yi = (int32_t)y0f, wi = (int32_t)w, hi = (int32_t)h;
+    const uint32_t xa = (uint32_t)(((xi % wi) + wi) % wi), ya = (uint32_t)(((yi % hi) + hi) % hi);
+    const uint32_t xb = xa + 1u == w ? 0u : xa + 1u, yb = ya + 1u == h ? 0u : ya + 1u;
+    const uint64_t n = (uint64_t)w * h;
+    const uint16_t *t00 = c->s->env + 4u * ck(c, (uint64_t)ya * w + xa, n);
+    const uint16_t *t10 = c->s->env + 4u * ck(c, (uint64_t)ya * w + xb, n);
+    const uint16_t *t01 = c->s->env + 4u * ck(c, (uint64_t)yb * w + xa, n);
+    const uint16_t *t11 = c->s->env + 4u * ck(c, (uint64_t)yb * w + xb, n);
+    float o[3];
+    for (int j = 0; j < 3; j++)
+        o[j] = mixf(mixf(half_to_float(t00[j]), half_to_float(t10[j]), fx),
+                    mixf(half_to_float(t01[j]), half_to_float(t11[j]), fx), fy);
+    return v3(o[0], o[1], o[2]);
+}
+/* env_pdf (1900-1913): the rows' and the row's CDF steps of the texel d falls in, per solid angle */
+static float env_pdf(const Ctx *c, V d) {
+    if (pf(c->P, ENV, 0) < 0.5f) return 1.0f / FOUR_PI;
+    const uint32_t w = c->s->env_w, h = c->s->env_h;
+    float u, v;
+    env_uv(d, &u, &v);
+    const uint32_t x = umin(u32_sat(u * (float)w), w - 1u), y = umin(u32_sat(v * (float)h), h - 1u);
+    const float row_p = env_cdf_at(c, y) - (y > 0u ? env_cdf_at(c, y - 1u) : 0.0f);
+    const uint64_t base = (uint64_t)h + (uint64_t)y * w;
+    const float col_p = env_cdf_at(c, base + x) - (x > 0u ? env_cdf_at(c, base + x - 1u) : 0.0f);
+    float sin_t, cos_t;
+    or_sincos(((float)y + 0.5f) / (float)h * PI_F, &sin_t, &cos_t);
+    return row_p * col_p * (float)w * (float)h / (2.0f * PI_F * PI_F * fmaxf(sin_t, 1e-4f));
+}
+/* lower_bound (1915-1923) */
+static uint32_t env_lower_bound(const Ctx *c, uint64_t start, uint32_t count, float x) {
+    uint32_t lo = 0u, hi = count;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) / 2u;
+        if (env_cdf_at(c, start + mid) < x)
+            lo = mid + 1u;
+        else
+            hi = mid;
+    }
+    return umin(lo, count - 1u);
+}
 static int env_sampled(const Ctx *c) {
     return pu(c->P, COUNTS2, 1) != 0u &&
            (pf(c->P, ENV, 0) > 0.5f || luminance(v3(pf(c->P, SKY, 0), pf(c->P, SKY, 1), pf(c->P, SKY, 2))) > 0.0f);
 }
+/* sample_env (1925-1941) for the uniforms u[4]: the uniform sphere for a constant sky; the map's row by its CDF,
+ * the texel in it by the row's, a uniform point of the texel in (u, v) */
+static V sample_env_u(const Ctx *c, const float *u) {
+    if (pf(c->P, ENV, 0) < 0.5f) {
+        float z = 1.0f - 2.0f * u[0];
+        float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+        float phi = TWO_PI * u[1];
+        float sp, cp;
+        or_sincos(phi, &sp, &cp);
+        return v3(r * cp, z, r * sp);
+    }
+    const uint32_t w = c->s->env_w, h = c->s->env_h;
+    const uint32_t y = env_lower_bound(c, 0u, h, u[0]);
+    const uint32_t x = env_lower_bound(c, (uint64_t)h + (uint64_t)y * w, w, u[1]);
+    const float uu = ((float)x + u[2]) / (float)w, vv = ((float)y + u[3]) / (float)h;
+    const float theta = vv * PI_F, phi = (uu - 0.5f) * 2.0f * PI_F;
+    float st, ct, sp, cp;
+    or_sincos(theta, &st, &ct);
+    or_sincos(phi, &sp, &cp);
+    return v3(st * sp, ct, -st * cp);
+}
 static V sample_env(Ctx *c, uint32_t rd) {
     float u[4];
     or_rng4(&c->g, rd, u);
-    float z = 1.0f - 2.0f * u[0];
-    float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
-    float phi = TWO_PI * u[1];
-    float sp, cp;
-    or_sincos(phi, &sp, &cp);
-    return v3(r * cp, z, r * sp);
+    return sample_env_u(c, u);
 }
 
 static V sun_disks(const Ctx *c, V d, float pdf) {
@@ -1003,7 +1150,7 @@ static V sun_disks(const Ctx *c, V d, float pdf) {
 }
 static V sky_mis(const Ctx *c, V d, float pdf) {
     float wgt = 1.0f;
-    if (pdf > 0.0f && pu(c->P, COUNTS2, 1) != 0u) wgt = power_heuristic(pdf, env_pdf());
+    if (pdf > 0.0f && pu(c->P, COUNTS2, 1) != 0u) wgt = power_heuristic(pdf, env_pdf(c, d));
     V col = vs(env_radiance(c, d), wgt);
     if (pdf > 0.0f) col = vadd(col, sun_disks(c, d, pdf));
     return col;
@@ -1038,21 +1185,6 @@ static const float *emitter_pick_u(const Ctx *c, float u) {
 static inline int lt_on(const Ctx *c) { return pf(c->P, LTREE, 3) > 0.5f; }
 static inline const float *lt_node(const Ctx *c, uint32_t i) {
     return c->s->light_tree + 4 * ck(c, i, c->s->light_tree_count);
-}
-/* unpack2x16float: IEEE binary16 to binary32, exactly */
-static float half_to_float(uint32_t h) {
-    uint32_t sign = (h & 0x8000u) << 16, e = (h >> 10) & 0x1fu, m = h & 0x3ffu, u;
-    if (e == 0x1fu)
-        u = sign | 0x7f800000u | (m << 13);
-    else if (e == 0u) {
-        float f = (float)m * 5.9604644775390625e-8f; /* m x 2^-24, exact */
-        memcpy(&u, &f, 4);
-        u |= sign;
-    } else
-        u = sign | ((e + 112u) << 23) | (m << 13);
-    float f;
-    memcpy(&f, &u, 4);
-    return f;
 }
 /* lt_imp (504-524): power x the best cosine any point of the node's sphere has at x / max(d^2, r^2); the offline
  * kernel has no approximations, so no node is cut for distance */
@@ -1320,7 +1452,7 @@ static V env_light(Ctx *c, const Surface *s, const Lobes *lb, V v, V o) {
     V l = sample_env(c, or_rng_dim(&c->g, D_ENV));
     V f = eval_bsdf(c, s, lb, v, l);
     if (any_pos(f)) {
-        float pdf_e = env_pdf();
+        float pdf_e = env_pdf(c, l);
         if (pdf_e > 0.0f) {
             V vis = shadow_tr(c, light_origin(c, s, o, l), l, OR_INF, 1); /* sky_tr (975-977) */
             if (any_pos(vis)) {
@@ -1745,5 +1877,17 @@ int or_tex(const or_render_scene *s, int32_t id, float u, float v, float lod, fl
     int64_t oob = 0;
     Ctx c = hook_ctx(s, NULL, &oob);
     tex(&c, id, u, v, lod, out);
+    return oob ? -1 : 0;
+}
+int or_env(const or_render_scene *s, const uint32_t *params, const float *d, const float *u, float *radiance,
+           float *pdf, float *sample) {
+    int64_t oob = 0;
+    Ctx c = hook_ctx(s, params, &oob);
+    const V dv = v3(d[0], d[1], d[2]);
+    const V r = env_radiance(&c, dv);
+    radiance[0] = r.x, radiance[1] = r.y, radiance[2] = r.z;
+    *pdf = env_pdf(&c, dv);
+    const V l = sample_env_u(&c, u);
+    sample[0] = l.x, sample[1] = l.y, sample[2] = l.z;
     return oob ? -1 : 0;
 }

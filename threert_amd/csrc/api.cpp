@@ -90,9 +90,9 @@ struct trt_scene {
     // shading tables (trt_render)
     void *vertices = nullptr, *indices = nullptr, *rows = nullptr, *materials = nullptr, *lights = nullptr,
          *emitters = nullptr, *emitter_tris = nullptr, *objects = nullptr, *energy = nullptr, *light_tree = nullptr,
-         *glass_energy = nullptr, *tex_pages = nullptr, *tex_info = nullptr;
-    uint64_t light_tree_count = 0, tex_info_vec4s = 0;
-    uint32_t tex_side = 0, tex_layers = 0;
+         *glass_energy = nullptr, *tex_pages = nullptr, *tex_info = nullptr, *env = nullptr, *env_cdf = nullptr;
+    uint64_t light_tree_count = 0, tex_info_vec4s = 0, env_cdf_count = 0;
+    uint32_t tex_side = 0, tex_layers = 0, env_w = 0, env_h = 0;  // env_w 0: no environment map
     bool tex = false;    // the scene has texture pages (the reference's F_TEX)
     bool shading = false;
     bool coat = false;   // a material has a clearcoat (layer.z > 0: the reference's F_COAT, gpu.rs features_of)
@@ -238,10 +238,11 @@ int trt_device_sync(trt_device *d) {
                                       "triangle indices", "vertex", "energy table", "emitter triangle (RIS candidate)",
                                       "leaf", "emitter triangle (RIS pick)", "emitter triangle (single draw)",
                                       "emitter triangle (a hit's own)", "light tree node", "glass energy table",
-                                      "texture page layer", "texture info"};
+                                      "texture page layer", "texture info", "environment texel",
+                                      "environment CDF"};
         std::string where;
         if (g[1]) {
-            const char *name = g[1] <= 10 ? what[g[1]] : (g[1] >= 16 && g[1] <= 33 ? table[g[1] - 16] : "?");
+            const char *name = g[1] <= 10 ? what[g[1]] : (g[1] >= 16 && g[1] <= 35 ? table[g[1] - 16] : "?");
             where = std::string(" (first: ") + name + " index " +
                     std::to_string((uint64_t(g[3]) << 32) | g[2]) + " of " + std::to_string(g[4]) + "; context " +
                     std::to_string(g[5]) + " " + std::to_string(g[6]) + " " + std::to_string(g[7]) + ")";
@@ -389,7 +390,8 @@ int trt_scene_release(trt_scene *s) {
     hipFree(s->wunits);
     hipFree(s->wleaves);
     for (void *p : {s->vertices, s->indices, s->rows, s->materials, s->lights, s->emitters, s->emitter_tris,
-                    s->objects, s->energy, s->light_tree, s->glass_energy, s->tex_pages, s->tex_info})
+                    s->objects, s->energy, s->light_tree, s->glass_energy, s->tex_pages, s->tex_info, s->env,
+                    s->env_cdf})
         hipFree(p);
     trt_device *d = s->dev;
     delete s;
@@ -486,6 +488,26 @@ static bool light_tree_ok(const trt_scene_desc *d, uint64_t entries, std::string
 // section rows, the rows' materials, the emitter entries' triangles and the emitters' triangles, instances, rows.
 // texpack.rs pages as the kernel reads them: a power-of-two side of 16 or more (5 levels), at least one layer, every
 // texture's rect on a layer the pages have and inside its page
+// binary32 to binary16, rounded to nearest even (half::f16::from_f32)
+static uint16_t half_of(float f) {
+    uint32_t x;
+    std::memcpy(&x, &f, 4);
+    const uint16_t sign = static_cast<uint16_t>((x >> 16) & 0x8000u);
+    const uint32_t ax = x & 0x7fffffffu;
+    if (ax >= 0x7f800000u) return sign | 0x7c00u | (ax > 0x7f800000u ? 0x200u : 0u);
+    if (ax >= 0x477ff000u) return sign | 0x7c00u;  // 65520 and above round to infinity
+    if (ax < 0x38800000u) {                        // below 2^-14: a subnormal half, m x 2^-24
+        float a;
+        const uint32_t bits = ax;
+        std::memcpy(&a, &bits, 4);
+        return sign | static_cast<uint16_t>(std::nearbyint(a * 16777216.0f));
+    }
+    uint32_t h = ((ax >> 23) - 112u) << 10 | ((ax & 0x7fffffu) >> 13);
+    const uint32_t rem = ax & 0x1fffu;
+    if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) h++;
+    return static_cast<uint16_t>(sign | h);
+}
+
 static uint64_t page_layer_bytes(uint32_t side) {
     uint64_t n = 0;
     for (uint32_t l = 0; l < 5; l++) n += uint64_t(side >> l) * (side >> l) * 4;
@@ -543,6 +565,20 @@ static bool check_shading(const trt_scene_desc *d, std::string *why) {
                            textures, why))
         return false;
     if (d->tex_info && !pages_ok(d, why)) return false;
+    if (d->env) {
+        const uint64_t w = d->env_width, h = d->env_height;
+        if (!d->env_cdf || w == 0 || h == 0 || w > 16384 || h > 16384 || d->env_cdf_count != h + w * h) {
+            *why = "an environment map needs 1 to 16384 texels a side and env_cdf of env_height + env_width x "
+                   "env_height floats";
+            return false;
+        }
+        const float *cdf = static_cast<const float *>(d->env_cdf);
+        for (uint64_t i = 0; i < d->env_cdf_count; i++)
+            if (!std::isfinite(cdf[i])) {
+                *why = "env_cdf holds a value that is not finite";
+                return false;
+            }
+    }
     const auto *inst = static_cast<const trt::Instance *>(d->instances);
     const auto *leaves = static_cast<const trt::Leaf *>(d->leaves);
     const auto *nodes = static_cast<const trt::Node *>(d->nodes);
@@ -816,6 +852,16 @@ int trt_scene_upload(trt_device *d, const trt_scene_desc *desc, trt_scene **out)
             s->tex_side = desc->tex_side, s->tex_layers = desc->tex_layers, s->tex_info_vec4s = desc->tex_info_vec4s;
             rc = upload(d, &s->tex_pages, desc->tex_pages, page_layer_bytes(desc->tex_side) * desc->tex_layers, &total);
             if (!rc) rc = upload(d, &s->tex_info, desc->tex_info, desc->tex_info_vec4s * 16, &total);
+        }
+        if (!rc && desc->env) {
+            // gpu.rs:3552-3561: the texels as half floats of min(v, 65000), rounded to nearest even
+            const uint64_t n = uint64_t(desc->env_width) * desc->env_height;
+            std::vector<uint16_t> half(4 * n);
+            const float *src = static_cast<const float *>(desc->env);
+            for (uint64_t i = 0; i < 4 * n; i++) half[i] = half_of(std::isnan(src[i]) ? 65000.0f : std::min(src[i], 65000.0f));
+            s->env_w = desc->env_width, s->env_h = desc->env_height, s->env_cdf_count = desc->env_cdf_count;
+            rc = upload(d, &s->env, half.data(), 8 * n, &total);
+            if (!rc) rc = upload(d, &s->env_cdf, desc->env_cdf, desc->env_cdf_count * 4, &total);
         }
     }
     if (rc) {
@@ -1119,7 +1165,12 @@ static int job_args(const trt_render_job &j, trt::RenderArgs &a, std::string *wh
     if (P[22] > 64) return *why = "more than 64 bounces", TRT_E_INVALID;
     // What the offline `main` would read and this core does not render (Params, pathtrace.wgsl:96-140), refused by
     // name rather than rendered without it.
-    if (trt::from_bits(P[24]) > 0.5f) return *why = "environment maps are not supported", TRT_E_INVALID;
+    // pathtrace.wgsl:1895-1941: P.env.x an environment map of P.env.yz texels, the scene's
+    if (trt::from_bits(P[24]) >= 0.5f &&
+        (!s->env || trt::from_bits(P[25]) != float(s->env_w) || trt::from_bits(P[26]) != float(s->env_h)))
+        return *why = s->env ? "P.env.yz is not the size of the scene's environment map"
+                             : "P.env.x asks for an environment map and the scene has none",
+               TRT_E_INVALID;
     if (trt::from_bits(P[44]) > 0.0f) return *why = "fog (P.fog.x) is not supported", TRT_E_INVALID;
     if (trt::from_bits(P[52]) > 0.0f) return *why = "volumetric scattering (P.vol.x) is not supported", TRT_E_INVALID;
     if (trt::from_bits(P[75]) > 0.0f) return *why = "the live light LOD (P.light_lod.w) is not supported", TRT_E_INVALID;
@@ -1163,6 +1214,10 @@ static int job_args(const trt_render_job &j, trt::RenderArgs &a, std::string *wh
     a.tex_info = static_cast<const float4 *>(s->tex_info);
     a.tex_info_count = s->tex_info_vec4s;
     a.tex_side = s->tex_side, a.tex_layers = s->tex_layers;
+    a.env = static_cast<const uint2 *>(s->env);
+    a.env_cdf = static_cast<const float *>(s->env_cdf);
+    a.env_cdf_count = s->env_cdf_count;
+    a.env_w = s->env_w, a.env_h = s->env_h;
     a.accum = static_cast<float4 *>(acc->buf);
     a.vertex_count = static_cast<uint32_t>(s->vertex_count);
     a.tri_count = static_cast<uint32_t>(s->tri_count);
@@ -1347,7 +1402,7 @@ static int render_batch(trt_device *d, const trt_render_job *jobs, uint32_t n, u
     table.reserve(n);
     uint64_t tiles = 0;
     uint32_t deepest = 0;
-    bool coat = false, tree = false, glass = false, tex = false, local = false;
+    bool coat = false, tree = false, glass = false, tex = false, local = false, env = false;
     for (uint32_t k = 0; k < n; k++) {
         if (jobs[k].spp == 0) continue;
         if (!jobs[k].scene || jobs[k].scene->dev != d)
@@ -1366,16 +1421,17 @@ static int render_batch(trt_device *d, const trt_render_job *jobs, uint32_t n, u
         glass = glass || jobs[k].scene->glass;
         tex = tex || jobs[k].scene->tex;
         local = local || j.args.params[20] > j.args.params[36];  // P.counts.x > P.counts2.x: local lights
+        env = env || trt::from_bits(j.args.params[24]) >= 0.5f;  // P.env.x: an environment map
         tree = tree || trt::from_bits(j.args.params[87]) > 0.5f;  // P.ltree.w: the job samples the light tree
         table.push_back(j);
     }
     if (tiles >= (uint64_t{1} << 31)) return fail(TRT_E_INVALID, "trt_render_batch: too many tiles in one launch");
     if (table.empty()) return ok();
-    // The wavefront takes batches whose jobs have at most kWfMaxSuns directional lights, no glass, no textures and no
-    // local lights; counting launches stay on the megakernel. TRT_RENDER_AUTO takes it when every scene has kWfAutoLeaves triangles or more,
-    // or for launches of a whole chunk of paths or more.
+    // The wavefront takes batches whose jobs have at most kWfMaxSuns directional lights, no glass, no textures, no
+    // local lights and no environment map; counting launches stay on the megakernel. TRT_RENDER_AUTO takes it when
+    // every scene has kWfAutoLeaves triangles or more, or for launches of a whole chunk of paths or more.
     const uint32_t how = d->render.load();
-    bool wf = reads == nullptr && how != TRT_RENDER_MEGAKERNEL && !glass && !tex && !local;
+    bool wf = reads == nullptr && how != TRT_RENDER_MEGAKERNEL && !glass && !tex && !local && !env;
     uint64_t paths = 0;
     bool large = true;
     std::vector<const trt_scene *> scene_of;
@@ -1441,7 +1497,7 @@ static int render_batch(trt_device *d, const trt_render_job *jobs, uint32_t n, u
         if (rc) return rc;
     } else if (trt::launch_render(static_cast<const trt::RenderJob *>(ring.dev[k]), static_cast<uint32_t>(table.size()),
                            static_cast<uint32_t>(tiles), d->counter, d->spill, threads,
-                           static_cast<uint32_t>(entries), d->guard, d->ray_counts, coat, tree, glass || tex || local, reads, grid,
+                           static_cast<uint32_t>(entries), d->guard, d->ray_counts, coat, tree, glass || tex || local || env, reads, grid,
                            d->stream))
         return fail(TRT_E_DEVICE, "render kernel launch failed");
     TRT_HIP(hipEventRecord(ring.ev[k], d->stream));

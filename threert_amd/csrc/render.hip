@@ -225,6 +225,64 @@ __device__ float exp2_c(float xx) {
     p = fmaf(p, x, 1.0f);
     return ldexpf(p, static_cast<int>(i0));
 }
+// atan2 and acos by Cephes atanf / asinf (S. L. Moshier): the argument reduced to [0, tan(pi/8)] or [0, 1/2] and a
+// minimax polynomial there, in fmaf; the oracle has the same steps.
+__device__ float atan_red(float x) {  // x >= 0
+    float y = 0.0f;
+    if (x > 2.414213562373095f) {
+        y = 1.5707963267948966f;
+        x = -(1.0f / x);
+    } else if (x > 0.4142135623730950f) {
+        y = 0.7853981633974483f;
+        x = (x - 1.0f) / (x + 1.0f);
+    }
+    const float z = x * x;
+    float p = fmaf(8.05374449538e-2f, z, -1.38776856032e-1f);
+    p = fmaf(p, z, 1.99777106478e-1f);
+    p = fmaf(p, z, -3.33329491539e-1f);
+    return y + fmaf(p * z, x, x);
+}
+__device__ float atan2_c(float y, float x) {
+    if (x != x || y != y) return x + y;
+    if (x == 0.0f) return y > 0.0f ? 1.5707963267948966f : (y < 0.0f ? -1.5707963267948966f : 0.0f);
+    float w = 0.0f;
+    if (x < 0.0f) w = y < 0.0f ? -3.141592653589793f : 3.141592653589793f;
+    const float q = y / x;
+    const float a = q < 0.0f ? -atan_red(-q) : atan_red(q);
+    return w + a;
+}
+__device__ float asin_red(float a) {  // a in [0, 1]
+    float x, z;
+    bool flag = false;
+    if (a > 0.5f) {
+        z = 0.5f * (1.0f - a);
+        x = sqrtf(z);
+        flag = true;
+    } else {
+        x = a;
+        z = x * x;
+    }
+    float r = x;
+    if (!(a < 1e-4f)) {
+        float p = fmaf(4.2163199048e-2f, z, 2.4181311049e-2f);
+        p = fmaf(p, z, 4.5470025998e-2f);
+        p = fmaf(p, z, 7.4953002686e-2f);
+        p = fmaf(p, z, 1.6666752422e-1f);
+        r = fmaf(p * z, x, x);
+    }
+    if (flag) r = 1.5707963267948966f - (r + r);
+    return r;
+}
+__device__ float acos_c(float x) {
+    if (!(x >= -1.0f && x <= 1.0f)) return __builtin_nanf("");
+    if (x < -0.5f) return 3.141592653589793f - 2.0f * asin_red(sqrtf(0.5f * (1.0f + x)));
+    if (x > 0.5f) return 2.0f * asin_red(sqrtf(0.5f * (1.0f - x)));
+    return 1.5707963267948966f - (x < 0.0f ? -asin_red(-x) : asin_red(x));
+}
+// WGSL's float to u32: saturating, NaN to 0
+__device__ __forceinline__ uint32_t u32_sat(float x) {
+    return x > 0.0f ? (x < 4294967040.0f ? static_cast<uint32_t>(x) : 0xffffff00u) : 0u;
+}
 
 // ---- the kernel's context ---------------------------------------------------------------------------------------
 struct Ctx {
@@ -383,6 +441,9 @@ template <bool kExt>
 __device__ __forceinline__ bool glass_on(const Ctx &c) { return kExt && c.a->glass_energy != nullptr; }
 template <bool kExt>
 __device__ __forceinline__ bool tex_on(const Ctx &c) { return kExt && c.a->tex_info != nullptr; }
+// An environment map: P.env.x, which the launch allows only for a scene that holds a map of P.env.yz's size.
+template <bool kExt>
+__device__ __forceinline__ bool env_on(const Ctx &c) { return kExt && c.pf(ENV, 0) >= 0.5f; }
 
 struct Surface {
     V p, n, ng, base, emissive;
@@ -893,24 +954,107 @@ __device__ __forceinline__ bool occluded(const Ctx &c, V o, V d, float tmax, uin
     return c.trace<kCount>(o, d, tmax, mode).tri != kNone;
 }
 __device__ __forceinline__ V sky(const Ctx &c) { return v3(c.pf(SKY, 0), c.pf(SKY, 1), c.pf(SKY, 2)); }
-__device__ __forceinline__ float env_pdf() { return 1.0f / kFourPi; }
+// ---- the environment (pathtrace.wgsl:1889-1941): a constant sky, or (kExt) an equirectangular map with its CDF --------
+__device__ __forceinline__ float half_f(uint32_t h) {  // IEEE binary16 to binary32, exactly
+    const uint32_t s = (h & 0x8000u) << 16, e = (h >> 10) & 0x1fu, m = h & 0x3ffu;
+    if (e == 0x1fu) return __uint_as_float(s | 0x7f800000u | (m << 13));
+    if (e == 0u) return __uint_as_float(__float_as_uint(static_cast<float>(m) * 5.9604644775390625e-8f) | s);
+    return __uint_as_float(s | ((e + 112u) << 23) | (m << 13));
+}
+// env_uv (1889-1893)
+__device__ __forceinline__ void env_uv(V d, float &u, float &v) {
+    u = atan2_c(d.x, -d.z) / kTwoPi + 0.5f;
+    v = acos_c(clampf(d.y, -1.0f, 1.0f)) / kPi;
+}
+__device__ __forceinline__ float env_cdf_at(const Ctx &c, uint64_t i) { return c.a->env_cdf[c.ck(i, c.a->env_cdf_count, 35)]; }
+// env_radiance (1895-1898): the map read at level 0 as the reference's sampler defines it (repeat in u and v,
+// linear), with float weights
+template <bool kExt>
+__device__ V env_radiance(const Ctx &c, V d) {
+    if (!env_on<kExt>(c)) return sky(c);
+    float u, v;
+    env_uv(d, u, v);
+    const uint32_t w = c.a->env_w, h = c.a->env_h;
+    const float x = u * static_cast<float>(w) - 0.5f, y = v * static_cast<float>(h) - 0.5f;
+    float x0f = floorf(x), y0f = floorf(y);
+    const float fx = x - x0f, fy = y - y0f;
+    if (!(x0f >= -1048576.0f && x0f <= 1048576.0f)) x0f = 0.0f;
+    if (!(y0f >= -1048576.0f && y0f <= 1048576.0f)) y0f = 0.0f;
+    const int32_t xi = static_cast<int32_t>(x0f), yi = static_cast<int32_t>(y0f);
+    const int32_t wi = static_cast<int32_t>(w), hi = static_cast<int32_t>(h);
+    const uint32_t xa = static_cast<uint32_t>(((xi % wi) + wi) % wi), ya = static_cast<uint32_t>(((yi % hi) + hi) % hi);
+    const uint32_t xb = xa + 1u == w ? 0u : xa + 1u, yb = ya + 1u == h ? 0u : ya + 1u;
+    const uint64_t n = uint64_t(w) * h;
+    const uint2 t00 = c.a->env[c.ck(uint64_t(ya) * w + xa, n, 34)], t10 = c.a->env[c.ck(uint64_t(ya) * w + xb, n, 34)];
+    const uint2 t01 = c.a->env[c.ck(uint64_t(yb) * w + xa, n, 34)], t11 = c.a->env[c.ck(uint64_t(yb) * w + xb, n, 34)];
+    const float r = mixf(mixf(half_f(t00.x & 0xffffu), half_f(t10.x & 0xffffu), fx),
+                         mixf(half_f(t01.x & 0xffffu), half_f(t11.x & 0xffffu), fx), fy);
+    const float g = mixf(mixf(half_f(t00.x >> 16), half_f(t10.x >> 16), fx),
+                         mixf(half_f(t01.x >> 16), half_f(t11.x >> 16), fx), fy);
+    const float b = mixf(mixf(half_f(t00.y & 0xffffu), half_f(t10.y & 0xffffu), fx),
+                         mixf(half_f(t01.y & 0xffffu), half_f(t11.y & 0xffffu), fx), fy);
+    return v3(r, g, b);
+}
+// env_pdf (1900-1913): the rows' and the row's CDF steps of the texel d falls in, per solid angle
+template <bool kExt>
+__device__ float env_pdf(const Ctx &c, V d) {
+    if (!env_on<kExt>(c)) return 1.0f / kFourPi;
+    const uint32_t w = c.a->env_w, h = c.a->env_h;
+    float u, v;
+    env_uv(d, u, v);
+    const uint32_t x = min(u32_sat(u * static_cast<float>(w)), w - 1u), y = min(u32_sat(v * static_cast<float>(h)), h - 1u);
+    const float row_p = env_cdf_at(c, y) - (y > 0u ? env_cdf_at(c, y - 1u) : 0.0f);
+    const uint64_t base = uint64_t(h) + uint64_t(y) * w;
+    const float col_p = env_cdf_at(c, base + x) - (x > 0u ? env_cdf_at(c, base + x - 1u) : 0.0f);
+    float sin_t, cos_t;
+    sincos_c((static_cast<float>(y) + 0.5f) / static_cast<float>(h) * kPi, sin_t, cos_t);
+    return row_p * col_p * static_cast<float>(w) * static_cast<float>(h) / (2.0f * kPi * kPi * fmaxf(sin_t, 1e-4f));
+}
+// lower_bound (1915-1923)
+__device__ uint32_t env_lower_bound(const Ctx &c, uint64_t start, uint32_t count, float x) {
+    uint32_t lo = 0u, hi = count;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) / 2u;
+        if (env_cdf_at(c, start + mid) < x)
+            lo = mid + 1u;
+        else
+            hi = mid;
+    }
+    return min(lo, count - 1u);
+}
 __device__ __forceinline__ bool env_sampled(const Ctx &c) {
     return c.pu(COUNTS2, 1) != 0u && (c.pf(ENV, 0) > 0.5f || luminance(sky(c)) > 0.0f);
 }
+// sample_env (1925-1941): the uniform sphere for a constant sky; the map's row by its CDF, the texel in it by the
+// row's, a uniform point of the texel in (u, v)
+template <bool kExt>
 __device__ V sample_env(Ctx &c, uint32_t rd) {
     float u[4];
     c.g.r4(rd, u);
-    const float z = 1.0f - 2.0f * u[0];
-    const float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
-    const float phi = kTwoPi * u[1];
-    float sp, cp;
+    if (!env_on<kExt>(c)) {
+        const float z = 1.0f - 2.0f * u[0];
+        const float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+        const float phi = kTwoPi * u[1];
+        float sp, cp;
+        sincos_c(phi, sp, cp);
+        return v3(r * cp, z, r * sp);
+    }
+    const uint32_t w = c.a->env_w, h = c.a->env_h;
+    const uint32_t y = env_lower_bound(c, 0u, h, u[0]);
+    const uint32_t x = env_lower_bound(c, uint64_t(h) + uint64_t(y) * w, w, u[1]);
+    const float uu = (static_cast<float>(x) + u[2]) / static_cast<float>(w);
+    const float vv = (static_cast<float>(y) + u[3]) / static_cast<float>(h);
+    const float theta = vv * kPi, phi = (uu - 0.5f) * 2.0f * kPi;
+    float st, ct, sp, cp;
+    sincos_c(theta, st, ct);
     sincos_c(phi, sp, cp);
-    return v3(r * cp, z, r * sp);
+    return v3(st * sp, ct, -st * cp);
 }
+template <bool kExt>
 __device__ V sky_mis(const Ctx &c, V d, float pdf) {
     float wgt = 1.0f;
-    if (pdf > 0.0f && c.pu(COUNTS2, 1) != 0u) wgt = power_heuristic(pdf, env_pdf());
-    V col = vs(sky(c), wgt);
+    if (pdf > 0.0f && c.pu(COUNTS2, 1) != 0u) wgt = power_heuristic(pdf, env_pdf<kExt>(c, d));
+    V col = vs(env_radiance<kExt>(c, d), wgt);
     if (pdf > 0.0f) {
         V sum = v3(0.0f, 0.0f, 0.0f);
         for (uint32_t i = 0; i < c.pu(COUNTS2, 0); i++) {
@@ -1538,23 +1682,24 @@ __device__ V direct_light(Ctx &c, const Surface &s, const Lobes &lb, V v, WfShad
         }
     }
     if (env_sampled(c)) {
-        const V l = sample_env(c, c.g.dim(D_ENV));
+        const V l = sample_env<kExt>(c, c.g.dim(D_ENV));
         const V f = eval_bsdf<kExt>(s, lb, v, l);
         if (kDefer && any_pos(f)) {
-            const float pdf_e = env_pdf();
+            const float pdf_e = env_pdf<kExt>(c, l);
             if (pdf_e > 0.0f)
                 q->put(vds(vs(vmul(sky(c), f), light_mis(pdf_e, s, lb, v, l)), pdf_e), true, o, l, kInf, kTraceShadow);
         } else if (glass && any_pos(f)) {
-            const float pdf_e = env_pdf();
+            const float pdf_e = env_pdf<kExt>(c, l);
             if (pdf_e > 0.0f) {
                 const V vis = shadow_tr<kCount, kExt>(c, light_origin(s, o, l, glass), l, kInf, true);
                 if (any_pos(vis))
-                    sum = vadd(sum, vmul(vds(vs(vmul(sky(c), f), light_mis<kExt>(pdf_e, s, lb, v, l)), pdf_e), vis));
+                    sum = vadd(sum, vmul(vds(vs(vmul(env_radiance<kExt>(c, l), f), light_mis<kExt>(pdf_e, s, lb, v, l)),
+                                             pdf_e), vis));
             }
         } else if (any_pos(f)) {
-            const float pdf_e = env_pdf();
+            const float pdf_e = env_pdf<kExt>(c, l);
             if (pdf_e > 0.0f && !occluded<kCount>(c, o, l, kInf, kTraceShadow))
-                sum = vadd(sum, vds(vs(vmul(sky(c), f), light_mis(pdf_e, s, lb, v, l)), pdf_e));
+                sum = vadd(sum, vds(vs(vmul(env_radiance<kExt>(c, l), f), light_mis(pdf_e, s, lb, v, l)), pdf_e));
         }
     }
     return sum;
@@ -1634,7 +1779,7 @@ __device__ void path(Ctx &c, uint32_t px, uint32_t py, uint32_t index, V &radian
         c.g.depth = depth;
         const HitRec hit = c.trace<kCount>(o, d, kInf, kTraceClosest);
         if (hit.tri == kNone) {
-            V col = vmul(thr, sky_mis(c, d, (depth > 0u && !(kExt && bent)) ? last_pdf : 0.0f));
+            V col = vmul(thr, sky_mis<kExt>(c, d, (depth > 0u && !(kExt && bent)) ? last_pdf : 0.0f));
             if (depth > 1u && !straight) col = clamp_lum(col, clamp_v);
             radiance = vadd(radiance, col);
             break;
@@ -1978,7 +2123,7 @@ __device__ uint32_t wf_bounce(Ctx &c, const WfArgs &a, uint32_t p, const WfPath 
         hit.part = hw[5];
         const uint32_t last = c.pu(COUNTS, 2);
         if (hit.tri == kNone) {
-            V col = vmul(thr, sky_mis(c, d, depth > 0u ? last_pdf : 0.0f));
+            V col = vmul(thr, sky_mis<false>(c, d, depth > 0u ? last_pdf : 0.0f));
             if (depth > 1u && !straight) col = clamp_lum(col, clamp_v);
             radiance = vadd(radiance, col);
         } else {

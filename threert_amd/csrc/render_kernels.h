@@ -34,6 +34,10 @@ struct RenderArgs {
     const float4 *tex_info;     // texpack.rs Atlas::info, 2 x float4 a texture
     uint64_t tex_info_count;    // its float4s
     uint32_t tex_side, tex_layers;
+    const uint2 *env;           // gpu.rs env_texture: RGBA half floats, env_w x env_h (a scene with an environment map)
+    const float *env_cdf;       // lib.rs build_env_cdf: env_h + env_w x env_h floats
+    uint64_t env_cdf_count;
+    uint32_t env_w, env_h;
     float4 *accum;              // records.wgsl Pixel, 5 x float4 a pixel
     // the tables' sizes (in their records), against which every index is checked
     uint32_t vertex_count, tri_count, row_count, material_count, light_count, emitter_vec4s, emitter_tri_count,

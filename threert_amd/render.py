@@ -416,13 +416,24 @@ class RenderScene:
     tex_side: int = 0
     tex_layers: int = 0
     tex_info: np.ndarray | None = None
+    # an environment map: RGBA float texels (h, w, 4), row 0 the top (+y), and lib.rs build_env_cdf's h + w h floats;
+    # None: a constant sky (the Params' colour)
+    env: np.ndarray | None = None
+    env_cdf: np.ndarray | None = None
+
+    def env_half(self) -> np.ndarray | None:
+        """gpu.rs:3552-3561 write_env_texels: the texels the kernel reads, half floats of min(v, 65000)."""
+        if self.env is None:
+            return None
+        return np.fmin(self.env, np.float32(65000.0)).astype(np.float16).view(np.uint16)
 
     def tables(self) -> dict:
         return {"vertices": self.vertices, "indices": self.indices, "rows": self.rows, "materials": self.materials,
                 "lights": self.lights, "emitters": self.emitters, "emitter_tris": self.emitter_tris,
                 "objects": self.objects, "energy": self.energy, "light_tree": self.light_tree,
                 "glass_energy": self.glass_energy, "tex_pages": self.tex_pages, "tex_side": self.tex_side,
-                "tex_layers": self.tex_layers, "tex_info": self.tex_info}
+                "tex_layers": self.tex_layers, "tex_info": self.tex_info, "env": self.env_half(),
+                "env_cdf": self.env_cdf}
 
     def params(self, width: int, height: int, camera: Camera, seed: int = 1, max_bounces: int = 4, rr_depth: int = 3,
                clamp: float = 1e9, energy_compensation: float = 0.0, sky=(0.0, 0.0, 0.0), env_nee: bool = True,
@@ -452,6 +463,10 @@ class RenderScene:
         f[28:32] = (clamp, self.total_weight, energy_compensation, 0.0)
         f[32:35] = sky
         env_on = 1 if (env_nee and (0.2126 * sky[0] + 0.7152 * sky[1] + 0.0722 * sky[2]) > 0) else 0
+        if self.env is not None:  # lib.rs:1004-1009 SkyBaked::write: a map, its size; sampled unless env_nee is off
+            f[24:27] = (1.0, self.env.shape[1], self.env.shape[0])
+            f[32:35] = 0.0
+            env_on = 1 if env_nee else 0
         # lib.rs:725 counts2: the directional lights, the sky's sampling, -, the first shadowed local light
         first_shadowed = next((k for k in range(len(self.lights)) if self.lights[k, 3] != 0.0 and
                                self.lights[k, 15] >= 0.5), len(self.lights))
@@ -467,8 +482,26 @@ class RenderScene:
         return p
 
 
+def build_env_cdf(texels: np.ndarray) -> np.ndarray:
+    """lib.rs:916-942 build_env_cdf, in f32 as the reference sums: per row the running sum of luminance (clamped at
+    0) x sin(theta at the row's centre) + 1e-12, divided by the row's total; then the rows' totals' running share.
+    Returns the h row entries followed by the h x w conditional ones."""
+    t = np.asarray(texels, np.float32)
+    h, w = t.shape[:2]
+    f32 = np.float32
+    sin_t = np.sin((np.arange(h, dtype=f32) + f32(0.5)) / f32(h) * f32(math.pi)).astype(f32)
+    lum = f32(0.2126) * t[..., 0] + f32(0.7152) * t[..., 1] + f32(0.0722) * t[..., 2]
+    term = (np.maximum(lum, f32(0.0)) * sin_t[:, None] + f32(1e-12)).astype(f32)
+    cond = np.cumsum(term, axis=1, dtype=f32)
+    rows = cond[:, -1].copy()
+    cond = (cond / rows[:, None]).astype(f32)
+    total = np.cumsum(rows, dtype=f32)[-1]
+    marg = np.cumsum((rows / total).astype(f32), dtype=f32)
+    return np.concatenate([marg, cond.reshape(-1)]).astype(f32)
+
+
 def build(meshes: list[RMesh], instances: list[RInstance], materials: list[np.ndarray],
-          lights: list[np.ndarray] = (), light_tree: bool = True, textures: list = ()) -> RenderScene:
+          lights: list[np.ndarray] = (), light_tree: bool = True, textures: list = (), env=None) -> RenderScene:
     """Lays out a scene as prepare does: trees and records (scene.prepare), then vertices, indices, rows, emitters.
     With 256 emitters or more that weigh something, a light tree over them (lib.rs:650-676, on unless THREERT_LTREE=0:
     here `light_tree=False`), whose emitters come first in the CDF in its leaf order. Glass (a material with
@@ -614,6 +647,13 @@ def build(meshes: list[RMesh], instances: list[RInstance], materials: list[np.nd
         rs.glass_energy = glass_energy_lut()
     if len(textures):
         rs.tex_pages, rs.tex_side, rs.tex_layers, rs.tex_info = pack_textures(list(textures))
+    if env is not None:  # an equirectangular map (h, w, 3 or 4): lib.rs SkyBaked's EnvMap and its CDF
+        e = np.asarray(env, np.float32)
+        if e.ndim != 3 or e.shape[2] not in (3, 4) or min(e.shape[:2]) < 1:
+            raise ValueError("env must be an (h, w, 3 or 4) array of texels")
+        rs.env = np.concatenate([e[..., :3], np.ones(e.shape[:2] + (1,), np.float32)], axis=2) if e.shape[2] == 3 \
+            else np.ascontiguousarray(e)
+        rs.env_cdf = build_env_cdf(rs.env)
     return rs
 
 
@@ -642,6 +682,12 @@ def upload(dev: Device, rs: RenderScene) -> Scene:
         keep += [pages, info]
         desc.tex_pages, desc.tex_side, desc.tex_layers = _ptr(pages), rs.tex_side, rs.tex_layers
         desc.tex_info, desc.tex_info_vec4s = _ptr(info), len(info)
+    if rs.env is not None:
+        env = np.ascontiguousarray(rs.env, np.float32)
+        cdf = np.ascontiguousarray(rs.env_cdf, np.float32)
+        keep += [env, cdf]
+        desc.env, desc.env_width, desc.env_height = _ptr(env), env.shape[1], env.shape[0]
+        desc.env_cdf, desc.env_cdf_count = _ptr(cdf), len(cdf)
     sc = Scene.__new__(Scene)
     sc.dev = dev
     h = C.c_void_p()
