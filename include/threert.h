@@ -292,7 +292,8 @@ int trt_accum_add_fp64(trt_accum *acc, double *d_sum);
 /* Samples first_sample .. first_sample + spp - 1 of every pixel added to `acc` (gpu.rs:4177-4196: the sample index
  * is P.size.z), on the device's stream. params: the 624-byte Params block (pathtrace.wgsl:96-140); its size.xy
  * must be the accumulator's. Needs a scene uploaded with its shading tables. Refuses (TRT_E_INVALID, named) what the
- * offline `main` would read and this core does not render: an environment map, fog, volumetric scattering,
+ * offline `main` would read and this core does not render: an environment map the scene has not (P.env.x without
+ * the scene description's env, or P.env.yz another size), fog, volumetric scattering,
  * procedural geometry, the live light LOD, aerial perspective, a scene background, more than 64 bounces, and
  * light-tree candidates (P.ltree.w) for a scene uploaded without a tree; and light counts the scene does not hold:
  * P.counts.x past its lights, P.counts2.x other than its directional lights when local lights follow, P.counts2.w

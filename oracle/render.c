@@ -2,7 +2,8 @@
  * Oracle (test infrastructure only): the offline path loop, pathtrace.wgsl `main` (LIVE = false) restated for the
  * features the workloads use: pinhole camera, instanced triangle meshes with per-section materials (Lambert + GGX
  * metal-rough, emission, clearcoat, glass, textures), directional lights (soft cone), point, spot and rect lights,
- * emissive triangles from the CDF or the light tree (RIS at camera hits), a constant sky with light sampling, MIS,
+ * emissive triangles from the CDF or the light tree (RIS at camera hits), a constant sky or an environment map with
+ * light sampling, MIS,
  * Russian roulette, the indirect clamp, FP32 accumulation. Not restated: media, fog, procedural geometry, cutouts,
  * live-only paths.
  *
@@ -15,7 +16,7 @@
  *   pathtrace.wgsl:1492-1505 energy_lookup    1521-1525 spec_terms   1642-1699 lobes
  *   pathtrace.wgsl:1707-1730 d_ggx / vis_smith_joint_approx / f_schlick   1733-1764 eval_bsdf_k   1766-1789 g1,
  *   pdf_bsdf   1797-1817 light_mis   1819-1844 basis / sample_vndf   1848-1884 sample_bsdf
- *   pathtrace.wgsl:1895-1930 env_radiance / env_pdf / sample_env (constant sky)   1943-1960 power_heuristic,
+ *   pathtrace.wgsl:1889-1941 env_uv / env_radiance / env_pdf / lower_bound / sample_env   1943-1960 power_heuristic,
  *   luminance, offset_origin   1963-1988 sky_mis / sun_disks   2009-2024 emit_mis
  *   pathtrace.wgsl:2269-2393 emitter_radiance / sample_emitter_ris / sample_emitter   2431-2442 env_light
  *   pathtrace.wgsl:2042-2104 local_light    2158-2262 local_lights (offline: no light tiles, no volumes)

@@ -5,7 +5,8 @@
 // tri_data 330-360, inst_vec / inst_sign / inst_normal 279-288, inst_attr 406-413, material_of 310-313; lobes
 // 1642-1699 with energy_lookup 1492-1505, spec_terms 1521-1525; d_ggx / vis_smith_joint_approx / f_schlick
 // 1707-1730; eval_bsdf_k 1733-1764; g1 / pdf_bsdf 1766-1789; light_mis 1797-1817; basis / sample_vndf 1819-1844;
-// sample_bsdf 1848-1884; env_radiance / env_pdf / sample_env 1895-1930 (constant sky); power_heuristic, luminance,
+// sample_bsdf 1848-1884; env_uv / env_radiance / env_pdf / lower_bound / sample_env 1889-1941 (a constant sky; a map
+// with its CDF: kExt); power_heuristic, luminance,
 // offset_origin 1943-1960; sky_mis / sun_disks 1963-1988; emit_mis 2009-2024; emitter_radiance /
 // sample_emitter_ris / sample_emitter 2269-2393; env_light 2431-2442; direct_light 2444-2540; emitter_draw /
 // emitter_pick_u / emitter_pick_range / lt_imp / emitter_pmf 463-618 (the light tree); the clearcoat; glass (kExt:
