@@ -191,7 +191,7 @@ static inline float pf(const uint32_t *p, int vec, int k) {
 }
 static inline uint32_t pu(const uint32_t *p, int vec, int k) { return p[4 * vec + k]; }
 enum { CAM_POS = 0, CAM_FWD = 1, CAM_RIGHT = 2, CAM_UP = 3, SIZE = 4, COUNTS = 5, ENV = 6, MISC = 7, SKY = 8,
-       COUNTS2 = 9, LIGHT_LOD = 18, LTREE = 21 };
+       COUNTS2 = 9, LIGHT_LOD = 18, LTREE = 21, BACKGROUND = 24 };
 
 enum { D_CAMERA = 0, D_BSDF = 1, D_RR = 2, D_SUN = 4, D_SUN_VIS = 5, D_LOCAL = 6, D_EMIT = 7, D_EMIT_VIS = 8, D_ENV = 9,
        D_ENV_VIS = 10 };
@@ -200,6 +200,7 @@ enum { D_CAMERA = 0, D_BSDF = 1, D_RR = 2, D_SUN = 4, D_SUN_VIS = 5, D_LOCAL = 6
 #define NO_EMITTER 0xffffffffu
 #define INST_MIRRORED 4u
 #define MAT_EMITTER_CARD 2
+#define MAT_UNLIT 4
 #define EMITTER_CARD 1u
 #define MIRROR_ROUGHNESS 0.3f
 #define OBJECT_STRIDE 3u
@@ -506,6 +507,14 @@ static Surface surface(Ctx *c, V o, V d, const or_hit *h, float width) {
         }
     }
     s.emissive = vs(emission(c, m, uv, lod), obj_power(c, object_of(&a)));
+    /* 1366-1372: three.js MeshBasicMaterial (tex1.z MAT_UNLIT): its colour to camera rays and mirror-like paths, lit
+     * by nothing, lighting nothing */
+    if (as_i32(m[18]) == MAT_UNLIT) {
+        s.emissive = (c->g.depth == 0u || c->mirror_path) ? base : v3(0, 0, 0);
+        base = v3(0, 0, 0);
+        spec = 0.0f;
+        metal = 0.0f;
+    }
     /* 1376-1387: the normal map where the surface is seen directly or in a mirror-like reflection */
     const int near = c->g.depth == 0u || c->mirror_path;
     if (tex_on(c) && as_i32(m[13]) >= 0 && near) {
@@ -1149,7 +1158,11 @@ static V sun_disks(const Ctx *c, V d, float pdf) {
     }
     return sum;
 }
+/* sky_mis (1963-1974); a camera ray (or a path bent straight through glass) that leaves the scene sees three.js's
+ * scene.background where P.background.w says there is one (the reference's F_BACKGROUND) */
 static V sky_mis(const Ctx *c, V d, float pdf) {
+    if (pdf <= 0.0f && pf(c->P, BACKGROUND, 3) > 0.5f)
+        return v3(pf(c->P, BACKGROUND, 0), pf(c->P, BACKGROUND, 1), pf(c->P, BACKGROUND, 2));
     float wgt = 1.0f;
     if (pdf > 0.0f && pu(c->P, COUNTS2, 1) != 0u) wgt = power_heuristic(pdf, env_pdf(c, d));
     V col = vs(env_radiance(c, d), wgt);

@@ -564,10 +564,10 @@ def test_the_serving_job_that_met_a_tie_matches_the_oracle(device):
 @pytest.mark.gpu
 def test_what_the_library_does_not_render_is_refused(device):
     """Records with cutout sections or with glass among shadow-casting sections (scene.rs INST_CUTOUT 16, INST_GLASS 32
-    in the bits word; the Cornell box's records cast shadows); materials with textures, normal maps, unlit or function
+    in the bits word; the Cornell box's records cast shadows); materials with textures, normal maps, function
     flags, transmission textures, vertex colours, alpha cutoffs, or glass without the glass energy table;
     and Params asking for an environment map, more than 64 bounces, more lights than the scene has, fog, volumetric
-    scattering, the live light LOD, procedural geometry, aerial perspective, a background or light-tree candidates
+    scattering, the live light LOD, procedural geometry, aerial perspective or light-tree candidates
     without a tree: each refused with TRT_E_INVALID and a message that names it. The device still renders the unchanged scene afterwards."""
     import copy
 
@@ -585,7 +585,7 @@ def test_what_the_library_does_not_render_is_refused(device):
     # materials that need what the kernel does not read (GpuMaterial words, scene.rs:480-491)
     as_int = lambda v: np.array([v], np.int32).view(np.float32)[0]  # noqa: E731
     for word, value, name in ((12, as_int(0), "base-colour texture"), (13, as_int(2), "normal map"),
-                              (16, as_int(1), "emissive texture"), (18, as_int(4), "unlit"),
+                              (16, as_int(1), "emissive texture"),
                               (19, as_int(1), "material function"), (22, 1.0, "transmission map"),
                               (23, 1.0, "vertex colours"), (27, 0.5, "alpha cutoff"),
                               (28, 1.0, "no glass energy table")):
@@ -602,7 +602,7 @@ def test_what_the_library_does_not_render_is_refused(device):
                          ("P.counts names more lights", lambda p: p.__setitem__(20, 99)), ("fog", f32(44, 0.01)),
                          ("volumetric scattering", f32(52, 0.5)), ("live light LOD", f32(75, 10.0)),
                          ("procedural geometry", lambda p: p.__setitem__(89, 1)),
-                         ("aerial perspective", f32(92, 1.0)), ("scene background", f32(99, 1.0)),
+                         ("aerial perspective", f32(92, 1.0)),
                          ("light-tree candidates", f32(87, 6.0))):
         print(f"refusal: {what}", flush=True)
         p = rs.params(16, 16, cam)

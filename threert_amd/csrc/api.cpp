@@ -97,6 +97,7 @@ struct trt_scene {
     bool shading = false;
     bool coat = false;   // a material has a clearcoat (layer.z > 0: the reference's F_COAT, gpu.rs features_of)
     bool glass = false;  // a material has glass (layer.x > 0: the reference's F_GLASS, gpu.rs:2030-2032)
+    bool unlit = false;  // a material is unlit (tex1.z MAT_UNLIT: the reference's F_UNLIT, gpu.rs:2037)
     uint64_t light_count = 0, emitter_entries = 0, emitter_tri_count = 0, n_dir = 0;  // n_dir: directional lights
     uint64_t vertex_count = 0, tri_count = 0, row_count = 0, material_count = 0, object_count = 0, emitter_vec4s = 0;
     uint64_t node_count = 0, leaf_count = 0, instance_count = 0, wnode_count = 0, wunit_count = 0;
@@ -407,7 +408,6 @@ int trt_scene_release(trt_scene *s) {
 // 16-19 (texture slots, -1 none; tex1.z the flags, tex1.w a function + 1), uv.z 22 (transmission texture + 1), uv.w
 // 23 (vertex colours), uv2.w 27 (alpha cutoff), layer.x 28 (transmission).
 static bool material_rendered(const float *m, uint64_t count, bool glass_table, uint64_t textures, std::string *why) {
-    constexpr int32_t kMatUnlit = 4;  // pathtrace.wgsl:54 MAT_UNLIT
     static const char *slot[5] = {"base-colour texture", "normal map", "roughness texture", "metalness texture",
                                   "emissive texture"};
     for (uint64_t k = 0; k < count; k++) {
@@ -424,8 +424,7 @@ static bool material_rendered(const float *m, uint64_t count, bool glass_table, 
             return false;
         }
         const char *what = nullptr;
-        if (w[6] == kMatUnlit) what = "an unlit material";
-        if (!what && w[7] > 0) what = "a material function";
+        if (w[7] > 0) what = "a material function";
         if (!what && f[23] > 0.0f) what = "vertex colours";
         if (!what && f[27] > 0.0f) what = "an alpha cutoff";
         if (what) {
@@ -825,6 +824,9 @@ int trt_scene_upload(trt_device *d, const trt_scene_desc *desc, trt_scene **out)
         for (uint64_t k = 0; k < desc->material_count; k++) {
             if (static_cast<const float *>(desc->materials)[72 * k + 30] > 0.0f) s->coat = true;
             if (static_cast<const float *>(desc->materials)[72 * k + 28] > 0.0f) s->glass = true;
+            int32_t flag;  // tex1.z
+            std::memcpy(&flag, static_cast<const float *>(desc->materials) + 72 * k + 18, 4);
+            if (flag == 4) s->unlit = true;  // pathtrace.wgsl:54 MAT_UNLIT
         }
         s->light_count = desc->light_count;
         s->n_dir = 0;
@@ -1177,8 +1179,6 @@ static int job_args(const trt_render_job &j, trt::RenderArgs &a, std::string *wh
     if (P[89] > 0) return *why = "procedural geometry (P.procs.y) is not supported", TRT_E_INVALID;
     if (trt::from_bits(P[92]) > 0.0f)
         return *why = "aerial perspective (P.aerial.x) is not supported", TRT_E_INVALID;
-    if (trt::from_bits(P[99]) > 0.5f)
-        return *why = "a scene background (P.background.w) is not supported", TRT_E_INVALID;
     if (trt::from_bits(P[87]) > 0.5f && !s->light_tree_count)
         return *why = "light-tree candidates (P.ltree.w) for a scene uploaded without a light tree", TRT_E_INVALID;
     if (trt::from_bits(P[87]) > 0.5f && (P[85] > s->emitter_entries || trt::from_bits(P[87]) > 64.0f))
@@ -1402,7 +1402,7 @@ static int render_batch(trt_device *d, const trt_render_job *jobs, uint32_t n, u
     table.reserve(n);
     uint64_t tiles = 0;
     uint32_t deepest = 0;
-    bool coat = false, tree = false, glass = false, tex = false, local = false, env = false;
+    bool coat = false, tree = false, glass = false, tex = false, local = false, env = false, other = false;
     for (uint32_t k = 0; k < n; k++) {
         if (jobs[k].spp == 0) continue;
         if (!jobs[k].scene || jobs[k].scene->dev != d)
@@ -1422,16 +1422,19 @@ static int render_batch(trt_device *d, const trt_render_job *jobs, uint32_t n, u
         tex = tex || jobs[k].scene->tex;
         local = local || j.args.params[20] > j.args.params[36];  // P.counts.x > P.counts2.x: local lights
         env = env || trt::from_bits(j.args.params[24]) >= 0.5f;  // P.env.x: an environment map
+        // unlit materials (F_UNLIT) and a background (P.background.w: F_BACKGROUND)
+        other = other || jobs[k].scene->unlit || trt::from_bits(j.args.params[99]) > 0.5f;
         tree = tree || trt::from_bits(j.args.params[87]) > 0.5f;  // P.ltree.w: the job samples the light tree
         table.push_back(j);
     }
     if (tiles >= (uint64_t{1} << 31)) return fail(TRT_E_INVALID, "trt_render_batch: too many tiles in one launch");
     if (table.empty()) return ok();
     // The wavefront takes batches whose jobs have at most kWfMaxSuns directional lights, no glass, no textures, no
-    // local lights and no environment map; counting launches stay on the megakernel. TRT_RENDER_AUTO takes it when
+    // local lights, no environment map, no unlit material and no background; counting launches stay on the
+    // megakernel. TRT_RENDER_AUTO takes it when
     // every scene has kWfAutoLeaves triangles or more, or for launches of a whole chunk of paths or more.
     const uint32_t how = d->render.load();
-    bool wf = reads == nullptr && how != TRT_RENDER_MEGAKERNEL && !glass && !tex && !local && !env;
+    bool wf = reads == nullptr && how != TRT_RENDER_MEGAKERNEL && !glass && !tex && !local && !env && !other;
     uint64_t paths = 0;
     bool large = true;
     std::vector<const trt_scene *> scene_of;
@@ -1497,7 +1500,7 @@ static int render_batch(trt_device *d, const trt_render_job *jobs, uint32_t n, u
         if (rc) return rc;
     } else if (trt::launch_render(static_cast<const trt::RenderJob *>(ring.dev[k]), static_cast<uint32_t>(table.size()),
                            static_cast<uint32_t>(tiles), d->counter, d->spill, threads,
-                           static_cast<uint32_t>(entries), d->guard, d->ray_counts, coat, tree, glass || tex || local || env, reads, grid,
+                           static_cast<uint32_t>(entries), d->guard, d->ray_counts, coat, tree, glass || tex || local || env || other, reads, grid,
                            d->stream))
         return fail(TRT_E_DEVICE, "render kernel launch failed");
     TRT_HIP(hipEventRecord(ring.ev[k], d->stream));

@@ -174,6 +174,7 @@ __device__ __forceinline__ uint32_t rng_sub(uint32_t d, uint32_t k) {
 enum : uint32_t { D_CAMERA = 0, D_BSDF = 1, D_RR = 2, D_SUN = 4, D_LOCAL = 6, D_EMIT = 7, D_ENV = 9 };
 constexpr uint32_t kEmitRisM = 6, kGuide = 65536, kNoEmitter = 0xffffffffu, kInstMirrored = 4, kEmitterCard = 1;
 constexpr int kMatEmitterCard = 2;
+constexpr int kMatUnlit = 4;  // pathtrace.wgsl:54 MAT_UNLIT (three.js MeshBasicMaterial)
 constexpr float kMirrorRoughness = 0.3f;
 constexpr uint32_t kObjectStride = 3;
 
@@ -330,7 +331,7 @@ struct Ctx {
     }
 };
 enum { CAM_POS = 0, CAM_FWD = 1, CAM_RIGHT = 2, CAM_UP = 3, SIZE = 4, COUNTS = 5, ENV = 6, MISC = 7, SKY = 8,
-       COUNTS2 = 9, LIGHT_LOD = 18, LTREE = 21 };
+       COUNTS2 = 9, LIGHT_LOD = 18, LTREE = 21, BACKGROUND = 24 };
 
 struct TriAttr {
     float4 n0, n1, n2, uv, x;
@@ -574,7 +575,7 @@ __device__ Surface surface(const Ctx &c, V o, V d, const HitRec &h, float width 
     s.card = static_cast<int>(__float_as_uint(m[4].z)) == kMatEmitterCard;
     const float4 params = m[2];
     V base = xyz(m[0]);
-    float rough = params.y, metal = params.x;
+    float rough = params.y, metal = params.x, spec = params.z;
     if (tex_on<kExt>(c)) {  // 1360-1364: the colour (sRGB), roughness (green) and metalness (blue) maps
         const float4 t0 = m[3];
         if (as_i32(t0.x) >= 0) {
@@ -585,6 +586,14 @@ __device__ Surface surface(const Ctx &c, V o, V d, const HitRec &h, float width 
         if (as_i32(t0.w) >= 0) metal *= tex(c, as_i32(t0.w), uu, vv, lod).z;
     }
     s.emissive = vs(emission<kExt>(c, m, uu, vv, lod), c.obj_power(__float_as_uint(a.x.y)));
+    // 1366-1372: three.js MeshBasicMaterial (tex1.z MAT_UNLIT): its colour to camera rays and mirror-like paths, lit
+    // by nothing, lighting nothing
+    if (kExt && as_i32(m[4].z) == kMatUnlit) {
+        s.emissive = (c.g.depth == 0u || c.mirror_path) ? base : v3(0.0f, 0.0f, 0.0f);
+        base = v3(0.0f, 0.0f, 0.0f);
+        spec = 0.0f;
+        metal = 0.0f;
+    }
     // 1376-1387: the normal map where the surface is seen directly or in a mirror-like reflection
     if (tex_on<kExt>(c) && as_i32(m[3].y) >= 0 && (c.g.depth == 0u || c.mirror_path)) {
         const float4 tx = tex(c, as_i32(m[3].y), uu, vv, lod);
@@ -602,7 +611,7 @@ __device__ Surface surface(const Ctx &c, V o, V d, const HitRec &h, float width 
     s.base = base;
     s.roughness = clampf(rough, 0.02f, 1.0f);
     s.metallic = clampf(metal, 0.0f, 1.0f);
-    s.specular = params.z;
+    s.specular = spec;
     s.ng = ng;
     s.n = n;
     s.coat = 0.0f;
@@ -1051,8 +1060,12 @@ __device__ V sample_env(Ctx &c, uint32_t rd) {
     sincos_c(phi, sp, cp);
     return v3(st * sp, ct, -st * cp);
 }
+// sky_mis (1963-1974); with kExt, a camera ray (or a path bent straight through glass) that leaves the scene sees
+// three.js's scene.background where P.background.w says there is one (the reference's F_BACKGROUND)
 template <bool kExt>
 __device__ V sky_mis(const Ctx &c, V d, float pdf) {
+    if (kExt && pdf <= 0.0f && c.pf(BACKGROUND, 3) > 0.5f)
+        return v3(c.pf(BACKGROUND, 0), c.pf(BACKGROUND, 1), c.pf(BACKGROUND, 2));
     float wgt = 1.0f;
     if (pdf > 0.0f && c.pu(COUNTS2, 1) != 0u) wgt = power_heuristic(pdf, env_pdf<kExt>(c, d));
     V col = vs(env_radiance<kExt>(c, d), wgt);

@@ -37,18 +37,19 @@ ROW_GLASS = 1 << 25  # scene.rs:103: a row's class bit for glass (mesh_class 2)
 def material(base=(0.8, 0.8, 0.8), emissive=(0.0, 0.0, 0.0), metallic=0.0, roughness=1.0, specular=0.5,
              clearcoat=0.0, clearcoat_roughness=0.0, transmission=0.0, ior=1.5, base_map=-1, normal_map=-1,
              roughness_map=-1, metalness_map=-1, emissive_map=-1, transmission_map=-1, normal_scale=1.0,
-             repeat=(1.0, 1.0), offset=(0.0, 0.0)) -> np.ndarray:
+             repeat=(1.0, 1.0), offset=(0.0, 0.0), unlit=False) -> np.ndarray:
     """scene.rs:480-491 GpuMaterial (72 words; scene.rs:1238-1262 build_material): base colour, emission, (metallic,
     roughness, specular, normal scale); the texture slots (tex0: base colour (sRGB), normal, roughness (green),
     metalness (blue); tex1.x emissive (sRGB); uv.z the transmission map's slot + 1 (red); -1 none), the uv repeat and
     offset; the layer (scene.rs:1257): transmission (glass, clamped to [0, 1]), index of refraction, clearcoat and its
-    roughness (both clamped to [0, 1])."""
+    roughness (both clamped to [0, 1]); `unlit`: three.js MeshBasicMaterial (tex1.z MAT_UNLIT 4: its colour to camera
+    rays and mirror-like paths, lit by nothing, lighting nothing)."""
     m = np.zeros(72, np.float32)
     m[0:3] = base
     m[3] = 1.0
     m[4:7] = emissive
     m[8:12] = (metallic, roughness, specular, normal_scale)
-    m[12:20] = np.array([base_map, normal_map, roughness_map, metalness_map, emissive_map, -1, 0, 0],
+    m[12:20] = np.array([base_map, normal_map, roughness_map, metalness_map, emissive_map, -1, 4 if unlit else 0, 0],
                         np.int32).view(np.float32)
     m[20:24] = (repeat[0], repeat[1], float(transmission_map) + 1.0, 0.0)
     m[24:26] = offset
@@ -437,9 +438,11 @@ class RenderScene:
 
     def params(self, width: int, height: int, camera: Camera, seed: int = 1, max_bounces: int = 4, rr_depth: int = 3,
                clamp: float = 1e9, energy_compensation: float = 0.0, sky=(0.0, 0.0, 0.0), env_nee: bool = True,
-               sample: int = 0, lens_radius: float = 0.0, focus: float = 1.0) -> np.ndarray:
+               sample: int = 0, lens_radius: float = 0.0, focus: float = 1.0, background=None) -> np.ndarray:
         """The 624-byte Params block (gpu.rs:129-185, lib.rs:713-756 for a camera); lens_radius > 0 makes the camera a
-        thin lens focused at `focus` along the view direction (cam_right.w, cam_up.w; pathtrace.wgsl:4017-4023)."""
+        thin lens focused at `focus` along the view direction (cam_right.w, cam_up.w; pathtrace.wgsl:4017-4023);
+        `background`: the radiance camera rays see where nothing is (three.js scene.background, Params.background =
+        (rgb, 1), the reference's F_BACKGROUND)."""
         p = np.zeros(156, np.uint32)
         f = p.view(np.float32)
         pos = np.asarray(camera.position, np.float64)
@@ -471,6 +474,8 @@ class RenderScene:
         first_shadowed = next((k for k in range(len(self.lights)) if self.lights[k, 3] != 0.0 and
                                self.lights[k, 15] >= 0.5), len(self.lights))
         p[36:40] = (self.n_directional, env_on, 0, first_shadowed)
+        if background is not None:
+            f[96:100] = (*background, 1.0)
         f[68] = 0.0
         f[70] = np.uint32(self.prep.top_first).view(np.float32)
         f[71] = np.uint32(self.prep.units).view(np.float32)
