@@ -198,8 +198,8 @@ typedef struct {
  * is borrowed for the call only. Refuses (TRT_E_INVALID) trees the reference's checks refuse (units.rs check_mesh /
  * check_top, ltree.rs LightTree::check), records with cutout bits or with glass among shadow-casting sections (the
  * reference's F_IGLASS; a record whose sections are all glass casts no shadow and is taken), materials that need
- * what this core does not render (textures, normal maps, alpha cutoffs, vertex colours, unlit or function
- * materials), and glass without the glass energy table, each named in trt_last_error. */
+ * what this core does not render (a texture slot naming no texture of the pages, alpha cutoffs, vertex colours,
+ * material functions), and glass without the glass energy table, each named in trt_last_error. */
 int trt_scene_upload(trt_device *dev, const trt_scene_desc *desc, trt_scene **out);
 int trt_scene_release(trt_scene *scene);
 int trt_scene_stats_get(trt_scene *scene, trt_scene_stats *out);
@@ -294,7 +294,7 @@ int trt_accum_add_fp64(trt_accum *acc, double *d_sum);
  * must be the accumulator's. Needs a scene uploaded with its shading tables. Refuses (TRT_E_INVALID, named) what the
  * offline `main` would read and this core does not render: an environment map the scene has not (P.env.x without
  * the scene description's env, or P.env.yz another size), fog, volumetric scattering,
- * procedural geometry, the live light LOD, aerial perspective, a scene background, more than 64 bounces, and
+ * procedural geometry, the live light LOD, aerial perspective, more than 64 bounces, and
  * light-tree candidates (P.ltree.w) for a scene uploaded without a tree; and light counts the scene does not hold:
  * P.counts.x past its lights, P.counts2.x other than its directional lights when local lights follow, P.counts2.w
  * (the first shadow-casting local light, lib.rs:700) outside the local lights. */
