@@ -177,6 +177,11 @@ typedef struct {
     uint32_t env_height;
     const void *env_cdf;
     uint64_t env_cdf_count;
+    /* three.js vertexColors (lib.rs:601 gpu_vcolors, scene.rs:2324-2334 tri_colors): three RGBA8 sRGB words a
+     * triangle, one a corner, in the leaves' order (as `indices`); vcolor_words = 3 x index_count. Read for materials
+     * whose uv.w is set; NULL: none, and uv.w is ignored, as the reference does without F_VCOLOR */
+    const void *vcolors;
+    uint64_t vcolor_words;
 } trt_scene_desc;
 
 typedef struct {

@@ -278,7 +278,8 @@ class _RenderScene(C.Structure):
                 ("light_tree_count", C.c_uint64), ("glass_energy", C.c_void_p), ("tex_pages", C.c_void_p),
                 ("tex_side", C.c_uint32), ("tex_layers", C.c_uint32), ("tex_info", C.c_void_p),
                 ("tex_info_vec4s", C.c_uint64), ("env", C.c_void_p), ("env_w", C.c_uint32), ("env_h", C.c_uint32),
-                ("env_cdf", C.c_void_p), ("env_cdf_count", C.c_uint64)]
+                ("env_cdf", C.c_void_p), ("env_cdf_count", C.c_uint64), ("vcolors", C.c_void_p),
+                ("vcolor_words", C.c_uint64)]
 
 
 class RenderScene:
@@ -286,7 +287,7 @@ class RenderScene:
 
     def __init__(self, nodes, leaves, instances, top, units, vertices, indices, rows, materials, lights, emitters,
                  emitter_tris, objects, energy, light_tree=None, glass_energy=None, tex_pages=None, tex_side=0,
-                 tex_layers=0, tex_info=None, env=None, env_cdf=None):
+                 tex_layers=0, tex_info=None, env=None, env_cdf=None, vcolors=None):
         self.geo = Scene(nodes, leaves, instances, top, units)
         if light_tree is None:
             light_tree = np.zeros((0, 4), np.float32)
@@ -315,13 +316,15 @@ class RenderScene:
             if np.asarray(env_cdf).size != env.shape[0] * (1 + env.shape[1]):
                 raise ValueError("env_cdf must hold h + w h floats")
         self.keep += [np.ascontiguousarray(env, np.uint16), np.ascontiguousarray(env_cdf, np.float32).reshape(-1)]
-        v, i, r, m, li, e, et, o, en, lt, ge, tp, ti, ev, ec = self.keep
+        # vcolors: 3 RGBA8 words a triangle in the leaves' order
+        self.keep.append(np.zeros(0, np.uint32) if vcolors is None else np.ascontiguousarray(vcolors, np.uint32).reshape(-1))
+        v, i, r, m, li, e, et, o, en, lt, ge, tp, ti, ev, ec, vc = self.keep
         self._s = _RenderScene(self.geo._s, _ptr(v), len(v), _ptr(i), len(i), _ptr(r), len(r), _ptr(m), len(m),
                                _ptr(li), len(li), _ptr(e), len(e), _ptr(et), len(et), _ptr(o), len(o) // 3,
                                _ptr(en), _ptr(lt) if len(lt) else None, len(lt), _ptr(ge) if len(ge) else None,
                                _ptr(tp) if len(ti) else None, tex_side, tex_layers, _ptr(ti) if len(ti) else None,
                                len(ti), _ptr(ev) if ev.size else None, ev.shape[1], ev.shape[0],
-                               _ptr(ec) if ev.size else None, len(ec))
+                               _ptr(ec) if ev.size else None, len(ec), _ptr(vc) if vc.size else None, len(vc))
         L = lib()
         L.or_render_sample.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_int]
         L.or_render_sample.restype = C.c_int64

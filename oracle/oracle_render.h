@@ -75,6 +75,10 @@ typedef struct {
     uint32_t env_w, env_h;
     const float *env_cdf;
     uint64_t env_cdf_count;
+    /* lib.rs:601 gpu_vcolors (scene.rs:2324-2334 tri_colors): three RGBA8 sRGB colours a triangle, one a corner, in
+     * the leaves' order; NULL: none (the reference's F_VCOLOR off; a material's uv.w flag is then ignored) */
+    const uint32_t *vcolors;
+    uint64_t vcolor_words;
 } or_render_scene;
 
 /* One sample (params word 18 = P.size.z) of every pixel of rows [y0, y1), accumulated into `accum` (80 bytes a

@@ -318,6 +318,16 @@ static inline float srgb1(float c) {
     return c <= 0.04045f ? c / 12.92f : or_exp2(2.4f * or_log2((c + 0.055f) / 1.055f));
 }
 static inline V srgb_to_linear(V c) { return v3(srgb1(c.x), srgb1(c.y), srgb1(c.z)); }
+/* vcolor (269-275): the triangle's three sRGB 8-bit corner colours (unpack4x8unorm), decoded and interpolated */
+static V vcolor(const Ctx *c, uint32_t t, float u, float v) {
+    V k[3];
+    for (uint32_t j = 0; j < 3; j++) {
+        const uint32_t w = c->s->vcolors[ck(c, (uint64_t)t * 3u + j, c->s->vcolor_words)];
+        k[j] = srgb_to_linear(v3((float)(w & 0xffu) / 255.0f, (float)((w >> 8) & 0xffu) / 255.0f,
+                                 (float)((w >> 16) & 0xffu) / 255.0f));
+    }
+    return vadd(vadd(vs(k[0], 1.0f - u - v), vs(k[1], u)), vs(k[2], v));
+}
 /* The pages' level `level` of layer `layer`, bilinear at page uv (pu, pv) with the reference's sampler (gpu.rs:2690:
  * repeat, linear): texel centres at (i + 0.5) / n, four texels weighed by the fractions, RGBA8 as unorm. */
 static void tex_bilinear(const Ctx *c, uint32_t layer, uint32_t level, float pu, float pv, float *out) {
@@ -490,6 +500,7 @@ static Surface surface(Ctx *c, V o, V d, const or_hit *h, float width) {
     }
     s.card = as_i32(m[4 * 4 + 2]) == MAT_EMITTER_CARD;
     V base = ld3(m);
+    if (c->s->vcolors && m[23] > 0.0f) base = vmul(base, vcolor(c, h->tri, h->u, h->v)); /* 1356 */
     float rough = m[8 + 1], metal = m[8 + 0], spec = m[8 + 2];
     if (tex_on(c)) { /* 1360-1364 */
         float tx[4];
@@ -930,6 +941,7 @@ static GlassAt glass_at(const Ctx *c, const or_hit *h, V d, float t) {
     g.tint = v3(1, 1, 1);
     if (clampf(m[28 + 0], 0.0f, 1.0f) * (1.0f - clampf(m[8 + 0], 0.0f, 1.0f)) <= 0.0f) return g;
     g.tint = ld3(m);
+    if (c->s->vcolors && m[23] > 0.0f) g.tint = vmul(g.tint, vcolor(c, h->tri, h->u, h->v)); /* 949 */
     float tr_map = 1.0f;
     if (tex_on(c) && (as_i32(m[12]) >= 0 || m[22] > 0.5f)) { /* 950-956 */
         float uv[2], den[2], tx[4];

@@ -33,8 +33,8 @@ def test_the_structs_match_the_header_sizes():
     from threert_amd import render as R
 
     # ... light_tree, light_tree_vec4s, glass_energy, tex_pages, tex_side + tex_layers, tex_info, tex_info_vec4s,
-    # env, env_width + env_height, env_cdf, env_cdf_count
-    assert ctypes.sizeof(R._RenderDesc) == 8 * 6 + 8 + 8 * 17 + 16 + 8 + 8 + 8 + 8 + 8 + 8 + 8 + 8 + 8
+    # env, env_width + env_height, env_cdf, env_cdf_count, vcolors, vcolor_words
+    assert ctypes.sizeof(R._RenderDesc) == 8 * 6 + 8 + 8 * 17 + 16 + 8 + 8 + 8 + 8 + 8 + 8 + 8 + 8 + 8 + 8 + 8
     assert ctypes.sizeof(T._Tree) == 8 * 4 + 8 + 24
 
 

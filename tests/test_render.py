@@ -565,7 +565,7 @@ def test_the_serving_job_that_met_a_tie_matches_the_oracle(device):
 def test_what_the_library_does_not_render_is_refused(device):
     """Records with cutout sections or with glass among shadow-casting sections (scene.rs INST_CUTOUT 16, INST_GLASS 32
     in the bits word; the Cornell box's records cast shadows); materials with textures, normal maps, function
-    flags, transmission textures, vertex colours, alpha cutoffs, or glass without the glass energy table;
+    flags, transmission textures, alpha cutoffs, or glass without the glass energy table;
     and Params asking for an environment map, more than 64 bounces, more lights than the scene has, fog, volumetric
     scattering, the live light LOD, procedural geometry, aerial perspective or light-tree candidates
     without a tree: each refused with TRT_E_INVALID and a message that names it. The device still renders the unchanged scene afterwards."""
@@ -587,7 +587,7 @@ def test_what_the_library_does_not_render_is_refused(device):
     for word, value, name in ((12, as_int(0), "base-colour texture"), (13, as_int(2), "normal map"),
                               (16, as_int(1), "emissive texture"),
                               (19, as_int(1), "material function"), (22, 1.0, "transmission map"),
-                              (23, 1.0, "vertex colours"), (27, 0.5, "alpha cutoff"),
+                              (27, 0.5, "alpha cutoff"),
                               (28, 1.0, "no glass energy table")):
         print(f"refusal: material word {word} ({name})", flush=True)
         bad = copy.deepcopy(rs)

@@ -77,7 +77,8 @@ class _SceneDesc(C.Structure):
                 ("light_tree_vec4s", C.c_uint64), ("glass_energy", C.c_void_p), ("tex_pages", C.c_void_p),
                 ("tex_side", C.c_uint32), ("tex_layers", C.c_uint32), ("tex_info", C.c_void_p),
                 ("tex_info_vec4s", C.c_uint64), ("env", C.c_void_p), ("env_width", C.c_uint32),
-                ("env_height", C.c_uint32), ("env_cdf", C.c_void_p), ("env_cdf_count", C.c_uint64)]
+                ("env_height", C.c_uint32), ("env_cdf", C.c_void_p), ("env_cdf_count", C.c_uint64),
+                ("vcolors", C.c_void_p), ("vcolor_words", C.c_uint64)]
 
 
 class _LightTree(C.Structure):

@@ -90,8 +90,9 @@ struct trt_scene {
     // shading tables (trt_render)
     void *vertices = nullptr, *indices = nullptr, *rows = nullptr, *materials = nullptr, *lights = nullptr,
          *emitters = nullptr, *emitter_tris = nullptr, *objects = nullptr, *energy = nullptr, *light_tree = nullptr,
-         *glass_energy = nullptr, *tex_pages = nullptr, *tex_info = nullptr, *env = nullptr, *env_cdf = nullptr;
-    uint64_t light_tree_count = 0, tex_info_vec4s = 0, env_cdf_count = 0;
+         *glass_energy = nullptr, *tex_pages = nullptr, *tex_info = nullptr, *env = nullptr, *env_cdf = nullptr,
+         *vcolors = nullptr;
+    uint64_t light_tree_count = 0, tex_info_vec4s = 0, env_cdf_count = 0, vcolor_words = 0;
     uint32_t tex_side = 0, tex_layers = 0, env_w = 0, env_h = 0;  // env_w 0: no environment map
     bool tex = false;    // the scene has texture pages (the reference's F_TEX)
     bool shading = false;
@@ -240,10 +241,10 @@ int trt_device_sync(trt_device *d) {
                                       "leaf", "emitter triangle (RIS pick)", "emitter triangle (single draw)",
                                       "emitter triangle (a hit's own)", "light tree node", "glass energy table",
                                       "texture page layer", "texture info", "environment texel",
-                                      "environment CDF"};
+                                      "environment CDF", "vertex colours"};
         std::string where;
         if (g[1]) {
-            const char *name = g[1] <= 10 ? what[g[1]] : (g[1] >= 16 && g[1] <= 35 ? table[g[1] - 16] : "?");
+            const char *name = g[1] <= 10 ? what[g[1]] : (g[1] >= 16 && g[1] <= 36 ? table[g[1] - 16] : "?");
             where = std::string(" (first: ") + name + " index " +
                     std::to_string((uint64_t(g[3]) << 32) | g[2]) + " of " + std::to_string(g[4]) + "; context " +
                     std::to_string(g[5]) + " " + std::to_string(g[6]) + " " + std::to_string(g[7]) + ")";
@@ -392,7 +393,7 @@ int trt_scene_release(trt_scene *s) {
     hipFree(s->wleaves);
     for (void *p : {s->vertices, s->indices, s->rows, s->materials, s->lights, s->emitters, s->emitter_tris,
                     s->objects, s->energy, s->light_tree, s->glass_energy, s->tex_pages, s->tex_info, s->env,
-                    s->env_cdf})
+                    s->env_cdf, s->vcolors})
         hipFree(p);
     trt_device *d = s->dev;
     delete s;
@@ -425,7 +426,6 @@ static bool material_rendered(const float *m, uint64_t count, bool glass_table, 
         }
         const char *what = nullptr;
         if (w[7] > 0) what = "a material function";
-        if (!what && f[23] > 0.0f) what = "vertex colours";
         if (!what && f[27] > 0.0f) what = "an alpha cutoff";
         if (what) {
             *why = "material " + std::to_string(k) + " needs " + what + ", which this core does not render";
@@ -564,6 +564,10 @@ static bool check_shading(const trt_scene_desc *d, std::string *why) {
                            textures, why))
         return false;
     if (d->tex_info && !pages_ok(d, why)) return false;
+    if (d->vcolors && d->vcolor_words != 3 * d->index_count) {
+        *why = "vcolors needs three words a triangle (vcolor_words = 3 x index_count)";
+        return false;
+    }
     if (d->env) {
         const uint64_t w = d->env_width, h = d->env_height;
         if (!d->env_cdf || w == 0 || h == 0 || w > 16384 || h > 16384 || d->env_cdf_count != h + w * h) {
@@ -854,6 +858,10 @@ int trt_scene_upload(trt_device *d, const trt_scene_desc *desc, trt_scene **out)
             s->tex_side = desc->tex_side, s->tex_layers = desc->tex_layers, s->tex_info_vec4s = desc->tex_info_vec4s;
             rc = upload(d, &s->tex_pages, desc->tex_pages, page_layer_bytes(desc->tex_side) * desc->tex_layers, &total);
             if (!rc) rc = upload(d, &s->tex_info, desc->tex_info, desc->tex_info_vec4s * 16, &total);
+        }
+        if (!rc && desc->vcolors) {
+            s->vcolor_words = desc->vcolor_words;
+            rc = upload(d, &s->vcolors, desc->vcolors, desc->vcolor_words * 4, &total);
         }
         if (!rc && desc->env) {
             // gpu.rs:3552-3561: the texels as half floats of min(v, 65000), rounded to nearest even
@@ -1218,6 +1226,8 @@ static int job_args(const trt_render_job &j, trt::RenderArgs &a, std::string *wh
     a.env_cdf = static_cast<const float *>(s->env_cdf);
     a.env_cdf_count = s->env_cdf_count;
     a.env_w = s->env_w, a.env_h = s->env_h;
+    a.vcolors = static_cast<const uint32_t *>(s->vcolors);
+    a.vcolor_words = s->vcolor_words;
     a.accum = static_cast<float4 *>(acc->buf);
     a.vertex_count = static_cast<uint32_t>(s->vertex_count);
     a.tri_count = static_cast<uint32_t>(s->tri_count);
@@ -1423,15 +1433,15 @@ static int render_batch(trt_device *d, const trt_render_job *jobs, uint32_t n, u
         local = local || j.args.params[20] > j.args.params[36];  // P.counts.x > P.counts2.x: local lights
         env = env || trt::from_bits(j.args.params[24]) >= 0.5f;  // P.env.x: an environment map
         // unlit materials (F_UNLIT) and a background (P.background.w: F_BACKGROUND)
-        other = other || jobs[k].scene->unlit || trt::from_bits(j.args.params[99]) > 0.5f;
+        other = other || jobs[k].scene->unlit || jobs[k].scene->vcolors || trt::from_bits(j.args.params[99]) > 0.5f;
         tree = tree || trt::from_bits(j.args.params[87]) > 0.5f;  // P.ltree.w: the job samples the light tree
         table.push_back(j);
     }
     if (tiles >= (uint64_t{1} << 31)) return fail(TRT_E_INVALID, "trt_render_batch: too many tiles in one launch");
     if (table.empty()) return ok();
     // The wavefront takes batches whose jobs have at most kWfMaxSuns directional lights, no glass, no textures, no
-    // local lights, no environment map, no unlit material and no background; counting launches stay on the
-    // megakernel. TRT_RENDER_AUTO takes it when
+    // local lights, no environment map, no unlit material, no vertex colours and no background; counting launches stay
+    // on the megakernel. TRT_RENDER_AUTO takes it when
     // every scene has kWfAutoLeaves triangles or more, or for launches of a whole chunk of paths or more.
     const uint32_t how = d->render.load();
     bool wf = reads == nullptr && how != TRT_RENDER_MEGAKERNEL && !glass && !tex && !local && !env && !other;

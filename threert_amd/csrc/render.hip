@@ -374,6 +374,17 @@ __device__ __forceinline__ float srgb1(float c) {
     return c <= 0.04045f ? c / 12.92f : exp2_c(2.4f * log2_c((c + 0.055f) / 1.055f));
 }
 __device__ __forceinline__ V srgb_to_linear(V c) { return v3(srgb1(c.x), srgb1(c.y), srgb1(c.z)); }
+// vcolor (269-275): the triangle's three sRGB 8-bit corner colours (unpack4x8unorm), decoded and interpolated
+__device__ V vcolor(const Ctx &c, uint32_t t, float u, float v) {
+    V k[3];
+#pragma unroll
+    for (uint32_t j = 0; j < 3; j++) {
+        const uint32_t w = c.a->vcolors[c.ck(uint64_t(t) * 3u + j, c.a->vcolor_words, 36)];
+        k[j] = srgb_to_linear(v3(static_cast<float>(w & 0xffu) / 255.0f, static_cast<float>((w >> 8) & 0xffu) / 255.0f,
+                                 static_cast<float>((w >> 16) & 0xffu) / 255.0f));
+    }
+    return vadd(vadd(vs(k[0], 1.0f - u - v), vs(k[1], u)), vs(k[2], v));
+}
 // Level `level` of page layer `layer`, bilinear at page uv with the reference's sampler (repeat, linear): texel
 // centres at (i + 0.5) / n, four texels weighed by the fractions, RGBA8 as unorm.
 __device__ float4 tex_bilinear(const Ctx &c, uint32_t layer, uint32_t level, float pu, float pv) {
@@ -443,6 +454,11 @@ template <bool kExt>
 __device__ __forceinline__ bool glass_on(const Ctx &c) { return kExt && c.a->glass_energy != nullptr; }
 template <bool kExt>
 __device__ __forceinline__ bool tex_on(const Ctx &c) { return kExt && c.a->tex_info != nullptr; }
+// Vertex colours: the scene's colours (the reference's F_VCOLOR), read for materials that ask (uv.w > 0).
+template <bool kExt>
+__device__ __forceinline__ bool vcolor_on(const Ctx &c, const float4 *m) {
+    return kExt && c.a->vcolors != nullptr && m[5].w > 0.0f;
+}
 // An environment map: P.env.x, which the launch allows only for a scene that holds a map of P.env.yz's size.
 template <bool kExt>
 __device__ __forceinline__ bool env_on(const Ctx &c) { return kExt && c.pf(ENV, 0) >= 0.5f; }
@@ -575,6 +591,7 @@ __device__ Surface surface(const Ctx &c, V o, V d, const HitRec &h, float width 
     s.card = static_cast<int>(__float_as_uint(m[4].z)) == kMatEmitterCard;
     const float4 params = m[2];
     V base = xyz(m[0]);
+    if (vcolor_on<kExt>(c, m)) base = vmul(base, vcolor(c, h.tri, h.u, h.v));  // 1356
     float rough = params.y, metal = params.x, spec = params.z;
     if (tex_on<kExt>(c)) {  // 1360-1364: the colour (sRGB), roughness (green) and metalness (blue) maps
         const float4 t0 = m[3];
@@ -1127,6 +1144,7 @@ __device__ V glass_tr(const Ctx &c, V o, V d, float tmax, bool to_sky) {
         const float4 *m = c.material(material_of(c, a, h.inst));
         const V ng = vs(normalize(cross(e1, e2)), side);
         V tint = xyz(m[0]);
+        if (vcolor_on<kExt>(c, m)) tint = vmul(tint, vcolor(c, h.tri, h.u, h.v));  // 949
         float tr_map = 1.0f;
         if (tex_on<kExt>(c) && clampf(m[7].x, 0.0f, 1.0f) * (1.0f - clampf(m[2].x, 0.0f, 1.0f)) > 0.0f &&
             (as_i32(m[3].x) >= 0 || m[5].z > 0.5f)) {  // 950-956

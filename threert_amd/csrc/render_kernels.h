@@ -38,6 +38,8 @@ struct RenderArgs {
     const float *env_cdf;       // lib.rs build_env_cdf: env_h + env_w x env_h floats
     uint64_t env_cdf_count;
     uint32_t env_w, env_h;
+    const uint32_t *vcolors;    // lib.rs gpu_vcolors: 3 RGBA8 sRGB corner colours a triangle, in the leaves' order
+    uint64_t vcolor_words;
     float4 *accum;              // records.wgsl Pixel, 5 x float4 a pixel
     // the tables' sizes (in their records), against which every index is checked
     uint32_t vertex_count, tri_count, row_count, material_count, light_count, emitter_vec4s, emitter_tri_count,

@@ -37,13 +37,14 @@ ROW_GLASS = 1 << 25  # scene.rs:103: a row's class bit for glass (mesh_class 2)
 def material(base=(0.8, 0.8, 0.8), emissive=(0.0, 0.0, 0.0), metallic=0.0, roughness=1.0, specular=0.5,
              clearcoat=0.0, clearcoat_roughness=0.0, transmission=0.0, ior=1.5, base_map=-1, normal_map=-1,
              roughness_map=-1, metalness_map=-1, emissive_map=-1, transmission_map=-1, normal_scale=1.0,
-             repeat=(1.0, 1.0), offset=(0.0, 0.0), unlit=False) -> np.ndarray:
+             repeat=(1.0, 1.0), offset=(0.0, 0.0), unlit=False, vertex_colors=False) -> np.ndarray:
     """scene.rs:480-491 GpuMaterial (72 words; scene.rs:1238-1262 build_material): base colour, emission, (metallic,
     roughness, specular, normal scale); the texture slots (tex0: base colour (sRGB), normal, roughness (green),
     metalness (blue); tex1.x emissive (sRGB); uv.z the transmission map's slot + 1 (red); -1 none), the uv repeat and
     offset; the layer (scene.rs:1257): transmission (glass, clamped to [0, 1]), index of refraction, clearcoat and its
     roughness (both clamped to [0, 1]); `unlit`: three.js MeshBasicMaterial (tex1.z MAT_UNLIT 4: its colour to camera
-    rays and mirror-like paths, lit by nothing, lighting nothing)."""
+    rays and mirror-like paths, lit by nothing, lighting nothing); `vertex_colors`: the mesh's vertex colours multiply
+    the base colour (uv.w)."""
     m = np.zeros(72, np.float32)
     m[0:3] = base
     m[3] = 1.0
@@ -51,7 +52,7 @@ def material(base=(0.8, 0.8, 0.8), emissive=(0.0, 0.0, 0.0), metallic=0.0, rough
     m[8:12] = (metallic, roughness, specular, normal_scale)
     m[12:20] = np.array([base_map, normal_map, roughness_map, metalness_map, emissive_map, -1, 4 if unlit else 0, 0],
                         np.int32).view(np.float32)
-    m[20:24] = (repeat[0], repeat[1], float(transmission_map) + 1.0, 0.0)
+    m[20:24] = (repeat[0], repeat[1], float(transmission_map) + 1.0, 1.0 if vertex_colors else 0.0)
     m[24:26] = offset
     m[28:32] = (min(max(transmission, 0.0), 1.0), ior, min(max(clearcoat, 0.0), 1.0),
                 min(max(clearcoat_roughness, 0.0), 1.0))
@@ -281,6 +282,7 @@ class RMesh:
     sections: np.ndarray | None = None  # (n,) section of each triangle
     uvs: np.ndarray | None = None      # (n, 3, 2) per corner; None: zeros
     tangents: np.ndarray | None = None  # (n, 3, 4) per corner (xyz, hand); None: zeros (hand 0: derived from the uvs)
+    colors: np.ndarray | None = None   # (n, 3, 3) linear vertex colours per corner (three.js vertexColors); None: white
 
 
 # ---- texpack.rs: texture pages ----------------------------------------------------------------------------------
@@ -421,6 +423,8 @@ class RenderScene:
     # None: a constant sky (the Params' colour)
     env: np.ndarray | None = None
     env_cdf: np.ndarray | None = None
+    # lib.rs gpu_vcolors: (triangles, 3) RGBA8 sRGB words in the leaves' order; None: no vertex colours
+    vcolors: np.ndarray | None = None
 
     def env_half(self) -> np.ndarray | None:
         """gpu.rs:3552-3561 write_env_texels: the texels the kernel reads, half floats of min(v, 65000)."""
@@ -434,7 +438,7 @@ class RenderScene:
                 "objects": self.objects, "energy": self.energy, "light_tree": self.light_tree,
                 "glass_energy": self.glass_energy, "tex_pages": self.tex_pages, "tex_side": self.tex_side,
                 "tex_layers": self.tex_layers, "tex_info": self.tex_info, "env": self.env_half(),
-                "env_cdf": self.env_cdf}
+                "env_cdf": self.env_cdf, "vcolors": self.vcolors}
 
     def params(self, width: int, height: int, camera: Camera, seed: int = 1, max_bounces: int = 4, rr_depth: int = 3,
                clamp: float = 1e9, energy_compensation: float = 0.0, sky=(0.0, 0.0, 0.0), env_nee: bool = True,
@@ -485,6 +489,16 @@ class RenderScene:
             f[86] = self.emitters[len(self.emitter_tris) + (EMITTER_GUIDE + 1 + 3) // 4, 1]
             f[87] = float(RIS_OFFLINE)
         return p
+
+
+def tri_color_words(colors: np.ndarray) -> np.ndarray:
+    """scene.rs:2324-2334 tri_colors: (n, 3, 3) linear colours to (n, 3) RGBA8 words, each channel clamped to [0, 1],
+    sRGB-encoded (post.rs:526-532 linear_to_srgb) and rounded half away from zero, alpha 255."""
+    c = np.clip(np.asarray(colors, np.float32), np.float32(0.0), np.float32(1.0))
+    s = np.where(c < np.float32(0.0031308), c * np.float32(12.92),
+                 np.float32(1.055) * np.power(c, np.float32(1.0 / 2.4)) - np.float32(0.055)).astype(np.float32)
+    q = np.floor(s.astype(np.float64) * 255.0 + 0.5).astype(np.uint32)
+    return (q[..., 0] | (q[..., 1] << 8) | (q[..., 2] << 16) | np.uint32(0xff << 24)).astype(np.uint32)
 
 
 def build_env_cdf(texels: np.ndarray) -> np.ndarray:
@@ -550,6 +564,19 @@ def build(meshes: list[RMesh], instances: list[RInstance], materials: list[np.nd
         idx_parts.append((order[:, None] * 3 + np.arange(3, dtype=np.uint32)[None, :]).astype(np.uint32))
     vertices = np.concatenate(verts) if verts else np.zeros((0, 12), np.float32)
     indices = np.concatenate(idx_parts) if idx_parts else np.zeros((0, 3), np.uint32)
+    # lib.rs:601 gpu_vcolors: scene.rs:2324-2334 tri_colors' words in the leaves' order, white (scene.rs WHITE) for a
+    # mesh without colours; none when no mesh has any
+    vcolors = None
+    if any(m.colors is not None for m in meshes):
+        parts = []
+        for k, m in enumerate(meshes):
+            n = len(np.asarray(m.tris).reshape(-1, 3, 3))
+            if m.colors is None:
+                parts.append(np.full((n, 3), 0xffffffff, np.uint32))
+                continue
+            order = prep.mesh_order[k]
+            parts.append(tri_color_words(np.asarray(m.colors, np.float32).reshape(n, 3, 3))[order])
+        vcolors = np.concatenate(parts).astype(np.uint32)
     # rows, emitters
     rows, em_tris, entries, centres = [], [], [], []
     n_em_so_far = 0
@@ -652,6 +679,7 @@ def build(meshes: list[RMesh], instances: list[RInstance], materials: list[np.nd
         rs.glass_energy = glass_energy_lut()
     if len(textures):
         rs.tex_pages, rs.tex_side, rs.tex_layers, rs.tex_info = pack_textures(list(textures))
+    rs.vcolors = vcolors
     if env is not None:  # an equirectangular map (h, w, 3 or 4): lib.rs SkyBaked's EnvMap and its CDF
         e = np.asarray(env, np.float32)
         if e.ndim != 3 or e.shape[2] not in (3, 4) or min(e.shape[:2]) < 1:
@@ -687,6 +715,10 @@ def upload(dev: Device, rs: RenderScene) -> Scene:
         keep += [pages, info]
         desc.tex_pages, desc.tex_side, desc.tex_layers = _ptr(pages), rs.tex_side, rs.tex_layers
         desc.tex_info, desc.tex_info_vec4s = _ptr(info), len(info)
+    if rs.vcolors is not None:
+        vc = np.ascontiguousarray(rs.vcolors, np.uint32)
+        keep.append(vc)
+        desc.vcolors, desc.vcolor_words = _ptr(vc), vc.size
     if rs.env is not None:
         env = np.ascontiguousarray(rs.env, np.float32)
         cdf = np.ascontiguousarray(rs.env_cdf, np.float32)
