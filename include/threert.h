@@ -203,8 +203,9 @@ typedef struct {
  * is borrowed for the call only. Refuses (TRT_E_INVALID) trees the reference's checks refuse (units.rs check_mesh /
  * check_top, ltree.rs LightTree::check), records with cutout bits or with glass among shadow-casting sections (the
  * reference's F_IGLASS; a record whose sections are all glass casts no shadow and is taken), materials that need
- * what this core does not render (a texture slot naming no texture of the pages, alpha cutoffs, vertex colours,
- * material functions), and glass without the glass energy table, each named in trt_last_error. */
+ * what this core does not render (a texture slot naming no texture of the pages, alpha cutoffs, material
+ * functions), glass without the glass energy table, and vertex colours of another length than 3 x index_count, each
+ * named in trt_last_error. */
 int trt_scene_upload(trt_device *dev, const trt_scene_desc *desc, trt_scene **out);
 int trt_scene_release(trt_scene *scene);
 int trt_scene_stats_get(trt_scene *scene, trt_scene_stats *out);
