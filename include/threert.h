@@ -148,7 +148,9 @@ typedef struct {
     uint64_t emitter_vec4s;
     const void *emitter_tris; /* lib.rs:143-157 GpuEmitterTri, 64 B */
     uint64_t emitter_tri_count;
-    const void *objects;      /* object table, 3 vec4 an object; the first's w is its power */
+    const void *objects;      /* object table, 3 vec4 an object (scene.rs:25 object_stride without instance colours;
+                               * a tinted scene's 4 vec4 an object, the reference's F_TINT, is not taken); the
+                               * first's w is its power */
     uint64_t object_count;
     const void *energy;       /* lib.rs:1088-1104 energy_lut: 512 vec4; NULL: no shading tables */
     const void *light_tree;   /* ltree.rs LightTree::nodes, one vec4 a node (gpu.rs:2613 `light_tree`); NULL / 0: none.
